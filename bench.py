@@ -1,0 +1,168 @@
+"""Benchmark: Mobius domain-wall even-odd Dslash (DhopEO) on one MI355X.
+
+Workload (the reference's Mobius configuration): 16^4 x Ls=16, b=1.5, c=0.5, M5=1.8, mf=0.01,
+random SU(3) links exp(i 0.3 H), Gaussian 5D source, FP64 (the reported value) and FP32.
+A step is one DhopEO over the whole odd->even half lattice, inputs resident in HBM.
+
+value = algorithmic (compulsory) bytes per call / average kernel time, with
+bytes = H (24 Ls + 72) c   (H = V4/2 half sites, c = bytes per complex): each input spinor and
+each link read once, each output spinor written once.  Kernel time comes from HIP events on the
+stream the kernels are launched on.  Also reported: GF/s at 1320 flop per 5D site, the FP32
+figure, Schur CG time to 1e-10 in FP64 and mixed FP32/FP64, and the CPU oracle's DhopEO on the
+same workload (cpu_baseline).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+DIMS = (16, 16, 16, 16)
+LS = 16
+PARAMS = dict(mass=0.01, M5=1.8, b=1.5, c=0.5)
+EPS = 0.3
+METRIC = 'Dslash GB/s (% HBM peak); CG time-to-1e-10 mixed vs FP64; solve p50/p99'
+DHOP_KERNEL = 'k_dhop<double, 16, false>'
+
+
+def pick(a, dims, cb):
+    """Full lexicographic field -> half field of checkerboard cb (Grid red-black order)."""
+    lx, ly, lz, lt = dims
+    t, z, y = np.meshgrid(np.arange(lt), np.arange(lz), np.arange(ly), indexing='ij')
+    off = (cb + t + z + y) % 2
+    xs = off[..., None] + 2 * np.arange(lx // 2)[None, None, None, :]
+    idx = xs.reshape(lt, lz, ly, lx // 2, *([1] * (a.ndim - 4)))
+    return np.ascontiguousarray(np.take_along_axis(a, idx, axis=3))
+
+
+def traffic_from_profile():
+    """Measured HBM bytes per DhopEO launch from the committed rocprofv3 counter summary."""
+    p = os.path.join(ROOT, 'profiles', 'dhop_mobius_16x16x16x16_ls16_f64_traffic.json')
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get('traffic_bytes_per_launch')
+    except Exception:
+        return None
+
+
+def time_dhop(ctx, D, fi, fo, steps, warmup):
+    for _ in range(warmup):
+        D.DhopEO(fi, fo, 0)
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(steps):
+        D.DhopEO(fi, fo, 0)
+    ctx.record(1)
+    ctx.sync()
+    return ctx.elapsed_ms(0, 1) / steps
+
+
+def cpu_baseline(U, psi_o, seconds=12.0):
+    from oracle import pyoracle
+    ncores = min(16, os.cpu_count() or 1)
+    pyoracle.set_threads(ncores)
+    O = pyoracle.Oracle(U, DIMS, LS, 'mobius', **PARAMS)
+    O.dhop(psi_o, 0, 0)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        O.dhop(psi_o, 0, 0)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds or n >= 200:
+            break
+    return el / n, n, ncores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--no-cg', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU halo exchange '
+              f'is not built). Run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+
+    import grid_amd as gx
+    ctx = gx.Context(0)
+    U = gx.random_gauge(DIMS, 1002, EPS)
+    psi_o = pick(gx.gaussian_fermion(DIMS, LS, 2002), DIMS, 1)
+    D = gx.FermionOp(ctx, U, DIMS, LS, 'mobius', **PARAMS)
+    H = np.prod(DIMS) // 2
+    res = {}
+    for prec, name, c in ((gx.F64, 'f64', 16), (gx.F32, 'f32', 8)):
+        fi = D.field(prec, 1).upload(psi_o)
+        fo = D.field(prec, 0)
+        ms = time_dhop(ctx, D, fi, fo, args.steps, args.warmup)
+        byts = H * (24 * LS + 72) * c
+        res[name] = dict(ms=ms, bytes=int(byts), gbs=byts / (ms * 1e-3) / 1e9, gflops=1320.0 * H * LS / (ms * 1e-3) / 1e9)
+        fi.close()
+        fo.close()
+
+    cg = None
+    if not args.no_cg:
+        src_full = gx.gaussian_fermion(DIMS, LS, 3003)
+        fs, fsol = D.field().upload(src_full), D.field()
+        st64 = D.schur_solve(fs, fsol, tol=1e-10, maxit=20000, mixed=False)
+        x64 = fsol.download()
+        fsol.zero()
+        stmx = D.schur_solve(fs, fsol, tol=1e-10, maxit=20000, mixed=True)
+        xmx = fsol.download()
+        cg = dict(fp64=dict(seconds=st64['seconds'], iterations=st64['iterations'], true_residual=st64['true_residual'],
+                            unprec_residual=st64['unprec_residual']),
+                  mixed_f32=dict(seconds=stmx['seconds'], inner_iterations=stmx['inner_iterations'],
+                                 outer=stmx['outer_iterations'], final_iterations=stmx['final_iterations'],
+                                 true_residual=stmx['true_residual'], unprec_residual=stmx['unprec_residual'],
+                                 rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))),
+                  speedup_mixed_vs_fp64=st64['seconds'] / stmx['seconds'])
+
+    cpu = None
+    if not args.no_cpu:
+        sec, n, cores = cpu_baseline(U, psi_o)
+        cpu = {'value': res['f64']['bytes'] / sec / 1e9, 'unit': 'GB/s', 'cores': cores, 'kind': 'port',
+               'sample': f'{n} DhopEO calls, Mobius 16^4xLs16 FP64, oracle/cpu_ref.cpp with {cores} OpenMP threads '
+                         f'({sec * 1e3:.1f} ms per call, {1320.0 * H * LS / sec / 1e9:.2f} GF/s)'}
+
+    r = res['f64']
+    out = {
+        'metric': METRIC,
+        'value': round(r['gbs'], 2),
+        'unit': 'GB/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(r['ms'], 5),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic (random SU(3) exp(i 0.3 H) links, Gaussian source)',
+        'config': {'workload': 'Mobius DWF DhopEO odd->even', 'dims': list(DIMS), 'Ls': LS, **PARAMS,
+                   'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': round(r['gbs'], 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(r['gbs'] / HBM_PEAK_GBS, 4), 'traffic': traffic_from_profile(),
+                     'kernel': DHOP_KERNEL, 'algorithmic_bytes_per_launch': r['bytes']},
+        'cpu_baseline': cpu,
+        'pct_hbm_peak': round(100 * r['gbs'] / HBM_PEAK_GBS, 2),
+        'gflops_1320': round(r['gflops'], 1),
+        'f32': {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res['f32'].items()},
+        'cg': cg,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,335 @@
+"""grid_amd: Python binding of libgx, the MI355X (gfx950) Wilson / Mobius domain-wall Dirac
+operator and even-odd CG library (C ABI in include/gx.h).
+
+The binding is a thin ctypes layer: every call goes to the HIP library.  There is no CPU
+fallback; importing works without a GPU, but creating a context needs one and raises
+otherwise.
+
+Host arrays use Grid's lexicographic order with s innermost, complex128:
+  full field (Lt, Lz, Ly, Lx, Ls, 4, 3), half field (Lt, Lz, Ly, Lx/2, Ls, 4, 3),
+  gauge (Lt, Lz, Ly, Lx, 4, 3, 3).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libgx.so')
+
+F64, F32, F16 = 0, 1, 2
+FULL, EVEN, ODD = -1, 0, 1
+WILSON, MOBIUS = 0, 1
+STATUS = {0: 'GX_OK', 1: 'GX_EINVAL', 2: 'GX_ENOMEM', 3: 'GX_ENOCONV', 4: 'GX_EHIP', 5: 'GX_ECB', 6: 'GX_ENAN'}
+
+
+class GxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f'{STATUS.get(code, code)}: {msg}')
+        self.code = code
+
+
+class SolverStats(ctypes.Structure):
+    _fields_ = [('iterations', ctypes.c_int), ('converged', ctypes.c_int), ('true_residual', ctypes.c_double),
+                ('computed_residual', ctypes.c_double), ('seconds', ctypes.c_double),
+                ('inner_iterations', ctypes.c_int), ('outer_iterations', ctypes.c_int),
+                ('final_iterations', ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_SIG = {
+    'gx_last_error': ([], ctypes.c_char_p),
+    'gx_version': ([], ctypes.c_char_p),
+    'gx_ctx_create': ([_I, ctypes.POINTER(_P)], _I),
+    'gx_ctx_destroy': ([_P], _I),
+    'gx_ctx_sync': ([_P], _I),
+    'gx_ctx_event_record': ([_P, _I], _I),
+    'gx_ctx_event_elapsed': ([_P, _I, _I, _DP], _I),
+    'gx_field_create': ([_P, _IP, _I, _I, _I, ctypes.POINTER(_P)], _I),
+    'gx_field_destroy': ([_P], _I),
+    'gx_field_import': ([_P, _DP], _I),
+    'gx_field_export': ([_P, _DP], _I),
+    'gx_field_checkerboard': ([_P, _IP], _I),
+    'gx_field_set_checkerboard_tag': ([_P, _I], _I),
+    'gx_field_zero': ([_P], _I),
+    'gx_field_copy': ([_P, _P], _I),
+    'gx_pick_checkerboard': ([_I, _P, _P], _I),
+    'gx_set_checkerboard': ([_P, _P], _I),
+    'gx_precision_change': ([_P, _P], _I),
+    'gx_axpy': ([_P, _D, _P, _P], _I),
+    'gx_axpby': ([_P, _D, _D, _P, _P], _I),
+    'gx_axpy_norm': ([_P, _D, _P, _P, _DP], _I),
+    'gx_scale': ([_P, _D, _P], _I),
+    'gx_norm2': ([_P, _DP], _I),
+    'gx_inner_product': ([_P, _P, _DP, _DP], _I),
+    'gx_inner_product_norm': ([_P, _P, _DP, _DP, _DP], _I),
+    'gx_wilson_create': ([_P, _IP, _I, _D, _DP, _DP, ctypes.POINTER(_P)], _I),
+    'gx_mobius_create': ([_P, _IP, _I, _D, _D, _D, _D, _DP, _DP, ctypes.POINTER(_P)], _I),
+    'gx_op_destroy': ([_P], _I),
+    'gx_op_import_gauge': ([_P, _DP], _I),
+    'gx_op_set_mass': ([_P, _D], _I),
+    'gx_M': ([_P, _P, _P], _I),
+    'gx_Mdag': ([_P, _P, _P], _I),
+    'gx_Meooe': ([_P, _P, _P], _I),
+    'gx_MeooeDag': ([_P, _P, _P], _I),
+    'gx_Mooee': ([_P, _P, _P], _I),
+    'gx_MooeeDag': ([_P, _P, _P], _I),
+    'gx_MooeeInv': ([_P, _P, _P], _I),
+    'gx_MooeeInvDag': ([_P, _P, _P], _I),
+    'gx_Dhop': ([_P, _P, _P, _I], _I),
+    'gx_DhopEO': ([_P, _P, _P, _I], _I),
+    'gx_DhopOE': ([_P, _P, _P, _I], _I),
+    'gx_M5D': ([_P, _P, _P], _I),
+    'gx_M5Ddag': ([_P, _P, _P], _I),
+    'gx_Meooe5D': ([_P, _P, _P], _I),
+    'gx_MeooeDag5D': ([_P, _P, _P], _I),
+    'gx_Mpc': ([_P, _P, _P], _I),
+    'gx_MpcDag': ([_P, _P, _P], _I),
+    'gx_MpcDagMpc': ([_P, _P, _P], _I),
+    'gx_HermOpAndNorm': ([_P, _P, _P, _DP, _DP], _I),
+    'gx_cg': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_mixed_cg': ([_P, _P, _P, _D, _D, _I, _I, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
+    'gx_random_gauge': ([_IP, ctypes.c_uint64, _D, _DP], _I),
+    'gx_gaussian_fermion': ([_IP, _I, ctypes.c_uint64, _DP], _I),
+    'gx_point_source': ([_IP, _I, _IP, _I, _I, _DP], _I),
+}
+
+
+def lib():
+    """Load libgx.so (built in-tree by __graft_entry__.build()); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f'{LIB_PATH} is missing: build it first (python -c "import __graft_entry__ as g; g.build()")')
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIG.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIG.keys())
+
+
+def _check(rc):
+    if rc != 0:
+        raise GxError(rc, lib().gx_last_error().decode())
+
+
+def _dims(d):
+    return (ctypes.c_int * 4)(*[int(x) for x in d])
+
+
+def _dp(a):
+    assert a.flags['C_CONTIGUOUS'] and a.dtype in (np.complex128, np.float64)
+    return a.ctypes.data_as(_DP)
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _P()
+        _check(lib().gx_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def sync(self):
+        _check(lib().gx_ctx_sync(self.h))
+
+    def record(self, slot):
+        _check(lib().gx_ctx_event_record(self.h, slot))
+
+    def elapsed_ms(self, s0, s1):
+        v = _D()
+        _check(lib().gx_ctx_event_elapsed(self.h, s0, s1, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().gx_ctx_destroy(self.h)
+            self.h = None
+
+
+class Field:
+    def __init__(self, ctx, dims, Ls=1, prec=F64, subset=FULL):
+        self.ctx, self.dims, self.Ls, self.prec, self.subset = ctx, tuple(int(d) for d in dims), int(Ls), prec, subset
+        h = _P()
+        _check(lib().gx_field_create(ctx.h, _dims(dims), self.Ls, prec, subset, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def shape(self):
+        lx, ly, lz, lt = self.dims
+        x = lx if self.subset == FULL else lx // 2
+        return (lt, lz, ly, x, self.Ls, 4, 3)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        _check(lib().gx_field_import(self.h, _dp(a)))
+        return self
+
+    def download(self):
+        out = np.zeros(self.shape, dtype=np.complex128)
+        _check(lib().gx_field_export(self.h, _dp(out)))
+        return out
+
+    @property
+    def checkerboard(self):
+        v = _I()
+        _check(lib().gx_field_checkerboard(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_checkerboard_tag(self, cb):
+        _check(lib().gx_field_set_checkerboard_tag(self.h, cb))
+
+    def zero(self):
+        _check(lib().gx_field_zero(self.h))
+
+    def norm2(self):
+        v = _D()
+        _check(lib().gx_norm2(self.h, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().gx_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def inner_product(x, y):
+    re, im = _D(), _D()
+    _check(lib().gx_inner_product(x.h, y.h, ctypes.byref(re), ctypes.byref(im)))
+    return complex(re.value, im.value)
+
+
+def axpy_norm(z, a, x, y):
+    v = _D()
+    _check(lib().gx_axpy_norm(z.h, a, x.h, y.h, ctypes.byref(v)))
+    return v.value
+
+
+def axpby(z, a, b, x, y):
+    _check(lib().gx_axpby(z.h, a, b, x.h, y.h))
+
+
+def precision_change(out, inp):
+    _check(lib().gx_precision_change(out.h, inp.h))
+
+
+def pick_checkerboard(cb, half, full):
+    _check(lib().gx_pick_checkerboard(cb, half.h, full.h))
+
+
+def set_checkerboard(full, half):
+    _check(lib().gx_set_checkerboard(full.h, half.h))
+
+
+class FermionOp:
+    """WilsonFermion (kind='wilson'; Ls>1 = multi-RHS) or MobiusFermion (kind='mobius')."""
+
+    def __init__(self, ctx, U, dims, Ls=1, kind='wilson', mass=0.1, M5=1.8, b=1.5, c=0.5, phases=None):
+        self.ctx, self.dims, self.Ls, self.kind = ctx, tuple(int(d) for d in dims), int(Ls), kind
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        ph = None
+        if phases is not None:
+            ph = np.array([[complex(p).real, complex(p).imag] for p in phases], dtype=np.float64).reshape(-1)
+        h = _P()
+        phptr = _dp(ph) if ph is not None else None
+        if kind == 'wilson':
+            _check(lib().gx_wilson_create(ctx.h, _dims(dims), self.Ls, mass, _dp(U), phptr, ctypes.byref(h)))
+        else:
+            _check(lib().gx_mobius_create(ctx.h, _dims(dims), self.Ls, mass, M5, b, c, _dp(U), phptr, ctypes.byref(h)))
+        self.h = h
+
+    def field(self, prec=F64, subset=FULL):
+        return Field(self.ctx, self.dims, self.Ls, prec, subset)
+
+    def _call(self, name, *args):
+        _check(getattr(lib(), 'gx_' + name)(self.h, *args))
+
+    def M(self, i, o): self._call('M', i.h, o.h)
+    def Mdag(self, i, o): self._call('Mdag', i.h, o.h)
+    def Meooe(self, i, o): self._call('Meooe', i.h, o.h)
+    def MeooeDag(self, i, o): self._call('MeooeDag', i.h, o.h)
+    def Mooee(self, i, o): self._call('Mooee', i.h, o.h)
+    def MooeeDag(self, i, o): self._call('MooeeDag', i.h, o.h)
+    def MooeeInv(self, i, o): self._call('MooeeInv', i.h, o.h)
+    def MooeeInvDag(self, i, o): self._call('MooeeInvDag', i.h, o.h)
+    def Dhop(self, i, o, dag=0): self._call('Dhop', i.h, o.h, int(dag))
+    def DhopEO(self, i, o, dag=0): self._call('DhopEO', i.h, o.h, int(dag))
+    def DhopOE(self, i, o, dag=0): self._call('DhopOE', i.h, o.h, int(dag))
+    def M5D(self, i, o): self._call('M5D', i.h, o.h)
+    def M5Ddag(self, i, o): self._call('M5Ddag', i.h, o.h)
+    def Meooe5D(self, i, o): self._call('Meooe5D', i.h, o.h)
+    def MeooeDag5D(self, i, o): self._call('MeooeDag5D', i.h, o.h)
+    def Mpc(self, i, o): self._call('Mpc', i.h, o.h)
+    def MpcDag(self, i, o): self._call('MpcDag', i.h, o.h)
+    def MpcDagMpc(self, i, o): self._call('MpcDagMpc', i.h, o.h)
+
+    def cg(self, src, psi, tol=1e-10, maxit=10000):
+        st = SolverStats()
+        rc = lib().gx_cg(self.h, src.h, psi.h, tol, maxit, ctypes.byref(st))
+        if rc not in (0, 3):
+            _check(rc)
+        return st.as_dict()
+
+    def mixed_cg(self, src, psi, tol=1e-10, inner_tol=1e-5, inner_prec=F32, maxinner=10000, maxouter=100):
+        st = SolverStats()
+        rc = lib().gx_mixed_cg(self.h, src.h, psi.h, tol, inner_tol, inner_prec, maxinner, maxouter, ctypes.byref(st))
+        if rc not in (0, 3):
+            _check(rc)
+        return st.as_dict()
+
+    def schur_solve(self, src, sol, tol=1e-10, maxit=10000, mixed=False):
+        st = SolverStats()
+        ur = _D()
+        rc = lib().gx_schur_solve(self.h, src.h, sol.h, tol, maxit, int(mixed), ctypes.byref(st), ctypes.byref(ur))
+        if rc not in (0, 3):
+            _check(rc)
+        d = st.as_dict()
+        d['unprec_residual'] = ur.value
+        return d
+
+    def close(self):
+        if self.h:
+            lib().gx_op_destroy(self.h)
+            self.h = None
+
+
+def random_gauge(dims, seed, eps):
+    lx, ly, lz, lt = dims
+    U = np.zeros((lt, lz, ly, lx, 4, 3, 3), dtype=np.complex128)
+    _check(lib().gx_random_gauge(_dims(dims), seed, eps, _dp(U)))
+    return U
+
+
+def gaussian_fermion(dims, Ls, seed):
+    lx, ly, lz, lt = dims
+    psi = np.zeros((lt, lz, ly, lx, Ls, 4, 3), dtype=np.complex128)
+    _check(lib().gx_gaussian_fermion(_dims(dims), int(Ls), seed, _dp(psi)))
+    return psi
+
+
+def point_source(dims, Ls, x, spin, colour):
+    lx, ly, lz, lt = dims
+    psi = np.zeros((lt, lz, ly, lx, Ls, 4, 3), dtype=np.complex128)
+    _check(lib().gx_point_source(_dims(dims), int(Ls), _dims(x), spin, colour, _dp(psi)))
+    return psi

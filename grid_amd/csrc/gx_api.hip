@@ -1,0 +1,1007 @@
+// libgx C ABI: contexts, fields, operators, BLAS1 and the even-odd solvers.
+//
+// Operator compositions follow Grid (paths relative to the Grid source tree):
+//   Wilson   M = Dhop + (4+m), Meooe = DhopEO/OE by input checkerboard, Mooee = (4+m),
+//            MooeeInv = 1/(4+m)          WilsonFermionImplementation.h:113-176,310-342
+//   Cayley   Meooe = Dhop o Meooe5D, MeooeDag = MeooeDag5D o Dhop^dag,
+//            Mooee, MooeeDag, MooeeInv(Dag), M5D(dag)   CayleyFermion5DImplementation.h:155-329
+//   Schur    Mpc = Mooee - Meooe MooeeInv Meooe       LinearOperator.h:325-349
+//   CG       ConjugateGradient.h:68-257; mixed precision ConjugateGradientMixedPrec.h:71-165;
+//            red-black solve SchurRedBlack.h:236-291,385-425.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <thread>
+
+#include "gx_internal.h"
+
+namespace gx {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// ---------------- host parallel loop ----------------
+static void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  if (nt < 1) nt = 1;
+  if (nt > 32) nt = 32;
+  if (n < 4096 || nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; t++) {
+    const int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=, &f]() { f(b, e); });
+  }
+  for (auto &x : th) x.join();
+}
+
+static bool geom_ok(const int dims[4], int Ls) {
+  if (!dims) return false;
+  for (int d = 0; d < 4; d++)
+    if (dims[d] < 2) return false;
+  if (dims[0] % 2) return false;
+  return Ls >= 1 && Ls <= kMaxLs;
+}
+
+static Geom make_geom(const int dims[4], int Ls) {
+  Geom g;
+  for (int d = 0; d < 4; d++) g.L[d] = dims[d];
+  g.Ls = Ls;
+  g.V4 = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+  g.H4 = g.V4 / 2;
+  return g;
+}
+
+static bool same_geom(const Geom &a, const Geom &b) {
+  for (int d = 0; d < 4; d++)
+    if (a.L[d] != b.L[d]) return false;
+  return a.Ls == b.Ls;
+}
+
+static inline void coor_lex(const Geom &g, int64_t idx, int x[4]) {
+  for (int d = 0; d < 4; d++) {
+    x[d] = (int)(idx % g.L[d]);
+    idx /= g.L[d];
+  }
+}
+static inline int64_t lex_index(const Geom &g, const int x[4]) {
+  return x[0] + (int64_t)g.L[0] * (x[1] + (int64_t)g.L[1] * (x[2] + (int64_t)g.L[2] * x[3]));
+}
+static inline int64_t half_index(const Geom &g, const int x[4]) {
+  return (x[0] >> 1) + (int64_t)(g.L[0] / 2) * (x[1] + (int64_t)g.L[1] * (x[2] + (int64_t)g.L[2] * x[3]));
+}
+static inline void coor_half(const Geom &g, int64_t h, int cb, int x[4]) {
+  const int lxh = g.L[0] / 2;
+  const int xh = (int)(h % lxh);
+  h /= lxh;
+  x[1] = (int)(h % g.L[1]);
+  h /= g.L[1];
+  x[2] = (int)(h % g.L[2]);
+  h /= g.L[2];
+  x[3] = (int)h;
+  x[0] = 2 * xh + ((cb + x[1] + x[2] + x[3]) & 1);
+}
+
+// ---------------- fields ----------------
+int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out) {
+  if (prec != GX_F64 && prec != GX_F32) {
+    set_error("field precision %d not supported", prec);
+    return GX_EINVAL;
+  }
+  Field *f = new Field();
+  f->ctx = ctx;
+  f->g = g;
+  f->prec = prec;
+  f->subset = subset;
+  f->cb = subset == GX_FULL ? 0 : subset;
+  f->bytes = (size_t)f->elems() * f->csize();
+  hipError_t e = hipMalloc(&f->data, f->bytes);
+  if (e != hipSuccess) {
+    delete f;
+    set_error("hipMalloc(%zu) failed: %s", (size_t)f->elems() * 16, hipGetErrorString(e));
+    return GX_ENOMEM;
+  }
+  hipMemsetAsync(f->data, 0, f->bytes, ctx->stream);
+  *out = f;
+  return GX_OK;
+}
+
+void field_destroy(Field *f) {
+  if (!f) return;
+  if (f->data) hipFree(f->data);
+  delete f;
+}
+
+// A non-owning half-field view of one checkerboard of a full field.
+static Field half_view(const Field *full, int cb) {
+  Field v = *full;
+  v.data = full->half(cb);
+  v.subset = cb;
+  v.cb = cb;
+  return v;
+}
+
+Field *op_tmp(Op *op, int prec, int i) {
+  Field *&t = op->tmp[prec][i];
+  if (!t) {
+    if (field_create(op->ctx, op->g, prec, GX_EVEN, &t) != GX_OK) return nullptr;
+  }
+  return t;
+}
+
+// ---------------- reductions ----------------
+int reduce_read(Ctx *ctx, int slot, int n, double *out) {
+  GX_CHECK_HIP(hipMemcpyAsync(ctx->pinned + slot, ctx->red_result + slot, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < n; i++) out[i] = ctx->pinned[slot + i];
+  return GX_OK;
+}
+
+static int64_t nsites_of(const Field *f) { return f->g.half_elems() * (f->subset == GX_FULL ? 2 : 1); }
+
+static bool conformable(const Field *a, const Field *b) {
+  return a && b && same_geom(a->g, b->g) && a->prec == b->prec && (a->subset == GX_FULL) == (b->subset == GX_FULL);
+}
+
+int blas_norm2(Field *x, double *nrm) {
+  int rc = x->prec == GX_F64
+               ? launch_reduce<double>(x->ctx, 1, nsites_of(x), x->g.half_elems(), x->data, nullptr, nullptr, 0, 0)
+               : launch_reduce<float>(x->ctx, 1, nsites_of(x), x->g.half_elems(), x->data, nullptr, nullptr, 0, 0);
+  GX_TRY(rc);
+  return reduce_read(x->ctx, 0, 1, nrm);
+}
+
+int blas_inner(Field *x, Field *y, double *re, double *im) {
+  int rc = x->prec == GX_F64
+               ? launch_reduce<double>(x->ctx, 0, nsites_of(x), x->g.half_elems(), x->data, y->data, nullptr, 0, 0)
+               : launch_reduce<float>(x->ctx, 0, nsites_of(x), x->g.half_elems(), x->data, y->data, nullptr, 0, 0);
+  GX_TRY(rc);
+  double v[2];
+  GX_TRY(reduce_read(x->ctx, 0, 2, v));
+  *re = v[0];
+  if (im) *im = v[1];
+  return GX_OK;
+}
+
+int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm) {
+  int rc = x->prec == GX_F64
+               ? launch_reduce<double>(x->ctx, 2, nsites_of(x), x->g.half_elems(), x->data, y->data, z->data, a, 0)
+               : launch_reduce<float>(x->ctx, 2, nsites_of(x), x->g.half_elems(), x->data, y->data, z->data, a, 0);
+  GX_TRY(rc);
+  z->cb = x->cb;
+  return reduce_read(x->ctx, 0, 1, nrm);
+}
+
+int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const CGState *gate) {
+  const int64_t n = x->elems();
+  z->cb = x->cb;
+  if (x->prec == GX_F64) return launch_axpby<double>(n, a, x->data, b, y ? y->data : nullptr, z->data, gate, x->ctx->stream);
+  return launch_axpby<float>(n, a, x->data, b, y ? y->data : nullptr, z->data, gate, x->ctx->stream);
+}
+
+int convert_field(Field *out, Field *in) {
+  const int64_t n = in->elems() * 2;  // reals
+  out->cb = in->cb;
+  if (in->prec == out->prec) {
+    GX_CHECK_HIP(hipMemcpyAsync(out->data, in->data, in->bytes, hipMemcpyDeviceToDevice, in->ctx->stream));
+    return GX_OK;
+  }
+  if (in->prec == GX_F64 && out->prec == GX_F32) return launch_convert<double, float>(n, in->data, out->data, in->ctx->stream);
+  if (in->prec == GX_F32 && out->prec == GX_F64) return launch_convert<float, double>(n, in->data, out->data, in->ctx->stream);
+  set_error("precision change %d -> %d unsupported", in->prec, out->prec);
+  return GX_EINVAL;
+}
+
+// ---------------- gauge field: DoubleStore into the device layout ----------------
+// Uds[mu] = -1/2 U_mu(x) * phase (x_mu = L-1); Uds[mu+4] = -1/2 U_mu(x-mu)^dag * conj(phase) (x_mu = 0)
+// WilsonImpl.h:127-171, WilsonFermionImplementation.h:82-107.
+static void double_store(const Geom &g, const double *U, const double *ph, int cb, std::vector<double> &out) {
+  const int64_t H4 = g.H4;
+  out.assign((size_t)8 * 9 * H4 * 2, 0.0);
+  parallel_for(H4, [&](int64_t b, int64_t e) {
+    for (int64_t h = b; h < e; h++) {
+      int x[4];
+      coor_half(g, h, cb, x);
+      const int64_t s = lex_index(g, x);
+      for (int mu = 0; mu < 4; mu++) {
+        const double pr = ph[2 * mu], pi = ph[2 * mu + 1];
+        // forward
+        {
+          const double *u = U + (s * 4 + mu) * 18;
+          const bool edge = (x[mu] == g.L[mu] - 1);
+          for (int k = 0; k < 9; k++) {
+            double re = u[2 * k], im = u[2 * k + 1];
+            if (edge) {
+              const double r2 = re * pr - im * pi, i2 = re * pi + im * pr;
+              re = r2;
+              im = i2;
+            }
+            double *o = &out[(((size_t)mu * 9 + k) * H4 + h) * 2];
+            o[0] = -0.5 * re;
+            o[1] = -0.5 * im;
+          }
+        }
+        // backward: U_mu(x-mu)^dag
+        {
+          int y[4] = {x[0], x[1], x[2], x[3]};
+          y[mu] = (x[mu] - 1 + g.L[mu]) % g.L[mu];
+          const double *u = U + (lex_index(g, y) * 4 + mu) * 18;
+          const bool edge = (x[mu] == 0);
+          for (int a = 0; a < 3; a++)
+            for (int bb = 0; bb < 3; bb++) {
+              double re = u[2 * (bb * 3 + a)], im = -u[2 * (bb * 3 + a) + 1];
+              if (edge) {  // * conj(phase)
+                const double r2 = re * pr + im * pi, i2 = im * pr - re * pi;
+                re = r2;
+                im = i2;
+              }
+              double *o = &out[(((size_t)(mu + 4) * 9 + a * 3 + bb) * H4 + h) * 2];
+              o[0] = -0.5 * re;
+              o[1] = -0.5 * im;
+            }
+        }
+      }
+    }
+  });
+}
+
+static void free_gauge(Op *op) {
+  for (int p = 0; p < 3; p++)
+    for (int cb = 0; cb < 2; cb++)
+      if (op->U[p][cb]) {
+        hipFree(op->U[p][cb]);
+        op->U[p][cb] = nullptr;
+      }
+}
+
+static int upload_gauge(Op *op) {
+  free_gauge(op);
+  const size_t n = (size_t)8 * 9 * op->g.H4;  // complex per checkerboard
+  for (int cb = 0; cb < 2; cb++) {
+    std::vector<double> h;
+    double_store(op->g, op->host_U.data(), op->phases, cb, h);
+    GX_CHECK_HIP(hipMalloc(&op->U[GX_F64][cb], n * 16));
+    GX_CHECK_HIP(hipMemcpy(op->U[GX_F64][cb], h.data(), n * 16, hipMemcpyHostToDevice));
+  }
+  return GX_OK;
+}
+
+int op_gauge(Op *op, int prec, void **Ue, void **Uo) {
+  if (!op->U[prec][0]) {
+    const size_t n = (size_t)8 * 9 * op->g.H4;
+    if (prec != GX_F32) {
+      set_error("gauge precision %d unsupported", prec);
+      return GX_EINVAL;
+    }
+    for (int cb = 0; cb < 2; cb++) {
+      GX_CHECK_HIP(hipMalloc(&op->U[prec][cb], n * 8));
+      GX_TRY((launch_convert<double, float>(2 * n, op->U[GX_F64][cb], op->U[prec][cb], op->ctx->stream)));
+    }
+  }
+  *Ue = op->U[prec][0];
+  *Uo = op->U[prec][1];
+  return GX_OK;
+}
+
+// ---------------- Cayley coefficients and fifth-dimension matrices ----------------
+// SetCoefficientsInternal, CayleyFermion5DImplementation.h:410-535, Mobius tanh (omega_s = 1).
+static void cayley_coeffs(Op *op) {
+  const int Ls = op->g.Ls;
+  const double b = op->b, c = op->c, M5 = op->M5, mass = op->mass;
+  auto &bs = op->bs, &cs = op->cs, &bee = op->bee, &cee = op->cee;
+  bs.assign(Ls, 0);
+  cs.assign(Ls, 0);
+  bee.assign(Ls, 0);
+  cee.assign(Ls, 0);
+  const double bpc = b + c, bmc = b - c;
+  for (int i = 0; i < Ls; i++) {
+    bs[i] = 0.5 * (bpc + bmc);
+    cs[i] = 0.5 * (bpc - bmc);
+    bee[i] = bs[i] * (4.0 - M5) + 1.0;
+    cee[i] = 1.0 - cs[i] * (4.0 - M5);
+  }
+  const std::vector<double> &aee = cee;
+  op->dee.assign(Ls, 0);
+  op->lee.assign(Ls, 0);
+  op->leem.assign(Ls, 0);
+  op->uee.assign(Ls, 0);
+  op->ueem.assign(Ls, 0);
+  for (int i = 0; i < Ls; i++) {
+    op->dee[i] = bee[i];
+    if (i < Ls - 1) {
+      op->lee[i] = -cee[i + 1] / bee[i];
+      op->leem[i] = mass * cee[Ls - 1] / bee[0];
+      for (int j = 0; j < i; j++) op->leem[i] *= aee[j] / bee[j + 1];
+      op->uee[i] = -aee[i] / bee[i];
+      op->ueem[i] = mass;
+      for (int j = 1; j <= i; j++) op->ueem[i] *= cee[j] / bee[j];
+      op->ueem[i] *= aee[0] / bee[0];
+    }
+  }
+  double delta_d = mass * cee[Ls - 1];
+  for (int j = 0; j < Ls - 1; j++) delta_d *= cee[j] / bee[j];
+  op->dee[Ls - 1] += delta_d;
+}
+
+// Matrices of a Grid M5D-type kernel (CayleyFermion5Dcache.h:41-115):
+//   non-dag: chi_s = diag_s phi_s + upper_s P- psi_{s+1} + lower_s P+ psi_{s-1}
+//   dag:     chi_s = diag_s phi_s + upper_s P+ psi_{s+1} + lower_s P- psi_{s-1}
+// P+ acts on spins 0,1 (matrix 0), P- on spins 2,3 (matrix 1).
+static void m5d_matrix(int Ls, const std::vector<double> &lower, const std::vector<double> &diag,
+                       const std::vector<double> &upper, bool dag, SMat &out) {
+  out.m.assign(2 * Ls * Ls, 0.0);
+  double *Ap = out.m.data(), *Am = out.m.data() + Ls * Ls;
+  for (int s = 0; s < Ls; s++) {
+    const int su = (s + 1) % Ls, sl = (s + Ls - 1) % Ls;
+    Ap[s * Ls + s] += diag[s];
+    Am[s * Ls + s] += diag[s];
+    if (!dag) {
+      Am[s * Ls + su] += upper[s];
+      Ap[s * Ls + sl] += lower[s];
+    } else {
+      Ap[s * Ls + su] += upper[s];
+      Am[s * Ls + sl] += lower[s];
+    }
+  }
+}
+
+// MooeeInv / MooeeInvDag as dense per-chirality matrices, built column by column by running
+// Grid's LDU sweeps (CayleyFermion5Dcache.h:117-230) on unit vectors in FP64.
+static void mooee_inv_matrix(const Op *op, bool dag, SMat &out) {
+  const int Ls = op->g.Ls;
+  out.m.assign(2 * Ls * Ls, 0.0);
+  for (int chi = 0; chi < 2; chi++) {
+    const bool up = (chi == 0);
+    const bool carry = dag ? !up : up;
+    const double *l1 = dag ? op->uee.data() : op->lee.data();
+    const double *l2 = dag ? op->ueem.data() : op->leem.data();
+    const double *u1 = dag ? op->lee.data() : op->uee.data();
+    const double *u2 = dag ? op->leem.data() : op->ueem.data();
+    for (int col = 0; col < Ls; col++) {
+      std::vector<double> psi(Ls, 0.0), res_v(Ls, 0.0);
+      psi[col] = 1.0;
+      double tmp = 0, acc = 0, res;
+      if (Ls == 1) {
+        res_v[0] = psi[0] / op->dee[0];
+      } else {
+        res = psi[0];
+        if (carry) tmp = res; else acc = l2[0] * res;
+        res_v[0] = res;
+        for (int s = 1; s < Ls - 1; s++) {
+          res = psi[s];
+          if (carry) res -= l1[s - 1] * tmp;
+          if (carry) tmp = res; else acc += l2[s] * res;
+          res_v[s] = res;
+        }
+        res = psi[Ls - 1];
+        if (carry) res -= l1[Ls - 2] * tmp;
+        if (!carry) res -= acc;
+        res = (1.0 / op->dee[Ls - 1]) * res;
+        res_v[Ls - 1] = res;
+        const double accb = carry ? res : 0.0;
+        double tmpb = carry ? 0.0 : res;
+        for (int s = Ls - 2; s >= 0; s--) {
+          res = (1.0 / op->dee[s]) * res_v[s] - u1[s] * tmpb - u2[s] * accb;
+          tmpb = carry ? 0.0 : res;
+          res_v[s] = res;
+        }
+      }
+      double *A = out.m.data() + chi * Ls * Ls;
+      for (int s = 0; s < Ls; s++) A[s * Ls + col] = res_v[s];
+    }
+  }
+}
+
+static void build_smats(Op *op) {
+  const int Ls = op->g.Ls;
+  const double mass = op->mass;
+  std::vector<double> lower(Ls), diag(Ls), upper(Ls);
+  // Meooe5D: diag bs, upper/lower cs, corners -mass*cs (CayleyFermion5DImplementation.h:165-174)
+  for (int s = 0; s < Ls; s++) { diag[s] = op->bs[s]; upper[s] = op->cs[s]; lower[s] = op->cs[s]; }
+  upper[Ls - 1] = -mass * upper[Ls - 1];
+  lower[0] = -mass * lower[0];
+  m5d_matrix(Ls, lower, diag, upper, false, op->smat[Op::S_MEOOE5D]);
+  // MeooeDag5D (:249-271)
+  for (int s = 0; s < Ls; s++) {
+    diag[s] = op->bs[s];
+    if (Ls == 1) { upper[s] = -mass * op->cs[0]; lower[s] = -mass * op->cs[0]; continue; }
+    if (s == 0) { upper[s] = op->cs[s + 1]; lower[s] = -mass * op->cs[Ls - 1]; }
+    else if (s == Ls - 1) { upper[s] = -mass * op->cs[0]; lower[s] = op->cs[s - 1]; }
+    else { upper[s] = op->cs[s + 1]; lower[s] = op->cs[s - 1]; }
+  }
+  m5d_matrix(Ls, lower, diag, upper, true, op->smat[Op::S_MEOOEDAG5D]);
+  // Mooee (:190-204)
+  for (int s = 0; s < Ls; s++) { diag[s] = op->bee[s]; upper[s] = -op->cee[s]; lower[s] = -op->cee[s]; }
+  upper[Ls - 1] = -mass * upper[Ls - 1];
+  lower[0] = -mass * lower[0];
+  m5d_matrix(Ls, lower, diag, upper, false, op->smat[Op::S_MOOEE]);
+  // MooeeDag (:205-233)
+  for (int s = 0; s < Ls; s++) {
+    diag[s] = op->bee[s];
+    if (Ls == 1) { upper[s] = mass * op->cee[0]; lower[s] = mass * op->cee[0]; continue; }
+    if (s == 0) { upper[s] = -op->cee[s + 1]; lower[s] = mass * op->cee[Ls - 1]; }
+    else if (s == Ls - 1) { upper[s] = mass * op->cee[0]; lower[s] = -op->cee[s - 1]; }
+    else { upper[s] = -op->cee[s + 1]; lower[s] = -op->cee[s - 1]; }
+  }
+  m5d_matrix(Ls, lower, diag, upper, true, op->smat[Op::S_MOOEEDAG]);
+  mooee_inv_matrix(op, false, op->smat[Op::S_MOOEEINV]);
+  mooee_inv_matrix(op, true, op->smat[Op::S_MOOEEINVDAG]);
+  // M5D / M5Ddag off-diagonal parts (diag 1 applies to phi): upper -1, lower -1, corners +mass
+  for (int s = 0; s < Ls; s++) { diag[s] = 0; upper[s] = -1; lower[s] = -1; }
+  upper[Ls - 1] = mass;
+  lower[0] = mass;
+  m5d_matrix(Ls, lower, diag, upper, false, op->smat[Op::S_M5D_OFF]);
+  m5d_matrix(Ls, lower, diag, upper, true, op->smat[Op::S_M5DDAG_OFF]);
+}
+
+static int upload_smats(Op *op) {
+  const int Ls = op->g.Ls;
+  const size_t per = 2 * (size_t)Ls * Ls;
+  std::vector<double> all(per * Op::S_COUNT);
+  for (int i = 0; i < Op::S_COUNT; i++) std::memcpy(&all[i * per], op->smat[i].m.data(), per * sizeof(double));
+  std::vector<float> allf(all.begin(), all.end());
+  for (int p = 0; p < 2; p++)
+    if (op->smat_dev[p]) {
+      hipFree(op->smat_dev[p]);
+      op->smat_dev[p] = nullptr;
+    }
+  GX_CHECK_HIP(hipMalloc(&op->smat_dev[GX_F64], all.size() * sizeof(double)));
+  GX_CHECK_HIP(hipMemcpy(op->smat_dev[GX_F64], all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+  GX_CHECK_HIP(hipMalloc(&op->smat_dev[GX_F32], allf.size() * sizeof(float)));
+  GX_CHECK_HIP(hipMemcpy(op->smat_dev[GX_F32], allf.data(), allf.size() * sizeof(float), hipMemcpyHostToDevice));
+  return GX_OK;
+}
+
+int op_smat(Op *op, int prec, int which, const void **mats) {
+  const size_t per = 2 * (size_t)op->g.Ls * op->g.Ls;
+  if (prec == GX_F64) *mats = (const double *)op->smat_dev[GX_F64] + which * per;
+  else *mats = (const float *)op->smat_dev[GX_F32] + which * per;
+  return GX_OK;
+}
+
+// ---------------- operator building blocks (half fields) ----------------
+int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const CGState *gate) {
+  void *Ue, *Uo;
+  if (in->prec == GX_F64) {
+    Ue = op->U[GX_F64][0];
+    Uo = op->U[GX_F64][1];
+  } else {
+    GX_TRY(op_gauge(op, in->prec, &Ue, &Uo));
+  }
+  void *U = cb_out == 0 ? Ue : Uo;
+  out->cb = cb_out;
+  if (in->prec == GX_F64)
+    return launch_dhop<double>(op->g, in->data, out->data, U, cb_out, dag, gate, op->ctx->stream);
+  return launch_dhop<float>(op->g, in->data, out->data, U, cb_out, dag, gate, op->ctx->stream);
+}
+
+static int apply_smat(Op *op, Field *in, Field *phi, Field *out, int which, const CGState *gate) {
+  const void *m;
+  GX_TRY(op_smat(op, in->prec, which, &m));
+  out->cb = in->cb;
+  if (in->prec == GX_F64)
+    return launch_smat<double>(op->g, in->data, phi ? phi->data : nullptr, out->data, m, gate, op->ctx->stream);
+  return launch_smat<float>(op->g, in->data, phi ? phi->data : nullptr, out->data, m, gate, op->ctx->stream);
+}
+
+int apply_meooe(Op *op, Field *in, Field *out, int dag, const CGState *gate) {
+  const int cb_out = 1 - in->cb;
+  if (op->kind == GX_WILSON) return apply_dhop_half(op, in, out, cb_out, dag, gate);
+  Field *t = op_tmp(op, in->prec, 3);
+  if (!t) return GX_ENOMEM;
+  if (!dag) {
+    GX_TRY(apply_smat(op, in, nullptr, t, Op::S_MEOOE5D, gate));
+    return apply_dhop_half(op, t, out, cb_out, 0, gate);
+  }
+  GX_TRY(apply_dhop_half(op, in, t, cb_out, 1, gate));
+  return apply_smat(op, t, nullptr, out, Op::S_MEOOEDAG5D, gate);
+}
+
+// which: 0 Mooee, 1 MooeeDag, 2 MooeeInv, 3 MooeeInvDag
+int apply_mooee(Op *op, Field *in, Field *out, int which, const CGState *gate) {
+  if (op->kind == GX_WILSON) {
+    const double d = 4.0 + op->mass;
+    return blas_axpby(out, which < 2 ? d : 1.0 / d, in, 0.0, nullptr, gate);
+  }
+  static const int map[4] = {Op::S_MOOEE, Op::S_MOOEEDAG, Op::S_MOOEEINV, Op::S_MOOEEINVDAG};
+  return apply_smat(op, in, nullptr, out, map[which], gate);
+}
+
+// SchurDiagMooeeOperator::Mpc / MpcDag, LinearOperator.h:330-348
+int apply_mpc(Op *op, Field *in, Field *out, int dag, const CGState *gate) {
+  Field *t1 = op_tmp(op, in->prec, 0), *t2 = op_tmp(op, in->prec, 1);
+  if (!t1 || !t2) return GX_ENOMEM;
+  GX_TRY(apply_meooe(op, in, t1, dag, gate));
+  GX_TRY(apply_mooee(op, t1, t2, dag ? 3 : 2, gate));
+  GX_TRY(apply_meooe(op, t2, t1, dag, gate));
+  GX_TRY(apply_mooee(op, in, out, dag ? 1 : 0, gate));
+  GX_TRY(blas_axpby(out, 1.0, out, -1.0, t1, gate));
+  out->cb = in->cb;
+  return GX_OK;
+}
+
+int apply_mpcdagmpc(Op *op, Field *in, Field *out, const CGState *gate) {
+  Field *t = op_tmp(op, in->prec, 2);
+  if (!t) return GX_ENOMEM;
+  GX_TRY(apply_mpc(op, in, t, 0, gate));
+  return apply_mpc(op, t, out, 1, gate);
+}
+
+// host (lexicographic, s innermost, AoS) <-> device (EO split, SoA) index maps
+template <typename T>
+static void gather_to_device_layout(const Field *f, const double *host, T *stage) {
+  const Geom &g = f->g;
+  const int Ls = g.Ls;
+  const int64_t st = g.half_elems();
+  if (f->subset == GX_FULL) {
+    parallel_for(g.V4, [&](int64_t b, int64_t e) {
+      for (int64_t s4 = b; s4 < e; s4++) {
+        int x[4];
+        coor_lex(g, s4, x);
+        const int cb = (x[0] + x[1] + x[2] + x[3]) & 1;
+        const int64_t h = half_index(g, x);
+        for (int s = 0; s < Ls; s++)
+          for (int k = 0; k < 12; k++) {
+            const double *src = host + ((s4 * Ls + s) * 12 + k) * 2;
+            T *dst = stage + (((int64_t)cb * 12 + k) * st + h * Ls + s) * 2;
+            dst[0] = (T)src[0];
+            dst[1] = (T)src[1];
+          }
+      }
+    });
+  } else {
+    parallel_for(g.H4, [&](int64_t b, int64_t e) {
+      for (int64_t h = b; h < e; h++)
+        for (int s = 0; s < Ls; s++)
+          for (int k = 0; k < 12; k++) {
+            const double *src = host + ((h * Ls + s) * 12 + k) * 2;
+            T *dst = stage + ((int64_t)k * st + h * Ls + s) * 2;
+            dst[0] = (T)src[0];
+            dst[1] = (T)src[1];
+          }
+    });
+  }
+}
+
+template <typename T>
+static void scatter_from_device_layout(const Field *f, const T *stage, double *host) {
+  const Geom &g = f->g;
+  const int Ls = g.Ls;
+  const int64_t st = g.half_elems();
+  if (f->subset == GX_FULL) {
+    parallel_for(g.V4, [&](int64_t b, int64_t e) {
+      for (int64_t s4 = b; s4 < e; s4++) {
+        int x[4];
+        coor_lex(g, s4, x);
+        const int cb = (x[0] + x[1] + x[2] + x[3]) & 1;
+        const int64_t h = half_index(g, x);
+        for (int s = 0; s < Ls; s++)
+          for (int k = 0; k < 12; k++) {
+            double *dst = host + ((s4 * Ls + s) * 12 + k) * 2;
+            const T *src = stage + (((int64_t)cb * 12 + k) * st + h * Ls + s) * 2;
+            dst[0] = (double)src[0];
+            dst[1] = (double)src[1];
+          }
+      }
+    });
+  } else {
+    parallel_for(g.H4, [&](int64_t b, int64_t e) {
+      for (int64_t h = b; h < e; h++)
+        for (int s = 0; s < Ls; s++)
+          for (int k = 0; k < 12; k++) {
+            double *dst = host + ((h * Ls + s) * 12 + k) * 2;
+            const T *src = stage + ((int64_t)k * st + h * Ls + s) * 2;
+            dst[0] = (double)src[0];
+            dst[1] = (double)src[1];
+          }
+    });
+  }
+}
+
+int apply_full_m(Op *op, Field *in, Field *out, int dag) {
+  for (int cb = 0; cb < 2; cb++) {
+    Field in_other = half_view(in, 1 - cb), in_same = half_view(in, cb), o = half_view(out, cb);
+    GX_TRY(apply_meooe(op, &in_other, &o, dag));
+    Field *t = op_tmp(op, in->prec, 1);
+    if (!t) return GX_ENOMEM;
+    t->cb = cb;
+    GX_TRY(apply_mooee(op, &in_same, t, dag ? 1 : 0));
+    GX_TRY(blas_axpby(&o, 1.0, &o, 1.0, t));
+  }
+  return GX_OK;
+}
+
+}  // namespace gx
+
+using namespace gx;
+
+struct gx_ctx_s : public Ctx {};
+struct gx_field_s : public Field {};
+struct gx_op_s : public Op {};
+
+static Field *F(gx_field f) { return static_cast<Field *>(f); }
+static Op *O(gx_op o) { return static_cast<Op *>(o); }
+
+#define GX_REQUIRE(cond, code, ...) \
+  do {                              \
+    if (!(cond)) {                  \
+      set_error(__VA_ARGS__);       \
+      return code;                  \
+    }                               \
+  } while (0)
+
+extern "C" {
+
+const char *gx_last_error(void) { return g_err; }
+const char *gx_version(void) { return "gx 0.1 (gfx950)"; }
+
+int gx_ctx_create(int device, gx_ctx *out) {
+  GX_REQUIRE(out, GX_EINVAL, "null output");
+  GX_CHECK_HIP(hipSetDevice(device));
+  gx_ctx c = new gx_ctx_s();
+  c->device = device;
+  GX_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int i = 0; i < 16; i++) GX_CHECK_HIP(hipEventCreate(&c->ev[i]));
+  GX_CHECK_HIP(hipMalloc(&c->red_partial, sizeof(double) * kRedBlocks * 4));
+  GX_CHECK_HIP(hipMalloc(&c->red_ticket, sizeof(unsigned int) * 16));
+  GX_CHECK_HIP(hipMemset(c->red_ticket, 0, sizeof(unsigned int) * 16));
+  GX_CHECK_HIP(hipMalloc(&c->red_result, sizeof(double) * kRedSlots));
+  GX_CHECK_HIP(hipHostMalloc(&c->pinned, sizeof(double) * kRedSlots, hipHostMallocDefault));
+  *out = c;
+  return GX_OK;
+}
+
+int gx_ctx_destroy(gx_ctx c) {
+  if (!c) return GX_OK;
+  hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 16; i++) hipEventDestroy(c->ev[i]);
+  hipFree(c->red_partial);
+  hipFree(c->red_ticket);
+  hipFree(c->red_result);
+  hipHostFree(c->pinned);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return GX_OK;
+}
+
+int gx_ctx_sync(gx_ctx c) {
+  GX_REQUIRE(c, GX_EINVAL, "null ctx");
+  GX_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_ctx_event_record(gx_ctx c, int slot) {
+  GX_REQUIRE(c && slot >= 0 && slot < 16, GX_EINVAL, "bad event slot");
+  GX_CHECK_HIP(hipEventRecord(c->ev[slot], c->stream));
+  return GX_OK;
+}
+
+int gx_ctx_event_elapsed(gx_ctx c, int s0, int s1, double *ms) {
+  GX_REQUIRE(c && s0 >= 0 && s0 < 16 && s1 >= 0 && s1 < 16 && ms, GX_EINVAL, "bad event slot");
+  GX_CHECK_HIP(hipEventSynchronize(c->ev[s1]));
+  float f = 0;
+  GX_CHECK_HIP(hipEventElapsedTime(&f, c->ev[s0], c->ev[s1]));
+  *ms = f;
+  return GX_OK;
+}
+
+// ---------------- fields ----------------
+int gx_field_create(gx_ctx ctx, const int dims[4], int Ls, int prec, int subset, gx_field *out) {
+  GX_REQUIRE(ctx && out && geom_ok(dims, Ls), GX_EINVAL, "bad field geometry");
+  GX_REQUIRE(subset >= -1 && subset <= 1, GX_EINVAL, "bad subset %d", subset);
+  Field *f;
+  GX_TRY(field_create(ctx, make_geom(dims, Ls), prec, subset, &f));
+  gx_field g = new gx_field_s();
+  *static_cast<Field *>(g) = *f;
+  delete f;
+  *out = g;
+  return GX_OK;
+}
+
+int gx_field_destroy(gx_field f) {
+  if (!f) return GX_OK;
+  if (f->data) hipFree(f->data);
+  delete f;
+  return GX_OK;
+}
+
+int gx_field_import(gx_field f, const double *host) {
+  GX_REQUIRE(f && host, GX_EINVAL, "null argument");
+  if (f->prec == GX_F64) {
+    std::vector<double> stage((size_t)f->elems() * 2);
+    gather_to_device_layout<double>(f, host, stage.data());
+    GX_CHECK_HIP(hipMemcpyAsync(f->data, stage.data(), f->bytes, hipMemcpyHostToDevice, f->ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
+  } else {
+    std::vector<float> stage((size_t)f->elems() * 2);
+    gather_to_device_layout<float>(f, host, stage.data());
+    GX_CHECK_HIP(hipMemcpyAsync(f->data, stage.data(), f->bytes, hipMemcpyHostToDevice, f->ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
+  }
+  return GX_OK;
+}
+
+int gx_field_export(gx_field f, double *host) {
+  GX_REQUIRE(f && host, GX_EINVAL, "null argument");
+  if (f->prec == GX_F64) {
+    std::vector<double> stage((size_t)f->elems() * 2);
+    GX_CHECK_HIP(hipMemcpyAsync(stage.data(), f->data, f->bytes, hipMemcpyDeviceToHost, f->ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
+    scatter_from_device_layout<double>(f, stage.data(), host);
+  } else {
+    std::vector<float> stage((size_t)f->elems() * 2);
+    GX_CHECK_HIP(hipMemcpyAsync(stage.data(), f->data, f->bytes, hipMemcpyDeviceToHost, f->ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
+    scatter_from_device_layout<float>(f, stage.data(), host);
+  }
+  return GX_OK;
+}
+
+int gx_field_checkerboard(gx_field f, int *cb) {
+  GX_REQUIRE(f && cb, GX_EINVAL, "null argument");
+  *cb = f->subset == GX_FULL ? GX_FULL : f->cb;
+  return GX_OK;
+}
+
+int gx_field_set_checkerboard_tag(gx_field f, int cb) {
+  GX_REQUIRE(f && f->subset != GX_FULL && (cb == 0 || cb == 1), GX_EINVAL, "tag applies to half fields");
+  f->cb = cb;
+  return GX_OK;
+}
+
+int gx_field_zero(gx_field f) {
+  GX_REQUIRE(f, GX_EINVAL, "null field");
+  GX_CHECK_HIP(hipMemsetAsync(f->data, 0, f->bytes, f->ctx->stream));
+  return GX_OK;
+}
+
+int gx_field_copy(gx_field dst, gx_field src) {
+  GX_REQUIRE(conformable(F(dst), F(src)), GX_EINVAL, "nonconformable copy");
+  GX_CHECK_HIP(hipMemcpyAsync(dst->data, src->data, src->bytes, hipMemcpyDeviceToDevice, src->ctx->stream));
+  dst->cb = src->cb;
+  return GX_OK;
+}
+
+int gx_pick_checkerboard(int cb, gx_field half, gx_field full) {
+  GX_REQUIRE(half && full && half->subset != GX_FULL && full->subset == GX_FULL && same_geom(half->g, full->g) &&
+                 half->prec == full->prec && (cb == 0 || cb == 1),
+             GX_EINVAL, "pickCheckerboard: nonconformable");
+  GX_CHECK_HIP(hipMemcpyAsync(half->data, full->half(cb), half->bytes, hipMemcpyDeviceToDevice, full->ctx->stream));
+  half->cb = cb;
+  return GX_OK;
+}
+
+int gx_set_checkerboard(gx_field full, gx_field half) {
+  GX_REQUIRE(half && full && half->subset != GX_FULL && full->subset == GX_FULL && same_geom(half->g, full->g) &&
+                 half->prec == full->prec,
+             GX_EINVAL, "setCheckerboard: nonconformable");
+  GX_CHECK_HIP(hipMemcpyAsync(full->half(half->cb), half->data, half->bytes, hipMemcpyDeviceToDevice, full->ctx->stream));
+  return GX_OK;
+}
+
+int gx_precision_change(gx_field out, gx_field in) {
+  GX_REQUIRE(out && in && same_geom(out->g, in->g) && (out->subset == GX_FULL) == (in->subset == GX_FULL), GX_EINVAL,
+             "precisionChange: nonconformable");
+  return convert_field(F(out), F(in));
+}
+
+// ---------------- BLAS1 ----------------
+int gx_axpy(gx_field z, double a, gx_field x, gx_field y) {
+  GX_REQUIRE(conformable(F(z), F(x)) && conformable(F(x), F(y)), GX_EINVAL, "axpy: nonconformable");
+  return blas_axpby(F(z), a, F(x), 1.0, F(y));
+}
+int gx_axpby(gx_field z, double a, double b, gx_field x, gx_field y) {
+  GX_REQUIRE(conformable(F(z), F(x)) && conformable(F(x), F(y)), GX_EINVAL, "axpby: nonconformable");
+  return blas_axpby(F(z), a, F(x), b, F(y));
+}
+int gx_scale(gx_field z, double a, gx_field x) {
+  GX_REQUIRE(conformable(F(z), F(x)), GX_EINVAL, "scale: nonconformable");
+  return blas_axpby(F(z), a, F(x), 0.0, nullptr);
+}
+int gx_axpy_norm(gx_field z, double a, gx_field x, gx_field y, double *nrm) {
+  GX_REQUIRE(conformable(F(z), F(x)) && conformable(F(x), F(y)) && nrm, GX_EINVAL, "axpy_norm: nonconformable");
+  return blas_axpy_norm(F(z), a, F(x), F(y), nrm);
+}
+int gx_norm2(gx_field x, double *nrm) {
+  GX_REQUIRE(x && nrm, GX_EINVAL, "null argument");
+  return blas_norm2(F(x), nrm);
+}
+int gx_inner_product(gx_field x, gx_field y, double *re, double *im) {
+  GX_REQUIRE(conformable(F(x), F(y)) && re, GX_EINVAL, "innerProduct: nonconformable");
+  return blas_inner(F(x), F(y), re, im);
+}
+int gx_inner_product_norm(gx_field x, gx_field y, double *re, double *im, double *nrm) {
+  GX_REQUIRE(conformable(F(x), F(y)) && re && nrm, GX_EINVAL, "innerProductNorm: nonconformable");
+  GX_TRY(blas_inner(F(x), F(y), re, im));
+  return blas_norm2(F(x), nrm);
+}
+
+// ---------------- operators ----------------
+static int op_common(gx_ctx ctx, const int dims[4], int Ls, double mass, const double *U, const double *phases,
+                     Op *op) {
+  op->ctx = ctx;
+  op->g = make_geom(dims, Ls);
+  op->mass = mass;
+  if (phases)
+    for (int i = 0; i < 8; i++) op->phases[i] = phases[i];
+  op->host_U.assign(U, U + (size_t)op->g.V4 * 4 * 18);
+  return upload_gauge(op);
+}
+
+int gx_wilson_create(gx_ctx ctx, const int dims[4], int Ls, double mass, const double *U, const double *phases,
+                     gx_op *out) {
+  GX_REQUIRE(ctx && out && U && geom_ok(dims, Ls), GX_EINVAL, "wilson: bad arguments");
+  gx_op op = new gx_op_s();
+  op->kind = GX_WILSON;
+  int rc = op_common(ctx, dims, Ls, mass, U, phases, op);
+  if (rc != GX_OK) {
+    gx_op_destroy(op);
+    return rc;
+  }
+  *out = op;
+  return GX_OK;
+}
+
+int gx_mobius_create(gx_ctx ctx, const int dims[4], int Ls, double mass, double M5, double b, double c,
+                     const double *U, const double *phases, gx_op *out) {
+  GX_REQUIRE(ctx && out && U && geom_ok(dims, Ls) && Ls >= 2, GX_EINVAL, "mobius: bad arguments");
+  gx_op op = new gx_op_s();
+  op->kind = GX_MOBIUS;
+  op->M5 = M5;
+  op->b = b;
+  op->c = c;
+  int rc = op_common(ctx, dims, Ls, mass, U, phases, op);
+  if (rc == GX_OK) {
+    cayley_coeffs(op);
+    build_smats(op);
+    rc = upload_smats(op);
+  }
+  if (rc != GX_OK) {
+    gx_op_destroy(op);
+    return rc;
+  }
+  *out = op;
+  return GX_OK;
+}
+
+int gx_op_destroy(gx_op op) {
+  if (!op) return GX_OK;
+  hipStreamSynchronize(op->ctx->stream);
+  free_gauge(op);
+  for (int p = 0; p < 3; p++) {
+    if (op->smat_dev[p]) hipFree(op->smat_dev[p]);
+    for (int i = 0; i < 10; i++) field_destroy(op->tmp[p][i]);
+  }
+  delete op;
+  return GX_OK;
+}
+
+int gx_op_import_gauge(gx_op op, const double *U) {
+  GX_REQUIRE(op && U, GX_EINVAL, "null argument");
+  hipStreamSynchronize(op->ctx->stream);
+  op->host_U.assign(U, U + (size_t)op->g.V4 * 4 * 18);
+  return upload_gauge(op);
+}
+
+int gx_op_set_mass(gx_op op, double mass) {
+  GX_REQUIRE(op, GX_EINVAL, "null op");
+  hipStreamSynchronize(op->ctx->stream);
+  op->mass = mass;
+  if (op->kind == GX_MOBIUS) {
+    cayley_coeffs(op);
+    build_smats(op);
+    return upload_smats(op);
+  }
+  return GX_OK;
+}
+
+static int check_io(Op *op, Field *in, Field *out, bool full) {
+  GX_REQUIRE(op && in && out, GX_EINVAL, "null argument");
+  GX_REQUIRE(same_geom(in->g, op->g) && same_geom(out->g, op->g), GX_EINVAL, "field geometry does not match operator");
+  GX_REQUIRE(in->prec == out->prec, GX_EINVAL, "in/out precision differ");
+  GX_REQUIRE(in->prec == GX_F64 || in->prec == GX_F32, GX_EINVAL, "operator precision unsupported");
+  GX_REQUIRE((in->subset == GX_FULL) == full && (out->subset == GX_FULL) == full, GX_EINVAL,
+             full ? "operator needs full fields" : "operator needs half fields");
+  GX_REQUIRE(in != out, GX_EINVAL, "in and out must differ");
+  return GX_OK;
+}
+
+int gx_M(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), true));
+  return apply_full_m(O(op), F(in), F(out), 0);
+}
+int gx_Mdag(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), true));
+  return apply_full_m(O(op), F(in), F(out), 1);
+}
+int gx_Meooe(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_meooe(O(op), F(in), F(out), 0);
+}
+int gx_MeooeDag(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_meooe(O(op), F(in), F(out), 1);
+}
+int gx_Mooee(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_mooee(O(op), F(in), F(out), 0);
+}
+int gx_MooeeDag(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_mooee(O(op), F(in), F(out), 1);
+}
+int gx_MooeeInv(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_mooee(O(op), F(in), F(out), 2);
+}
+int gx_MooeeInvDag(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_mooee(O(op), F(in), F(out), 3);
+}
+int gx_Dhop(gx_op op, gx_field in, gx_field out, int dag) {
+  GX_TRY(check_io(O(op), F(in), F(out), true));
+  for (int cb = 0; cb < 2; cb++) {
+    Field i = half_view(F(in), 1 - cb), o = half_view(F(out), cb);
+    GX_TRY(apply_dhop_half(O(op), &i, &o, cb, dag));
+  }
+  return GX_OK;
+}
+int gx_DhopEO(gx_op op, gx_field in, gx_field out, int dag) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  GX_REQUIRE(in->cb == GX_ODD, GX_ECB, "DhopEO: input must be on the odd checkerboard");
+  return apply_dhop_half(O(op), F(in), F(out), GX_EVEN, dag);
+}
+int gx_DhopOE(gx_op op, gx_field in, gx_field out, int dag) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  GX_REQUIRE(in->cb == GX_EVEN, GX_ECB, "DhopOE: input must be on the even checkerboard");
+  return apply_dhop_half(O(op), F(in), F(out), GX_ODD, dag);
+}
+
+static int m5d_common(gx_op op, gx_field psi, gx_field chi, int which, bool with_phi) {
+  GX_TRY(check_io(O(op), F(psi), F(chi), false));
+  GX_REQUIRE(op->kind == GX_MOBIUS, GX_EINVAL, "M5D family needs a Mobius operator");
+  return apply_smat(O(op), F(psi), with_phi ? F(chi) : nullptr, F(chi), which, nullptr);
+}
+int gx_M5D(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, chi, Op::S_M5D_OFF, true); }
+int gx_M5Ddag(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, chi, Op::S_M5DDAG_OFF, true); }
+int gx_Meooe5D(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, chi, Op::S_MEOOE5D, false); }
+int gx_MeooeDag5D(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, chi, Op::S_MEOOEDAG5D, false); }
+
+int gx_Mpc(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_mpc(O(op), F(in), F(out), 0);
+}
+int gx_MpcDag(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_mpc(O(op), F(in), F(out), 1);
+}
+int gx_MpcDagMpc(gx_op op, gx_field in, gx_field out) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  return apply_mpcdagmpc(O(op), F(in), F(out));
+}
+int gx_HermOpAndNorm(gx_op op, gx_field in, gx_field out, double *n1, double *n2) {
+  GX_TRY(check_io(O(op), F(in), F(out), false));
+  GX_TRY(apply_mpcdagmpc(O(op), F(in), F(out)));
+  double im;
+  GX_TRY(blas_inner(F(in), F(out), n1, &im));
+  return blas_norm2(F(out), n2);
+}
+
+}  // extern "C"

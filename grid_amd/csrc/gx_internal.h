@@ -1,0 +1,192 @@
+// Internal declarations of libgx: host-side objects and kernel launchers.
+//
+// Device layout (HBM), chosen for coalesced wave64 access on gfx950:
+//   * every field is stored per checkerboard; a full field is [even half | odd half];
+//   * a half field is structure-of-arrays over the 12 spin-colour components,
+//       data[k][h*Ls + s],  k = spin*3 + colour,  h = half-lattice 4D site (Grid's red-black
+//       order, x halved), s = fifth-dimension / right-hand-side index (innermost),
+//     so a wave whose lanes walk (s, h) with s fastest reads 64 consecutive complex numbers
+//     per component;
+//   * doubled gauge links per output checkerboard, SoA over sites:
+//       U_cb[p][k][h],  p = 0..7 (x+,y+,z+,t+,x-,y-,z-,t-), k = a*3+b;
+//     U_cb[p<4] = -1/2 U_mu(x) (boundary phase), U_cb[p>=4] = -1/2 U_mu(x-mu)^dag.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gx.h"
+
+namespace gx {
+
+void set_error(const char *fmt, ...);
+
+#define GX_CHECK_HIP(call)                                                                             \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      gx::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_));                 \
+      return GX_EHIP;                                                                                  \
+    }                                                                                                  \
+  } while (0)
+
+#define GX_TRY(call)                                                                                   \
+  do {                                                                                                 \
+    int r_ = (call);                                                                                   \
+    if (r_ != GX_OK) return r_;                                                                        \
+  } while (0)
+
+constexpr int kMaxLs = 16;         // fifth-dimension extent / right-hand sides per launch
+constexpr int kRedBlocks = 1024;   // fixed grid of the deterministic reductions
+constexpr int kRedSlots = 64;      // device result slots per context
+
+struct Geom {
+  int L[4];
+  int Ls;
+  int64_t V4, H4;
+  int64_t half_elems() const { return H4 * Ls; }   // complex numbers per component per checkerboard
+};
+
+// Device-resident solver scalars (one per running CG).  Kernels of iteration k run only while
+// k <= k_conv, so a host can enqueue iterations ahead and read the outcome later.
+struct CGState {
+  double c, cp, a, b, d, rsq;
+  int k;        // current iteration (1-based)
+  int k_conv;   // iteration at which cp <= rsq was first seen (INT_MAX until then)
+  int kmax;     // last iteration allowed (MaxIterations)
+  int pad;
+};
+
+// Per-RHS solver scalars for the multi-RHS CG (right-hand side = slice s).
+struct MrhsState {
+  double c[kMaxLs], cp[kMaxLs], a[kMaxLs], b[kMaxLs], d[kMaxLs], rsq[kMaxLs];
+  int k;
+  int k_conv[kMaxLs];
+  int active;   // number of right-hand sides still iterating
+  int nrhs;
+  int k_all;    // iteration at which the last RHS converged (INT_MAX until then)
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[16] = {};
+  double *red_partial = nullptr;     // [kRedBlocks * 4]
+  unsigned int *red_ticket = nullptr;
+  double *red_result = nullptr;      // [kRedSlots]
+  double *pinned = nullptr;          // host mirror of red_result
+  int next_slot = 0;
+};
+
+struct Field {
+  Ctx *ctx = nullptr;
+  Geom g{};
+  int prec = GX_F64;
+  int subset = GX_FULL;   // GX_FULL or a half field
+  int cb = 0;             // checkerboard tag of a half field
+  void *data = nullptr;   // half: 12*H4*Ls complex; full: 2x that (even then odd)
+  float *scale = nullptr; // FP16 storage: per (h, s) scale factor, per checkerboard
+  size_t bytes = 0;
+  int64_t elems() const { return 12 * g.half_elems() * (subset == GX_FULL ? 2 : 1); }
+  size_t csize() const { return prec == GX_F64 ? 16 : (prec == GX_F32 ? 8 : 4); }
+  // pointer to the half with checkerboard cb (full fields) or the data (half fields)
+  void *half(int which) const {
+    if (subset != GX_FULL) return data;
+    return (char *)data + (size_t)which * 12 * g.half_elems() * csize();
+  }
+  float *half_scale(int which) const {
+    if (!scale) return nullptr;
+    if (subset != GX_FULL) return scale;
+    return scale + (size_t)which * g.half_elems();
+  }
+};
+
+// Per-chirality Ls x Ls real matrices of a fifth-dimension operator, row-major:
+// mat[0] acts on spins 0,1 (P+), mat[1] on spins 2,3 (P-).
+struct SMat {
+  std::vector<double> m;   // 2*Ls*Ls
+};
+
+struct Op {
+  Ctx *ctx = nullptr;
+  int kind = GX_WILSON;
+  Geom g{};
+  double mass = 0, M5 = 0, b = 0, c = 0;
+  double phases[8] = {1, 0, 1, 0, 1, 0, 1, 0};
+  void *U[3][2] = {};           // [prec][output cb] doubled links
+  std::vector<double> host_U;   // host copy of the gauge field (lexicographic), for re-import
+  // Cayley coefficients (CayleyFermion5DImplementation.h:410-535)
+  std::vector<double> bs, cs, bee, cee, dee, lee, leem, uee, ueem;
+  // fifth-dimension operators as per-chirality matrices: 0 Meooe5D, 1 MeooeDag5D, 2 Mooee,
+  // 3 MooeeDag, 4 MooeeInv, 5 MooeeInvDag, 6 M5D off-diagonal part, 7 M5Ddag off-diagonal part
+  enum { S_MEOOE5D = 0, S_MEOOEDAG5D, S_MOOEE, S_MOOEEDAG, S_MOOEEINV, S_MOOEEINVDAG, S_M5D_OFF, S_M5DDAG_OFF, S_COUNT };
+  SMat smat[S_COUNT];
+  void *smat_dev[3] = {};       // [prec] device copy of all S_COUNT matrices (2*Ls*Ls each)
+  // per precision temporaries (half fields): 0-3 operator compositions, 4-6 CG vectors,
+  // 7-8 mixed-precision outer vectors
+  Field *tmp[3][10] = {};
+};
+
+// ---------------- launchers (gx_kernels.hip) ----------------
+// All pointers are device pointers to half fields in the layout above; T = double or float.
+template <typename T>
+int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const CGState *gate,
+                hipStream_t st);
+// out = A_chi(in) per chirality (+ dphi[s] * phi if phi), mats = 2*Ls*Ls device matrix in T
+template <typename T>
+int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const CGState *gate,
+                hipStream_t st);
+// z = a x + b y over n complex numbers (a, b real)
+template <typename T>
+int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const CGState *gate,
+                 hipStream_t st);
+// reductions into ctx slot(s); mode 0: <x|y> -> slot, slot+1 (re, im); 1: |x|^2 -> slot;
+// 2: z = a x + y and |z|^2 -> slot (fused axpy_norm)
+template <typename T>
+int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void *x, const void *y, void *z, double a,
+                  int slot);
+template <typename Tin, typename Tout>
+int launch_convert(int64_t n, const void *in, void *out, hipStream_t st);
+
+// CG iteration kernels on device scalars (gx_kernels.hip)
+template <typename T>
+int launch_cg_dot(Ctx *ctx, int64_t nsites, int64_t stride, const void *p, const void *mmp, CGState *state);
+template <typename T>
+int launch_cg_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const void *mmp, CGState *state);
+template <typename T>
+int launch_cg_update(Ctx *ctx, int64_t n, void *psi, void *p, const void *r, CGState *state);
+
+// multi-RHS CG kernels (slice s = right-hand side)
+template <typename T>
+int launch_mrhs_dot(Ctx *ctx, const Geom &g, const void *p, const void *mmp, MrhsState *state);
+template <typename T>
+int launch_mrhs_axpy_norm(Ctx *ctx, const Geom &g, void *r, const void *mmp, MrhsState *state);
+template <typename T>
+int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state);
+template <typename T>
+int launch_slice_norm(Ctx *ctx, const Geom &g, const void *x, double *out_dev);
+
+// ---------------- api helpers (gx_api.hip) ----------------
+int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out);
+void field_destroy(Field *f);
+Field *op_tmp(Op *op, int prec, int i);
+int op_gauge(Op *op, int prec, void **Ue, void **Uo);
+int op_smat(Op *op, int prec, int which, const void **mats);
+int reduce_read(Ctx *ctx, int slot, int n, double *out);
+
+// Operator building blocks on half fields (in/out precision must match)
+int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const CGState *gate = nullptr);
+int apply_meooe(Op *op, Field *in, Field *out, int dag, const CGState *gate = nullptr);
+int apply_mooee(Op *op, Field *in, Field *out, int which, const CGState *gate = nullptr);
+int apply_mpc(Op *op, Field *in, Field *out, int dag, const CGState *gate = nullptr);
+int apply_mpcdagmpc(Op *op, Field *in, Field *out, const CGState *gate = nullptr);
+int apply_full_m(Op *op, Field *in, Field *out, int dag);
+int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const CGState *gate = nullptr);
+int blas_norm2(Field *x, double *nrm);
+int blas_inner(Field *x, Field *y, double *re, double *im);
+int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm);
+int convert_field(Field *out, Field *in);
+
+}  // namespace gx

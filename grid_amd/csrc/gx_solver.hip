@@ -1,0 +1,309 @@
+// Even-odd solvers of libgx.
+//
+// ConjugateGradient (Grid/algorithms/iterative/ConjugateGradient.h:68-257): Hestenes-Stiefel CG
+// on HermOp = MpcDagMpc.  The iteration runs on device scalars: the reduction kernels compute
+// a = c/d and b = cp/c in their last block, and every kernel of iteration k checks
+// k <= min(k_conv, kmax) before doing anything, so the host enqueues iterations in batches and
+// reads the convergence flag without stalling the device between iterations.  The arithmetic is
+// the same sequence as Grid's loop: d = Re<p|A p>, a = c/d, r -= a A p, cp = |r|^2, b = cp/c,
+// psi += a p, p = b p + r, stop when cp <= tol^2 |src|^2, then the true residual is recomputed.
+//
+// MixedPrecisionConjugateGradient (ConjugateGradientMixedPrec.h:71-165): FP64 outer residual,
+// inner CG in FP32 with adaptive tolerance, FP64 accumulation of the solution, final FP64 CG.
+//
+// SchurRedBlackDiagMooeeSolve (SchurRedBlack.h:236-291,385-425).
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "gx_internal.h"
+
+namespace gx {
+
+struct CGWork {
+  CGState *dev = nullptr;
+  CGState *host = nullptr;  // pinned, 2 slots
+  hipEvent_t ev[2] = {};
+};
+
+static CGWork &cg_work(Op *op) {
+  static thread_local std::vector<std::pair<Op *, CGWork>> cache;
+  for (auto &e : cache)
+    if (e.first == op) return e.second;
+  cache.emplace_back(op, CGWork());
+  CGWork &w = cache.back().second;
+  hipMalloc(&w.dev, sizeof(CGState));
+  hipHostMalloc(&w.host, 2 * sizeof(CGState), hipHostMallocDefault);
+  hipEventCreateWithFlags(&w.ev[0], hipEventDisableTiming);
+  hipEventCreateWithFlags(&w.ev[1], hipEventDisableTiming);
+  return w;
+}
+
+template <typename T>
+static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGState *st) {
+  GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
+  Ctx *ctx = op->ctx;
+  const int64_t nsites = p->g.half_elems();
+  GX_TRY(launch_cg_dot<T>(ctx, nsites, nsites, p->data, mmp->data, st));
+  GX_TRY(launch_cg_axpy_norm<T>(ctx, nsites, nsites, r->data, mmp->data, st));
+  GX_TRY(launch_cg_update<T>(ctx, p->elems(), psi->data, p->data, r->data, st));
+  return GX_OK;
+}
+
+int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *out) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int prec = src->prec;
+  Field *p = op_tmp(op, prec, 4), *mmp = op_tmp(op, prec, 5), *r = op_tmp(op, prec, 6);
+  if (!p || !mmp || !r) return GX_ENOMEM;
+  p->cb = mmp->cb = r->cb = psi->cb = src->cb;
+  gx_solver_stats st{};
+  double ssq, guess, a;
+  GX_TRY(blas_norm2(src, &ssq));
+  GX_TRY(blas_norm2(psi, &guess));
+  if (std::isnan(guess) || std::isnan(ssq)) {
+    set_error("CG: NaN in source or guess");
+    return GX_ENAN;
+  }
+  Ctx *ctx = op->ctx;
+  if (guess == 0.0) {
+    GX_CHECK_HIP(hipMemcpyAsync(r->data, src->data, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    GX_CHECK_HIP(hipMemcpyAsync(p->data, src->data, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    a = ssq;
+  } else {
+    GX_TRY(apply_mpcdagmpc(op, psi, mmp));
+    GX_TRY(blas_axpby(r, 1.0, src, -1.0, mmp));
+    GX_CHECK_HIP(hipMemcpyAsync(p->data, r->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    GX_TRY(blas_norm2(p, &a));
+  }
+  double cp = a;
+  if (ssq == 0.0) {
+    GX_CHECK_HIP(hipMemsetAsync(psi->data, 0, psi->bytes, ctx->stream));
+    st.iterations = 1;
+    st.converged = 1;
+    st.true_residual = 0;
+    if (out) *out = st;
+    return GX_OK;
+  }
+  const double rsq = tol * tol * ssq;
+  if (cp <= rsq) {
+    st.iterations = 0;
+    st.converged = 1;
+    st.true_residual = std::sqrt(a / ssq);
+    st.computed_residual = st.true_residual;
+    if (out) *out = st;
+    return GX_OK;
+  }
+  CGWork &w = cg_work(op);
+  CGState init{};
+  init.c = cp;
+  init.cp = cp;
+  init.rsq = rsq;
+  init.k = 1;
+  init.k_conv = INT_MAX;
+  init.kmax = maxit;
+  w.host[0] = init;
+  GX_CHECK_HIP(hipMemcpyAsync(w.dev, &w.host[0], sizeof(CGState), hipMemcpyHostToDevice, ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+
+  // Batches of iterations; two batches in flight so the device never waits for the host.
+  const int64_t vol = op->g.half_elems();
+  const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
+  int enqueued = 0, slot = 0;
+  auto enqueue = [&](int s) -> int {
+    for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+      int rc = prec == GX_F64 ? cg_iteration<double>(op, psi, p, mmp, r, w.dev)
+                              : cg_iteration<float>(op, psi, p, mmp, r, w.dev);
+      GX_TRY(rc);
+    }
+    GX_CHECK_HIP(hipMemcpyAsync(&w.host[s], w.dev, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
+    GX_CHECK_HIP(hipEventRecord(w.ev[s], ctx->stream));
+    return GX_OK;
+  };
+  GX_TRY(enqueue(0));
+  GX_TRY(enqueue(1));
+  CGState fin{};
+  for (;;) {
+    GX_CHECK_HIP(hipEventSynchronize(w.ev[slot]));
+    fin = w.host[slot];
+    if (fin.k_conv != INT_MAX || fin.k > maxit) break;
+    if (enqueued >= maxit) {
+      // everything enqueued: wait for the other slot, which holds the last state
+      GX_CHECK_HIP(hipEventSynchronize(w.ev[1 - slot]));
+      fin = w.host[1 - slot];
+      break;
+    }
+    GX_TRY(enqueue(slot));
+    slot = 1 - slot;
+  }
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const bool converged = fin.k_conv != INT_MAX;
+  st.iterations = converged ? fin.k_conv : maxit;
+  st.converged = converged;
+  st.computed_residual = std::sqrt(fin.cp / ssq);
+  // true residual: |MpcDagMpc psi - src| / |src|   (ConjugateGradient.h:201-208)
+  GX_TRY(apply_mpcdagmpc(op, psi, mmp));
+  GX_TRY(blas_axpby(p, 1.0, mmp, -1.0, src));
+  double rn, sn;
+  GX_TRY(blas_norm2(p, &rn));
+  GX_TRY(blas_norm2(src, &sn));
+  st.true_residual = std::sqrt(rn) / std::sqrt(sn);
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (out) *out = st;
+  if (!converged) {
+    set_error("CG did not converge in %d iterations (residual %g)", maxit, st.computed_residual);
+    return GX_ENOCONV;
+  }
+  return GX_OK;
+}
+
+int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int inner_prec, int maxinner,
+                 int maxouter, gx_solver_stats *out) {
+  auto t0 = std::chrono::steady_clock::now();
+  gx_solver_stats st{};
+  const int cb = src_in->cb;
+  sol->cb = cb;
+  double src_norm;
+  GX_TRY(blas_norm2(src_in, &src_norm));
+  const double stop = src_norm * tol * tol;
+  Field *tmp_d = op_tmp(op, GX_F64, 7), *src_d = op_tmp(op, GX_F64, 8);
+  Field *src_f = op_tmp(op, inner_prec, 7), *sol_f = op_tmp(op, inner_prec, 8);
+  if (!tmp_d || !src_d || !src_f || !sol_f) return GX_ENOMEM;
+  tmp_d->cb = src_d->cb = src_f->cb = sol_f->cb = cb;
+  double inner_tol = inner_tol0;
+  int outer = 0;
+  const double OuterLoopNormMult = 100.0;
+  for (outer = 0; outer < maxouter; outer++) {
+    GX_TRY(apply_mpcdagmpc(op, sol, tmp_d));
+    double norm;
+    GX_TRY(blas_axpy_norm(src_d, -1.0, tmp_d, src_in, &norm));   // src_d = src - A sol
+    if (norm < OuterLoopNormMult * stop) break;
+    while (norm * inner_tol * inner_tol < stop) inner_tol *= 2;
+    GX_TRY(convert_field(src_f, src_d));
+    GX_CHECK_HIP(hipMemsetAsync(sol_f->data, 0, sol_f->bytes, op->ctx->stream));
+    gx_solver_stats ist{};
+    int rc = cg_run(op, src_f, sol_f, inner_tol, maxinner, &ist);
+    if (rc != GX_OK && rc != GX_ENOCONV) return rc;
+    st.inner_iterations += ist.iterations;
+    GX_TRY(convert_field(tmp_d, sol_f));
+    GX_TRY(blas_axpby(sol, 1.0, sol, 1.0, tmp_d));
+  }
+  st.outer_iterations = outer;
+  gx_solver_stats fst{};
+  const int rc = cg_run(op, src_in, sol, tol, maxinner, &fst);
+  st.final_iterations = fst.iterations;
+  st.iterations = st.inner_iterations + fst.iterations;
+  st.true_residual = fst.true_residual;
+  st.computed_residual = fst.computed_residual;
+  st.converged = fst.converged;
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (out) *out = st;
+  return rc;
+}
+
+}  // namespace gx
+
+using namespace gx;
+
+struct gx_field_s : public Field {};
+struct gx_op_s : public Op {};
+
+extern "C" {
+
+int gx_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver_stats *st) {
+  if (!op || !src || !psi || src == psi) {
+    set_error("cg: bad arguments");
+    return GX_EINVAL;
+  }
+  Field *s = src, *p = psi;
+  if (s->subset == GX_FULL || p->subset == GX_FULL || s->prec != p->prec || (s->prec != GX_F64 && s->prec != GX_F32)) {
+    set_error("cg: needs half fields of one precision (F64 or F32)");
+    return GX_EINVAL;
+  }
+  return cg_run(op, s, p, tol, maxit, st);
+}
+
+int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_tol, int inner_prec, int maxinner,
+                int maxouter, gx_solver_stats *st) {
+  if (!op || !src || !psi || src == psi || src->prec != GX_F64 || psi->prec != GX_F64 ||
+      src->subset == GX_FULL || psi->subset == GX_FULL) {
+    set_error("mixed_cg: needs F64 half fields");
+    return GX_EINVAL;
+  }
+  if (inner_prec != GX_F32) {
+    set_error("mixed_cg: inner precision %d unsupported", inner_prec);
+    return GX_EINVAL;
+  }
+  return mixed_cg_run(op, src, psi, tol, inner_tol, inner_prec, maxinner, maxouter, st);
+}
+
+int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxit, int mixed, gx_solver_stats *st,
+                   double *unprec_residual) {
+  Op *op = op_;
+  Field *src = src_, *sol = sol_;
+  if (!op || !src || !sol || src->subset != GX_FULL || sol->subset != GX_FULL || src->prec != GX_F64 ||
+      sol->prec != GX_F64) {
+    set_error("schur_solve: needs full F64 fields");
+    return GX_EINVAL;
+  }
+  Field *se, *so, *t, *Mt, *solo;
+  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_EVEN, &se));
+  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_ODD, &so));
+  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_EVEN, &t));
+  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_EVEN, &Mt));
+  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_ODD, &solo));
+  hipStream_t s = op->ctx->stream;
+  const size_t hb = se->bytes;
+  int rc = GX_OK;
+  do {
+    // RedBlackSource: src_o = MpcDag(src_o - Moe Mee^-1 src_e)
+    if ((rc = (hipMemcpyAsync(se->data, src->half(0), hb, hipMemcpyDeviceToDevice, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
+    if ((rc = (hipMemcpyAsync(so->data, src->half(1), hb, hipMemcpyDeviceToDevice, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
+    se->cb = 0;
+    so->cb = 1;
+    if ((rc = apply_mooee(op, se, t, 2))) break;          // tmp = MooeeInv src_e   (Even)
+    if ((rc = apply_meooe(op, t, Mt, 0))) break;          // Mtmp = Meooe tmp       (Odd)
+    if ((rc = blas_axpby(t, 1.0, so, -1.0, Mt))) break;   // tmp = src_o - Mtmp     (Odd)
+    if ((rc = apply_mpc(op, t, so, 1))) break;            // src_o = MpcDag tmp
+    so->cb = 1;
+    // RedBlackSolve with the zero guess
+    solo->cb = 1;
+    hipMemsetAsync(solo->data, 0, solo->bytes, s);
+    if (mixed)
+      rc = mixed_cg_run(op, so, solo, tol, 1e-5, GX_F32, maxit, 100, st);
+    else
+      rc = cg_run(op, so, solo, tol, maxit, st);
+    if (rc != GX_OK && rc != GX_ENOCONV) break;
+    const int solve_rc = rc;
+    // RedBlackSolution: sol_e = MooeeInv(src_e - Meo sol_o)
+    if ((rc = apply_meooe(op, solo, t, 0))) break;        // tmp = Meooe sol_o (Even)
+    if ((rc = (hipMemcpyAsync(se->data, src->half(0), hb, hipMemcpyDeviceToDevice, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
+    if ((rc = blas_axpby(Mt, 1.0, se, -1.0, t))) break;   // src_e - tmp
+    Mt->cb = 0;
+    if ((rc = apply_mooee(op, Mt, t, 2))) break;          // sol_e
+    if ((rc = (hipMemcpyAsync(sol->half(0), t->data, hb, hipMemcpyDeviceToDevice, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
+    if ((rc = (hipMemcpyAsync(sol->half(1), solo->data, hb, hipMemcpyDeviceToDevice, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
+    if (unprec_residual) {
+      Field *res;
+      if ((rc = field_create(op->ctx, op->g, GX_F64, GX_FULL, &res))) break;
+      rc = apply_full_m(op, sol, res, 0);
+      double nr = 0, ns = 0;
+      if (!rc) rc = blas_axpby(res, 1.0, res, -1.0, src);
+      if (!rc) rc = blas_norm2(res, &nr);
+      if (!rc) rc = blas_norm2(src, &ns);
+      field_destroy(res);
+      if (rc) break;
+      *unprec_residual = std::sqrt(nr / ns);
+    }
+    rc = solve_rc;
+  } while (0);
+  hipStreamSynchronize(s);
+  field_destroy(se);
+  field_destroy(so);
+  field_destroy(t);
+  field_destroy(Mt);
+  field_destroy(solo);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+/*
+ * gx.h — C ABI of the MI355X (gfx950) Wilson / Mobius domain-wall Dirac operator and
+ * even-odd conjugate-gradient library (libgx.so).
+ *
+ * Every entry point returns an int status (GX_OK = 0); no C++ exception crosses the ABI.
+ * All work is enqueued on the context's HIP stream.  Calls that hand back host scalars
+ * (norms, inner products, solver statistics) synchronise that stream; the others do not.
+ * gx_ctx_sync() waits for everything enqueued so far.
+ *
+ * Citations name the Grid (paboyle/Grid) interface each entry point replaces; paths are
+ * relative to the Grid source tree.
+ *
+ * Host field layout (import/export): Grid's lexicographic site order, x fastest, with the
+ * fifth dimension s innermost, and complex numbers as interleaved (re, im) doubles:
+ *   full field : psi[((t*Lz + z)*Ly + y)*Lx + x][s][spin][colour]
+ *   half field : psi[((t*Lz + z)*Ly + y)*(Lx/2) + x/2][s][spin][colour]   (one checkerboard)
+ *   gauge      : U[site][mu][a][b], mu = 0..3 for x,y,z,t
+ * Checkerboard (parity) of a site = (x+y+z+t) & 1; Even = 0, Odd = 1
+ * (Grid/cartesian/Cartesian_red_black.h:39-95; 5D: Grid/qcd/utils/SpaceTimeGrid.cc:66-78).
+ */
+#ifndef GX_H
+#define GX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gx_status {
+  GX_OK = 0,
+  GX_EINVAL = 1,    /* bad argument / nonconformable fields (Grid: conformable() assert) */
+  GX_ENOMEM = 2,    /* device allocation failed, or serving admission refused */
+  GX_ENOCONV = 3,   /* solver did not converge (Grid: ErrorOnNoConverge assert) */
+  GX_EHIP = 4,      /* HIP runtime error */
+  GX_ECB = 5,       /* wrong checkerboard on input (Grid: assert(in.Checkerboard()==...)) */
+  GX_ENAN = 6       /* NaN/Inf detected in an input (Grid: assert(std::isnan(guess)==0)) */
+};
+
+enum gx_prec { GX_F64 = 0, GX_F32 = 1, GX_F16 = 2 };   /* storage precision of a field */
+enum gx_subset { GX_FULL = -1, GX_EVEN = 0, GX_ODD = 1 };
+enum gx_kind { GX_WILSON = 0, GX_MOBIUS = 1 };
+
+typedef struct gx_ctx_s *gx_ctx;
+typedef struct gx_field_s *gx_field;
+typedef struct gx_op_s *gx_op;
+
+/* Last error message of the calling thread (static storage). */
+const char *gx_last_error(void);
+/* Library version string. */
+const char *gx_version(void);
+
+/* ---- context: device + stream (Grid: acceleratorInit, Grid/threads/Accelerator.cc:120-200) ---- */
+int gx_ctx_create(int device, gx_ctx *out);
+int gx_ctx_destroy(gx_ctx ctx);
+int gx_ctx_sync(gx_ctx ctx);
+/* Timing on the context's stream: record an event pair and read the elapsed milliseconds. */
+int gx_ctx_event_record(gx_ctx ctx, int slot);               /* slot 0..15 */
+int gx_ctx_event_elapsed(gx_ctx ctx, int slot0, int slot1, double *ms);
+
+/* ---- fields (Grid: Lattice<vobj> on a GridCartesian / GridRedBlackCartesian,
+ *      Grid/lattice/Lattice_base.h:46; EO pick/set Grid/lattice/Lattice_transfer.h:50-80) ---- */
+/* subset: GX_FULL for the whole lattice, GX_EVEN / GX_ODD for a half (checkerboarded) field. */
+int gx_field_create(gx_ctx ctx, const int dims[4], int Ls, int prec, int subset, gx_field *out);
+int gx_field_destroy(gx_field f);
+int gx_field_import(gx_field f, const double *host);        /* host layout above */
+int gx_field_export(gx_field f, double *host);
+int gx_field_checkerboard(gx_field f, int *cb);              /* Grid: Lattice::Checkerboard() */
+int gx_field_set_checkerboard_tag(gx_field f, int cb);       /* Grid: Checkerboard() = cb */
+int gx_field_zero(gx_field f);                               /* Grid: field = Zero() */
+int gx_field_copy(gx_field dst, gx_field src);               /* Grid: dst = src (same precision) */
+/* Grid: pickCheckerboard(cb, half, full) / setCheckerboard(full, half), Lattice_transfer.h:50-80 */
+int gx_pick_checkerboard(int cb, gx_field half, gx_field full);
+int gx_set_checkerboard(gx_field full, gx_field half);
+/* Grid: precisionChange(out, in), Lattice_transfer.h:1312-1492 */
+int gx_precision_change(gx_field out, gx_field in);
+
+/* ---- BLAS1 / reductions (Grid/lattice/Lattice_arith.h:231-265; Lattice_reduction.h:202-410).
+ *      Per-site products in the field's precision, global sums accumulated in double with a
+ *      fixed-order two-stage reduction (bitwise reproducible run to run). ---- */
+int gx_axpy(gx_field z, double a, gx_field x, gx_field y);                   /* z = a x + y */
+int gx_axpby(gx_field z, double a, double b, gx_field x, gx_field y);        /* z = a x + b y */
+int gx_axpy_norm(gx_field z, double a, gx_field x, gx_field y, double *nrm); /* z = a x + y, |z|^2 */
+int gx_scale(gx_field z, double a, gx_field x);                              /* z = a x */
+int gx_norm2(gx_field x, double *nrm);
+int gx_inner_product(gx_field x, gx_field y, double *re, double *im);        /* <x|y> = sum conj(x) y */
+int gx_inner_product_norm(gx_field x, gx_field y, double *re, double *im, double *nrm_x);
+
+/* ---- fermion operators (Grid: WilsonFermion ctor WilsonFermion.h:139-142,
+ *      MobiusFermion ctor MobiusFermion.h:45-51, ImportGauge FermionOperator.h:144).
+ * U is the host gauge field (layout above); phases are the boundary phases as 4 (re,im)
+ * pairs (NULL = periodic; Grid WilsonImplParams::boundary_phases, ActionParams.h:49-65).
+ * For GX_WILSON, Ls > 1 gives Ls independent copies of the 4D operator sharing the gauge
+ * links: the multi-right-hand-side operator (Grid: WilsonFermion5D with Ls = Nrhs,
+ * WilsonFermion5D.h:40-65). ---- */
+int gx_wilson_create(gx_ctx ctx, const int dims[4], int Ls, double mass, const double *U,
+                     const double *phases, gx_op *out);
+int gx_mobius_create(gx_ctx ctx, const int dims[4], int Ls, double mass, double M5, double b, double c,
+                     const double *U, const double *phases, gx_op *out);
+int gx_op_destroy(gx_op op);
+int gx_op_import_gauge(gx_op op, const double *U);           /* Grid: ImportGauge */
+int gx_op_set_mass(gx_op op, double mass);                   /* Grid: CayleyFermion5D::SetMass / new op */
+
+/* Operator applications; fields may be F64 or F32, in and out of the same precision.
+ * Grid: FermionOperator.h:63-77; WilsonFermionImplementation.h:113-176,310-342;
+ * CayleyFermion5DImplementation.h:155-329. */
+int gx_M(gx_op op, gx_field in, gx_field out);
+int gx_Mdag(gx_op op, gx_field in, gx_field out);
+int gx_Meooe(gx_op op, gx_field in, gx_field out);
+int gx_MeooeDag(gx_op op, gx_field in, gx_field out);
+int gx_Mooee(gx_op op, gx_field in, gx_field out);
+int gx_MooeeDag(gx_op op, gx_field in, gx_field out);
+int gx_MooeeInv(gx_op op, gx_field in, gx_field out);
+int gx_MooeeInvDag(gx_op op, gx_field in, gx_field out);
+int gx_Dhop(gx_op op, gx_field in, gx_field out, int dag);     /* full lattice */
+int gx_DhopEO(gx_op op, gx_field in, gx_field out, int dag);   /* odd in -> even out */
+int gx_DhopOE(gx_op op, gx_field in, gx_field out, int dag);   /* even in -> odd out */
+/* Cayley 5th-dimension pieces (Grid: CayleyFermion5D.h:43-116); Mobius operators only. */
+int gx_M5D(gx_op op, gx_field psi, gx_field chi);       /* chi = M5D(psi, phi=chi) as Grid's M5D(psi,chi) */
+int gx_M5Ddag(gx_op op, gx_field psi, gx_field chi);
+int gx_Meooe5D(gx_op op, gx_field psi, gx_field chi);
+int gx_MeooeDag5D(gx_op op, gx_field psi, gx_field chi);
+
+/* ---- Schur even-odd operator (Grid: SchurDiagMooeeOperator, LinearOperator.h:325-349;
+ *      SchurOperatorBase::MpcDagMpc/HermOpAndNorm :286-320) ---- */
+int gx_Mpc(gx_op op, gx_field in, gx_field out);
+int gx_MpcDag(gx_op op, gx_field in, gx_field out);
+int gx_MpcDagMpc(gx_op op, gx_field in, gx_field out);
+int gx_HermOpAndNorm(gx_op op, gx_field in, gx_field out, double *n1, double *n2);
+
+/* ---- solvers ---- */
+typedef struct {
+  int iterations;          /* Grid: IterationsToComplete */
+  int converged;
+  double true_residual;    /* Grid: TrueResidual = |MpcDagMpc x - b| / |b| recomputed in FP64 */
+  double computed_residual;/* sqrt(cp/ssq) at exit */
+  double seconds;          /* wall time of the solve */
+  int inner_iterations;    /* mixed precision: total inner iterations */
+  int outer_iterations;    /* mixed precision: restarts; block CG: -- */
+  int final_iterations;    /* mixed precision: FP64 patch-up iterations */
+} gx_solver_stats;
+
+/* Grid: ConjugateGradient<Field>(tol, maxit)(SchurDiagMooeeOperator(Dw), src, psi),
+ * ConjugateGradient.h:43-257.  src, psi are half fields on the same checkerboard; psi holds
+ * the initial guess. Returns GX_ENOCONV if not converged in maxit. */
+int gx_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver_stats *st);
+
+/* Grid: MixedPrecisionConjugateGradient<FieldD,FieldF>(tol, maxinner, maxouter, grid_f, Linop_f,
+ * Linop_d)(src_d, sol_d), ConjugateGradientMixedPrec.h:57-165.  Inner solves run on fields of
+ * precision inner_prec (GX_F32, or GX_F16 for FP16 storage with FP32 arithmetic) with
+ * relative tolerance inner_tol; the outer residual, the solution accumulation and the final
+ * patch-up CG are FP64.  src, psi: F64 half fields. */
+int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_tol, int inner_prec,
+                int maxinner, int maxouter, gx_solver_stats *st);
+
+/* Grid: SchurRedBlackDiagMooeeSolve<Field>(CG)(Dw, src, sol), SchurRedBlack.h:236-291,373-436.
+ * src, sol: full F64 fields.  mixed != 0 uses gx_mixed_cg (inner FP32) for the red-black solve.
+ * unprec_residual = |M sol - src| / |src|. */
+int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, int mixed,
+                   gx_solver_stats *st, double *unprec_residual);
+
+/* ---- synthetic inputs (host utilities, no device work) ----
+ * Gauge: per link G with Re,Im ~ N(0,1); H = (G+G^dag)/2 - tr/3; U = exp(i eps H) by
+ * scaling and squaring of the Taylor series, projected back onto SU(3).  Gaussian numbers
+ * from SplitMix64 counters keyed by (seed, global site, mu/s, element) and Box-Muller. */
+int gx_random_gauge(const int dims[4], uint64_t seed, double eps, double *U);
+int gx_gaussian_fermion(const int dims[4], int Ls, uint64_t seed, double *psi);   /* full field */
+int gx_point_source(const int dims[4], int Ls, const int x[4], int spin, int colour, double *psi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GX_H */

@@ -1,0 +1,221 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle and the golden fixtures.
+
+Tolerances (per site, max over sites of |gpu - ref| / |ref|): 1e-13 for FP64 fields,
+1e-6 for FP32 fields (compared with the FP64 oracle).  Solvers: true FP64 residual <= tol and
+agreement with the oracle's FP64 CG solution.
+"""
+import numpy as np
+import pytest
+
+import grid_amd as gx
+import npref as R
+from conftest import max_site_rel_err, rel_err
+from oracle.pyoracle import Oracle, EVEN, ODD
+
+pytestmark = pytest.mark.gpu
+
+TOL = {gx.F64: 1e-13, gx.F32: 1e-6}
+
+
+@pytest.fixture(scope='module')
+def ctx():
+    c = gx.Context(0)
+    yield c
+    c.close()
+
+
+def inputs(dims, Ls, seed, eps=0.5):
+    return gx.random_gauge(dims, seed, eps), gx.gaussian_fermion(dims, Ls, seed + 17)
+
+
+def half(a, dims, cb):
+    return R.pick(a, dims, cb)
+
+
+def run_half(D, prec, fn, psi_h, cb_in):
+    i = D.field(prec, cb_in).upload(psi_h)
+    o = D.field(prec, 1 - cb_in)
+    fn(i, o)
+    return o.download()
+
+
+@pytest.mark.parametrize('dims', [(8, 8, 8, 8), (4, 6, 8, 6), (16, 8, 4, 12)])
+@pytest.mark.parametrize('prec', [gx.F64, gx.F32])
+def test_wilson_dhop(ctx, dims, prec):
+    U, psi = inputs(dims, 1, 11)
+    O = Oracle(U, dims, 1, 'wilson', mass=0.1)
+    D = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=0.1)
+    for dag in (0, 1):
+        fi = D.field(prec).upload(psi)
+        fo = D.field(prec)
+        D.Dhop(fi, fo, dag)
+        assert max_site_rel_err(fo.download(), O.dhop(psi, -1, dag)) < TOL[prec]
+        po, pe = half(psi, dims, ODD), half(psi, dims, EVEN)
+        out = run_half(D, prec, lambda i, o: D.DhopEO(i, o, dag), po, ODD)
+        assert max_site_rel_err(out, O.dhop(po, EVEN, dag)) < TOL[prec]
+        out = run_half(D, prec, lambda i, o: D.DhopOE(i, o, dag), pe, EVEN)
+        assert max_site_rel_err(out, O.dhop(pe, ODD, dag)) < TOL[prec]
+    D.close()
+
+
+def test_wilson_golden(ctx, wilson_golden):
+    G = wilson_golden
+    dims = tuple(int(x) for x in G['dims'])
+    D = gx.FermionOp(ctx, G['U'], dims, 1, 'wilson', mass=float(G['mass']))
+    fi, fo = D.field().upload(G['psi']), D.field()
+    D.Dhop(fi, fo, 0)
+    assert max_site_rel_err(fo.download(), G['dhop']) < 1e-13
+    D.Dhop(fi, fo, 1)
+    assert max_site_rel_err(fo.download(), G['dhop_dag']) < 1e-13
+    D.M(fi, fo)
+    assert max_site_rel_err(fo.download(), G['M']) < 1e-13
+    D.Mdag(fi, fo)
+    assert max_site_rel_err(fo.download(), G['Mdag']) < 1e-13
+    out = run_half(D, gx.F64, lambda i, o: D.DhopEO(i, o, 0), half(G['psi'], dims, ODD), ODD)
+    assert max_site_rel_err(out, G['dhop_eo']) < 1e-13
+    Dap = gx.FermionOp(ctx, G['U'], dims, 1, 'wilson', mass=0.1, phases=(1, 1, 1, -1))
+    Dap.Dhop(fi, fo, 0)
+    assert max_site_rel_err(fo.download(), G['dhop_antiperiodic_t']) < 1e-13
+    sol = D.field()
+    st = D.schur_solve(fi, sol, tol=1e-12)
+    assert st['converged'] and st['unprec_residual'] < 1e-10
+    assert rel_err(sol.download(), G['x']) < 1e-9
+
+
+def test_mobius_golden(ctx, mobius_golden):
+    G = mobius_golden
+    dims = tuple(int(x) for x in G['dims'])
+    Ls = int(G['Ls'])
+    D = gx.FermionOp(ctx, G['U'], dims, Ls, 'mobius', mass=float(G['mass']), M5=float(G['M5']), b=float(G['b']),
+                     c=float(G['c']))
+    fi, fo = D.field().upload(G['psi']), D.field()
+    D.M(fi, fo)
+    assert max_site_rel_err(fo.download(), G['M']) < 1e-13
+    D.Mdag(fi, fo)
+    assert max_site_rel_err(fo.download(), G['Mdag']) < 1e-13
+    po, pe = half(G['psi'], dims, ODD), half(G['psi'], dims, EVEN)
+    assert max_site_rel_err(run_half(D, gx.F64, D.Meooe, po, ODD), G['meooe_o']) < 1e-13
+    assert max_site_rel_err(run_half(D, gx.F64, D.MeooeDag, po, ODD), G['meooedag_o']) < 1e-13
+    for name, key in [('Mooee', 'mooee_e'), ('MooeeDag', 'mooeedag_e'), ('MooeeInv', 'mooeeinv_e'),
+                      ('MooeeInvDag', 'mooeeinvdag_e')]:
+        i = D.field(gx.F64, EVEN).upload(pe)
+        o = D.field(gx.F64, EVEN)
+        getattr(D, name)(i, o)
+        assert max_site_rel_err(o.download(), G[key]) < 1e-13, name
+    for name, key in [('Mpc', 'mpc_o'), ('MpcDag', 'mpcdag_o')]:
+        i = D.field(gx.F64, ODD).upload(po)
+        o = D.field(gx.F64, ODD)
+        getattr(D, name)(i, o)
+        assert max_site_rel_err(o.download(), G[key]) < 1e-13, name
+
+
+@pytest.mark.parametrize('Ls,dims', [(8, (8, 8, 8, 8)), (16, (8, 4, 4, 8)), (12, (4, 6, 4, 8))])
+@pytest.mark.parametrize('prec', [gx.F64, gx.F32])
+def test_mobius_ops(ctx, Ls, dims, prec):
+    U, psi = inputs(dims, Ls, 23, eps=0.3)
+    kw = dict(mass=0.01, M5=1.8, b=1.5, c=0.5)
+    O = Oracle(U, dims, Ls, 'mobius', **kw)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    tol = TOL[prec]
+    fi, fo = D.field(prec).upload(psi), D.field(prec)
+    D.M(fi, fo)
+    assert max_site_rel_err(fo.download(), O.apply('M', psi)) < tol
+    D.Mdag(fi, fo)
+    assert max_site_rel_err(fo.download(), O.apply('Mdag', psi)) < tol
+    po, pe = half(psi, dims, ODD), half(psi, dims, EVEN)
+    for dag in (0, 1):
+        out = run_half(D, prec, lambda i, o: D.DhopEO(i, o, dag), po, ODD)
+        assert max_site_rel_err(out, O.dhop(po, EVEN, dag)) < tol
+    assert max_site_rel_err(run_half(D, prec, D.Meooe, po, ODD), O.apply('Meooe', po, ODD)) < tol
+    assert max_site_rel_err(run_half(D, prec, D.MeooeDag, po, ODD), O.apply('MeooeDag', po, ODD)) < tol
+    for name in ['Mooee', 'MooeeDag', 'MooeeInv', 'MooeeInvDag', 'Meooe5D', 'MeooeDag5D']:
+        i = D.field(prec, EVEN).upload(pe)
+        o = D.field(prec, EVEN)
+        getattr(D, name)(i, o)
+        assert max_site_rel_err(o.download(), O.apply(name, pe, EVEN)) < tol, name
+    for name in ['M5D', 'M5Ddag']:   # Grid's M5D(psi, chi) with chi = psi on entry
+        i = D.field(prec, EVEN).upload(pe)
+        o = D.field(prec, EVEN).upload(pe)
+        getattr(D, name)(i, o)
+        assert max_site_rel_err(o.download(), O.apply(name, pe, EVEN)) < tol, name
+    for name in ['Mpc', 'MpcDag', 'MpcDagMpc']:
+        i = D.field(prec, ODD).upload(po)
+        o = D.field(prec, ODD)
+        getattr(D, name)(i, o)
+        assert max_site_rel_err(o.download(), O.schur(name, po)) < tol * 10, name
+    D.close()
+
+
+@pytest.mark.parametrize('nrhs', [4, 12])
+def test_multi_rhs_wilson_matches_single(ctx, nrhs):
+    dims = (8, 4, 4, 8)
+    U, psi = inputs(dims, nrhs, 31)
+    O1 = Oracle(U, dims, 1, 'wilson', mass=0.05)
+    D = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=0.05)
+    po = half(psi, dims, ODD)
+    out = run_half(D, gx.F64, lambda i, o: D.DhopEO(i, o, 0), po, ODD)
+    for k in range(nrhs):
+        ref = O1.dhop(np.ascontiguousarray(po[..., k:k + 1, :, :]), EVEN, 0)
+        assert max_site_rel_err(out[..., k:k + 1, :, :], ref) < 1e-13
+
+
+def test_reductions(ctx):
+    dims = (8, 8, 4, 4)
+    x = gx.gaussian_fermion(dims, 4, 1)
+    y = gx.gaussian_fermion(dims, 4, 2)
+    D = gx.FermionOp(ctx, gx.random_gauge(dims, 5, 0.5), dims, 4, 'wilson')
+    fx, fy, fz = D.field().upload(x), D.field().upload(y), D.field()
+    assert abs(fx.norm2() - np.vdot(x, x).real) / np.vdot(x, x).real < 1e-14
+    ip = gx.inner_product(fx, fy)
+    assert abs(ip - np.vdot(x, y)) / abs(np.vdot(x, y)) < 1e-12
+    n = gx.axpy_norm(fz, -0.7, fx, fy)
+    z = -0.7 * x + y
+    assert abs(n - np.vdot(z, z).real) / np.vdot(z, z).real < 1e-14
+    assert rel_err(fz.download(), z) < 1e-15
+    # bitwise reproducible
+    assert all(fx.norm2() == fx.norm2() for _ in range(3))
+    assert gx.inner_product(fx, fy) == ip
+
+
+def test_checkerboard_errors(ctx):
+    dims = (4, 4, 4, 4)
+    D = gx.FermionOp(ctx, gx.random_gauge(dims, 5, 0.5), dims, 1, 'wilson')
+    i, o = D.field(gx.F64, EVEN), D.field(gx.F64, ODD)
+    with pytest.raises(gx.GxError) as e:
+        D.DhopEO(i, o)
+    assert e.value.code == 5
+    with pytest.raises(gx.GxError) as e:
+        D.Dhop(i, o)
+    assert e.value.code == 1
+
+
+@pytest.mark.parametrize('kind,dims,Ls,mass', [('wilson', (8, 8, 8, 8), 1, 0.1), ('mobius', (8, 8, 4, 4), 8, 0.05)])
+def test_cg_matches_oracle(ctx, kind, dims, Ls, mass):
+    U = gx.random_gauge(dims, 41, 0.5 if kind == 'wilson' else 0.3)
+    src = half(gx.gaussian_fermion(dims, Ls, 42), dims, ODD)
+    kw = dict(mass=mass, M5=1.8, b=1.5, c=0.5)
+    O = Oracle(U, dims, Ls, kind, **kw)
+    ref, rst = O.cg(src, tol=1e-10)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, **kw)
+    fs, fp = D.field(gx.F64, ODD).upload(src), D.field(gx.F64, ODD)
+    st = D.cg(fs, fp, tol=1e-10)
+    assert st['converged'] and st['true_residual'] < 1e-9
+    assert abs(st['iterations'] - rst['iterations']) <= 2
+    assert rel_err(fp.download(), ref) < 1e-8
+    # mixed precision: FP32 inner, FP64 outer; reaches the same FP64 tolerance
+    fm = D.field(gx.F64, ODD)
+    mst = D.mixed_cg(fs, fm, tol=1e-10, inner_tol=1e-5)
+    assert mst['converged'] and mst['true_residual'] < 1e-9
+    assert rel_err(fm.download(), ref) < 1e-7
+
+
+def test_schur_solve_matches_oracle(ctx):
+    dims = (8, 8, 8, 8)
+    U, src = inputs(dims, 1, 51)
+    O = Oracle(U, dims, 1, 'wilson', mass=0.1)
+    ref, rst = O.schur_solve(src, tol=1e-10)
+    D = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=0.1)
+    fs, fsol = D.field().upload(src), D.field()
+    st = D.schur_solve(fs, fsol, tol=1e-10)
+    assert st['converged'] and st['unprec_residual'] < 1e-8
+    assert rel_err(fsol.download(), ref) < 1e-8
