@@ -99,6 +99,8 @@ _SIG = {
     'gx_cg': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_mixed_cg': ([_P, _P, _P, _D, _D, _I, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
+    'gx_tune_set': ([_I, _I], _I),
+    'gx_tune_get': ([_I, _IP], _I),
     'gx_random_gauge': ([_IP, ctypes.c_uint64, _D, _DP], _I),
     'gx_gaussian_fermion': ([_IP, _I, ctypes.c_uint64, _DP], _I),
     'gx_point_source': ([_IP, _I, _IP, _I, _I, _DP], _I),
@@ -228,6 +230,23 @@ def axpy_norm(z, a, x, y):
 
 def axpby(z, a, b, x, y):
     _check(lib().gx_axpby(z.h, a, b, x.h, y.h))
+
+
+def field_copy(dst, src):
+    _check(lib().gx_field_copy(dst.h, src.h))
+
+
+TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER = 0, 1, 2, 3
+
+
+def tune_set(knob, value):
+    _check(lib().gx_tune_set(int(knob), int(value)))
+
+
+def tune_get(knob):
+    v = _I()
+    _check(lib().gx_tune_get(int(knob), ctypes.byref(v)))
+    return v.value
 
 
 def precision_change(out, inp):

@@ -25,6 +25,9 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
+static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 0, 0, 0, 0};
+int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
+
 void set_error(const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -651,6 +654,19 @@ extern "C" {
 
 const char *gx_last_error(void) { return g_err; }
 const char *gx_version(void) { return "gx 0.1 (gfx950)"; }
+
+int gx_tune_set(int knob, int value) {
+  GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT, GX_EINVAL, "unknown tuning knob %d", knob);
+  if (knob <= GX_TUNE_DHOP_ORDER) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
+  g_tune[knob] = value;
+  return GX_OK;
+}
+
+int gx_tune_get(int knob, int *value) {
+  GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT && value, GX_EINVAL, "unknown tuning knob %d", knob);
+  *value = g_tune[knob];
+  return GX_OK;
+}
 
 int gx_ctx_create(int device, gx_ctx *out) {
   GX_REQUIRE(out, GX_EINVAL, "null output");

@@ -23,6 +23,9 @@ namespace gx {
 
 void set_error(const char *fmt, ...);
 
+// process-wide launch-shape knobs (gx_tune_set / gx_tune_get)
+int tune_get(int knob);
+
 #define GX_CHECK_HIP(call)                                                                             \
   do {                                                                                                 \
     hipError_t e_ = (call);                                                                            \

@@ -124,85 +124,197 @@ __device__ __forceinline__ void su3_mul(const cx<T> *u, const cx<T> *h, cx<T> *g
     }
 }
 
-template <typename T, int LS, int P, bool PLUS>
-__device__ __forceinline__ void hop_leg(cx<T> *acc, const cx<T> *__restrict__ in, const cx<T> *__restrict__ U,
-                                        int64_t nb, int s, int64_t h, const KGeom &g) {
-  constexpr int MU = P & 3;
-  cx<T> f[12];
-  const int64_t idx = nb * LS + s;
-#pragma unroll
-  for (int k = 0; k < 12; k++) f[k] = in[k * g.stride + idx];
+// Links are staged per workgroup in LDS (one coalesced load, row stride kLinkPad complex per
+// site to spread LDS banks) when the tile fits; otherwise each lane reads them from global
+// memory (SoA over sites, coalesced across lanes when one lane = one site).
+constexpr int kLinkPad = 73;
+
+// Workgroup tile -> lattice mapping.  map = 0: block b covers tile b.  map = 1 (XCD-aware): the
+// hardware deals consecutive workgroups round-robin to the 8 XCDs, so tile (b % 8) * per + b / 8
+// gives each XCD (and its L2) one contiguous slab of the lattice.
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nblocks, int map) {
+  if (map == 1) {
+    const uint32_t per = (nblocks + 7) >> 3;
+    return (b & 7) * per + (b >> 3);
+  }
+  return b;
+}
+
+template <typename T, int MU, bool PLUS, bool LDSU>
+__device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<T> *__restrict__ U,
+                                         int P, uint32_t h, uint32_t H4) {
   cx<T> hs[6];
   project<T, MU, PLUS>(f, hs);
   cx<T> u[9];
+  if (LDSU) {
 #pragma unroll
-  for (int k = 0; k < 9; k++) u[k] = U[(P * 9 + k) * g.H4 + h];
+    for (int k = 0; k < 9; k++) u[k] = ul[P * 9 + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[k] = U[(P * 9 + k) * H4 + h];
+  }
   cx<T> gs[6];
   su3_mul<T>(u, hs, gs);
   accum_recon<T, MU, PLUS>(acc, gs);
 }
 
-// One lane per (4D half-site h, s); s fastest, 64/LS sites per wave; lanes of one site share
-// the link loads (same address within the wave).
-template <typename T, int LS, bool DAG>
+template <typename T>
+__device__ __forceinline__ void load_spinor(cx<T> *f, const cx<T> *__restrict__ in, uint32_t idx, uint32_t stride) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) f[k] = in[k * stride + idx];
+}
+
+template <typename T>
+__device__ __forceinline__ void store_spinor(cx<T> *__restrict__ out, const cx<T> *acc, uint32_t o, uint32_t stride,
+                                             bool nt) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  if (nt) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      v2 v = {acc[k].r, acc[k].i};
+      __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(out + k * stride + o));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) out[k * stride + o] = acc[k];
+  }
+}
+
+// One leg: LEG-th entry of the leg sequence.  ORD = 0 is Grid's order (DaggerNo: Xm,Ym,Zm,Tm with
+// (1+gamma) then Xp..Tp with (1-gamma); DaggerYes: Xp..Tp with (1+gamma) then Xm..Tm with
+// (1-gamma)); ORD = 1 pairs the two legs of each direction so x-neighbour loads reuse L1.
+template <bool DAG, int ORD, int LEG>
+struct LegSel {
+  static constexpr int MU = ORD == 0 ? (LEG & 3) : (LEG >> 1);
+  static constexpr bool PLUS = ORD == 0 ? (LEG < 4) : ((LEG & 1) == 0);
+  static constexpr int P = (PLUS != DAG) ? MU + 4 : MU;   // (1+gamma) legs read x-mu when !DAG
+};
+
+template <typename T, bool DAG, int ORD, int LEG, bool LDSU>
+__device__ __forceinline__ void do_leg(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<T> *__restrict__ U,
+                                       uint32_t h, uint32_t H4) {
+  typedef LegSel<DAG, ORD, LEG> L;
+  leg_math<T, L::MU, L::PLUS, LDSU>(acc, f, ul, U, L::P, h, H4);
+}
+
+template <typename T, bool DAG, int ORD, int LEG>
+__device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, int s) {
+  return nb[LegSel<DAG, ORD, LEG>::P] * Ls + s;
+}
+
+template <typename T, int LS, bool DAG, bool PF, int ORD, bool LDSU>
+__device__ __forceinline__ void hop_site(cx<T> *acc, const cx<T> *__restrict__ in, const cx<T> *__restrict__ U,
+                                         const cx<T> *ul, const uint32_t *nb, int s, uint32_t h, uint32_t H4,
+                                         uint32_t stride) {
+#define IDX(L) leg_idx<T, DAG, ORD, L>(nb, LS, s)
+#define LEG(L, F) do_leg<T, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4)
+  if (PF) {
+    // two register sets: the loads of leg L+1 are in flight while leg L is computed
+    cx<T> fa[12], fb[12];
+    load_spinor<T>(fa, in, IDX(0), stride);
+    load_spinor<T>(fb, in, IDX(1), stride);
+    __builtin_amdgcn_sched_barrier(0);
+    LEG(0, fa);
+    load_spinor<T>(fa, in, IDX(2), stride);
+    __builtin_amdgcn_sched_barrier(0);
+    LEG(1, fb);
+    load_spinor<T>(fb, in, IDX(3), stride);
+    __builtin_amdgcn_sched_barrier(0);
+    LEG(2, fa);
+    load_spinor<T>(fa, in, IDX(4), stride);
+    __builtin_amdgcn_sched_barrier(0);
+    LEG(3, fb);
+    load_spinor<T>(fb, in, IDX(5), stride);
+    __builtin_amdgcn_sched_barrier(0);
+    LEG(4, fa);
+    load_spinor<T>(fa, in, IDX(6), stride);
+    __builtin_amdgcn_sched_barrier(0);
+    LEG(5, fb);
+    load_spinor<T>(fb, in, IDX(7), stride);
+    __builtin_amdgcn_sched_barrier(0);
+    LEG(6, fa);
+    LEG(7, fb);
+  } else {
+    cx<T> f[12];
+    load_spinor<T>(f, in, IDX(0), stride);
+    LEG(0, f);
+    load_spinor<T>(f, in, IDX(1), stride);
+    LEG(1, f);
+    load_spinor<T>(f, in, IDX(2), stride);
+    LEG(2, f);
+    load_spinor<T>(f, in, IDX(3), stride);
+    LEG(3, f);
+    load_spinor<T>(f, in, IDX(4), stride);
+    LEG(4, f);
+    load_spinor<T>(f, in, IDX(5), stride);
+    LEG(5, f);
+    load_spinor<T>(f, in, IDX(6), stride);
+    LEG(6, f);
+    load_spinor<T>(f, in, IDX(7), stride);
+    LEG(7, f);
+  }
+#undef IDX
+#undef LEG
+}
+
+// One lane per (4D half-site h, s): s fastest, 64/LS sites per wave, 4 waves per workgroup, so a
+// wave reads each neighbour's s-row contiguously and the lanes of one site share its links.
+// flags: bit 0 = non-temporal output stores.  ORD: leg order (0 Grid's, 1 paired directions).
+template <typename T, int LS, bool DAG, bool PF, int ORD>
 __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                               const cx<T> *__restrict__ U, KGeom g, int cb_out,
-                                              const CGState *__restrict__ gate) {
+                                              const CGState *__restrict__ gate, uint32_t ntiles, int map, int flags) {
   if (gated(gate)) return;
   constexpr int SPW = 64 / LS;
+  constexpr int SPB = 4 * SPW;
+  constexpr bool LDSU = (LS > 1) && (SPB * kLinkPad * sizeof(cx<T>) <= 40960);
+  __shared__ cx<T> ulds[LDSU ? SPB * kLinkPad : 1];
+  const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, map);
+  if (tile >= ntiles) return;  // uniform across the workgroup
+  const uint32_t H4 = (uint32_t)g.H4, stride = (uint32_t)g.stride;
+  const uint32_t h0 = tile * SPB;
+  if (LDSU) {
+    for (uint32_t e = threadIdx.x; e < SPB * 72; e += 256) {
+      const uint32_t pk = e / SPB, site = e - pk * SPB;
+      if (h0 + site < H4) ulds[site * kLinkPad + pk] = U[pk * H4 + h0 + site];
+    }
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int w = threadIdx.x >> 6;
   const int ls = lane / LS;
   const int s = lane - ls * LS;
-  if (ls >= SPW) return;
-  const int64_t h = wave * SPW + ls;
-  if (h >= g.H4) return;
-  int64_t r = h;
-  const int xh = (int)(r % g.Lxh);
-  r /= g.Lxh;
-  const int y = (int)(r % g.L[1]);
-  r /= g.L[1];
-  const int z = (int)(r % g.L[2]);
-  const int t = (int)(r / g.L[2]);
+  const int lsb = w * SPW + ls;
+  const uint32_t h = h0 + lsb;
+  if (ls >= SPW || h >= H4) return;
+  uint32_t r = h;
+  const uint32_t Lxh = (uint32_t)g.Lxh;
+  const int xh = (int)(r % Lxh);
+  r /= Lxh;
+  const int y = (int)(r % (uint32_t)g.L[1]);
+  r /= (uint32_t)g.L[1];
+  const int z = (int)(r % (uint32_t)g.L[2]);
+  const int t = (int)(r / (uint32_t)g.L[2]);
   const int x = 2 * xh + ((cb_out + y + z + t) & 1);
-  const int64_t base = h - xh;
+  const uint32_t base = h - xh;
   const int xp = (x + 1 == g.L[0]) ? 0 : x + 1;
   const int xm = (x == 0) ? g.L[0] - 1 : x - 1;
-  int64_t nb[8];
+  const uint32_t sY = (uint32_t)g.sY, sZ = (uint32_t)g.sZ, sT = (uint32_t)g.sT;
+  uint32_t nb[8];
   nb[0] = base + (xp >> 1);
   nb[4] = base + (xm >> 1);
-  nb[1] = (y + 1 == g.L[1]) ? h - (int64_t)(g.L[1] - 1) * g.sY : h + g.sY;
-  nb[5] = (y == 0) ? h + (int64_t)(g.L[1] - 1) * g.sY : h - g.sY;
-  nb[2] = (z + 1 == g.L[2]) ? h - (int64_t)(g.L[2] - 1) * g.sZ : h + g.sZ;
-  nb[6] = (z == 0) ? h + (int64_t)(g.L[2] - 1) * g.sZ : h - g.sZ;
-  nb[3] = (t + 1 == g.L[3]) ? h - (int64_t)(g.L[3] - 1) * g.sT : h + g.sT;
-  nb[7] = (t == 0) ? h + (int64_t)(g.L[3] - 1) * g.sT : h - g.sT;
-
+  nb[1] = (y + 1 == g.L[1]) ? h - (uint32_t)(g.L[1] - 1) * sY : h + sY;
+  nb[5] = (y == 0) ? h + (uint32_t)(g.L[1] - 1) * sY : h - sY;
+  nb[2] = (z + 1 == g.L[2]) ? h - (uint32_t)(g.L[2] - 1) * sZ : h + sZ;
+  nb[6] = (z == 0) ? h + (uint32_t)(g.L[2] - 1) * sZ : h - sZ;
+  nb[3] = (t + 1 == g.L[3]) ? h - (uint32_t)(g.L[3] - 1) * sT : h + sT;
+  nb[7] = (t == 0) ? h + (uint32_t)(g.L[3] - 1) * sT : h - sT;
+  const cx<T> *ul = ulds + (LDSU ? lsb * kLinkPad : 0);
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  if (!DAG) {
-    hop_leg<T, LS, 4, true>(acc, in, U, nb[4], s, h, g);
-    hop_leg<T, LS, 5, true>(acc, in, U, nb[5], s, h, g);
-    hop_leg<T, LS, 6, true>(acc, in, U, nb[6], s, h, g);
-    hop_leg<T, LS, 7, true>(acc, in, U, nb[7], s, h, g);
-    hop_leg<T, LS, 0, false>(acc, in, U, nb[0], s, h, g);
-    hop_leg<T, LS, 1, false>(acc, in, U, nb[1], s, h, g);
-    hop_leg<T, LS, 2, false>(acc, in, U, nb[2], s, h, g);
-    hop_leg<T, LS, 3, false>(acc, in, U, nb[3], s, h, g);
-  } else {
-    hop_leg<T, LS, 0, true>(acc, in, U, nb[0], s, h, g);
-    hop_leg<T, LS, 1, true>(acc, in, U, nb[1], s, h, g);
-    hop_leg<T, LS, 2, true>(acc, in, U, nb[2], s, h, g);
-    hop_leg<T, LS, 3, true>(acc, in, U, nb[3], s, h, g);
-    hop_leg<T, LS, 4, false>(acc, in, U, nb[4], s, h, g);
-    hop_leg<T, LS, 5, false>(acc, in, U, nb[5], s, h, g);
-    hop_leg<T, LS, 6, false>(acc, in, U, nb[6], s, h, g);
-    hop_leg<T, LS, 7, false>(acc, in, U, nb[7], s, h, g);
-  }
-  const int64_t o = h * LS + s;
-#pragma unroll
-  for (int k = 0; k < 12; k++) out[k * g.stride + o] = acc[k];
+  hop_site<T, LS, DAG, PF, ORD, LDSU>(acc, in, U, ul, nb, s, h, H4, stride);
+  store_spinor<T>(out, acc, h * LS + s, stride, (flags & 1) != 0);
 }
 
 #define GX_LS_CASES(M) \
@@ -212,18 +324,40 @@ template <typename T>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const CGState *gate,
                 hipStream_t st) {
   const KGeom k = kgeom(g);
-  const int spw = 64 / g.Ls;
-  const int64_t waves = (g.H4 + spw - 1) / spw;
-  const int64_t blocks = (waves * 64 + 255) / 256;
+  if (12 * 2 * k.stride >= (int64_t)1 << 31) {
+    set_error("dhop: lattice too large for 32-bit site indexing");
+    return GX_EINVAL;
+  }
+  const int map = tune_get(GX_TUNE_DHOP_MAP);
+  const bool pf = tune_get(GX_TUNE_DHOP_PF) != 0;
+  const int flags = tune_get(GX_TUNE_DHOP_NT) ? 1 : 0;
+  const bool ord = tune_get(GX_TUNE_DHOP_ORDER) != 0 && (g.Ls == 1 || g.Ls == 12 || g.Ls == 16);
   const cx<T> *pin = (const cx<T> *)in;
   cx<T> *pout = (cx<T> *)out;
   const cx<T> *pU = (const cx<T> *)U;
-#define GX_DHOP_CASE(LS)                                                                          \
-  case LS:                                                                                        \
-    if (dag)                                                                                      \
-      hipLaunchKernelGGL((k_dhop<T, LS, true>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, gate); \
-    else                                                                                          \
-      hipLaunchKernelGGL((k_dhop<T, LS, false>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, gate); \
+  const int spb = 4 * (64 / g.Ls);
+  const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
+  const uint32_t blocks = map == 1 ? ((ntiles + 7) / 8) * 8 : ntiles;
+#define GX_DHOP_GO(LS, DAG, PF, ORD)                                                                       \
+  hipLaunchKernelGGL((k_dhop<T, LS, DAG, PF, ORD>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
+                     gate, ntiles, map, flags)
+#define GX_DHOP_CASE_O(LS, ORD)                   \
+  if (dag) {                                      \
+    if (pf) GX_DHOP_GO(LS, true, true, ORD);      \
+    else GX_DHOP_GO(LS, true, false, ORD);        \
+  } else {                                        \
+    if (pf) GX_DHOP_GO(LS, false, true, ORD);     \
+    else GX_DHOP_GO(LS, false, false, ORD);       \
+  }
+#define GX_DHOP_CASE(LS)                                         \
+  case LS:                                                       \
+    if (ord && (LS == 1 || LS == 12 || LS == 16)) {              \
+      if constexpr (LS == 1 || LS == 12 || LS == 16) {           \
+        GX_DHOP_CASE_O(LS, 1)                                    \
+      }                                                          \
+    } else {                                                     \
+      GX_DHOP_CASE_O(LS, 0)                                      \
+    }                                                            \
     break;
   switch (g.Ls) {
     GX_LS_CASES(GX_DHOP_CASE)
@@ -232,6 +366,8 @@ int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_
       return GX_EINVAL;
   }
 #undef GX_DHOP_CASE
+#undef GX_DHOP_CASE_O
+#undef GX_DHOP_GO
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -353,9 +489,11 @@ int launch_convert(int64_t n, const void *in, void *out, hipStream_t st) {
 }
 
 // ---------------- deterministic reductions ----------------
-// Block tree (wave shuffles, then LDS across the 4 waves) followed by a last-block-done
-// ticket: the last block sums the per-block partials in block order.  Grid size depends only
-// on the problem size, so the result is bitwise reproducible.
+// Two kernels: every block of the streaming kernel writes its partial sum (wave shuffles, then LDS
+// across the 4 waves), and a one-block kernel adds the partials in block order.  The kernel
+// boundary makes the partials visible to the second kernel, so no device-scope fence or atomic
+// is needed (on a multi-XCD part such a fence writes back the XCD's L2).  The grid size depends
+// only on the problem size, so results are bitwise reproducible.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -376,32 +514,22 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[4]) {
   __syncthreads();
 }
 
-// Returns true in every thread of the last block to finish; partial[blockIdx*NV + j] = v[j].
 template <int NV>
-__device__ __forceinline__ bool publish_partial(const double (&v)[NV], double *partial, unsigned int *ticket) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
+__device__ __forceinline__ void write_partial(const double (&v)[NV], double *partial) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int j = 0; j < NV; j++) partial[blockIdx.x * NV + j] = v[j];
-    __threadfence();
-    const unsigned int t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  return last;
 }
 
+// sum of nparts partials (NV values each) in block order, by one block of 256 threads
 template <int NV>
-__device__ __forceinline__ void final_sum(double (&v)[NV], const double *partial, unsigned int *ticket,
-                                          double (*sh)[4]) {
-  __threadfence();
+__device__ __forceinline__ void sum_partials(double (&v)[NV], const double *partial, int nparts, double (*sh)[4]) {
 #pragma unroll
   for (int j = 0; j < NV; j++) v[j] = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x)
 #pragma unroll
-    for (int j = 0; j < NV; j++) v[j] += ((volatile const double *)partial)[b * NV + j];
+    for (int j = 0; j < NV; j++) v[j] += partial[b * NV + j];
   block_sum<NV>(v, sh);
-  if (threadIdx.x == 0) *ticket = 0;
 }
 
 // site j of a (possibly full) field: half = j / stride, i = j % stride;
@@ -409,7 +537,7 @@ __device__ __forceinline__ void final_sum(double (&v)[NV], const double *partial
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, const cx<T> *__restrict__ x,
                                                 const cx<T> *__restrict__ y, cx<T> *__restrict__ z, T a,
-                                                double *partial, unsigned int *ticket, double *result) {
+                                                double *partial) {
   __shared__ double sh[2][4];
   double v[2] = {0, 0};
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
@@ -440,12 +568,16 @@ __global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, 
     v[1] += (double)si;
   }
   block_sum<2>(v, sh);
-  if (publish_partial<2>(v, partial, ticket)) {
-    final_sum<2>(v, partial, ticket, sh);
-    if (threadIdx.x == 0) {
-      result[0] = v[0];
-      if (MODE == 0) result[1] = v[1];
-    }
+  write_partial<2>(v, partial);
+}
+
+__global__ void __launch_bounds__(256) k_reduce_fin(const double *partial, int nparts, double *result, int nout) {
+  __shared__ double sh[2][4];
+  double v[2];
+  sum_partials<2>(v, partial, nparts, sh);
+  if (threadIdx.x == 0) {
+    result[0] = v[0];
+    if (nout > 1) result[1] = v[1];
   }
 }
 
@@ -462,26 +594,30 @@ int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void
   const cx<T> *px = (const cx<T> *)x;
   const cx<T> *py = (const cx<T> *)y;
   cx<T> *pz = (cx<T> *)z;
-  double *res = ctx->red_result + slot;
   if (mode == 0)
     hipLaunchKernelGGL((k_reduce<T, 0>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
-                       ctx->red_partial, ctx->red_ticket, res);
+                       ctx->red_partial);
   else if (mode == 1)
     hipLaunchKernelGGL((k_reduce<T, 1>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
-                       ctx->red_partial, ctx->red_ticket, res);
+                       ctx->red_partial);
   else
     hipLaunchKernelGGL((k_reduce<T, 2>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
-                       ctx->red_partial, ctx->red_ticket, res);
+                       ctx->red_partial);
+  hipLaunchKernelGGL(k_reduce_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb,
+                     ctx->red_result + slot, mode == 0 ? 2 : 1);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
 
 // ---------------- CG iteration on device scalars (ConjugateGradient.h:151-198) ----------------
-// d = Re<p|mmp>; a = c / d
+// Kernels of iteration k run while k <= min(k_conv, kmax).  The norm finaliser advances k, so the
+// solution update of the same iteration (launched after it) tests k-1.
+__device__ __forceinline__ bool gated_prev(const CGState *st) { return st->k - 1 > st->k_conv || st->k - 1 > st->kmax; }
+
+// partials of d = Re<p|mmp>
 template <typename T>
 __global__ void __launch_bounds__(256) k_cg_dot(int64_t nsites, int64_t stride, const cx<T> *__restrict__ p,
-                                                const cx<T> *__restrict__ mmp, CGState *st, double *partial,
-                                                unsigned int *ticket) {
+                                                const cx<T> *__restrict__ mmp, const CGState *st, double *partial) {
   if (gated(st)) return;
   __shared__ double sh[1][4];
   double v[1] = {0};
@@ -498,20 +634,14 @@ __global__ void __launch_bounds__(256) k_cg_dot(int64_t nsites, int64_t stride, 
     v[0] += (double)sr;
   }
   block_sum<1>(v, sh);
-  if (publish_partial<1>(v, partial, ticket)) {
-    final_sum<1>(v, partial, ticket, sh);
-    if (threadIdx.x == 0) {
-      st->d = v[0];
-      st->a = st->c / st->d;
-    }
-  }
+  write_partial<1>(v, partial);
 }
 
-// r = r - a mmp; cp = |r|^2; b = cp / c; convergence test cp <= rsq
+// partials of cp = |r - a mmp|^2 with r updated in place
 template <typename T>
 __global__ void __launch_bounds__(256) k_cg_axpy_norm(int64_t nsites, int64_t stride, cx<T> *__restrict__ r,
-                                                      const cx<T> *__restrict__ mmp, CGState *st, double *partial,
-                                                      unsigned int *ticket) {
+                                                      const cx<T> *__restrict__ mmp, const CGState *st,
+                                                      double *partial) {
   if (gated(st)) return;
   __shared__ double sh[1][4];
   const T a = (T)(-st->a);
@@ -531,63 +661,70 @@ __global__ void __launch_bounds__(256) k_cg_axpy_norm(int64_t nsites, int64_t st
     v[0] += (double)sr;
   }
   block_sum<1>(v, sh);
-  if (publish_partial<1>(v, partial, ticket)) {
-    final_sum<1>(v, partial, ticket, sh);
-    if (threadIdx.x == 0) {
-      const double cp = v[0];
-      st->b = cp / st->c;
-      st->cp = cp;
-      st->c = cp;
-      if (cp <= st->rsq && st->k_conv == INT_MAX) st->k_conv = st->k;
+  write_partial<1>(v, partial);
+}
+
+// which = 0: d = sum, a = c / d.   which = 1: cp = sum, b = cp / c, c = cp, convergence test.
+// The which = 1 finaliser advances k in every iteration, also once the solve has stopped, so the
+// gate of the following solution update (k-1) closes after the last real iteration.
+__global__ void __launch_bounds__(256) k_cg_fin(const double *partial, int nparts, CGState *st, int which) {
+  const bool skip = gated(st);
+  if (skip && which == 0) return;
+  __shared__ double sh[1][4];
+  double v[1] = {0};
+  if (!skip) sum_partials<1>(v, partial, nparts, sh);
+  if (threadIdx.x == 0) {
+    if (which == 0) {
+      st->d = v[0];
+      st->a = st->c / st->d;
+    } else {
+      if (!skip) {
+        const double cp = v[0];
+        st->b = cp / st->c;
+        st->cp = cp;
+        st->c = cp;
+        if (cp <= st->rsq && st->k_conv == INT_MAX) st->k_conv = st->k;
+      }
+      st->k = st->k + 1;
     }
   }
 }
 
-// psi = a p + psi; p = b p + r; the last block advances the iteration counter
+// psi = a p + psi; p = b p + r
 template <typename T>
 __global__ void __launch_bounds__(256) k_cg_update(int64_t n, cx<T> *__restrict__ psi, cx<T> *__restrict__ p,
-                                                   const cx<T> *__restrict__ r, CGState *st, unsigned int *ticket) {
-  if (gated(st)) return;
+                                                   const cx<T> *__restrict__ r, const CGState *st) {
+  if (gated_prev(st)) return;
   const T a = (T)st->a, b = (T)st->b;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const cx<T> pv = p[i], xv = psi[i], rv = r[i];
     psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
     p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
   }
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned int t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    *ticket = 0;
-    st->k = st->k + 1;
-  }
 }
 
 template <typename T>
 int launch_cg_dot(Ctx *ctx, int64_t nsites, int64_t stride, const void *p, const void *mmp, CGState *state) {
-  hipLaunchKernelGGL((k_cg_dot<T>), dim3(red_blocks(nsites)), dim3(256), 0, ctx->stream, nsites, stride,
-                     (const cx<T> *)p, (const cx<T> *)mmp, state, ctx->red_partial, ctx->red_ticket);
+  const int nb = red_blocks(nsites);
+  hipLaunchKernelGGL((k_cg_dot<T>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (const cx<T> *)p,
+                     (const cx<T> *)mmp, (const CGState *)state, ctx->red_partial);
+  hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb, state, 0);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
 template <typename T>
 int launch_cg_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const void *mmp, CGState *state) {
-  hipLaunchKernelGGL((k_cg_axpy_norm<T>), dim3(red_blocks(nsites)), dim3(256), 0, ctx->stream, nsites, stride,
-                     (cx<T> *)r, (const cx<T> *)mmp, state, ctx->red_partial, ctx->red_ticket);
+  const int nb = red_blocks(nsites);
+  hipLaunchKernelGGL((k_cg_axpy_norm<T>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (cx<T> *)r,
+                     (const cx<T> *)mmp, (const CGState *)state, ctx->red_partial);
+  hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb, state, 1);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
 template <typename T>
 int launch_cg_update(Ctx *ctx, int64_t n, void *psi, void *p, const void *r, CGState *state) {
-  int64_t nb = stream_blocks(n);
-  if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL((k_cg_update<T>), dim3(nb), dim3(256), 0, ctx->stream, n, (cx<T> *)psi, (cx<T> *)p,
-                     (const cx<T> *)r, state, ctx->red_ticket + 1);
+  hipLaunchKernelGGL((k_cg_update<T>), dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, (cx<T> *)psi,
+                     (cx<T> *)p, (const cx<T> *)r, (const CGState *)state);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }

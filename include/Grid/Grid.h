@@ -1,0 +1,798 @@
+// Grid/Grid.h — C++ facade with Grid's operator and solver API over libgx (include/gx.h).
+//
+// A Grid user's code for the even-odd Wilson / Mobius solve path compiles against this header
+// unchanged in form: the same class names, constructors, method names, argument meaning,
+// checkerboard tags and assert-on-error behaviour, with every operator application, BLAS1 call
+// and solve executed by the HIP library on one MI355X.  Fermion fields live in device memory;
+// gauge fields are host arrays handed to the operators (ImportGauge), as Grid's constructors take
+// them.  Single process, single GPU: the MPI layout arguments are accepted and ignored.
+//
+// Reference interfaces (paths relative to the Grid source tree):
+//   Grid_init / GridDefaultLatt           Grid/util/Init.cc:196-547
+//   GridCartesian / GridRedBlackCartesian  Grid/cartesian/Cartesian_full.h, Cartesian_red_black.h:56-304
+//   SpaceTimeGrid::make*Grid              Grid/qcd/utils/SpaceTimeGrid.cc:36-113
+//   Lattice BLAS1 / reductions            Grid/lattice/Lattice_arith.h:231-265, Lattice_reduction.h:202-410
+//   pick/setCheckerboard, precisionChange Grid/lattice/Lattice_transfer.h:50-80,1312-1492
+//   CheckerBoardedSparseMatrixBase        Grid/algorithms/SparseMatrix.h:37-82
+//   WilsonFermion                         Grid/qcd/action/fermion/WilsonFermion.h:139-142
+//   MobiusFermion                         Grid/qcd/action/fermion/MobiusFermion.h:45-71
+//   LinearOperatorBase / Schur operators  Grid/algorithms/LinearOperator.h:44-56,286-349
+//   ConjugateGradient                     Grid/algorithms/iterative/ConjugateGradient.h:43-257
+//   MixedPrecisionConjugateGradient       Grid/algorithms/iterative/ConjugateGradientMixedPrec.h:33-166
+//   SchurRedBlackDiagMooeeSolve           Grid/algorithms/iterative/SchurRedBlack.h:236-436
+#ifndef GRID_AMD_GRID_H
+#define GRID_AMD_GRID_H
+
+#include <cassert>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../gx.h"
+
+namespace Grid {
+
+typedef double RealD;
+typedef float RealF;
+typedef RealD Real;
+typedef std::complex<double> ComplexD;
+typedef ComplexD Complex;
+typedef int Integer;
+
+static const int Nd = 4;
+static const int Nc = 3;
+static const int Ns = 4;
+static const int Even = 0;
+static const int Odd = 1;
+static const int DaggerNo = 0;
+static const int DaggerYes = 1;
+
+#define GridLogMessage "Grid : Message : "
+#define GridLogIterative "Grid : Iterative : "
+#define GridLogPerformance "Grid : Performance : "
+#define GridLogDebug "Grid : Debug : "
+
+class Coordinate : public std::vector<int> {
+ public:
+  Coordinate() = default;
+  Coordinate(std::initializer_list<int> l) : std::vector<int>(l) {}
+  explicit Coordinate(size_t n, int v = 0) : std::vector<int>(n, v) {}
+  Coordinate(const std::vector<int> &v) : std::vector<int>(v) {}
+};
+
+struct Zero {};
+
+namespace detail {
+inline gx_ctx &context() {
+  static gx_ctx ctx = nullptr;
+  return ctx;
+}
+inline Coordinate &default_latt() {
+  static Coordinate latt({8, 8, 8, 8});
+  return latt;
+}
+// Grid reports errors by assert; so does the facade, with the library's message first.
+inline void check(int rc, const char *what) {
+  if (rc != GX_OK) {
+    std::cerr << "Grid (gx) error in " << what << ": " << gx_last_error() << " (status " << rc << ")" << std::endl;
+    assert(rc == GX_OK);
+    std::abort();
+  }
+}
+inline uint64_t mix(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+}  // namespace detail
+
+// Grid_init: parses --grid X.Y.Z.T and --device N; creates the HIP context.
+inline void Grid_init(int *argc, char ***argv) {
+  int device = 0;
+  for (int i = 1; argc && i < *argc; i++) {
+    std::string a = (*argv)[i];
+    if (a == "--grid" && i + 1 < *argc) {
+      Coordinate c;
+      std::stringstream ss((*argv)[i + 1]);
+      std::string tok;
+      while (std::getline(ss, tok, '.')) c.push_back(std::atoi(tok.c_str()));
+      if (c.size() == 4) detail::default_latt() = c;
+    } else if (a == "--device" && i + 1 < *argc) {
+      device = std::atoi((*argv)[i + 1]);
+    }
+  }
+  if (!detail::context()) detail::check(gx_ctx_create(device, &detail::context()), "Grid_init");
+}
+// Grid_finalize: waits for the device.  The context stays alive until the process exits, because
+// Grid programs call Grid_finalize() while their fields and operators are still in scope.
+inline void Grid_finalize() {
+  if (detail::context()) gx_ctx_sync(detail::context());
+}
+inline Coordinate GridDefaultLatt() { return detail::default_latt(); }
+inline Coordinate GridDefaultSimd(int nd, int) { return Coordinate(nd, 1); }
+inline Coordinate GridDefaultMpi() { return Coordinate(4, 1); }
+struct vComplex {
+  static int Nsimd() { return 1; }
+};
+typedef vComplex vComplexD;
+typedef vComplex vComplexF;
+struct GridThread {
+  static int GetThreads() { return 1; }
+};
+
+// ---------------- grids ----------------
+class GridBase {
+ public:
+  int dims[4] = {0, 0, 0, 0};
+  int Ls = 1;
+  bool checkerboarded = false;
+  virtual ~GridBase() = default;
+  int isCheckerBoarded() const { return checkerboarded ? 1 : 0; }
+  // Grid's 5D grids put s first: (Ls, x, y, z, t)
+  Coordinate GlobalDimensions() const {
+    Coordinate c;
+    if (Ls > 1) c.push_back(Ls);
+    for (int d = 0; d < 4; d++) c.push_back(dims[d]);
+    return c;
+  }
+  int Nd() const { return Ls > 1 ? 5 : 4; }
+  int64_t gSites() const { return (int64_t)dims[0] * dims[1] * dims[2] * dims[3] * Ls / (checkerboarded ? 2 : 1); }
+  int64_t oSites() const { return gSites(); }
+  int64_t lSites() const { return gSites(); }
+};
+
+class GridCartesian : public GridBase {
+ public:
+  GridCartesian(const Coordinate &latt, const Coordinate & = Coordinate(), const Coordinate & = Coordinate()) {
+    assert(latt.size() == 4);
+    for (int d = 0; d < 4; d++) dims[d] = latt[d];
+  }
+  GridCartesian(int ls, const GridCartesian *four) {
+    for (int d = 0; d < 4; d++) dims[d] = four->dims[d];
+    Ls = ls;
+  }
+};
+
+class GridRedBlackCartesian : public GridBase {
+ public:
+  explicit GridRedBlackCartesian(const GridBase *base) {
+    for (int d = 0; d < 4; d++) dims[d] = base->dims[d];
+    Ls = base->Ls;
+    checkerboarded = true;
+    assert(dims[0] % 2 == 0);
+  }
+};
+
+namespace SpaceTimeGrid {
+inline GridCartesian *makeFourDimGrid(const Coordinate &latt, const Coordinate &simd, const Coordinate &mpi) {
+  return new GridCartesian(latt, simd, mpi);
+}
+inline GridRedBlackCartesian *makeFourDimRedBlackGrid(const GridCartesian *g) { return new GridRedBlackCartesian(g); }
+inline GridCartesian *makeFiveDimGrid(int Ls, const GridCartesian *g) { return new GridCartesian(Ls, g); }
+inline GridRedBlackCartesian *makeFiveDimRedBlackGrid(int Ls, const GridCartesian *g) {
+  GridCartesian five(Ls, g);
+  return new GridRedBlackCartesian(&five);
+}
+}  // namespace SpaceTimeGrid
+
+// ---------------- random numbers ----------------
+class GridParallelRNG {
+ public:
+  uint64_t seed = 0x5EED;
+  uint64_t draws = 0;
+  explicit GridParallelRNG(const GridBase *) {}
+  void SeedFixedIntegers(const std::vector<int> &seeds) {
+    uint64_t s = 0x123456789ABCDEFULL;
+    for (int v : seeds) s = detail::mix(s ^ (uint64_t)(uint32_t)v);
+    seed = s;
+    draws = 0;
+  }
+  uint64_t next() { return detail::mix(seed + 0x1000193ULL * (++draws)); }
+};
+typedef GridParallelRNG GridSerialRNG;
+
+// ---------------- fermion fields (device resident) ----------------
+template <int PREC>
+class FermionLattice {
+ public:
+  GridBase *_grid = nullptr;
+  gx_field f = nullptr;
+
+  explicit FermionLattice(GridBase *g) : _grid(g) {
+    detail::check(gx_field_create(detail::context(), g->dims, g->Ls, PREC, g->checkerboarded ? GX_EVEN : GX_FULL, &f),
+                  "Lattice(grid)");
+  }
+  FermionLattice(const FermionLattice &o) : FermionLattice(o._grid) { *this = o; }
+  FermionLattice(FermionLattice &&o) noexcept : _grid(o._grid), f(o.f) { o.f = nullptr; }
+  ~FermionLattice() {
+    if (f) gx_field_destroy(f);
+  }
+  FermionLattice &operator=(const FermionLattice &o) {
+    if (this != &o) detail::check(gx_field_copy(f, o.f), "Lattice copy");
+    return *this;
+  }
+  FermionLattice &operator=(FermionLattice &&o) noexcept {
+    std::swap(f, o.f);
+    _grid = o._grid;
+    return *this;
+  }
+  FermionLattice &operator=(const Zero &) {
+    detail::check(gx_field_zero(f), "Lattice = Zero()");
+    return *this;
+  }
+  GridBase *Grid() const { return _grid; }
+  // Grid: Lattice::Checkerboard() (read and assign)
+  struct CbRef {
+    gx_field f;
+    operator int() const {
+      int cb;
+      gx_field_checkerboard(f, &cb);
+      return cb < 0 ? 0 : cb;
+    }
+    CbRef &operator=(int cb) {
+      int cur;
+      gx_field_checkerboard(f, &cur);
+      if (cur >= 0) gx_field_set_checkerboard_tag(f, cb);
+      return *this;
+    }
+  };
+  CbRef Checkerboard() { return CbRef{f}; }
+  int Checkerboard() const {
+    int cb;
+    gx_field_checkerboard(f, &cb);
+    return cb < 0 ? 0 : cb;
+  }
+  // host transfer in Grid's lexicographic layout (s innermost), complex doubles
+  size_t hostSize() const { return (size_t)_grid->gSites() * 12; }
+  void importHost(const ComplexD *p) { detail::check(gx_field_import(f, reinterpret_cast<const double *>(p)), "import"); }
+  void exportHost(ComplexD *p) const { detail::check(gx_field_export(f, reinterpret_cast<double *>(p)), "export"); }
+};
+
+typedef FermionLattice<GX_F64> LatticeFermionD;
+typedef FermionLattice<GX_F32> LatticeFermionF;
+typedef LatticeFermionD LatticeFermion;
+
+template <int P>
+inline RealD norm2(const FermionLattice<P> &x) {
+  double n;
+  detail::check(gx_norm2(x.f, &n), "norm2");
+  return n;
+}
+template <int P>
+inline ComplexD innerProduct(const FermionLattice<P> &x, const FermionLattice<P> &y) {
+  double re, im;
+  detail::check(gx_inner_product(x.f, y.f, &re, &im), "innerProduct");
+  return ComplexD(re, im);
+}
+template <int P>
+inline void innerProductNorm(ComplexD &ip, RealD &nrm, const FermionLattice<P> &x, const FermionLattice<P> &y) {
+  double re, im;
+  detail::check(gx_inner_product_norm(x.f, y.f, &re, &im, &nrm), "innerProductNorm");
+  ip = ComplexD(re, im);
+}
+template <int P>
+inline void axpy(FermionLattice<P> &z, RealD a, const FermionLattice<P> &x, const FermionLattice<P> &y) {
+  detail::check(gx_axpy(z.f, a, x.f, y.f), "axpy");
+}
+template <int P>
+inline void axpby(FermionLattice<P> &z, RealD a, RealD b, const FermionLattice<P> &x, const FermionLattice<P> &y) {
+  detail::check(gx_axpby(z.f, a, b, x.f, y.f), "axpby");
+}
+template <int P>
+inline RealD axpy_norm(FermionLattice<P> &z, RealD a, const FermionLattice<P> &x, const FermionLattice<P> &y) {
+  double n;
+  detail::check(gx_axpy_norm(z.f, a, x.f, y.f, &n), "axpy_norm");
+  return n;
+}
+template <int P>
+inline FermionLattice<P> operator-(const FermionLattice<P> &a, const FermionLattice<P> &b) {
+  FermionLattice<P> r(a.Grid());
+  axpby(r, 1.0, -1.0, a, b);
+  return r;
+}
+template <int P>
+inline FermionLattice<P> operator+(const FermionLattice<P> &a, const FermionLattice<P> &b) {
+  FermionLattice<P> r(a.Grid());
+  axpby(r, 1.0, 1.0, a, b);
+  return r;
+}
+template <int P>
+inline FermionLattice<P> operator*(RealD s, const FermionLattice<P> &a) {
+  FermionLattice<P> r(a.Grid());
+  detail::check(gx_scale(r.f, s, a.f), "scale");
+  return r;
+}
+template <int P>
+inline void pickCheckerboard(int cb, FermionLattice<P> &half, const FermionLattice<P> &full) {
+  detail::check(gx_pick_checkerboard(cb, half.f, full.f), "pickCheckerboard");
+}
+template <int P>
+inline void setCheckerboard(FermionLattice<P> &full, const FermionLattice<P> &half) {
+  detail::check(gx_set_checkerboard(full.f, half.f), "setCheckerboard");
+}
+template <int P, int Q>
+inline void precisionChange(FermionLattice<P> &out, const FermionLattice<Q> &in) {
+  detail::check(gx_precision_change(out.f, in.f), "precisionChange");
+}
+template <int P>
+inline void gaussian(GridParallelRNG &rng, FermionLattice<P> &x) {
+  GridBase *g = x.Grid();
+  GridCartesian full(Coordinate({g->dims[0], g->dims[1], g->dims[2], g->dims[3]}));
+  std::vector<ComplexD> host((size_t)g->dims[0] * g->dims[1] * g->dims[2] * g->dims[3] * g->Ls * 12);
+  detail::check(gx_gaussian_fermion(g->dims, g->Ls, rng.next(), reinterpret_cast<double *>(host.data())), "gaussian");
+  if (!g->checkerboarded) {
+    x.importHost(host.data());
+    return;
+  }
+  // a half field takes the sites of its checkerboard (x halved, Grid red-black order)
+  const int cb = x.Checkerboard();
+  std::vector<ComplexD> half(host.size() / 2);
+  const int lx = g->dims[0], ly = g->dims[1], lz = g->dims[2], lt = g->dims[3], Ls = g->Ls;
+  for (int t = 0; t < lt; t++)
+    for (int z = 0; z < lz; z++)
+      for (int y = 0; y < ly; y++)
+        for (int xx = 0; xx < lx; xx++) {
+          if (((xx + y + z + t) & 1) != cb) continue;
+          const int64_t s4 = xx + (int64_t)lx * (y + (int64_t)ly * (z + (int64_t)lz * t));
+          const int64_t h = (xx >> 1) + (int64_t)(lx / 2) * (y + (int64_t)ly * (z + (int64_t)lz * t));
+          std::memcpy(&half[h * Ls * 12], &host[s4 * Ls * 12], sizeof(ComplexD) * Ls * 12);
+        }
+  x.importHost(half.data());
+}
+template <int P>
+inline void random(GridParallelRNG &rng, FermionLattice<P> &x) {
+  gaussian(rng, x);
+}
+
+// ---------------- gauge fields (host resident, Grid lexicographic [site][mu][a][b]) ----------------
+class LatticeGaugeFieldD {
+ public:
+  GridBase *_grid;
+  std::vector<ComplexD> data;
+  explicit LatticeGaugeFieldD(GridBase *g) : _grid(g), data((size_t)g->dims[0] * g->dims[1] * g->dims[2] * g->dims[3] * 36) {}
+  GridBase *Grid() const { return _grid; }
+  LatticeGaugeFieldD &operator=(const Zero &) {
+    std::fill(data.begin(), data.end(), ComplexD(0));
+    return *this;
+  }
+  const double *raw() const { return reinterpret_cast<const double *>(data.data()); }
+  double *raw() { return reinterpret_cast<double *>(data.data()); }
+};
+typedef LatticeGaugeFieldD LatticeGaugeFieldF;
+typedef LatticeGaugeFieldD LatticeGaugeField;
+inline void precisionChange(LatticeGaugeFieldD &out, const LatticeGaugeFieldD &in) { out.data = in.data; }
+
+template <int N>
+struct SU {
+  // Random SU(3) links exp(i eps H), H Hermitian traceless Gaussian; eps = 1 for a hot start.
+  static void HotConfiguration(GridParallelRNG &rng, LatticeGaugeFieldD &U, double eps = 1.0) {
+    detail::check(gx_random_gauge(U.Grid()->dims, rng.next(), eps, U.raw()), "HotConfiguration");
+  }
+  static void ColdConfiguration(GridParallelRNG &, LatticeGaugeFieldD &U) { ColdConfiguration(U); }
+  static void ColdConfiguration(LatticeGaugeFieldD &U) {
+    std::fill(U.data.begin(), U.data.end(), ComplexD(0));
+    for (size_t l = 0; l < U.data.size() / 9; l++)
+      for (int a = 0; a < 3; a++) U.data[l * 9 + a * 4] = 1.0;
+  }
+};
+typedef SU<3> SU3;
+
+// ---------------- operators ----------------
+struct WilsonImplParams {
+  std::vector<ComplexD> boundary_phases{1, 1, 1, 1};
+};
+typedef WilsonImplParams ImplParams;
+
+template <class Field>
+class SparseMatrixBase {
+ public:
+  virtual GridBase *Grid() = 0;
+  virtual void M(const Field &in, Field &out) = 0;
+  virtual void Mdag(const Field &in, Field &out) = 0;
+  virtual void MdagM(const Field &in, Field &out) {
+    Field tmp(in.Grid());
+    M(in, tmp);
+    Mdag(tmp, out);
+  }
+  virtual ~SparseMatrixBase() = default;
+};
+
+template <class Field>
+class CheckerBoardedSparseMatrixBase : public SparseMatrixBase<Field> {
+ public:
+  virtual GridBase *RedBlackGrid() = 0;
+  virtual RealD Mass() { return 0.0; }
+  virtual void Meooe(const Field &in, Field &out) = 0;
+  virtual void Mooee(const Field &in, Field &out) = 0;
+  virtual void MooeeInv(const Field &in, Field &out) = 0;
+  virtual void MeooeDag(const Field &in, Field &out) = 0;
+  virtual void MooeeDag(const Field &in, Field &out) = 0;
+  virtual void MooeeInvDag(const Field &in, Field &out) = 0;
+  // the underlying libgx operator (lets the solvers run device-resident)
+  virtual gx_op handle() const = 0;
+};
+
+// Common body of WilsonFermion / MobiusFermion: a gx operator plus Grid's method names.
+template <int PREC>
+class FermionOperatorBase : public CheckerBoardedSparseMatrixBase<FermionLattice<PREC>> {
+ public:
+  typedef FermionLattice<PREC> FermionField;
+  typedef LatticeGaugeFieldD GaugeField;
+  GridBase *_FGrid, *_FrbGrid;
+  gx_op op = nullptr;
+  RealD mass;
+  FermionOperatorBase(GridBase *fg, GridBase *frb, RealD m) : _FGrid(fg), _FrbGrid(frb), mass(m) {}
+  ~FermionOperatorBase() override {
+    if (op) gx_op_destroy(op);
+  }
+  FermionOperatorBase(const FermionOperatorBase &) = delete;
+  FermionOperatorBase &operator=(const FermionOperatorBase &) = delete;
+  gx_op handle() const override { return op; }
+  GridBase *Grid() override { return _FGrid; }
+  GridBase *RedBlackGrid() override { return _FrbGrid; }
+  GridBase *FermionGrid() { return _FGrid; }
+  GridBase *FermionRedBlackGrid() { return _FrbGrid; }
+  RealD Mass() override { return mass; }
+  void ImportGauge(const GaugeField &U) { detail::check(gx_op_import_gauge(op, U.raw()), "ImportGauge"); }
+  void M(const FermionField &in, FermionField &out) override { detail::check(gx_M(op, in.f, out.f), "M"); }
+  void Mdag(const FermionField &in, FermionField &out) override { detail::check(gx_Mdag(op, in.f, out.f), "Mdag"); }
+  void Meooe(const FermionField &in, FermionField &out) override { detail::check(gx_Meooe(op, in.f, out.f), "Meooe"); }
+  void MeooeDag(const FermionField &in, FermionField &out) override {
+    detail::check(gx_MeooeDag(op, in.f, out.f), "MeooeDag");
+  }
+  void Mooee(const FermionField &in, FermionField &out) override { detail::check(gx_Mooee(op, in.f, out.f), "Mooee"); }
+  void MooeeDag(const FermionField &in, FermionField &out) override {
+    detail::check(gx_MooeeDag(op, in.f, out.f), "MooeeDag");
+  }
+  void MooeeInv(const FermionField &in, FermionField &out) override {
+    detail::check(gx_MooeeInv(op, in.f, out.f), "MooeeInv");
+  }
+  void MooeeInvDag(const FermionField &in, FermionField &out) override {
+    detail::check(gx_MooeeInvDag(op, in.f, out.f), "MooeeInvDag");
+  }
+  void Mdiag(const FermionField &in, FermionField &out) { Mooee(in, out); }
+  void Dhop(const FermionField &in, FermionField &out, int dag) { detail::check(gx_Dhop(op, in.f, out.f, dag), "Dhop"); }
+  void DhopEO(const FermionField &in, FermionField &out, int dag) {
+    detail::check(gx_DhopEO(op, in.f, out.f, dag), "DhopEO");
+  }
+  void DhopOE(const FermionField &in, FermionField &out, int dag) {
+    detail::check(gx_DhopOE(op, in.f, out.f, dag), "DhopOE");
+  }
+
+ protected:
+  static const double *phases_of(const ImplParams &p, std::vector<double> &buf) {
+    buf.resize(8);
+    for (int mu = 0; mu < 4; mu++) {
+      buf[2 * mu] = p.boundary_phases[mu].real();
+      buf[2 * mu + 1] = p.boundary_phases[mu].imag();
+    }
+    return buf.data();
+  }
+};
+
+// Grid: WilsonFermion<Impl>(GaugeField&, GridCartesian&, GridRedBlackCartesian&, RealD mass, params)
+template <int PREC>
+class WilsonFermion : public FermionOperatorBase<PREC> {
+ public:
+  WilsonFermion(const LatticeGaugeFieldD &U, GridCartesian &Fgrid, GridRedBlackCartesian &Hgrid, RealD mass,
+                const ImplParams &p = ImplParams())
+      : FermionOperatorBase<PREC>(&Fgrid, &Hgrid, mass) {
+    std::vector<double> ph;
+    detail::check(gx_wilson_create(detail::context(), Fgrid.dims, Fgrid.Ls, mass, U.raw(),
+                                   FermionOperatorBase<PREC>::phases_of(p, ph), &this->op),
+                  "WilsonFermion");
+  }
+};
+typedef WilsonFermion<GX_F64> WilsonFermionD;
+typedef WilsonFermion<GX_F32> WilsonFermionF;
+typedef WilsonFermionD WilsonFermionR;
+
+// Grid: MobiusFermion(U, FiveDimGrid, FiveDimRedBlackGrid, FourDimGrid, FourDimRedBlackGrid, mass, M5, b, c)
+template <int PREC>
+class MobiusFermion : public FermionOperatorBase<PREC> {
+ public:
+  RealD M5, _b, _c;
+  MobiusFermion(const LatticeGaugeFieldD &U, GridCartesian &FiveDimGrid, GridRedBlackCartesian &FiveDimRedBlackGrid,
+                GridCartesian &FourDimGrid, GridRedBlackCartesian &, RealD mass, RealD m5, RealD b, RealD c,
+                const ImplParams &p = ImplParams())
+      : FermionOperatorBase<PREC>(&FiveDimGrid, &FiveDimRedBlackGrid, mass), M5(m5), _b(b), _c(c) {
+    assert(FiveDimGrid.Ls > 1);
+    for (int d = 0; d < 4; d++) assert(FourDimGrid.dims[d] == FiveDimGrid.dims[d]);
+    std::vector<double> ph;
+    detail::check(gx_mobius_create(detail::context(), FiveDimGrid.dims, FiveDimGrid.Ls, mass, m5, b, c, U.raw(),
+                                   FermionOperatorBase<PREC>::phases_of(p, ph), &this->op),
+                  "MobiusFermion");
+  }
+  typedef FermionLattice<PREC> FermionField;
+  void M5D(const FermionField &psi, FermionField &chi) { detail::check(gx_M5D(this->op, psi.f, chi.f), "M5D"); }
+  void M5Ddag(const FermionField &psi, FermionField &chi) {
+    detail::check(gx_M5Ddag(this->op, psi.f, chi.f), "M5Ddag");
+  }
+  void Meooe5D(const FermionField &psi, FermionField &chi) {
+    detail::check(gx_Meooe5D(this->op, psi.f, chi.f), "Meooe5D");
+  }
+  void MeooeDag5D(const FermionField &psi, FermionField &chi) {
+    detail::check(gx_MeooeDag5D(this->op, psi.f, chi.f), "MeooeDag5D");
+  }
+};
+typedef MobiusFermion<GX_F64> MobiusFermionD;
+typedef MobiusFermion<GX_F32> MobiusFermionF;
+typedef MobiusFermionD MobiusFermionR;
+
+// ---------------- linear operators ----------------
+template <class Field>
+class LinearOperatorBase {
+ public:
+  virtual void Op(const Field &in, Field &out) = 0;
+  virtual void AdjOp(const Field &in, Field &out) = 0;
+  virtual void HermOpAndNorm(const Field &in, Field &out, RealD &n1, RealD &n2) = 0;
+  virtual void HermOp(const Field &in, Field &out) = 0;
+  // set when the operator is a Schur operator over a libgx action: solvers then run on device
+  virtual gx_op schur_handle() const { return nullptr; }
+  virtual ~LinearOperatorBase() = default;
+};
+
+template <class Field>
+class SchurOperatorBase : public LinearOperatorBase<Field> {
+ public:
+  virtual void Mpc(const Field &in, Field &out) = 0;
+  virtual void MpcDag(const Field &in, Field &out) = 0;
+  virtual void MpcDagMpc(const Field &in, Field &out) {
+    Field tmp(in.Grid());
+    tmp.Checkerboard() = in.Checkerboard();
+    Mpc(in, tmp);
+    MpcDag(tmp, out);
+  }
+  void HermOpAndNorm(const Field &in, Field &out, RealD &n1, RealD &n2) override {
+    MpcDagMpc(in, out);
+    ComplexD dot = innerProduct(in, out);
+    n1 = dot.real();
+    n2 = norm2(out);
+  }
+  void HermOp(const Field &in, Field &out) override { MpcDagMpc(in, out); }
+  void Op(const Field &in, Field &out) override { Mpc(in, out); }
+  void AdjOp(const Field &in, Field &out) override { MpcDag(in, out); }
+};
+
+// Grid: SchurDiagMooeeOperator<Matrix,Field>(Matrix&), Mpc = Mooee - Meo MooeeInv Moe
+template <class Matrix, class Field>
+class SchurDiagMooeeOperator : public SchurOperatorBase<Field> {
+ public:
+  Matrix &_Mat;
+  explicit SchurDiagMooeeOperator(Matrix &Mat) : _Mat(Mat) {}
+  void Mpc(const Field &in, Field &out) override { detail::check(gx_Mpc(_Mat.handle(), in.f, out.f), "Mpc"); }
+  void MpcDag(const Field &in, Field &out) override { detail::check(gx_MpcDag(_Mat.handle(), in.f, out.f), "MpcDag"); }
+  void MpcDagMpc(const Field &in, Field &out) override {
+    detail::check(gx_MpcDagMpc(_Mat.handle(), in.f, out.f), "MpcDagMpc");
+  }
+  void HermOpAndNorm(const Field &in, Field &out, RealD &n1, RealD &n2) override {
+    detail::check(gx_HermOpAndNorm(_Mat.handle(), in.f, out.f, &n1, &n2), "HermOpAndNorm");
+  }
+  gx_op schur_handle() const override { return _Mat.handle(); }
+};
+
+// Grid: MdagMLinearOperator<Matrix,Field> (unpreconditioned normal operator)
+template <class Matrix, class Field>
+class MdagMLinearOperator : public LinearOperatorBase<Field> {
+ public:
+  Matrix &_Mat;
+  explicit MdagMLinearOperator(Matrix &Mat) : _Mat(Mat) {}
+  void Op(const Field &in, Field &out) override { _Mat.M(in, out); }
+  void AdjOp(const Field &in, Field &out) override { _Mat.Mdag(in, out); }
+  void HermOp(const Field &in, Field &out) override {
+    Field tmp(in.Grid());
+    _Mat.M(in, tmp);
+    _Mat.Mdag(tmp, out);
+  }
+  void HermOpAndNorm(const Field &in, Field &out, RealD &n1, RealD &n2) override {
+    HermOp(in, out);
+    n1 = innerProduct(in, out).real();
+    n2 = norm2(out);
+  }
+};
+
+template <class Field>
+class OperatorFunction {
+ public:
+  virtual void operator()(LinearOperatorBase<Field> &Linop, const Field &in, Field &out) = 0;
+  virtual ~OperatorFunction() = default;
+};
+
+// Grid: ConjugateGradient<Field>(tol, maxit, err_on_no_conv).  On a Schur operator over a libgx
+// action the whole iteration runs on the device (gx_cg); on any other LinearOperatorBase the same
+// Hestenes-Stiefel loop (ConjugateGradient.h:68-257) runs here over the facade's BLAS1.
+template <class Field>
+class ConjugateGradient : public OperatorFunction<Field> {
+ public:
+  RealD Tolerance;
+  Integer MaxIterations;
+  bool ErrorOnNoConverge;
+  Integer IterationsToComplete = 0;
+  RealD TrueResidual = 0;
+  ConjugateGradient(RealD tol, Integer maxit, bool err_on_no_conv = true)
+      : Tolerance(tol), MaxIterations(maxit), ErrorOnNoConverge(err_on_no_conv) {}
+
+  void operator()(LinearOperatorBase<Field> &Linop, const Field &src, Field &psi) override {
+    psi.Checkerboard() = src.Checkerboard();
+    if (gx_op op = Linop.schur_handle()) {
+      gx_solver_stats st;
+      const int rc = gx_cg(op, src.f, psi.f, Tolerance, MaxIterations, &st);
+      IterationsToComplete = st.iterations;
+      TrueResidual = st.true_residual;
+      std::cout << GridLogMessage << "ConjugateGradient Converged on iteration " << st.iterations
+                << "\tComputed residual " << st.computed_residual << "\tTrue residual " << st.true_residual
+                << "\tTarget " << Tolerance << std::endl;
+      if (rc == GX_ENOCONV) {
+        std::cout << GridLogMessage << "ConjugateGradient did NOT converge " << MaxIterations << std::endl;
+        if (ErrorOnNoConverge) assert(0);
+        return;
+      }
+      detail::check(rc, "ConjugateGradient");
+      if (ErrorOnNoConverge) assert(TrueResidual / Tolerance < 10000.0);
+      return;
+    }
+    host_cg(Linop, src, psi);
+  }
+
+ private:
+  void host_cg(LinearOperatorBase<Field> &Linop, const Field &src, Field &psi) {
+    RealD cp, c, a, d, b, ssq;
+    Field p(src.Grid()), mmp(src.Grid()), r(src.Grid());
+    p.Checkerboard() = mmp.Checkerboard() = r.Checkerboard() = src.Checkerboard();
+    ssq = norm2(src);
+    RealD guess = norm2(psi);
+    assert(std::isnan(guess) == 0);
+    if (guess == 0.0) {
+      r = src;
+      p = r;
+      a = ssq;
+    } else {
+      Linop.HermOpAndNorm(psi, mmp, d, b);
+      r = src - mmp;
+      p = r;
+      a = norm2(p);
+    }
+    cp = a;
+    if (ssq == 0.) {
+      psi = Zero();
+      IterationsToComplete = 1;
+      TrueResidual = 0.;
+      return;
+    }
+    const RealD rsq = Tolerance * Tolerance * ssq;
+    if (cp <= rsq) {
+      TrueResidual = std::sqrt(a / ssq);
+      IterationsToComplete = 0;
+      return;
+    }
+    for (int k = 1; k <= MaxIterations; k++) {
+      c = cp;
+      Linop.HermOp(p, mmp);
+      d = innerProduct(p, mmp).real();
+      a = c / d;
+      cp = axpy_norm(r, -a, mmp, r);
+      b = cp / c;
+      axpy(psi, a, p, psi);
+      axpby(p, b, 1.0, p, r);
+      if (cp <= rsq) {
+        Linop.HermOp(psi, mmp);
+        p = mmp - src;
+        TrueResidual = std::sqrt(norm2(p)) / std::sqrt(norm2(src));
+        IterationsToComplete = k;
+        std::cout << GridLogMessage << "ConjugateGradient Converged on iteration " << k << "\tComputed residual "
+                  << std::sqrt(cp / ssq) << "\tTrue residual " << TrueResidual << "\tTarget " << Tolerance
+                  << std::endl;
+        if (ErrorOnNoConverge) assert(TrueResidual / Tolerance < 10000.0);
+        return;
+      }
+    }
+    std::cout << GridLogMessage << "ConjugateGradient did NOT converge " << MaxIterations << std::endl;
+    if (ErrorOnNoConverge) assert(0);
+    IterationsToComplete = MaxIterations;
+  }
+};
+
+// Grid: MixedPrecisionConjugateGradient<FieldD,FieldF>(tol, maxinner, maxouter, sp_grid, Linop_f, Linop_d).
+// Defect correction with FP32 inner CG on the device; Linop_f must wrap the same action in single
+// precision (Grid builds it from a precisionChange'd gauge field; libgx keeps both precisions of
+// the one action).
+template <class FieldD, class FieldF>
+class MixedPrecisionConjugateGradient {
+ public:
+  RealD Tolerance, InnerTolerance;
+  Integer MaxInnerIterations, MaxOuterIterations;
+  GridBase *SinglePrecGrid;
+  LinearOperatorBase<FieldF> &Linop_f;
+  LinearOperatorBase<FieldD> &Linop_d;
+  Integer TotalInnerIterations = 0, TotalOuterIterations = 0, TotalFinalStepIterations = 0;
+  RealD TrueResidual = 0;
+  MixedPrecisionConjugateGradient(RealD tol, Integer maxinnerit, Integer maxouterit, GridBase *sp_grid,
+                                  LinearOperatorBase<FieldF> &lf, LinearOperatorBase<FieldD> &ld)
+      : Tolerance(tol), InnerTolerance(tol), MaxInnerIterations(maxinnerit), MaxOuterIterations(maxouterit),
+        SinglePrecGrid(sp_grid), Linop_f(lf), Linop_d(ld) {}
+  void operator()(const FieldD &src_d_in, FieldD &sol_d) {
+    gx_op op = Linop_d.schur_handle();
+    assert(op && Linop_f.schur_handle());
+    sol_d.Checkerboard() = src_d_in.Checkerboard();
+    gx_solver_stats st;
+    const int rc = gx_mixed_cg(op, src_d_in.f, sol_d.f, Tolerance, InnerTolerance, GX_F32, MaxInnerIterations,
+                               MaxOuterIterations, &st);
+    TotalInnerIterations = st.inner_iterations;
+    TotalOuterIterations = st.outer_iterations;
+    TotalFinalStepIterations = st.final_iterations;
+    TrueResidual = st.true_residual;
+    std::cout << GridLogMessage << "MixedPrecisionConjugateGradient: Inner CG iterations " << TotalInnerIterations
+              << " Restarts " << TotalOuterIterations << " Final CG iterations " << TotalFinalStepIterations
+              << " True residual " << TrueResidual << std::endl;
+    if (rc != GX_ENOCONV) detail::check(rc, "MixedPrecisionConjugateGradient");
+  }
+};
+
+// Grid: SchurRedBlackDiagMooeeSolve<Field>(OperatorFunction&)(Matrix&, in, out), SchurRedBlack.h:236-436
+template <class Field>
+class SchurRedBlackDiagMooeeSolve {
+ public:
+  OperatorFunction<Field> &_HermitianRBSolver;
+  explicit SchurRedBlackDiagMooeeSolve(OperatorFunction<Field> &solver, bool = false, bool = false)
+      : _HermitianRBSolver(solver) {}
+  template <class Matrix>
+  void operator()(Matrix &_Matrix, const Field &in, Field &out) {
+    GridBase *grid = _Matrix.RedBlackGrid();
+    GridBase *fgrid = _Matrix.Grid();
+    Field resid(fgrid), src_o(grid), src_e(grid), sol_o(grid);
+    RedBlackSource(_Matrix, in, src_e, src_o);
+    sol_o = Zero();
+    sol_o.Checkerboard() = Odd;
+    SchurDiagMooeeOperator<Matrix, Field> HermOpEO(_Matrix);
+    _HermitianRBSolver(HermOpEO, src_o, sol_o);
+    assert(sol_o.Checkerboard() == Odd);
+    RedBlackSolution(_Matrix, sol_o, src_e, out);
+    _Matrix.M(out, resid);
+    resid = resid - in;
+    RealD ns = norm2(in), nr = norm2(resid);
+    std::cout << GridLogMessage << "SchurRedBlackBase solver true unprec resid " << std::sqrt(nr / ns) << std::endl;
+  }
+  template <class Matrix>
+  void RedBlackSource(Matrix &_Matrix, const Field &src, Field &src_e, Field &src_o) {
+    GridBase *grid = _Matrix.RedBlackGrid();
+    Field tmp(grid), Mtmp(grid);
+    pickCheckerboard(Even, src_e, src);
+    pickCheckerboard(Odd, src_o, src);
+    _Matrix.MooeeInv(src_e, tmp);
+    assert(tmp.Checkerboard() == Even);
+    _Matrix.Meooe(tmp, Mtmp);
+    assert(Mtmp.Checkerboard() == Odd);
+    tmp = src_o - Mtmp;
+    assert(tmp.Checkerboard() == Odd);
+    SchurDiagMooeeOperator<Matrix, Field> HermOpEO(_Matrix);
+    HermOpEO.MpcDag(tmp, src_o);
+    assert(src_o.Checkerboard() == Odd);
+  }
+  template <class Matrix>
+  void RedBlackSolution(Matrix &_Matrix, const Field &sol_o, const Field &src_e, Field &sol) {
+    GridBase *grid = _Matrix.RedBlackGrid();
+    Field tmp(grid), sol_e(grid), src_e_i(grid);
+    _Matrix.Meooe(sol_o, tmp);
+    assert(tmp.Checkerboard() == Even);
+    src_e_i = src_e - tmp;
+    assert(src_e_i.Checkerboard() == Even);
+    _Matrix.MooeeInv(src_e_i, sol_e);
+    assert(sol_e.Checkerboard() == Even);
+    setCheckerboard(sol, sol_e);
+    setCheckerboard(sol, sol_o);
+  }
+};
+
+}  // namespace Grid
+
+#endif

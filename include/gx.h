@@ -50,7 +50,8 @@ const char *gx_last_error(void);
 /* Library version string. */
 const char *gx_version(void);
 
-/* ---- context: device + stream (Grid: acceleratorInit, Grid/threads/Accelerator.cc:120-200) ---- */
+/* ---- context: device + stream (Grid: acceleratorInit, Grid/threads/Accelerator.cc:120-200).
+ *      Destroy every field and operator created on a context before the context itself. ---- */
 int gx_ctx_create(int device, gx_ctx *out);
 int gx_ctx_destroy(gx_ctx ctx);
 int gx_ctx_sync(gx_ctx ctx);
@@ -158,6 +159,19 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
  * unprec_residual = |M sol - src| / |src|. */
 int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, int mixed,
                    gx_solver_stats *st, double *unprec_residual);
+
+/* ---- launch-shape tuning (no Grid counterpart: Grid fixes its launch shape at build time,
+ *      Grid/threads/Accelerator.h:416-461).  Process-wide knobs read by the kernel launchers;
+ *      the autotuner in grid_amd/autotune.py sets them from measurements. ---- */
+enum gx_tune_knob {
+  GX_TUNE_DHOP_MAP = 0,     /* 0: linear workgroup->tile map, 1: XCD-contiguous slabs */
+  GX_TUNE_DHOP_PF = 1,      /* 1: issue the next leg's neighbour loads before this leg's math */
+  GX_TUNE_DHOP_NT = 2,      /* 1: non-temporal (streaming) output stores */
+  GX_TUNE_DHOP_ORDER = 3,   /* 0: Grid's leg order, 1: the two legs of each direction adjacent */
+  GX_TUNE_COUNT = 8
+};
+int gx_tune_set(int knob, int value);
+int gx_tune_get(int knob, int *value);
 
 /* ---- synthetic inputs (host utilities, no device work) ----
  * Gauge: per link G with Re,Im ~ N(0,1); H = (G+G^dag)/2 - tr/3; U = exp(i eps H) by

@@ -1,0 +1,52 @@
+// Mirrors Grid's tests/Test_dwf_mixedcg_prec.cc: Mobius DWF Schur system solved with
+// MixedPrecisionConjugateGradient (FP32 inner, FP64 outer) and with plain FP64 ConjugateGradient;
+// both reach the FP64 tolerance and agree.
+#include <Grid/Grid.h>
+
+using namespace Grid;
+
+int main(int argc, char **argv) {
+  Grid_init(&argc, &argv);
+  const int Ls = 8;
+  GridCartesian *UGrid = SpaceTimeGrid::makeFourDimGrid(GridDefaultLatt(), GridDefaultSimd(Nd, vComplex::Nsimd()),
+                                                        GridDefaultMpi());
+  GridRedBlackCartesian *UrbGrid = SpaceTimeGrid::makeFourDimRedBlackGrid(UGrid);
+  GridCartesian *FGrid = SpaceTimeGrid::makeFiveDimGrid(Ls, UGrid);
+  GridRedBlackCartesian *FrbGrid = SpaceTimeGrid::makeFiveDimRedBlackGrid(Ls, UGrid);
+  GridParallelRNG RNG5(FGrid);
+  RNG5.SeedFixedIntegers(std::vector<int>({1, 2, 3, 4}));
+  GridParallelRNG RNG4(UGrid);
+  RNG4.SeedFixedIntegers(std::vector<int>({5, 6, 7, 8}));
+  LatticeFermionD src(FGrid); random(RNG5, src);
+  LatticeGaugeFieldD Umu(UGrid);
+  SU<Nc>::HotConfiguration(RNG4, Umu, 0.3);
+  LatticeGaugeFieldF Umu_f(UGrid);
+  precisionChange(Umu_f, Umu);
+
+  RealD mass = 0.1, M5 = 1.8, b = 1.5, c = 0.5;
+  MobiusFermionD Ddwf(Umu, *FGrid, *FrbGrid, *UGrid, *UrbGrid, mass, M5, b, c);
+  MobiusFermionF Ddwf_f(Umu_f, *FGrid, *FrbGrid, *UGrid, *UrbGrid, mass, M5, b, c);
+  LatticeFermionD src_o(FrbGrid), result_o(FrbGrid), result_o_2(FrbGrid);
+  pickCheckerboard(Odd, src_o, src);
+  result_o = Zero();
+  result_o.Checkerboard() = Odd;
+  result_o_2 = Zero();
+  result_o_2.Checkerboard() = Odd;
+
+  SchurDiagMooeeOperator<MobiusFermionD, LatticeFermionD> HermOpEO(Ddwf);
+  SchurDiagMooeeOperator<MobiusFermionF, LatticeFermionF> HermOpEO_f(Ddwf_f);
+  MixedPrecisionConjugateGradient<LatticeFermionD, LatticeFermionF> mCG(1.0e-10, 10000, 50, FrbGrid, HermOpEO_f,
+                                                                         HermOpEO);
+  mCG.InnerTolerance = 1.0e-5;
+  mCG(src_o, result_o);
+  ConjugateGradient<LatticeFermionD> CG(1.0e-10, 10000);
+  CG(HermOpEO, src_o, result_o_2);
+  LatticeFermionD diff = result_o - result_o_2;
+  const double rel = std::sqrt(norm2(diff) / norm2(result_o_2));
+  std::cout << GridLogMessage << "mixed true residual " << mCG.TrueResidual << ", FP64 true residual "
+            << CG.TrueResidual << ", relative difference " << rel << std::endl;
+  const int fail = !(mCG.TrueResidual < 1e-9) || !(CG.TrueResidual < 1e-9) || !(rel < 1e-7);
+  Grid_finalize();
+  std::cout << (fail ? "FAILED" : "PASSED") << std::endl;
+  return fail;
+}
