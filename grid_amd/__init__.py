@@ -92,6 +92,8 @@ _SIG = {
     'gx_M5Ddag': ([_P, _P, _P], _I),
     'gx_Meooe5D': ([_P, _P, _P], _I),
     'gx_MeooeDag5D': ([_P, _P, _P], _I),
+    'gx_import_physical_fermion_source': ([_P, _P, _P], _I),
+    'gx_export_physical_fermion_solution': ([_P, _P, _P], _I),
     'gx_Mpc': ([_P, _P, _P], _I),
     'gx_MpcDag': ([_P, _P, _P], _I),
     'gx_MpcDagMpc': ([_P, _P, _P], _I),
@@ -99,6 +101,7 @@ _SIG = {
     'gx_cg': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_mixed_cg': ([_P, _P, _P, _D, _D, _I, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
+    'gx_block_cg': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_tune_set': ([_I, _I], _I),
     'gx_tune_get': ([_I, _IP], _I),
     'gx_random_gauge': ([_IP, ctypes.c_uint64, _D, _DP], _I),
@@ -299,6 +302,8 @@ class FermionOp:
     def M5Ddag(self, i, o): self._call('M5Ddag', i.h, o.h)
     def Meooe5D(self, i, o): self._call('Meooe5D', i.h, o.h)
     def MeooeDag5D(self, i, o): self._call('MeooeDag5D', i.h, o.h)
+    def ImportPhysicalFermionSource(self, in4d, out5d): self._call('import_physical_fermion_source', in4d.h, out5d.h)
+    def ExportPhysicalFermionSolution(self, sol5d, out4d): self._call('export_physical_fermion_solution', sol5d.h, out4d.h)
     def Mpc(self, i, o): self._call('Mpc', i.h, o.h)
     def MpcDag(self, i, o): self._call('MpcDag', i.h, o.h)
     def MpcDagMpc(self, i, o): self._call('MpcDagMpc', i.h, o.h)
@@ -316,6 +321,14 @@ class FermionOp:
         if rc not in (0, 3):
             _check(rc)
         return st.as_dict()
+
+    def block_cg(self, src, psi, tol=1e-10, maxit=10000, inner_prec=F64):
+        """Multi-RHS CG (right-hand side k = slice s = k); returns one stats dict per RHS."""
+        st = (SolverStats * self.Ls)()
+        rc = lib().gx_block_cg(self.h, src.h, psi.h, tol, maxit, inner_prec, st)
+        if rc not in (0, 3):
+            _check(rc)
+        return [s.as_dict() for s in st]
 
     def schur_solve(self, src, sol, tol=1e-10, maxit=10000, mixed=False):
         st = SolverStats()

@@ -192,7 +192,7 @@ int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm) {
   return reduce_read(x->ctx, 0, 1, nrm);
 }
 
-int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const CGState *gate) {
+int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const Gate *gate) {
   const int64_t n = x->elems();
   z->cb = x->cb;
   if (x->prec == GX_F64) return launch_axpby<double>(n, a, x->data, b, y ? y->data : nullptr, z->data, gate, x->ctx->stream);
@@ -480,7 +480,7 @@ int op_smat(Op *op, int prec, int which, const void **mats) {
 }
 
 // ---------------- operator building blocks (half fields) ----------------
-int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const CGState *gate) {
+int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const Gate *gate) {
   void *Ue, *Uo;
   if (in->prec == GX_F64) {
     Ue = op->U[GX_F64][0];
@@ -495,7 +495,7 @@ int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const CG
   return launch_dhop<float>(op->g, in->data, out->data, U, cb_out, dag, gate, op->ctx->stream);
 }
 
-static int apply_smat(Op *op, Field *in, Field *phi, Field *out, int which, const CGState *gate) {
+static int apply_smat(Op *op, Field *in, Field *phi, Field *out, int which, const Gate *gate) {
   const void *m;
   GX_TRY(op_smat(op, in->prec, which, &m));
   out->cb = in->cb;
@@ -504,7 +504,7 @@ static int apply_smat(Op *op, Field *in, Field *phi, Field *out, int which, cons
   return launch_smat<float>(op->g, in->data, phi ? phi->data : nullptr, out->data, m, gate, op->ctx->stream);
 }
 
-int apply_meooe(Op *op, Field *in, Field *out, int dag, const CGState *gate) {
+int apply_meooe(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
   const int cb_out = 1 - in->cb;
   if (op->kind == GX_WILSON) return apply_dhop_half(op, in, out, cb_out, dag, gate);
   Field *t = op_tmp(op, in->prec, 3);
@@ -518,7 +518,7 @@ int apply_meooe(Op *op, Field *in, Field *out, int dag, const CGState *gate) {
 }
 
 // which: 0 Mooee, 1 MooeeDag, 2 MooeeInv, 3 MooeeInvDag
-int apply_mooee(Op *op, Field *in, Field *out, int which, const CGState *gate) {
+int apply_mooee(Op *op, Field *in, Field *out, int which, const Gate *gate) {
   if (op->kind == GX_WILSON) {
     const double d = 4.0 + op->mass;
     return blas_axpby(out, which < 2 ? d : 1.0 / d, in, 0.0, nullptr, gate);
@@ -528,7 +528,7 @@ int apply_mooee(Op *op, Field *in, Field *out, int which, const CGState *gate) {
 }
 
 // SchurDiagMooeeOperator::Mpc / MpcDag, LinearOperator.h:330-348
-int apply_mpc(Op *op, Field *in, Field *out, int dag, const CGState *gate) {
+int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
   Field *t1 = op_tmp(op, in->prec, 0), *t2 = op_tmp(op, in->prec, 1);
   if (!t1 || !t2) return GX_ENOMEM;
   GX_TRY(apply_meooe(op, in, t1, dag, gate));
@@ -540,7 +540,7 @@ int apply_mpc(Op *op, Field *in, Field *out, int dag, const CGState *gate) {
   return GX_OK;
 }
 
-int apply_mpcdagmpc(Op *op, Field *in, Field *out, const CGState *gate) {
+int apply_mpcdagmpc(Op *op, Field *in, Field *out, const Gate *gate) {
   Field *t = op_tmp(op, in->prec, 2);
   if (!t) return GX_ENOMEM;
   GX_TRY(apply_mpc(op, in, t, 0, gate));
@@ -675,7 +675,7 @@ int gx_ctx_create(int device, gx_ctx *out) {
   c->device = device;
   GX_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (int i = 0; i < 16; i++) GX_CHECK_HIP(hipEventCreate(&c->ev[i]));
-  GX_CHECK_HIP(hipMalloc(&c->red_partial, sizeof(double) * kRedBlocks * 4));
+  GX_CHECK_HIP(hipMalloc(&c->red_partial, sizeof(double) * kRedBlocks * kMaxLs));
   GX_CHECK_HIP(hipMalloc(&c->red_ticket, sizeof(unsigned int) * 16));
   GX_CHECK_HIP(hipMemset(c->red_ticket, 0, sizeof(unsigned int) * 16));
   GX_CHECK_HIP(hipMalloc(&c->red_result, sizeof(double) * kRedSlots));
@@ -999,6 +999,62 @@ int gx_M5D(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, ch
 int gx_M5Ddag(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, chi, Op::S_M5DDAG_OFF, true); }
 int gx_Meooe5D(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, chi, Op::S_MEOOE5D, false); }
 int gx_MeooeDag5D(gx_op op, gx_field psi, gx_field chi) { return m5d_common(op, psi, chi, Op::S_MEOOEDAG5D, false); }
+
+// 4D physical source -> 5D source: tmp[0] = P+ in, tmp[Ls-1] = P- in, then Dminus = 1 - c DW,
+// DW = Dhop + (4 - M5).  CayleyFermion5DImplementation.h:101-129 (ImportPhysicalFermionSource, Dminus).
+int gx_import_physical_fermion_source(gx_op op_, gx_field in4d, gx_field out5d) {
+  Op *op = op_;
+  Field *in = in4d, *out = out5d;
+  GX_REQUIRE(op && in && out && op->kind == GX_MOBIUS, GX_EINVAL, "import: needs a Mobius operator");
+  GX_REQUIRE(in->subset == GX_FULL && out->subset == GX_FULL && in->g.Ls == 1 && same_geom(out->g, op->g) &&
+                 in->prec == out->prec && (in->prec == GX_F64 || in->prec == GX_F32),
+             GX_EINVAL, "import: needs a full 4D field (Ls=1) and a full 5D field of the operator");
+  for (int d = 0; d < 4; d++) GX_REQUIRE(in->g.L[d] == op->g.L[d], GX_EINVAL, "import: 4D dimensions differ");
+  const int Ls = op->g.Ls;
+  hipStream_t st = op->ctx->stream;
+  GX_CHECK_HIP(hipMemsetAsync(out->data, 0, out->bytes, st));
+  if (in->prec == GX_F64) {
+    GX_TRY(launch_slice_proj<double>(op->g, in->data, 1, 0, out->data, Ls, 0, 0x3, 0, st));
+    GX_TRY(launch_slice_proj<double>(op->g, in->data, 1, 0, out->data, Ls, Ls - 1, 0xC, 0, st));
+  } else {
+    GX_TRY(launch_slice_proj<float>(op->g, in->data, 1, 0, out->data, Ls, 0, 0x3, 0, st));
+    GX_TRY(launch_slice_proj<float>(op->g, in->data, 1, 0, out->data, Ls, Ls - 1, 0xC, 0, st));
+  }
+  Field *t;
+  GX_TRY(field_create(op->ctx, op->g, in->prec, GX_FULL, &t));
+  int rc = GX_OK;
+  for (int cb = 0; cb < 2 && rc == GX_OK; cb++) {
+    Field i = half_view(out, 1 - cb), o = half_view(t, cb);
+    rc = apply_dhop_half(op, &i, &o, cb, 0);
+  }
+  const double c = op->cs[0];
+  if (rc == GX_OK) rc = blas_axpby(out, 1.0 - c * (4.0 - op->M5), out, -c, t);
+  hipStreamSynchronize(st);
+  field_destroy(t);
+  return rc;
+}
+
+// 5D solution -> 4D physical propagator: P- sol[0] + P+ sol[Ls-1].
+// CayleyFermion5DImplementation.h:57-67 (ExportPhysicalFermionSolution).
+int gx_export_physical_fermion_solution(gx_op op_, gx_field sol5d, gx_field out4d) {
+  Op *op = op_;
+  Field *sol = sol5d, *out = out4d;
+  GX_REQUIRE(op && sol && out && op->kind == GX_MOBIUS, GX_EINVAL, "export: needs a Mobius operator");
+  GX_REQUIRE(sol->subset == GX_FULL && out->subset == GX_FULL && out->g.Ls == 1 && same_geom(sol->g, op->g) &&
+                 sol->prec == out->prec && (sol->prec == GX_F64 || sol->prec == GX_F32),
+             GX_EINVAL, "export: needs a full 5D field of the operator and a full 4D field (Ls=1)");
+  for (int d = 0; d < 4; d++) GX_REQUIRE(out->g.L[d] == op->g.L[d], GX_EINVAL, "export: 4D dimensions differ");
+  const int Ls = op->g.Ls;
+  hipStream_t st = op->ctx->stream;
+  if (sol->prec == GX_F64) {
+    GX_TRY(launch_slice_proj<double>(op->g, sol->data, Ls, 0, out->data, 1, 0, 0xC, 0, st));
+    GX_TRY(launch_slice_proj<double>(op->g, sol->data, Ls, Ls - 1, out->data, 1, 0, 0x3, 1, st));
+  } else {
+    GX_TRY(launch_slice_proj<float>(op->g, sol->data, Ls, 0, out->data, 1, 0, 0xC, 0, st));
+    GX_TRY(launch_slice_proj<float>(op->g, sol->data, Ls, Ls - 1, out->data, 1, 0, 0x3, 1, st));
+  }
+  return GX_OK;
+}
 
 int gx_Mpc(gx_op op, gx_field in, gx_field out) {
   GX_TRY(check_io(O(op), F(in), F(out), false));

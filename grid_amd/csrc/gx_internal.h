@@ -52,31 +52,33 @@ struct Geom {
   int64_t half_elems() const { return H4 * Ls; }   // complex numbers per component per checkerboard
 };
 
-// Device-resident solver scalars (one per running CG).  Kernels of iteration k run only while
-// k <= k_conv, so a host can enqueue iterations ahead and read the outcome later.
-struct CGState {
-  double c, cp, a, b, d, rsq;
+// Iteration gate shared by the device-resident solvers: a kernel of iteration k runs only while
+// k <= min(k_conv, kmax), so a host can enqueue iterations ahead and read the outcome later.
+struct Gate {
   int k;        // current iteration (1-based)
-  int k_conv;   // iteration at which cp <= rsq was first seen (INT_MAX until then)
+  int k_conv;   // iteration at which the stopping test first passed (INT_MAX until then)
   int kmax;     // last iteration allowed (MaxIterations)
   int pad;
 };
 
-// Per-RHS solver scalars for the multi-RHS CG (right-hand side = slice s).
-struct MrhsState {
-  double c[kMaxLs], cp[kMaxLs], a[kMaxLs], b[kMaxLs], d[kMaxLs], rsq[kMaxLs];
-  int k;
-  int k_conv[kMaxLs];
-  int active;   // number of right-hand sides still iterating
-  int nrhs;
-  int k_all;    // iteration at which the last RHS converged (INT_MAX until then)
+// Device-resident scalars of one CG (ConjugateGradient.h:68-257).
+struct CGState : Gate {
+  double c, cp, a, b, d, rsq;
+};
+
+// Per-RHS solver scalars for the multi-RHS CG (right-hand side = slice s of the fifth dimension).
+// As in Grid's CGmultiRHS every right-hand side iterates until the largest relative residual
+// max_s cp_s / ssq_s drops below tol^2; k and k_conv gate the kernels as in CGState.
+struct MrhsState : Gate {
+  double c[kMaxLs], cp[kMaxLs], a[kMaxLs], b[kMaxLs], d[kMaxLs], ssq[kMaxLs];
+  double tol2;
 };
 
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[16] = {};
-  double *red_partial = nullptr;     // [kRedBlocks * 4]
+  double *red_partial = nullptr;     // [kRedBlocks * kMaxLs]
   unsigned int *red_ticket = nullptr;
   double *red_result = nullptr;      // [kRedSlots]
   double *pinned = nullptr;          // host mirror of red_result
@@ -135,15 +137,15 @@ struct Op {
 // ---------------- launchers (gx_kernels.hip) ----------------
 // All pointers are device pointers to half fields in the layout above; T = double or float.
 template <typename T>
-int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const CGState *gate,
+int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st);
 // out = A_chi(in) per chirality (+ dphi[s] * phi if phi), mats = 2*Ls*Ls device matrix in T
 template <typename T>
-int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const CGState *gate,
+int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
                 hipStream_t st);
 // z = a x + b y over n complex numbers (a, b real)
 template <typename T>
-int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const CGState *gate,
+int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const Gate *gate,
                  hipStream_t st);
 // reductions into ctx slot(s); mode 0: <x|y> -> slot, slot+1 (re, im); 1: |x|^2 -> slot;
 // 2: z = a x + y and |z|^2 -> slot (fused axpy_norm)
@@ -152,6 +154,10 @@ int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void
                   int slot);
 template <typename Tin, typename Tout>
 int launch_convert(int64_t n, const void *in, void *out, hipStream_t st);
+// z[sz] (=|+=) (spins in mask) x[sx] over both checkerboards of full fields (rows Lsx / Lsz)
+template <typename T>
+int launch_slice_proj(const Geom &g, const void *x, int Lsx, int sx, void *z, int Lsz, int sz, int mask,
+                      int accumulate, hipStream_t st);
 
 // CG iteration kernels on device scalars (gx_kernels.hip)
 template <typename T>
@@ -161,15 +167,14 @@ int launch_cg_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const
 template <typename T>
 int launch_cg_update(Ctx *ctx, int64_t n, void *psi, void *p, const void *r, CGState *state);
 
-// multi-RHS CG kernels (slice s = right-hand side)
+// Multi-RHS (slice) kernels; a half field is viewed as rows x Ls with s innermost.
+// mode 0: d_s = Re<x_s|y_s> -> a_s = c_s / d_s (state);   mode 2: x_s -= a_s y_s, cp_s = |x_s|^2 ->
+// b_s, convergence, k += 1 (state);   mode 1: out_s = |x_s|^2;   mode 3: z_s = y_s - x_s, out_s = |z_s|^2
 template <typename T>
-int launch_mrhs_dot(Ctx *ctx, const Geom &g, const void *p, const void *mmp, MrhsState *state);
-template <typename T>
-int launch_mrhs_axpy_norm(Ctx *ctx, const Geom &g, void *r, const void *mmp, MrhsState *state);
+int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y, void *z, MrhsState *state,
+                       double *out_dev);
 template <typename T>
 int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state);
-template <typename T>
-int launch_slice_norm(Ctx *ctx, const Geom &g, const void *x, double *out_dev);
 
 // ---------------- api helpers (gx_api.hip) ----------------
 int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out);
@@ -180,13 +185,13 @@ int op_smat(Op *op, int prec, int which, const void **mats);
 int reduce_read(Ctx *ctx, int slot, int n, double *out);
 
 // Operator building blocks on half fields (in/out precision must match)
-int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const CGState *gate = nullptr);
-int apply_meooe(Op *op, Field *in, Field *out, int dag, const CGState *gate = nullptr);
-int apply_mooee(Op *op, Field *in, Field *out, int which, const CGState *gate = nullptr);
-int apply_mpc(Op *op, Field *in, Field *out, int dag, const CGState *gate = nullptr);
-int apply_mpcdagmpc(Op *op, Field *in, Field *out, const CGState *gate = nullptr);
+int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const Gate *gate = nullptr);
+int apply_meooe(Op *op, Field *in, Field *out, int dag, const Gate *gate = nullptr);
+int apply_mooee(Op *op, Field *in, Field *out, int which, const Gate *gate = nullptr);
+int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate = nullptr);
+int apply_mpcdagmpc(Op *op, Field *in, Field *out, const Gate *gate = nullptr);
 int apply_full_m(Op *op, Field *in, Field *out, int dag);
-int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const CGState *gate = nullptr);
+int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const Gate *gate = nullptr);
 int blas_norm2(Field *x, double *nrm);
 int blas_inner(Field *x, Field *y, double *re, double *im);
 int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm);

@@ -53,7 +53,7 @@ static KGeom kgeom(const Geom &g) {
   return k;
 }
 
-__device__ __forceinline__ bool gated(const CGState *gate) {
+__device__ __forceinline__ bool gated(const Gate *gate) {
   return gate && (gate->k > gate->k_conv || gate->k > gate->kmax);
 }
 
@@ -202,17 +202,16 @@ __device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, int
   return nb[LegSel<DAG, ORD, LEG>::P] * Ls + s;
 }
 
+// The 8 legs of one output site.  The caller has issued the neighbour loads of leg 0 (into fa)
+// and, with PF, of leg 1 (into fb); with PF the loads of leg L+1 are in flight while leg L is
+// computed (sched_barrier keeps the compiler from sinking them).
 template <typename T, int LS, bool DAG, bool PF, int ORD, bool LDSU>
-__device__ __forceinline__ void hop_site(cx<T> *acc, const cx<T> *__restrict__ in, const cx<T> *__restrict__ U,
-                                         const cx<T> *ul, const uint32_t *nb, int s, uint32_t h, uint32_t H4,
-                                         uint32_t stride) {
+__device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const cx<T> *__restrict__ in,
+                                         const cx<T> *__restrict__ U, const cx<T> *ul, const uint32_t *nb, int s,
+                                         uint32_t h, uint32_t H4, uint32_t stride) {
 #define IDX(L) leg_idx<T, DAG, ORD, L>(nb, LS, s)
 #define LEG(L, F) do_leg<T, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4)
   if (PF) {
-    // two register sets: the loads of leg L+1 are in flight while leg L is computed
-    cx<T> fa[12], fb[12];
-    load_spinor<T>(fa, in, IDX(0), stride);
-    load_spinor<T>(fb, in, IDX(1), stride);
     __builtin_amdgcn_sched_barrier(0);
     LEG(0, fa);
     load_spinor<T>(fa, in, IDX(2), stride);
@@ -235,23 +234,21 @@ __device__ __forceinline__ void hop_site(cx<T> *acc, const cx<T> *__restrict__ i
     LEG(6, fa);
     LEG(7, fb);
   } else {
-    cx<T> f[12];
-    load_spinor<T>(f, in, IDX(0), stride);
-    LEG(0, f);
-    load_spinor<T>(f, in, IDX(1), stride);
-    LEG(1, f);
-    load_spinor<T>(f, in, IDX(2), stride);
-    LEG(2, f);
-    load_spinor<T>(f, in, IDX(3), stride);
-    LEG(3, f);
-    load_spinor<T>(f, in, IDX(4), stride);
-    LEG(4, f);
-    load_spinor<T>(f, in, IDX(5), stride);
-    LEG(5, f);
-    load_spinor<T>(f, in, IDX(6), stride);
-    LEG(6, f);
-    load_spinor<T>(f, in, IDX(7), stride);
-    LEG(7, f);
+    LEG(0, fa);
+    load_spinor<T>(fa, in, IDX(1), stride);
+    LEG(1, fa);
+    load_spinor<T>(fa, in, IDX(2), stride);
+    LEG(2, fa);
+    load_spinor<T>(fa, in, IDX(3), stride);
+    LEG(3, fa);
+    load_spinor<T>(fa, in, IDX(4), stride);
+    LEG(4, fa);
+    load_spinor<T>(fa, in, IDX(5), stride);
+    LEG(5, fa);
+    load_spinor<T>(fa, in, IDX(6), stride);
+    LEG(6, fa);
+    load_spinor<T>(fa, in, IDX(7), stride);
+    LEG(7, fa);
   }
 #undef IDX
 #undef LEG
@@ -263,7 +260,7 @@ __device__ __forceinline__ void hop_site(cx<T> *acc, const cx<T> *__restrict__ i
 template <typename T, int LS, bool DAG, bool PF, int ORD>
 __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                               const cx<T> *__restrict__ U, KGeom g, int cb_out,
-                                              const CGState *__restrict__ gate, uint32_t ntiles, int map, int flags) {
+                                              const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags) {
   if (gated(gate)) return;
   constexpr int SPW = 64 / LS;
   constexpr int SPB = 4 * SPW;
@@ -273,20 +270,13 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
   if (tile >= ntiles) return;  // uniform across the workgroup
   const uint32_t H4 = (uint32_t)g.H4, stride = (uint32_t)g.stride;
   const uint32_t h0 = tile * SPB;
-  if (LDSU) {
-    for (uint32_t e = threadIdx.x; e < SPB * 72; e += 256) {
-      const uint32_t pk = e / SPB, site = e - pk * SPB;
-      if (h0 + site < H4) ulds[site * kLinkPad + pk] = U[pk * H4 + h0 + site];
-    }
-    __syncthreads();
-  }
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ls = lane / LS;
   const int s = lane - ls * LS;
   const int lsb = w * SPW + ls;
-  const uint32_t h = h0 + lsb;
-  if (ls >= SPW || h >= H4) return;
+  const bool valid = (ls < SPW) && (h0 + lsb < H4);
+  const uint32_t h = valid ? h0 + lsb : h0;  // idle lanes compute in-bounds addresses and exit later
   uint32_t r = h;
   const uint32_t Lxh = (uint32_t)g.Lxh;
   const int xh = (int)(r % Lxh);
@@ -309,11 +299,37 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
   nb[6] = (z == 0) ? h + (uint32_t)(g.L[2] - 1) * sZ : h - sZ;
   nb[3] = (t + 1 == g.L[3]) ? h - (uint32_t)(g.L[3] - 1) * sT : h + sT;
   nb[7] = (t == 0) ? h + (uint32_t)(g.L[3] - 1) * sT : h - sT;
+  // Link tile of the workgroup: global -> registers first, then the first neighbour loads, then
+  // registers -> LDS and a barrier that waits for LDS traffic only, so the neighbour loads stay in
+  // flight across it.
+  constexpr int NU = LDSU ? (SPB * 72 + 255) / 256 : 1;
+  cx<T> ureg[NU];
+  if (LDSU) {
+#pragma unroll
+    for (int j = 0; j < NU; j++) {
+      const uint32_t e = threadIdx.x + 256 * j;
+      const uint32_t pk = e / SPB, site = e - pk * SPB;
+      ureg[j] = (e < SPB * 72 && h0 + site < H4) ? U[pk * H4 + h0 + site] : cx<T>{0, 0};
+    }
+  }
+  cx<T> fa[12], fb[12];
+  load_spinor<T>(fa, in, leg_idx<T, DAG, ORD, 0>(nb, LS, s), stride);
+  if (PF) load_spinor<T>(fb, in, leg_idx<T, DAG, ORD, 1>(nb, LS, s), stride);
+  if (LDSU) {
+#pragma unroll
+    for (int j = 0; j < NU; j++) {
+      const uint32_t e = threadIdx.x + 256 * j;
+      const uint32_t pk = e / SPB, site = e - pk * SPB;
+      if (e < SPB * 72) ulds[site * kLinkPad + pk] = ureg[j];
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  }
+  if (!valid) return;
   const cx<T> *ul = ulds + (LDSU ? lsb * kLinkPad : 0);
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  hop_site<T, LS, DAG, PF, ORD, LDSU>(acc, in, U, ul, nb, s, h, H4, stride);
+  hop_site<T, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4, stride);
   store_spinor<T>(out, acc, h * LS + s, stride, (flags & 1) != 0);
 }
 
@@ -321,7 +337,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
   M(1) M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12) M(13) M(14) M(15) M(16)
 
 template <typename T>
-int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const CGState *gate,
+int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st) {
   const KGeom k = kgeom(g);
   if (12 * 2 * k.stride >= (int64_t)1 << 31) {
@@ -376,7 +392,7 @@ int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_
 template <typename T, int LS>
 __global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, const cx<T> *__restrict__ phi,
                                               cx<T> *__restrict__ out, const T *__restrict__ mats, int64_t H4,
-                                              int64_t stride, const CGState *__restrict__ gate) {
+                                              int64_t stride, const Gate *__restrict__ gate) {
   if (gated(gate)) return;
   constexpr int SPW = 64 / LS;
   __shared__ cx<T> lds[4][12][64];
@@ -418,7 +434,7 @@ __global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, cons
 }
 
 template <typename T>
-int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const CGState *gate,
+int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
                 hipStream_t st) {
   const int spw = 64 / g.Ls;
   const int64_t waves = (g.H4 + spw - 1) / spw;
@@ -447,7 +463,7 @@ int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const
 template <typename T>
 __global__ void __launch_bounds__(256) k_axpby(int64_t n, T a, const cx<T> *__restrict__ x, T b,
                                                const cx<T> *__restrict__ y, cx<T> *__restrict__ z,
-                                               const CGState *__restrict__ gate) {
+                                               const Gate *__restrict__ gate) {
   if (gated(gate)) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const cx<T> xv = x[i];
@@ -466,7 +482,7 @@ static int64_t stream_blocks(int64_t n) {
 }
 
 template <typename T>
-int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const CGState *gate,
+int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const Gate *gate,
                  hipStream_t st) {
   hipLaunchKernelGGL((k_axpby<T>), dim3(stream_blocks(n)), dim3(256), 0, st, n, (T)a, (const cx<T> *)x, (T)b,
                      (const cx<T> *)y, (cx<T> *)z, gate);
@@ -729,12 +745,186 @@ int launch_cg_update(Ctx *ctx, int64_t n, void *psi, void *p, const void *r, CGS
   return GX_OK;
 }
 
+// ---------------- multi-RHS slice reductions (BlockConjugateGradient.h:412-545, CGmultiRHS) ----------
+// A half field is rows x Ls complex numbers with s innermost (row = k*H4 + h).  Each block uses
+// floor(256/Ls)*Ls threads: thread t works on slice s = t % Ls and strides over rows, so the
+// per-slice partial sums come out of a fixed-order LDS reduction; a one-block kernel adds the
+// block partials in block order.
+__device__ __forceinline__ bool mgated(const MrhsState *st) { return gated(st); }
+__device__ __forceinline__ bool mgated_prev(const MrhsState *st) {
+  return st->k - 1 > st->k_conv || st->k - 1 > st->kmax;
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t rows, int Ls, cx<T> *__restrict__ x,
+                                                     const cx<T> *__restrict__ y, cx<T> *__restrict__ z,
+                                                     const MrhsState *st, double *partial) {
+  if ((MODE == 0 || MODE == 2) && mgated(st)) return;
+  __shared__ double sh[256];
+  const int rpb = 256 / Ls;
+  const int s = threadIdx.x % Ls;
+  const int rt = threadIdx.x / Ls;
+  double acc = 0;
+  if (rt < rpb) {
+    const T a = (MODE == 2) ? (T)(-st->a[s]) : (T)0;
+    for (int64_t row = (int64_t)blockIdx.x * rpb + rt; row < rows; row += (int64_t)gridDim.x * rpb) {
+      const int64_t i = row * Ls + s;
+      const cx<T> xv = x[i];
+      if (MODE == 0) {
+        const cx<T> yv = y[i];
+        acc += (double)fma(xv.r, yv.r, xv.i * yv.i);
+      } else if (MODE == 1) {
+        acc += (double)fma(xv.r, xv.r, xv.i * xv.i);
+      } else if (MODE == 2) {
+        const cx<T> yv = y[i];
+        const cx<T> v = {fma(a, yv.r, xv.r), fma(a, yv.i, xv.i)};
+        x[i] = v;
+        acc += (double)fma(v.r, v.r, v.i * v.i);
+      } else {
+        const cx<T> yv = y[i];
+        const cx<T> v = {yv.r - xv.r, yv.i - xv.i};
+        z[i] = v;
+        acc += (double)fma(v.r, v.r, v.i * v.i);
+      }
+    }
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < Ls) {
+    double v = 0;
+    for (int r = 0; r < rpb; r++) v += sh[r * Ls + threadIdx.x];
+    partial[blockIdx.x * Ls + threadIdx.x] = v;
+  }
+}
+
+// which: 0 -> d_s, a_s;  2 -> cp_s, b_s, convergence, k += 1;  1, 3 -> out[s] = sum
+__global__ void __launch_bounds__(64) k_mrhs_fin(const double *partial, int nparts, int Ls, MrhsState *st,
+                                                 double *out, int which) {
+  const bool skip = (which == 0 || which == 2) && mgated(st);
+  if (skip && which == 0) return;
+  const int s = threadIdx.x;
+  double v = 0;
+  if (s < Ls && !skip)
+    for (int b = 0; b < nparts; b++) v += partial[b * Ls + s];
+  if (which == 1 || which == 3) {
+    if (s < Ls) out[s] = v;
+    return;
+  }
+  __shared__ double rel[64];
+  if (which == 0) {
+    if (s < Ls) {
+      st->d[s] = v;
+      st->a[s] = (v != 0.0) ? st->c[s] / v : 0.0;
+    }
+    return;
+  }
+  if (!skip && s < Ls) {
+    const double c = st->c[s];
+    st->b[s] = (c != 0.0) ? v / c : 0.0;
+    st->cp[s] = v;
+    st->c[s] = v;
+    rel[s] = v / st->ssq[s];
+  }
+  __syncthreads();
+  if (s == 0) {
+    if (!skip) {
+      double mx = 0;
+      for (int j = 0; j < Ls; j++) mx = rel[j] > mx ? rel[j] : mx;
+      if (mx < st->tol2 && st->k_conv == INT_MAX) st->k_conv = st->k;
+    }
+    st->k = st->k + 1;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mrhs_update(int64_t n, int Ls, cx<T> *__restrict__ psi, cx<T> *__restrict__ p,
+                                                     const cx<T> *__restrict__ r, const MrhsState *st) {
+  if (mgated_prev(st)) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(i % Ls);
+    const T a = (T)st->a[s], b = (T)st->b[s];
+    const cx<T> pv = p[i], xv = psi[i], rv = r[i];
+    psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
+    p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
+  }
+}
+
+template <typename T>
+int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y, void *z, MrhsState *state,
+                       double *out_dev) {
+  const int64_t rows = 12 * g.H4;
+  const int rpb = 256 / g.Ls;
+  int64_t nb = (rows + rpb - 1) / rpb;
+  if (nb > kRedBlocks) nb = kRedBlocks;
+  cx<T> *px = (cx<T> *)x;
+  const cx<T> *py = (const cx<T> *)y;
+  cx<T> *pz = (cx<T> *)z;
+#define GX_MR(M)                                                                                                  \
+  hipLaunchKernelGGL((k_mrhs_reduce<T, M>), dim3(nb), dim3(256), 0, ctx->stream, rows, g.Ls, px, py, pz,           \
+                     (const MrhsState *)state, ctx->red_partial)
+  switch (mode) {
+    case 0: GX_MR(0); break;
+    case 1: GX_MR(1); break;
+    case 2: GX_MR(2); break;
+    case 3: GX_MR(3); break;
+    default: set_error("mrhs reduce mode %d", mode); return GX_EINVAL;
+  }
+#undef GX_MR
+  hipLaunchKernelGGL(k_mrhs_fin, dim3(1), dim3(64), 0, ctx->stream, (const double *)ctx->red_partial, (int)nb, g.Ls,
+                     state, out_dev, mode);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+template <typename T>
+int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state) {
+  const int64_t n = 12 * g.H4 * g.Ls;
+  hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, g.Ls, (cx<T> *)psi,
+                     (cx<T> *)p, (const cx<T> *)r, (const MrhsState *)state);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+// ---------------- fifth-dimension slice transfer (LinalgUtils.h:140-183, axpby_ssp_pplus/pminus) ------
+// z[s = sz] (=|+=) (spins in mask) x[s = sx] over both checkerboards of full fields; x has row length
+// Lsx, z has Lsz.  Used for the physical 4D <-> 5D source / solution maps.
+template <typename T>
+__global__ void __launch_bounds__(256) k_slice_proj(const cx<T> *__restrict__ x, int Lsx, int sx, cx<T> *__restrict__ z,
+                                                    int Lsz, int sz, int mask, int accumulate, int64_t H4) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < 2 * H4; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cb = j / H4, h = j - cb * H4;
+    const int64_t bx = cb * 12 * H4 * Lsx + h * Lsx + sx, bz = cb * 12 * H4 * Lsz + h * Lsz + sz;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      const bool on = (mask >> (k / 3)) & 1;
+      cx<T> v = on ? x[bx + k * H4 * Lsx] : cx<T>{0, 0};
+      if (accumulate) {
+        const cx<T> o = z[bz + k * H4 * Lsz];
+        v = {o.r + v.r, o.i + v.i};
+      }
+      z[bz + k * H4 * Lsz] = v;
+    }
+  }
+}
+
+template <typename T>
+int launch_slice_proj(const Geom &gx_, const void *x, int Lsx, int sx, void *z, int Lsz, int sz, int mask,
+                      int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL((k_slice_proj<T>), dim3(stream_blocks(2 * gx_.H4)), dim3(256), 0, st, (const cx<T> *)x, Lsx, sx,
+                     (cx<T> *)z, Lsz, sz, mask, accumulate, gx_.H4);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
 #define GX_INST(T)                                                                                                 \
-  template int launch_dhop<T>(const Geom &, const void *, void *, const void *, int, int, const CGState *,        \
+  template int launch_slice_proj<T>(const Geom &, const void *, int, int, void *, int, int, int, int, hipStream_t); \
+  template int launch_mrhs_reduce<T>(Ctx *, int, const Geom &, void *, const void *, void *, MrhsState *, double *); \
+  template int launch_mrhs_update<T>(Ctx *, const Geom &, void *, void *, const void *, MrhsState *);             \
+  template int launch_dhop<T>(const Geom &, const void *, void *, const void *, int, int, const Gate *,           \
                               hipStream_t);                                                                        \
-  template int launch_smat<T>(const Geom &, const void *, const void *, void *, const void *, const CGState *,     \
+  template int launch_smat<T>(const Geom &, const void *, const void *, void *, const void *, const Gate *,        \
                               hipStream_t);                                                                        \
-  template int launch_axpby<T>(int64_t, double, const void *, double, const void *, void *, const CGState *,       \
+  template int launch_axpby<T>(int64_t, double, const void *, double, const void *, void *, const Gate *,          \
                                hipStream_t);                                                                       \
   template int launch_reduce<T>(Ctx *, int, int64_t, int64_t, const void *, const void *, void *, double, int);   \
   template int launch_cg_dot<T>(Ctx *, int64_t, int64_t, const void *, const void *, CGState *);                   \

@@ -201,6 +201,207 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
   return rc;
 }
 
+// ---------------- multi-RHS CG (BlockConjugateGradient.h:412-545, CGmultiRHS) ----------------
+struct MrhsWork {
+  MrhsState *dev = nullptr;
+  MrhsState *host = nullptr;  // pinned, 2 slots
+  double *out_dev = nullptr;  // per-slice results
+  double *out_host = nullptr; // pinned mirror
+  hipEvent_t ev[2] = {};
+};
+
+static MrhsWork &mrhs_work(Op *op) {
+  static thread_local std::vector<std::pair<Op *, MrhsWork>> cache;
+  for (auto &e : cache)
+    if (e.first == op) return e.second;
+  cache.emplace_back(op, MrhsWork());
+  MrhsWork &w = cache.back().second;
+  hipMalloc(&w.dev, sizeof(MrhsState));
+  hipHostMalloc(&w.host, 2 * sizeof(MrhsState), hipHostMallocDefault);
+  hipMalloc(&w.out_dev, sizeof(double) * kMaxLs);
+  hipHostMalloc(&w.out_host, sizeof(double) * kMaxLs, hipHostMallocDefault);
+  hipEventCreateWithFlags(&w.ev[0], hipEventDisableTiming);
+  hipEventCreateWithFlags(&w.ev[1], hipEventDisableTiming);
+  return w;
+}
+
+// per-slice reduction that returns host values: mode 1 |x_s|^2, mode 3 z = y - x and |z_s|^2
+static int slice_reduce(Op *op, int mode, Field *x, Field *y, Field *z, double *out) {
+  MrhsWork &w = mrhs_work(op);
+  Ctx *ctx = op->ctx;
+  const int rc = x->prec == GX_F64
+                     ? launch_mrhs_reduce<double>(ctx, mode, op->g, x->data, y ? y->data : nullptr,
+                                                  z ? z->data : nullptr, w.dev, w.out_dev)
+                     : launch_mrhs_reduce<float>(ctx, mode, op->g, x->data, y ? y->data : nullptr,
+                                                 z ? z->data : nullptr, w.dev, w.out_dev);
+  GX_TRY(rc);
+  GX_CHECK_HIP(hipMemcpyAsync(w.out_host, w.out_dev, sizeof(double) * op->g.Ls, hipMemcpyDeviceToHost, ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (int s = 0; s < op->g.Ls; s++) out[s] = w.out_host[s];
+  if (z) z->cb = x->cb;
+  return GX_OK;
+}
+
+template <typename T>
+static int mrhs_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, MrhsState *st) {
+  GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
+  GX_TRY(launch_mrhs_reduce<T>(op->ctx, 0, op->g, p->data, mmp->data, nullptr, st, nullptr));
+  GX_TRY(launch_mrhs_reduce<T>(op->ctx, 2, op->g, r->data, mmp->data, nullptr, st, nullptr));
+  GX_TRY(launch_mrhs_update<T>(op->ctx, op->g, psi->data, p->data, r->data, st));
+  return GX_OK;
+}
+
+// CGmultiRHS on HermOp = MpcDagMpc with right-hand side k in slice s = k.  st: Ls entries or null.
+int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *st) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int Ls = op->g.Ls, prec = src->prec;
+  Field *p = op_tmp(op, prec, 4), *mmp = op_tmp(op, prec, 5), *r = op_tmp(op, prec, 6);
+  if (!p || !mmp || !r) return GX_ENOMEM;
+  p->cb = mmp->cb = r->cb = psi->cb = src->cb;
+  std::vector<double> ssq(Ls), rr(Ls);
+  GX_TRY(slice_reduce(op, 1, src, nullptr, nullptr, ssq.data()));
+  for (int s = 0; s < Ls; s++)
+    if (std::isnan(ssq[s])) {
+      set_error("block CG: NaN in source slice %d", s);
+      return GX_ENAN;
+    }
+  // initial search direction from the guess: AP = A psi, R = src - AP, P = R
+  GX_TRY(apply_mpcdagmpc(op, psi, mmp));
+  GX_TRY(slice_reduce(op, 3, mmp, src, r, rr.data()));
+  for (int s = 0; s < Ls; s++)
+    if (std::isnan(rr[s])) {
+      set_error("block CG: NaN in guess slice %d", s);
+      return GX_ENAN;
+    }
+  Ctx *ctx = op->ctx;
+  GX_CHECK_HIP(hipMemcpyAsync(p->data, r->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  MrhsWork &w = mrhs_work(op);
+  MrhsState init{};
+  double mx = 0;
+  for (int s = 0; s < Ls; s++) {
+    init.c[s] = rr[s];
+    init.cp[s] = rr[s];
+    init.ssq[s] = ssq[s] > 0 ? ssq[s] : 1.0;
+    mx = std::max(mx, rr[s] / init.ssq[s]);
+  }
+  init.tol2 = tol * tol;
+  init.k = 1;
+  init.k_conv = INT_MAX;
+  init.kmax = maxit;
+  int iterations = 0;
+  bool converged = false;
+  MrhsState fin{};
+  if (mx < init.tol2) {
+    converged = true;
+    fin = init;
+  } else {
+    w.host[0] = init;
+    GX_CHECK_HIP(hipMemcpyAsync(w.dev, &w.host[0], sizeof(MrhsState), hipMemcpyHostToDevice, ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t vol = op->g.half_elems();
+    const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
+    int enqueued = 0, slot = 0;
+    auto enqueue = [&](int sl) -> int {
+      for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+        int rc = prec == GX_F64 ? mrhs_iteration<double>(op, psi, p, mmp, r, w.dev)
+                                : mrhs_iteration<float>(op, psi, p, mmp, r, w.dev);
+        GX_TRY(rc);
+      }
+      GX_CHECK_HIP(hipMemcpyAsync(&w.host[sl], w.dev, sizeof(MrhsState), hipMemcpyDeviceToHost, ctx->stream));
+      GX_CHECK_HIP(hipEventRecord(w.ev[sl], ctx->stream));
+      return GX_OK;
+    };
+    GX_TRY(enqueue(0));
+    GX_TRY(enqueue(1));
+    for (;;) {
+      GX_CHECK_HIP(hipEventSynchronize(w.ev[slot]));
+      fin = w.host[slot];
+      if (fin.k_conv != INT_MAX || fin.k > maxit) break;
+      if (enqueued >= maxit) {
+        GX_CHECK_HIP(hipEventSynchronize(w.ev[1 - slot]));
+        fin = w.host[1 - slot];
+        break;
+      }
+      GX_TRY(enqueue(slot));
+      slot = 1 - slot;
+    }
+    GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    converged = fin.k_conv != INT_MAX;
+    iterations = converged ? fin.k_conv : maxit;
+  }
+  // true residual per right-hand side
+  std::vector<double> res(Ls);
+  GX_TRY(apply_mpcdagmpc(op, psi, mmp));
+  GX_TRY(slice_reduce(op, 3, src, mmp, p, res.data()));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st)
+    for (int s = 0; s < Ls; s++) {
+      st[s] = gx_solver_stats{};
+      st[s].iterations = iterations;
+      st[s].converged = converged;
+      st[s].true_residual = std::sqrt(res[s] / (ssq[s] > 0 ? ssq[s] : 1.0));
+      st[s].computed_residual = std::sqrt(fin.cp[s] / (ssq[s] > 0 ? ssq[s] : 1.0));
+      st[s].seconds = secs;
+    }
+  if (!converged) {
+    set_error("block CG did not converge in %d iterations", maxit);
+    return GX_ENOCONV;
+  }
+  return GX_OK;
+}
+
+// Batched defect correction (ConjugateGradientMixedPrecBatched.h:100-213) with the inner solves of
+// all right-hand sides done together by one FP32 multi-RHS CG, so they share every link load.
+int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int maxinner, int maxouter,
+                   gx_solver_stats *st) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int Ls = op->g.Ls, cb = src_in->cb;
+  sol->cb = cb;
+  std::vector<double> src_norm(Ls), stop(Ls), norm(Ls);
+  GX_TRY(slice_reduce(op, 1, src_in, nullptr, nullptr, src_norm.data()));
+  for (int s = 0; s < Ls; s++) stop[s] = src_norm[s] * tol * tol;
+  Field *tmp_d = op_tmp(op, GX_F64, 7), *src_d = op_tmp(op, GX_F64, 8);
+  Field *src_f = op_tmp(op, GX_F32, 7), *sol_f = op_tmp(op, GX_F32, 8);
+  if (!tmp_d || !src_d || !src_f || !sol_f) return GX_ENOMEM;
+  tmp_d->cb = src_d->cb = src_f->cb = sol_f->cb = cb;
+  double inner_tol = inner_tol0;
+  int outer = 0, inner_total = 0;
+  for (outer = 0; outer < maxouter; outer++) {
+    GX_TRY(apply_mpcdagmpc(op, sol, tmp_d));
+    GX_TRY(slice_reduce(op, 3, tmp_d, src_in, src_d, norm.data()));   // src_d = src - A sol
+    bool all = true;
+    double nmax = 0, smax = 0;
+    for (int s = 0; s < Ls; s++) {
+      if (norm[s] > 100.0 * stop[s]) all = false;
+      nmax = std::max(nmax, norm[s]);
+      smax = std::max(smax, stop[s]);
+    }
+    if (all) break;
+    while (nmax * inner_tol * inner_tol < smax) inner_tol *= 2;
+    GX_TRY(convert_field(src_f, src_d));
+    GX_CHECK_HIP(hipMemsetAsync(sol_f->data, 0, sol_f->bytes, op->ctx->stream));
+    std::vector<gx_solver_stats> ist(Ls);
+    int rc = mrhs_cg_run(op, src_f, sol_f, inner_tol, maxinner, ist.data());
+    if (rc != GX_OK && rc != GX_ENOCONV) return rc;
+    inner_total += ist[0].iterations;
+    GX_TRY(convert_field(tmp_d, sol_f));
+    GX_TRY(blas_axpby(sol, 1.0, sol, 1.0, tmp_d));
+  }
+  std::vector<gx_solver_stats> fst(Ls);
+  const int rc = mrhs_cg_run(op, src_in, sol, tol, maxinner, fst.data());
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st)
+    for (int s = 0; s < Ls; s++) {
+      st[s] = fst[s];
+      st[s].inner_iterations = inner_total;
+      st[s].outer_iterations = outer;
+      st[s].final_iterations = fst[s].iterations;
+      st[s].iterations = inner_total + fst[s].iterations;
+      st[s].seconds = secs;
+    }
+  return rc;
+}
+
 }  // namespace gx
 
 using namespace gx;
@@ -235,6 +436,21 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
     return GX_EINVAL;
   }
   return mixed_cg_run(op, src, psi, tol, inner_tol, inner_prec, maxinner, maxouter, st);
+}
+
+int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, int inner_prec,
+                gx_solver_stats *st) {
+  Op *op = op_;
+  Field *src = src_, *psi = psi_;
+  if (!op || !src || !psi || src == psi || op->kind != GX_WILSON || src->subset == GX_FULL ||
+      psi->subset == GX_FULL || src->prec != GX_F64 || psi->prec != GX_F64) {
+    set_error("block_cg: needs a multi-RHS Wilson operator and F64 half fields");
+    return GX_EINVAL;
+  }
+  if (inner_prec == GX_F64) return mrhs_cg_run(op, src, psi, tol, maxit, st);
+  if (inner_prec == GX_F32) return mrhs_mixed_run(op, src, psi, tol, 1e-5, maxit, 100, st);
+  set_error("block_cg: inner precision %d unsupported", inner_prec);
+  return GX_EINVAL;
 }
 
 int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxit, int mixed, gx_solver_stats *st,

@@ -122,6 +122,13 @@ int gx_M5Ddag(gx_op op, gx_field psi, gx_field chi);
 int gx_Meooe5D(gx_op op, gx_field psi, gx_field chi);
 int gx_MeooeDag5D(gx_op op, gx_field psi, gx_field chi);
 
+/* 4D <-> 5D maps of a Mobius operator (full fields; the 4D field has Ls = 1 and the operator's
+ * 4D dimensions).  Grid: CayleyFermion5D::ImportPhysicalFermionSource (P+ at s=0, P- at s=Ls-1,
+ * then Dminus = 1 - c DW) and ExportPhysicalFermionSolution (P- psi(0) + P+ psi(Ls-1)),
+ * CayleyFermion5DImplementation.h:57-129. */
+int gx_import_physical_fermion_source(gx_op op, gx_field in4d, gx_field out5d);
+int gx_export_physical_fermion_solution(gx_op op, gx_field sol5d, gx_field out4d);
+
 /* ---- Schur even-odd operator (Grid: SchurDiagMooeeOperator, LinearOperator.h:325-349;
  *      SchurOperatorBase::MpcDagMpc/HermOpAndNorm :286-320) ---- */
 int gx_Mpc(gx_op op, gx_field in, gx_field out);
@@ -159,6 +166,16 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
  * unprec_residual = |M sol - src| / |src|. */
 int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, int mixed,
                    gx_solver_stats *st, double *unprec_residual);
+
+/* Grid: BlockConjugateGradient(CGmultiRHS, Orthog = 0, tol, maxit)(Linop, src, psi),
+ * BlockConjugateGradient.h:412-545: CG with per-right-hand-side alpha and beta on a multi-RHS
+ * Wilson operator (GX_WILSON with Ls = Nrhs; right-hand side k is slice s = k), iterating every
+ * right-hand side until the largest relative residual is below tol.  inner_prec = GX_F64 runs it
+ * in FP64; GX_F32 runs FP32 multi-RHS inner solves inside FP64 defect correction
+ * (ConjugateGradientMixedPrecBatched.h:100-213, inner tolerance 1e-5) and an FP64 patch-up.
+ * src, psi: F64 half fields; st: array of Ls statistics (one per right-hand side). */
+int gx_block_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, int inner_prec,
+                gx_solver_stats *st);
 
 /* ---- launch-shape tuning (no Grid counterpart: Grid fixes its launch shape at build time,
  *      Grid/threads/Accelerator.h:416-461).  Process-wide knobs read by the kernel launchers;

@@ -577,6 +577,47 @@ void oracle_schur(const oracle_params *P, const double *Uds, const double *in, d
   delete C;
 }
 
+// ImportPhysicalFermionSource (CayleyFermion5DImplementation.h:114-129): InsertSlice at s=0 and
+// s=Ls-1, P+ on slice 0 and P- on slice Ls-1 (LinalgUtils.h:140-183), then Dminus (:130-139):
+// chi_s = psi_s - cs_s DW(psi)_s with DW = Dhop + (4-M5) (WilsonFermion5DImplementation.h:447-452).
+void oracle_import_physical(const oracle_params *P, const double *Uds, const double *in4d_, double *out5d_) {
+  Geom g(P->dims, P->Ls);
+  Cayley C(P->Ls, P->mass, P->M5, P->b, P->c);
+  const int Ls = g.Ls;
+  const cd *in = reinterpret_cast<const cd *>(in4d_);
+  cd *out = reinterpret_cast<cd *>(out5d_);
+  std::vector<cd> tmp(g.V4 * Ls * 12, 0.0), dw(g.V4 * Ls * 12);
+  for (int64_t s4 = 0; s4 < g.V4; s4++)
+    for (int k = 0; k < 12; k++) {
+      const bool up = k < 6;
+      tmp[(s4 * Ls + 0) * 12 + k] = up ? in[s4 * 12 + k] : cd(0);
+      tmp[(s4 * Ls + Ls - 1) * 12 + k] = up ? cd(0) : in[s4 * 12 + k];
+    }
+  dhop(g, reinterpret_cast<const cd *>(Uds), tmp.data(), dw.data(), -1, 0);
+  axpby((int64_t)dw.size(), 1.0, dw.data(), 4.0 - P->M5, tmp.data(), dw.data());
+  for (int64_t s4 = 0; s4 < g.V4; s4++)
+    for (int s = 0; s < Ls; s++)
+      for (int k = 0; k < 12; k++) {
+        const int64_t i = (s4 * Ls + s) * 12 + k;
+        out[i] = 1.0 * tmp[i] + (-C.cs[s]) * dw[i];
+      }
+}
+
+// ExportPhysicalFermionSolution (CayleyFermion5DImplementation.h:57-67): P- psi(0) + P+ psi(Ls-1).
+void oracle_export_physical(const oracle_params *P, const double *sol5d_, double *out4d_) {
+  Geom g(P->dims, P->Ls);
+  const int Ls = g.Ls;
+  const cd *sol = reinterpret_cast<const cd *>(sol5d_);
+  cd *out = reinterpret_cast<cd *>(out4d_);
+  for (int64_t s4 = 0; s4 < g.V4; s4++)
+    for (int k = 0; k < 12; k++) {
+      const bool up = k < 6;
+      const cd lo = up ? cd(0) : sol[(s4 * Ls + 0) * 12 + k];
+      const cd hi = up ? sol[(s4 * Ls + Ls - 1) * 12 + k] : cd(0);
+      out[s4 * 12 + k] = lo + hi;
+    }
+}
+
 // Cayley coefficient vectors in order bs,cs,bee,cee,dee,lee,leem,uee,ueem (each Ls doubles)
 void oracle_cayley_coeffs(const oracle_params *P, double *out) {
   Cayley C(P->Ls, P->mass, P->M5, P->b, P->c);

@@ -51,6 +51,8 @@ def lib():
         _lib.oracle_schur_solve.argtypes = [pp, dp, dp, dp, ctypes.c_double, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int), dp, dp]
         _lib.oracle_schur_solve.restype = ctypes.c_int
+        _lib.oracle_import_physical.argtypes = [pp, dp, dp, dp]
+        _lib.oracle_export_physical.argtypes = [pp, dp, dp]
         _lib.oracle_set_threads.argtypes = [ctypes.c_int]
         _lib.oracle_num_threads.restype = ctypes.c_int
     return _lib
@@ -111,6 +113,20 @@ class Oracle:
         out = np.zeros(self.half_shape, dtype=np.complex128)
         lib().oracle_schur(ctypes.byref(self.p), _dp(self.Uds), _dp(psi), _dp(out),
                            dict(Mpc=0, MpcDag=1, MpcDagMpc=2)[which], cb)
+        return out
+
+    def import_physical(self, in4d):
+        lx, ly, lz, lt = self.dims
+        in4d = self._in(in4d, (lt, lz, ly, lx, 1, 4, 3))
+        out = np.zeros(self.full_shape, dtype=np.complex128)
+        lib().oracle_import_physical(ctypes.byref(self.p), _dp(self.Uds), _dp(in4d), _dp(out))
+        return out
+
+    def export_physical(self, sol5d):
+        lx, ly, lz, lt = self.dims
+        sol5d = self._in(sol5d, self.full_shape)
+        out = np.zeros((lt, lz, ly, lx, 1, 4, 3), dtype=np.complex128)
+        lib().oracle_export_physical(ctypes.byref(self.p), _dp(sol5d), _dp(out))
         return out
 
     def coeffs(self):

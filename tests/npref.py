@@ -182,6 +182,22 @@ def mobius_mooee_inv(psi, mf, M5, b, c, dag=False):
     return out
 
 
+def import_physical(U, in4d, ls, M5, c):
+    """5D source from a 4D one: P+ b at s=0, P- b at s=Ls-1, then (1 - c DW)."""
+    shp = in4d.shape[:4] + (ls, 4, 3)
+    tmp = np.zeros(shp, dtype=complex)
+    tmp[..., 0, :, :] = spin_mul(PPLUS, in4d[..., 0, :, :])
+    tmp[..., ls - 1, :, :] = spin_mul(PMINUS, in4d[..., 0, :, :])
+    return tmp - c * (dhop(U, tmp) + (4.0 - M5) * tmp)
+
+
+def export_physical(sol5d):
+    """4D propagator from a 5D solution: P- psi(0) + P+ psi(Ls-1)."""
+    ls = sol5d.shape[4]
+    out = spin_mul(PMINUS, sol5d[..., 0, :, :]) + spin_mul(PPLUS, sol5d[..., ls - 1, :, :])
+    return out[..., None, :, :]
+
+
 def wilson_M(U, psi, mass, dag=False):
     return dhop(U, psi, dag) + (4.0 + mass) * psi
 

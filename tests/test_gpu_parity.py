@@ -146,6 +146,24 @@ def test_mobius_ops(ctx, Ls, dims, prec):
     D.close()
 
 
+@pytest.mark.parametrize('prec', [gx.F64, gx.F32])
+def test_physical_source_maps(ctx, prec):
+    dims, Ls = (8, 4, 4, 8), 8
+    kw = dict(mass=0.05, M5=1.8, b=1.5, c=0.5)
+    U = gx.random_gauge(dims, 71, 0.3)
+    b4 = gx.gaussian_fermion(dims, 1, 72)
+    sol = gx.gaussian_fermion(dims, Ls, 73)
+    O = Oracle(U, dims, Ls, 'mobius', **kw)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    f4 = gx.Field(ctx, dims, 1, prec, gx.FULL).upload(b4)
+    f5 = D.field(prec)
+    D.ImportPhysicalFermionSource(f4, f5)
+    assert max_site_rel_err(f5.download(), O.import_physical(b4)) < TOL[prec] * 10
+    f5.upload(sol)
+    D.ExportPhysicalFermionSolution(f5, f4)
+    assert max_site_rel_err(f4.download(), O.export_physical(sol)) < TOL[prec]
+
+
 @pytest.mark.parametrize('nrhs', [4, 12])
 def test_multi_rhs_wilson_matches_single(ctx, nrhs):
     dims = (8, 4, 4, 8)
@@ -219,3 +237,20 @@ def test_schur_solve_matches_oracle(ctx):
     st = D.schur_solve(fs, fsol, tol=1e-10)
     assert st['converged'] and st['unprec_residual'] < 1e-8
     assert rel_err(fsol.download(), ref) < 1e-8
+
+
+@pytest.mark.parametrize('inner', [gx.F64, gx.F32])
+def test_block_cg_matches_single_rhs(ctx, inner):
+    """CGmultiRHS: every right-hand side agrees with the oracle's single-RHS FP64 CG."""
+    dims, nrhs, mass = (8, 8, 4, 8), 4, 0.1
+    U = gx.random_gauge(dims, 61, 0.5)
+    src = half(gx.gaussian_fermion(dims, nrhs, 62), dims, ODD)
+    D = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+    fs, fp = D.field(gx.F64, ODD).upload(src), D.field(gx.F64, ODD)
+    sts = D.block_cg(fs, fp, tol=1e-10, inner_prec=inner)
+    x = fp.download()
+    O1 = Oracle(U, dims, 1, 'wilson', mass=mass)
+    for k in range(nrhs):
+        ref, rst = O1.cg(np.ascontiguousarray(src[..., k:k + 1, :, :]), tol=1e-10)
+        assert sts[k]['converged'] and sts[k]['true_residual'] < 1e-9, sts[k]
+        assert rel_err(x[..., k:k + 1, :, :], ref) < 1e-8

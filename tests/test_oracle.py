@@ -182,6 +182,24 @@ def test_mobius_reduces_to_shamir(lat8):
     assert rel_err(O.apply('M', psi), expect) < TOL64
 
 
+def test_physical_source_maps(lat8):
+    """Import (P+ at s=0, P- at s=Ls-1, then 1 - c DW) and export (P- psi(0) + P+ psi(Ls-1))."""
+    dims, U, rng = lat8
+    Ls, M5, c = 6, 1.8, 0.5
+    O = Oracle(U, dims, Ls, 'mobius', mass=0.05, M5=M5, b=1.5, c=c)
+    b4 = R.gaussian_fermion(rng, dims, 1)
+    assert rel_err(O.import_physical(b4), R.import_physical(U, b4, Ls, M5, c)) < TOL64
+    sol = R.gaussian_fermion(rng, dims, Ls)
+    assert rel_err(O.export_physical(sol), R.export_physical(sol)) < 1e-15
+    # with c = 0 the import is the bare chiral embedding and export undoes it
+    O0 = Oracle(U, dims, Ls, 'mobius', mass=0.05, M5=M5, b=1.0, c=0.0)
+    emb = O0.import_physical(b4)
+    flipped = np.zeros_like(emb)
+    flipped[..., 0, :, :] = emb[..., Ls - 1, :, :]
+    flipped[..., Ls - 1, :, :] = emb[..., 0, :, :]
+    assert rel_err(O0.export_physical(flipped), b4) < 1e-15
+
+
 def test_cayley_coefficients_mobius():
     U = np.zeros((2, 2, 2, 2, 4, 3, 3), dtype=complex)
     U[...] = np.eye(3)
