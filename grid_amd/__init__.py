@@ -239,7 +239,7 @@ def field_copy(dst, src):
     _check(lib().gx_field_copy(dst.h, src.h))
 
 
-TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER = 0, 1, 2, 3
+TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH = 0, 1, 2, 3, 4
 
 
 def tune_set(knob, value):

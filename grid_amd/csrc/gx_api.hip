@@ -25,8 +25,10 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
-static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 0, 0, 0, 0};
+static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 0, 0, 0};
+static int g_tune_version = 0;
 int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
+int tune_version() { return g_tune_version; }
 
 void set_error(const char *fmt, ...) {
   va_list ap;
@@ -657,8 +659,9 @@ const char *gx_version(void) { return "gx 0.1 (gfx950)"; }
 
 int gx_tune_set(int knob, int value) {
   GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT, GX_EINVAL, "unknown tuning knob %d", knob);
-  if (knob <= GX_TUNE_DHOP_ORDER) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
+  if (knob <= GX_TUNE_CG_GRAPH) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   g_tune[knob] = value;
+  g_tune_version++;
   return GX_OK;
 }
 

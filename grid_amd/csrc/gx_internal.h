@@ -23,8 +23,9 @@ namespace gx {
 
 void set_error(const char *fmt, ...);
 
-// process-wide launch-shape knobs (gx_tune_set / gx_tune_get)
+// process-wide launch-shape knobs (gx_tune_set / gx_tune_get); the version changes on every set
 int tune_get(int knob);
+int tune_version();
 
 #define GX_CHECK_HIP(call)                                                                             \
   do {                                                                                                 \

@@ -21,10 +21,51 @@
 
 namespace gx {
 
+// One captured batch of solver iterations.  The graph is rebuilt when the buffers it was captured
+// with, the batch length or the launch-shape knobs change.
+struct BatchGraph {
+  hipGraphExec_t exec = nullptr;
+  const void *key[6] = {};
+  int batch = 0, version = -1;
+};
+
+// Enqueues one batch of iterations: body() issues them; with graphs on, the first call captures
+// body() into a hipGraph and every call replays it.  The iterations contain kernel launches only
+// (no memset or copy nodes); their device-side gate makes replays past convergence no-ops.
+template <class F>
+static int run_batch(Ctx *ctx, BatchGraph &g, const void *const key[6], int batch, F body) {
+  const int ver = tune_version();
+  const bool hit = g.exec && g.batch == batch && g.version == ver && std::memcmp(g.key, key, sizeof(g.key)) == 0;
+  if (!hit) {
+    if (g.exec) {
+      hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    GX_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != GX_OK) {
+      if (e == hipSuccess && graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    GX_CHECK_HIP(e);
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    GX_CHECK_HIP(ie);
+    std::memcpy(g.key, key, sizeof(g.key));
+    g.batch = batch;
+    g.version = ver;
+  }
+  GX_CHECK_HIP(hipGraphLaunch(g.exec, ctx->stream));
+  return GX_OK;
+}
+
 struct CGWork {
   CGState *dev = nullptr;
   CGState *host = nullptr;  // pinned, 2 slots
   hipEvent_t ev[2] = {};
+  BatchGraph graph[3];  // per precision
 };
 
 static CGWork &cg_work(Op *op) {
@@ -110,11 +151,28 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
   const int64_t vol = op->g.half_elems();
   const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
   int enqueued = 0, slot = 0;
-  auto enqueue = [&](int s) -> int {
-    for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+  const bool use_graph = tune_get(GX_TUNE_CG_GRAPH) != 0;
+  for (int i = 0; i < 4; i++)
+    if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
+  auto one_batch = [&]() -> int {
+    for (int i = 0; i < batch; i++) {
       int rc = prec == GX_F64 ? cg_iteration<double>(op, psi, p, mmp, r, w.dev)
                               : cg_iteration<float>(op, psi, p, mmp, r, w.dev);
       GX_TRY(rc);
+    }
+    return GX_OK;
+  };
+  auto enqueue = [&](int s) -> int {
+    if (use_graph) {
+      const void *key[6] = {psi->data, p->data, mmp->data, r->data, (const void *)(intptr_t)(prec + 1), w.dev};
+      GX_TRY(run_batch(ctx, w.graph[prec], key, batch, one_batch));
+      enqueued += batch;
+    } else {
+      for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+        int rc = prec == GX_F64 ? cg_iteration<double>(op, psi, p, mmp, r, w.dev)
+                                : cg_iteration<float>(op, psi, p, mmp, r, w.dev);
+        GX_TRY(rc);
+      }
     }
     GX_CHECK_HIP(hipMemcpyAsync(&w.host[s], w.dev, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
     GX_CHECK_HIP(hipEventRecord(w.ev[s], ctx->stream));
@@ -208,6 +266,7 @@ struct MrhsWork {
   double *out_dev = nullptr;  // per-slice results
   double *out_host = nullptr; // pinned mirror
   hipEvent_t ev[2] = {};
+  BatchGraph graph[3];  // per precision
 };
 
 static MrhsWork &mrhs_work(Op *op) {
@@ -301,11 +360,28 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
     const int64_t vol = op->g.half_elems();
     const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
     int enqueued = 0, slot = 0;
-    auto enqueue = [&](int sl) -> int {
-      for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+    const bool use_graph = tune_get(GX_TUNE_CG_GRAPH) != 0;
+    for (int i = 0; i < 4; i++)
+      if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
+    auto one_batch = [&]() -> int {
+      for (int i = 0; i < batch; i++) {
         int rc = prec == GX_F64 ? mrhs_iteration<double>(op, psi, p, mmp, r, w.dev)
                                 : mrhs_iteration<float>(op, psi, p, mmp, r, w.dev);
         GX_TRY(rc);
+      }
+      return GX_OK;
+    };
+    auto enqueue = [&](int sl) -> int {
+      if (use_graph) {
+        const void *key[6] = {psi->data, p->data, mmp->data, r->data, (const void *)(intptr_t)(prec + 1), w.dev};
+        GX_TRY(run_batch(ctx, w.graph[prec], key, batch, one_batch));
+        enqueued += batch;
+      } else {
+        for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+          int rc = prec == GX_F64 ? mrhs_iteration<double>(op, psi, p, mmp, r, w.dev)
+                                  : mrhs_iteration<float>(op, psi, p, mmp, r, w.dev);
+          GX_TRY(rc);
+        }
       }
       GX_CHECK_HIP(hipMemcpyAsync(&w.host[sl], w.dev, sizeof(MrhsState), hipMemcpyDeviceToHost, ctx->stream));
       GX_CHECK_HIP(hipEventRecord(w.ev[sl], ctx->stream));

@@ -185,6 +185,7 @@ enum gx_tune_knob {
   GX_TUNE_DHOP_PF = 1,      /* 1: issue the next leg's neighbour loads before this leg's math */
   GX_TUNE_DHOP_NT = 2,      /* 1: non-temporal (streaming) output stores */
   GX_TUNE_DHOP_ORDER = 3,   /* 0: Grid's leg order, 1: the two legs of each direction adjacent */
+  GX_TUNE_CG_GRAPH = 4,     /* 1: solver iteration batches are captured once and replayed as hipGraphs */
   GX_TUNE_COUNT = 8
 };
 int gx_tune_set(int knob, int value);

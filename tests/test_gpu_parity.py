@@ -239,6 +239,36 @@ def test_schur_solve_matches_oracle(ctx):
     assert rel_err(fsol.download(), ref) < 1e-8
 
 
+@pytest.mark.parametrize('kind,Ls', [('wilson', 1), ('mobius', 8), ('wilson', 4)])
+def test_graph_replay_matches_eager(ctx, kind, Ls):
+    """CG iteration batches replayed as hipGraphs give bitwise the eager result, on every replay,
+    with eager work and synchronisations between the replays."""
+    dims = (8, 8, 4, 4)
+    U = gx.random_gauge(dims, 81, 0.4)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, mass=0.1)
+    srcs = [half(gx.gaussian_fermion(dims, Ls, 82 + i), dims, ODD) for i in range(3)]
+    solve = (lambda s, x: D.block_cg(s, x, tol=1e-9)) if Ls > 1 else (lambda s, x: D.cg(s, x, tol=1e-9))
+    eager = []
+    gx.tune_set(gx.TUNE_CG_GRAPH, 0)
+    try:
+        for s in srcs:
+            fs, fx = D.field(gx.F64, ODD).upload(s), D.field(gx.F64, ODD)
+            solve(fs, fx)
+            eager.append(fx.download())
+        gx.tune_set(gx.TUNE_CG_GRAPH, 1)
+        fs, fx = D.field(gx.F64, ODD), D.field(gx.F64, ODD)
+        for rep in range(2):
+            for s, ref in zip(srcs, eager):
+                fs.upload(s)
+                fx.zero()
+                solve(fs, fx)
+                ctx.sync()
+                assert np.array_equal(fx.download(), ref), (rep, kind, Ls)
+                fs.norm2()  # eager work between replays
+    finally:
+        gx.tune_set(gx.TUNE_CG_GRAPH, 1)
+
+
 @pytest.mark.parametrize('inner', [gx.F64, gx.F32])
 def test_block_cg_matches_single_rhs(ctx, inner):
     """CGmultiRHS: every right-hand side agrees with the oracle's single-RHS FP64 CG."""
