@@ -30,7 +30,7 @@ LS = 16
 PARAMS = dict(mass=0.01, M5=1.8, b=1.5, c=0.5)
 EPS = 0.3
 METRIC = 'Dslash GB/s (% HBM peak); CG time-to-1e-10 mixed vs FP64; solve p50/p99'
-DHOP_KERNEL = 'k_dhop<double, 16, false>'
+DHOP_KERNEL = 'gx::k_dhop<double, 16, false, true, 0, 0>'
 
 
 def pick(a, dims, cb):

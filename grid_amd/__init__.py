@@ -39,6 +39,28 @@ class SolverStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Request(ctypes.Structure):
+    _fields_ = [('dims', ctypes.c_int * 4), ('nsrc', ctypes.c_int), ('mass', ctypes.c_double),
+                ('gauge', ctypes.c_int), ('seed', ctypes.c_uint64), ('arrival', ctypes.c_double)]
+
+
+class Response(ctypes.Structure):
+    _fields_ = [('status', ctypes.c_int), ('slot', ctypes.c_int), ('iterations', ctypes.c_int),
+                ('outer_iterations', ctypes.c_int), ('residual', ctypes.c_double),
+                ('solver_residual', ctypes.c_double), ('t_admit', ctypes.c_double),
+                ('t_done', ctypes.c_double), ('footprint', ctypes.c_double), ('digest', ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeParams(ctypes.Structure):
+    _fields_ = [('device', ctypes.c_int), ('slots', ctypes.c_int), ('budget', ctypes.c_double),
+                ('n_gauge', ctypes.c_int), ('gauge_seed', ctypes.c_uint64), ('gauge_eps', ctypes.c_double),
+                ('tol', ctypes.c_double), ('mixed', ctypes.c_int), ('maxit', ctypes.c_int),
+                ('realtime', ctypes.c_int)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -107,6 +129,14 @@ _SIG = {
     'gx_random_gauge': ([_IP, ctypes.c_uint64, _D, _DP], _I),
     'gx_gaussian_fermion': ([_IP, _I, ctypes.c_uint64, _DP], _I),
     'gx_point_source': ([_IP, _I, _IP, _I, _I, _DP], _I),
+    'gx_wilson_clone': ([_P, _P, _I, _D, ctypes.POINTER(_P)], _I),
+    'gx_op_share_links': ([_P, _P], _I),
+    'gx_op_footprint': ([_P, ctypes.POINTER(ctypes.c_size_t)], _I),
+    'gx_field_gaussian': ([_P, ctypes.c_uint64], _I),
+    'gx_field_digest': ([_P, ctypes.POINTER(ctypes.c_uint64)], _I),
+    'gx_serve_footprint': ([_IP, _I, _I, _DP], _I),
+    'gx_serve': ([ctypes.POINTER(ServeParams), ctypes.POINTER(Request), _I, ctypes.POINTER(Response),
+                  ctypes.POINTER(_DP), _DP, _DP], _I),
 }
 
 
@@ -207,6 +237,16 @@ class Field:
         _check(lib().gx_norm2(self.h, ctypes.byref(v)))
         return v.value
 
+    def gaussian(self, seed):
+        """Fill a full field on the device with gaussian_fermion(dims, Ls, seed)."""
+        _check(lib().gx_field_gaussian(self.h, ctypes.c_uint64(seed)))
+        return self
+
+    def digest(self):
+        v = ctypes.c_uint64()
+        _check(lib().gx_field_digest(self.h, ctypes.byref(v)))
+        return v.value
+
     def close(self):
         if self.h:
             lib().gx_field_destroy(self.h)
@@ -239,7 +279,7 @@ def field_copy(dst, src):
     _check(lib().gx_field_copy(dst.h, src.h))
 
 
-TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH = 0, 1, 2, 3, 4
+TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH, TUNE_FUSED = 0, 1, 2, 3, 4, 5
 
 
 def tune_set(knob, value):
@@ -283,6 +323,32 @@ class FermionOp:
 
     def field(self, prec=F64, subset=FULL):
         return Field(self.ctx, self.dims, self.Ls, prec, subset)
+
+    def set_mass(self, mass):
+        self._call('op_set_mass', ctypes.c_double(mass))
+
+    def clone(self, ctx, Ls, mass):
+        """Wilson operator reading this operator's resident links, with its own context, Ls and mass."""
+        c = FermionOp.__new__(FermionOp)
+        c.ctx, c.dims, c.Ls, c.kind = ctx, self.dims, int(Ls), 'wilson'
+        h = _P()
+        _check(lib().gx_wilson_clone(self.h, ctx.h, int(Ls), mass, ctypes.byref(h)))
+        c.h = h
+        c._base = self  # the links must outlive the clone
+        return c
+
+    def share_links(self, base):
+        self._call('op_share_links', base.h)
+        self._base = base
+
+    def footprint(self):
+        v = ctypes.c_size_t()
+        self._call('op_footprint', ctypes.byref(v))
+        return v.value
+
+    def import_gauge(self, U):
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        self._call('op_import_gauge', _dp(U))
 
     def _call(self, name, *args):
         _check(getattr(lib(), 'gx_' + name)(self.h, *args))

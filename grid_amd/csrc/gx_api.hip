@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <thread>
 
 #include "gx_internal.h"
@@ -25,7 +26,7 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
-static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 0, 0, 0};
+static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0};
 static int g_tune_version = 0;
 int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
 int tune_version() { return g_tune_version; }
@@ -73,7 +74,7 @@ static Geom make_geom(const int dims[4], int Ls) {
   return g;
 }
 
-static bool same_geom(const Geom &a, const Geom &b) {
+bool same_geom(const Geom &a, const Geom &b) {
   for (int d = 0; d < 4; d++)
     if (a.L[d] != b.L[d]) return false;
   return a.Ls == b.Ls;
@@ -118,23 +119,28 @@ int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out) {
   f->bytes = (size_t)f->elems() * f->csize();
   hipError_t e = hipMalloc(&f->data, f->bytes);
   if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) failed: %s", f->bytes, hipGetErrorString(e));
     delete f;
-    set_error("hipMalloc(%zu) failed: %s", (size_t)f->elems() * 16, hipGetErrorString(e));
     return GX_ENOMEM;
   }
-  hipMemsetAsync(f->data, 0, f->bytes, ctx->stream);
+  e = hipMemsetAsync(f->data, 0, f->bytes, ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(f->data);
+    delete f;
+    GX_CHECK_HIP(e);
+  }
   *out = f;
   return GX_OK;
 }
 
 void field_destroy(Field *f) {
   if (!f) return;
-  if (f->data) hipFree(f->data);
+  if (f->data) (void)hipFree(f->data);
   delete f;
 }
 
 // A non-owning half-field view of one checkerboard of a full field.
-static Field half_view(const Field *full, int cb) {
+Field half_view(const Field *full, int cb) {
   Field v = *full;
   v.data = full->half(cb);
   v.subset = cb;
@@ -148,6 +154,17 @@ Field *op_tmp(Op *op, int prec, int i) {
     if (field_create(op->ctx, op->g, prec, GX_EVEN, &t) != GX_OK) return nullptr;
   }
   return t;
+}
+
+// Creates everything the operator applications of one precision allocate on first use (its link
+// copy and temporaries 0-3), so that they can be captured into a graph afterwards.
+int op_prepare(Op *op, int prec) {
+  void *ue, *uo;
+  GX_TRY(op_gauge(op, prec, &ue, &uo));
+  for (int i = 0; i < 4; i++)
+    if (!op_tmp(op, prec, i)) return GX_ENOMEM;
+  GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
+  return GX_OK;
 }
 
 // ---------------- reductions ----------------
@@ -271,7 +288,7 @@ static void free_gauge(Op *op) {
   for (int p = 0; p < 3; p++)
     for (int cb = 0; cb < 2; cb++)
       if (op->U[p][cb]) {
-        hipFree(op->U[p][cb]);
+        if (!op->link_owner) (void)hipFree(op->U[p][cb]);
         op->U[p][cb] = nullptr;
       }
 }
@@ -454,6 +471,19 @@ static void build_smats(Op *op) {
   lower[0] = mass;
   m5d_matrix(Ls, lower, diag, upper, false, op->smat[Op::S_M5D_OFF]);
   m5d_matrix(Ls, lower, diag, upper, true, op->smat[Op::S_M5DDAG_OFF]);
+  // products for the fused Schur operator: (A B)[s][t] = sum_u A[s][u] B[u][t] per chirality
+  auto product = [Ls](const SMat &A, const SMat &B, SMat &C) {
+    C.m.assign(2 * Ls * Ls, 0.0);
+    for (int chi = 0; chi < 2; chi++)
+      for (int s = 0; s < Ls; s++)
+        for (int t = 0; t < Ls; t++) {
+          double v = 0;
+          for (int u = 0; u < Ls; u++) v += A.m[chi * Ls * Ls + s * Ls + u] * B.m[chi * Ls * Ls + u * Ls + t];
+          C.m[chi * Ls * Ls + s * Ls + t] = v;
+        }
+  };
+  product(op->smat[Op::S_MEOOE5D], op->smat[Op::S_MOOEEINV], op->smat[Op::S_MEO5D_INV]);
+  product(op->smat[Op::S_MOOEEINVDAG], op->smat[Op::S_MEOOEDAG5D], op->smat[Op::S_INVDAG_MEODAG5D]);
 }
 
 static int upload_smats(Op *op) {
@@ -464,7 +494,7 @@ static int upload_smats(Op *op) {
   std::vector<float> allf(all.begin(), all.end());
   for (int p = 0; p < 2; p++)
     if (op->smat_dev[p]) {
-      hipFree(op->smat_dev[p]);
+      (void)hipFree(op->smat_dev[p]);
       op->smat_dev[p] = nullptr;
     }
   GX_CHECK_HIP(hipMalloc(&op->smat_dev[GX_F64], all.size() * sizeof(double)));
@@ -530,7 +560,57 @@ int apply_mooee(Op *op, Field *in, Field *out, int which, const Gate *gate) {
 }
 
 // SchurDiagMooeeOperator::Mpc / MpcDag, LinearOperator.h:330-348
+// Fused Schur operator (same operator as below, fewer passes over the fields):
+//   Wilson  Mpc    = d in - (1/d) Dhop(Dhop(in)),  d = 4 + m                      2 kernels
+//   Cayley  Mpc    = Mooee in - Dhop(P1 Dhop(Meooe5D in)),  P1 = Meooe5D MooeeInv   3 kernels
+//           MpcDag = MooeeDag in - MeooeDag5D Dhop^dag(P2 Dhop^dag in),
+//                    P2 = MooeeInvDag MeooeDag5D                                   3 kernels
+// The fifth-dimension matrices act across each site's s-row in the hop kernel's epilogue.
+template <typename T>
+static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
+  const int prec = in->prec, cb_in = in->cb, cb_mid = 1 - cb_in;
+  void *Ue, *Uo;
+  if (prec == GX_F64) {
+    Ue = op->U[GX_F64][0];
+    Uo = op->U[GX_F64][1];
+  } else {
+    GX_TRY(op_gauge(op, prec, &Ue, &Uo));
+  }
+  void *U_mid = cb_mid == 0 ? Ue : Uo, *U_in = cb_in == 0 ? Ue : Uo;
+  hipStream_t st = op->ctx->stream;
+  const Geom &g = op->g;
+  Field *t0 = op_tmp(op, prec, 0), *t1 = op_tmp(op, prec, 1), *t3 = op_tmp(op, prec, 3);
+  if (!t0 || !t1 || !t3) return GX_ENOMEM;
+  if (op->kind == GX_WILSON) {
+    const double d = 4.0 + op->mass;
+    GX_TRY(launch_dhop_epi<T>(g, in->data, t0->data, U_mid, cb_mid, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0));
+    GX_TRY(launch_dhop_epi<T>(g, t0->data, out->data, U_in, cb_in, dag, gate, st, 1, in->data, nullptr, -1.0 / d, d));
+  } else if (!dag) {
+    const void *m_meo, *m_mooee, *m_p1;
+    GX_TRY(op_smat(op, prec, Op::S_MEOOE5D, &m_meo));
+    GX_TRY(op_smat(op, prec, Op::S_MOOEE, &m_mooee));
+    GX_TRY(op_smat(op, prec, Op::S_MEO5D_INV, &m_p1));
+    // t0 = Meooe5D in, t1 = Mooee in (one read of in)
+    GX_TRY(launch_smat2<T>(g, in->data, nullptr, 0.0, t0->data, m_meo, t1->data, m_mooee, gate, st));
+    GX_TRY(launch_dhop_epi<T>(g, t0->data, t3->data, U_mid, cb_mid, 0, gate, st, 2, nullptr, m_p1, 1.0, 0.0));
+    GX_TRY(launch_dhop_epi<T>(g, t3->data, out->data, U_in, cb_in, 0, gate, st, 1, t1->data, nullptr, -1.0, 1.0));
+  } else {
+    const void *m_mooeedag, *m_p2, *m_meodag;
+    GX_TRY(op_smat(op, prec, Op::S_MOOEEDAG, &m_mooeedag));
+    GX_TRY(op_smat(op, prec, Op::S_INVDAG_MEODAG5D, &m_p2));
+    GX_TRY(op_smat(op, prec, Op::S_MEOOEDAG5D, &m_meodag));
+    GX_TRY(launch_smat2<T>(g, in->data, nullptr, 0.0, t1->data, m_mooeedag, nullptr, nullptr, gate, st));
+    GX_TRY(launch_dhop_epi<T>(g, in->data, t3->data, U_mid, cb_mid, 1, gate, st, 2, nullptr, m_p2, 1.0, 0.0));
+    GX_TRY(launch_dhop_epi<T>(g, t3->data, out->data, U_in, cb_in, 1, gate, st, 2, t1->data, m_meodag, -1.0, 1.0));
+  }
+  out->cb = cb_in;
+  return GX_OK;
+}
+
 int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
+  if (tune_get(GX_TUNE_FUSED)) {
+    return in->prec == GX_F64 ? mpc_fused<double>(op, in, out, dag, gate) : mpc_fused<float>(op, in, out, dag, gate);
+  }
   Field *t1 = op_tmp(op, in->prec, 0), *t2 = op_tmp(op, in->prec, 1);
   if (!t1 || !t2) return GX_ENOMEM;
   GX_TRY(apply_meooe(op, in, t1, dag, gate));
@@ -659,7 +739,7 @@ const char *gx_version(void) { return "gx 0.1 (gfx950)"; }
 
 int gx_tune_set(int knob, int value) {
   GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT, GX_EINVAL, "unknown tuning knob %d", knob);
-  if (knob <= GX_TUNE_CG_GRAPH) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
+  if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   g_tune[knob] = value;
   g_tune_version++;
   return GX_OK;
@@ -679,8 +759,7 @@ int gx_ctx_create(int device, gx_ctx *out) {
   GX_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (int i = 0; i < 16; i++) GX_CHECK_HIP(hipEventCreate(&c->ev[i]));
   GX_CHECK_HIP(hipMalloc(&c->red_partial, sizeof(double) * kRedBlocks * kMaxLs));
-  GX_CHECK_HIP(hipMalloc(&c->red_ticket, sizeof(unsigned int) * 16));
-  GX_CHECK_HIP(hipMemset(c->red_ticket, 0, sizeof(unsigned int) * 16));
+  GX_CHECK_HIP(hipMalloc(&c->digest, sizeof(unsigned long long)));
   GX_CHECK_HIP(hipMalloc(&c->red_result, sizeof(double) * kRedSlots));
   GX_CHECK_HIP(hipHostMalloc(&c->pinned, sizeof(double) * kRedSlots, hipHostMallocDefault));
   *out = c;
@@ -689,13 +768,13 @@ int gx_ctx_create(int device, gx_ctx *out) {
 
 int gx_ctx_destroy(gx_ctx c) {
   if (!c) return GX_OK;
-  hipStreamSynchronize(c->stream);
-  for (int i = 0; i < 16; i++) hipEventDestroy(c->ev[i]);
-  hipFree(c->red_partial);
-  hipFree(c->red_ticket);
-  hipFree(c->red_result);
-  hipHostFree(c->pinned);
-  hipStreamDestroy(c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 16; i++) (void)hipEventDestroy(c->ev[i]);
+  (void)hipFree(c->red_partial);
+  (void)hipFree(c->digest);
+  (void)hipFree(c->red_result);
+  (void)hipHostFree(c->pinned);
+  (void)hipStreamDestroy(c->stream);
   delete c;
   return GX_OK;
 }
@@ -736,7 +815,7 @@ int gx_field_create(gx_ctx ctx, const int dims[4], int Ls, int prec, int subset,
 
 int gx_field_destroy(gx_field f) {
   if (!f) return GX_OK;
-  if (f->data) hipFree(f->data);
+  if (f->data) (void)hipFree(f->data);
   delete f;
   return GX_OK;
 }
@@ -755,6 +834,26 @@ int gx_field_import(gx_field f, const double *host) {
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
   }
   return GX_OK;
+}
+
+int gx_field_gaussian(gx_field f, uint64_t seed) {
+  GX_REQUIRE(f && f->subset == GX_FULL && (f->prec == GX_F64 || f->prec == GX_F32), GX_EINVAL,
+             "field_gaussian: needs a full F64 or F32 field");
+  // the host generator's key schedule (SplitMix64 of an affine map of the seed)
+  uint64_t x = seed * 0x9E6C63D0676A9A99ULL + 0xBB67AE8584CAA73BULL + 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  const uint64_t key = x ^ (x >> 31);
+  if (f->prec == GX_F64) GX_TRY(launch_gaussian<double>(f->g, f->data, key, f->ctx->stream));
+  else GX_TRY(launch_gaussian<float>(f->g, f->data, key, f->ctx->stream));
+  f->cb = 0;
+  return GX_OK;
+}
+
+int gx_field_digest(gx_field f, uint64_t *digest) {
+  GX_REQUIRE(f && digest, GX_EINVAL, "null argument");
+  GX_REQUIRE(f->bytes % 8 == 0, GX_EINVAL, "field_digest: size not a multiple of 8 bytes");
+  return launch_digest(f->ctx, f->data, (int64_t)(f->bytes / 8), digest);
 }
 
 int gx_field_export(gx_field f, double *host) {
@@ -861,6 +960,7 @@ static int op_common(gx_ctx ctx, const int dims[4], int Ls, double mass, const d
   if (phases)
     for (int i = 0; i < 8; i++) op->phases[i] = phases[i];
   op->host_U.assign(U, U + (size_t)op->g.V4 * 4 * 18);
+  op->epoch++;
   return upload_gauge(op);
 }
 
@@ -902,27 +1002,92 @@ int gx_mobius_create(gx_ctx ctx, const int dims[4], int Ls, double mass, double 
 
 int gx_op_destroy(gx_op op) {
   if (!op) return GX_OK;
-  hipStreamSynchronize(op->ctx->stream);
+  (void)hipStreamSynchronize(op->ctx->stream);
+  solver_work_free(op->work);
   free_gauge(op);
   for (int p = 0; p < 3; p++) {
-    if (op->smat_dev[p]) hipFree(op->smat_dev[p]);
-    for (int i = 0; i < 10; i++) field_destroy(op->tmp[p][i]);
+    if (op->smat_dev[p]) (void)hipFree(op->smat_dev[p]);
+    for (int i = 0; i < Op::kTmp; i++) field_destroy(op->tmp[p][i]);
   }
+  field_destroy(op->full_tmp);
   delete op;
+  return GX_OK;
+}
+
+// Links of a base Wilson operator in every precision a solve may use, ready on base's stream.
+static std::mutex g_share_mu;
+static int share_links(Op *op, Op *base) {
+  std::lock_guard<std::mutex> lk(g_share_mu);
+  void *ue, *uo;
+  GX_TRY(op_gauge(base, GX_F32, &ue, &uo));
+  GX_CHECK_HIP(hipStreamSynchronize(base->ctx->stream));
+  for (int p = 0; p < 3; p++)
+    for (int cb = 0; cb < 2; cb++) op->U[p][cb] = base->U[p][cb];
+  op->link_owner = base;
+  for (int i = 0; i < 8; i++) op->phases[i] = base->phases[i];
+  op->epoch++;
+  return GX_OK;
+}
+
+int gx_wilson_clone(gx_op base, gx_ctx ctx, int Ls, double mass, gx_op *out) {
+  GX_REQUIRE(base && ctx && out && base->kind == GX_WILSON, GX_EINVAL, "wilson_clone: needs a Wilson base operator");
+  GX_REQUIRE(!base->link_owner, GX_EINVAL, "wilson_clone: base is itself a clone");
+  GX_REQUIRE(Ls >= 1 && Ls <= kMaxLs, GX_EINVAL, "wilson_clone: Ls=%d out of range", Ls);
+  gx_op op = new gx_op_s();
+  op->kind = GX_WILSON;
+  op->ctx = ctx;
+  op->g = make_geom(base->g.L, Ls);
+  op->mass = mass;
+  const int rc = share_links(op, base);
+  if (rc != GX_OK) {
+    delete op;
+    return rc;
+  }
+  *out = op;
+  return GX_OK;
+}
+
+int gx_op_share_links(gx_op op, gx_op base) {
+  GX_REQUIRE(op && base && op->link_owner && !base->link_owner && base->kind == GX_WILSON, GX_EINVAL,
+             "share_links: needs a clone and a Wilson base operator");
+  for (int mu = 0; mu < 4; mu++)
+    GX_REQUIRE(op->g.L[mu] == base->g.L[mu], GX_EINVAL, "share_links: lattice differs from the base's");
+  GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
+  return share_links(op, base);
+}
+
+int gx_op_footprint(gx_op op, size_t *bytes) {
+  GX_REQUIRE(op && bytes, GX_EINVAL, "null argument");
+  size_t b = 0;
+  const size_t nlink = (size_t)8 * 9 * op->g.H4 * 2;  // complex numbers, both checkerboards
+  if (!op->link_owner) {
+    if (op->U[GX_F64][0]) b += nlink * 16;
+    if (op->U[GX_F32][0]) b += nlink * 8;
+  }
+  for (int p = 0; p < 3; p++) {
+    if (op->smat_dev[p]) b += (size_t)Op::S_COUNT * 2 * op->g.Ls * op->g.Ls * (p == GX_F64 ? 8 : 4);
+    for (int i = 0; i < Op::kTmp; i++)
+      if (op->tmp[p][i]) b += op->tmp[p][i]->bytes;
+  }
+  if (op->full_tmp) b += op->full_tmp->bytes;
+  *bytes = b;
   return GX_OK;
 }
 
 int gx_op_import_gauge(gx_op op, const double *U) {
   GX_REQUIRE(op && U, GX_EINVAL, "null argument");
-  hipStreamSynchronize(op->ctx->stream);
+  GX_REQUIRE(!op->link_owner, GX_EINVAL, "import_gauge: the operator shares another operator's links");
+  GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
   op->host_U.assign(U, U + (size_t)op->g.V4 * 4 * 18);
+  op->epoch++;
   return upload_gauge(op);
 }
 
 int gx_op_set_mass(gx_op op, double mass) {
   GX_REQUIRE(op, GX_EINVAL, "null op");
-  hipStreamSynchronize(op->ctx->stream);
+  GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
   op->mass = mass;
+  op->epoch++;
   if (op->kind == GX_MOBIUS) {
     cayley_coeffs(op);
     build_smats(op);
@@ -1032,7 +1197,7 @@ int gx_import_physical_fermion_source(gx_op op_, gx_field in4d, gx_field out5d) 
   }
   const double c = op->cs[0];
   if (rc == GX_OK) rc = blas_axpby(out, 1.0 - c * (4.0 - op->M5), out, -c, t);
-  hipStreamSynchronize(st);
+  if (hipStreamSynchronize(st) != hipSuccess && rc == GX_OK) rc = GX_EHIP;
   field_destroy(t);
   return rc;
 }

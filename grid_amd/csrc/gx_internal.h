@@ -80,7 +80,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[16] = {};
   double *red_partial = nullptr;     // [kRedBlocks * kMaxLs]
-  unsigned int *red_ticket = nullptr;
+  unsigned long long *digest = nullptr;  // field digest accumulator
   double *red_result = nullptr;      // [kRedSlots]
   double *pinned = nullptr;          // host mirror of red_result
   int next_slot = 0;
@@ -115,6 +115,11 @@ struct SMat {
   std::vector<double> m;   // 2*Ls*Ls
 };
 
+// Device state, pinned mirrors, events and captured graphs of the solvers of one operator
+// (gx_solver.hip); released with the operator.
+struct SolverWork;
+void solver_work_free(SolverWork *w);
+
 struct Op {
   Ctx *ctx = nullptr;
   int kind = GX_WILSON;
@@ -126,24 +131,47 @@ struct Op {
   // Cayley coefficients (CayleyFermion5DImplementation.h:410-535)
   std::vector<double> bs, cs, bee, cee, dee, lee, leem, uee, ueem;
   // fifth-dimension operators as per-chirality matrices: 0 Meooe5D, 1 MeooeDag5D, 2 Mooee,
-  // 3 MooeeDag, 4 MooeeInv, 5 MooeeInvDag, 6 M5D off-diagonal part, 7 M5Ddag off-diagonal part
-  enum { S_MEOOE5D = 0, S_MEOOEDAG5D, S_MOOEE, S_MOOEEDAG, S_MOOEEINV, S_MOOEEINVDAG, S_M5D_OFF, S_M5DDAG_OFF, S_COUNT };
+  // 3 MooeeDag, 4 MooeeInv, 5 MooeeInvDag, 6 M5D off-diagonal part, 7 M5Ddag off-diagonal part,
+  // 8 Meooe5D * MooeeInv, 9 MooeeInvDag * MeooeDag5D (products used by the fused Schur operator)
+  enum {
+    S_MEOOE5D = 0, S_MEOOEDAG5D, S_MOOEE, S_MOOEEDAG, S_MOOEEINV, S_MOOEEINVDAG, S_M5D_OFF, S_M5DDAG_OFF,
+    S_MEO5D_INV, S_INVDAG_MEODAG5D, S_COUNT
+  };
   SMat smat[S_COUNT];
   void *smat_dev[3] = {};       // [prec] device copy of all S_COUNT matrices (2*Ls*Ls each)
   // per precision temporaries (half fields): 0-3 operator compositions, 4-6 CG vectors,
-  // 7-8 mixed-precision outer vectors
-  Field *tmp[3][10] = {};
+  // 7-8 mixed-precision outer vectors, 10-14 the red-black solve's source and solution halves
+  static constexpr int kTmp = 16;
+  Field *tmp[3][kTmp] = {};
+  Field *full_tmp = nullptr;    // F64 full field: the red-black solve's residual check
+  SolverWork *work = nullptr;   // solver state, created on the first solve
+  uint64_t epoch = 0;           // bumped whenever links, mass or matrices change (invalidates graphs)
+  Op *link_owner = nullptr;     // clones read this operator's links and own none
 };
 
 // ---------------- launchers (gx_kernels.hip) ----------------
+// full field of Gaussian complex numbers with the host generator's counter numbering
+template <typename T>
+int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st);
+// order-independent 64-bit digest of nwords 64-bit words: sum over i of mix(word_i, i) mod 2^64
+int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out);
 // All pointers are device pointers to half fields in the layout above; T = double or float.
 template <typename T>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st);
-// out = A_chi(in) per chirality (+ dphi[s] * phi if phi), mats = 2*Ls*Ls device matrix in T
+// hop with an epilogue: epi 0 out = hop; 1 out = alpha hop + beta phi; 2 out = alpha A(hop) + beta phi
+// with A a per-chirality Ls x Ls matrix (2*Ls*Ls in T) applied across each site's s-row
+template <typename T>
+int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
+                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta);
+// out = A_chi(in) per chirality (+ phi if phi), mats = 2*Ls*Ls device matrix in T
 template <typename T>
 int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
                 hipStream_t st);
+// out = A(in) + beta phi and, if out2, out2 = C(in) from one read of in
+template <typename T>
+int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, void *out, const void *mats, void *out2,
+                 const void *mats2, const Gate *gate, hipStream_t st);
 // z = a x + b y over n complex numbers (a, b real)
 template <typename T>
 int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const Gate *gate,
@@ -182,6 +210,9 @@ int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out);
 void field_destroy(Field *f);
 Field *op_tmp(Op *op, int prec, int i);
 int op_gauge(Op *op, int prec, void **Ue, void **Uo);
+bool same_geom(const Geom &a, const Geom &b);
+Field half_view(const Field *full, int cb);  // non-owning view of one checkerboard of a full field
+int op_prepare(Op *op, int prec);  // allocate lazily created resources of a precision ahead of capture
 int op_smat(Op *op, int prec, int which, const void **mats);
 int reduce_read(Ctx *ctx, int slot, int n, double *out);
 

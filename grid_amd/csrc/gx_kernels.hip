@@ -13,6 +13,7 @@
 //   reductions: per-site products in the field precision, sums in double
 //   (Grid/lattice/Lattice_reduction.h:255-287), in a fixed order.
 #include <climits>
+#include <cstring>
 
 #include "gx_internal.h"
 
@@ -254,18 +255,33 @@ __device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const
 #undef LEG
 }
 
+// Epilogue of the hop.  EPI 0: out = hop; 1: out = alpha hop + beta phi; 2: out = alpha A(hop);
+// 3: out = alpha A(hop) + beta phi, with A a per-chirality Ls x Ls matrix applied across the s-row
+// of each site from LDS and phi in the layout of out.  Whether phi is read is a template choice: a
+// run-time null test inside the unrolled epilogue makes the compiler spill the whole hop.
+template <typename T>
+struct HopEpi {
+  const cx<T> *phi;
+  const T *mat;   // EPI 2, 3: [2][LS][LS], chirality 0 = spins 0,1
+  T alpha, beta;
+};
+
 // One lane per (4D half-site h, s): s fastest, 64/LS sites per wave, 4 waves per workgroup, so a
 // wave reads each neighbour's s-row contiguously and the lanes of one site share its links.
 // flags: bit 0 = non-temporal output stores.  ORD: leg order (0 Grid's, 1 paired directions).
-template <typename T, int LS, bool DAG, bool PF, int ORD>
+template <typename T, int LS, bool DAG, bool PF, int ORD, int EPI>
 __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                               const cx<T> *__restrict__ U, KGeom g, int cb_out,
-                                              const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags) {
+                                              const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags,
+                                              HopEpi<T> ep) {
   if (gated(gate)) return;
   constexpr int SPW = 64 / LS;
   constexpr int SPB = 4 * SPW;
   constexpr bool LDSU = (LS > 1) && (SPB * kLinkPad * sizeof(cx<T>) <= 40960);
-  __shared__ cx<T> ulds[LDSU ? SPB * kLinkPad : 1];
+  constexpr int NLINK = LDSU ? SPB * kLinkPad : 1;
+  constexpr bool MAT = EPI >= 2, PHI = (EPI & 1) != 0;
+  constexpr int NSTAGE = MAT ? 4 * 12 * 64 : 1;
+  __shared__ cx<T> lds[NLINK > NSTAGE ? NLINK : NSTAGE];  // links, then (MAT) the hop results
   const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, map);
   if (tile >= ntiles) return;  // uniform across the workgroup
   const uint32_t H4 = (uint32_t)g.H4, stride = (uint32_t)g.stride;
@@ -276,7 +292,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
   const int s = lane - ls * LS;
   const int lsb = w * SPW + ls;
   const bool valid = (ls < SPW) && (h0 + lsb < H4);
-  const uint32_t h = valid ? h0 + lsb : h0;  // idle lanes compute in-bounds addresses and exit later
+  const uint32_t h = valid ? h0 + lsb : h0;  // idle lanes compute in-bounds addresses and skip the work
   uint32_t r = h;
   const uint32_t Lxh = (uint32_t)g.Lxh;
   const int xh = (int)(r % Lxh);
@@ -320,59 +336,119 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
     for (int j = 0; j < NU; j++) {
       const uint32_t e = threadIdx.x + 256 * j;
       const uint32_t pk = e / SPB, site = e - pk * SPB;
-      if (e < SPB * 72) ulds[site * kLinkPad + pk] = ureg[j];
+      if (e < SPB * 72) lds[site * kLinkPad + pk] = ureg[j];
     }
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
   }
-  if (!valid) return;
-  const cx<T> *ul = ulds + (LDSU ? lsb * kLinkPad : 0);
+  if (!MAT && !valid) return;
+  const cx<T> *ul = lds + (LDSU ? lsb * kLinkPad : 0);
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  hop_site<T, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4, stride);
-  store_spinor<T>(out, acc, h * LS + s, stride, (flags & 1) != 0);
+  if (valid) hop_site<T, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4, stride);
+  const uint32_t o = h * LS + s;
+  if (MAT) {
+    // every wave has finished reading links: reuse the LDS for the hop results [w][k][lane]
+    __syncthreads();
+    cx<T> *stg = lds + w * 12 * 64;
+#pragma unroll
+    for (int k = 0; k < 12; k++) stg[k * 64 + lane] = acc[k];
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = {0, 0};
+    const T *ap = ep.mat + s * LS, *am = ep.mat + LS * LS + s * LS;
+#pragma unroll
+    for (int tt = 0; tt < LS; tt++) {
+      const T cp = ap[tt], cm = am[tt];
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        const cx<T> v = stg[k * 64 + ls * LS + tt];
+        const T c = k < 6 ? cp : cm;
+        acc[k].r = fma(c, v.r, acc[k].r);
+        acc[k].i = fma(c, v.i, acc[k].i);
+      }
+    }
+  }
+  if (EPI != 0) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      cx<T> v = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
+      if (PHI) {
+        const cx<T> p = ep.phi[k * stride + o];
+        v.r = fma(ep.beta, p.r, v.r);
+        v.i = fma(ep.beta, p.i, v.i);
+      }
+      acc[k] = v;
+    }
+  }
+  store_spinor<T>(out, acc, o, stride, (flags & 1) != 0);
 }
 
+#ifndef GX_LS_CASES
 #define GX_LS_CASES(M) \
   M(1) M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12) M(13) M(14) M(15) M(16)
+#endif
 
+// epi: 0 out = hop; 1 out = alpha hop + beta phi; 2 out = alpha A(hop) [+ beta phi when phi is
+// given] (A: mat)
 template <typename T>
-int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
-                hipStream_t st) {
+int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
+                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta) {
   const KGeom k = kgeom(g);
   if (12 * 2 * k.stride >= (int64_t)1 << 31) {
     set_error("dhop: lattice too large for 32-bit site indexing");
     return GX_EINVAL;
   }
+  if (epi == 2 && g.Ls < 2) {
+    set_error("dhop: matrix epilogue needs Ls >= 2");
+    return GX_EINVAL;
+  }
+  if ((epi == 1 && !phi) || (epi == 2 && !mat)) {
+    set_error("dhop: epilogue operand missing");
+    return GX_EINVAL;
+  }
+  const int kepi = (epi == 2 && phi) ? 3 : epi;
   const int map = tune_get(GX_TUNE_DHOP_MAP);
   const bool pf = tune_get(GX_TUNE_DHOP_PF) != 0;
   const int flags = tune_get(GX_TUNE_DHOP_NT) ? 1 : 0;
-  const bool ord = tune_get(GX_TUNE_DHOP_ORDER) != 0 && (g.Ls == 1 || g.Ls == 12 || g.Ls == 16);
+  const bool ord = epi == 0 && tune_get(GX_TUNE_DHOP_ORDER) != 0 && (g.Ls == 1 || g.Ls == 12 || g.Ls == 16);
   const cx<T> *pin = (const cx<T> *)in;
   cx<T> *pout = (cx<T> *)out;
   const cx<T> *pU = (const cx<T> *)U;
+  HopEpi<T> ep{(const cx<T> *)phi, (const T *)mat, (T)alpha, (T)beta};
   const int spb = 4 * (64 / g.Ls);
   const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
   const uint32_t blocks = map == 1 ? ((ntiles + 7) / 8) * 8 : ntiles;
-#define GX_DHOP_GO(LS, DAG, PF, ORD)                                                                       \
-  hipLaunchKernelGGL((k_dhop<T, LS, DAG, PF, ORD>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
-                     gate, ntiles, map, flags)
-#define GX_DHOP_CASE_O(LS, ORD)                   \
-  if (dag) {                                      \
-    if (pf) GX_DHOP_GO(LS, true, true, ORD);      \
-    else GX_DHOP_GO(LS, true, false, ORD);        \
-  } else {                                        \
-    if (pf) GX_DHOP_GO(LS, false, true, ORD);     \
-    else GX_DHOP_GO(LS, false, false, ORD);       \
+#define GX_DHOP_GO(LS, DAG, PF, ORD, EPI)                                                                         \
+  hipLaunchKernelGGL((k_dhop<T, LS, DAG, PF, ORD, EPI>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
+                     gate, ntiles, map, flags, ep)
+#define GX_DHOP_CASE_O(LS, ORD, EPI)                   \
+  if (dag) {                                           \
+    if (pf) GX_DHOP_GO(LS, true, true, ORD, EPI);      \
+    else GX_DHOP_GO(LS, true, false, ORD, EPI);        \
+  } else {                                             \
+    if (pf) GX_DHOP_GO(LS, false, true, ORD, EPI);     \
+    else GX_DHOP_GO(LS, false, false, ORD, EPI);       \
   }
 #define GX_DHOP_CASE(LS)                                         \
   case LS:                                                       \
-    if (ord && (LS == 1 || LS == 12 || LS == 16)) {              \
+    if (kepi == 1) {                                             \
+      GX_DHOP_CASE_O(LS, 0, 1)                                   \
+    } else if (kepi == 2) {                                      \
+      if constexpr (LS >= 2) {                                   \
+        GX_DHOP_CASE_O(LS, 0, 2)                                 \
+      }                                                          \
+    } else if (kepi == 3) {                                      \
+      if constexpr (LS >= 2) {                                   \
+        GX_DHOP_CASE_O(LS, 0, 3)                                 \
+      }                                                          \
+    } else if (ord && (LS == 1 || LS == 12 || LS == 16)) {       \
       if constexpr (LS == 1 || LS == 12 || LS == 16) {           \
-        GX_DHOP_CASE_O(LS, 1)                                    \
+        GX_DHOP_CASE_O(LS, 1, 0)                                 \
       }                                                          \
     } else {                                                     \
-      GX_DHOP_CASE_O(LS, 0)                                      \
+      GX_DHOP_CASE_O(LS, 0, 0)                                   \
     }                                                            \
     break;
   switch (g.Ls) {
@@ -388,10 +464,19 @@ int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_
   return GX_OK;
 }
 
-// out_s = sum_t A_chi[s][t] in_t  (+ phi_s).  mats: [2][LS][LS] in T, chirality 0 = spins 0,1.
-template <typename T, int LS>
-__global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, const cx<T> *__restrict__ phi,
-                                              cx<T> *__restrict__ out, const T *__restrict__ mats, int64_t H4,
+template <typename T>
+int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
+                hipStream_t st) {
+  return launch_dhop_epi<T>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0);
+}
+
+// out_s = sum_t A_chi[s][t] in_t (+ beta phi_s with PHI), and with OUT2 also
+// out2_s = sum_t C_chi[s][t] in_t from the same staged input.  mats: [2][LS][LS] in T,
+// chirality 0 = spins 0,1.  One lane per (h, s); the site's s-row is staged in LDS.
+template <typename T, int LS, bool OUT2, bool PHI>
+__global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, const cx<T> *__restrict__ phi, T beta,
+                                              cx<T> *__restrict__ out, const T *__restrict__ mats,
+                                              cx<T> *__restrict__ out2, const T *__restrict__ mats2, int64_t H4,
                                               int64_t stride, const Gate *__restrict__ gate) {
   if (gated(gate)) return;
   constexpr int SPW = 64 / LS;
@@ -408,45 +493,69 @@ __global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, cons
   for (int k = 0; k < 12; k++) lds[w][k][lane] = valid ? in[k * stride + o] : cx<T>{0, 0};
   __syncthreads();
   if (!valid) return;
-  T ap[LS], am[LS];
+  cx<T> acc[12], acc2[OUT2 ? 12 : 1];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc[k] = {0, 0};
+  if (OUT2)
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc2[k] = {0, 0};
 #pragma unroll
   for (int t = 0; t < LS; t++) {
-    ap[t] = mats[s * LS + t];
-    am[t] = mats[LS * LS + s * LS + t];
+    const T ap = mats[s * LS + t], am = mats[LS * LS + s * LS + t];
+    T cp = 0, cm = 0;
+    if (OUT2) {
+      cp = mats2[s * LS + t];
+      cm = mats2[LS * LS + s * LS + t];
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      const cx<T> v = lds[w][k][ls * LS + t];
+      const T a = k < 6 ? ap : am;
+      acc[k].r = fma(a, v.r, acc[k].r);
+      acc[k].i = fma(a, v.i, acc[k].i);
+      if (OUT2) {
+        const T c = k < 6 ? cp : cm;
+        acc2[k].r = fma(c, v.r, acc2[k].r);
+        acc2[k].i = fma(c, v.i, acc2[k].i);
+      }
+    }
   }
 #pragma unroll
   for (int k = 0; k < 12; k++) {
-    T re = 0, im = 0;
-#pragma unroll
-    for (int t = 0; t < LS; t++) {
-      const cx<T> v = lds[w][k][ls * LS + t];
-      const T a = (k < 6) ? ap[t] : am[t];
-      re = fma(a, v.r, re);
-      im = fma(a, v.i, im);
-    }
-    if (phi) {
+    cx<T> v = acc[k];
+    if (PHI) {
       const cx<T> p = phi[k * stride + o];
-      re += p.r;
-      im += p.i;
+      v.r = fma(beta, p.r, v.r);
+      v.i = fma(beta, p.i, v.i);
     }
-    out[k * stride + o] = {re, im};
+    out[k * stride + o] = v;
+    if (OUT2) out2[k * stride + o] = acc2[k];
   }
 }
 
 template <typename T>
-int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
-                hipStream_t st) {
+int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, void *out, const void *mats, void *out2,
+                 const void *mats2, const Gate *gate, hipStream_t st) {
   const int spw = 64 / g.Ls;
   const int64_t waves = (g.H4 + spw - 1) / spw;
   const int64_t blocks = (waves * 64 + 255) / 256;
   const int64_t stride = g.H4 * g.Ls;
   const cx<T> *pin = (const cx<T> *)in;
   const cx<T> *pphi = (const cx<T> *)phi;
-  cx<T> *pout = (cx<T> *)out;
-  const T *pm = (const T *)mats;
-#define GX_SMAT_CASE(LS)                                                                                   \
-  case LS:                                                                                                 \
-    hipLaunchKernelGGL((k_smat<T, LS>), dim3(blocks), dim3(256), 0, st, pin, pphi, pout, pm, g.H4, stride, gate); \
+  cx<T> *pout = (cx<T> *)out, *pout2 = (cx<T> *)out2;
+  const T *pm = (const T *)mats, *pm2 = (const T *)mats2;
+#define GX_SMAT_GO(LS, OUT2, PHI)                                                                           \
+  hipLaunchKernelGGL((k_smat<T, LS, OUT2, PHI>), dim3(blocks), dim3(256), 0, st, pin, pphi, (T)beta, pout, pm, \
+                     pout2, pm2, g.H4, stride, gate)
+#define GX_SMAT_CASE(LS)                    \
+  case LS:                                  \
+    if (out2) {                             \
+      if (phi) GX_SMAT_GO(LS, true, true);  \
+      else GX_SMAT_GO(LS, true, false);     \
+    } else {                                \
+      if (phi) GX_SMAT_GO(LS, false, true); \
+      else GX_SMAT_GO(LS, false, false);    \
+    }                                       \
     break;
   switch (g.Ls) {
     GX_LS_CASES(GX_SMAT_CASE)
@@ -455,8 +564,15 @@ int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const
       return GX_EINVAL;
   }
 #undef GX_SMAT_CASE
+#undef GX_SMAT_GO
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
+}
+
+template <typename T>
+int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
+                hipStream_t st) {
+  return launch_smat2<T>(g, in, phi, 1.0, out, mats, nullptr, nullptr, gate, st);
 }
 
 // ---------------- BLAS1 ----------------
@@ -916,7 +1032,91 @@ int launch_slice_proj(const Geom &gx_, const void *x, int Lsx, int sx, void *z, 
   return GX_OK;
 }
 
+// ---------------- synthetic sources and field digests ----------------
+__device__ __forceinline__ uint64_t d_splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ double d_u01(uint64_t key, uint64_t ctr) {
+  const uint64_t r = d_splitmix64(key ^ d_splitmix64(ctr));
+  return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Element (cb, h, s, k) of a full field gets the Box-Muller pair of counter (lex * Ls + s) * 12 + k,
+// lex the lexicographic 4D site: the numbering of the host generator gx_gaussian_fermion, so the
+// two agree up to the rounding of log, sqrt, sin and cos.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gaussian(cx<T> *__restrict__ out, KGeom g, int Ls, uint64_t key) {
+  const int64_t per_cb = g.H4 * Ls;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < 2 * per_cb;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cb = j / per_cb, rem = j - cb * per_cb;
+    const int64_t h = rem / Ls;
+    const int s = (int)(rem - h * Ls);
+    int64_t r = h;
+    const int xh = (int)(r % g.Lxh);
+    r /= g.Lxh;
+    const int y = (int)(r % g.L[1]);
+    r /= g.L[1];
+    const int z = (int)(r % g.L[2]);
+    const int t = (int)(r / g.L[2]);
+    const int x = 2 * xh + (int)((cb + y + z + t) & 1);
+    const int64_t lex = x + (int64_t)g.L[0] * (y + (int64_t)g.L[1] * (z + (int64_t)g.L[2] * t));
+    const uint64_t c0 = (uint64_t)(lex * Ls + s) * 12;
+    cx<T> *o = out + cb * 12 * per_cb + rem;
+#pragma unroll 2
+    for (int k = 0; k < 12; k++) {
+      const double a = d_u01(key, 2 * (c0 + k)), b = d_u01(key, 2 * (c0 + k) + 1);
+      const double rad = sqrt(-2.0 * log(a));
+      double sn, cs;
+      sincos(2.0 * M_PI * b, &sn, &cs);
+      o[k * per_cb] = {(T)(rad * cs), (T)(rad * sn)};
+    }
+  }
+}
+
+template <typename T>
+int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st) {
+  const KGeom k = kgeom(g);
+  hipLaunchKernelGGL((k_gaussian<T>), dim3(stream_blocks(2 * g.H4 * g.Ls)), dim3(256), 0, st, (cx<T> *)full, k, g.Ls,
+                     key);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+// digest = sum_i splitmix64(word_i ^ splitmix64(i)) mod 2^64: integer addition is exact and
+// commutative, so the result is independent of the order in which blocks finish.
+__global__ void __launch_bounds__(256) k_digest(const uint64_t *__restrict__ w, int64_t n, unsigned long long *out) {
+  uint64_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += d_splitmix64(w[i] ^ d_splitmix64((uint64_t)i));
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  __shared__ uint64_t part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
+int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
+  GX_CHECK_HIP(hipMemsetAsync(ctx->digest, 0, sizeof(unsigned long long), ctx->stream));
+  hipLaunchKernelGGL(k_digest, dim3(stream_blocks(nwords)), dim3(256), 0, ctx->stream, (const uint64_t *)data, nwords,
+                     ctx->digest);
+  GX_CHECK_HIP(hipGetLastError());
+  GX_CHECK_HIP(hipMemcpyAsync(ctx->pinned, ctx->digest, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(out, ctx->pinned, sizeof(uint64_t));
+  return GX_OK;
+}
+
 #define GX_INST(T)                                                                                                 \
+  template int launch_dhop_epi<T>(const Geom &, const void *, void *, const void *, int, int, const Gate *,       \
+                                  hipStream_t, int, const void *, const void *, double, double);                  \
+  template int launch_smat2<T>(const Geom &, const void *, const void *, double, void *, const void *, void *,     \
+                               const void *, const Gate *, hipStream_t);                                         \
   template int launch_slice_proj<T>(const Geom &, const void *, int, int, void *, int, int, int, int, hipStream_t); \
   template int launch_mrhs_reduce<T>(Ctx *, int, const Geom &, void *, const void *, void *, MrhsState *, double *); \
   template int launch_mrhs_update<T>(Ctx *, const Geom &, void *, void *, const void *, MrhsState *);             \
@@ -929,7 +1129,8 @@ int launch_slice_proj(const Geom &gx_, const void *x, int Lsx, int sx, void *z, 
   template int launch_reduce<T>(Ctx *, int, int64_t, int64_t, const void *, const void *, void *, double, int);   \
   template int launch_cg_dot<T>(Ctx *, int64_t, int64_t, const void *, const void *, CGState *);                   \
   template int launch_cg_axpy_norm<T>(Ctx *, int64_t, int64_t, void *, const void *, CGState *);                   \
-  template int launch_cg_update<T>(Ctx *, int64_t, void *, void *, const void *, CGState *);
+  template int launch_cg_update<T>(Ctx *, int64_t, void *, void *, const void *, CGState *);                   \
+  template int launch_gaussian<T>(const Geom &, void *, uint64_t, hipStream_t);
 GX_INST(double)
 GX_INST(float)
 template int launch_convert<double, float>(int64_t, const void *, void *, hipStream_t);

@@ -25,7 +25,7 @@ namespace gx {
 // with, the batch length or the launch-shape knobs change.
 struct BatchGraph {
   hipGraphExec_t exec = nullptr;
-  const void *key[6] = {};
+  const void *key[8] = {};
   int batch = 0, version = -1;
 };
 
@@ -33,12 +33,12 @@ struct BatchGraph {
 // body() into a hipGraph and every call replays it.  The iterations contain kernel launches only
 // (no memset or copy nodes); their device-side gate makes replays past convergence no-ops.
 template <class F>
-static int run_batch(Ctx *ctx, BatchGraph &g, const void *const key[6], int batch, F body) {
+static int run_batch(Ctx *ctx, BatchGraph &g, const void *const key[8], int batch, F body) {
   const int ver = tune_version();
   const bool hit = g.exec && g.batch == batch && g.version == ver && std::memcmp(g.key, key, sizeof(g.key)) == 0;
   if (!hit) {
     if (g.exec) {
-      hipGraphExecDestroy(g.exec);
+      (void)hipGraphExecDestroy(g.exec);
       g.exec = nullptr;
     }
     hipGraph_t graph = nullptr;
@@ -46,12 +46,12 @@ static int run_batch(Ctx *ctx, BatchGraph &g, const void *const key[6], int batc
     const int rc = body();
     const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (rc != GX_OK) {
-      if (e == hipSuccess && graph) hipGraphDestroy(graph);
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
       return rc;
     }
     GX_CHECK_HIP(e);
     const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     GX_CHECK_HIP(ie);
     std::memcpy(g.key, key, sizeof(g.key));
     g.batch = batch;
@@ -68,17 +68,69 @@ struct CGWork {
   BatchGraph graph[3];  // per precision
 };
 
-static CGWork &cg_work(Op *op) {
-  static thread_local std::vector<std::pair<Op *, CGWork>> cache;
-  for (auto &e : cache)
-    if (e.first == op) return e.second;
-  cache.emplace_back(op, CGWork());
-  CGWork &w = cache.back().second;
-  hipMalloc(&w.dev, sizeof(CGState));
-  hipHostMalloc(&w.host, 2 * sizeof(CGState), hipHostMallocDefault);
-  hipEventCreateWithFlags(&w.ev[0], hipEventDisableTiming);
-  hipEventCreateWithFlags(&w.ev[1], hipEventDisableTiming);
-  return w;
+struct MrhsWork {
+  MrhsState *dev = nullptr;
+  MrhsState *host = nullptr;  // pinned, 2 slots
+  double *out_dev = nullptr;  // per-slice results
+  double *out_host = nullptr; // pinned mirror
+  hipEvent_t ev[2] = {};
+  BatchGraph graph[3];  // per precision
+};
+
+struct SolverWork {
+  CGWork cg;
+  MrhsWork mrhs;
+};
+
+static void free_graphs(BatchGraph *g, int n) {
+  for (int i = 0; i < n; i++)
+    if (g[i].exec) (void)hipGraphExecDestroy(g[i].exec);
+}
+
+void solver_work_free(SolverWork *w) {
+  if (!w) return;
+  CGWork &c = w->cg;
+  MrhsWork &m = w->mrhs;
+  free_graphs(c.graph, 3);
+  free_graphs(m.graph, 3);
+  if (c.dev) (void)hipFree(c.dev);
+  if (c.host) (void)hipHostFree(c.host);
+  if (m.dev) (void)hipFree(m.dev);
+  if (m.out_dev) (void)hipFree(m.out_dev);
+  if (m.host) (void)hipHostFree(m.host);
+  if (m.out_host) (void)hipHostFree(m.out_host);
+  for (int i = 0; i < 2; i++) {
+    if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
+    if (m.ev[i]) (void)hipEventDestroy(m.ev[i]);
+  }
+  delete w;
+}
+
+// The operator's solver state, allocated on first use: device state, pinned mirrors and events.
+static int solver_work(Op *op, SolverWork **out) {
+  if (!op->work) {
+    SolverWork *w = new SolverWork();
+    CGWork &c = w->cg;
+    MrhsWork &m = w->mrhs;
+    hipError_t e = hipMalloc(&c.dev, sizeof(CGState));
+    if (e == hipSuccess) e = hipHostMalloc(&c.host, 2 * sizeof(CGState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&m.dev, sizeof(MrhsState));
+    if (e == hipSuccess) e = hipHostMalloc(&m.host, 2 * sizeof(MrhsState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&m.out_dev, sizeof(double) * kMaxLs);
+    if (e == hipSuccess) e = hipHostMalloc(&m.out_host, sizeof(double) * kMaxLs, hipHostMallocDefault);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) {
+      e = hipEventCreateWithFlags(&c.ev[i], hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&m.ev[i], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+      solver_work_free(w);
+      set_error("solver state: %s", hipGetErrorString(e));
+      return GX_ENOMEM;
+    }
+    op->work = w;
+  }
+  *out = op->work;
+  return GX_OK;
 }
 
 template <typename T>
@@ -95,6 +147,7 @@ static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGSt
 int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *out) {
   auto t0 = std::chrono::steady_clock::now();
   const int prec = src->prec;
+  GX_TRY(op_prepare(op, prec));
   Field *p = op_tmp(op, prec, 4), *mmp = op_tmp(op, prec, 5), *r = op_tmp(op, prec, 6);
   if (!p || !mmp || !r) return GX_ENOMEM;
   p->cb = mmp->cb = r->cb = psi->cb = src->cb;
@@ -135,7 +188,9 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
     if (out) *out = st;
     return GX_OK;
   }
-  CGWork &w = cg_work(op);
+  SolverWork *sw = nullptr;
+  GX_TRY(solver_work(op, &sw));
+  CGWork &w = sw->cg;
   CGState init{};
   init.c = cp;
   init.cp = cp;
@@ -164,7 +219,8 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
   };
   auto enqueue = [&](int s) -> int {
     if (use_graph) {
-      const void *key[6] = {psi->data, p->data, mmp->data, r->data, (const void *)(intptr_t)(prec + 1), w.dev};
+      const void *key[8] = {psi->data, p->data,   mmp->data, r->data, (const void *)(intptr_t)(prec + 1),
+                             w.dev,     (const void *)op, (const void *)(uintptr_t)op->epoch};
       GX_TRY(run_batch(ctx, w.graph[prec], key, batch, one_batch));
       enqueued += batch;
     } else {
@@ -260,33 +316,11 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
 }
 
 // ---------------- multi-RHS CG (BlockConjugateGradient.h:412-545, CGmultiRHS) ----------------
-struct MrhsWork {
-  MrhsState *dev = nullptr;
-  MrhsState *host = nullptr;  // pinned, 2 slots
-  double *out_dev = nullptr;  // per-slice results
-  double *out_host = nullptr; // pinned mirror
-  hipEvent_t ev[2] = {};
-  BatchGraph graph[3];  // per precision
-};
-
-static MrhsWork &mrhs_work(Op *op) {
-  static thread_local std::vector<std::pair<Op *, MrhsWork>> cache;
-  for (auto &e : cache)
-    if (e.first == op) return e.second;
-  cache.emplace_back(op, MrhsWork());
-  MrhsWork &w = cache.back().second;
-  hipMalloc(&w.dev, sizeof(MrhsState));
-  hipHostMalloc(&w.host, 2 * sizeof(MrhsState), hipHostMallocDefault);
-  hipMalloc(&w.out_dev, sizeof(double) * kMaxLs);
-  hipHostMalloc(&w.out_host, sizeof(double) * kMaxLs, hipHostMallocDefault);
-  hipEventCreateWithFlags(&w.ev[0], hipEventDisableTiming);
-  hipEventCreateWithFlags(&w.ev[1], hipEventDisableTiming);
-  return w;
-}
-
 // per-slice reduction that returns host values: mode 1 |x_s|^2, mode 3 z = y - x and |z_s|^2
 static int slice_reduce(Op *op, int mode, Field *x, Field *y, Field *z, double *out) {
-  MrhsWork &w = mrhs_work(op);
+  SolverWork *sw = nullptr;
+  GX_TRY(solver_work(op, &sw));
+  MrhsWork &w = sw->mrhs;
   Ctx *ctx = op->ctx;
   const int rc = x->prec == GX_F64
                      ? launch_mrhs_reduce<double>(ctx, mode, op->g, x->data, y ? y->data : nullptr,
@@ -314,6 +348,7 @@ static int mrhs_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, Mr
 int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *st) {
   auto t0 = std::chrono::steady_clock::now();
   const int Ls = op->g.Ls, prec = src->prec;
+  GX_TRY(op_prepare(op, prec));
   Field *p = op_tmp(op, prec, 4), *mmp = op_tmp(op, prec, 5), *r = op_tmp(op, prec, 6);
   if (!p || !mmp || !r) return GX_ENOMEM;
   p->cb = mmp->cb = r->cb = psi->cb = src->cb;
@@ -334,7 +369,9 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
     }
   Ctx *ctx = op->ctx;
   GX_CHECK_HIP(hipMemcpyAsync(p->data, r->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
-  MrhsWork &w = mrhs_work(op);
+  SolverWork *sw = nullptr;
+  GX_TRY(solver_work(op, &sw));
+  MrhsWork &w = sw->mrhs;
   MrhsState init{};
   double mx = 0;
   for (int s = 0; s < Ls; s++) {
@@ -373,7 +410,8 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
     };
     auto enqueue = [&](int sl) -> int {
       if (use_graph) {
-        const void *key[6] = {psi->data, p->data, mmp->data, r->data, (const void *)(intptr_t)(prec + 1), w.dev};
+        const void *key[8] = {psi->data, p->data,   mmp->data, r->data, (const void *)(intptr_t)(prec + 1),
+                             w.dev,     (const void *)op, (const void *)(uintptr_t)op->epoch};
         GX_TRY(run_batch(ctx, w.graph[prec], key, batch, one_batch));
         enqueued += batch;
       } else {
@@ -533,17 +571,17 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
                    double *unprec_residual) {
   Op *op = op_;
   Field *src = src_, *sol = sol_;
-  if (!op || !src || !sol || src->subset != GX_FULL || sol->subset != GX_FULL || src->prec != GX_F64 ||
-      sol->prec != GX_F64) {
-    set_error("schur_solve: needs full F64 fields");
+  if (!op || !src || !sol || src == sol || src->subset != GX_FULL || sol->subset != GX_FULL || src->prec != GX_F64 ||
+      sol->prec != GX_F64 || !same_geom(src->g, op->g) || !same_geom(sol->g, op->g)) {
+    set_error("schur_solve: needs two distinct full F64 fields on the operator's lattice");
     return GX_EINVAL;
   }
-  Field *se, *so, *t, *Mt, *solo;
-  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_EVEN, &se));
-  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_ODD, &so));
-  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_EVEN, &t));
-  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_EVEN, &Mt));
-  GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_ODD, &solo));
+  // the operator's own temporaries: repeated solves allocate (and free) nothing
+  Field *se = op_tmp(op, GX_F64, 10), *so = op_tmp(op, GX_F64, 11), *t = op_tmp(op, GX_F64, 12);
+  Field *Mt = op_tmp(op, GX_F64, 13), *solo = op_tmp(op, GX_F64, 14);
+  if (!se || !so || !t || !Mt || !solo) return GX_ENOMEM;
+  const bool multi = op->kind == GX_WILSON && op->g.Ls > 1;
+  const int Ls = op->g.Ls;
   hipStream_t s = op->ctx->stream;
   const size_t hb = se->bytes;
   int rc = GX_OK;
@@ -560,11 +598,27 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
     so->cb = 1;
     // RedBlackSolve with the zero guess
     solo->cb = 1;
-    hipMemsetAsync(solo->data, 0, solo->bytes, s);
-    if (mixed)
+    if ((rc = (hipMemsetAsync(solo->data, 0, solo->bytes, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
+    if (multi) {
+      // Ls independent right-hand sides: every one to tol (CGmultiRHS)
+      std::vector<gx_solver_stats> bst(Ls);
+      rc = mixed ? mrhs_mixed_run(op, so, solo, tol, 1e-5, maxit, 100, bst.data())
+                 : mrhs_cg_run(op, so, solo, tol, maxit, bst.data());
+      if (st) {
+        *st = bst[0];
+        for (int k = 1; k < Ls; k++) {
+          st->iterations = std::max(st->iterations, bst[k].iterations);
+          st->final_iterations = std::max(st->final_iterations, bst[k].final_iterations);
+          st->converged = st->converged && bst[k].converged;
+          st->true_residual = std::max(st->true_residual, bst[k].true_residual);
+          st->computed_residual = std::max(st->computed_residual, bst[k].computed_residual);
+        }
+      }
+    } else if (mixed) {
       rc = mixed_cg_run(op, so, solo, tol, 1e-5, GX_F32, maxit, 100, st);
-    else
+    } else {
       rc = cg_run(op, so, solo, tol, maxit, st);
+    }
     if (rc != GX_OK && rc != GX_ENOCONV) break;
     const int solve_rc = rc;
     // RedBlackSolution: sol_e = MooeeInv(src_e - Meo sol_o)
@@ -576,25 +630,34 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
     if ((rc = (hipMemcpyAsync(sol->half(0), t->data, hb, hipMemcpyDeviceToDevice, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
     if ((rc = (hipMemcpyAsync(sol->half(1), solo->data, hb, hipMemcpyDeviceToDevice, s) == hipSuccess) ? GX_OK : GX_EHIP)) break;
     if (unprec_residual) {
-      Field *res;
-      if ((rc = field_create(op->ctx, op->g, GX_F64, GX_FULL, &res))) break;
-      rc = apply_full_m(op, sol, res, 0);
-      double nr = 0, ns = 0;
-      if (!rc) rc = blas_axpby(res, 1.0, res, -1.0, src);
-      if (!rc) rc = blas_norm2(res, &nr);
-      if (!rc) rc = blas_norm2(src, &ns);
-      field_destroy(res);
-      if (rc) break;
-      *unprec_residual = std::sqrt(nr / ns);
+      if (!op->full_tmp && (rc = field_create(op->ctx, op->g, GX_F64, GX_FULL, &op->full_tmp))) break;
+      Field *res = op->full_tmp;
+      if ((rc = apply_full_m(op, sol, res, 0))) break;
+      if ((rc = blas_axpby(res, 1.0, res, -1.0, src))) break;
+      if (multi) {
+        // largest per-right-hand-side |M sol_k - src_k| / |src_k|, both checkerboards
+        std::vector<double> r0(Ls), r1(Ls), s0(Ls), s1(Ls);
+        Field rh0 = half_view(res, 0), rh1 = half_view(res, 1), sh0 = half_view(src, 0), sh1 = half_view(src, 1);
+        if ((rc = slice_reduce(op, 1, &rh0, nullptr, nullptr, r0.data()))) break;
+        if ((rc = slice_reduce(op, 1, &rh1, nullptr, nullptr, r1.data()))) break;
+        if ((rc = slice_reduce(op, 1, &sh0, nullptr, nullptr, s0.data()))) break;
+        if ((rc = slice_reduce(op, 1, &sh1, nullptr, nullptr, s1.data()))) break;
+        double worst = 0;
+        for (int k = 0; k < Ls; k++) {
+          const double den = s0[k] + s1[k];
+          worst = std::max(worst, den > 0 ? std::sqrt((r0[k] + r1[k]) / den) : std::sqrt(r0[k] + r1[k]));
+        }
+        *unprec_residual = worst;
+      } else {
+        double nr = 0, ns = 0;
+        if ((rc = blas_norm2(res, &nr))) break;
+        if ((rc = blas_norm2(src, &ns))) break;
+        *unprec_residual = std::sqrt(nr / ns);
+      }
     }
     rc = solve_rc;
   } while (0);
-  hipStreamSynchronize(s);
-  field_destroy(se);
-  field_destroy(so);
-  field_destroy(t);
-  field_destroy(Mt);
-  field_destroy(solo);
+  if (hipStreamSynchronize(s) != hipSuccess && (rc == GX_OK || rc == GX_ENOCONV)) rc = GX_EHIP;
   return rc;
 }
 

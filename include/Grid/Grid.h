@@ -625,7 +625,7 @@ class ConjugateGradient : public OperatorFunction<Field> {
   void operator()(LinearOperatorBase<Field> &Linop, const Field &src, Field &psi) override {
     psi.Checkerboard() = src.Checkerboard();
     if (gx_op op = Linop.schur_handle()) {
-      gx_solver_stats st;
+      gx_solver_stats st{};
       const int rc = gx_cg(op, src.f, psi.f, Tolerance, MaxIterations, &st);
       IterationsToComplete = st.iterations;
       TrueResidual = st.true_residual;
@@ -724,17 +724,17 @@ class MixedPrecisionConjugateGradient {
     gx_op op = Linop_d.schur_handle();
     assert(op && Linop_f.schur_handle());
     sol_d.Checkerboard() = src_d_in.Checkerboard();
-    gx_solver_stats st;
+    gx_solver_stats st{};
     const int rc = gx_mixed_cg(op, src_d_in.f, sol_d.f, Tolerance, InnerTolerance, GX_F32, MaxInnerIterations,
                                MaxOuterIterations, &st);
     TotalInnerIterations = st.inner_iterations;
     TotalOuterIterations = st.outer_iterations;
     TotalFinalStepIterations = st.final_iterations;
     TrueResidual = st.true_residual;
+    if (rc != GX_ENOCONV) detail::check(rc, "MixedPrecisionConjugateGradient");
     std::cout << GridLogMessage << "MixedPrecisionConjugateGradient: Inner CG iterations " << TotalInnerIterations
               << " Restarts " << TotalOuterIterations << " Final CG iterations " << TotalFinalStepIterations
               << " True residual " << TrueResidual << std::endl;
-    if (rc != GX_ENOCONV) detail::check(rc, "MixedPrecisionConjugateGradient");
   }
 };
 

@@ -21,6 +21,7 @@
 #ifndef GX_H
 #define GX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -101,6 +102,16 @@ int gx_mobius_create(gx_ctx ctx, const int dims[4], int Ls, double mass, double 
 int gx_op_destroy(gx_op op);
 int gx_op_import_gauge(gx_op op, const double *U);           /* Grid: ImportGauge */
 int gx_op_set_mass(gx_op op, double mass);                   /* Grid: CayleyFermion5D::SetMass / new op */
+/* No Grid counterpart (Grid copies the gauge field into every operator, WilsonFermion.h:161-169):
+ * a Wilson operator on base's 4D lattice that reads base's resident device links instead of
+ * holding its own, with its own context (stream), Ls (right-hand sides) and mass, so concurrent
+ * solves on one gauge field hold one copy of the links.  base must be a GX_WILSON operator
+ * and must outlive the clone; gx_op_import_gauge on a clone is rejected. */
+int gx_wilson_clone(gx_op base, gx_ctx ctx, int Ls, double mass, gx_op *out);
+/* Re-point a clone at another base operator's links (same 4D lattice and boundary phases). */
+int gx_op_share_links(gx_op clone, gx_op base);
+/* Device bytes the operator holds now (links it owns, fifth-dimension matrices, temporaries). */
+int gx_op_footprint(gx_op op, size_t *bytes);
 
 /* Operator applications; fields may be F64 or F32, in and out of the same precision.
  * Grid: FermionOperator.h:63-77; WilsonFermionImplementation.h:113-176,310-342;
@@ -163,7 +174,10 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
 
 /* Grid: SchurRedBlackDiagMooeeSolve<Field>(CG)(Dw, src, sol), SchurRedBlack.h:236-291,373-436.
  * src, sol: full F64 fields.  mixed != 0 uses gx_mixed_cg (inner FP32) for the red-black solve.
- * unprec_residual = |M sol - src| / |src|. */
+ * unprec_residual = |M sol - src| / |src|.  On a multi-RHS Wilson operator (Ls > 1) the
+ * red-black solve is gx_block_cg (every right-hand side to tol), st holds the largest iteration
+ * count and residuals over the right-hand sides, and unprec_residual is the largest
+ * per-right-hand-side |M sol_k - src_k| / |src_k|. */
 int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, int mixed,
                    gx_solver_stats *st, double *unprec_residual);
 
@@ -177,6 +191,59 @@ int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, 
 int gx_block_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, int inner_prec,
                 gx_solver_stats *st);
 
+/* ---- solve serving (no Grid counterpart) ----
+ * A request asks for the Wilson solves M x_k = b_k, k < nsrc, on one of the resident gauge fields
+ * of its lattice, with Gaussian sources b (gx_field_gaussian, Ls = nsrc, the request's seed).
+ * gx_serve replays a trace of requests: request i becomes visible arrival[i] seconds after the
+ * start (realtime = 1; all at the start with realtime = 0) and waits in FIFO order until a slot
+ * is idle and its device-memory footprint fits the budget; it then runs as a red-black
+ * (Schur) solve of all its sources at once (gx_schur_solve on a multi-RHS clone of the resident
+ * operator, gx_wilson_clone) on that slot's own stream and host thread, overlapping on the device
+ * with the solves of the other slots.  Resident gauge fields are built before the clock starts:
+ * field g of a lattice comes from gx_random_gauge(seed = gauge_seed + g, eps = gauge_eps). */
+typedef struct {
+  int dims[4];
+  int nsrc;          /* right-hand sides, 1..16 */
+  double mass;
+  int gauge;         /* resident gauge field, 0 .. n_gauge-1 */
+  uint64_t seed;     /* source seed */
+  double arrival;    /* seconds after the start */
+} gx_request;
+
+typedef struct {
+  int status;              /* GX_OK, GX_ENOCONV, GX_ENOMEM (larger than the whole budget), ... */
+  int slot;
+  int iterations;          /* largest over the right-hand sides, inner + patch-up */
+  int outer_iterations;    /* defect-correction restarts */
+  double residual;         /* largest |M x_k - b_k| / |b_k| recomputed in FP64 */
+  double solver_residual;  /* largest true residual of the red-black CG (normal equations) */
+  double t_admit, t_done;  /* seconds after the start */
+  double footprint;        /* bytes charged to the budget when first placed on its slot */
+  uint64_t digest;         /* gx_field_digest of the FP64 solution */
+} gx_response;
+
+typedef struct {
+  int device;
+  int slots;               /* concurrent solves: one stream and one host thread each */
+  double budget;           /* bytes of device memory for request work (resident links excluded) */
+  int n_gauge;             /* resident gauge fields per lattice */
+  uint64_t gauge_seed;
+  double gauge_eps;
+  double tol;              /* FP64 true-residual target of every right-hand side */
+  int mixed;               /* 1: FP32 inner solves inside FP64 defect correction */
+  int maxit;
+  int realtime;
+} gx_serve_params;
+
+/* Device bytes a request of nsrc sources on lattice dims holds while it runs (fields and the
+ * operator's temporaries; links excluded). */
+int gx_serve_footprint(const int dims[4], int nsrc, int mixed, double *bytes);
+/* resp: n entries.  solutions: NULL, or n pointers (each NULL or room for the full host field of
+ * the request's solution, layout of gx_field_export with Ls = nsrc).  wall: seconds from the start
+ * to the last completion; resident: device bytes of the resident links. */
+int gx_serve(const gx_serve_params *p, const gx_request *req, int n, gx_response *resp, double **solutions,
+             double *wall, double *resident);
+
 /* ---- launch-shape tuning (no Grid counterpart: Grid fixes its launch shape at build time,
  *      Grid/threads/Accelerator.h:416-461).  Process-wide knobs read by the kernel launchers;
  *      the autotuner in grid_amd/autotune.py sets them from measurements. ---- */
@@ -186,6 +253,7 @@ enum gx_tune_knob {
   GX_TUNE_DHOP_NT = 2,      /* 1: non-temporal (streaming) output stores */
   GX_TUNE_DHOP_ORDER = 3,   /* 0: Grid's leg order, 1: the two legs of each direction adjacent */
   GX_TUNE_CG_GRAPH = 4,     /* 1: solver iteration batches are captured once and replayed as hipGraphs */
+  GX_TUNE_FUSED = 5,        /* 1: Schur operator with fused hop epilogues (2-3 kernels per Mpc) */
   GX_TUNE_COUNT = 8
 };
 int gx_tune_set(int knob, int value);
@@ -198,6 +266,12 @@ int gx_tune_get(int knob, int *value);
 int gx_random_gauge(const int dims[4], uint64_t seed, double eps, double *U);
 int gx_gaussian_fermion(const int dims[4], int Ls, uint64_t seed, double *psi);   /* full field */
 int gx_point_source(const int dims[4], int Ls, const int x[4], int spin, int colour, double *psi);
+/* Device-side gx_gaussian_fermion straight into a full F64 or F32 field (same numbering; equal to
+ * the host field up to the rounding of log, sqrt, sin and cos). */
+int gx_field_gaussian(gx_field f, uint64_t seed);
+/* Order-independent 64-bit digest of a field's stored bits: equal digests <=> (with overwhelming
+ * probability) bitwise equal fields.  No Grid counterpart; used to compare solutions cheaply. */
+int gx_field_digest(gx_field f, uint64_t *digest);
 
 #ifdef __cplusplus
 }

@@ -146,6 +146,29 @@ def test_mobius_ops(ctx, Ls, dims, prec):
     D.close()
 
 
+@pytest.mark.parametrize('fused', [0, 1])
+@pytest.mark.parametrize('kind,Ls,dims', [('wilson', 1, (8, 8, 8, 8)), ('wilson', 12, (4, 6, 4, 8)),
+                                          ('mobius', 16, (8, 4, 4, 8)), ('mobius', 12, (4, 6, 4, 8)),
+                                          ('mobius', 5, (8, 4, 4, 4))])
+@pytest.mark.parametrize('prec', [gx.F64, gx.F32])
+def test_schur_fused_and_unfused(ctx, fused, kind, Ls, dims, prec):
+    U, psi = inputs(dims, Ls, 91, eps=0.3)
+    kw = dict(mass=0.05, M5=1.8, b=1.5, c=0.5)
+    O = Oracle(U, dims, Ls, kind, **kw)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, **kw)
+    po = half(psi, dims, ODD)
+    gx.tune_set(gx.TUNE_FUSED, fused)
+    try:
+        for name in ['Mpc', 'MpcDag', 'MpcDagMpc']:
+            i = D.field(prec, ODD).upload(po)
+            o = D.field(prec, ODD)
+            getattr(D, name)(i, o)
+            assert max_site_rel_err(o.download(), O.schur(name, po)) < TOL[prec] * 10, (name, fused)
+    finally:
+        gx.tune_set(gx.TUNE_FUSED, 1)
+    D.close()
+
+
 @pytest.mark.parametrize('prec', [gx.F64, gx.F32])
 def test_physical_source_maps(ctx, prec):
     dims, Ls = (8, 4, 4, 8), 8
@@ -247,7 +270,8 @@ def test_graph_replay_matches_eager(ctx, kind, Ls):
     U = gx.random_gauge(dims, 81, 0.4)
     D = gx.FermionOp(ctx, U, dims, Ls, kind, mass=0.1)
     srcs = [half(gx.gaussian_fermion(dims, Ls, 82 + i), dims, ODD) for i in range(3)]
-    solve = (lambda s, x: D.block_cg(s, x, tol=1e-9)) if Ls > 1 else (lambda s, x: D.cg(s, x, tol=1e-9))
+    multi = kind == 'wilson' and Ls > 1
+    solve = (lambda s, x: D.block_cg(s, x, tol=1e-9)) if multi else (lambda s, x: D.cg(s, x, tol=1e-9))
     eager = []
     gx.tune_set(gx.TUNE_CG_GRAPH, 0)
     try:
@@ -267,6 +291,41 @@ def test_graph_replay_matches_eager(ctx, kind, Ls):
                 fs.norm2()  # eager work between replays
     finally:
         gx.tune_set(gx.TUNE_CG_GRAPH, 1)
+
+
+@pytest.mark.parametrize('kind,Ls', [('wilson', 1), ('mobius', 4)])
+def test_graph_follows_mass_and_gauge(ctx, kind, Ls):
+    """A captured CG batch bakes in the links and the mass: after set_mass or import_gauge the
+    graph-mode solve must agree bitwise with an eager solve of the changed operator."""
+    dims = (8, 4, 4, 4)
+    U1, U2 = gx.random_gauge(dims, 91, 0.4), gx.random_gauge(dims, 92, 0.4)
+    src = half(gx.gaussian_fermion(dims, Ls, 93), dims, ODD)
+    D = gx.FermionOp(ctx, U1, dims, Ls, kind, mass=0.1)
+    fs, fx = D.field(gx.F64, ODD).upload(src), D.field(gx.F64, ODD)
+
+    def solve():
+        fx.zero()
+        D.cg(fs, fx, tol=1e-9)
+        ctx.sync()
+        return fx.download()
+
+    try:
+        gx.tune_set(gx.TUNE_CG_GRAPH, 1)
+        solve()                      # capture with (U1, m=0.1)
+        D.set_mass(0.2)
+        got_m = solve()
+        D.import_gauge(U2)
+        got_u = solve()
+        gx.tune_set(gx.TUNE_CG_GRAPH, 0)
+        D.import_gauge(U1)
+        want_m = solve()             # eager (U1, m=0.2)
+        D.import_gauge(U2)
+        want_u = solve()             # eager (U2, m=0.2)
+    finally:
+        gx.tune_set(gx.TUNE_CG_GRAPH, 1)
+    assert np.array_equal(got_m, want_m)
+    assert np.array_equal(got_u, want_u)
+    assert not np.array_equal(got_m, got_u)
 
 
 @pytest.mark.parametrize('inner', [gx.F64, gx.F32])

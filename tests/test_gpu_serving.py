@@ -1,0 +1,144 @@
+"""Serving path on the GPU: device sources, digests, link-sharing clones, the multi-RHS red-black
+solve, and the gx_serve runtime (continuous admission, overlap, alone-vs-mix identity)."""
+import numpy as np
+import pytest
+
+import grid_amd as gx
+from grid_amd import serving
+from conftest import rel_err
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def ctx():
+    c = gx.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize('prec,tol', [(gx.F64, 1e-13), (gx.F32, 1e-6)])
+def test_device_gaussian_matches_host(ctx, prec, tol):
+    dims, Ls = (8, 4, 6, 4), 3
+    host = gx.gaussian_fermion(dims, Ls, 4242)
+    f = gx.Field(ctx, dims, Ls, prec).gaussian(4242)
+    got = f.download()
+    assert np.max(np.abs(got - host)) <= tol * np.max(np.abs(host))
+    # the numbering is the host's: a different seed gives a different field
+    assert rel_err(gx.Field(ctx, dims, Ls, prec).gaussian(4243).download(), host) > 0.5
+
+
+def test_digest_tracks_bits(ctx):
+    dims = (4, 4, 4, 4)
+    a = gx.gaussian_fermion(dims, 2, 7)
+    f, g = gx.Field(ctx, dims, 2).upload(a), gx.Field(ctx, dims, 2).upload(a)
+    assert f.digest() == g.digest() == f.digest()
+    b = a.copy()
+    b.flat[123] = np.nextafter(b.flat[123].real, np.inf) + 1j * b.flat[123].imag  # one ulp
+    assert g.upload(b).digest() != f.digest()
+    c = a.copy()
+    c.flat[[5, 9]] = c.flat[[9, 5]]  # same multiset of values, other positions
+    assert g.upload(c).digest() != f.digest()
+
+
+def test_clone_matches_fresh_operator(ctx):
+    """A clone reading a resident operator's links computes bitwise what a freshly built operator
+    on the same links computes, holds no links of its own, and follows share_links."""
+    dims, nrhs, mass = (8, 4, 4, 8), 3, 0.17
+    U1, U2 = gx.random_gauge(dims, 31, 0.4), gx.random_gauge(dims, 32, 0.4)
+    base1 = gx.FermionOp(ctx, U1, dims, 1, 'wilson', mass=0.1)
+    base2 = gx.FermionOp(ctx, U2, dims, 1, 'wilson', mass=0.1)
+    ctx2 = gx.Context(0)
+    cl = base1.clone(ctx2, nrhs, mass)
+    src = gx.gaussian_fermion(dims, nrhs, 33)
+    for U, base in ((U1, None), (U2, base2)):
+        if base is not None:
+            cl.share_links(base)
+        fresh = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+        a_in, a_out = cl.field().upload(src), cl.field()
+        b_in, b_out = fresh.field().upload(src), fresh.field()
+        cl.M(a_in, a_out)
+        fresh.M(b_in, b_out)
+        ctx2.sync()
+        assert np.array_equal(a_out.download(), b_out.download())
+        assert cl.footprint() < fresh.footprint()
+    ctx2.close()
+
+
+def test_multi_rhs_schur_solve(ctx):
+    """Red-black solve of a multi-RHS Wilson operator: every right-hand side agrees with the
+    oracle's single-RHS solve and the reported residual is the worst per-RHS one."""
+    dims, nrhs, mass = (8, 8, 4, 4), 3, 0.12
+    U = gx.random_gauge(dims, 41, 0.4)
+    src = gx.gaussian_fermion(dims, nrhs, 42)
+    D = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+    O = Oracle(U, dims, 1, 'wilson', mass=mass)
+    for mixed in (False, True):
+        fs, fx = D.field().upload(src), D.field()
+        st = D.schur_solve(fs, fx, tol=1e-10, mixed=mixed)
+        x = fx.download()
+        assert st['converged'] and st['true_residual'] < 1e-9 and st['unprec_residual'] < 1e-8, st
+        worst = 0.0
+        for k in range(nrhs):
+            ref, _ = O.schur_solve(np.ascontiguousarray(src[..., k:k + 1, :, :]), tol=1e-10)
+            assert rel_err(x[..., k:k + 1, :, :], ref) < 1e-8
+            Mx = O.apply(np.ascontiguousarray(x[..., k:k + 1, :, :]), 'M')
+            worst = max(worst, rel_err(Mx, src[..., k:k + 1, :, :]))
+        assert abs(worst - st['unprec_residual']) <= 1e-3 * worst + 1e-15
+
+
+def test_serve_footprint_matches_allocation(ctx):
+    """The admission footprint equals what a served request really holds on the device."""
+    dims, nrhs = (8, 4, 4, 8), 5
+    base = gx.FermionOp(ctx, gx.random_gauge(dims, 51, 0.3), dims, 1, 'wilson', mass=0.1)
+    cl = base.clone(ctx, nrhs, 0.2)
+    fs, fx = cl.field().gaussian(52), cl.field()
+    cl.schur_solve(fs, fx, tol=1e-10, mixed=True)
+    held = cl.footprint() + 2 * (2 * 12 * 16 * np.prod(dims) // 2 * nrhs)
+    assert held == serving.footprint(dims, nrhs, mixed=True)
+
+
+SMALL = ((4, 4, 4, 4), (8, 4, 4, 4), (4, 4, 4, 8), (8, 8, 4, 4))
+
+
+def test_serve_alone_equals_mix(ctx):
+    """Continuous batching: requests admitted while others run give bitwise the result they give
+    alone, every one reaches its tolerance, and the served solutions agree with the oracle."""
+    reqs = serving.make_trace(14, rate=400.0, seed=77, volumes=SMALL, nsrc_range=(1, 4), n_gauge=2)
+    kw = dict(n_gauge=2, gauge_seed=900, gauge_eps=0.4, tol=1e-10, mixed=True)
+    mix, info = serving.serve(reqs, slots=3, budget=2e9, want_solutions=True, **kw)
+    assert all(r['status'] == 0 for r in mix), [r['status'] for r in mix]
+    assert len({r['slot'] for r in mix}) > 1
+    # some request was admitted before an earlier one finished: solves overlapped
+    assert any(b['t_admit'] < a['t_done'] for a, b in zip(mix, mix[1:]))
+    for i, r in enumerate(reqs):
+        alone, _ = serving.serve([dict(r, arrival=0.0)], slots=1, budget=2e9, realtime=False, **kw)
+        assert alone[0]['status'] == 0
+        assert alone[0]['digest'] == mix[i]['digest'], i
+        assert mix[i]['solver_residual'] < 1e-9 and mix[i]['residual'] < 1e-8, mix[i]
+    # two served solutions against the oracle (host gauge and sources from the same recipes)
+    for i in (0, len(reqs) - 1):
+        r = reqs[i]
+        U = gx.random_gauge(r['dims'], 900 + r['gauge'], 0.4)
+        src = gx.gaussian_fermion(r['dims'], r['nsrc'], r['seed'])
+        O = Oracle(U, r['dims'], 1, 'wilson', mass=r['mass'])
+        for k in range(r['nsrc']):
+            ref, _ = O.schur_solve(np.ascontiguousarray(src[..., k:k + 1, :, :]), tol=1e-10)
+            assert rel_err(mix[i]['solution'][..., k:k + 1, :, :], ref) < 1e-7
+    s = serving.summarize(mix, info)
+    assert s['completed'] == len(reqs) and s['p99_latency_s'] >= s['p50_latency_s'] > 0
+
+
+def test_serve_admission_respects_budget(ctx):
+    """With a budget that fits one request at a time, requests run one by one (no two solves
+    overlap) and a request larger than the whole budget is refused, not run."""
+    dims = (8, 4, 4, 4)
+    one = serving.footprint(dims, 2, mixed=True)
+    reqs = [dict(dims=dims, nsrc=2, mass=0.2, gauge=0, seed=10 + i, arrival=0.0) for i in range(4)]
+    reqs.append(dict(dims=(8, 8, 8, 8), nsrc=4, mass=0.2, gauge=0, seed=99, arrival=0.0))
+    res, _ = serving.serve(reqs, slots=3, budget=1.5 * one, n_gauge=1, realtime=False)
+    ok = sorted((r for r in res if r['status'] == 0), key=lambda r: r['t_admit'])
+    assert len(ok) == 4 and res[-1]['status'] == 2  # GX_ENOMEM
+    for a, b in zip(ok, ok[1:]):
+        assert b['t_admit'] >= a['t_done'] - 1e-6 or b['slot'] == a['slot']
