@@ -11,7 +11,17 @@ stream the kernels are launched on.  Also reported: GF/s at 1320 flop per 5D sit
 figure, Schur CG time to 1e-10 in FP64 and mixed FP32/FP64, and the CPU oracle's DhopEO on the
 same workload (cpu_baseline).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg] [--no-cpu]
+Further legs, reported beside the headline value (each can be switched off):
+  c3     Mobius 24^4 x Ls12 (mf 0.005): red-black solve to 1e-10 in FP64 CG and in defect
+         correction with FP32 and FP16-storage inner solves;
+  c4     Wilson 32^3x64 (m 0.05), the 12 spin-colour point sources of one site: one mixed-precision
+         multi-RHS solve against 12 sequential FP64 solves;
+  serve  a Poisson trace of Wilson solve requests (volumes 8^4 .. 24^4, 1-12 sources, masses
+         0.05-0.3, 8 resident gauge fields per volume) through the serving runtime: throughput and
+         p50/p99 latency (--serve-requests 4096 for the full trace).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg] [--no-cpu] [--no-c3] [--no-c4]
+                       [--serve-requests N] [--serve-rate R]
 """
 import argparse
 import json
@@ -83,6 +93,91 @@ def cpu_baseline(U, psi_o, seconds=12.0):
     return el / n, n, ncores
 
 
+def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10):
+    """Mobius 24^4 x Ls12 (b=1.5, c=0.5, M5=1.8, mf=0.005, eps 0.3), Gaussian 5D source: red-black
+    solve to 1e-10 in FP64 CG and in defect correction with FP32 and with FP16-storage inner solves
+    (inner tolerance 1e-5, FP64 outer residual and patch-up).  Wall clock around each solve."""
+    kw = dict(mass=0.005, M5=1.8, b=1.5, c=0.5)
+    U = gx.random_gauge(dims, 1003, 0.3)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    src = D.field().gaussian(2003)
+    out, ref = {'workload': 'Mobius 24^4 x Ls12 mf=0.005, Gaussian source'}, None
+    for name, mixed in (('fp64', 0), ('mixed_f32', 1), ('mixed_f16', 2)):
+        x = D.field()
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = D.schur_solve(src, x, tol=tol, maxit=50000, mixed=mixed)
+        ctx.sync()
+        sec = time.perf_counter() - t0
+        d = dict(seconds=round(sec, 4), iterations=st['iterations'], inner_iterations=st['inner_iterations'],
+                 outer=st['outer_iterations'], final_iterations=st['final_iterations'],
+                 true_residual=st['true_residual'], unprec_residual=st['unprec_residual'])
+        if ref is None:
+            ref = x
+        else:
+            t = D.field()
+            gx.axpby(t, 1.0, -1.0, x, ref)
+            d['rel_err_vs_fp64'] = (t.norm2() / ref.norm2()) ** 0.5
+            d['speedup_vs_fp64'] = round(out['fp64']['seconds'] / sec, 3)
+        out[name] = d
+    return out
+
+
+def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-10):
+    """Wilson 32^3x64 (eps 0.3, m 0.05), the 12 spin-colour point sources of one seeded site: one
+    mixed-precision multi-RHS red-black solve (FP32 CGmultiRHS inner solves sharing every link load
+    across the 12 sources, FP64 defect correction and patch-up) against 12 sequential FP64
+    red-black CG solves, all to 1e-10.  Wall clock around each solve."""
+    nrhs = 12
+    U = gx.random_gauge(dims, 1004, eps)
+    rng = np.random.default_rng(4004)
+    x0 = tuple(int(rng.integers(d)) for d in dims)
+    base = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=mass)
+    s1 = base.field()
+    seq, seq_sec, seq_it = [], 0.0, []
+    for k in range(nrhs):
+        s1.point_source(x0, k // 3, k % 3)
+        x = base.field()
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = base.schur_solve(s1, x, tol=tol, maxit=20000, mixed=False)
+        ctx.sync()
+        seq_sec += time.perf_counter() - t0
+        seq_it.append(st['iterations'])
+        seq.append(x)
+    B = base.clone(ctx, nrhs, mass)
+    sb, xb = B.field().point_source(x0), B.field()
+    ctx.sync()
+    t0 = time.perf_counter()
+    stb = B.schur_solve(sb, xb, tol=tol, maxit=20000, mixed=True)
+    ctx.sync()
+    blk_sec = time.perf_counter() - t0
+    t4, worst = base.field(), 0.0
+    for k in range(nrhs):
+        gx.extract_slice(t4, xb, k)
+        gx.axpby(t4, 1.0, -1.0, t4, seq[k])
+        worst = max(worst, (t4.norm2() / seq[k].norm2()) ** 0.5)
+    return {'workload': 'Wilson 32^3x64 m=0.05, 12 point sources', 'origin': list(x0),
+            'sequential_fp64': {'seconds': round(seq_sec, 4), 'per_rhs_seconds': round(seq_sec / nrhs, 4),
+                                'iterations': seq_it},
+            'multi_rhs_mixed': {'seconds': round(blk_sec, 4), 'per_rhs_seconds': round(blk_sec / nrhs, 4),
+                                'inner_iterations': stb['inner_iterations'], 'outer': stb['outer_iterations'],
+                                'final_iterations': stb['final_iterations'], 'true_residual': stb['true_residual'],
+                                'unprec_residual': stb['unprec_residual']},
+            'speedup': round(seq_sec / blk_sec, 3), 'max_rel_diff_vs_sequential': worst}
+
+
+def bench_serving(n, rate, slots=4):
+    """Poisson trace of n Wilson solve requests at `rate` per second through the serving runtime."""
+    from grid_amd import serving
+    trace = serving.make_trace(n, rate, seed=5000)
+    res, info = serving.serve(trace, slots=slots, budget=128e9, n_gauge=8, gauge_eps=0.3, tol=1e-10, mixed=True)
+    s = serving.summarize(res, info)
+    s.update({'arrival_rate_per_s': rate, 'slots': slots, 'admission_budget_bytes': 128e9,
+              'mix': 'volumes 8^4,12^4,16^4,16^3x32,24^4; 1-12 Gaussian sources; m in [0.05,0.3]; mixed CG to 1e-10'})
+    return {k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -90,6 +185,10 @@ def main():
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--no-cg', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-c3', action='store_true')
+    ap.add_argument('--no-c4', action='store_true')
+    ap.add_argument('--serve-requests', type=int, default=96)
+    ap.add_argument('--serve-rate', type=float, default=6.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU halo exchange '
@@ -103,11 +202,13 @@ def main():
     D = gx.FermionOp(ctx, U, DIMS, LS, 'mobius', **PARAMS)
     H = np.prod(DIMS) // 2
     res = {}
-    for prec, name, c in ((gx.F64, 'f64', 16), (gx.F32, 'f32', 8)):
+    for prec, name, c in ((gx.F64, 'f64', 16), (gx.F32, 'f32', 8), (gx.F16, 'f16', 4)):
         fi = D.field(prec, 1).upload(psi_o)
         fo = D.field(prec, 0)
         ms = time_dhop(ctx, D, fi, fo, args.steps, args.warmup)
-        byts = H * (24 * LS + 72) * c
+        # FP16 adds a 4-byte scale per 5D site to the input read and the output written (links
+        # are unscaled)
+        byts = H * (24 * LS + 72) * c + (H * LS * 8 if prec == gx.F16 else 0)
         res[name] = dict(ms=ms, bytes=int(byts), gbs=byts / (ms * 1e-3) / 1e9, gflops=1320.0 * H * LS / (ms * 1e-3) / 1e9)
         if prec == gx.F64:
             # one device-to-device copy of known size (12 H Ls complex read and written) after the
@@ -134,6 +235,10 @@ def main():
                                  true_residual=stmx['true_residual'], unprec_residual=stmx['unprec_residual'],
                                  rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))),
                   speedup_mixed_vs_fp64=st64['seconds'] / stmx['seconds'])
+
+    c3 = None if args.no_c3 else bench_mixed_dwf(gx, ctx)
+    c4 = None if args.no_c4 else bench_multi_rhs(gx, ctx)
+    serve = bench_serving(args.serve_requests, args.serve_rate) if args.serve_requests > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -165,7 +270,11 @@ def main():
         'pct_hbm_peak': round(100 * r['gbs'] / HBM_PEAK_GBS, 2),
         'gflops_1320': round(r['gflops'], 1),
         'f32': {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res['f32'].items()},
+        'f16': {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res['f16'].items()},
         'cg': cg,
+        'c3_mixed_dwf': c3,
+        'c4_multi_rhs': c4,
+        'serving': serve,
     }
     print(json.dumps(out))
 

@@ -134,6 +134,9 @@ _SIG = {
     'gx_op_footprint': ([_P, ctypes.POINTER(ctypes.c_size_t)], _I),
     'gx_field_gaussian': ([_P, ctypes.c_uint64], _I),
     'gx_field_digest': ([_P, ctypes.POINTER(ctypes.c_uint64)], _I),
+    'gx_field_point_source': ([_P, _IP, _I, _I], _I),
+    'gx_extract_slice': ([_P, _P, _I], _I),
+    'gx_insert_slice': ([_P, _P, _I], _I),
     'gx_serve_footprint': ([_IP, _I, _I, _DP], _I),
     'gx_serve': ([ctypes.POINTER(ServeParams), ctypes.POINTER(Request), _I, ctypes.POINTER(Response),
                   ctypes.POINTER(_DP), _DP, _DP], _I),
@@ -242,6 +245,12 @@ class Field:
         _check(lib().gx_field_gaussian(self.h, ctypes.c_uint64(seed)))
         return self
 
+    def point_source(self, x, spin=-1, colour=-1):
+        """Device point source at site x = (x, y, z, t): spin = colour = -1 puts spin-colour
+        component s in slice s (the 12 point sources of a site as one multi-RHS field)."""
+        _check(lib().gx_field_point_source(self.h, _dims(x), int(spin), int(colour)))
+        return self
+
     def digest(self):
         v = ctypes.c_uint64()
         _check(lib().gx_field_digest(self.h, ctypes.byref(v)))
@@ -273,6 +282,14 @@ def axpy_norm(z, a, x, y):
 
 def axpby(z, a, b, x, y):
     _check(lib().gx_axpby(z.h, a, b, x.h, y.h))
+
+
+def extract_slice(out4d, in5d, s):
+    _check(lib().gx_extract_slice(out4d.h, in5d.h, int(s)))
+
+
+def insert_slice(in4d, out5d, s):
+    _check(lib().gx_insert_slice(in4d.h, out5d.h, int(s)))
 
 
 def field_copy(dst, src):

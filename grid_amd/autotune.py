@@ -1,0 +1,167 @@
+"""Launch-shape autotuner for the hop stencil and the solver loop.
+
+The knobs are libgx's process-wide launch choices (gx_tune_set):
+  GX_TUNE_DHOP_MAP    workgroup -> lattice tile map (linear, or one contiguous slab per XCD)
+  GX_TUNE_DHOP_PF     software prefetch of the next leg's neighbours
+  GX_TUNE_DHOP_NT     non-temporal output stores
+  GX_TUNE_DHOP_ORDER  leg order (Grid's, or the two legs of a direction adjacent; Ls in {1,12,16})
+  GX_TUNE_FUSED       fused Schur operator (hop epilogues) or the unfused composition
+  GX_TUNE_CG_GRAPH    solver iteration batches replayed as hipGraphs
+The hop knobs are chosen by timing DhopEO on the caller's own lattice, Ls and precision (HIP
+events over repeated launches, best of several rounds); the solver knobs by timing a fixed number
+of CG iterations.  Results are cached in a JSON file keyed by device and workload, so a later
+process applies them without measuring.  No Grid counterpart: Grid fixes its launch shape when it
+is compiled (Grid/threads/Accelerator.h:416-461).
+"""
+import itertools
+import json
+import os
+import time
+
+import numpy as np
+
+from . import (EVEN, F16, F32, F64, ODD, TUNE_CG_GRAPH, TUNE_DHOP_MAP, TUNE_DHOP_NT, TUNE_DHOP_ORDER,
+               TUNE_DHOP_PF, TUNE_FUSED, tune_get, tune_set)
+
+HOP_KNOBS = (TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER)
+SOLVER_KNOBS = (TUNE_FUSED, TUNE_CG_GRAPH)
+KNOB_NAMES = {TUNE_DHOP_MAP: 'dhop_map', TUNE_DHOP_PF: 'dhop_pf', TUNE_DHOP_NT: 'dhop_nt',
+              TUNE_DHOP_ORDER: 'dhop_order', TUNE_FUSED: 'fused', TUNE_CG_GRAPH: 'cg_graph'}
+PREC_NAMES = {F64: 'f64', F32: 'f32', F16: 'f16'}
+DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'grid_amd', 'tune.json')
+
+
+def workload_key(device, kind, dims, Ls, prec):
+    return f'{device}|{kind}|{"x".join(str(d) for d in dims)}|Ls{Ls}|{PREC_NAMES[prec]}'
+
+
+def search(candidates, measure, rounds=3):
+    """Best candidate by the minimum over `rounds` of measure(candidate) (seconds); candidates are
+    measured round-robin so drift affects them alike.  Returns (best, {candidate: time})."""
+    best_t = {c: float('inf') for c in candidates}
+    for _ in range(rounds):
+        for c in candidates:
+            best_t[c] = min(best_t[c], measure(c))
+    best = min(candidates, key=lambda c: (best_t[c], candidates.index(c)))
+    return best, best_t
+
+
+def hop_candidates(Ls):
+    """Hop knob settings to try, each a tuple of (knob, value) pairs."""
+    orders = (0, 1) if Ls in (1, 12, 16) else (0,)
+    return [tuple(zip(HOP_KNOBS, v)) for v in itertools.product((0, 1), (0, 1), (0, 1), orders)]
+
+
+def solver_candidates():
+    return [tuple(zip(SOLVER_KNOBS, v)) for v in itertools.product((1, 0), (1, 0))]
+
+
+class Knobs:
+    """Set knobs for the duration of a with-block and restore the previous values."""
+
+    def __init__(self, values):
+        self.values = dict(values)
+
+    def __enter__(self):
+        self.saved = {k: tune_get(k) for k in self.values}
+        for k, v in self.values.items():
+            tune_set(k, v)
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.saved.items():
+            tune_set(k, v)
+
+
+def _device_name():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.get_device_properties(0).gcnArchName.split(':')[0]
+    except Exception:
+        pass
+    return 'gfx950'
+
+
+class Autotuner:
+    def __init__(self, ctx, cache_path=DEFAULT_CACHE, device=None):
+        self.ctx = ctx
+        self.cache_path = cache_path
+        self.device = device or _device_name()
+        self.cache = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.cache = json.load(f)
+
+    def save(self):
+        if not self.cache_path:
+            return
+        os.makedirs(os.path.dirname(self.cache_path), exist_ok=True)
+        tmp = self.cache_path + '.tmp'
+        with open(tmp, 'w') as f:
+            json.dump(self.cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.cache_path)
+
+    def _time_dhop(self, D, fi, fo, iters):
+        D.DhopEO(fi, fo, 0)
+        self.ctx.sync()
+        self.ctx.record(0)
+        for _ in range(iters):
+            D.DhopEO(fi, fo, 0)
+        self.ctx.record(1)
+        self.ctx.sync()
+        return self.ctx.elapsed_ms(0, 1) * 1e-3 / iters
+
+    def _time_cg(self, D, fs, fx, iters):
+        fx.zero()
+        self.ctx.sync()
+        t0 = time.perf_counter()
+        D.cg(fs, fx, tol=1e-30, maxit=iters)  # runs exactly maxit iterations
+        self.ctx.sync()
+        return (time.perf_counter() - t0) / iters
+
+    def tune(self, D, prec=F64, dhop_iters=20, cg_iters=64, rounds=3, force=False):
+        """Tune for operator D at precision prec; returns the chosen knob values (and applies them)."""
+        key = workload_key(self.device, D.kind, D.dims, D.Ls, prec)
+        if key in self.cache and not force:
+            return self.apply_entry(self.cache[key])
+        lt, lz, ly, lx = D.dims[3], D.dims[2], D.dims[1], D.dims[0]
+        rng = np.random.default_rng(1)
+        shape = (lt, lz, ly, lx // 2, D.Ls, 4, 3)
+        psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        fi, fo = D.field(prec, ODD).upload(psi), D.field(prec, EVEN)
+
+        def hop(c):
+            with Knobs(c):
+                return self._time_dhop(D, fi, fo, dhop_iters)
+
+        hop_best, hop_times = search(hop_candidates(D.Ls), hop, rounds)
+        fs, fx = D.field(prec, ODD).upload(psi), D.field(prec, ODD)
+
+        def cg(c):
+            with Knobs(hop_best + c):
+                return self._time_cg(D, fs, fx, cg_iters)
+
+        solver_best, solver_times = search(solver_candidates(), cg, rounds)
+        entry = {'knobs': {KNOB_NAMES[k]: v for k, v in hop_best + solver_best},
+                 'dhop_seconds': hop_times[hop_best],
+                 'cg_iteration_seconds': solver_times[solver_best],
+                 'measured': time.strftime('%Y-%m-%dT%H:%M:%S')}
+        self.cache[key] = entry
+        self.save()
+        for f in (fi, fo, fs, fx):
+            f.close()
+        return self.apply_entry(entry)
+
+    @staticmethod
+    def apply_entry(entry):
+        byname = {v: k for k, v in KNOB_NAMES.items()}
+        vals = {byname[n]: int(v) for n, v in entry['knobs'].items()}
+        for k, v in vals.items():
+            tune_set(k, v)
+        return vals
+
+    def apply(self, kind, dims, Ls, prec=F64):
+        """Apply cached knobs for a workload; returns them, or None when it was never tuned."""
+        e = self.cache.get(workload_key(self.device, kind, tuple(dims), Ls, prec))
+        return self.apply_entry(e) if e else None

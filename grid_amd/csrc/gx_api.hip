@@ -10,6 +10,7 @@
 //            red-black solve SchurRedBlack.h:236-291,385-425.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -106,7 +107,7 @@ static inline void coor_half(const Geom &g, int64_t h, int cb, int x[4]) {
 
 // ---------------- fields ----------------
 int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out) {
-  if (prec != GX_F64 && prec != GX_F32) {
+  if (prec != GX_F64 && prec != GX_F32 && prec != GX_F16) {
     set_error("field precision %d not supported", prec);
     return GX_EINVAL;
   }
@@ -116,7 +117,7 @@ int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out) {
   f->prec = prec;
   f->subset = subset;
   f->cb = subset == GX_FULL ? 0 : subset;
-  f->bytes = (size_t)f->elems() * f->csize();
+  f->bytes = f->half_bytes() * (subset == GX_FULL ? 2 : 1);
   hipError_t e = hipMalloc(&f->data, f->bytes);
   if (e != hipSuccess) {
     set_error("hipMalloc(%zu) failed: %s", f->bytes, hipGetErrorString(e));
@@ -183,18 +184,16 @@ static bool conformable(const Field *a, const Field *b) {
 }
 
 int blas_norm2(Field *x, double *nrm) {
-  int rc = x->prec == GX_F64
-               ? launch_reduce<double>(x->ctx, 1, nsites_of(x), x->g.half_elems(), x->data, nullptr, nullptr, 0, 0)
-               : launch_reduce<float>(x->ctx, 1, nsites_of(x), x->g.half_elems(), x->data, nullptr, nullptr, 0, 0);
-  GX_TRY(rc);
+  GX_TRY(by_prec(x->prec, [&](auto t) {
+    return launch_reduce<decltype(t)>(x->ctx, 1, nsites_of(x), x->g.half_elems(), x->data, nullptr, nullptr, 0, 0);
+  }));
   return reduce_read(x->ctx, 0, 1, nrm);
 }
 
 int blas_inner(Field *x, Field *y, double *re, double *im) {
-  int rc = x->prec == GX_F64
-               ? launch_reduce<double>(x->ctx, 0, nsites_of(x), x->g.half_elems(), x->data, y->data, nullptr, 0, 0)
-               : launch_reduce<float>(x->ctx, 0, nsites_of(x), x->g.half_elems(), x->data, y->data, nullptr, 0, 0);
-  GX_TRY(rc);
+  GX_TRY(by_prec(x->prec, [&](auto t) {
+    return launch_reduce<decltype(t)>(x->ctx, 0, nsites_of(x), x->g.half_elems(), x->data, y->data, nullptr, 0, 0);
+  }));
   double v[2];
   GX_TRY(reduce_read(x->ctx, 0, 2, v));
   *re = v[0];
@@ -203,19 +202,19 @@ int blas_inner(Field *x, Field *y, double *re, double *im) {
 }
 
 int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm) {
-  int rc = x->prec == GX_F64
-               ? launch_reduce<double>(x->ctx, 2, nsites_of(x), x->g.half_elems(), x->data, y->data, z->data, a, 0)
-               : launch_reduce<float>(x->ctx, 2, nsites_of(x), x->g.half_elems(), x->data, y->data, z->data, a, 0);
-  GX_TRY(rc);
+  GX_TRY(by_prec(x->prec, [&](auto t) {
+    return launch_reduce<decltype(t)>(x->ctx, 2, nsites_of(x), x->g.half_elems(), x->data, y->data, z->data, a, 0);
+  }));
   z->cb = x->cb;
   return reduce_read(x->ctx, 0, 1, nrm);
 }
 
 int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const Gate *gate) {
-  const int64_t n = x->elems();
   z->cb = x->cb;
-  if (x->prec == GX_F64) return launch_axpby<double>(n, a, x->data, b, y ? y->data : nullptr, z->data, gate, x->ctx->stream);
-  return launch_axpby<float>(n, a, x->data, b, y ? y->data : nullptr, z->data, gate, x->ctx->stream);
+  return by_prec(x->prec, [&](auto t) {
+    return launch_axpby<decltype(t)>(nsites_of(x), x->g.half_elems(), a, x->data, b, y ? y->data : nullptr, z->data,
+                                     gate, x->ctx->stream);
+  });
 }
 
 int convert_field(Field *out, Field *in) {
@@ -227,6 +226,12 @@ int convert_field(Field *out, Field *in) {
   }
   if (in->prec == GX_F64 && out->prec == GX_F32) return launch_convert<double, float>(n, in->data, out->data, in->ctx->stream);
   if (in->prec == GX_F32 && out->prec == GX_F64) return launch_convert<float, double>(n, in->data, out->data, in->ctx->stream);
+  const int64_t ns = nsites_of(in), st = in->g.half_elems();
+  hipStream_t s = in->ctx->stream;
+  if (in->prec == GX_F64 && out->prec == GX_F16) return launch_convert_site<double, half_t>(ns, st, in->data, out->data, s);
+  if (in->prec == GX_F16 && out->prec == GX_F64) return launch_convert_site<half_t, double>(ns, st, in->data, out->data, s);
+  if (in->prec == GX_F32 && out->prec == GX_F16) return launch_convert_site<float, half_t>(ns, st, in->data, out->data, s);
+  if (in->prec == GX_F16 && out->prec == GX_F32) return launch_convert_site<half_t, float>(ns, st, in->data, out->data, s);
   set_error("precision change %d -> %d unsupported", in->prec, out->prec);
   return GX_EINVAL;
 }
@@ -305,16 +310,43 @@ static int upload_gauge(Op *op) {
   return GX_OK;
 }
 
+// Links of precision prec, derived from the FP64 ones on first use (FP16: unscaled binary16).
+// A clone takes them from the operator that owns its links.
+static std::mutex g_share_mu;
 int op_gauge(Op *op, int prec, void **Ue, void **Uo) {
+  if (!op->U[prec][0] && op->link_owner) {
+    std::lock_guard<std::mutex> lk(g_share_mu);
+    Op *own = op->link_owner;
+    if (!own->U[prec][0]) {
+      const size_t n = (size_t)8 * 9 * own->g.H4;
+      if (prec != GX_F32 && prec != GX_F16) {
+        set_error("gauge precision %d unsupported", prec);
+        return GX_EINVAL;
+      }
+      for (int cb = 0; cb < 2; cb++) {
+        GX_CHECK_HIP(hipMalloc(&own->U[prec][cb], n * (prec == GX_F32 ? 8 : 4)));
+        if (prec == GX_F32)
+          GX_TRY((launch_convert<double, float>(2 * n, own->U[GX_F64][cb], own->U[prec][cb], own->ctx->stream)));
+        else
+          GX_TRY((launch_convert<double, half_t>(2 * n, own->U[GX_F64][cb], own->U[prec][cb], own->ctx->stream)));
+      }
+      GX_CHECK_HIP(hipStreamSynchronize(own->ctx->stream));
+    }
+    op->U[prec][0] = own->U[prec][0];
+    op->U[prec][1] = own->U[prec][1];
+  }
   if (!op->U[prec][0]) {
     const size_t n = (size_t)8 * 9 * op->g.H4;
-    if (prec != GX_F32) {
+    if (prec != GX_F32 && prec != GX_F16) {
       set_error("gauge precision %d unsupported", prec);
       return GX_EINVAL;
     }
     for (int cb = 0; cb < 2; cb++) {
-      GX_CHECK_HIP(hipMalloc(&op->U[prec][cb], n * 8));
-      GX_TRY((launch_convert<double, float>(2 * n, op->U[GX_F64][cb], op->U[prec][cb], op->ctx->stream)));
+      GX_CHECK_HIP(hipMalloc(&op->U[prec][cb], n * (prec == GX_F32 ? 8 : 4)));
+      if (prec == GX_F32)
+        GX_TRY((launch_convert<double, float>(2 * n, op->U[GX_F64][cb], op->U[prec][cb], op->ctx->stream)));
+      else
+        GX_TRY((launch_convert<double, half_t>(2 * n, op->U[GX_F64][cb], op->U[prec][cb], op->ctx->stream)));
     }
   }
   *Ue = op->U[prec][0];
@@ -504,6 +536,7 @@ static int upload_smats(Op *op) {
   return GX_OK;
 }
 
+// matrices in the arithmetic precision of storage prec (FP16 storage computes in float)
 int op_smat(Op *op, int prec, int which, const void **mats) {
   const size_t per = 2 * (size_t)op->g.Ls * op->g.Ls;
   if (prec == GX_F64) *mats = (const double *)op->smat_dev[GX_F64] + which * per;
@@ -514,26 +547,21 @@ int op_smat(Op *op, int prec, int which, const void **mats) {
 // ---------------- operator building blocks (half fields) ----------------
 int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const Gate *gate) {
   void *Ue, *Uo;
-  if (in->prec == GX_F64) {
-    Ue = op->U[GX_F64][0];
-    Uo = op->U[GX_F64][1];
-  } else {
-    GX_TRY(op_gauge(op, in->prec, &Ue, &Uo));
-  }
+  GX_TRY(op_gauge(op, in->prec, &Ue, &Uo));
   void *U = cb_out == 0 ? Ue : Uo;
   out->cb = cb_out;
-  if (in->prec == GX_F64)
-    return launch_dhop<double>(op->g, in->data, out->data, U, cb_out, dag, gate, op->ctx->stream);
-  return launch_dhop<float>(op->g, in->data, out->data, U, cb_out, dag, gate, op->ctx->stream);
+  return by_prec(in->prec, [&](auto t) {
+    return launch_dhop<decltype(t)>(op->g, in->data, out->data, U, cb_out, dag, gate, op->ctx->stream);
+  });
 }
 
 static int apply_smat(Op *op, Field *in, Field *phi, Field *out, int which, const Gate *gate) {
   const void *m;
   GX_TRY(op_smat(op, in->prec, which, &m));
   out->cb = in->cb;
-  if (in->prec == GX_F64)
-    return launch_smat<double>(op->g, in->data, phi ? phi->data : nullptr, out->data, m, gate, op->ctx->stream);
-  return launch_smat<float>(op->g, in->data, phi ? phi->data : nullptr, out->data, m, gate, op->ctx->stream);
+  return by_prec(in->prec, [&](auto t) {
+    return launch_smat<decltype(t)>(op->g, in->data, phi ? phi->data : nullptr, out->data, m, gate, op->ctx->stream);
+  });
 }
 
 int apply_meooe(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
@@ -570,12 +598,7 @@ template <typename T>
 static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
   const int prec = in->prec, cb_in = in->cb, cb_mid = 1 - cb_in;
   void *Ue, *Uo;
-  if (prec == GX_F64) {
-    Ue = op->U[GX_F64][0];
-    Uo = op->U[GX_F64][1];
-  } else {
-    GX_TRY(op_gauge(op, prec, &Ue, &Uo));
-  }
+  GX_TRY(op_gauge(op, prec, &Ue, &Uo));
   void *U_mid = cb_mid == 0 ? Ue : Uo, *U_in = cb_in == 0 ? Ue : Uo;
   hipStream_t st = op->ctx->stream;
   const Geom &g = op->g;
@@ -608,9 +631,8 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
 }
 
 int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
-  if (tune_get(GX_TUNE_FUSED)) {
-    return in->prec == GX_F64 ? mpc_fused<double>(op, in, out, dag, gate) : mpc_fused<float>(op, in, out, dag, gate);
-  }
+  if (tune_get(GX_TUNE_FUSED))
+    return by_prec(in->prec, [&](auto t) { return mpc_fused<decltype(t)>(op, in, out, dag, gate); });
   Field *t1 = op_tmp(op, in->prec, 0), *t2 = op_tmp(op, in->prec, 1);
   if (!t1 || !t2) return GX_ENOMEM;
   GX_TRY(apply_meooe(op, in, t1, dag, gate));
@@ -820,9 +842,57 @@ int gx_field_destroy(gx_field f) {
   return GX_OK;
 }
 
+// FP16 storage <-> FP32 device layout on the host: per half block, 12 columns of binary16 pairs
+// scaled by each site's largest |component| (column 12, float).
+static void pack_f16(const Field *f, const float *in, uint32_t *out) {
+  const int64_t st = f->g.half_elems(), nh = f->subset == GX_FULL ? 2 : 1;
+  for (int64_t hf = 0; hf < nh; hf++)
+    parallel_for(st, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; i++) {
+        float m = 0.f;
+        for (int k = 0; k < 12; k++) {
+          const float *v = in + ((hf * 12 + k) * st + i) * 2;
+          m = std::max(m, std::max(std::fabs(v[0]), std::fabs(v[1])));
+        }
+        const float inv = m > 0.f ? 1.f / m : 0.f;
+        for (int k = 0; k < 12; k++) {
+          const float *v = in + ((hf * 12 + k) * st + i) * 2;
+          const half_t pr[2] = {(half_t)(v[0] * inv), (half_t)(v[1] * inv)};
+          std::memcpy(&out[(hf * 13 + k) * st + i], pr, 4);
+        }
+        std::memcpy(&out[(hf * 13 + 12) * st + i], &m, 4);
+      }
+    });
+}
+
+static void unpack_f16(const Field *f, const uint32_t *in, float *out) {
+  const int64_t st = f->g.half_elems(), nh = f->subset == GX_FULL ? 2 : 1;
+  for (int64_t hf = 0; hf < nh; hf++)
+    parallel_for(st, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; i++) {
+        float m;
+        std::memcpy(&m, &in[(hf * 13 + 12) * st + i], 4);
+        for (int k = 0; k < 12; k++) {
+          half_t pr[2];
+          std::memcpy(pr, &in[(hf * 13 + k) * st + i], 4);
+          float *v = out + ((hf * 12 + k) * st + i) * 2;
+          v[0] = (float)pr[0] * m;
+          v[1] = (float)pr[1] * m;
+        }
+      }
+    });
+}
+
 int gx_field_import(gx_field f, const double *host) {
   GX_REQUIRE(f && host, GX_EINVAL, "null argument");
-  if (f->prec == GX_F64) {
+  if (f->prec == GX_F16) {
+    std::vector<float> stage((size_t)f->elems() * 2);
+    gather_to_device_layout<float>(f, host, stage.data());
+    std::vector<uint32_t> packed(f->bytes / 4);
+    pack_f16(f, stage.data(), packed.data());
+    GX_CHECK_HIP(hipMemcpyAsync(f->data, packed.data(), f->bytes, hipMemcpyHostToDevice, f->ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
+  } else if (f->prec == GX_F64) {
     std::vector<double> stage((size_t)f->elems() * 2);
     gather_to_device_layout<double>(f, host, stage.data());
     GX_CHECK_HIP(hipMemcpyAsync(f->data, stage.data(), f->bytes, hipMemcpyHostToDevice, f->ctx->stream));
@@ -850,6 +920,50 @@ int gx_field_gaussian(gx_field f, uint64_t seed) {
   return GX_OK;
 }
 
+int gx_field_point_source(gx_field f, const int x[4], int spin, int colour) {
+  GX_REQUIRE(f && x && f->subset == GX_FULL && (f->prec == GX_F64 || f->prec == GX_F32), GX_EINVAL,
+             "point_source: needs a full F64 or F32 field");
+  const bool batch = spin == -1 && colour == -1;
+  GX_REQUIRE(batch ? f->g.Ls <= 12 : (spin >= 0 && spin < 4 && colour >= 0 && colour < 3), GX_EINVAL,
+             "point_source: spin/colour out of range (or Ls > 12 for the spin-colour batch)");
+  for (int d = 0; d < 4; d++) GX_REQUIRE(x[d] >= 0 && x[d] < f->g.L[d], GX_EINVAL, "point_source: site outside");
+  hipStream_t st = f->ctx->stream;
+  GX_CHECK_HIP(hipMemsetAsync(f->data, 0, f->bytes, st));
+  const int cb = (x[0] + x[1] + x[2] + x[3]) & 1;
+  const int64_t h = half_index(f->g, x);
+  const double one64[2] = {1.0, 0.0};
+  const float one32[2] = {1.0f, 0.0f};
+  const void *one = f->prec == GX_F64 ? (const void *)one64 : (const void *)one32;
+  for (int s = 0; s < f->g.Ls; s++) {
+    const int k = batch ? s : spin * 3 + colour;
+    const int64_t e = ((int64_t)cb * 12 + k) * f->g.half_elems() + h * f->g.Ls + s;
+    GX_CHECK_HIP(hipMemcpyAsync((char *)f->data + e * f->csize(), one, f->csize(), hipMemcpyHostToDevice, st));
+  }
+  GX_CHECK_HIP(hipStreamSynchronize(st));
+  f->cb = 0;
+  return GX_OK;
+}
+
+static int slice_transfer(Field *lo, Field *hi, int s, bool extract) {
+  GX_REQUIRE(lo && hi && lo->subset == GX_FULL && hi->subset == GX_FULL && lo->g.Ls == 1 && lo->prec == hi->prec &&
+                 (lo->prec == GX_F64 || lo->prec == GX_F32) && s >= 0 && s < hi->g.Ls,
+             GX_EINVAL, "slice: needs full fields of one precision, the 4D one with Ls = 1, and 0 <= s < Ls");
+  for (int d = 0; d < 4; d++) GX_REQUIRE(lo->g.L[d] == hi->g.L[d], GX_EINVAL, "slice: 4D dimensions differ");
+  hipStream_t st = hi->ctx->stream;
+  const int Ls = hi->g.Ls;
+  if (lo->prec == GX_F64) {
+    if (extract) GX_TRY(launch_slice_proj<double>(hi->g, hi->data, Ls, s, lo->data, 1, 0, 0xF, 0, st));
+    else GX_TRY(launch_slice_proj<double>(hi->g, lo->data, 1, 0, hi->data, Ls, s, 0xF, 0, st));
+  } else {
+    if (extract) GX_TRY(launch_slice_proj<float>(hi->g, hi->data, Ls, s, lo->data, 1, 0, 0xF, 0, st));
+    else GX_TRY(launch_slice_proj<float>(hi->g, lo->data, 1, 0, hi->data, Ls, s, 0xF, 0, st));
+  }
+  return GX_OK;
+}
+
+int gx_extract_slice(gx_field out4d, gx_field in5d, int s) { return slice_transfer(out4d, in5d, s, true); }
+int gx_insert_slice(gx_field in4d, gx_field out5d, int s) { return slice_transfer(in4d, out5d, s, false); }
+
 int gx_field_digest(gx_field f, uint64_t *digest) {
   GX_REQUIRE(f && digest, GX_EINVAL, "null argument");
   GX_REQUIRE(f->bytes % 8 == 0, GX_EINVAL, "field_digest: size not a multiple of 8 bytes");
@@ -858,7 +972,14 @@ int gx_field_digest(gx_field f, uint64_t *digest) {
 
 int gx_field_export(gx_field f, double *host) {
   GX_REQUIRE(f && host, GX_EINVAL, "null argument");
-  if (f->prec == GX_F64) {
+  if (f->prec == GX_F16) {
+    std::vector<uint32_t> packed(f->bytes / 4);
+    GX_CHECK_HIP(hipMemcpyAsync(packed.data(), f->data, f->bytes, hipMemcpyDeviceToHost, f->ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
+    std::vector<float> stage((size_t)f->elems() * 2);
+    unpack_f16(f, packed.data(), stage.data());
+    scatter_from_device_layout<float>(f, stage.data(), host);
+  } else if (f->prec == GX_F64) {
     std::vector<double> stage((size_t)f->elems() * 2);
     GX_CHECK_HIP(hipMemcpyAsync(stage.data(), f->data, f->bytes, hipMemcpyDeviceToHost, f->ctx->stream));
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
@@ -1015,9 +1136,8 @@ int gx_op_destroy(gx_op op) {
 }
 
 // Links of a base Wilson operator in every precision a solve may use, ready on base's stream.
-static std::mutex g_share_mu;
 static int share_links(Op *op, Op *base) {
-  std::lock_guard<std::mutex> lk(g_share_mu);
+  std::lock_guard<std::mutex> lk(g_share_mu);  // base is not a clone: op_gauge takes no lock here
   void *ue, *uo;
   GX_TRY(op_gauge(base, GX_F32, &ue, &uo));
   GX_CHECK_HIP(hipStreamSynchronize(base->ctx->stream));
@@ -1063,6 +1183,7 @@ int gx_op_footprint(gx_op op, size_t *bytes) {
   if (!op->link_owner) {
     if (op->U[GX_F64][0]) b += nlink * 16;
     if (op->U[GX_F32][0]) b += nlink * 8;
+    if (op->U[GX_F16][0]) b += nlink * 4;
   }
   for (int p = 0; p < 3; p++) {
     if (op->smat_dev[p]) b += (size_t)Op::S_COUNT * 2 * op->g.Ls * op->g.Ls * (p == GX_F64 ? 8 : 4);

@@ -6,7 +6,7 @@
 //       data[k][h*Ls + s],  k = spin*3 + colour,  h = half-lattice 4D site (Grid's red-black
 //       order, x halved), s = fifth-dimension / right-hand-side index (innermost),
 //     so a wave whose lanes walk (s, h) with s fastest reads 64 consecutive complex numbers
-//     per component;
+//     per component; FP16 fields add a 13th column data[12][h*Ls + s] of per-site float scales;
 //   * doubled gauge links per output checkerboard, SoA over sites:
 //       U_cb[p][k][h],  p = 0..7 (x+,y+,z+,t+,x-,y-,z-,t-), k = a*3+b;
 //     U_cb[p<4] = -1/2 U_mu(x) (boundary phase), U_cb[p>=4] = -1/2 U_mu(x-mu)^dag.
@@ -21,7 +21,19 @@
 
 namespace gx {
 
+typedef _Float16 half_t;  // FP16 storage (GX_F16)
+
 void set_error(const char *fmt, ...);
+
+// f(S()) with S the storage type of precision prec (double, float, half_t)
+template <class F>
+int by_prec(int prec, F &&f) {
+  if (prec == GX_F64) return f(double());
+  if (prec == GX_F32) return f(float());
+  if (prec == GX_F16) return f(half_t());
+  set_error("precision %d unsupported", prec);
+  return GX_EINVAL;
+}
 
 // process-wide launch-shape knobs (gx_tune_set / gx_tune_get); the version changes on every set
 int tune_get(int knob);
@@ -92,20 +104,19 @@ struct Field {
   int prec = GX_F64;
   int subset = GX_FULL;   // GX_FULL or a half field
   int cb = 0;             // checkerboard tag of a half field
-  void *data = nullptr;   // half: 12*H4*Ls complex; full: 2x that (even then odd)
-  float *scale = nullptr; // FP16 storage: per (h, s) scale factor, per checkerboard
+  // half: ncol() columns of H4*Ls entries (component k = column k); full: 2 halves (even, odd).
+  // GX_F16 stores a complex number as two binary16 values and adds a 13th column: one float per
+  // (h, s) site, the site's largest |component|, by which its stored values are multiplied.
+  void *data = nullptr;
   size_t bytes = 0;
   int64_t elems() const { return 12 * g.half_elems() * (subset == GX_FULL ? 2 : 1); }
   size_t csize() const { return prec == GX_F64 ? 16 : (prec == GX_F32 ? 8 : 4); }
+  int ncol() const { return prec == GX_F16 ? 13 : 12; }
+  size_t half_bytes() const { return (size_t)ncol() * g.half_elems() * csize(); }
   // pointer to the half with checkerboard cb (full fields) or the data (half fields)
   void *half(int which) const {
     if (subset != GX_FULL) return data;
-    return (char *)data + (size_t)which * 12 * g.half_elems() * csize();
-  }
-  float *half_scale(int which) const {
-    if (!scale) return nullptr;
-    if (subset != GX_FULL) return scale;
-    return scale + (size_t)which * g.half_elems();
+    return (char *)data + (size_t)which * half_bytes();
   }
 };
 
@@ -155,8 +166,9 @@ template <typename T>
 int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st);
 // order-independent 64-bit digest of nwords 64-bit words: sum over i of mix(word_i, i) mod 2^64
 int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out);
-// All pointers are device pointers to half fields in the layout above; T = double or float.
-template <typename T>
+// All pointers are device pointers to fields in the layout above.  S is the storage type: double,
+// float or half_t (FP16 storage with float arithmetic; its matrices and scalars are float).
+template <typename S>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st);
 // hop with an epilogue: epi 0 out = hop; 1 out = alpha hop + beta phi; 2 out = alpha A(hop) + beta phi
@@ -172,17 +184,19 @@ int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const
 template <typename T>
 int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, void *out, const void *mats, void *out2,
                  const void *mats2, const Gate *gate, hipStream_t st);
-// z = a x + b y over n complex numbers (a, b real)
-template <typename T>
-int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const Gate *gate,
-                 hipStream_t st);
+// z = a x + b y over nsites sites (stride sites per half; a, b real; y optional)
+template <typename S>
+int launch_axpby(int64_t nsites, int64_t stride, double a, const void *x, double b, const void *y, void *z,
+                 const Gate *gate, hipStream_t st);
 // reductions into ctx slot(s); mode 0: <x|y> -> slot, slot+1 (re, im); 1: |x|^2 -> slot;
 // 2: z = a x + y and |z|^2 -> slot (fused axpy_norm)
 template <typename T>
 int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void *x, const void *y, void *z, double a,
                   int slot);
 template <typename Tin, typename Tout>
-int launch_convert(int64_t n, const void *in, void *out, hipStream_t st);
+int launch_convert(int64_t n, const void *in, void *out, hipStream_t st);       // n plain entries
+template <typename Sin, typename Sout>
+int launch_convert_site(int64_t nsites, int64_t stride, const void *in, void *out, hipStream_t st);
 // z[sz] (=|+=) (spins in mask) x[sx] over both checkerboards of full fields (rows Lsx / Lsz)
 template <typename T>
 int launch_slice_proj(const Geom &g, const void *x, int Lsx, int sx, void *z, int Lsz, int sz, int mask,
@@ -193,8 +207,8 @@ template <typename T>
 int launch_cg_dot(Ctx *ctx, int64_t nsites, int64_t stride, const void *p, const void *mmp, CGState *state);
 template <typename T>
 int launch_cg_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const void *mmp, CGState *state);
-template <typename T>
-int launch_cg_update(Ctx *ctx, int64_t n, void *psi, void *p, const void *r, CGState *state);
+template <typename S>
+int launch_cg_update(Ctx *ctx, int64_t nsites, int64_t stride, void *psi, void *p, const void *r, CGState *state);
 
 // Multi-RHS (slice) kernels; a half field is viewed as rows x Ls with s innermost.
 // mode 0: d_s = Re<x_s|y_s> -> a_s = c_s / d_s (state);   mode 2: x_s -= a_s y_s, cp_s = |x_s|^2 ->

@@ -34,6 +34,75 @@ __device__ __forceinline__ cx<T> caddi(cx<T> a, cx<T> b) { return {a.r - b.i, a.
 template <typename T>
 __device__ __forceinline__ cx<T> csubi(cx<T> a, cx<T> b) { return {a.r + b.i, a.i - b.r}; }
 
+// Storage of field entries: double, float, or FP16 (half_t).  FP16 keeps a complex number as two
+// binary16 values and every (h, s) site of a half block carries a float scale in column 12 (its
+// largest |component|), so stored values lie in [-1, 1]; its arithmetic type is float.  FP16 links
+// are stored unscaled (SU(3) entries are bounded by 1).
+template <typename S>
+struct Sto {
+  typedef S C;
+  static constexpr bool H = false;
+  static constexpr int NCOL = 12;
+};
+template <>
+struct Sto<half_t> {
+  typedef float C;
+  static constexpr bool H = true;
+  static constexpr int NCOL = 13;
+};
+
+template <typename S>
+__device__ __forceinline__ cx<typename Sto<S>::C> ldc(const cx<S> *__restrict__ p, int64_t i) {
+  const cx<S> v = p[i];
+  return {(typename Sto<S>::C)v.r, (typename Sto<S>::C)v.i};
+}
+
+// scale of site i of the half block whose column 0 starts at p (FP16 storage; 1 otherwise)
+template <typename S>
+__device__ __forceinline__ typename Sto<S>::C site_scale(const cx<S> *__restrict__ p, int64_t i, int64_t stride) {
+  if constexpr (Sto<S>::H) return reinterpret_cast<const float *>(p + 12 * stride)[i];
+  else return (typename Sto<S>::C)1;
+}
+
+template <typename S>
+__device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ p, int64_t i,
+                                          int64_t stride) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) f[k] = ldc<S>(p, k * stride + i);
+  if constexpr (Sto<S>::H) {
+    const float sc = site_scale<S>(p, i, stride);
+#pragma unroll
+    for (int k = 0; k < 12; k++) f[k] = {f[k].r * sc, f[k].i * sc};
+  }
+}
+
+// store the 12 components of site i (FP16: with the site's scale); nt = non-temporal stores
+template <typename S>
+__device__ __forceinline__ void store_site(cx<S> *__restrict__ p, const cx<typename Sto<S>::C> *a, int64_t i,
+                                           int64_t stride, bool nt) {
+  if constexpr (Sto<S>::H) {
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(a[k].r), fabsf(a[k].i)));
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) p[k * stride + i] = {(half_t)(a[k].r * inv), (half_t)(a[k].i * inv)};
+    reinterpret_cast<float *>(p + 12 * stride)[i] = m;
+  } else {
+    typedef S v2 __attribute__((ext_vector_type(2)));
+    if (nt) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        v2 v = {a[k].r, a[k].i};
+        __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(p + k * stride + i));
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) p[k * stride + i] = a[k];
+    }
+  }
+}
+
 struct KGeom {
   int L[4];
   int Lxh;
@@ -141,8 +210,8 @@ __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nblocks, 
   return b;
 }
 
-template <typename T, int MU, bool PLUS, bool LDSU>
-__device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<T> *__restrict__ U,
+template <typename T, typename S, int MU, bool PLUS, bool LDSU>
+__device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<S> *__restrict__ U,
                                          int P, uint32_t h, uint32_t H4) {
   cx<T> hs[6];
   project<T, MU, PLUS>(f, hs);
@@ -152,33 +221,11 @@ __device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T>
     for (int k = 0; k < 9; k++) u[k] = ul[P * 9 + k];
   } else {
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = U[(P * 9 + k) * H4 + h];
+    for (int k = 0; k < 9; k++) u[k] = ldc<S>(U, (P * 9 + k) * H4 + h);
   }
   cx<T> gs[6];
   su3_mul<T>(u, hs, gs);
   accum_recon<T, MU, PLUS>(acc, gs);
-}
-
-template <typename T>
-__device__ __forceinline__ void load_spinor(cx<T> *f, const cx<T> *__restrict__ in, uint32_t idx, uint32_t stride) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) f[k] = in[k * stride + idx];
-}
-
-template <typename T>
-__device__ __forceinline__ void store_spinor(cx<T> *__restrict__ out, const cx<T> *acc, uint32_t o, uint32_t stride,
-                                             bool nt) {
-  typedef T v2 __attribute__((ext_vector_type(2)));
-  if (nt) {
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-      v2 v = {acc[k].r, acc[k].i};
-      __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(out + k * stride + o));
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 12; k++) out[k * stride + o] = acc[k];
-  }
 }
 
 // One leg: LEG-th entry of the leg sequence.  ORD = 0 is Grid's order (DaggerNo: Xm,Ym,Zm,Tm with
@@ -191,14 +238,14 @@ struct LegSel {
   static constexpr int P = (PLUS != DAG) ? MU + 4 : MU;   // (1+gamma) legs read x-mu when !DAG
 };
 
-template <typename T, bool DAG, int ORD, int LEG, bool LDSU>
-__device__ __forceinline__ void do_leg(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<T> *__restrict__ U,
+template <typename T, typename S, bool DAG, int ORD, int LEG, bool LDSU>
+__device__ __forceinline__ void do_leg(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<S> *__restrict__ U,
                                        uint32_t h, uint32_t H4) {
   typedef LegSel<DAG, ORD, LEG> L;
-  leg_math<T, L::MU, L::PLUS, LDSU>(acc, f, ul, U, L::P, h, H4);
+  leg_math<T, S, L::MU, L::PLUS, LDSU>(acc, f, ul, U, L::P, h, H4);
 }
 
-template <typename T, bool DAG, int ORD, int LEG>
+template <bool DAG, int ORD, int LEG>
 __device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, int s) {
   return nb[LegSel<DAG, ORD, LEG>::P] * Ls + s;
 }
@@ -206,49 +253,49 @@ __device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, int
 // The 8 legs of one output site.  The caller has issued the neighbour loads of leg 0 (into fa)
 // and, with PF, of leg 1 (into fb); with PF the loads of leg L+1 are in flight while leg L is
 // computed (sched_barrier keeps the compiler from sinking them).
-template <typename T, int LS, bool DAG, bool PF, int ORD, bool LDSU>
-__device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const cx<T> *__restrict__ in,
-                                         const cx<T> *__restrict__ U, const cx<T> *ul, const uint32_t *nb, int s,
+template <typename T, typename S, int LS, bool DAG, bool PF, int ORD, bool LDSU>
+__device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const cx<S> *__restrict__ in,
+                                         const cx<S> *__restrict__ U, const cx<T> *ul, const uint32_t *nb, int s,
                                          uint32_t h, uint32_t H4, uint32_t stride) {
-#define IDX(L) leg_idx<T, DAG, ORD, L>(nb, LS, s)
-#define LEG(L, F) do_leg<T, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4)
+#define IDX(L) leg_idx<DAG, ORD, L>(nb, LS, s)
+#define LEG(L, F) do_leg<T, S, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4)
   if (PF) {
     __builtin_amdgcn_sched_barrier(0);
     LEG(0, fa);
-    load_spinor<T>(fa, in, IDX(2), stride);
+    load_site<S>(fa, in, IDX(2), stride);
     __builtin_amdgcn_sched_barrier(0);
     LEG(1, fb);
-    load_spinor<T>(fb, in, IDX(3), stride);
+    load_site<S>(fb, in, IDX(3), stride);
     __builtin_amdgcn_sched_barrier(0);
     LEG(2, fa);
-    load_spinor<T>(fa, in, IDX(4), stride);
+    load_site<S>(fa, in, IDX(4), stride);
     __builtin_amdgcn_sched_barrier(0);
     LEG(3, fb);
-    load_spinor<T>(fb, in, IDX(5), stride);
+    load_site<S>(fb, in, IDX(5), stride);
     __builtin_amdgcn_sched_barrier(0);
     LEG(4, fa);
-    load_spinor<T>(fa, in, IDX(6), stride);
+    load_site<S>(fa, in, IDX(6), stride);
     __builtin_amdgcn_sched_barrier(0);
     LEG(5, fb);
-    load_spinor<T>(fb, in, IDX(7), stride);
+    load_site<S>(fb, in, IDX(7), stride);
     __builtin_amdgcn_sched_barrier(0);
     LEG(6, fa);
     LEG(7, fb);
   } else {
     LEG(0, fa);
-    load_spinor<T>(fa, in, IDX(1), stride);
+    load_site<S>(fa, in, IDX(1), stride);
     LEG(1, fa);
-    load_spinor<T>(fa, in, IDX(2), stride);
+    load_site<S>(fa, in, IDX(2), stride);
     LEG(2, fa);
-    load_spinor<T>(fa, in, IDX(3), stride);
+    load_site<S>(fa, in, IDX(3), stride);
     LEG(3, fa);
-    load_spinor<T>(fa, in, IDX(4), stride);
+    load_site<S>(fa, in, IDX(4), stride);
     LEG(4, fa);
-    load_spinor<T>(fa, in, IDX(5), stride);
+    load_site<S>(fa, in, IDX(5), stride);
     LEG(5, fa);
-    load_spinor<T>(fa, in, IDX(6), stride);
+    load_site<S>(fa, in, IDX(6), stride);
     LEG(6, fa);
-    load_spinor<T>(fa, in, IDX(7), stride);
+    load_site<S>(fa, in, IDX(7), stride);
     LEG(7, fa);
   }
 #undef IDX
@@ -259,9 +306,10 @@ __device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const
 // 3: out = alpha A(hop) + beta phi, with A a per-chirality Ls x Ls matrix applied across the s-row
 // of each site from LDS and phi in the layout of out.  Whether phi is read is a template choice: a
 // run-time null test inside the unrolled epilogue makes the compiler spill the whole hop.
-template <typename T>
+template <typename S>
 struct HopEpi {
-  const cx<T> *phi;
+  typedef typename Sto<S>::C T;
+  const cx<S> *phi;
   const T *mat;   // EPI 2, 3: [2][LS][LS], chirality 0 = spins 0,1
   T alpha, beta;
 };
@@ -269,11 +317,12 @@ struct HopEpi {
 // One lane per (4D half-site h, s): s fastest, 64/LS sites per wave, 4 waves per workgroup, so a
 // wave reads each neighbour's s-row contiguously and the lanes of one site share its links.
 // flags: bit 0 = non-temporal output stores.  ORD: leg order (0 Grid's, 1 paired directions).
-template <typename T, int LS, bool DAG, bool PF, int ORD, int EPI>
-__global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
-                                              const cx<T> *__restrict__ U, KGeom g, int cb_out,
+template <typename S, int LS, bool DAG, bool PF, int ORD, int EPI>
+__global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S> *__restrict__ out,
+                                              const cx<S> *__restrict__ U, KGeom g, int cb_out,
                                               const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags,
-                                              HopEpi<T> ep) {
+                                              HopEpi<S> ep) {
+  typedef typename Sto<S>::C T;
   if (gated(gate)) return;
   constexpr int SPW = 64 / LS;
   constexpr int SPB = 4 * SPW;
@@ -325,12 +374,12 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
     for (int j = 0; j < NU; j++) {
       const uint32_t e = threadIdx.x + 256 * j;
       const uint32_t pk = e / SPB, site = e - pk * SPB;
-      ureg[j] = (e < SPB * 72 && h0 + site < H4) ? U[pk * H4 + h0 + site] : cx<T>{0, 0};
+      ureg[j] = (e < SPB * 72 && h0 + site < H4) ? ldc<S>(U, pk * H4 + h0 + site) : cx<T>{0, 0};
     }
   }
   cx<T> fa[12], fb[12];
-  load_spinor<T>(fa, in, leg_idx<T, DAG, ORD, 0>(nb, LS, s), stride);
-  if (PF) load_spinor<T>(fb, in, leg_idx<T, DAG, ORD, 1>(nb, LS, s), stride);
+  load_site<S>(fa, in, leg_idx<DAG, ORD, 0>(nb, LS, s), stride);
+  if (PF) load_site<S>(fb, in, leg_idx<DAG, ORD, 1>(nb, LS, s), stride);
   if (LDSU) {
 #pragma unroll
     for (int j = 0; j < NU; j++) {
@@ -345,7 +394,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  if (valid) hop_site<T, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4, stride);
+  if (valid) hop_site<T, S, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4, stride);
   const uint32_t o = h * LS + s;
   if (MAT) {
     // every wave has finished reading links: reuse the LDS for the hop results [w][k][lane]
@@ -371,18 +420,19 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
     }
   }
   if (EPI != 0) {
+    const T bsc = PHI ? ep.beta * site_scale<S>(ep.phi, o, stride) : T(0);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
       cx<T> v = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
       if (PHI) {
-        const cx<T> p = ep.phi[k * stride + o];
-        v.r = fma(ep.beta, p.r, v.r);
-        v.i = fma(ep.beta, p.i, v.i);
+        const cx<T> p = ldc<S>(ep.phi, k * stride + o);
+        v.r = fma(bsc, p.r, v.r);
+        v.i = fma(bsc, p.i, v.i);
       }
       acc[k] = v;
     }
   }
-  store_spinor<T>(out, acc, o, stride, (flags & 1) != 0);
+  store_site<S>(out, acc, o, stride, (flags & 1) != 0);
 }
 
 #ifndef GX_LS_CASES
@@ -392,11 +442,12 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<T> *__restrict__ in, cx<T
 
 // epi: 0 out = hop; 1 out = alpha hop + beta phi; 2 out = alpha A(hop) [+ beta phi when phi is
 // given] (A: mat)
-template <typename T>
+template <typename S>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                     hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta) {
+  typedef typename Sto<S>::C T;
   const KGeom k = kgeom(g);
-  if (12 * 2 * k.stride >= (int64_t)1 << 31) {
+  if (13 * 2 * k.stride >= (int64_t)1 << 31) {
     set_error("dhop: lattice too large for 32-bit site indexing");
     return GX_EINVAL;
   }
@@ -413,15 +464,15 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   const bool pf = tune_get(GX_TUNE_DHOP_PF) != 0;
   const int flags = tune_get(GX_TUNE_DHOP_NT) ? 1 : 0;
   const bool ord = epi == 0 && tune_get(GX_TUNE_DHOP_ORDER) != 0 && (g.Ls == 1 || g.Ls == 12 || g.Ls == 16);
-  const cx<T> *pin = (const cx<T> *)in;
-  cx<T> *pout = (cx<T> *)out;
-  const cx<T> *pU = (const cx<T> *)U;
-  HopEpi<T> ep{(const cx<T> *)phi, (const T *)mat, (T)alpha, (T)beta};
+  const cx<S> *pin = (const cx<S> *)in;
+  cx<S> *pout = (cx<S> *)out;
+  const cx<S> *pU = (const cx<S> *)U;
+  HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta};
   const int spb = 4 * (64 / g.Ls);
   const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
   const uint32_t blocks = map == 1 ? ((ntiles + 7) / 8) * 8 : ntiles;
 #define GX_DHOP_GO(LS, DAG, PF, ORD, EPI)                                                                         \
-  hipLaunchKernelGGL((k_dhop<T, LS, DAG, PF, ORD, EPI>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
+  hipLaunchKernelGGL((k_dhop<S, LS, DAG, PF, ORD, EPI>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
                      gate, ntiles, map, flags, ep)
 #define GX_DHOP_CASE_O(LS, ORD, EPI)                   \
   if (dag) {                                           \
@@ -464,20 +515,22 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   return GX_OK;
 }
 
-template <typename T>
+template <typename S>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st) {
-  return launch_dhop_epi<T>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0);
+  return launch_dhop_epi<S>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0);
 }
 
 // out_s = sum_t A_chi[s][t] in_t (+ beta phi_s with PHI), and with OUT2 also
 // out2_s = sum_t C_chi[s][t] in_t from the same staged input.  mats: [2][LS][LS] in T,
 // chirality 0 = spins 0,1.  One lane per (h, s); the site's s-row is staged in LDS.
-template <typename T, int LS, bool OUT2, bool PHI>
-__global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, const cx<T> *__restrict__ phi, T beta,
-                                              cx<T> *__restrict__ out, const T *__restrict__ mats,
-                                              cx<T> *__restrict__ out2, const T *__restrict__ mats2, int64_t H4,
+template <typename S, int LS, bool OUT2, bool PHI>
+__global__ void __launch_bounds__(256) k_smat(const cx<S> *__restrict__ in, const cx<S> *__restrict__ phi,
+                                              typename Sto<S>::C beta, cx<S> *__restrict__ out,
+                                              const typename Sto<S>::C *__restrict__ mats, cx<S> *__restrict__ out2,
+                                              const typename Sto<S>::C *__restrict__ mats2, int64_t H4,
                                               int64_t stride, const Gate *__restrict__ gate) {
+  typedef typename Sto<S>::C T;
   if (gated(gate)) return;
   constexpr int SPW = 64 / LS;
   __shared__ cx<T> lds[4][12][64];
@@ -489,8 +542,17 @@ __global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, cons
   const int64_t h = wave * SPW + ls;
   const bool valid = (ls < SPW) && (h < H4);
   const int64_t o = h * LS + s;
+  {
+    cx<T> f[12];
+    if (valid) {
+      load_site<S>(f, in, o, stride);
+    } else {
 #pragma unroll
-  for (int k = 0; k < 12; k++) lds[w][k][lane] = valid ? in[k * stride + o] : cx<T>{0, 0};
+      for (int k = 0; k < 12; k++) f[k] = {0, 0};
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) lds[w][k][lane] = f[k];
+  }
   __syncthreads();
   if (!valid) return;
   cx<T> acc[12], acc2[OUT2 ? 12 : 1];
@@ -520,32 +582,33 @@ __global__ void __launch_bounds__(256) k_smat(const cx<T> *__restrict__ in, cons
       }
     }
   }
+  if (PHI) {
+    const T bsc = beta * site_scale<S>(phi, o, stride);
 #pragma unroll
-  for (int k = 0; k < 12; k++) {
-    cx<T> v = acc[k];
-    if (PHI) {
-      const cx<T> p = phi[k * stride + o];
-      v.r = fma(beta, p.r, v.r);
-      v.i = fma(beta, p.i, v.i);
+    for (int k = 0; k < 12; k++) {
+      const cx<T> p = ldc<S>(phi, k * stride + o);
+      acc[k].r = fma(bsc, p.r, acc[k].r);
+      acc[k].i = fma(bsc, p.i, acc[k].i);
     }
-    out[k * stride + o] = v;
-    if (OUT2) out2[k * stride + o] = acc2[k];
   }
+  store_site<S>(out, acc, o, stride, false);
+  if (OUT2) store_site<S>(out2, acc2, o, stride, false);
 }
 
-template <typename T>
+template <typename S>
 int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, void *out, const void *mats, void *out2,
                  const void *mats2, const Gate *gate, hipStream_t st) {
+  typedef typename Sto<S>::C T;
   const int spw = 64 / g.Ls;
   const int64_t waves = (g.H4 + spw - 1) / spw;
   const int64_t blocks = (waves * 64 + 255) / 256;
   const int64_t stride = g.H4 * g.Ls;
-  const cx<T> *pin = (const cx<T> *)in;
-  const cx<T> *pphi = (const cx<T> *)phi;
-  cx<T> *pout = (cx<T> *)out, *pout2 = (cx<T> *)out2;
+  const cx<S> *pin = (const cx<S> *)in;
+  const cx<S> *pphi = (const cx<S> *)phi;
+  cx<S> *pout = (cx<S> *)out, *pout2 = (cx<S> *)out2;
   const T *pm = (const T *)mats, *pm2 = (const T *)mats2;
 #define GX_SMAT_GO(LS, OUT2, PHI)                                                                           \
-  hipLaunchKernelGGL((k_smat<T, LS, OUT2, PHI>), dim3(blocks), dim3(256), 0, st, pin, pphi, (T)beta, pout, pm, \
+  hipLaunchKernelGGL((k_smat<S, LS, OUT2, PHI>), dim3(blocks), dim3(256), 0, st, pin, pphi, (T)beta, pout, pm, \
                      pout2, pm2, g.H4, stride, gate)
 #define GX_SMAT_CASE(LS)                    \
   case LS:                                  \
@@ -569,43 +632,69 @@ int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, vo
   return GX_OK;
 }
 
-template <typename T>
+template <typename S>
 int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
                 hipStream_t st) {
-  return launch_smat2<T>(g, in, phi, 1.0, out, mats, nullptr, nullptr, gate, st);
+  return launch_smat2<S>(g, in, phi, 1.0, out, mats, nullptr, nullptr, gate, st);
 }
 
 // ---------------- BLAS1 ----------------
-template <typename T>
-__global__ void __launch_bounds__(256) k_axpby(int64_t n, T a, const cx<T> *__restrict__ x, T b,
-                                               const cx<T> *__restrict__ y, cx<T> *__restrict__ z,
-                                               const Gate *__restrict__ gate) {
-  if (gated(gate)) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const cx<T> xv = x[i];
-    if (y) {
-      const cx<T> yv = y[i];
-      z[i] = {fma(a, xv.r, b * yv.r), fma(a, xv.i, b * yv.i)};
-    } else {
-      z[i] = {a * xv.r, a * xv.i};
-    }
-  }
-}
-
 static int64_t stream_blocks(int64_t n) {
   int64_t b = (n + 255) / 256;
   return b < 1 ? 1 : (b > 65536 ? 65536 : b);
 }
 
-template <typename T>
-int launch_axpby(int64_t n, double a, const void *x, double b, const void *y, void *z, const Gate *gate,
-                 hipStream_t st) {
-  hipLaunchKernelGGL((k_axpby<T>), dim3(stream_blocks(n)), dim3(256), 0, st, n, (T)a, (const cx<T> *)x, (T)b,
-                     (const cx<T> *)y, (cx<T> *)z, gate);
+// z = a x + b y (y optional) over nsites sites of half or full fields (stride sites per half).
+// FP64/FP32 stream the entries; FP16 works per site (its stores need the site's scale).
+template <typename S>
+__global__ void __launch_bounds__(256) k_axpby(int64_t nsites, int64_t stride, typename Sto<S>::C a,
+                                               const cx<S> *__restrict__ x, typename Sto<S>::C b,
+                                               const cx<S> *__restrict__ y, cx<S> *__restrict__ z,
+                                               const Gate *__restrict__ gate) {
+  typedef typename Sto<S>::C T;
+  if (gated(gate)) return;
+  if constexpr (!Sto<S>::H) {
+    const int64_t n = 12 * nsites;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+      const cx<T> xv = x[i];
+      if (y) {
+        const cx<T> yv = y[i];
+        z[i] = {fma(a, xv.r, b * yv.r), fma(a, xv.i, b * yv.i)};
+      } else {
+        z[i] = {a * xv.r, a * xv.i};
+      }
+    }
+  } else {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites;
+         j += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
+      cx<T> xv[12], yv[12];
+      load_site<S>(xv, x + base, i, stride);
+      if (y) {
+        load_site<S>(yv, y + base, i, stride);
+#pragma unroll
+        for (int k = 0; k < 12; k++) xv[k] = {fma(a, xv[k].r, b * yv[k].r), fma(a, xv[k].i, b * yv[k].i)};
+      } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) xv[k] = {a * xv[k].r, a * xv[k].i};
+      }
+      store_site<S>(z + base, xv, i, stride, false);
+    }
+  }
+}
+
+template <typename S>
+int launch_axpby(int64_t nsites, int64_t stride, double a, const void *x, double b, const void *y, void *z,
+                 const Gate *gate, hipStream_t st) {
+  typedef typename Sto<S>::C T;
+  const int64_t work = Sto<S>::H ? nsites : 12 * nsites;
+  hipLaunchKernelGGL((k_axpby<S>), dim3(stream_blocks(work)), dim3(256), 0, st, nsites, stride, (T)a,
+                     (const cx<S> *)x, (T)b, (const cx<S> *)y, (cx<S> *)z, gate);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
 
+// flat element conversion (links, and FP64 <-> FP32 fields)
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(256) k_convert(int64_t n, const Tin *__restrict__ in, Tout *__restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -616,6 +705,29 @@ template <typename Tin, typename Tout>
 int launch_convert(int64_t n, const void *in, void *out, hipStream_t st) {
   hipLaunchKernelGGL((k_convert<Tin, Tout>), dim3(stream_blocks(n)), dim3(256), 0, st, n, (const Tin *)in,
                      (Tout *)out);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+// per-site field conversion between storages (any pair that involves FP16)
+template <typename Sin, typename Sout>
+__global__ void __launch_bounds__(256) k_convert_site(int64_t nsites, int64_t stride, const cx<Sin> *__restrict__ in,
+                                                      cx<Sout> *__restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t hf = j / stride, i = j - hf * stride;
+    cx<typename Sto<Sin>::C> a[12];
+    cx<typename Sto<Sout>::C> b[12];
+    load_site<Sin>(a, in + hf * Sto<Sin>::NCOL * stride, i, stride);
+#pragma unroll
+    for (int k = 0; k < 12; k++) b[k] = {(typename Sto<Sout>::C)a[k].r, (typename Sto<Sout>::C)a[k].i};
+    store_site<Sout>(out + hf * Sto<Sout>::NCOL * stride, b, i, stride, false);
+  }
+}
+
+template <typename Sin, typename Sout>
+int launch_convert_site(int64_t nsites, int64_t stride, const void *in, void *out, hipStream_t st) {
+  hipLaunchKernelGGL((k_convert_site<Sin, Sout>), dim3(stream_blocks(nsites)), dim3(256), 0, st, nsites, stride,
+                     (const cx<Sin> *)in, (cx<Sout> *)out);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -666,36 +778,36 @@ __device__ __forceinline__ void sum_partials(double (&v)[NV], const double *part
 
 // site j of a (possibly full) field: half = j / stride, i = j % stride;
 // component k at half*12*stride + k*stride + i.
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, const cx<T> *__restrict__ x,
-                                                const cx<T> *__restrict__ y, cx<T> *__restrict__ z, T a,
-                                                double *partial) {
+template <typename S, int MODE>
+__global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, const cx<S> *__restrict__ x,
+                                                const cx<S> *__restrict__ y, cx<S> *__restrict__ z,
+                                                typename Sto<S>::C a, double *partial) {
+  typedef typename Sto<S>::C T;
   __shared__ double sh[2][4];
   double v[2] = {0, 0};
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t hf = j / stride;
-    const int64_t base = hf * 12 * stride + (j - hf * stride);
+    const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
+    cx<T> xv[12], yv[12];
+    load_site<S>(xv, x + base, i, stride);
+    if (MODE != 1) load_site<S>(yv, y + base, i, stride);
     T sr = 0, si = 0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      const cx<T> xv = x[base + k * stride];
       if (MODE == 0) {
-        const cx<T> yv = y[base + k * stride];
-        sr = fma(xv.r, yv.r, sr);
-        sr = fma(xv.i, yv.i, sr);
-        si = fma(xv.r, yv.i, si);
-        si = fma(-xv.i, yv.r, si);
+        sr = fma(xv[k].r, yv[k].r, sr);
+        sr = fma(xv[k].i, yv[k].i, sr);
+        si = fma(xv[k].r, yv[k].i, si);
+        si = fma(-xv[k].i, yv[k].r, si);
       } else if (MODE == 1) {
-        sr = fma(xv.r, xv.r, sr);
-        sr = fma(xv.i, xv.i, sr);
+        sr = fma(xv[k].r, xv[k].r, sr);
+        sr = fma(xv[k].i, xv[k].i, sr);
       } else {
-        const cx<T> yv = y[base + k * stride];
-        const cx<T> zv = {fma(a, xv.r, yv.r), fma(a, xv.i, yv.i)};
-        z[base + k * stride] = zv;
-        sr = fma(zv.r, zv.r, sr);
-        sr = fma(zv.i, zv.i, sr);
+        yv[k] = {fma(a, xv[k].r, yv[k].r), fma(a, xv[k].i, yv[k].i)};
+        sr = fma(yv[k].r, yv[k].r, sr);
+        sr = fma(yv[k].i, yv[k].i, sr);
       }
     }
+    if (MODE == 2) store_site<S>(z + base, yv, i, stride, false);
     v[0] += (double)sr;
     v[1] += (double)si;
   }
@@ -719,21 +831,22 @@ static int red_blocks(int64_t nsites) {
   return (int)(b < 1 ? 1 : b);
 }
 
-template <typename T>
+template <typename S>
 int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void *x, const void *y, void *z, double a,
                   int slot) {
+  typedef typename Sto<S>::C T;
   const int nb = red_blocks(nsites);
-  const cx<T> *px = (const cx<T> *)x;
-  const cx<T> *py = (const cx<T> *)y;
-  cx<T> *pz = (cx<T> *)z;
+  const cx<S> *px = (const cx<S> *)x;
+  const cx<S> *py = (const cx<S> *)y;
+  cx<S> *pz = (cx<S> *)z;
   if (mode == 0)
-    hipLaunchKernelGGL((k_reduce<T, 0>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
+    hipLaunchKernelGGL((k_reduce<S, 0>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
                        ctx->red_partial);
   else if (mode == 1)
-    hipLaunchKernelGGL((k_reduce<T, 1>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
+    hipLaunchKernelGGL((k_reduce<S, 1>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
                        ctx->red_partial);
   else
-    hipLaunchKernelGGL((k_reduce<T, 2>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
+    hipLaunchKernelGGL((k_reduce<S, 2>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
                        ctx->red_partial);
   hipLaunchKernelGGL(k_reduce_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb,
                      ctx->red_result + slot, mode == 0 ? 2 : 1);
@@ -747,21 +860,23 @@ int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void
 __device__ __forceinline__ bool gated_prev(const CGState *st) { return st->k - 1 > st->k_conv || st->k - 1 > st->kmax; }
 
 // partials of d = Re<p|mmp>
-template <typename T>
-__global__ void __launch_bounds__(256) k_cg_dot(int64_t nsites, int64_t stride, const cx<T> *__restrict__ p,
-                                                const cx<T> *__restrict__ mmp, const CGState *st, double *partial) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_cg_dot(int64_t nsites, int64_t stride, const cx<S> *__restrict__ p,
+                                                const cx<S> *__restrict__ mmp, const CGState *st, double *partial) {
+  typedef typename Sto<S>::C T;
   if (gated(st)) return;
   __shared__ double sh[1][4];
   double v[1] = {0};
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t hf = j / stride;
-    const int64_t base = hf * 12 * stride + (j - hf * stride);
+    const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
+    cx<T> xv[12], yv[12];
+    load_site<S>(xv, p + base, i, stride);
+    load_site<S>(yv, mmp + base, i, stride);
     T sr = 0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      const cx<T> xv = p[base + k * stride], yv = mmp[base + k * stride];
-      sr = fma(xv.r, yv.r, sr);
-      sr = fma(xv.i, yv.i, sr);
+      sr = fma(xv[k].r, yv[k].r, sr);
+      sr = fma(xv[k].i, yv[k].i, sr);
     }
     v[0] += (double)sr;
   }
@@ -770,26 +885,28 @@ __global__ void __launch_bounds__(256) k_cg_dot(int64_t nsites, int64_t stride, 
 }
 
 // partials of cp = |r - a mmp|^2 with r updated in place
-template <typename T>
-__global__ void __launch_bounds__(256) k_cg_axpy_norm(int64_t nsites, int64_t stride, cx<T> *__restrict__ r,
-                                                      const cx<T> *__restrict__ mmp, const CGState *st,
+template <typename S>
+__global__ void __launch_bounds__(256) k_cg_axpy_norm(int64_t nsites, int64_t stride, cx<S> *__restrict__ r,
+                                                      const cx<S> *__restrict__ mmp, const CGState *st,
                                                       double *partial) {
+  typedef typename Sto<S>::C T;
   if (gated(st)) return;
   __shared__ double sh[1][4];
   const T a = (T)(-st->a);
   double v[1] = {0};
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t hf = j / stride;
-    const int64_t base = hf * 12 * stride + (j - hf * stride);
+    const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
+    cx<T> xv[12], yv[12];
+    load_site<S>(xv, mmp + base, i, stride);
+    load_site<S>(yv, r + base, i, stride);
     T sr = 0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      const cx<T> xv = mmp[base + k * stride], yv = r[base + k * stride];
-      const cx<T> zv = {fma(a, xv.r, yv.r), fma(a, xv.i, yv.i)};
-      r[base + k * stride] = zv;
-      sr = fma(zv.r, zv.r, sr);
-      sr = fma(zv.i, zv.i, sr);
+      yv[k] = {fma(a, xv[k].r, yv[k].r), fma(a, xv[k].i, yv[k].i)};
+      sr = fma(yv[k].r, yv[k].r, sr);
+      sr = fma(yv[k].i, yv[k].i, sr);
     }
+    store_site<S>(r + base, yv, i, stride, false);
     v[0] += (double)sr;
   }
   block_sum<1>(v, sh);
@@ -822,41 +939,63 @@ __global__ void __launch_bounds__(256) k_cg_fin(const double *partial, int npart
   }
 }
 
-// psi = a p + psi; p = b p + r
-template <typename T>
-__global__ void __launch_bounds__(256) k_cg_update(int64_t n, cx<T> *__restrict__ psi, cx<T> *__restrict__ p,
-                                                   const cx<T> *__restrict__ r, const CGState *st) {
+// psi = a p + psi; p = b p + r  (FP64/FP32 per entry, FP16 per site)
+template <typename S>
+__global__ void __launch_bounds__(256) k_cg_update(int64_t nsites, int64_t stride, cx<S> *__restrict__ psi,
+                                                   cx<S> *__restrict__ p, const cx<S> *__restrict__ r,
+                                                   const CGState *st) {
+  typedef typename Sto<S>::C T;
   if (gated_prev(st)) return;
   const T a = (T)st->a, b = (T)st->b;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const cx<T> pv = p[i], xv = psi[i], rv = r[i];
-    psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
-    p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
+  if constexpr (!Sto<S>::H) {
+    const int64_t n = 12 * nsites;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+      const cx<T> pv = p[i], xv = psi[i], rv = r[i];
+      psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
+      p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
+    }
+  } else {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites;
+         j += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
+      cx<T> pv[12], xv[12], rv[12];
+      load_site<S>(pv, p + base, i, stride);
+      load_site<S>(xv, psi + base, i, stride);
+      load_site<S>(rv, r + base, i, stride);
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        xv[k] = {fma(a, pv[k].r, xv[k].r), fma(a, pv[k].i, xv[k].i)};
+        rv[k] = {fma(b, pv[k].r, rv[k].r), fma(b, pv[k].i, rv[k].i)};
+      }
+      store_site<S>(psi + base, xv, i, stride, false);
+      store_site<S>(p + base, rv, i, stride, false);
+    }
   }
 }
 
-template <typename T>
+template <typename S>
 int launch_cg_dot(Ctx *ctx, int64_t nsites, int64_t stride, const void *p, const void *mmp, CGState *state) {
   const int nb = red_blocks(nsites);
-  hipLaunchKernelGGL((k_cg_dot<T>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (const cx<T> *)p,
-                     (const cx<T> *)mmp, (const CGState *)state, ctx->red_partial);
+  hipLaunchKernelGGL((k_cg_dot<S>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (const cx<S> *)p,
+                     (const cx<S> *)mmp, (const CGState *)state, ctx->red_partial);
   hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb, state, 0);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
-template <typename T>
+template <typename S>
 int launch_cg_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const void *mmp, CGState *state) {
   const int nb = red_blocks(nsites);
-  hipLaunchKernelGGL((k_cg_axpy_norm<T>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (cx<T> *)r,
-                     (const cx<T> *)mmp, (const CGState *)state, ctx->red_partial);
+  hipLaunchKernelGGL((k_cg_axpy_norm<S>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (cx<S> *)r,
+                     (const cx<S> *)mmp, (const CGState *)state, ctx->red_partial);
   hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb, state, 1);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
-template <typename T>
-int launch_cg_update(Ctx *ctx, int64_t n, void *psi, void *p, const void *r, CGState *state) {
-  hipLaunchKernelGGL((k_cg_update<T>), dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, (cx<T> *)psi,
-                     (cx<T> *)p, (const cx<T> *)r, (const CGState *)state);
+template <typename S>
+int launch_cg_update(Ctx *ctx, int64_t nsites, int64_t stride, void *psi, void *p, const void *r, CGState *state) {
+  const int64_t work = Sto<S>::H ? nsites : 12 * nsites;
+  hipLaunchKernelGGL((k_cg_update<S>), dim3(stream_blocks(work)), dim3(256), 0, ctx->stream, nsites, stride,
+                     (cx<S> *)psi, (cx<S> *)p, (const cx<S> *)r, (const CGState *)state);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1112,28 +1251,37 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
   return GX_OK;
 }
 
-#define GX_INST(T)                                                                                                 \
-  template int launch_dhop_epi<T>(const Geom &, const void *, void *, const void *, int, int, const Gate *,       \
+#define GX_INST(S)                                                                                                 \
+  template int launch_dhop_epi<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,       \
                                   hipStream_t, int, const void *, const void *, double, double);                  \
-  template int launch_smat2<T>(const Geom &, const void *, const void *, double, void *, const void *, void *,     \
+  template int launch_smat2<S>(const Geom &, const void *, const void *, double, void *, const void *, void *,     \
                                const void *, const Gate *, hipStream_t);                                         \
+  template int launch_dhop<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,           \
+                              hipStream_t);                                                                        \
+  template int launch_smat<S>(const Geom &, const void *, const void *, void *, const void *, const Gate *,        \
+                              hipStream_t);                                                                        \
+  template int launch_axpby<S>(int64_t, int64_t, double, const void *, double, const void *, void *, const Gate *, \
+                               hipStream_t);                                                                       \
+  template int launch_reduce<S>(Ctx *, int, int64_t, int64_t, const void *, const void *, void *, double, int);   \
+  template int launch_cg_dot<S>(Ctx *, int64_t, int64_t, const void *, const void *, CGState *);                   \
+  template int launch_cg_axpy_norm<S>(Ctx *, int64_t, int64_t, void *, const void *, CGState *);                   \
+  template int launch_cg_update<S>(Ctx *, int64_t, int64_t, void *, void *, const void *, CGState *);
+GX_INST(double)
+GX_INST(float)
+GX_INST(half_t)
+#define GX_INST_REAL(T)                                                                                            \
   template int launch_slice_proj<T>(const Geom &, const void *, int, int, void *, int, int, int, int, hipStream_t); \
   template int launch_mrhs_reduce<T>(Ctx *, int, const Geom &, void *, const void *, void *, MrhsState *, double *); \
   template int launch_mrhs_update<T>(Ctx *, const Geom &, void *, void *, const void *, MrhsState *);             \
-  template int launch_dhop<T>(const Geom &, const void *, void *, const void *, int, int, const Gate *,           \
-                              hipStream_t);                                                                        \
-  template int launch_smat<T>(const Geom &, const void *, const void *, void *, const void *, const Gate *,        \
-                              hipStream_t);                                                                        \
-  template int launch_axpby<T>(int64_t, double, const void *, double, const void *, void *, const Gate *,          \
-                               hipStream_t);                                                                       \
-  template int launch_reduce<T>(Ctx *, int, int64_t, int64_t, const void *, const void *, void *, double, int);   \
-  template int launch_cg_dot<T>(Ctx *, int64_t, int64_t, const void *, const void *, CGState *);                   \
-  template int launch_cg_axpy_norm<T>(Ctx *, int64_t, int64_t, void *, const void *, CGState *);                   \
-  template int launch_cg_update<T>(Ctx *, int64_t, void *, void *, const void *, CGState *);                   \
   template int launch_gaussian<T>(const Geom &, void *, uint64_t, hipStream_t);
-GX_INST(double)
-GX_INST(float)
+GX_INST_REAL(double)
+GX_INST_REAL(float)
 template int launch_convert<double, float>(int64_t, const void *, void *, hipStream_t);
 template int launch_convert<float, double>(int64_t, const void *, void *, hipStream_t);
+template int launch_convert<double, half_t>(int64_t, const void *, void *, hipStream_t);
+template int launch_convert_site<double, half_t>(int64_t, int64_t, const void *, void *, hipStream_t);
+template int launch_convert_site<half_t, double>(int64_t, int64_t, const void *, void *, hipStream_t);
+template int launch_convert_site<float, half_t>(int64_t, int64_t, const void *, void *, hipStream_t);
+template int launch_convert_site<half_t, float>(int64_t, int64_t, const void *, void *, hipStream_t);
 
 }  // namespace gx

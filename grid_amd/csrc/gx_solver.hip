@@ -140,7 +140,7 @@ static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGSt
   const int64_t nsites = p->g.half_elems();
   GX_TRY(launch_cg_dot<T>(ctx, nsites, nsites, p->data, mmp->data, st));
   GX_TRY(launch_cg_axpy_norm<T>(ctx, nsites, nsites, r->data, mmp->data, st));
-  GX_TRY(launch_cg_update<T>(ctx, p->elems(), psi->data, p->data, r->data, st));
+  GX_TRY(launch_cg_update<T>(ctx, nsites, nsites, psi->data, p->data, r->data, st));
   return GX_OK;
 }
 
@@ -211,9 +211,7 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
     if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
   auto one_batch = [&]() -> int {
     for (int i = 0; i < batch; i++) {
-      int rc = prec == GX_F64 ? cg_iteration<double>(op, psi, p, mmp, r, w.dev)
-                              : cg_iteration<float>(op, psi, p, mmp, r, w.dev);
-      GX_TRY(rc);
+      GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
     }
     return GX_OK;
   };
@@ -225,9 +223,7 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
       enqueued += batch;
     } else {
       for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
-        int rc = prec == GX_F64 ? cg_iteration<double>(op, psi, p, mmp, r, w.dev)
-                                : cg_iteration<float>(op, psi, p, mmp, r, w.dev);
-        GX_TRY(rc);
+        GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
       }
     }
     GX_CHECK_HIP(hipMemcpyAsync(&w.host[s], w.dev, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
@@ -545,7 +541,7 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
     set_error("mixed_cg: needs F64 half fields");
     return GX_EINVAL;
   }
-  if (inner_prec != GX_F32) {
+  if (inner_prec != GX_F32 && inner_prec != GX_F16) {
     set_error("mixed_cg: inner precision %d unsupported", inner_prec);
     return GX_EINVAL;
   }
@@ -582,6 +578,10 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
   if (!se || !so || !t || !Mt || !solo) return GX_ENOMEM;
   const bool multi = op->kind == GX_WILSON && op->g.Ls > 1;
   const int Ls = op->g.Ls;
+  if (mixed < 0 || mixed > 2 || (multi && mixed == 2)) {
+    set_error("schur_solve: mixed must be 0 (FP64), 1 (FP32 inner) or 2 (FP16 inner, single-RHS operators)");
+    return GX_EINVAL;
+  }
   hipStream_t s = op->ctx->stream;
   const size_t hb = se->bytes;
   int rc = GX_OK;
@@ -615,7 +615,7 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
         }
       }
     } else if (mixed) {
-      rc = mixed_cg_run(op, so, solo, tol, 1e-5, GX_F32, maxit, 100, st);
+      rc = mixed_cg_run(op, so, solo, tol, 1e-5, mixed == 2 ? GX_F16 : GX_F32, maxit, 100, st);
     } else {
       rc = cg_run(op, so, solo, tol, maxit, st);
     }

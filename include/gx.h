@@ -76,6 +76,11 @@ int gx_pick_checkerboard(int cb, gx_field half, gx_field full);
 int gx_set_checkerboard(gx_field full, gx_field half);
 /* Grid: precisionChange(out, in), Lattice_transfer.h:1312-1492 */
 int gx_precision_change(gx_field out, gx_field in);
+/* Grid: ExtractSlice(lowDim, higherDim, slice, orthog) / InsertSlice(lowDim, higherDim, slice,
+ * orthog), Lattice_transfer.h:949-1047, with orthog the fifth dimension: a 4D field (Ls = 1) from
+ * or into slice s of a field on the same 4D lattice.  Full fields of one precision. */
+int gx_extract_slice(gx_field out4d, gx_field in5d, int s);
+int gx_insert_slice(gx_field in4d, gx_field out5d, int s);
 
 /* ---- BLAS1 / reductions (Grid/lattice/Lattice_arith.h:231-265; Lattice_reduction.h:202-410).
  *      Per-site products in the field's precision, global sums accumulated in double with a
@@ -173,7 +178,8 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
                 int maxinner, int maxouter, gx_solver_stats *st);
 
 /* Grid: SchurRedBlackDiagMooeeSolve<Field>(CG)(Dw, src, sol), SchurRedBlack.h:236-291,373-436.
- * src, sol: full F64 fields.  mixed != 0 uses gx_mixed_cg (inner FP32) for the red-black solve.
+ * src, sol: full F64 fields.  mixed = 1 (2) runs the red-black solve as gx_mixed_cg with FP32
+ * (FP16-storage) inner solves, mixed = 0 as FP64 gx_cg.
  * unprec_residual = |M sol - src| / |src|.  On a multi-RHS Wilson operator (Ls > 1) the
  * red-black solve is gx_block_cg (every right-hand side to tol), st holds the largest iteration
  * count and residuals over the right-hand sides, and unprec_residual is the largest
@@ -269,6 +275,10 @@ int gx_point_source(const int dims[4], int Ls, const int x[4], int spin, int col
 /* Device-side gx_gaussian_fermion straight into a full F64 or F32 field (same numbering; equal to
  * the host field up to the rounding of log, sqrt, sin and cos). */
 int gx_field_gaussian(gx_field f, uint64_t seed);
+/* Device-side gx_point_source into a full field: 1 at (x, spin, colour) in every slice s; with
+ * spin = colour = -1, slice s (s < 12) gets spin-colour component s instead, the 12 point sources
+ * of one site as one multi-RHS field. */
+int gx_field_point_source(gx_field f, const int x[4], int spin, int colour);
 /* Order-independent 64-bit digest of a field's stored bits: equal digests <=> (with overwhelming
  * probability) bitwise equal fields.  No Grid counterpart; used to compare solutions cheaply. */
 int gx_field_digest(gx_field f, uint64_t *digest);

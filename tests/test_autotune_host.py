@@ -1,0 +1,58 @@
+"""CPU checks of the autotuner's search and cache logic (no device work: the timer is injected)."""
+import json
+
+import grid_amd as gx
+from grid_amd import autotune as at
+
+
+def test_search_picks_minimum_over_rounds():
+    calls = []
+
+    def measure(c):
+        calls.append(c)
+        base = {'a': 3.0, 'b': 1.0, 'c': 2.0}[c]
+        return base + (0.0 if len(calls) > 3 else 10.0)  # the first round is noisy
+
+    best, times = at.search(['a', 'b', 'c'], measure, rounds=3)
+    assert best == 'b' and times == {'a': 3.0, 'b': 1.0, 'c': 2.0}
+    assert calls == ['a', 'b', 'c'] * 3  # round-robin
+
+
+def test_search_breaks_ties_by_order():
+    best, _ = at.search(['x', 'y'], lambda c: 1.0, rounds=2)
+    assert best == 'x'
+
+
+def test_candidates_cover_knob_space():
+    c16 = at.hop_candidates(16)
+    c8 = at.hop_candidates(8)
+    assert len(c16) == 16 and len(c8) == 8 and len(set(c16)) == 16
+    assert all(dict(c)[gx.TUNE_DHOP_ORDER] == 0 for c in c8)
+    assert len(at.solver_candidates()) == 4
+
+
+def test_cache_round_trip_and_apply(tmp_path):
+    path = tmp_path / 'tune.json'
+    key = at.workload_key('gfx950', 'mobius', (16, 16, 16, 16), 16, gx.F64)
+    entry = {'knobs': {'dhop_map': 1, 'dhop_pf': 0, 'dhop_nt': 1, 'dhop_order': 0, 'fused': 1, 'cg_graph': 0}}
+    path.write_text(json.dumps({key: entry}))
+    saved = {k: gx.tune_get(k) for k in at.KNOB_NAMES}
+    try:
+        t = at.Autotuner(ctx=None, cache_path=str(path), device='gfx950')
+        vals = t.apply('mobius', (16, 16, 16, 16), 16, gx.F64)
+        assert vals[gx.TUNE_DHOP_MAP] == 1 and gx.tune_get(gx.TUNE_DHOP_PF) == 0
+        assert gx.tune_get(gx.TUNE_CG_GRAPH) == 0
+        assert t.apply('mobius', (8, 8, 8, 8), 16, gx.F64) is None
+        t.cache['other'] = {'knobs': {}}
+        t.save()
+        assert set(json.loads(path.read_text())) == {key, 'other'}
+    finally:
+        for k, v in saved.items():
+            gx.tune_set(k, v)
+
+
+def test_knobs_context_restores():
+    before = gx.tune_get(gx.TUNE_DHOP_NT)
+    with at.Knobs(((gx.TUNE_DHOP_NT, 1 - before),)):
+        assert gx.tune_get(gx.TUNE_DHOP_NT) == 1 - before
+    assert gx.tune_get(gx.TUNE_DHOP_NT) == before

@@ -83,7 +83,7 @@ def test_multi_rhs_schur_solve(ctx):
         for k in range(nrhs):
             ref, _ = O.schur_solve(np.ascontiguousarray(src[..., k:k + 1, :, :]), tol=1e-10)
             assert rel_err(x[..., k:k + 1, :, :], ref) < 1e-8
-            Mx = O.apply(np.ascontiguousarray(x[..., k:k + 1, :, :]), 'M')
+            Mx = O.apply('M', np.ascontiguousarray(x[..., k:k + 1, :, :]))
             worst = max(worst, rel_err(Mx, src[..., k:k + 1, :, :]))
         assert abs(worst - st['unprec_residual']) <= 1e-3 * worst + 1e-15
 
@@ -142,3 +142,25 @@ def test_serve_admission_respects_budget(ctx):
     assert len(ok) == 4 and res[-1]['status'] == 2  # GX_ENOMEM
     for a, b in zip(ok, ok[1:]):
         assert b['t_admit'] >= a['t_done'] - 1e-6 or b['slot'] == a['slot']
+
+
+def test_device_point_sources_and_slices(ctx):
+    """Device point sources equal the host ones; the spin-colour batch puts component s in slice
+    s; ExtractSlice/InsertSlice move one fifth-dimension slice bitwise."""
+    dims, x = (4, 6, 4, 8), (3, 1, 2, 5)
+    for prec in (gx.F64, gx.F32):
+        f = gx.Field(ctx, dims, 3, prec).point_source(x, 2, 1)
+        assert np.array_equal(f.download(), gx.point_source(dims, 3, x, 2, 1))
+        b = gx.Field(ctx, dims, 12, prec).point_source(x).download()
+        for s in range(12):
+            want = gx.point_source(dims, 1, x, s // 3, s % 3)
+            assert np.array_equal(b[..., s:s + 1, :, :], want)
+    a = gx.gaussian_fermion(dims, 5, 8)
+    f5, f4 = gx.Field(ctx, dims, 5).upload(a), gx.Field(ctx, dims, 1)
+    gx.extract_slice(f4, f5, 3)
+    assert np.array_equal(f4.download(), a[..., 3:4, :, :])
+    g5 = gx.Field(ctx, dims, 5)
+    g5.zero()
+    gx.insert_slice(f4, g5, 1)
+    got = g5.download()
+    assert np.array_equal(got[..., 1:2, :, :], a[..., 3:4, :, :]) and not np.any(got[..., [0, 2, 3, 4], :, :])
