@@ -124,6 +124,7 @@ _SIG = {
     'gx_mixed_cg': ([_P, _P, _P, _D, _D, _I, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
     'gx_block_cg': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_reliable_update_cg': ([_P, _P, _P, _D, _I, _D, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_tune_set': ([_I, _I], _I),
     'gx_tune_get': ([_I, _IP], _I),
     'gx_random_gauge': ([_IP, ctypes.c_uint64, _D, _DP], _I),
@@ -401,6 +402,14 @@ class FermionOp:
     def mixed_cg(self, src, psi, tol=1e-10, inner_tol=1e-5, inner_prec=F32, maxinner=10000, maxouter=100):
         st = SolverStats()
         rc = lib().gx_mixed_cg(self.h, src.h, psi.h, tol, inner_tol, inner_prec, maxinner, maxouter, ctypes.byref(st))
+        if rc not in (0, 3):
+            _check(rc)
+        return st.as_dict()
+
+    def reliable_update_cg(self, src, psi, tol=1e-10, maxit=10000, delta=0.1, sloppy_prec=F32):
+        """ConjugateGradientReliableUpdate: sloppy-precision CG with FP64 reliable updates."""
+        st = SolverStats()
+        rc = lib().gx_reliable_update_cg(self.h, src.h, psi.h, tol, maxit, delta, sloppy_prec, ctypes.byref(st))
         if rc not in (0, 3):
             _check(rc)
         return st.as_dict()

@@ -75,8 +75,14 @@ struct Gate {
 };
 
 // Device-resident scalars of one CG (ConjugateGradient.h:68-257).
+// With delta > 0 (reliable-update CG, ConjugateGradientReliableUpdate.h:140-252) the norm
+// finaliser also tracks the largest residual since the last reliable update and, when the
+// residual falls below delta times it, stops the iteration with ru = 1 so the host can recompute
+// the residual in FP64; the solution update of that iteration then leaves p for the host.
 struct CGState : Gate {
   double c, cp, a, b, d, rsq;
+  double delta, maxres, cprev;  // cprev: c of the last finished iteration
+  int ru, pad2;
 };
 
 // Per-RHS solver scalars for the multi-RHS CG (right-hand side = slice s of the fifth dimension).

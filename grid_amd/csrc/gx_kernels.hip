@@ -929,10 +929,17 @@ __global__ void __launch_bounds__(256) k_cg_fin(const double *partial, int npart
     } else {
       if (!skip) {
         const double cp = v[0];
+        st->cprev = st->c;
         st->b = cp / st->c;
         st->cp = cp;
         st->c = cp;
-        if (cp <= st->rsq && st->k_conv == INT_MAX) st->k_conv = st->k;
+        if (st->delta > 0 && cp > st->maxres) st->maxres = cp;
+        if (cp <= st->rsq && st->k_conv == INT_MAX) {
+          st->k_conv = st->k;
+        } else if (st->delta > 0 && cp < st->delta * st->maxres && st->k_conv == INT_MAX) {
+          st->ru = 1;
+          st->k_conv = st->k;
+        }
       }
       st->k = st->k + 1;
     }
@@ -947,12 +954,16 @@ __global__ void __launch_bounds__(256) k_cg_update(int64_t nsites, int64_t strid
   typedef typename Sto<S>::C T;
   if (gated_prev(st)) return;
   const T a = (T)st->a, b = (T)st->b;
+  const bool keep_p = st->ru != 0;  // reliable update pending: the host forms the new p
   if constexpr (!Sto<S>::H) {
     const int64_t n = 12 * nsites;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-      const cx<T> pv = p[i], xv = psi[i], rv = r[i];
+      const cx<T> pv = p[i], xv = psi[i];
       psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
-      p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
+      if (!keep_p) {
+        const cx<T> rv = r[i];
+        p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
+      }
     }
   } else {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites;
@@ -968,7 +979,7 @@ __global__ void __launch_bounds__(256) k_cg_update(int64_t nsites, int64_t strid
         rv[k] = {fma(b, pv[k].r, rv[k].r), fma(b, pv[k].i, rv[k].i)};
       }
       store_site<S>(psi + base, xv, i, stride, false);
-      store_site<S>(p + base, rv, i, stride, false);
+      if (!keep_p) store_site<S>(p + base, rv, i, stride, false);
     }
   }
 }

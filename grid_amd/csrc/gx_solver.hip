@@ -144,6 +144,61 @@ static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGSt
   return GX_OK;
 }
 
+// Device-resident CG iterations on (psi, p, mmp, r) of precision prec, from the state in w.dev,
+// until the state stops them (convergence, or a reliable update with delta > 0) or k passes maxit.
+// Batches of iterations are enqueued two ahead so the device never waits for the host; enqueued
+// counts the iterations enqueued so far and carries over between calls.  *out: the final state.
+static int cg_device_loop(Op *op, int prec, Field *psi, Field *p, Field *mmp, Field *r, CGWork &w, int maxit,
+                          int &enqueued, CGState *out) {
+  Ctx *ctx = op->ctx;
+  const int64_t vol = op->g.half_elems();
+  const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
+  const bool use_graph = tune_get(GX_TUNE_CG_GRAPH) != 0;
+  for (int i = 0; i < 4; i++)
+    if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
+  auto one_batch = [&]() -> int {
+    for (int i = 0; i < batch; i++) {
+      GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
+    }
+    return GX_OK;
+  };
+  auto enqueue = [&](int s) -> int {
+    if (use_graph) {
+      const void *key[8] = {psi->data, p->data,   mmp->data, r->data, (const void *)(intptr_t)(prec + 1),
+                             w.dev,     (const void *)op, (const void *)(uintptr_t)op->epoch};
+      GX_TRY(run_batch(ctx, w.graph[prec], key, batch, one_batch));
+      enqueued += batch;
+    } else {
+      for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+        GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
+      }
+    }
+    GX_CHECK_HIP(hipMemcpyAsync(&w.host[s], w.dev, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
+    GX_CHECK_HIP(hipEventRecord(w.ev[s], ctx->stream));
+    return GX_OK;
+  };
+  int slot = 0;
+  GX_TRY(enqueue(0));
+  GX_TRY(enqueue(1));
+  CGState fin{};
+  for (;;) {
+    GX_CHECK_HIP(hipEventSynchronize(w.ev[slot]));
+    fin = w.host[slot];
+    if (fin.k_conv != INT_MAX || fin.k > maxit) break;
+    if (enqueued >= maxit) {
+      // everything enqueued: wait for the other slot, which holds the last state
+      GX_CHECK_HIP(hipEventSynchronize(w.ev[1 - slot]));
+      fin = w.host[1 - slot];
+      break;
+    }
+    GX_TRY(enqueue(slot));
+    slot = 1 - slot;
+  }
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  *out = fin;
+  return GX_OK;
+}
+
 int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *out) {
   auto t0 = std::chrono::steady_clock::now();
   const int prec = src->prec;
@@ -202,51 +257,9 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
   GX_CHECK_HIP(hipMemcpyAsync(w.dev, &w.host[0], sizeof(CGState), hipMemcpyHostToDevice, ctx->stream));
   GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
 
-  // Batches of iterations; two batches in flight so the device never waits for the host.
-  const int64_t vol = op->g.half_elems();
-  const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
-  int enqueued = 0, slot = 0;
-  const bool use_graph = tune_get(GX_TUNE_CG_GRAPH) != 0;
-  for (int i = 0; i < 4; i++)
-    if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
-  auto one_batch = [&]() -> int {
-    for (int i = 0; i < batch; i++) {
-      GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
-    }
-    return GX_OK;
-  };
-  auto enqueue = [&](int s) -> int {
-    if (use_graph) {
-      const void *key[8] = {psi->data, p->data,   mmp->data, r->data, (const void *)(intptr_t)(prec + 1),
-                             w.dev,     (const void *)op, (const void *)(uintptr_t)op->epoch};
-      GX_TRY(run_batch(ctx, w.graph[prec], key, batch, one_batch));
-      enqueued += batch;
-    } else {
-      for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
-        GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
-      }
-    }
-    GX_CHECK_HIP(hipMemcpyAsync(&w.host[s], w.dev, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
-    GX_CHECK_HIP(hipEventRecord(w.ev[s], ctx->stream));
-    return GX_OK;
-  };
-  GX_TRY(enqueue(0));
-  GX_TRY(enqueue(1));
+  int enqueued = 0;
   CGState fin{};
-  for (;;) {
-    GX_CHECK_HIP(hipEventSynchronize(w.ev[slot]));
-    fin = w.host[slot];
-    if (fin.k_conv != INT_MAX || fin.k > maxit) break;
-    if (enqueued >= maxit) {
-      // everything enqueued: wait for the other slot, which holds the last state
-      GX_CHECK_HIP(hipEventSynchronize(w.ev[1 - slot]));
-      fin = w.host[1 - slot];
-      break;
-    }
-    GX_TRY(enqueue(slot));
-    slot = 1 - slot;
-  }
-  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  GX_TRY(cg_device_loop(op, prec, psi, p, mmp, r, w, maxit, enqueued, &fin));
   const bool converged = fin.k_conv != INT_MAX;
   st.iterations = converged ? fin.k_conv : maxit;
   st.converged = converged;
@@ -303,6 +316,106 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
   const int rc = cg_run(op, src_in, sol, tol, maxinner, &fst);
   st.final_iterations = fst.iterations;
   st.iterations = st.inner_iterations + fst.iterations;
+  st.true_residual = fst.true_residual;
+  st.computed_residual = fst.computed_residual;
+  st.converged = fst.converged;
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (out) *out = st;
+  return rc;
+}
+
+// ConjugateGradientReliableUpdate (ConjugateGradientReliableUpdate.h:77-269): the Krylov vectors
+// (psi_f, p, mmp, r) live in the sloppy precision sp while psi accumulates in FP64.  The device
+// iteration stops itself when the residual falls below delta times its largest value since the
+// last update; the host then folds psi_f into psi, recomputes r = src - A psi in FP64, restarts
+// psi_f at zero, forms p = (cp_new / c) p + r, and resumes.  After convergence an FP64 CG from psi
+// cleans up (DoFinalCleanup).
+int ru_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, double delta, int sp, gx_solver_stats *out) {
+  auto t0 = std::chrono::steady_clock::now();
+  GX_TRY(op_prepare(op, GX_F64));
+  GX_TRY(op_prepare(op, sp));
+  Field *md = op_tmp(op, GX_F64, 7), *rd = op_tmp(op, GX_F64, 8);
+  Field *pf = op_tmp(op, sp, 4), *mf = op_tmp(op, sp, 5), *rf = op_tmp(op, sp, 6), *xf = op_tmp(op, sp, 8);
+  if (!md || !rd || !pf || !mf || !rf || !xf) return GX_ENOMEM;
+  const int cb = src->cb;
+  psi->cb = md->cb = rd->cb = pf->cb = mf->cb = rf->cb = xf->cb = cb;
+  Ctx *ctx = op->ctx;
+  gx_solver_stats st{};
+  double ssq, guess, cp;
+  GX_TRY(blas_norm2(src, &ssq));
+  GX_TRY(blas_norm2(psi, &guess));
+  if (std::isnan(guess) || std::isnan(ssq)) {
+    set_error("reliable-update CG: NaN in source or guess");
+    return GX_ENAN;
+  }
+  if (ssq == 0.0) {
+    GX_CHECK_HIP(hipMemsetAsync(psi->data, 0, psi->bytes, ctx->stream));
+    st.converged = 1;
+    if (out) *out = st;
+    return GX_OK;
+  }
+  GX_TRY(apply_mpcdagmpc(op, psi, md));
+  GX_TRY(blas_axpy_norm(rd, -1.0, md, src, &cp));  // r = src - A psi
+  const double rsq = tol * tol * ssq;
+  if (cp <= rsq) {
+    st.converged = 1;
+    st.true_residual = st.computed_residual = std::sqrt(cp / ssq);
+    if (out) *out = st;
+    return GX_OK;
+  }
+  GX_TRY(convert_field(rf, rd));
+  GX_CHECK_HIP(hipMemsetAsync(xf->data, 0, xf->bytes, ctx->stream));
+  GX_CHECK_HIP(hipMemcpyAsync(pf->data, rf->data, rf->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  SolverWork *sw = nullptr;
+  GX_TRY(solver_work(op, &sw));
+  CGWork &w = sw->cg;
+  CGState init{};
+  init.c = init.cp = init.maxres = cp;
+  init.rsq = rsq;
+  init.delta = delta;
+  init.k = 1;
+  init.k_conv = INT_MAX;
+  init.kmax = maxit;
+  w.host[0] = init;
+  GX_CHECK_HIP(hipMemcpyAsync(w.dev, &w.host[0], sizeof(CGState), hipMemcpyHostToDevice, ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  int enqueued = 0, updates = 0, sloppy_its = maxit;
+  for (;;) {
+    CGState fin{};
+    GX_TRY(cg_device_loop(op, sp, xf, pf, mf, rf, w, maxit, enqueued, &fin));
+    GX_TRY(convert_field(md, xf));
+    GX_TRY(blas_axpby(psi, 1.0, psi, 1.0, md));  // psi += psi_f
+    if (fin.k_conv == INT_MAX || !fin.ru) {        // converged, or out of iterations
+      if (fin.k_conv != INT_MAX) sloppy_its = fin.k_conv;
+      break;
+    }
+    // reliable update (ConjugateGradientReliableUpdate.h:209-231)
+    double cpn;
+    GX_TRY(apply_mpcdagmpc(op, psi, md));
+    GX_TRY(blas_axpy_norm(rd, -1.0, md, src, &cpn));
+    GX_TRY(convert_field(rf, rd));
+    GX_CHECK_HIP(hipMemsetAsync(xf->data, 0, xf->bytes, ctx->stream));
+    const double b = cpn / fin.cprev;
+    GX_TRY(blas_axpby(pf, b, pf, 1.0, rf));  // p = b p + r (ConjugateGradientReliableUpdate.h:234)
+    updates++;
+    CGState rs = fin;
+    rs.c = rs.cp = rs.maxres = cpn;
+    rs.b = b;
+    rs.ru = 0;
+    rs.k = fin.k_conv + 1;
+    rs.k_conv = INT_MAX;
+    enqueued = fin.k_conv;
+    if (rs.k > maxit) break;
+    w.host[0] = rs;
+    GX_CHECK_HIP(hipMemcpyAsync(w.dev, &w.host[0], sizeof(CGState), hipMemcpyHostToDevice, ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  gx_solver_stats fst{};
+  const int rc = cg_run(op, src, psi, tol, maxit, &fst);  // final FP64 cleanup
+  st.inner_iterations = sloppy_its;
+  st.outer_iterations = updates;
+  st.final_iterations = fst.iterations;
+  st.iterations = sloppy_its + fst.iterations;
   st.true_residual = fst.true_residual;
   st.computed_residual = fst.computed_residual;
   st.converged = fst.converged;
@@ -546,6 +659,20 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
     return GX_EINVAL;
   }
   return mixed_cg_run(op, src, psi, tol, inner_tol, inner_prec, maxinner, maxouter, st);
+}
+
+int gx_reliable_update_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, double delta, int sloppy_prec,
+                          gx_solver_stats *st) {
+  if (!op || !src || !psi || src == psi || src->prec != GX_F64 || psi->prec != GX_F64 || src->subset == GX_FULL ||
+      psi->subset == GX_FULL || !same_geom(src->g, op->g) || !same_geom(psi->g, op->g)) {
+    set_error("reliable_update_cg: needs F64 half fields on the operator's lattice");
+    return GX_EINVAL;
+  }
+  if (!(delta > 0.0 && delta < 1.0) || (sloppy_prec != GX_F32 && sloppy_prec != GX_F16)) {
+    set_error("reliable_update_cg: needs 0 < delta < 1 and a GX_F32 or GX_F16 sloppy precision");
+    return GX_EINVAL;
+  }
+  return ru_cg_run(op, src, psi, tol, maxit, delta, sloppy_prec, st);
 }
 
 int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, int inner_prec,

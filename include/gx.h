@@ -177,6 +177,16 @@ int gx_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver
 int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_tol, int inner_prec,
                 int maxinner, int maxouter, gx_solver_stats *st);
 
+/* Grid: ConjugateGradientReliableUpdate<FieldD,FieldF>(tol, maxit, delta, sp_grid, Linop_f,
+ * Linop_d)(src, psi), ConjugateGradientReliableUpdate.h:77-269.  CG on MpcDagMpc whose Krylov
+ * vectors live in sloppy_prec (GX_F32, or GX_F16 storage) while psi accumulates in FP64; when the
+ * residual falls below delta times its largest value since the last update, the residual is
+ * recomputed in FP64 from psi (a reliable update); a final FP64 CG from psi cleans up
+ * (DoFinalCleanup).  st: inner_iterations = sloppy iterations, outer_iterations = reliable
+ * updates, final_iterations = cleanup iterations.  src, psi: F64 half fields; 0 < delta < 1. */
+int gx_reliable_update_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, double delta,
+                          int sloppy_prec, gx_solver_stats *st);
+
 /* Grid: SchurRedBlackDiagMooeeSolve<Field>(CG)(Dw, src, sol), SchurRedBlack.h:236-291,373-436.
  * src, sol: full F64 fields.  mixed = 1 (2) runs the red-black solve as gx_mixed_cg with FP32
  * (FP16-storage) inner solves, mixed = 0 as FP64 gx_cg.

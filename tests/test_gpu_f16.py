@@ -88,3 +88,22 @@ def test_f16_inner_mixed_cg(ctx, kind, Ls):
     assert s16['converged'] and s16['true_residual'] < 1e-9, s16
     assert s16['outer_iterations'] >= 1 and s16['inner_iterations'] > 0
     assert rel_err(x16.download(), x64.download()) < 1e-8
+
+
+@pytest.mark.parametrize('sloppy', [gx.F32, gx.F16])
+@pytest.mark.parametrize('kind,Ls', [('wilson', 1), ('mobius', 8)])
+def test_reliable_update_cg(ctx, kind, Ls, sloppy):
+    """ConjugateGradientReliableUpdate: sloppy-precision Krylov vectors with FP64 reliable updates
+    reach the FP64 tolerance and the FP64 CG solution; with delta = 0.1 several updates happen."""
+    dims = (8, 8, 8, 4)
+    U = gx.random_gauge(dims, 85, 0.3)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, mass=0.05)
+    src = half(gx.gaussian_fermion(dims, Ls, 86), dims, gx.ODD)
+    fs = D.field(gx.F64, gx.ODD).upload(src)
+    x64, xru = D.field(gx.F64, gx.ODD), D.field(gx.F64, gx.ODD)
+    s64 = D.cg(fs, x64, tol=1e-10)
+    sru = D.reliable_update_cg(fs, xru, tol=1e-10, delta=0.1, sloppy_prec=sloppy)
+    assert sru['converged'] and sru['true_residual'] < 1e-9, sru
+    assert sru['outer_iterations'] >= 2, sru          # reliable updates performed
+    assert sru['final_iterations'] < 0.2 * s64['iterations'] + 5, (sru, s64)
+    assert rel_err(xru.download(), x64.download()) < 1e-8
