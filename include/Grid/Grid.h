@@ -529,6 +529,21 @@ typedef MobiusFermion<GX_F64> MobiusFermionD;
 typedef MobiusFermion<GX_F32> MobiusFermionF;
 typedef MobiusFermionD MobiusFermionR;
 
+// Grid: DomainWallFermion(U, FGrid, FrbGrid, UGrid, UrbGrid, mass, M5), DomainWallFermion.h:120-135:
+// the Shamir kernel, i.e. the Cayley form with b = 1, c = 0.
+template <int PREC>
+class DomainWallFermion : public MobiusFermion<PREC> {
+ public:
+  DomainWallFermion(const LatticeGaugeFieldD &U, GridCartesian &FiveDimGrid, GridRedBlackCartesian &FiveDimRedBlackGrid,
+                    GridCartesian &FourDimGrid, GridRedBlackCartesian &FourDimRedBlackGrid, RealD mass, RealD m5,
+                    const ImplParams &p = ImplParams())
+      : MobiusFermion<PREC>(U, FiveDimGrid, FiveDimRedBlackGrid, FourDimGrid, FourDimRedBlackGrid, mass, m5, 1.0, 0.0,
+                            p) {}
+};
+typedef DomainWallFermion<GX_F64> DomainWallFermionD;
+typedef DomainWallFermion<GX_F32> DomainWallFermionF;
+typedef DomainWallFermionD DomainWallFermionR;
+
 // ---------------- linear operators ----------------
 template <class Field>
 class LinearOperatorBase {
@@ -735,6 +750,54 @@ class MixedPrecisionConjugateGradient {
     std::cout << GridLogMessage << "MixedPrecisionConjugateGradient: Inner CG iterations " << TotalInnerIterations
               << " Restarts " << TotalOuterIterations << " Final CG iterations " << TotalFinalStepIterations
               << " True residual " << TrueResidual << std::endl;
+  }
+};
+
+// Grid: ConjugateGradientReliableUpdate<FieldD,FieldF>(tol, maxit, delta, sp_grid, Linop_f, Linop_d,
+// err_on_no_conv)(src, psi), ConjugateGradientReliableUpdate.h:36-269.  The sloppy Krylov vectors
+// are FieldF's precision (FP32); SloppyHalf = true keeps them in FP16 storage instead.  The final
+// FP64 cleanup CG always runs (Grid's DoFinalCleanup default).
+template <class FieldD, class FieldF>
+class ConjugateGradientReliableUpdate {
+ public:
+  bool ErrorOnNoConverge;
+  RealD Tolerance;
+  Integer MaxIterations;
+  Integer IterationsToComplete = 0, ReliableUpdatesPerformed = 0, IterationsToCleanup = 0;
+  bool DoFinalCleanup = true;
+  bool SloppyHalf = false;
+  LinearOperatorBase<FieldF> &Linop_f;
+  LinearOperatorBase<FieldD> &Linop_d;
+  GridBase *SinglePrecGrid;
+  RealD Delta;
+  RealD TrueResidual = 0;
+  ConjugateGradientReliableUpdate(RealD tol, Integer maxit, RealD delta, GridBase *sp_grid,
+                                  LinearOperatorBase<FieldF> &lf, LinearOperatorBase<FieldD> &ld,
+                                  bool err_on_no_conv = true)
+      : ErrorOnNoConverge(err_on_no_conv), Tolerance(tol), MaxIterations(maxit), Linop_f(lf), Linop_d(ld),
+        SinglePrecGrid(sp_grid), Delta(delta) {
+    assert(Delta > 0. && Delta < 1. && "Expect  0 < Delta < 1");
+  }
+  void operator()(const FieldD &src, FieldD &psi) {
+    gx_op op = Linop_d.schur_handle();
+    assert(op && Linop_f.schur_handle());
+    psi.Checkerboard() = src.Checkerboard();
+    gx_solver_stats st{};
+    const int rc = gx_reliable_update_cg(op, src.f, psi.f, Tolerance, MaxIterations, Delta,
+                                         SloppyHalf ? GX_F16 : GX_F32, &st);
+    IterationsToComplete = st.inner_iterations;
+    ReliableUpdatesPerformed = st.outer_iterations;
+    IterationsToCleanup = st.final_iterations;
+    TrueResidual = st.true_residual;
+    if (rc == GX_ENOCONV) {
+      std::cout << GridLogMessage << "ConjugateGradientReliableUpdate did NOT converge" << std::endl;
+      if (ErrorOnNoConverge) assert(0);
+      return;
+    }
+    detail::check(rc, "ConjugateGradientReliableUpdate");
+    std::cout << GridLogMessage << "ConjugateGradientReliableUpdate Converged on iteration " << IterationsToComplete
+              << " after " << ReliableUpdatesPerformed << " reliable updates; cleanup iterations "
+              << IterationsToCleanup << "; true residual " << TrueResidual << std::endl;
   }
 };
 

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(ROOT, 'tests', 'cpp', 'bin')
 CASES = [('Test_wilson_even_odd', '8.8.8.8'), ('Test_wilson_even_odd', '4.6.8.6'),
          ('Test_mobius_even_odd', '8.8.4.4'), ('Test_wilson_cg_schur', '8.8.8.8'),
-         ('Test_dwf_mixedcg_prec', '8.8.8.8')]
+         ('Test_dwf_mixedcg_prec', '8.8.8.8'), ('Test_dwf_relupcg_prec', '8.8.8.8')]
 
 
 @pytest.mark.parametrize('name,grid', CASES)
