@@ -1221,7 +1221,7 @@ static int check_io(Op *op, Field *in, Field *out, bool full) {
   GX_REQUIRE(op && in && out, GX_EINVAL, "null argument");
   GX_REQUIRE(same_geom(in->g, op->g) && same_geom(out->g, op->g), GX_EINVAL, "field geometry does not match operator");
   GX_REQUIRE(in->prec == out->prec, GX_EINVAL, "in/out precision differ");
-  GX_REQUIRE(in->prec == GX_F64 || in->prec == GX_F32, GX_EINVAL, "operator precision unsupported");
+  GX_REQUIRE(in->prec == GX_F64 || in->prec == GX_F32 || in->prec == GX_F16, GX_EINVAL, "operator precision unsupported");
   GX_REQUIRE((in->subset == GX_FULL) == full && (out->subset == GX_FULL) == full, GX_EINVAL,
              full ? "operator needs full fields" : "operator needs half fields");
   GX_REQUIRE(in != out, GX_EINVAL, "in and out must differ");
