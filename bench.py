@@ -93,13 +93,15 @@ def cpu_baseline(U, psi_o, seconds=12.0):
     return el / n, n, ncores
 
 
-def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10):
+def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10, tune=None):
     """Mobius 24^4 x Ls12 (b=1.5, c=0.5, M5=1.8, mf=0.005, eps 0.3), Gaussian 5D source: red-black
     solve to 1e-10 in FP64 CG and in defect correction with FP32 and with FP16-storage inner solves
     (inner tolerance 1e-5, FP64 outer residual and patch-up).  Wall clock around each solve."""
     kw = dict(mass=0.005, M5=1.8, b=1.5, c=0.5)
     U = gx.random_gauge(dims, 1003, 0.3)
     D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    if tune:
+        tune('c3', D, gx.F32, rounds=2, cg_iters=32)
     src = D.field().gaussian(2003)
     out, ref = {'workload': 'Mobius 24^4 x Ls12 mf=0.005, Gaussian source'}, None
     for name, mixed in (('fp64', 0), ('mixed_f32', 1), ('mixed_f16', 2)):
@@ -123,7 +125,7 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10):
     return out
 
 
-def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-10):
+def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-10, tune=None):
     """Wilson 32^3x64 (eps 0.3, m 0.05), the 12 spin-colour point sources of one seeded site: one
     mixed-precision multi-RHS red-black solve (FP32 CGmultiRHS inner solves sharing every link load
     across the 12 sources, FP64 defect correction and patch-up) against 12 sequential FP64
@@ -133,6 +135,8 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-1
     rng = np.random.default_rng(4004)
     x0 = tuple(int(rng.integers(d)) for d in dims)
     base = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=mass)
+    if tune:
+        tune('c4_sequential', base, gx.F64, rounds=2, cg_iters=32)
     s1 = base.field()
     seq, seq_sec, seq_it = [], 0.0, []
     for k in range(nrhs):
@@ -146,6 +150,8 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-1
         seq_it.append(st['iterations'])
         seq.append(x)
     B = base.clone(ctx, nrhs, mass)
+    if tune:
+        tune('c4', B, gx.F32, dhop_iters=5, rounds=2, cg_iters=16)
     sb, xb = B.field().point_source(x0), B.field()
     ctx.sync()
     t0 = time.perf_counter()
@@ -175,7 +181,7 @@ def bench_serving(n, rate, slots=4):
     s = serving.summarize(res, info)
     s.update({'arrival_rate_per_s': rate, 'slots': slots, 'admission_budget_bytes': 128e9,
               'mix': 'volumes 8^4,12^4,16^4,16^3x32,24^4; 1-12 Gaussian sources; m in [0.05,0.3]; mixed CG to 1e-10'})
-    return {k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()}
+    return {k: (float(f'{v:.6g}') if isinstance(v, float) else v) for k, v in s.items()}
 
 
 def main():
@@ -187,9 +193,23 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-c3', action='store_true')
     ap.add_argument('--no-c4', action='store_true')
-    ap.add_argument('--serve-requests', type=int, default=96)
-    ap.add_argument('--serve-rate', type=float, default=6.0)
+    ap.add_argument('--serve-requests', type=int, default=256)
+    ap.add_argument('--serve-rate', type=float, default=20.0)
+    ap.add_argument('--serve-slots', type=int, default=4)
+    ap.add_argument('--legs', default='dhop,cg,c3,c4,serve,cpu',
+                    help='comma list of legs to run: dhop, cg, c3, c4, serve, cpu (profiling runs pick one)')
+    ap.add_argument('--no-graphs', action='store_true',
+                    help='solver iterations as plain launches (for profiler runs; same kernels and timings)')
+    ap.add_argument('--no-tune', action='store_true',
+                    help='keep the library default launch knobs instead of autotuning each workload')
     args = ap.parse_args()
+    legs = set(args.legs.split(','))
+    args.no_cg = args.no_cg or 'cg' not in legs
+    args.no_cpu = args.no_cpu or 'cpu' not in legs
+    args.no_c3 = args.no_c3 or 'c3' not in legs
+    args.no_c4 = args.no_c4 or 'c4' not in legs
+    if 'serve' not in legs:
+        args.serve_requests = 0
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU halo exchange '
               f'is not built). Run with --gpus 1.', file=sys.stderr)
@@ -197,12 +217,26 @@ def main():
 
     import grid_amd as gx
     ctx = gx.Context(0)
+    if args.no_graphs:
+        gx.tune_set(gx.TUNE_CG_GRAPH, 0)
+    from grid_amd import autotune
+    tuner = None if args.no_tune else autotune.Autotuner(ctx, cache_path=None)
+    tuned = {}
+
+    def tune(key, op, prec, **kw):
+        if tuner is not None:
+            vals = tuner.tune(op, prec, **kw)
+            tuned[key] = {autotune.KNOB_NAMES[k]: v for k, v in vals.items()}
+        if args.no_graphs:
+            gx.tune_set(gx.TUNE_CG_GRAPH, 0)
+
     U = gx.random_gauge(DIMS, 1002, EPS)
     psi_o = pick(gx.gaussian_fermion(DIMS, LS, 2002), DIMS, 1)
     D = gx.FermionOp(ctx, U, DIMS, LS, 'mobius', **PARAMS)
     H = np.prod(DIMS) // 2
     res = {}
     for prec, name, c in ((gx.F64, 'f64', 16), (gx.F32, 'f32', 8), (gx.F16, 'f16', 4)):
+        tune('dhop_' + name, D, prec)
         fi = D.field(prec, 1).upload(psi_o)
         fo = D.field(prec, 0)
         ms = time_dhop(ctx, D, fi, fo, args.steps, args.warmup)
@@ -221,6 +255,7 @@ def main():
 
     cg = None
     if not args.no_cg:
+        tune('cg', D, gx.F32)
         src_full = gx.gaussian_fermion(DIMS, LS, 3003)
         fs, fsol = D.field().upload(src_full), D.field()
         st64 = D.schur_solve(fs, fsol, tol=1e-10, maxit=20000, mixed=False)
@@ -236,9 +271,9 @@ def main():
                                  rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))),
                   speedup_mixed_vs_fp64=st64['seconds'] / stmx['seconds'])
 
-    c3 = None if args.no_c3 else bench_mixed_dwf(gx, ctx)
-    c4 = None if args.no_c4 else bench_multi_rhs(gx, ctx)
-    serve = bench_serving(args.serve_requests, args.serve_rate) if args.serve_requests > 0 else None
+    c3 = None if args.no_c3 else bench_mixed_dwf(gx, ctx, tune=tune)
+    c4 = None if args.no_c4 else bench_multi_rhs(gx, ctx, tune=tune)
+    serve = bench_serving(args.serve_requests, args.serve_rate, args.serve_slots) if args.serve_requests > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -271,6 +306,7 @@ def main():
         'gflops_1320': round(r['gflops'], 1),
         'f32': {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res['f32'].items()},
         'f16': {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res['f16'].items()},
+        'tuned_knobs': tuned,
         'cg': cg,
         'c3_mixed_dwf': c3,
         'c4_multi_rhs': c4,

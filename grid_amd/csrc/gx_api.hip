@@ -177,7 +177,9 @@ int reduce_read(Ctx *ctx, int slot, int n, double *out) {
   return GX_OK;
 }
 
-static int64_t nsites_of(const Field *f) { return f->g.half_elems() * (f->subset == GX_FULL ? 2 : 1); }
+// entries of a field as the BLAS and reduction kernels walk them: every stored entry of each half
+// (padding entries hold zeros and contribute nothing)
+static int64_t nsites_of(const Field *f) { return f->g.half_pad() * f->halves(); }
 
 static bool conformable(const Field *a, const Field *b) {
   return a && b && same_geom(a->g, b->g) && a->prec == b->prec && (a->subset == GX_FULL) == (b->subset == GX_FULL);
@@ -185,14 +187,14 @@ static bool conformable(const Field *a, const Field *b) {
 
 int blas_norm2(Field *x, double *nrm) {
   GX_TRY(by_prec(x->prec, [&](auto t) {
-    return launch_reduce<decltype(t)>(x->ctx, 1, nsites_of(x), x->g.half_elems(), x->data, nullptr, nullptr, 0, 0);
+    return launch_reduce<decltype(t)>(x->ctx, 1, nsites_of(x), x->g.half_pad(), x->data, nullptr, nullptr, 0, 0);
   }));
   return reduce_read(x->ctx, 0, 1, nrm);
 }
 
 int blas_inner(Field *x, Field *y, double *re, double *im) {
   GX_TRY(by_prec(x->prec, [&](auto t) {
-    return launch_reduce<decltype(t)>(x->ctx, 0, nsites_of(x), x->g.half_elems(), x->data, y->data, nullptr, 0, 0);
+    return launch_reduce<decltype(t)>(x->ctx, 0, nsites_of(x), x->g.half_pad(), x->data, y->data, nullptr, 0, 0);
   }));
   double v[2];
   GX_TRY(reduce_read(x->ctx, 0, 2, v));
@@ -203,7 +205,7 @@ int blas_inner(Field *x, Field *y, double *re, double *im) {
 
 int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm) {
   GX_TRY(by_prec(x->prec, [&](auto t) {
-    return launch_reduce<decltype(t)>(x->ctx, 2, nsites_of(x), x->g.half_elems(), x->data, y->data, z->data, a, 0);
+    return launch_reduce<decltype(t)>(x->ctx, 2, nsites_of(x), x->g.half_pad(), x->data, y->data, z->data, a, 0);
   }));
   z->cb = x->cb;
   return reduce_read(x->ctx, 0, 1, nrm);
@@ -212,13 +214,13 @@ int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm) {
 int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const Gate *gate) {
   z->cb = x->cb;
   return by_prec(x->prec, [&](auto t) {
-    return launch_axpby<decltype(t)>(nsites_of(x), x->g.half_elems(), a, x->data, b, y ? y->data : nullptr, z->data,
+    return launch_axpby<decltype(t)>(nsites_of(x), x->g.half_pad(), a, x->data, b, y ? y->data : nullptr, z->data,
                                      gate, x->ctx->stream);
   });
 }
 
 int convert_field(Field *out, Field *in) {
-  const int64_t n = in->elems() * 2;  // reals
+  const int64_t n = in->stored() * 2;  // reals, padding included
   out->cb = in->cb;
   if (in->prec == out->prec) {
     GX_CHECK_HIP(hipMemcpyAsync(out->data, in->data, in->bytes, hipMemcpyDeviceToDevice, in->ctx->stream));
@@ -226,7 +228,7 @@ int convert_field(Field *out, Field *in) {
   }
   if (in->prec == GX_F64 && out->prec == GX_F32) return launch_convert<double, float>(n, in->data, out->data, in->ctx->stream);
   if (in->prec == GX_F32 && out->prec == GX_F64) return launch_convert<float, double>(n, in->data, out->data, in->ctx->stream);
-  const int64_t ns = nsites_of(in), st = in->g.half_elems();
+  const int64_t ns = nsites_of(in), st = in->g.half_pad();
   hipStream_t s = in->ctx->stream;
   if (in->prec == GX_F64 && out->prec == GX_F16) return launch_convert_site<double, half_t>(ns, st, in->data, out->data, s);
   if (in->prec == GX_F16 && out->prec == GX_F64) return launch_convert_site<half_t, double>(ns, st, in->data, out->data, s);
@@ -651,12 +653,13 @@ int apply_mpcdagmpc(Op *op, Field *in, Field *out, const Gate *gate) {
   return apply_mpc(op, t, out, 1, gate);
 }
 
-// host (lexicographic, s innermost, AoS) <-> device (EO split, SoA) index maps
+// host (lexicographic, s innermost, AoS) <-> device (EO split, 64-entry blocks) index maps; the
+// stage holds 12 columns per half (half_pad entries each, padding zero)
 template <typename T>
 static void gather_to_device_layout(const Field *f, const double *host, T *stage) {
   const Geom &g = f->g;
   const int Ls = g.Ls;
-  const int64_t st = g.half_elems();
+  const int64_t hp = 12 * g.half_pad();
   if (f->subset == GX_FULL) {
     parallel_for(g.V4, [&](int64_t b, int64_t e) {
       for (int64_t s4 = b; s4 < e; s4++) {
@@ -667,7 +670,7 @@ static void gather_to_device_layout(const Field *f, const double *host, T *stage
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             const double *src = host + ((s4 * Ls + s) * 12 + k) * 2;
-            T *dst = stage + (((int64_t)cb * 12 + k) * st + h * Ls + s) * 2;
+            T *dst = stage + (cb * hp + blk_index(h * Ls + s, k, 12)) * 2;
             dst[0] = (T)src[0];
             dst[1] = (T)src[1];
           }
@@ -679,7 +682,7 @@ static void gather_to_device_layout(const Field *f, const double *host, T *stage
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             const double *src = host + ((h * Ls + s) * 12 + k) * 2;
-            T *dst = stage + ((int64_t)k * st + h * Ls + s) * 2;
+            T *dst = stage + blk_index(h * Ls + s, k, 12) * 2;
             dst[0] = (T)src[0];
             dst[1] = (T)src[1];
           }
@@ -691,7 +694,7 @@ template <typename T>
 static void scatter_from_device_layout(const Field *f, const T *stage, double *host) {
   const Geom &g = f->g;
   const int Ls = g.Ls;
-  const int64_t st = g.half_elems();
+  const int64_t hp = 12 * g.half_pad();
   if (f->subset == GX_FULL) {
     parallel_for(g.V4, [&](int64_t b, int64_t e) {
       for (int64_t s4 = b; s4 < e; s4++) {
@@ -702,7 +705,7 @@ static void scatter_from_device_layout(const Field *f, const T *stage, double *h
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             double *dst = host + ((s4 * Ls + s) * 12 + k) * 2;
-            const T *src = stage + (((int64_t)cb * 12 + k) * st + h * Ls + s) * 2;
+            const T *src = stage + (cb * hp + blk_index(h * Ls + s, k, 12)) * 2;
             dst[0] = (double)src[0];
             dst[1] = (double)src[1];
           }
@@ -714,7 +717,7 @@ static void scatter_from_device_layout(const Field *f, const T *stage, double *h
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             double *dst = host + ((h * Ls + s) * 12 + k) * 2;
-            const T *src = stage + ((int64_t)k * st + h * Ls + s) * 2;
+            const T *src = stage + blk_index(h * Ls + s, k, 12) * 2;
             dst[0] = (double)src[0];
             dst[1] = (double)src[1];
           }
@@ -761,7 +764,8 @@ const char *gx_version(void) { return "gx 0.1 (gfx950)"; }
 
 int gx_tune_set(int knob, int value) {
   GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT, GX_EINVAL, "unknown tuning knob %d", knob);
-  if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
+  if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 3, GX_EINVAL, "tile map takes 0..3");
+  else if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   g_tune[knob] = value;
   g_tune_version++;
   return GX_OK;
@@ -845,37 +849,41 @@ int gx_field_destroy(gx_field f) {
 // FP16 storage <-> FP32 device layout on the host: per half block, 12 columns of binary16 pairs
 // scaled by each site's largest |component| (column 12, float).
 static void pack_f16(const Field *f, const float *in, uint32_t *out) {
-  const int64_t st = f->g.half_elems(), nh = f->subset == GX_FULL ? 2 : 1;
+  const int64_t P = f->g.half_pad(), nh = f->halves();
   for (int64_t hf = 0; hf < nh; hf++)
-    parallel_for(st, [&](int64_t b, int64_t e) {
+    parallel_for(P, [&](int64_t b, int64_t e) {
+      const float *ih = in + hf * 12 * P * 2;
+      uint32_t *oh = out + hf * 13 * P;
       for (int64_t i = b; i < e; i++) {
         float m = 0.f;
         for (int k = 0; k < 12; k++) {
-          const float *v = in + ((hf * 12 + k) * st + i) * 2;
+          const float *v = ih + blk_index(i, k, 12) * 2;
           m = std::max(m, std::max(std::fabs(v[0]), std::fabs(v[1])));
         }
         const float inv = m > 0.f ? 1.f / m : 0.f;
         for (int k = 0; k < 12; k++) {
-          const float *v = in + ((hf * 12 + k) * st + i) * 2;
+          const float *v = ih + blk_index(i, k, 12) * 2;
           const half_t pr[2] = {(half_t)(v[0] * inv), (half_t)(v[1] * inv)};
-          std::memcpy(&out[(hf * 13 + k) * st + i], pr, 4);
+          std::memcpy(&oh[blk_index(i, k, 13)], pr, 4);
         }
-        std::memcpy(&out[(hf * 13 + 12) * st + i], &m, 4);
+        std::memcpy(&oh[blk_index(i, 12, 13)], &m, 4);
       }
     });
 }
 
 static void unpack_f16(const Field *f, const uint32_t *in, float *out) {
-  const int64_t st = f->g.half_elems(), nh = f->subset == GX_FULL ? 2 : 1;
+  const int64_t P = f->g.half_pad(), nh = f->halves();
   for (int64_t hf = 0; hf < nh; hf++)
-    parallel_for(st, [&](int64_t b, int64_t e) {
+    parallel_for(P, [&](int64_t b, int64_t e) {
+      const uint32_t *ih = in + hf * 13 * P;
+      float *oh = out + hf * 12 * P * 2;
       for (int64_t i = b; i < e; i++) {
         float m;
-        std::memcpy(&m, &in[(hf * 13 + 12) * st + i], 4);
+        std::memcpy(&m, &ih[blk_index(i, 12, 13)], 4);
         for (int k = 0; k < 12; k++) {
           half_t pr[2];
-          std::memcpy(pr, &in[(hf * 13 + k) * st + i], 4);
-          float *v = out + ((hf * 12 + k) * st + i) * 2;
+          std::memcpy(pr, &ih[blk_index(i, k, 13)], 4);
+          float *v = oh + blk_index(i, k, 12) * 2;
           v[0] = (float)pr[0] * m;
           v[1] = (float)pr[1] * m;
         }
@@ -886,19 +894,19 @@ static void unpack_f16(const Field *f, const uint32_t *in, float *out) {
 int gx_field_import(gx_field f, const double *host) {
   GX_REQUIRE(f && host, GX_EINVAL, "null argument");
   if (f->prec == GX_F16) {
-    std::vector<float> stage((size_t)f->elems() * 2);
+    std::vector<float> stage((size_t)f->stored() * 2, 0.f);
     gather_to_device_layout<float>(f, host, stage.data());
     std::vector<uint32_t> packed(f->bytes / 4);
     pack_f16(f, stage.data(), packed.data());
     GX_CHECK_HIP(hipMemcpyAsync(f->data, packed.data(), f->bytes, hipMemcpyHostToDevice, f->ctx->stream));
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
   } else if (f->prec == GX_F64) {
-    std::vector<double> stage((size_t)f->elems() * 2);
+    std::vector<double> stage((size_t)f->stored() * 2, 0.0);
     gather_to_device_layout<double>(f, host, stage.data());
     GX_CHECK_HIP(hipMemcpyAsync(f->data, stage.data(), f->bytes, hipMemcpyHostToDevice, f->ctx->stream));
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
   } else {
-    std::vector<float> stage((size_t)f->elems() * 2);
+    std::vector<float> stage((size_t)f->stored() * 2, 0.f);
     gather_to_device_layout<float>(f, host, stage.data());
     GX_CHECK_HIP(hipMemcpyAsync(f->data, stage.data(), f->bytes, hipMemcpyHostToDevice, f->ctx->stream));
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
@@ -936,7 +944,7 @@ int gx_field_point_source(gx_field f, const int x[4], int spin, int colour) {
   const void *one = f->prec == GX_F64 ? (const void *)one64 : (const void *)one32;
   for (int s = 0; s < f->g.Ls; s++) {
     const int k = batch ? s : spin * 3 + colour;
-    const int64_t e = ((int64_t)cb * 12 + k) * f->g.half_elems() + h * f->g.Ls + s;
+    const int64_t e = (int64_t)cb * 12 * f->g.half_pad() + blk_index(h * f->g.Ls + s, k, 12);
     GX_CHECK_HIP(hipMemcpyAsync((char *)f->data + e * f->csize(), one, f->csize(), hipMemcpyHostToDevice, st));
   }
   GX_CHECK_HIP(hipStreamSynchronize(st));
@@ -976,16 +984,16 @@ int gx_field_export(gx_field f, double *host) {
     std::vector<uint32_t> packed(f->bytes / 4);
     GX_CHECK_HIP(hipMemcpyAsync(packed.data(), f->data, f->bytes, hipMemcpyDeviceToHost, f->ctx->stream));
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
-    std::vector<float> stage((size_t)f->elems() * 2);
+    std::vector<float> stage((size_t)f->stored() * 2);
     unpack_f16(f, packed.data(), stage.data());
     scatter_from_device_layout<float>(f, stage.data(), host);
   } else if (f->prec == GX_F64) {
-    std::vector<double> stage((size_t)f->elems() * 2);
+    std::vector<double> stage((size_t)f->stored() * 2);
     GX_CHECK_HIP(hipMemcpyAsync(stage.data(), f->data, f->bytes, hipMemcpyDeviceToHost, f->ctx->stream));
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
     scatter_from_device_layout<double>(f, stage.data(), host);
   } else {
-    std::vector<float> stage((size_t)f->elems() * 2);
+    std::vector<float> stage((size_t)f->stored() * 2);
     GX_CHECK_HIP(hipMemcpyAsync(stage.data(), f->data, f->bytes, hipMemcpyDeviceToHost, f->ctx->stream));
     GX_CHECK_HIP(hipStreamSynchronize(f->ctx->stream));
     scatter_from_device_layout<float>(f, stage.data(), host);

@@ -2,11 +2,12 @@
 //
 // Device layout (HBM), chosen for coalesced wave64 access on gfx950:
 //   * every field is stored per checkerboard; a full field is [even half | odd half];
-//   * a half field is structure-of-arrays over the 12 spin-colour components,
-//       data[k][h*Ls + s],  k = spin*3 + colour,  h = half-lattice 4D site (Grid's red-black
-//       order, x halved), s = fifth-dimension / right-hand-side index (innermost),
-//     so a wave whose lanes walk (s, h) with s fastest reads 64 consecutive complex numbers
-//     per component; FP16 fields add a 13th column data[12][h*Ls + s] of per-site float scales;
+//   * a half field holds entries e = h*Ls + s (h = half-lattice 4D site in Grid's red-black order,
+//     x halved; s = fifth-dimension / right-hand-side index, innermost) of the 12 spin-colour
+//     components k = spin*3 + colour, in blocks of 64 entries: data[e/64][k][e%64].  A wave whose
+//     lanes walk (s, h) with s fastest reads 64 consecutive complex numbers per component, and the
+//     12 components of those entries are one contiguous run; FP16 fields add a 13th column of
+//     per-site float scales;
 //   * doubled gauge links per output checkerboard, SoA over sites:
 //       U_cb[p][k][h],  p = 0..7 (x+,y+,z+,t+,x-,y-,z-,t-), k = a*3+b;
 //     U_cb[p<4] = -1/2 U_mu(x) (boundary phase), U_cb[p>=4] = -1/2 U_mu(x-mu)^dag.
@@ -58,11 +59,18 @@ constexpr int kMaxLs = 16;         // fifth-dimension extent / right-hand sides 
 constexpr int kRedBlocks = 1024;   // fixed grid of the deterministic reductions
 constexpr int kRedSlots = 64;      // device result slots per context
 
+// Fields are stored in blocks of kBlk consecutive entries (h, s) of one checkerboard: entry e =
+// h*Ls + s of column k lives at ((e / kBlk) * ncol + k) * kBlk + e % kBlk, so the 12 components of a
+// wave's 64 entries form one contiguous run instead of 12 runs a whole column apart.
+constexpr int kBlk = 64;
+inline int64_t blk_index(int64_t e, int k, int ncol) { return ((e / kBlk) * ncol + k) * kBlk + e % kBlk; }
+
 struct Geom {
   int L[4];
   int Ls;
   int64_t V4, H4;
-  int64_t half_elems() const { return H4 * Ls; }   // complex numbers per component per checkerboard
+  int64_t half_elems() const { return H4 * Ls; }   // entries (h, s) per checkerboard
+  int64_t half_pad() const { return (H4 * Ls + kBlk - 1) / kBlk * kBlk; }  // entries incl. block padding
 };
 
 // Iteration gate shared by the device-resident solvers: a kernel of iteration k runs only while
@@ -110,15 +118,18 @@ struct Field {
   int prec = GX_F64;
   int subset = GX_FULL;   // GX_FULL or a half field
   int cb = 0;             // checkerboard tag of a half field
-  // half: ncol() columns of H4*Ls entries (component k = column k); full: 2 halves (even, odd).
-  // GX_F16 stores a complex number as two binary16 values and adds a 13th column: one float per
-  // (h, s) site, the site's largest |component|, by which its stored values are multiplied.
+  // half: ncol() columns (component k = column k) over g.half_pad() entries in the block layout
+  // (blk_index); full: 2 halves (even, odd).  GX_F16 stores a complex number as two binary16 values
+  // and adds a 13th column: one float per (h, s) site, the site's largest |component|, by which
+  // its stored values are multiplied.  Padding entries hold zeros.
   void *data = nullptr;
   size_t bytes = 0;
-  int64_t elems() const { return 12 * g.half_elems() * (subset == GX_FULL ? 2 : 1); }
+  int halves() const { return subset == GX_FULL ? 2 : 1; }
+  int64_t elems() const { return 12 * g.half_elems() * halves(); }           // logical complex numbers
+  int64_t stored() const { return 12 * g.half_pad() * halves(); }            // stored, padding included
   size_t csize() const { return prec == GX_F64 ? 16 : (prec == GX_F32 ? 8 : 4); }
   int ncol() const { return prec == GX_F16 ? 13 : 12; }
-  size_t half_bytes() const { return (size_t)ncol() * g.half_elems() * csize(); }
+  size_t half_bytes() const { return (size_t)ncol() * g.half_pad() * csize(); }
   // pointer to the half with checkerboard cb (full fields) or the data (half fields)
   void *half(int which) const {
     if (subset != GX_FULL) return data;

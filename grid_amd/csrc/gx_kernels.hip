@@ -51,54 +51,58 @@ struct Sto<half_t> {
   static constexpr int NCOL = 13;
 };
 
+// position of column k of entry e in a half block of storage S (block layout, see Field)
+template <typename S, typename I>
+__device__ __forceinline__ I bidx(I e, int k) {
+  return ((e >> 6) * (I)Sto<S>::NCOL + (I)k) * (I)kBlk + (e & (I)(kBlk - 1));
+}
+
 template <typename S>
 __device__ __forceinline__ cx<typename Sto<S>::C> ldc(const cx<S> *__restrict__ p, int64_t i) {
   const cx<S> v = p[i];
   return {(typename Sto<S>::C)v.r, (typename Sto<S>::C)v.i};
 }
 
-// scale of site i of the half block whose column 0 starts at p (FP16 storage; 1 otherwise)
-template <typename S>
-__device__ __forceinline__ typename Sto<S>::C site_scale(const cx<S> *__restrict__ p, int64_t i, int64_t stride) {
-  if constexpr (Sto<S>::H) return reinterpret_cast<const float *>(p + 12 * stride)[i];
+// scale of entry i of the half block at p (FP16 storage; 1 otherwise)
+template <typename S, typename I>
+__device__ __forceinline__ typename Sto<S>::C site_scale(const cx<S> *__restrict__ p, I i) {
+  if constexpr (Sto<S>::H) return reinterpret_cast<const float *>(p)[bidx<S, I>(i, 12)];
   else return (typename Sto<S>::C)1;
 }
 
-template <typename S>
-__device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ p, int64_t i,
-                                          int64_t stride) {
+template <typename S, typename I>
+__device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ p, I i) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) f[k] = ldc<S>(p, k * stride + i);
+  for (int k = 0; k < 12; k++) f[k] = ldc<S>(p, bidx<S, I>(i, k));
   if constexpr (Sto<S>::H) {
-    const float sc = site_scale<S>(p, i, stride);
+    const float sc = site_scale<S, I>(p, i);
 #pragma unroll
     for (int k = 0; k < 12; k++) f[k] = {f[k].r * sc, f[k].i * sc};
   }
 }
 
-// store the 12 components of site i (FP16: with the site's scale); nt = non-temporal stores
-template <typename S>
-__device__ __forceinline__ void store_site(cx<S> *__restrict__ p, const cx<typename Sto<S>::C> *a, int64_t i,
-                                           int64_t stride, bool nt) {
+// store the 12 components of entry i (FP16: with the site's scale); nt = non-temporal stores
+template <typename S, typename I>
+__device__ __forceinline__ void store_site(cx<S> *__restrict__ p, const cx<typename Sto<S>::C> *a, I i, bool nt) {
   if constexpr (Sto<S>::H) {
     float m = 0.f;
 #pragma unroll
     for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(a[k].r), fabsf(a[k].i)));
     const float inv = m > 0.f ? 1.f / m : 0.f;
 #pragma unroll
-    for (int k = 0; k < 12; k++) p[k * stride + i] = {(half_t)(a[k].r * inv), (half_t)(a[k].i * inv)};
-    reinterpret_cast<float *>(p + 12 * stride)[i] = m;
+    for (int k = 0; k < 12; k++) p[bidx<S, I>(i, k)] = {(half_t)(a[k].r * inv), (half_t)(a[k].i * inv)};
+    reinterpret_cast<float *>(p)[bidx<S, I>(i, 12)] = m;
   } else {
     typedef S v2 __attribute__((ext_vector_type(2)));
     if (nt) {
 #pragma unroll
       for (int k = 0; k < 12; k++) {
         v2 v = {a[k].r, a[k].i};
-        __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(p + k * stride + i));
+        __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(p + bidx<S, I>(i, k)));
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) p[k * stride + i] = a[k];
+      for (int k = 0; k < 12; k++) p[bidx<S, I>(i, k)] = a[k];
     }
   }
 }
@@ -107,7 +111,7 @@ struct KGeom {
   int L[4];
   int Lxh;
   int64_t H4;
-  int64_t stride;          // complex per component per checkerboard = H4*Ls
+  int64_t stride;          // entries per checkerboard including block padding (Geom::half_pad)
   int64_t sY, sZ, sT;      // half-site strides of y, z, t
 };
 
@@ -116,7 +120,7 @@ static KGeom kgeom(const Geom &g) {
   for (int d = 0; d < 4; d++) k.L[d] = g.L[d];
   k.Lxh = g.L[0] / 2;
   k.H4 = g.H4;
-  k.stride = g.H4 * g.Ls;
+  k.stride = g.half_pad();
   k.sY = k.Lxh;
   k.sZ = k.sY * g.L[1];
   k.sT = k.sZ * g.L[2];
@@ -199,13 +203,21 @@ __device__ __forceinline__ void su3_mul(const cx<T> *u, const cx<T> *h, cx<T> *g
 // memory (SoA over sites, coalesced across lanes when one lane = one site).
 constexpr int kLinkPad = 73;
 
-// Workgroup tile -> lattice mapping.  map = 0: block b covers tile b.  map = 1 (XCD-aware): the
-// hardware deals consecutive workgroups round-robin to the 8 XCDs, so tile (b % 8) * per + b / 8
-// gives each XCD (and its L2) one contiguous slab of the lattice.
+// Workgroup tile -> lattice mapping.  The hardware deals consecutive workgroups round-robin to the
+// 8 XCDs.  map = 0: block b covers tile b (the XCDs interleave tile by tile).  map = 1: tile
+// (b % 8) * per + b / 8 gives each XCD (and its L2) one contiguous slab of the lattice.  map = 2, 3:
+// within every group of 8*R consecutive tiles (R = 8, 32) each XCD takes a run of R tiles, so the
+// XCDs sweep the lattice together (shared Infinity Cache window) while each keeps runs in its L2.
+// The grid is padded to a multiple of 8 (map 1) or 8*R (map 2, 3) blocks; tiles past the end exit.
 __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nblocks, int map) {
   if (map == 1) {
     const uint32_t per = (nblocks + 7) >> 3;
     return (b & 7) * per + (b >> 3);
+  }
+  if (map == 2 || map == 3) {
+    const uint32_t R = map == 2 ? 8 : 32;
+    const uint32_t j = b % (8 * R);
+    return (b - j) + (j & 7) * R + (j >> 3);
   }
   return b;
 }
@@ -256,46 +268,46 @@ __device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, int
 template <typename T, typename S, int LS, bool DAG, bool PF, int ORD, bool LDSU>
 __device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const cx<S> *__restrict__ in,
                                          const cx<S> *__restrict__ U, const cx<T> *ul, const uint32_t *nb, int s,
-                                         uint32_t h, uint32_t H4, uint32_t stride) {
+                                         uint32_t h, uint32_t H4) {
 #define IDX(L) leg_idx<DAG, ORD, L>(nb, LS, s)
 #define LEG(L, F) do_leg<T, S, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4)
   if (PF) {
     __builtin_amdgcn_sched_barrier(0);
     LEG(0, fa);
-    load_site<S>(fa, in, IDX(2), stride);
+    load_site<S>(fa, in, IDX(2));
     __builtin_amdgcn_sched_barrier(0);
     LEG(1, fb);
-    load_site<S>(fb, in, IDX(3), stride);
+    load_site<S>(fb, in, IDX(3));
     __builtin_amdgcn_sched_barrier(0);
     LEG(2, fa);
-    load_site<S>(fa, in, IDX(4), stride);
+    load_site<S>(fa, in, IDX(4));
     __builtin_amdgcn_sched_barrier(0);
     LEG(3, fb);
-    load_site<S>(fb, in, IDX(5), stride);
+    load_site<S>(fb, in, IDX(5));
     __builtin_amdgcn_sched_barrier(0);
     LEG(4, fa);
-    load_site<S>(fa, in, IDX(6), stride);
+    load_site<S>(fa, in, IDX(6));
     __builtin_amdgcn_sched_barrier(0);
     LEG(5, fb);
-    load_site<S>(fb, in, IDX(7), stride);
+    load_site<S>(fb, in, IDX(7));
     __builtin_amdgcn_sched_barrier(0);
     LEG(6, fa);
     LEG(7, fb);
   } else {
     LEG(0, fa);
-    load_site<S>(fa, in, IDX(1), stride);
+    load_site<S>(fa, in, IDX(1));
     LEG(1, fa);
-    load_site<S>(fa, in, IDX(2), stride);
+    load_site<S>(fa, in, IDX(2));
     LEG(2, fa);
-    load_site<S>(fa, in, IDX(3), stride);
+    load_site<S>(fa, in, IDX(3));
     LEG(3, fa);
-    load_site<S>(fa, in, IDX(4), stride);
+    load_site<S>(fa, in, IDX(4));
     LEG(4, fa);
-    load_site<S>(fa, in, IDX(5), stride);
+    load_site<S>(fa, in, IDX(5));
     LEG(5, fa);
-    load_site<S>(fa, in, IDX(6), stride);
+    load_site<S>(fa, in, IDX(6));
     LEG(6, fa);
-    load_site<S>(fa, in, IDX(7), stride);
+    load_site<S>(fa, in, IDX(7));
     LEG(7, fa);
   }
 #undef IDX
@@ -333,7 +345,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   __shared__ cx<T> lds[NLINK > NSTAGE ? NLINK : NSTAGE];  // links, then (MAT) the hop results
   const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, map);
   if (tile >= ntiles) return;  // uniform across the workgroup
-  const uint32_t H4 = (uint32_t)g.H4, stride = (uint32_t)g.stride;
+  const uint32_t H4 = (uint32_t)g.H4;
   const uint32_t h0 = tile * SPB;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -378,8 +390,8 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     }
   }
   cx<T> fa[12], fb[12];
-  load_site<S>(fa, in, leg_idx<DAG, ORD, 0>(nb, LS, s), stride);
-  if (PF) load_site<S>(fb, in, leg_idx<DAG, ORD, 1>(nb, LS, s), stride);
+  load_site<S>(fa, in, leg_idx<DAG, ORD, 0>(nb, LS, s));
+  if (PF) load_site<S>(fb, in, leg_idx<DAG, ORD, 1>(nb, LS, s));
   if (LDSU) {
 #pragma unroll
     for (int j = 0; j < NU; j++) {
@@ -394,7 +406,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  if (valid) hop_site<T, S, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4, stride);
+  if (valid) hop_site<T, S, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4);
   const uint32_t o = h * LS + s;
   if (MAT) {
     // every wave has finished reading links: reuse the LDS for the hop results [w][k][lane]
@@ -420,19 +432,19 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     }
   }
   if (EPI != 0) {
-    const T bsc = PHI ? ep.beta * site_scale<S>(ep.phi, o, stride) : T(0);
+    const T bsc = PHI ? ep.beta * site_scale<S>(ep.phi, o) : T(0);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
       cx<T> v = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
       if (PHI) {
-        const cx<T> p = ldc<S>(ep.phi, k * stride + o);
+        const cx<T> p = ldc<S>(ep.phi, bidx<S>(o, k));
         v.r = fma(bsc, p.r, v.r);
         v.i = fma(bsc, p.i, v.i);
       }
       acc[k] = v;
     }
   }
-  store_site<S>(out, acc, o, stride, (flags & 1) != 0);
+  store_site<S>(out, acc, o, (flags & 1) != 0);
 }
 
 #ifndef GX_LS_CASES
@@ -470,7 +482,8 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta};
   const int spb = 4 * (64 / g.Ls);
   const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
-  const uint32_t blocks = map == 1 ? ((ntiles + 7) / 8) * 8 : ntiles;
+  const uint32_t pad = map == 1 ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
+  const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
 #define GX_DHOP_GO(LS, DAG, PF, ORD, EPI)                                                                         \
   hipLaunchKernelGGL((k_dhop<S, LS, DAG, PF, ORD, EPI>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
                      gate, ntiles, map, flags, ep)
@@ -545,7 +558,7 @@ __global__ void __launch_bounds__(256) k_smat(const cx<S> *__restrict__ in, cons
   {
     cx<T> f[12];
     if (valid) {
-      load_site<S>(f, in, o, stride);
+      load_site<S>(f, in, o);
     } else {
 #pragma unroll
       for (int k = 0; k < 12; k++) f[k] = {0, 0};
@@ -583,16 +596,16 @@ __global__ void __launch_bounds__(256) k_smat(const cx<S> *__restrict__ in, cons
     }
   }
   if (PHI) {
-    const T bsc = beta * site_scale<S>(phi, o, stride);
+    const T bsc = beta * site_scale<S>(phi, o);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      const cx<T> p = ldc<S>(phi, k * stride + o);
+      const cx<T> p = ldc<S>(phi, bidx<S>(o, k));
       acc[k].r = fma(bsc, p.r, acc[k].r);
       acc[k].i = fma(bsc, p.i, acc[k].i);
     }
   }
-  store_site<S>(out, acc, o, stride, false);
-  if (OUT2) store_site<S>(out2, acc2, o, stride, false);
+  store_site<S>(out, acc, o, false);
+  if (OUT2) store_site<S>(out2, acc2, o, false);
 }
 
 template <typename S>
@@ -602,7 +615,7 @@ int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, vo
   const int spw = 64 / g.Ls;
   const int64_t waves = (g.H4 + spw - 1) / spw;
   const int64_t blocks = (waves * 64 + 255) / 256;
-  const int64_t stride = g.H4 * g.Ls;
+  const int64_t stride = g.half_pad();
   const cx<S> *pin = (const cx<S> *)in;
   const cx<S> *pphi = (const cx<S> *)phi;
   cx<S> *pout = (cx<S> *)out, *pout2 = (cx<S> *)out2;
@@ -669,16 +682,16 @@ __global__ void __launch_bounds__(256) k_axpby(int64_t nsites, int64_t stride, t
          j += (int64_t)gridDim.x * blockDim.x) {
       const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
       cx<T> xv[12], yv[12];
-      load_site<S>(xv, x + base, i, stride);
+      load_site<S>(xv, x + base, i);
       if (y) {
-        load_site<S>(yv, y + base, i, stride);
+        load_site<S>(yv, y + base, i);
 #pragma unroll
         for (int k = 0; k < 12; k++) xv[k] = {fma(a, xv[k].r, b * yv[k].r), fma(a, xv[k].i, b * yv[k].i)};
       } else {
 #pragma unroll
         for (int k = 0; k < 12; k++) xv[k] = {a * xv[k].r, a * xv[k].i};
       }
-      store_site<S>(z + base, xv, i, stride, false);
+      store_site<S>(z + base, xv, i, false);
     }
   }
 }
@@ -717,10 +730,10 @@ __global__ void __launch_bounds__(256) k_convert_site(int64_t nsites, int64_t st
     const int64_t hf = j / stride, i = j - hf * stride;
     cx<typename Sto<Sin>::C> a[12];
     cx<typename Sto<Sout>::C> b[12];
-    load_site<Sin>(a, in + hf * Sto<Sin>::NCOL * stride, i, stride);
+    load_site<Sin>(a, in + hf * Sto<Sin>::NCOL * stride, i);
 #pragma unroll
     for (int k = 0; k < 12; k++) b[k] = {(typename Sto<Sout>::C)a[k].r, (typename Sto<Sout>::C)a[k].i};
-    store_site<Sout>(out + hf * Sto<Sout>::NCOL * stride, b, i, stride, false);
+    store_site<Sout>(out + hf * Sto<Sout>::NCOL * stride, b, i, false);
   }
 }
 
@@ -776,8 +789,8 @@ __device__ __forceinline__ void sum_partials(double (&v)[NV], const double *part
   block_sum<NV>(v, sh);
 }
 
-// site j of a (possibly full) field: half = j / stride, i = j % stride;
-// component k at half*12*stride + k*stride + i.
+// entry j of a (possibly full) field: half hf = j / stride (stride = padded entries per half),
+// entry i = j % stride of that half, whose block starts hf * NCOL * stride entries in.
 template <typename S, int MODE>
 __global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, const cx<S> *__restrict__ x,
                                                 const cx<S> *__restrict__ y, cx<S> *__restrict__ z,
@@ -788,8 +801,8 @@ __global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, 
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
     cx<T> xv[12], yv[12];
-    load_site<S>(xv, x + base, i, stride);
-    if (MODE != 1) load_site<S>(yv, y + base, i, stride);
+    load_site<S>(xv, x + base, i);
+    if (MODE != 1) load_site<S>(yv, y + base, i);
     T sr = 0, si = 0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
@@ -807,7 +820,7 @@ __global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, 
         sr = fma(yv[k].i, yv[k].i, sr);
       }
     }
-    if (MODE == 2) store_site<S>(z + base, yv, i, stride, false);
+    if (MODE == 2) store_site<S>(z + base, yv, i, false);
     v[0] += (double)sr;
     v[1] += (double)si;
   }
@@ -870,8 +883,8 @@ __global__ void __launch_bounds__(256) k_cg_dot(int64_t nsites, int64_t stride, 
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
     cx<T> xv[12], yv[12];
-    load_site<S>(xv, p + base, i, stride);
-    load_site<S>(yv, mmp + base, i, stride);
+    load_site<S>(xv, p + base, i);
+    load_site<S>(yv, mmp + base, i);
     T sr = 0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
@@ -897,8 +910,8 @@ __global__ void __launch_bounds__(256) k_cg_axpy_norm(int64_t nsites, int64_t st
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
     cx<T> xv[12], yv[12];
-    load_site<S>(xv, mmp + base, i, stride);
-    load_site<S>(yv, r + base, i, stride);
+    load_site<S>(xv, mmp + base, i);
+    load_site<S>(yv, r + base, i);
     T sr = 0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
@@ -906,7 +919,7 @@ __global__ void __launch_bounds__(256) k_cg_axpy_norm(int64_t nsites, int64_t st
       sr = fma(yv[k].r, yv[k].r, sr);
       sr = fma(yv[k].i, yv[k].i, sr);
     }
-    store_site<S>(r + base, yv, i, stride, false);
+    store_site<S>(r + base, yv, i, false);
     v[0] += (double)sr;
   }
   block_sum<1>(v, sh);
@@ -970,16 +983,16 @@ __global__ void __launch_bounds__(256) k_cg_update(int64_t nsites, int64_t strid
          j += (int64_t)gridDim.x * blockDim.x) {
       const int64_t hf = j / stride, i = j - hf * stride, base = hf * Sto<S>::NCOL * stride;
       cx<T> pv[12], xv[12], rv[12];
-      load_site<S>(pv, p + base, i, stride);
-      load_site<S>(xv, psi + base, i, stride);
-      load_site<S>(rv, r + base, i, stride);
+      load_site<S>(pv, p + base, i);
+      load_site<S>(xv, psi + base, i);
+      load_site<S>(rv, r + base, i);
 #pragma unroll
       for (int k = 0; k < 12; k++) {
         xv[k] = {fma(a, pv[k].r, xv[k].r), fma(a, pv[k].i, xv[k].i)};
         rv[k] = {fma(b, pv[k].r, rv[k].r), fma(b, pv[k].i, rv[k].i)};
       }
-      store_site<S>(psi + base, xv, i, stride, false);
-      if (!keep_p) store_site<S>(p + base, rv, i, stride, false);
+      store_site<S>(psi + base, xv, i, false);
+      if (!keep_p) store_site<S>(p + base, rv, i, false);
     }
   }
 }
@@ -1012,17 +1025,17 @@ int launch_cg_update(Ctx *ctx, int64_t nsites, int64_t stride, void *psi, void *
 }
 
 // ---------------- multi-RHS slice reductions (BlockConjugateGradient.h:412-545, CGmultiRHS) ----------
-// A half field is rows x Ls complex numbers with s innermost (row = k*H4 + h).  Each block uses
-// floor(256/Ls)*Ls threads: thread t works on slice s = t % Ls and strides over rows, so the
-// per-slice partial sums come out of a fixed-order LDS reduction; a one-block kernel adds the
-// block partials in block order.
+// Right-hand side s of a half field is the entries e = h*Ls + s.  Each block uses floor(256/Ls)*Ls
+// threads: thread t works on slice s = t % Ls and strides over the sites h, summing the 12
+// components, so the per-slice partial sums come out of a fixed-order LDS reduction; a one-block
+// kernel adds the block partials in block order.
 __device__ __forceinline__ bool mgated(const MrhsState *st) { return gated(st); }
 __device__ __forceinline__ bool mgated_prev(const MrhsState *st) {
   return st->k - 1 > st->k_conv || st->k - 1 > st->kmax;
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t rows, int Ls, cx<T> *__restrict__ x,
+__global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<T> *__restrict__ x,
                                                      const cx<T> *__restrict__ y, cx<T> *__restrict__ z,
                                                      const MrhsState *st, double *partial) {
   if ((MODE == 0 || MODE == 2) && mgated(st)) return;
@@ -1033,24 +1046,28 @@ __global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t rows, int Ls, cx<T>
   double acc = 0;
   if (rt < rpb) {
     const T a = (MODE == 2) ? (T)(-st->a[s]) : (T)0;
-    for (int64_t row = (int64_t)blockIdx.x * rpb + rt; row < rows; row += (int64_t)gridDim.x * rpb) {
-      const int64_t i = row * Ls + s;
-      const cx<T> xv = x[i];
-      if (MODE == 0) {
-        const cx<T> yv = y[i];
-        acc += (double)fma(xv.r, yv.r, xv.i * yv.i);
-      } else if (MODE == 1) {
-        acc += (double)fma(xv.r, xv.r, xv.i * xv.i);
-      } else if (MODE == 2) {
-        const cx<T> yv = y[i];
-        const cx<T> v = {fma(a, yv.r, xv.r), fma(a, yv.i, xv.i)};
-        x[i] = v;
-        acc += (double)fma(v.r, v.r, v.i * v.i);
-      } else {
-        const cx<T> yv = y[i];
-        const cx<T> v = {yv.r - xv.r, yv.i - xv.i};
-        z[i] = v;
-        acc += (double)fma(v.r, v.r, v.i * v.i);
+    for (int64_t h = (int64_t)blockIdx.x * rpb + rt; h < H4; h += (int64_t)gridDim.x * rpb) {
+      const int64_t e = h * Ls + s;
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        const int64_t i = bidx<T>(e, k);
+        const cx<T> xv = x[i];
+        if (MODE == 0) {
+          const cx<T> yv = y[i];
+          acc += (double)fma(xv.r, yv.r, xv.i * yv.i);
+        } else if (MODE == 1) {
+          acc += (double)fma(xv.r, xv.r, xv.i * xv.i);
+        } else if (MODE == 2) {
+          const cx<T> yv = y[i];
+          const cx<T> v = {fma(a, yv.r, xv.r), fma(a, yv.i, xv.i)};
+          x[i] = v;
+          acc += (double)fma(v.r, v.r, v.i * v.i);
+        } else {
+          const cx<T> yv = y[i];
+          const cx<T> v = {yv.r - xv.r, yv.i - xv.i};
+          z[i] = v;
+          acc += (double)fma(v.r, v.r, v.i * v.i);
+        }
       }
     }
   }
@@ -1102,12 +1119,14 @@ __global__ void __launch_bounds__(64) k_mrhs_fin(const double *partial, int npar
   }
 }
 
+// psi_s += a_s p_s; p_s = b_s p_s + r_s over every stored entry (padding entries are zero)
 template <typename T>
 __global__ void __launch_bounds__(256) k_mrhs_update(int64_t n, int Ls, cx<T> *__restrict__ psi, cx<T> *__restrict__ p,
                                                      const cx<T> *__restrict__ r, const MrhsState *st) {
   if (mgated_prev(st)) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int s = (int)(i % Ls);
+    const int64_t e = (i / (12 * kBlk)) * kBlk + (i & (kBlk - 1));  // entry of stored position i
+    const int s = (int)(e % Ls);
     const T a = (T)st->a[s], b = (T)st->b[s];
     const cx<T> pv = p[i], xv = psi[i], rv = r[i];
     psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
@@ -1118,15 +1137,14 @@ __global__ void __launch_bounds__(256) k_mrhs_update(int64_t n, int Ls, cx<T> *_
 template <typename T>
 int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y, void *z, MrhsState *state,
                        double *out_dev) {
-  const int64_t rows = 12 * g.H4;
   const int rpb = 256 / g.Ls;
-  int64_t nb = (rows + rpb - 1) / rpb;
+  int64_t nb = (g.H4 + rpb - 1) / rpb;
   if (nb > kRedBlocks) nb = kRedBlocks;
   cx<T> *px = (cx<T> *)x;
   const cx<T> *py = (const cx<T> *)y;
   cx<T> *pz = (cx<T> *)z;
 #define GX_MR(M)                                                                                                  \
-  hipLaunchKernelGGL((k_mrhs_reduce<T, M>), dim3(nb), dim3(256), 0, ctx->stream, rows, g.Ls, px, py, pz,           \
+  hipLaunchKernelGGL((k_mrhs_reduce<T, M>), dim3(nb), dim3(256), 0, ctx->stream, g.H4, g.Ls, px, py, pz,           \
                      (const MrhsState *)state, ctx->red_partial)
   switch (mode) {
     case 0: GX_MR(0); break;
@@ -1144,7 +1162,7 @@ int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y
 
 template <typename T>
 int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state) {
-  const int64_t n = 12 * g.H4 * g.Ls;
+  const int64_t n = 12 * g.half_pad();
   hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, g.Ls, (cx<T> *)psi,
                      (cx<T> *)p, (const cx<T> *)r, (const MrhsState *)state);
   GX_CHECK_HIP(hipGetLastError());
@@ -1152,23 +1170,27 @@ int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *
 }
 
 // ---------------- fifth-dimension slice transfer (LinalgUtils.h:140-183, axpby_ssp_pplus/pminus) ------
-// z[s = sz] (=|+=) (spins in mask) x[s = sx] over both checkerboards of full fields; x has row length
-// Lsx, z has Lsz.  Used for the physical 4D <-> 5D source / solution maps.
+// z[s = sz] (=|+=) (spins in mask) x[s = sx] over both checkerboards of full fields; x has Lsx
+// slices (Px padded entries per half), z has Lsz (Pz).  Used for the physical 4D <-> 5D source /
+// solution maps and for ExtractSlice / InsertSlice.
 template <typename T>
-__global__ void __launch_bounds__(256) k_slice_proj(const cx<T> *__restrict__ x, int Lsx, int sx, cx<T> *__restrict__ z,
-                                                    int Lsz, int sz, int mask, int accumulate, int64_t H4) {
+__global__ void __launch_bounds__(256) k_slice_proj(const cx<T> *__restrict__ x, int Lsx, int sx, int64_t Px,
+                                                    cx<T> *__restrict__ z, int Lsz, int sz, int64_t Pz, int mask,
+                                                    int accumulate, int64_t H4) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < 2 * H4; j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t cb = j / H4, h = j - cb * H4;
-    const int64_t bx = cb * 12 * H4 * Lsx + h * Lsx + sx, bz = cb * 12 * H4 * Lsz + h * Lsz + sz;
+    const cx<T> *xb = x + cb * 12 * Px;
+    cx<T> *zb = z + cb * 12 * Pz;
+    const int64_t ex = h * Lsx + sx, ez = h * Lsz + sz;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
       const bool on = (mask >> (k / 3)) & 1;
-      cx<T> v = on ? x[bx + k * H4 * Lsx] : cx<T>{0, 0};
+      cx<T> v = on ? xb[bidx<T>(ex, k)] : cx<T>{0, 0};
       if (accumulate) {
-        const cx<T> o = z[bz + k * H4 * Lsz];
+        const cx<T> o = zb[bidx<T>(ez, k)];
         v = {o.r + v.r, o.i + v.i};
       }
-      z[bz + k * H4 * Lsz] = v;
+      zb[bidx<T>(ez, k)] = v;
     }
   }
 }
@@ -1176,8 +1198,9 @@ __global__ void __launch_bounds__(256) k_slice_proj(const cx<T> *__restrict__ x,
 template <typename T>
 int launch_slice_proj(const Geom &gx_, const void *x, int Lsx, int sx, void *z, int Lsz, int sz, int mask,
                       int accumulate, hipStream_t st) {
+  const int64_t Px = (gx_.H4 * Lsx + kBlk - 1) / kBlk * kBlk, Pz = (gx_.H4 * Lsz + kBlk - 1) / kBlk * kBlk;
   hipLaunchKernelGGL((k_slice_proj<T>), dim3(stream_blocks(2 * gx_.H4)), dim3(256), 0, st, (const cx<T> *)x, Lsx, sx,
-                     (cx<T> *)z, Lsz, sz, mask, accumulate, gx_.H4);
+                     Px, (cx<T> *)z, Lsz, sz, Pz, mask, accumulate, gx_.H4);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1200,7 +1223,7 @@ __device__ __forceinline__ double d_u01(uint64_t key, uint64_t ctr) {
 // two agree up to the rounding of log, sqrt, sin and cos.
 template <typename T>
 __global__ void __launch_bounds__(256) k_gaussian(cx<T> *__restrict__ out, KGeom g, int Ls, uint64_t key) {
-  const int64_t per_cb = g.H4 * Ls;
+  const int64_t per_cb = g.H4 * Ls;  // entries per checkerboard; g.stride = the padded count
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < 2 * per_cb;
        j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t cb = j / per_cb, rem = j - cb * per_cb;
@@ -1216,14 +1239,14 @@ __global__ void __launch_bounds__(256) k_gaussian(cx<T> *__restrict__ out, KGeom
     const int x = 2 * xh + (int)((cb + y + z + t) & 1);
     const int64_t lex = x + (int64_t)g.L[0] * (y + (int64_t)g.L[1] * (z + (int64_t)g.L[2] * t));
     const uint64_t c0 = (uint64_t)(lex * Ls + s) * 12;
-    cx<T> *o = out + cb * 12 * per_cb + rem;
+    cx<T> *o = out + cb * 12 * g.stride;
 #pragma unroll 2
     for (int k = 0; k < 12; k++) {
       const double a = d_u01(key, 2 * (c0 + k)), b = d_u01(key, 2 * (c0 + k) + 1);
       const double rad = sqrt(-2.0 * log(a));
       double sn, cs;
       sincos(2.0 * M_PI * b, &sn, &cs);
-      o[k * per_cb] = {(T)(rad * cs), (T)(rad * sn)};
+      o[bidx<T>(rem, k)] = {(T)(rad * cs), (T)(rad * sn)};
     }
   }
 }

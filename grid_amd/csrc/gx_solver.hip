@@ -137,7 +137,7 @@ template <typename T>
 static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGState *st) {
   GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
   Ctx *ctx = op->ctx;
-  const int64_t nsites = p->g.half_elems();
+  const int64_t nsites = p->g.half_pad();
   GX_TRY(launch_cg_dot<T>(ctx, nsites, nsites, p->data, mmp->data, st));
   GX_TRY(launch_cg_axpy_norm<T>(ctx, nsites, nsites, r->data, mmp->data, st));
   GX_TRY(launch_cg_update<T>(ctx, nsites, nsites, psi->data, p->data, r->data, st));
@@ -640,8 +640,9 @@ int gx_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver
     return GX_EINVAL;
   }
   Field *s = src, *p = psi;
-  if (s->subset == GX_FULL || p->subset == GX_FULL || s->prec != p->prec || (s->prec != GX_F64 && s->prec != GX_F32)) {
-    set_error("cg: needs half fields of one precision (F64 or F32)");
+  if (s->subset == GX_FULL || p->subset == GX_FULL || s->prec != p->prec || !same_geom(s->g, op->g) ||
+      !same_geom(p->g, op->g)) {
+    set_error("cg: needs half fields of one precision on the operator's lattice");
     return GX_EINVAL;
   }
   return cg_run(op, s, p, tol, maxit, st);
@@ -650,8 +651,8 @@ int gx_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver
 int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_tol, int inner_prec, int maxinner,
                 int maxouter, gx_solver_stats *st) {
   if (!op || !src || !psi || src == psi || src->prec != GX_F64 || psi->prec != GX_F64 ||
-      src->subset == GX_FULL || psi->subset == GX_FULL) {
-    set_error("mixed_cg: needs F64 half fields");
+      src->subset == GX_FULL || psi->subset == GX_FULL || !same_geom(src->g, op->g) || !same_geom(psi->g, op->g)) {
+    set_error("mixed_cg: needs F64 half fields on the operator's lattice");
     return GX_EINVAL;
   }
   if (inner_prec != GX_F32 && inner_prec != GX_F16) {
@@ -680,7 +681,8 @@ int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, 
   Op *op = op_;
   Field *src = src_, *psi = psi_;
   if (!op || !src || !psi || src == psi || op->kind != GX_WILSON || src->subset == GX_FULL ||
-      psi->subset == GX_FULL || src->prec != GX_F64 || psi->prec != GX_F64) {
+      psi->subset == GX_FULL || src->prec != GX_F64 || psi->prec != GX_F64 || !same_geom(src->g, op->g) ||
+      !same_geom(psi->g, op->g)) {
     set_error("block_cg: needs a multi-RHS Wilson operator and F64 half fields");
     return GX_EINVAL;
   }
