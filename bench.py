@@ -202,6 +202,8 @@ def main():
                     help='solver iterations as plain launches (for profiler runs; same kernels and timings)')
     ap.add_argument('--no-tune', action='store_true',
                     help='keep the library default launch knobs instead of autotuning each workload')
+    ap.add_argument('--knobs', default='',
+                    help='explicit hop knobs map,pf,nt,order (implies --no-tune), e.g. 0,1,1,0')
     args = ap.parse_args()
     legs = set(args.legs.split(','))
     args.no_cg = args.no_cg or 'cg' not in legs
@@ -220,6 +222,11 @@ def main():
     if args.no_graphs:
         gx.tune_set(gx.TUNE_CG_GRAPH, 0)
     from grid_amd import autotune
+    if args.knobs:
+        args.no_tune = True
+        for k, v in zip((gx.TUNE_DHOP_MAP, gx.TUNE_DHOP_PF, gx.TUNE_DHOP_NT, gx.TUNE_DHOP_ORDER),
+                        (int(x) for x in args.knobs.split(','))):
+            gx.tune_set(k, v)
     tuner = None if args.no_tune else autotune.Autotuner(ctx, cache_path=None)
     tuned = {}
 
@@ -245,10 +252,11 @@ def main():
         byts = H * (24 * LS + 72) * c + (H * LS * 8 if prec == gx.F16 else 0)
         res[name] = dict(ms=ms, bytes=int(byts), gbs=byts / (ms * 1e-3) / 1e9, gflops=1320.0 * H * LS / (ms * 1e-3) / 1e9)
         if prec == gx.F64:
-            # one device-to-device copy of known size (12 H Ls complex read and written) after the
-            # timed region: the calibration point for the HBM byte counters of a profiled run
+            # after the timed region, one streaming pass of known size (fo = 1.0 * fi: the stored
+            # half field read once and written once, k_axpby): the calibration point for the HBM
+            # byte counters of a profiled run
             fo.set_checkerboard_tag(1)
-            gx.field_copy(fo, fi)
+            gx.lib().gx_scale(fo.h, 1.0, fi.h)
             ctx.sync()
         fi.close()
         fo.close()
