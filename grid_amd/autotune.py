@@ -49,7 +49,7 @@ def search(candidates, measure, rounds=3):
 def hop_candidates(Ls):
     """Hop knob settings to try, each a tuple of (knob, value) pairs."""
     orders = (0, 1) if Ls in (1, 12, 16) else (0,)
-    return [tuple(zip(HOP_KNOBS, v)) for v in itertools.product((0, 1, 2, 3), (0, 1), (0, 1), orders)]
+    return [tuple(zip(HOP_KNOBS, v)) for v in itertools.product((0, 1, 2, 3, 4), (0, 1), (0, 1), orders)]
 
 
 def solver_candidates():

@@ -764,7 +764,7 @@ const char *gx_version(void) { return "gx 0.1 (gfx950)"; }
 
 int gx_tune_set(int knob, int value) {
   GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT, GX_EINVAL, "unknown tuning knob %d", knob);
-  if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 3, GX_EINVAL, "tile map takes 0..3");
+  if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 4, GX_EINVAL, "tile map takes 0..4");
   else if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   g_tune[knob] = value;
   g_tune_version++;

@@ -107,13 +107,46 @@ __device__ __forceinline__ void store_site(cx<S> *__restrict__ p, const cx<typen
   }
 }
 
+// n / d for 0 <= n < 2^31 by multiply and shift (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", 1994): l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (umulhi(n, m) + n) >> l.  The hop's site decomposition needs several divisions by lattice
+// extents per lane; the hardware has no integer divide, so each would cost ~35 instructions.
+struct FDiv {
+  uint32_t d, m;
+  int l;
+};
+
+static FDiv make_fdiv(uint32_t d) {
+  FDiv f;
+  f.d = d;
+  f.l = 0;
+  while ((1u << f.l) < d) f.l++;
+  f.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << f.l) - d)) / d + 1);
+  return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FDiv &f) { return (__umulhi(n, f.m) + n) >> f.l; }
+
 struct KGeom {
   int L[4];
   int Lxh;
   int64_t H4;
   int64_t stride;          // entries per checkerboard including block padding (Geom::half_pad)
   int64_t sY, sZ, sT;      // half-site strides of y, z, t
+  // blocked sweep (tile map 4): blocks of Lxh x by x bz sites in one t-slice; a block's t-slices
+  // are consecutive in the sweep (t fastest), blocks ordered y-block first
+  int by, bz, nby;
+  FDiv fLxh, fL1, fL2, fL3, fby, fnby, fbs;  // division by Lxh, Ly, Lz, Lt, by, nby, Lxh*by*bz
 };
+
+static void set_sweep_blocks(KGeom &k, int by, int bz) {
+  k.by = by;
+  k.bz = bz;
+  k.nby = k.L[1] / by;
+  k.fby = make_fdiv(by);
+  k.fnby = make_fdiv(k.nby);
+  k.fbs = make_fdiv((uint32_t)k.Lxh * by * bz);
+}
 
 static KGeom kgeom(const Geom &g) {
   KGeom k;
@@ -124,7 +157,23 @@ static KGeom kgeom(const Geom &g) {
   k.sY = k.Lxh;
   k.sZ = k.sY * g.L[1];
   k.sT = k.sZ * g.L[2];
+  k.fLxh = make_fdiv(k.Lxh);
+  k.fL1 = make_fdiv(g.L[1]);
+  k.fL2 = make_fdiv(g.L[2]);
+  k.fL3 = make_fdiv(g.L[3]);
+  set_sweep_blocks(k, g.L[1], g.L[2]);
   return k;
+}
+
+// Site (half-lattice index h) visited at position r of the blocked sweep.
+__device__ __forceinline__ uint32_t sweep_site(const KGeom &g, uint32_t r) {
+  const uint32_t q = fdiv(r, g.fbs), w = r - q * g.fbs.d;   // block-slice, position in it
+  const uint32_t yzb = fdiv(q, g.fL3), t = q - yzb * (uint32_t)g.L[3];
+  const uint32_t bzi = fdiv(yzb, g.fnby), byi = yzb - bzi * (uint32_t)g.nby;
+  const uint32_t w2 = fdiv(w, g.fLxh), xh = w - w2 * (uint32_t)g.Lxh;
+  const uint32_t zz = fdiv(w2, g.fby), yy = w2 - zz * (uint32_t)g.by;
+  const uint32_t y = byi * g.by + yy, z = bzi * g.bz + zz;
+  return xh + (uint32_t)g.Lxh * (y + (uint32_t)g.L[1] * (z + (uint32_t)g.L[2] * t));
 }
 
 __device__ __forceinline__ bool gated(const Gate *gate) {
@@ -208,7 +257,8 @@ constexpr int kLinkPad = 73;
 // (b % 8) * per + b / 8 gives each XCD (and its L2) one contiguous slab of the lattice.  map = 2, 3:
 // within every group of 8*R consecutive tiles (R = 8, 32) each XCD takes a run of R tiles, so the
 // XCDs sweep the lattice together (shared Infinity Cache window) while each keeps runs in its L2.
-// The grid is padded to a multiple of 8 (map 1) or 8*R (map 2, 3) blocks; tiles past the end exit.
+// map = 4: tiles are runs of the blocked sweep (sweep_site) and the XCDs take slabs as in map 1.
+// The grid is padded to a multiple of 8 (map 1, 4) or 8*R (map 2, 3) blocks; tiles past the end exit.
 __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nblocks, int map) {
   if (map == 1) {
     const uint32_t per = (nblocks + 7) >> 3;
@@ -343,7 +393,8 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   constexpr bool MAT = EPI >= 2, PHI = (EPI & 1) != 0;
   constexpr int NSTAGE = MAT ? 4 * 12 * 64 : 1;
   __shared__ cx<T> lds[NLINK > NSTAGE ? NLINK : NSTAGE];  // links, then (MAT) the hop results
-  const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, map);
+  const bool sweep = map == 4;  // blocked sweep: tiles are runs of the sweep order, XCD slabs
+  const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, sweep ? 1 : map);
   if (tile >= ntiles) return;  // uniform across the workgroup
   const uint32_t H4 = (uint32_t)g.H4;
   const uint32_t h0 = tile * SPB;
@@ -353,15 +404,13 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   const int s = lane - ls * LS;
   const int lsb = w * SPW + ls;
   const bool valid = (ls < SPW) && (h0 + lsb < H4);
-  const uint32_t h = valid ? h0 + lsb : h0;  // idle lanes compute in-bounds addresses and skip the work
-  uint32_t r = h;
-  const uint32_t Lxh = (uint32_t)g.Lxh;
-  const int xh = (int)(r % Lxh);
-  r /= Lxh;
-  const int y = (int)(r % (uint32_t)g.L[1]);
-  r /= (uint32_t)g.L[1];
-  const int z = (int)(r % (uint32_t)g.L[2]);
-  const int t = (int)(r / (uint32_t)g.L[2]);
+  // idle lanes compute in-bounds addresses and skip the work
+  const uint32_t h = sweep ? sweep_site(g, valid ? h0 + lsb : h0) : (valid ? h0 + lsb : h0);
+  const uint32_t r1 = fdiv(h, g.fLxh), r2 = fdiv(r1, g.fL1), r3 = fdiv(r2, g.fL2);
+  const int xh = (int)(h - r1 * (uint32_t)g.Lxh);
+  const int y = (int)(r1 - r2 * (uint32_t)g.L[1]);
+  const int z = (int)(r2 - r3 * (uint32_t)g.L[2]);
+  const int t = (int)r3;
   const int x = 2 * xh + ((cb_out + y + z + t) & 1);
   const uint32_t base = h - xh;
   const int xp = (x + 1 == g.L[0]) ? 0 : x + 1;
@@ -386,7 +435,9 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     for (int j = 0; j < NU; j++) {
       const uint32_t e = threadIdx.x + 256 * j;
       const uint32_t pk = e / SPB, site = e - pk * SPB;
-      ureg[j] = (e < SPB * 72 && h0 + site < H4) ? ldc<S>(U, pk * H4 + h0 + site) : cx<T>{0, 0};
+      const bool in = e < SPB * 72 && h0 + site < H4;
+      const uint32_t hs = sweep && in ? sweep_site(g, h0 + site) : h0 + site;
+      ureg[j] = in ? ldc<S>(U, pk * H4 + hs) : cx<T>{0, 0};
     }
   }
   cx<T> fa[12], fb[12];
@@ -458,8 +509,7 @@ template <typename S>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                     hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta) {
   typedef typename Sto<S>::C T;
-  const KGeom k = kgeom(g);
-  if (13 * 2 * k.stride >= (int64_t)1 << 31) {
+  if (13 * 2 * g.half_pad() >= (int64_t)1 << 31) {
     set_error("dhop: lattice too large for 32-bit site indexing");
     return GX_EINVAL;
   }
@@ -472,7 +522,21 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
     return GX_EINVAL;
   }
   const int kepi = (epi == 2 && phi) ? 3 : epi;
-  const int map = tune_get(GX_TUNE_DHOP_MAP);
+  int map = tune_get(GX_TUNE_DHOP_MAP);
+  KGeom k = kgeom(g);
+  if (map == 4) {
+    // blocks of Lxh x by x bz sites per t-slice with three t-slices of the input (and the links)
+    // within 3.5 MB of an XCD's L2; halve by or bz while they are even and the block is too big
+    const double site_bytes = (double)g.Ls * 12 * sizeof(cx<S>) + 72.0 * sizeof(cx<S>);
+    int by = g.L[1], bz = g.L[2];
+    while (3.0 * k.Lxh * by * bz * site_bytes > 3.5e6) {
+      if (bz >= by && bz % 2 == 0) bz /= 2;
+      else if (by % 2 == 0) by /= 2;
+      else if (bz % 2 == 0) bz /= 2;
+      else break;
+    }
+    set_sweep_blocks(k, by, bz);
+  }
   const bool pf = tune_get(GX_TUNE_DHOP_PF) != 0;
   const int flags = tune_get(GX_TUNE_DHOP_NT) ? 1 : 0;
   const bool ord = epi == 0 && tune_get(GX_TUNE_DHOP_ORDER) != 0 && (g.Ls == 1 || g.Ls == 12 || g.Ls == 16);
@@ -482,7 +546,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta};
   const int spb = 4 * (64 / g.Ls);
   const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
-  const uint32_t pad = map == 1 ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
+  const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
   const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
 #define GX_DHOP_GO(LS, DAG, PF, ORD, EPI)                                                                         \
   hipLaunchKernelGGL((k_dhop<S, LS, DAG, PF, ORD, EPI>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
@@ -1081,14 +1145,25 @@ __global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<T> *
 }
 
 // which: 0 -> d_s, a_s;  2 -> cp_s, b_s, convergence, k += 1;  1, 3 -> out[s] = sum
-__global__ void __launch_bounds__(64) k_mrhs_fin(const double *partial, int nparts, int Ls, MrhsState *st,
-                                                 double *out, int which) {
+// 256 threads: thread t adds the partials b = t/Ls, t/Ls + 256/Ls, ... of slice t % Ls, and the
+// per-thread sums of each slice are then added in thread order (fixed order, so deterministic).
+__global__ void __launch_bounds__(256) k_mrhs_fin(const double *partial, int nparts, int Ls, MrhsState *st,
+                                                  double *out, int which) {
   const bool skip = (which == 0 || which == 2) && mgated(st);
   if (skip && which == 0) return;
+  __shared__ double sh[256];
+  const int per = 256 / Ls;
+  const int ts = threadIdx.x % Ls, tp = threadIdx.x / Ls;
+  double acc = 0;
+  if (tp < per && !skip)
+    for (int b = tp; b < nparts; b += per) acc += partial[b * Ls + ts];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
   const int s = threadIdx.x;
   double v = 0;
-  if (s < Ls && !skip)
-    for (int b = 0; b < nparts; b++) v += partial[b * Ls + s];
+  if (s < Ls)
+    for (int j = 0; j < per; j++) v += sh[j * Ls + s];
+  __syncthreads();
   if (which == 1 || which == 3) {
     if (s < Ls) out[s] = v;
     return;
@@ -1154,7 +1229,7 @@ int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y
     default: set_error("mrhs reduce mode %d", mode); return GX_EINVAL;
   }
 #undef GX_MR
-  hipLaunchKernelGGL(k_mrhs_fin, dim3(1), dim3(64), 0, ctx->stream, (const double *)ctx->red_partial, (int)nb, g.Ls,
+  hipLaunchKernelGGL(k_mrhs_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, (int)nb, g.Ls,
                      state, out_dev, mode);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;

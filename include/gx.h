@@ -265,7 +265,8 @@ int gx_serve(const gx_serve_params *p, const gx_request *req, int n, gx_response
  *      the autotuner in grid_amd/autotune.py sets them from measurements. ---- */
 enum gx_tune_knob {
   GX_TUNE_DHOP_MAP = 0,     /* 0: linear workgroup->tile map, 1: XCD-contiguous slabs,
-                               2, 3: XCDs take runs of 8 / 32 tiles of a common sweep */
+                               2, 3: XCDs take runs of 8 / 32 tiles of a common sweep,
+                               4: blocked sweep (y-z blocks swept along t, sized for the L2), XCD slabs */
   GX_TUNE_DHOP_PF = 1,      /* 1: issue the next leg's neighbour loads before this leg's math */
   GX_TUNE_DHOP_NT = 2,      /* 1: non-temporal (streaming) output stores */
   GX_TUNE_DHOP_ORDER = 3,   /* 0: Grid's leg order, 1: the two legs of each direction adjacent */

@@ -26,7 +26,7 @@ def test_search_breaks_ties_by_order():
 def test_candidates_cover_knob_space():
     c16 = at.hop_candidates(16)
     c8 = at.hop_candidates(8)
-    assert len(c16) == 32 and len(c8) == 16 and len(set(c16)) == 32
+    assert len(c16) == 40 and len(c8) == 20 and len(set(c16)) == 40
     assert all(dict(c)[gx.TUNE_DHOP_ORDER] == 0 for c in c8)
     assert len(at.solver_candidates()) == 4
 

@@ -343,3 +343,27 @@ def test_block_cg_matches_single_rhs(ctx, inner):
         ref, rst = O1.cg(np.ascontiguousarray(src[..., k:k + 1, :, :]), tol=1e-10)
         assert sts[k]['converged'] and sts[k]['true_residual'] < 1e-9, sts[k]
         assert rel_err(x[..., k:k + 1, :, :], ref) < 1e-8
+
+
+@pytest.mark.parametrize('kind,dims,Ls,prec', [('mobius', (8, 8, 8, 8), 16, gx.F64), ('wilson', (8, 4, 8, 8), 12, gx.F32),
+                                               ('wilson', (8, 8, 4, 4), 1, gx.F64), ('mobius', (4, 6, 4, 8), 5, gx.F16)])
+def test_tile_maps_bitwise(ctx, kind, dims, Ls, prec):
+    """Every workgroup->tile map (linear, XCD slabs, XCD runs, blocked sweep) only reorders the
+    work: the hop's output is bitwise the same."""
+    U, psi = inputs(dims, Ls, 121)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, mass=0.1)
+    fi, fo = D.field(prec, ODD).upload(half(psi, dims, ODD)), D.field(prec, EVEN)
+    saved = gx.tune_get(gx.TUNE_DHOP_MAP)
+    outs = []
+    try:
+        for m in range(5):
+            gx.tune_set(gx.TUNE_DHOP_MAP, m)
+            fo.zero()
+            D.DhopEO(fi, fo, 0)
+            outs.append(fo.download())
+    finally:
+        gx.tune_set(gx.TUNE_DHOP_MAP, saved)
+    for m in range(1, 5):
+        assert np.array_equal(outs[m], outs[0]), m
+    O = Oracle(U, dims, Ls, kind, mass=0.1)
+    assert max_site_rel_err(outs[0], O.dhop(half(psi, dims, ODD), EVEN, 0)) < {gx.F64: 1e-13, gx.F32: 1e-6, gx.F16: 5e-3}[prec]
