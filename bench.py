@@ -126,10 +126,11 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10, tune=None)
 
 
 def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-10, tune=None):
-    """Wilson 32^3x64 (eps 0.3, m 0.05), the 12 spin-colour point sources of one seeded site: one
-    mixed-precision multi-RHS red-black solve (FP32 CGmultiRHS inner solves sharing every link load
-    across the 12 sources, FP64 defect correction and patch-up) against 12 sequential FP64
-    red-black CG solves, all to 1e-10.  Wall clock around each solve."""
+    """Wilson 32^3x64 (eps 0.3, m 0.05), the 12 spin-colour point sources of one seeded site:
+    mixed-precision multi-RHS red-black solves (CGmultiRHS inner solves in FP32, and in FP16
+    storage, sharing every link load across the 12 sources, FP64 defect correction and patch-up)
+    against 12 sequential FP64 red-black CG solves, all to 1e-10.  Wall clock around each solve;
+    'speedup' is the faster mixed variant's (named in 'speedup_variant')."""
     nrhs = 12
     U = gx.random_gauge(dims, 1004, eps)
     rng = np.random.default_rng(4004)
@@ -150,27 +151,38 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-1
         seq_it.append(st['iterations'])
         seq.append(x)
     B = base.clone(ctx, nrhs, mass)
-    if tune:
-        tune('c4', B, gx.F32, dhop_iters=5, rounds=2, cg_iters=16)
-    sb, xb = B.field().point_source(x0), B.field()
-    ctx.sync()
-    t0 = time.perf_counter()
-    stb = B.schur_solve(sb, xb, tol=tol, maxit=20000, mixed=True)
-    ctx.sync()
-    blk_sec = time.perf_counter() - t0
-    t4, worst = base.field(), 0.0
-    for k in range(nrhs):
-        gx.extract_slice(t4, xb, k)
-        gx.axpby(t4, 1.0, -1.0, t4, seq[k])
-        worst = max(worst, (t4.norm2() / seq[k].norm2()) ** 0.5)
-    return {'workload': 'Wilson 32^3x64 m=0.05, 12 point sources', 'origin': list(x0),
-            'sequential_fp64': {'seconds': round(seq_sec, 4), 'per_rhs_seconds': round(seq_sec / nrhs, 4),
-                                'iterations': seq_it},
-            'multi_rhs_mixed': {'seconds': round(blk_sec, 4), 'per_rhs_seconds': round(blk_sec / nrhs, 4),
-                                'inner_iterations': stb['inner_iterations'], 'outer': stb['outer_iterations'],
-                                'final_iterations': stb['final_iterations'], 'true_residual': stb['true_residual'],
-                                'unprec_residual': stb['unprec_residual']},
-            'speedup': round(seq_sec / blk_sec, 3), 'max_rel_diff_vs_sequential': worst}
+    sb = B.field().point_source(x0)
+    t4 = base.field()
+    out = {'workload': 'Wilson 32^3x64 m=0.05, 12 point sources', 'origin': list(x0),
+           'sequential_fp64': {'seconds': round(seq_sec, 4), 'per_rhs_seconds': round(seq_sec / nrhs, 4),
+                               'iterations': seq_it}}
+    best = None
+    for name, mixed, prec, tkey in (('multi_rhs_mixed', 1, gx.F32, 'c4'), ('multi_rhs_mixed_f16', 2, gx.F16, 'c4_f16')):
+        if tune:
+            tune(tkey, B, prec, dhop_iters=5, rounds=2, cg_iters=16)
+        xb = B.field()
+        ctx.sync()
+        t0 = time.perf_counter()
+        stb = B.schur_solve(sb, xb, tol=tol, maxit=20000, mixed=mixed)
+        ctx.sync()
+        blk_sec = time.perf_counter() - t0
+        worst = 0.0
+        for k in range(nrhs):
+            gx.extract_slice(t4, xb, k)
+            gx.axpby(t4, 1.0, -1.0, t4, seq[k])
+            worst = max(worst, (t4.norm2() / seq[k].norm2()) ** 0.5)
+        xb.close()
+        out[name] = {'inner_storage': {gx.F32: 'fp32', gx.F16: 'fp16'}[prec],
+                     'seconds': round(blk_sec, 4), 'per_rhs_seconds': round(blk_sec / nrhs, 4),
+                     'inner_iterations': stb['inner_iterations'], 'outer': stb['outer_iterations'],
+                     'final_iterations': stb['final_iterations'], 'true_residual': stb['true_residual'],
+                     'unprec_residual': stb['unprec_residual'], 'speedup': round(seq_sec / blk_sec, 3),
+                     'max_rel_diff_vs_sequential': worst}
+        if best is None or blk_sec < out[best]['seconds']:
+            best = name
+    out['speedup'] = out[best]['speedup']
+    out['speedup_variant'] = best
+    return out
 
 
 def bench_serving(n, rate, slots=4):

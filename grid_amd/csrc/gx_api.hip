@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -860,10 +861,12 @@ static void pack_f16(const Field *f, const float *in, uint32_t *out) {
           const float *v = ih + blk_index(i, k, 12) * 2;
           m = std::max(m, std::max(std::fabs(v[0]), std::fabs(v[1])));
         }
-        const float inv = m > 0.f ? 1.f / m : 0.f;
+        // a subnormal maximum is scaled by 2^64 (exact) first, as 1/m would overflow
+        const float up = (m > 0.f && m < FLT_MIN) ? 0x1p64f : 1.f;
+        const float inv = m > 0.f ? 1.f / (m * up) : 0.f;
         for (int k = 0; k < 12; k++) {
           const float *v = ih + blk_index(i, k, 12) * 2;
-          const half_t pr[2] = {(half_t)(v[0] * inv), (half_t)(v[1] * inv)};
+          const half_t pr[2] = {(half_t)((v[0] * up) * inv), (half_t)((v[1] * up) * inv)};
           std::memcpy(&oh[blk_index(i, k, 13)], pr, 4);
         }
         std::memcpy(&oh[blk_index(i, 12, 13)], &m, 4);

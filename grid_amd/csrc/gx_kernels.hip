@@ -12,10 +12,18 @@
 //   chirality (CayleyFermion5Dcache.h:41-230); the kernel applies them from LDS.
 //   reductions: per-site products in the field precision, sums in double
 //   (Grid/lattice/Lattice_reduction.h:255-287), in a fixed order.
+#include <cfloat>
 #include <climits>
 #include <cstring>
 
 #include "gx_internal.h"
+
+// The file is compiled as three translation units (GX_PART 0, 1, 2 instantiate the FP64, FP32 and
+// FP16 storage types; gx_kernels_f32.hip and gx_kernels_f16.hip include it) so the storage types
+// build in parallel.  Non-template kernels are static: every unit carries its own copy.
+#ifndef GX_PART
+#define GX_PART 0
+#endif
 
 namespace gx {
 
@@ -70,12 +78,16 @@ __device__ __forceinline__ typename Sto<S>::C site_scale(const cx<S> *__restrict
   else return (typename Sto<S>::C)1;
 }
 
+// Column k of an entry lies k * kBlk storage elements after its column 0: the site helpers form
+// one 64-bit address per entry and reach the columns by constant offsets (immediate offsets of the
+// loads and stores instead of one 64-bit address computation per column).
 template <typename S, typename I>
 __device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ p, I i) {
+  const cx<S> *__restrict__ q = p + bidx<S, I>(i, 0);
 #pragma unroll
-  for (int k = 0; k < 12; k++) f[k] = ldc<S>(p, bidx<S, I>(i, k));
+  for (int k = 0; k < 12; k++) f[k] = ldc<S>(q, k * kBlk);
   if constexpr (Sto<S>::H) {
-    const float sc = site_scale<S, I>(p, i);
+    const float sc = reinterpret_cast<const float *>(q)[12 * kBlk];
 #pragma unroll
     for (int k = 0; k < 12; k++) f[k] = {f[k].r * sc, f[k].i * sc};
   }
@@ -84,25 +96,34 @@ __device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S>
 // store the 12 components of entry i (FP16: with the site's scale); nt = non-temporal stores
 template <typename S, typename I>
 __device__ __forceinline__ void store_site(cx<S> *__restrict__ p, const cx<typename Sto<S>::C> *a, I i, bool nt) {
+  cx<S> *__restrict__ q = p + bidx<S, I>(i, 0);
   if constexpr (Sto<S>::H) {
     float m = 0.f;
 #pragma unroll
     for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(a[k].r), fabsf(a[k].i)));
-    const float inv = m > 0.f ? 1.f / m : 0.f;
+    if (__builtin_expect(m > 0.f && m < FLT_MIN, 0)) {
+      // subnormal site maximum: 1/m overflows, so scale the site by 2^64 (exact) before it
+      const float inv = 1.f / (m * 0x1p64f);
 #pragma unroll
-    for (int k = 0; k < 12; k++) p[bidx<S, I>(i, k)] = {(half_t)(a[k].r * inv), (half_t)(a[k].i * inv)};
-    reinterpret_cast<float *>(p)[bidx<S, I>(i, 12)] = m;
+      for (int k = 0; k < 12; k++)
+        q[k * kBlk] = {(half_t)((a[k].r * 0x1p64f) * inv), (half_t)((a[k].i * 0x1p64f) * inv)};
+    } else {
+      const float inv = m > 0.f ? 1.f / m : 0.f;
+#pragma unroll
+      for (int k = 0; k < 12; k++) q[k * kBlk] = {(half_t)(a[k].r * inv), (half_t)(a[k].i * inv)};
+    }
+    reinterpret_cast<float *>(q)[12 * kBlk] = m;
   } else {
     typedef S v2 __attribute__((ext_vector_type(2)));
     if (nt) {
 #pragma unroll
       for (int k = 0; k < 12; k++) {
         v2 v = {a[k].r, a[k].i};
-        __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(p + bidx<S, I>(i, k)));
+        __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(q + k * kBlk));
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) p[bidx<S, I>(i, k)] = a[k];
+      for (int k = 0; k < 12; k++) q[k * kBlk] = a[k];
     }
   }
 }
@@ -428,27 +449,31 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   // Link tile of the workgroup: global -> registers first, then the first neighbour loads, then
   // registers -> LDS and a barrier that waits for LDS traffic only, so the neighbour loads stay in
   // flight across it.
-  constexpr int NU = LDSU ? (SPB * 72 + 255) / 256 : 1;
+  // Thread t < G * SPB stages link entries pk = t / SPB + G j of tile site t % SPB (G = 256 / SPB
+  // groups), so it finds its site's lattice index once; the lanes of a group read consecutive sites.
+  constexpr int G = 256 / SPB;
+  constexpr int NU = LDSU ? (72 + G - 1) / G : 1;
+  const uint32_t ugrp = threadIdx.x / SPB, usite = threadIdx.x - ugrp * SPB;
+  const bool ustage = LDSU && ugrp < (uint32_t)G && h0 + usite < H4;
   cx<T> ureg[NU];
   if (LDSU) {
+    const uint32_t hs = sweep && ustage ? sweep_site(g, h0 + usite) : h0 + usite;
 #pragma unroll
     for (int j = 0; j < NU; j++) {
-      const uint32_t e = threadIdx.x + 256 * j;
-      const uint32_t pk = e / SPB, site = e - pk * SPB;
-      const bool in = e < SPB * 72 && h0 + site < H4;
-      const uint32_t hs = sweep && in ? sweep_site(g, h0 + site) : h0 + site;
-      ureg[j] = in ? ldc<S>(U, pk * H4 + hs) : cx<T>{0, 0};
+      const uint32_t pk = ugrp + G * j;
+      ureg[j] = ustage && pk < 72 ? ldc<S>(U, pk * H4 + hs) : cx<T>{0, 0};
     }
   }
   cx<T> fa[12], fb[12];
   load_site<S>(fa, in, leg_idx<DAG, ORD, 0>(nb, LS, s));
   if (PF) load_site<S>(fb, in, leg_idx<DAG, ORD, 1>(nb, LS, s));
   if (LDSU) {
+    if (ugrp < (uint32_t)G) {
 #pragma unroll
-    for (int j = 0; j < NU; j++) {
-      const uint32_t e = threadIdx.x + 256 * j;
-      const uint32_t pk = e / SPB, site = e - pk * SPB;
-      if (e < SPB * 72) lds[site * kLinkPad + pk] = ureg[j];
+      for (int j = 0; j < NU; j++) {
+        const uint32_t pk = ugrp + G * j;
+        if (pk < 72) lds[usite * kLinkPad + pk] = ureg[j];
+      }
     }
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
   }
@@ -483,12 +508,13 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     }
   }
   if (EPI != 0) {
-    const T bsc = PHI ? ep.beta * site_scale<S>(ep.phi, o) : T(0);
+    const cx<S> *__restrict__ qp = PHI ? ep.phi + bidx<S>(o, 0) : ep.phi;
+    const T bsc = PHI ? ep.beta * site_scale<S>(qp, 0u) : T(0);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
       cx<T> v = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
       if (PHI) {
-        const cx<T> p = ldc<S>(ep.phi, bidx<S>(o, k));
+        const cx<T> p = ldc<S>(qp, k * kBlk);
         v.r = fma(bsc, p.r, v.r);
         v.i = fma(bsc, p.i, v.i);
       }
@@ -660,10 +686,11 @@ __global__ void __launch_bounds__(256) k_smat(const cx<S> *__restrict__ in, cons
     }
   }
   if (PHI) {
-    const T bsc = beta * site_scale<S>(phi, o);
+    const cx<S> *__restrict__ qp = phi + bidx<S>(o, 0);
+    const T bsc = beta * site_scale<S>(qp, 0u);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      const cx<T> p = ldc<S>(phi, bidx<S>(o, k));
+      const cx<T> p = ldc<S>(qp, k * kBlk);
       acc[k].r = fma(bsc, p.r, acc[k].r);
       acc[k].i = fma(bsc, p.i, acc[k].i);
     }
@@ -892,7 +919,7 @@ __global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, 
   write_partial<2>(v, partial);
 }
 
-__global__ void __launch_bounds__(256) k_reduce_fin(const double *partial, int nparts, double *result, int nout) {
+static __global__ void __launch_bounds__(256) k_reduce_fin(const double *partial, int nparts, double *result, int nout) {
   __shared__ double sh[2][4];
   double v[2];
   sum_partials<2>(v, partial, nparts, sh);
@@ -993,7 +1020,7 @@ __global__ void __launch_bounds__(256) k_cg_axpy_norm(int64_t nsites, int64_t st
 // which = 0: d = sum, a = c / d.   which = 1: cp = sum, b = cp / c, c = cp, convergence test.
 // The which = 1 finaliser advances k in every iteration, also once the solve has stopped, so the
 // gate of the following solution update (k-1) closes after the last real iteration.
-__global__ void __launch_bounds__(256) k_cg_fin(const double *partial, int nparts, CGState *st, int which) {
+static __global__ void __launch_bounds__(256) k_cg_fin(const double *partial, int nparts, CGState *st, int which) {
   const bool skip = gated(st);
   if (skip && which == 0) return;
   __shared__ double sh[1][4];
@@ -1098,10 +1125,11 @@ __device__ __forceinline__ bool mgated_prev(const MrhsState *st) {
   return st->k - 1 > st->k_conv || st->k - 1 > st->kmax;
 }
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<T> *__restrict__ x,
-                                                     const cx<T> *__restrict__ y, cx<T> *__restrict__ z,
+template <typename S, int MODE>
+__global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<S> *__restrict__ x,
+                                                     const cx<S> *__restrict__ y, cx<S> *__restrict__ z,
                                                      const MrhsState *st, double *partial) {
+  typedef typename Sto<S>::C T;
   if ((MODE == 0 || MODE == 2) && mgated(st)) return;
   __shared__ double sh[256];
   const int rpb = 256 / Ls;
@@ -1112,27 +1140,25 @@ __global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<T> *
     const T a = (MODE == 2) ? (T)(-st->a[s]) : (T)0;
     for (int64_t h = (int64_t)blockIdx.x * rpb + rt; h < H4; h += (int64_t)gridDim.x * rpb) {
       const int64_t e = h * Ls + s;
+      cx<T> xv[12], yv[12];
+      load_site<S>(xv, x, e);
+      if (MODE != 1) load_site<S>(yv, y, e);
 #pragma unroll
       for (int k = 0; k < 12; k++) {
-        const int64_t i = bidx<T>(e, k);
-        const cx<T> xv = x[i];
         if (MODE == 0) {
-          const cx<T> yv = y[i];
-          acc += (double)fma(xv.r, yv.r, xv.i * yv.i);
+          acc += (double)fma(xv[k].r, yv[k].r, xv[k].i * yv[k].i);
         } else if (MODE == 1) {
-          acc += (double)fma(xv.r, xv.r, xv.i * xv.i);
+          acc += (double)fma(xv[k].r, xv[k].r, xv[k].i * xv[k].i);
         } else if (MODE == 2) {
-          const cx<T> yv = y[i];
-          const cx<T> v = {fma(a, yv.r, xv.r), fma(a, yv.i, xv.i)};
-          x[i] = v;
-          acc += (double)fma(v.r, v.r, v.i * v.i);
+          xv[k] = {fma(a, yv[k].r, xv[k].r), fma(a, yv[k].i, xv[k].i)};
+          acc += (double)fma(xv[k].r, xv[k].r, xv[k].i * xv[k].i);
         } else {
-          const cx<T> yv = y[i];
-          const cx<T> v = {yv.r - xv.r, yv.i - xv.i};
-          z[i] = v;
-          acc += (double)fma(v.r, v.r, v.i * v.i);
+          xv[k] = {yv[k].r - xv[k].r, yv[k].i - xv[k].i};
+          acc += (double)fma(xv[k].r, xv[k].r, xv[k].i * xv[k].i);
         }
       }
+      if (MODE == 2) store_site<S>(x, xv, e, false);
+      if (MODE == 3) store_site<S>(z, xv, e, false);
     }
   }
   sh[threadIdx.x] = acc;
@@ -1147,7 +1173,7 @@ __global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<T> *
 // which: 0 -> d_s, a_s;  2 -> cp_s, b_s, convergence, k += 1;  1, 3 -> out[s] = sum
 // 256 threads: thread t adds the partials b = t/Ls, t/Ls + 256/Ls, ... of slice t % Ls, and the
 // per-thread sums of each slice are then added in thread order (fixed order, so deterministic).
-__global__ void __launch_bounds__(256) k_mrhs_fin(const double *partial, int nparts, int Ls, MrhsState *st,
+static __global__ void __launch_bounds__(256) k_mrhs_fin(const double *partial, int nparts, int Ls, MrhsState *st,
                                                   double *out, int which) {
   const bool skip = (which == 0 || which == 2) && mgated(st);
   if (skip && which == 0) return;
@@ -1209,6 +1235,28 @@ __global__ void __launch_bounds__(256) k_mrhs_update(int64_t n, int Ls, cx<T> *_
   }
 }
 
+// FP16 storage: the same update per (h, s) entry (each store rescales its entry)
+static __global__ void __launch_bounds__(256) k_mrhs_update_h(int64_t nent, int Ls, cx<half_t> *__restrict__ psi,
+                                                       cx<half_t> *__restrict__ p, const cx<half_t> *__restrict__ r,
+                                                       const MrhsState *st) {
+  if (mgated_prev(st)) return;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nent; e += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e % Ls);
+    const float a = (float)st->a[s], b = (float)st->b[s];
+    cx<float> pv[12], xv[12], rv[12];
+    load_site<half_t>(pv, p, e);
+    load_site<half_t>(xv, psi, e);
+    load_site<half_t>(rv, r, e);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      xv[k] = {fma(a, pv[k].r, xv[k].r), fma(a, pv[k].i, xv[k].i)};
+      pv[k] = {fma(b, pv[k].r, rv[k].r), fma(b, pv[k].i, rv[k].i)};
+    }
+    store_site<half_t>(psi, xv, e, false);
+    store_site<half_t>(p, pv, e, false);
+  }
+}
+
 template <typename T>
 int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y, void *z, MrhsState *state,
                        double *out_dev) {
@@ -1237,9 +1285,16 @@ int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y
 
 template <typename T>
 int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state) {
-  const int64_t n = 12 * g.half_pad();
-  hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, g.Ls, (cx<T> *)psi,
-                     (cx<T> *)p, (const cx<T> *)r, (const MrhsState *)state);
+  if constexpr (Sto<T>::H) {
+    // every (h, s) entry of the half field, padding excluded (its entries stay zero)
+    const int64_t nent = g.H4 * g.Ls;
+    hipLaunchKernelGGL(k_mrhs_update_h, dim3(stream_blocks(nent)), dim3(256), 0, ctx->stream, nent, g.Ls,
+                       (cx<half_t> *)psi, (cx<half_t> *)p, (const cx<half_t> *)r, (const MrhsState *)state);
+  } else {
+    const int64_t n = 12 * g.half_pad();
+    hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, g.Ls, (cx<T> *)psi,
+                       (cx<T> *)p, (const cx<T> *)r, (const MrhsState *)state);
+  }
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1337,7 +1392,7 @@ int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st) {
 
 // digest = sum_i splitmix64(word_i ^ splitmix64(i)) mod 2^64: integer addition is exact and
 // commutative, so the result is independent of the order in which blocks finish.
-__global__ void __launch_bounds__(256) k_digest(const uint64_t *__restrict__ w, int64_t n, unsigned long long *out) {
+static __global__ void __launch_bounds__(256) k_digest(const uint64_t *__restrict__ w, int64_t n, unsigned long long *out) {
   uint64_t acc = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     acc += d_splitmix64(w[i] ^ d_splitmix64((uint64_t)i));
@@ -1348,6 +1403,7 @@ __global__ void __launch_bounds__(256) k_digest(const uint64_t *__restrict__ w, 
   if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+#if GX_PART == 0
 int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
   GX_CHECK_HIP(hipMemsetAsync(ctx->digest, 0, sizeof(unsigned long long), ctx->stream));
   hipLaunchKernelGGL(k_digest, dim3(stream_blocks(nwords)), dim3(256), 0, ctx->stream, (const uint64_t *)data, nwords,
@@ -1359,6 +1415,7 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
   std::memcpy(out, ctx->pinned, sizeof(uint64_t));
   return GX_OK;
 }
+#endif
 
 #define GX_INST(S)                                                                                                 \
   template int launch_dhop_epi<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,       \
@@ -1375,22 +1432,28 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
   template int launch_cg_dot<S>(Ctx *, int64_t, int64_t, const void *, const void *, CGState *);                   \
   template int launch_cg_axpy_norm<S>(Ctx *, int64_t, int64_t, void *, const void *, CGState *);                   \
   template int launch_cg_update<S>(Ctx *, int64_t, int64_t, void *, void *, const void *, CGState *);
-GX_INST(double)
-GX_INST(float)
-GX_INST(half_t)
 #define GX_INST_REAL(T)                                                                                            \
   template int launch_slice_proj<T>(const Geom &, const void *, int, int, void *, int, int, int, int, hipStream_t); \
   template int launch_mrhs_reduce<T>(Ctx *, int, const Geom &, void *, const void *, void *, MrhsState *, double *); \
   template int launch_mrhs_update<T>(Ctx *, const Geom &, void *, void *, const void *, MrhsState *);             \
   template int launch_gaussian<T>(const Geom &, void *, uint64_t, hipStream_t);
+#if GX_PART == 0
+GX_INST(double)
 GX_INST_REAL(double)
-GX_INST_REAL(float)
 template int launch_convert<double, float>(int64_t, const void *, void *, hipStream_t);
 template int launch_convert<float, double>(int64_t, const void *, void *, hipStream_t);
+#elif GX_PART == 1
+GX_INST(float)
+GX_INST_REAL(float)
+#else
+GX_INST(half_t)
+template int launch_mrhs_reduce<half_t>(Ctx *, int, const Geom &, void *, const void *, void *, MrhsState *, double *);
+template int launch_mrhs_update<half_t>(Ctx *, const Geom &, void *, void *, const void *, MrhsState *);
 template int launch_convert<double, half_t>(int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<double, half_t>(int64_t, int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<half_t, double>(int64_t, int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<float, half_t>(int64_t, int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<half_t, float>(int64_t, int64_t, const void *, void *, hipStream_t);
+#endif
 
 }  // namespace gx

@@ -199,7 +199,19 @@ static int cg_device_loop(Op *op, int prec, Field *psi, Field *p, Field *mmp, Fi
   return GX_OK;
 }
 
-int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *out) {
+// Options of one CG run inside the composite solvers (defect correction).  A guess the caller has
+// just zeroed skips the guess norm; an initial residual the caller already holds (r0 = src - A psi
+// with |r0|^2, per slice for multi-RHS) replaces A psi; inner solves skip the closing true-residual
+// check, whose only use is the statistics of a solve (the outer loop recomputes the FP64 residual).
+// The iterates are the ones the plain run produces: A 0 = 0 exactly and r0 is the same vector.
+struct CGOpts {
+  bool zero_guess = false;
+  const Field *r0 = nullptr;
+  const double *r0_norm = nullptr;
+  bool true_residual = true;
+};
+
+int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *out, const CGOpts &opt = CGOpts()) {
   auto t0 = std::chrono::steady_clock::now();
   const int prec = src->prec;
   GX_TRY(op_prepare(op, prec));
@@ -207,15 +219,19 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
   if (!p || !mmp || !r) return GX_ENOMEM;
   p->cb = mmp->cb = r->cb = psi->cb = src->cb;
   gx_solver_stats st{};
-  double ssq, guess, a;
+  double ssq, guess = 0.0, a;
   GX_TRY(blas_norm2(src, &ssq));
-  GX_TRY(blas_norm2(psi, &guess));
+  if (!opt.zero_guess && !opt.r0) GX_TRY(blas_norm2(psi, &guess));
   if (std::isnan(guess) || std::isnan(ssq)) {
     set_error("CG: NaN in source or guess");
     return GX_ENAN;
   }
   Ctx *ctx = op->ctx;
-  if (guess == 0.0) {
+  if (opt.r0) {
+    GX_CHECK_HIP(hipMemcpyAsync(r->data, opt.r0->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    GX_CHECK_HIP(hipMemcpyAsync(p->data, opt.r0->data, p->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    a = *opt.r0_norm;
+  } else if (guess == 0.0) {
     GX_CHECK_HIP(hipMemcpyAsync(r->data, src->data, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
     GX_CHECK_HIP(hipMemcpyAsync(p->data, src->data, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
     a = ssq;
@@ -265,12 +281,16 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
   st.converged = converged;
   st.computed_residual = std::sqrt(fin.cp / ssq);
   // true residual: |MpcDagMpc psi - src| / |src|   (ConjugateGradient.h:201-208)
-  GX_TRY(apply_mpcdagmpc(op, psi, mmp));
-  GX_TRY(blas_axpby(p, 1.0, mmp, -1.0, src));
-  double rn, sn;
-  GX_TRY(blas_norm2(p, &rn));
-  GX_TRY(blas_norm2(src, &sn));
-  st.true_residual = std::sqrt(rn) / std::sqrt(sn);
+  if (opt.true_residual) {
+    GX_TRY(apply_mpcdagmpc(op, psi, mmp));
+    GX_TRY(blas_axpby(p, 1.0, mmp, -1.0, src));
+    double rn, sn;
+    GX_TRY(blas_norm2(p, &rn));
+    GX_TRY(blas_norm2(src, &sn));
+    st.true_residual = std::sqrt(rn) / std::sqrt(sn);
+  } else {
+    st.true_residual = st.computed_residual;
+  }
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (out) *out = st;
   if (!converged) {
@@ -296,24 +316,47 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
   double inner_tol = inner_tol0;
   int outer = 0;
   const double OuterLoopNormMult = 100.0;
+  double sol_norm;
+  GX_TRY(blas_norm2(sol, &sol_norm));
+  bool sol_zero = sol_norm == 0.0;
+  Field *res = src_d;         // src - A sol of the latest check
+  double norm = 0;
+  bool fresh = false;         // res belongs to the current sol
   for (outer = 0; outer < maxouter; outer++) {
-    GX_TRY(apply_mpcdagmpc(op, sol, tmp_d));
-    double norm;
-    GX_TRY(blas_axpy_norm(src_d, -1.0, tmp_d, src_in, &norm));   // src_d = src - A sol
+    if (sol_zero) {
+      res = src_in;           // A 0 = 0
+      norm = src_norm;
+    } else {
+      GX_TRY(apply_mpcdagmpc(op, sol, tmp_d));
+      GX_TRY(blas_axpy_norm(src_d, -1.0, tmp_d, src_in, &norm));   // src_d = src - A sol
+      res = src_d;
+    }
+    fresh = true;
     if (norm < OuterLoopNormMult * stop) break;
     while (norm * inner_tol * inner_tol < stop) inner_tol *= 2;
-    GX_TRY(convert_field(src_f, src_d));
+    GX_TRY(convert_field(src_f, res));
     GX_CHECK_HIP(hipMemsetAsync(sol_f->data, 0, sol_f->bytes, op->ctx->stream));
     gx_solver_stats ist{};
-    int rc = cg_run(op, src_f, sol_f, inner_tol, maxinner, &ist);
+    CGOpts io;
+    io.zero_guess = true;
+    io.true_residual = false;
+    int rc = cg_run(op, src_f, sol_f, inner_tol, maxinner, &ist, io);
     if (rc != GX_OK && rc != GX_ENOCONV) return rc;
     st.inner_iterations += ist.iterations;
     GX_TRY(convert_field(tmp_d, sol_f));
     GX_TRY(blas_axpby(sol, 1.0, sol, 1.0, tmp_d));
+    sol_zero = false;
+    fresh = false;
   }
   st.outer_iterations = outer;
   gx_solver_stats fst{};
-  const int rc = cg_run(op, src_in, sol, tol, maxinner, &fst);
+  // the final FP64 solve starts from the residual of the last check when it is current
+  CGOpts fo;
+  if (fresh) {
+    fo.r0 = res;
+    fo.r0_norm = &norm;
+  }
+  const int rc = cg_run(op, src_in, sol, tol, maxinner, &fst, fo);
   st.final_iterations = fst.iterations;
   st.iterations = st.inner_iterations + fst.iterations;
   st.true_residual = fst.true_residual;
@@ -431,11 +474,10 @@ static int slice_reduce(Op *op, int mode, Field *x, Field *y, Field *z, double *
   GX_TRY(solver_work(op, &sw));
   MrhsWork &w = sw->mrhs;
   Ctx *ctx = op->ctx;
-  const int rc = x->prec == GX_F64
-                     ? launch_mrhs_reduce<double>(ctx, mode, op->g, x->data, y ? y->data : nullptr,
-                                                  z ? z->data : nullptr, w.dev, w.out_dev)
-                     : launch_mrhs_reduce<float>(ctx, mode, op->g, x->data, y ? y->data : nullptr,
-                                                 z ? z->data : nullptr, w.dev, w.out_dev);
+  const int rc = by_prec(x->prec, [&](auto tag) {
+    return launch_mrhs_reduce<decltype(tag)>(ctx, mode, op->g, x->data, y ? y->data : nullptr, z ? z->data : nullptr,
+                                             w.dev, w.out_dev);
+  });
   GX_TRY(rc);
   GX_CHECK_HIP(hipMemcpyAsync(w.out_host, w.out_dev, sizeof(double) * op->g.Ls, hipMemcpyDeviceToHost, ctx->stream));
   GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -454,29 +496,46 @@ static int mrhs_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, Mr
 }
 
 // CGmultiRHS on HermOp = MpcDagMpc with right-hand side k in slice s = k.  st: Ls entries or null.
-int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *st) {
+int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stats *st,
+                const CGOpts &opt = CGOpts()) {
   auto t0 = std::chrono::steady_clock::now();
   const int Ls = op->g.Ls, prec = src->prec;
   GX_TRY(op_prepare(op, prec));
   Field *p = op_tmp(op, prec, 4), *mmp = op_tmp(op, prec, 5), *r = op_tmp(op, prec, 6);
   if (!p || !mmp || !r) return GX_ENOMEM;
   p->cb = mmp->cb = r->cb = psi->cb = src->cb;
-  std::vector<double> ssq(Ls), rr(Ls);
+  std::vector<double> ssq(Ls), rr(Ls), gn(Ls, 0.0);
   GX_TRY(slice_reduce(op, 1, src, nullptr, nullptr, ssq.data()));
   for (int s = 0; s < Ls; s++)
     if (std::isnan(ssq[s])) {
       set_error("block CG: NaN in source slice %d", s);
       return GX_ENAN;
     }
-  // initial search direction from the guess: AP = A psi, R = src - AP, P = R
-  GX_TRY(apply_mpcdagmpc(op, psi, mmp));
-  GX_TRY(slice_reduce(op, 3, mmp, src, r, rr.data()));
+  Ctx *ctx = op->ctx;
+  bool zero = opt.zero_guess;
+  if (!zero && !opt.r0) {
+    GX_TRY(slice_reduce(op, 1, psi, nullptr, nullptr, gn.data()));
+    zero = true;
+    for (int s = 0; s < Ls; s++) zero = zero && gn[s] == 0.0;
+  }
+  if (opt.r0) {
+    // the caller's residual of this guess
+    GX_CHECK_HIP(hipMemcpyAsync(r->data, opt.r0->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    for (int s = 0; s < Ls; s++) rr[s] = opt.r0_norm[s];
+  } else if (zero) {
+    // A 0 = 0: R = src
+    GX_CHECK_HIP(hipMemcpyAsync(r->data, src->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    rr = ssq;
+  } else {
+    // initial search direction from the guess: AP = A psi, R = src - AP, P = R
+    GX_TRY(apply_mpcdagmpc(op, psi, mmp));
+    GX_TRY(slice_reduce(op, 3, mmp, src, r, rr.data()));
+  }
   for (int s = 0; s < Ls; s++)
     if (std::isnan(rr[s])) {
       set_error("block CG: NaN in guess slice %d", s);
       return GX_ENAN;
     }
-  Ctx *ctx = op->ctx;
   GX_CHECK_HIP(hipMemcpyAsync(p->data, r->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
   SolverWork *sw = nullptr;
   GX_TRY(solver_work(op, &sw));
@@ -511,8 +570,7 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
       if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
     auto one_batch = [&]() -> int {
       for (int i = 0; i < batch; i++) {
-        int rc = prec == GX_F64 ? mrhs_iteration<double>(op, psi, p, mmp, r, w.dev)
-                                : mrhs_iteration<float>(op, psi, p, mmp, r, w.dev);
+        int rc = by_prec(prec, [&](auto tag) { return mrhs_iteration<decltype(tag)>(op, psi, p, mmp, r, w.dev); });
         GX_TRY(rc);
       }
       return GX_OK;
@@ -525,8 +583,7 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
         enqueued += batch;
       } else {
         for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
-          int rc = prec == GX_F64 ? mrhs_iteration<double>(op, psi, p, mmp, r, w.dev)
-                                  : mrhs_iteration<float>(op, psi, p, mmp, r, w.dev);
+          int rc = by_prec(prec, [&](auto tag) { return mrhs_iteration<decltype(tag)>(op, psi, p, mmp, r, w.dev); });
           GX_TRY(rc);
         }
       }
@@ -552,10 +609,14 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
     converged = fin.k_conv != INT_MAX;
     iterations = converged ? fin.k_conv : maxit;
   }
-  // true residual per right-hand side
+  // true residual per right-hand side (inner solves report the recursive one)
   std::vector<double> res(Ls);
-  GX_TRY(apply_mpcdagmpc(op, psi, mmp));
-  GX_TRY(slice_reduce(op, 3, src, mmp, p, res.data()));
+  if (opt.true_residual) {
+    GX_TRY(apply_mpcdagmpc(op, psi, mmp));
+    GX_TRY(slice_reduce(op, 3, src, mmp, p, res.data()));
+  } else {
+    for (int s = 0; s < Ls; s++) res[s] = fin.cp[s];
+  }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st)
     for (int s = 0; s < Ls; s++) {
@@ -574,9 +635,10 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
 }
 
 // Batched defect correction (ConjugateGradientMixedPrecBatched.h:100-213) with the inner solves of
-// all right-hand sides done together by one FP32 multi-RHS CG, so they share every link load.
-int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int maxinner, int maxouter,
-                   gx_solver_stats *st) {
+// all right-hand sides done together by one multi-RHS CG in FP32 or FP16 storage (inner_prec), so
+// they share every link load.
+int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int inner_prec, int maxinner,
+                   int maxouter, gx_solver_stats *st) {
   auto t0 = std::chrono::steady_clock::now();
   const int Ls = op->g.Ls, cb = src_in->cb;
   sol->cb = cb;
@@ -584,14 +646,27 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
   GX_TRY(slice_reduce(op, 1, src_in, nullptr, nullptr, src_norm.data()));
   for (int s = 0; s < Ls; s++) stop[s] = src_norm[s] * tol * tol;
   Field *tmp_d = op_tmp(op, GX_F64, 7), *src_d = op_tmp(op, GX_F64, 8);
-  Field *src_f = op_tmp(op, GX_F32, 7), *sol_f = op_tmp(op, GX_F32, 8);
+  Field *src_f = op_tmp(op, inner_prec, 7), *sol_f = op_tmp(op, inner_prec, 8);
   if (!tmp_d || !src_d || !src_f || !sol_f) return GX_ENOMEM;
   tmp_d->cb = src_d->cb = src_f->cb = sol_f->cb = cb;
   double inner_tol = inner_tol0;
   int outer = 0, inner_total = 0;
+  std::vector<double> sn(Ls);
+  GX_TRY(slice_reduce(op, 1, sol, nullptr, nullptr, sn.data()));
+  bool sol_zero = true;
+  for (int s = 0; s < Ls; s++) sol_zero = sol_zero && sn[s] == 0.0;
+  Field *res = src_d;   // src - A sol of the latest check
+  bool fresh = false;   // res belongs to the current sol
   for (outer = 0; outer < maxouter; outer++) {
-    GX_TRY(apply_mpcdagmpc(op, sol, tmp_d));
-    GX_TRY(slice_reduce(op, 3, tmp_d, src_in, src_d, norm.data()));   // src_d = src - A sol
+    if (sol_zero) {
+      res = src_in;     // A 0 = 0
+      norm = src_norm;
+    } else {
+      GX_TRY(apply_mpcdagmpc(op, sol, tmp_d));
+      GX_TRY(slice_reduce(op, 3, tmp_d, src_in, src_d, norm.data()));   // src_d = src - A sol
+      res = src_d;
+    }
+    fresh = true;
     bool all = true;
     double nmax = 0, smax = 0;
     for (int s = 0; s < Ls; s++) {
@@ -601,17 +676,28 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
     }
     if (all) break;
     while (nmax * inner_tol * inner_tol < smax) inner_tol *= 2;
-    GX_TRY(convert_field(src_f, src_d));
+    GX_TRY(convert_field(src_f, res));
     GX_CHECK_HIP(hipMemsetAsync(sol_f->data, 0, sol_f->bytes, op->ctx->stream));
     std::vector<gx_solver_stats> ist(Ls);
-    int rc = mrhs_cg_run(op, src_f, sol_f, inner_tol, maxinner, ist.data());
+    CGOpts io;
+    io.zero_guess = true;
+    io.true_residual = false;
+    int rc = mrhs_cg_run(op, src_f, sol_f, inner_tol, maxinner, ist.data(), io);
     if (rc != GX_OK && rc != GX_ENOCONV) return rc;
     inner_total += ist[0].iterations;
     GX_TRY(convert_field(tmp_d, sol_f));
     GX_TRY(blas_axpby(sol, 1.0, sol, 1.0, tmp_d));
+    sol_zero = false;
+    fresh = false;
   }
   std::vector<gx_solver_stats> fst(Ls);
-  const int rc = mrhs_cg_run(op, src_in, sol, tol, maxinner, fst.data());
+  // the final FP64 solve starts from the residual of the last check when it is current
+  CGOpts fo;
+  if (fresh) {
+    fo.r0 = res;
+    fo.r0_norm = norm.data();
+  }
+  const int rc = mrhs_cg_run(op, src_in, sol, tol, maxinner, fst.data(), fo);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st)
     for (int s = 0; s < Ls; s++) {
@@ -687,7 +773,7 @@ int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, 
     return GX_EINVAL;
   }
   if (inner_prec == GX_F64) return mrhs_cg_run(op, src, psi, tol, maxit, st);
-  if (inner_prec == GX_F32) return mrhs_mixed_run(op, src, psi, tol, 1e-5, maxit, 100, st);
+  if (inner_prec == GX_F32 || inner_prec == GX_F16) return mrhs_mixed_run(op, src, psi, tol, 1e-5, inner_prec, maxit, 100, st);
   set_error("block_cg: inner precision %d unsupported", inner_prec);
   return GX_EINVAL;
 }
@@ -707,8 +793,8 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
   if (!se || !so || !t || !Mt || !solo) return GX_ENOMEM;
   const bool multi = op->kind == GX_WILSON && op->g.Ls > 1;
   const int Ls = op->g.Ls;
-  if (mixed < 0 || mixed > 2 || (multi && mixed == 2)) {
-    set_error("schur_solve: mixed must be 0 (FP64), 1 (FP32 inner) or 2 (FP16 inner, single-RHS operators)");
+  if (mixed < 0 || mixed > 2) {
+    set_error("schur_solve: mixed must be 0 (FP64), 1 (FP32 inner) or 2 (FP16 inner)");
     return GX_EINVAL;
   }
   hipStream_t s = op->ctx->stream;
@@ -731,7 +817,7 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
     if (multi) {
       // Ls independent right-hand sides: every one to tol (CGmultiRHS)
       std::vector<gx_solver_stats> bst(Ls);
-      rc = mixed ? mrhs_mixed_run(op, so, solo, tol, 1e-5, maxit, 100, bst.data())
+      rc = mixed ? mrhs_mixed_run(op, so, solo, tol, 1e-5, mixed == 2 ? GX_F16 : GX_F32, maxit, 100, bst.data())
                  : mrhs_cg_run(op, so, solo, tol, maxit, bst.data());
       if (st) {
         *st = bst[0];

@@ -191,7 +191,8 @@ int gx_reliable_update_cg(gx_op op, gx_field src, gx_field psi, double tol, int 
  * src, sol: full F64 fields.  mixed = 1 (2) runs the red-black solve as gx_mixed_cg with FP32
  * (FP16-storage) inner solves, mixed = 0 as FP64 gx_cg.
  * unprec_residual = |M sol - src| / |src|.  On a multi-RHS Wilson operator (Ls > 1) the
- * red-black solve is gx_block_cg (every right-hand side to tol), st holds the largest iteration
+ * red-black solve is gx_block_cg with inner_prec F64, F32 or F16 for mixed = 0, 1, 2 (every
+ * right-hand side to tol), st holds the largest iteration
  * count and residuals over the right-hand sides, and unprec_residual is the largest
  * per-right-hand-side |M sol_k - src_k| / |src_k|. */
 int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, int mixed,
@@ -201,8 +202,9 @@ int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, 
  * BlockConjugateGradient.h:412-545: CG with per-right-hand-side alpha and beta on a multi-RHS
  * Wilson operator (GX_WILSON with Ls = Nrhs; right-hand side k is slice s = k), iterating every
  * right-hand side until the largest relative residual is below tol.  inner_prec = GX_F64 runs it
- * in FP64; GX_F32 runs FP32 multi-RHS inner solves inside FP64 defect correction
- * (ConjugateGradientMixedPrecBatched.h:100-213, inner tolerance 1e-5) and an FP64 patch-up.
+ * in FP64; GX_F32 (GX_F16) runs FP32 (FP16-storage) multi-RHS inner solves inside FP64 defect
+ * correction (ConjugateGradientMixedPrecBatched.h:100-213, inner tolerance 1e-5) and an FP64
+ * patch-up.
  * src, psi: F64 half fields; st: array of Ls statistics (one per right-hand side). */
 int gx_block_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, int inner_prec,
                 gx_solver_stats *st);

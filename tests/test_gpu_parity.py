@@ -328,9 +328,10 @@ def test_graph_follows_mass_and_gauge(ctx, kind, Ls):
     assert not np.array_equal(got_m, got_u)
 
 
-@pytest.mark.parametrize('inner', [gx.F64, gx.F32])
+@pytest.mark.parametrize('inner', [gx.F64, gx.F32, gx.F16])
 def test_block_cg_matches_single_rhs(ctx, inner):
-    """CGmultiRHS: every right-hand side agrees with the oracle's single-RHS FP64 CG."""
+    """CGmultiRHS: every right-hand side agrees with the oracle's single-RHS FP64 CG (FP64, or FP32
+    and FP16-storage inner solves under FP64 defect correction)."""
     dims, nrhs, mass = (8, 8, 4, 8), 4, 0.1
     U = gx.random_gauge(dims, 61, 0.5)
     src = half(gx.gaussian_fermion(dims, nrhs, 62), dims, ODD)
@@ -343,6 +344,43 @@ def test_block_cg_matches_single_rhs(ctx, inner):
         ref, rst = O1.cg(np.ascontiguousarray(src[..., k:k + 1, :, :]), tol=1e-10)
         assert sts[k]['converged'] and sts[k]['true_residual'] < 1e-9, sts[k]
         assert rel_err(x[..., k:k + 1, :, :], ref) < 1e-8
+
+
+@pytest.mark.parametrize('inner', [gx.F64, gx.F32])
+def test_block_cg_from_guess(ctx, inner):
+    """A nonzero starting guess enters through its residual src - A psi: the solves reach the
+    oracle's solutions (which start from zero)."""
+    dims, nrhs, mass = (8, 8, 4, 8), 3, 0.1
+    U = gx.random_gauge(dims, 63, 0.5)
+    src = half(gx.gaussian_fermion(dims, nrhs, 64), dims, ODD)
+    guess = 0.3 * half(gx.gaussian_fermion(dims, nrhs, 65), dims, ODD)
+    D = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+    fs, fp = D.field(gx.F64, ODD).upload(src), D.field(gx.F64, ODD).upload(guess)
+    sts = D.block_cg(fs, fp, tol=1e-10, inner_prec=inner)
+    x = fp.download()
+    O1 = Oracle(U, dims, 1, 'wilson', mass=mass)
+    for k in range(nrhs):
+        ref, _ = O1.cg(np.ascontiguousarray(src[..., k:k + 1, :, :]), tol=1e-10)
+        assert sts[k]['converged'] and sts[k]['true_residual'] < 1e-9, sts[k]
+        assert rel_err(x[..., k:k + 1, :, :], ref) < 1e-8
+
+
+@pytest.mark.parametrize('inner', [gx.F32, gx.F16])
+def test_mixed_cg_from_guess(ctx, inner):
+    """Defect correction from a nonzero guess, and from the zero guess, reaches the oracle's CG
+    solution; the final FP64 solve starts from the last outer residual."""
+    dims, mass = (8, 8, 8, 8), 0.1
+    U = gx.random_gauge(dims, 66, 0.5)
+    src = half(gx.gaussian_fermion(dims, 1, 67), dims, ODD)
+    ref, _ = Oracle(U, dims, 1, 'wilson', mass=mass).cg(src, tol=1e-10)
+    D = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=mass)
+    fs = D.field(gx.F64, ODD).upload(src)
+    for guess in (np.zeros_like(src), 0.3 * half(gx.gaussian_fermion(dims, 1, 68), dims, ODD)):
+        fx = D.field(gx.F64, ODD).upload(guess)
+        st = D.mixed_cg(fs, fx, tol=1e-10, inner_tol=1e-5, inner_prec=inner)
+        assert st['converged'] and st['true_residual'] < 1e-9, st
+        assert st['outer_iterations'] >= 1 and st['inner_iterations'] > 0
+        assert rel_err(fx.download(), ref) < 1e-7
 
 
 @pytest.mark.parametrize('kind,dims,Ls,prec', [('mobius', (8, 8, 8, 8), 16, gx.F64), ('wilson', (8, 4, 8, 8), 12, gx.F32),

@@ -67,14 +67,15 @@ def test_clone_matches_fresh_operator(ctx):
 
 
 def test_multi_rhs_schur_solve(ctx):
-    """Red-black solve of a multi-RHS Wilson operator: every right-hand side agrees with the
-    oracle's single-RHS solve and the reported residual is the worst per-RHS one."""
+    """Red-black solve of a multi-RHS Wilson operator (FP64, FP32 and FP16-storage inner solves):
+    every right-hand side agrees with the oracle's single-RHS solve and the reported residual is the
+    worst per-RHS one."""
     dims, nrhs, mass = (8, 8, 4, 4), 3, 0.12
     U = gx.random_gauge(dims, 41, 0.4)
     src = gx.gaussian_fermion(dims, nrhs, 42)
     D = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
     O = Oracle(U, dims, 1, 'wilson', mass=mass)
-    for mixed in (False, True):
+    for mixed in (0, 1, 2):
         fs, fx = D.field().upload(src), D.field()
         st = D.schur_solve(fs, fx, tol=1e-10, mixed=mixed)
         x = fx.download()
