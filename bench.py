@@ -8,11 +8,13 @@ value = algorithmic (compulsory) bytes per call / average kernel time, with
 bytes = H (24 Ls + 72) c   (H = V4/2 half sites, c = bytes per complex): each input spinor and
 each link read once, each output spinor written once.  Kernel time comes from HIP events on the
 stream the kernels are launched on.  Also reported: GF/s at 1320 flop per 5D site, the FP32
-figure, Schur CG time to 1e-10 in FP64 and mixed FP32/FP64, and the CPU oracle's DhopEO on the
-same workload (cpu_baseline).
+figure, Schur CG time-to-solution in FP64 and mixed FP32/FP64, and the CPU oracle's DhopEO on the
+same workload (cpu_baseline).  Every solve runs its red-black CG to 1e-11 (Wilson) or 1e-12
+(Mobius) on the normal equations so that the full-system residual |b - M x| / |b| (reported as
+unprec_residual, with meets_target) is <= 1e-10.
 
 Further legs, reported beside the headline value (each can be switched off):
-  c3     Mobius 24^4 x Ls12 (mf 0.005): red-black solve to 1e-10 in FP64 CG and in defect
+  c3     Mobius 24^4 x Ls12 (mf 0.005): red-black solve in FP64 CG and in defect
          correction with FP32 and FP16-storage inner solves;
   c4     Wilson 32^3x64 (m 0.05), the 12 spin-colour point sources of one site: one mixed-precision
          multi-RHS solve against 12 sequential FP64 solves;
@@ -41,6 +43,14 @@ PARAMS = dict(mass=0.01, M5=1.8, b=1.5, c=0.5)
 EPS = 0.3
 METRIC = 'Dslash GB/s (% HBM peak); CG time-to-1e-10 mixed vs FP64; solve p50/p99'
 DHOP_KERNEL = 'gx::k_dhop<double, 16, false, true, 0, 0>'
+# Every solve must reach |b - M x| / |b| <= 1e-10 on the full (unpreconditioned) system.  The
+# red-black CG's tolerance applies to the normal equations of the Schur system, whose residual
+# runs below the full-system one by a factor the operator sets: 1.5-2x for these Wilson systems,
+# 10-50x for the light-quark Mobius ones.  The solves use 1e-11 (Wilson) and 1e-12 (Mobius), the
+# same for the FP64 baseline and the mixed solves, and report the full residual.
+CG_TOL = 1e-11
+CG_TOL_MOBIUS = 1e-12
+TARGET_RESIDUAL = 1e-10
 
 
 def pick(a, dims, cb):
@@ -93,17 +103,19 @@ def cpu_baseline(U, psi_o, seconds=12.0):
     return el / n, n, ncores
 
 
-def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10, tune=None):
+def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tune=None):
     """Mobius 24^4 x Ls12 (b=1.5, c=0.5, M5=1.8, mf=0.005, eps 0.3), Gaussian 5D source: red-black
-    solve to 1e-10 in FP64 CG and in defect correction with FP32 and with FP16-storage inner solves
-    (inner tolerance 1e-5, FP64 outer residual and patch-up).  Wall clock around each solve."""
+    solve in FP64 CG and in defect correction with FP32 and with FP16-storage inner solves (inner
+    tolerance 1e-5, FP64 outer residual and patch-up), each to |b - M x| / |b| <= 1e-10.  Wall
+    clock around each solve."""
     kw = dict(mass=0.005, M5=1.8, b=1.5, c=0.5)
     U = gx.random_gauge(dims, 1003, 0.3)
     D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
     if tune:
         tune('c3', D, gx.F32, rounds=2, cg_iters=32)
     src = D.field().gaussian(2003)
-    out, ref = {'workload': 'Mobius 24^4 x Ls12 mf=0.005, Gaussian source'}, None
+    out, ref = {'workload': 'Mobius 24^4 x Ls12 mf=0.005, Gaussian source', 'cg_tol': tol,
+                'target_unprec_residual': TARGET_RESIDUAL}, None
     for name, mixed in (('fp64', 0), ('mixed_f32', 1), ('mixed_f16', 2)):
         x = D.field()
         ctx.sync()
@@ -113,7 +125,8 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10, tune=None)
         sec = time.perf_counter() - t0
         d = dict(seconds=round(sec, 4), iterations=st['iterations'], inner_iterations=st['inner_iterations'],
                  outer=st['outer_iterations'], final_iterations=st['final_iterations'],
-                 true_residual=st['true_residual'], unprec_residual=st['unprec_residual'])
+                 true_residual=st['true_residual'], unprec_residual=st['unprec_residual'],
+                 meets_target=bool(st['unprec_residual'] <= TARGET_RESIDUAL))
         if ref is None:
             ref = x
         else:
@@ -125,11 +138,12 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=1e-10, tune=None)
     return out
 
 
-def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-10, tune=None):
+def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=CG_TOL, tune=None):
     """Wilson 32^3x64 (eps 0.3, m 0.05), the 12 spin-colour point sources of one seeded site:
     mixed-precision multi-RHS red-black solves (CGmultiRHS inner solves in FP32, and in FP16
     storage, sharing every link load across the 12 sources, FP64 defect correction and patch-up)
-    against 12 sequential FP64 red-black CG solves, all to 1e-10.  Wall clock around each solve;
+    against 12 sequential FP64 red-black CG solves, all to |b - M x| / |b| <= 1e-10 per source.
+    Wall clock around each solve;
     'speedup' is the faster mixed variant's (named in 'speedup_variant')."""
     nrhs = 12
     U = gx.random_gauge(dims, 1004, eps)
@@ -139,7 +153,7 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-1
     if tune:
         tune('c4_sequential', base, gx.F64, rounds=2, cg_iters=32)
     s1 = base.field()
-    seq, seq_sec, seq_it = [], 0.0, []
+    seq, seq_sec, seq_it, seq_res = [], 0.0, [], 0.0
     for k in range(nrhs):
         s1.point_source(x0, k // 3, k % 3)
         x = base.field()
@@ -149,13 +163,16 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-1
         ctx.sync()
         seq_sec += time.perf_counter() - t0
         seq_it.append(st['iterations'])
+        seq_res = max(seq_res, st['unprec_residual'])
         seq.append(x)
     B = base.clone(ctx, nrhs, mass)
     sb = B.field().point_source(x0)
     t4 = base.field()
     out = {'workload': 'Wilson 32^3x64 m=0.05, 12 point sources', 'origin': list(x0),
+           'cg_tol': tol, 'target_unprec_residual': TARGET_RESIDUAL,
            'sequential_fp64': {'seconds': round(seq_sec, 4), 'per_rhs_seconds': round(seq_sec / nrhs, 4),
-                               'iterations': seq_it}}
+                               'iterations': seq_it, 'max_unprec_residual': seq_res,
+                               'meets_target': bool(seq_res <= TARGET_RESIDUAL)}}
     best = None
     for name, mixed, prec, tkey in (('multi_rhs_mixed', 1, gx.F32, 'c4'), ('multi_rhs_mixed_f16', 2, gx.F16, 'c4_f16')):
         if tune:
@@ -176,7 +193,9 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-1
                      'seconds': round(blk_sec, 4), 'per_rhs_seconds': round(blk_sec / nrhs, 4),
                      'inner_iterations': stb['inner_iterations'], 'outer': stb['outer_iterations'],
                      'final_iterations': stb['final_iterations'], 'true_residual': stb['true_residual'],
-                     'unprec_residual': stb['unprec_residual'], 'speedup': round(seq_sec / blk_sec, 3),
+                     'unprec_residual': stb['unprec_residual'],
+                     'meets_target': bool(stb['unprec_residual'] <= TARGET_RESIDUAL),
+                     'speedup': round(seq_sec / blk_sec, 3),
                      'max_rel_diff_vs_sequential': worst}
         if best is None or blk_sec < out[best]['seconds']:
             best = name
@@ -189,10 +208,11 @@ def bench_serving(n, rate, slots=4):
     """Poisson trace of n Wilson solve requests at `rate` per second through the serving runtime."""
     from grid_amd import serving
     trace = serving.make_trace(n, rate, seed=5000)
-    res, info = serving.serve(trace, slots=slots, budget=128e9, n_gauge=8, gauge_eps=0.3, tol=1e-10, mixed=True)
+    res, info = serving.serve(trace, slots=slots, budget=128e9, n_gauge=8, gauge_eps=0.3, tol=CG_TOL, mixed=True)
     s = serving.summarize(res, info)
+    s['meets_target'] = bool(s['max_residual'] <= TARGET_RESIDUAL)
     s.update({'arrival_rate_per_s': rate, 'slots': slots, 'admission_budget_bytes': 128e9,
-              'mix': 'volumes 8^4,12^4,16^4,16^3x32,24^4; 1-12 Gaussian sources; m in [0.05,0.3]; mixed CG to 1e-10'})
+              'mix': 'volumes 8^4,12^4,16^4,16^3x32,24^4; 1-12 Gaussian sources; m in [0.05,0.3]; mixed CG to 1e-11 (|b - M x|/|b| <= 1e-10)'})
     return {k: (float(f'{v:.6g}') if isinstance(v, float) else v) for k, v in s.items()}
 
 
@@ -278,16 +298,19 @@ def main():
         tune('cg', D, gx.F32)
         src_full = gx.gaussian_fermion(DIMS, LS, 3003)
         fs, fsol = D.field().upload(src_full), D.field()
-        st64 = D.schur_solve(fs, fsol, tol=1e-10, maxit=20000, mixed=False)
+        st64 = D.schur_solve(fs, fsol, tol=CG_TOL_MOBIUS, maxit=20000, mixed=False)
         x64 = fsol.download()
         fsol.zero()
-        stmx = D.schur_solve(fs, fsol, tol=1e-10, maxit=20000, mixed=True)
+        stmx = D.schur_solve(fs, fsol, tol=CG_TOL_MOBIUS, maxit=20000, mixed=True)
         xmx = fsol.download()
-        cg = dict(fp64=dict(seconds=st64['seconds'], iterations=st64['iterations'], true_residual=st64['true_residual'],
-                            unprec_residual=st64['unprec_residual']),
+        cg = dict(cg_tol=CG_TOL_MOBIUS,
+                  fp64=dict(seconds=st64['seconds'], iterations=st64['iterations'], true_residual=st64['true_residual'],
+                            unprec_residual=st64['unprec_residual'],
+                            meets_target=bool(st64['unprec_residual'] <= TARGET_RESIDUAL)),
                   mixed_f32=dict(seconds=stmx['seconds'], inner_iterations=stmx['inner_iterations'],
                                  outer=stmx['outer_iterations'], final_iterations=stmx['final_iterations'],
                                  true_residual=stmx['true_residual'], unprec_residual=stmx['unprec_residual'],
+                                 meets_target=bool(stmx['unprec_residual'] <= TARGET_RESIDUAL),
                                  rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))),
                   speedup_mixed_vs_fp64=st64['seconds'] / stmx['seconds'])
 
