@@ -224,48 +224,39 @@ __device__ __forceinline__ void project(const cx<T> *f, cx<T> *h) {
   }
 }
 
-// accumulate the reconstruction of a projected 2-spinor, TwoSpinor.h:193-323
+// accumulate the reconstruction of colour c of a projected 2-spinor whose spin rows are h0, h1,
+// TwoSpinor.h:193-323
 template <typename T, int MU, bool PLUS>
-__device__ __forceinline__ void accum_recon(cx<T> *a, const cx<T> *g) {
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const cx<T> h0 = g[c], h1 = g[3 + c];
-    a[c] = cadd(a[c], h0);
-    a[3 + c] = cadd(a[3 + c], h1);
-    if (MU == 0) {
-      if (PLUS) { a[6 + c] = csubi(a[6 + c], h1); a[9 + c] = csubi(a[9 + c], h0); }
-      else      { a[6 + c] = caddi(a[6 + c], h1); a[9 + c] = caddi(a[9 + c], h0); }
-    } else if (MU == 1) {
-      if (PLUS) { a[6 + c] = cadd(a[6 + c], h1); a[9 + c] = csub(a[9 + c], h0); }
-      else      { a[6 + c] = csub(a[6 + c], h1); a[9 + c] = cadd(a[9 + c], h0); }
-    } else if (MU == 2) {
-      if (PLUS) { a[6 + c] = csubi(a[6 + c], h0); a[9 + c] = caddi(a[9 + c], h1); }
-      else      { a[6 + c] = caddi(a[6 + c], h0); a[9 + c] = csubi(a[9 + c], h1); }
-    } else {
-      if (PLUS) { a[6 + c] = cadd(a[6 + c], h0); a[9 + c] = cadd(a[9 + c], h1); }
-      else      { a[6 + c] = csub(a[6 + c], h0); a[9 + c] = csub(a[9 + c], h1); }
-    }
+__device__ __forceinline__ void recon_colour(cx<T> *a, int c, cx<T> h0, cx<T> h1) {
+  a[c] = cadd(a[c], h0);
+  a[3 + c] = cadd(a[3 + c], h1);
+  if (MU == 0) {
+    if (PLUS) { a[6 + c] = csubi(a[6 + c], h1); a[9 + c] = csubi(a[9 + c], h0); }
+    else      { a[6 + c] = caddi(a[6 + c], h1); a[9 + c] = caddi(a[9 + c], h0); }
+  } else if (MU == 1) {
+    if (PLUS) { a[6 + c] = cadd(a[6 + c], h1); a[9 + c] = csub(a[9 + c], h0); }
+    else      { a[6 + c] = csub(a[6 + c], h1); a[9 + c] = cadd(a[9 + c], h0); }
+  } else if (MU == 2) {
+    if (PLUS) { a[6 + c] = csubi(a[6 + c], h0); a[9 + c] = caddi(a[9 + c], h1); }
+    else      { a[6 + c] = caddi(a[6 + c], h0); a[9 + c] = csubi(a[9 + c], h1); }
+  } else {
+    if (PLUS) { a[6 + c] = cadd(a[6 + c], h0); a[9 + c] = cadd(a[9 + c], h1); }
+    else      { a[6 + c] = csub(a[6 + c], h0); a[9 + c] = csub(a[9 + c], h1); }
   }
 }
 
-// g = U h for both spin rows (SU(3) x colour vector, twice)
+// row r of U times the colour vector h (one spin row of the projected half-spinor)
 template <typename T>
-__device__ __forceinline__ void su3_mul(const cx<T> *u, const cx<T> *h, cx<T> *g) {
+__device__ __forceinline__ cx<T> su3_row(const cx<T> *u, const cx<T> *h) {
+  T re = 0, im = 0;
 #pragma unroll
-  for (int sp = 0; sp < 2; sp++)
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      T re = 0, im = 0;
-#pragma unroll
-      for (int b = 0; b < 3; b++) {
-        const cx<T> uu = u[a * 3 + b], hh = h[sp * 3 + b];
-        re = fma(uu.r, hh.r, re);
-        re = fma(-uu.i, hh.i, re);
-        im = fma(uu.r, hh.i, im);
-        im = fma(uu.i, hh.r, im);
-      }
-      g[sp * 3 + a] = {re, im};
-    }
+  for (int b = 0; b < 3; b++) {
+    re = fma(u[b].r, h[b].r, re);
+    re = fma(-u[b].i, h[b].i, re);
+    im = fma(u[b].r, h[b].i, im);
+    im = fma(u[b].i, h[b].r, im);
+  }
+  return {re, im};
 }
 
 // Links are staged per workgroup in LDS (one coalesced load, row stride kLinkPad complex per
@@ -293,22 +284,20 @@ __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nblocks, 
   return b;
 }
 
+// One leg on its neighbour spinor f: project to the half-spinor, multiply by the link and
+// reconstruct, one colour row at a time (three link entries and two results live at once).
 template <typename T, typename S, int MU, bool PLUS, bool LDSU>
 __device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<S> *__restrict__ U,
                                          int P, uint32_t h, uint32_t H4) {
   cx<T> hs[6];
   project<T, MU, PLUS>(f, hs);
-  cx<T> u[9];
-  if (LDSU) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = ul[P * 9 + k];
-  } else {
+  for (int r = 0; r < 3; r++) {
+    cx<T> u[3];
 #pragma unroll
-    for (int k = 0; k < 9; k++) u[k] = ldc<S>(U, (P * 9 + k) * H4 + h);
+    for (int b = 0; b < 3; b++) u[b] = LDSU ? ul[P * 9 + r * 3 + b] : ldc<S>(U, (P * 9 + r * 3 + b) * H4 + h);
+    recon_colour<T, MU, PLUS>(acc, r, su3_row<T>(u, hs), su3_row<T>(u, hs + 3));
   }
-  cx<T> gs[6];
-  su3_mul<T>(u, hs, gs);
-  accum_recon<T, MU, PLUS>(acc, gs);
 }
 
 // One leg: LEG-th entry of the leg sequence.  ORD = 0 is Grid's order (DaggerNo: Xm,Ym,Zm,Tm with
