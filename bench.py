@@ -204,14 +204,17 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=CG_T
     return out
 
 
-def bench_serving(n, rate, slots=4):
-    """Poisson trace of n Wilson solve requests at `rate` per second through the serving runtime."""
+def bench_serving(n, rate, slots=4, mixed=1):
+    """Poisson trace of n Wilson solve requests at `rate` per second through the serving runtime
+    (mixed = 1: FP32 inner solves, 2: FP16-storage inner solves)."""
     from grid_amd import serving
     trace = serving.make_trace(n, rate, seed=5000)
-    res, info = serving.serve(trace, slots=slots, budget=128e9, n_gauge=8, gauge_eps=0.3, tol=CG_TOL, mixed=True)
+    res, info = serving.serve(trace, slots=slots, budget=128e9, n_gauge=8, gauge_eps=0.3, tol=CG_TOL, mixed=mixed,
+                              progress=max(64, n // 16))
     s = serving.summarize(res, info)
     s['meets_target'] = bool(s['max_residual'] <= TARGET_RESIDUAL)
     s.update({'arrival_rate_per_s': rate, 'slots': slots, 'admission_budget_bytes': 128e9,
+              'inner_storage': {1: 'fp32', 2: 'fp16'}[mixed],
               'mix': 'volumes 8^4,12^4,16^4,16^3x32,24^4; 1-12 Gaussian sources; m in [0.05,0.3]; mixed CG to 1e-11 (|b - M x|/|b| <= 1e-10)'})
     return {k: (float(f'{v:.6g}') if isinstance(v, float) else v) for k, v in s.items()}
 
@@ -228,6 +231,8 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=256)
     ap.add_argument('--serve-rate', type=float, default=20.0)
     ap.add_argument('--serve-slots', type=int, default=4)
+    ap.add_argument('--serve-mixed', type=int, default=2, choices=(1, 2),
+                    help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
     ap.add_argument('--legs', default='dhop,cg,c3,c4,serve,cpu',
                     help='comma list of legs to run: dhop, cg, c3, c4, serve, cpu (profiling runs pick one)')
     ap.add_argument('--no-graphs', action='store_true',
@@ -316,7 +321,7 @@ def main():
 
     c3 = None if args.no_c3 else bench_mixed_dwf(gx, ctx, tune=tune)
     c4 = None if args.no_c4 else bench_multi_rhs(gx, ctx, tune=tune)
-    serve = bench_serving(args.serve_requests, args.serve_rate, args.serve_slots) if args.serve_requests > 0 else None
+    serve = bench_serving(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_mixed) if args.serve_requests > 0 else None
 
     cpu = None
     if not args.no_cpu:

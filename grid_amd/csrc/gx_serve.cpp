@@ -15,6 +15,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -41,9 +42,11 @@ struct Lattice {
 // Half-field count of a multi-RHS red-black solve on a cloned Wilson operator (FP64 halves):
 // operator temporaries 0-3, CG vectors 4-6, the red-black halves 10-14, the residual check's
 // full field (2), source and solution (4), and with mixed precision the FP64 outer vectors 7-8
-// plus nine FP32 halves (4.5 FP64 halves).
+// plus nine inner-precision halves: FP32 (mixed = 1, 4.5 FP64 halves) or FP16 storage (mixed =
+// 2, 52 of every 192 bytes: 12 binary16 pairs and the scale per entry).
 double footprint(const int d[4], int nsrc, int mixed) {
   const double hb = (double)d[0] * d[1] * d[2] * d[3] / 2 * nsrc * 12 * 16;
+  if (mixed == 2) return hb * (20.0 + 9.0 * 52.0 / 192.0);
   return hb * (mixed ? 24.5 : 18.0);
 }
 
@@ -167,8 +170,17 @@ struct Server {
           c.last_use = ++tick;
       s.reserve = 0;
       s.req = -1;
-      done++;
+      note_done();
       cv_disp.notify_all();
+    }
+  }
+
+  // One more request finished (called with mu held); with p.progress > 0, report every N of them.
+  void note_done() {
+    done++;
+    if (p.progress > 0 && (done % p.progress == 0 || done == n)) {
+      std::fprintf(stderr, "gx_serve: %d of %d requests done, %.1f s\n", done, n, now());
+      std::fflush(stderr);
     }
   }
 
@@ -238,7 +250,7 @@ struct Server {
           resp[r].footprint = need;
           resp[r].t_admit = resp[r].t_done = t;
           pending.pop_front();
-          done++;
+          note_done();
           continue;
         }
         if (in_use + need > p.budget) evict_for(need, k, hit);

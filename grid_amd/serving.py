@@ -36,18 +36,20 @@ def make_trace(n, rate, seed=5000, volumes=VOLUMES, mass_range=MASS_RANGE, nsrc_
 
 
 def footprint(dims, nsrc, mixed=True):
-    """Device bytes a request holds while it runs (fields and operator temporaries)."""
+    """Device bytes a request holds while it runs (fields and operator temporaries).  mixed: 0 or
+    False (FP64 CG), 1 or True (FP32 inner solves), 2 (FP16-storage inner solves)."""
     v = ctypes.c_double()
-    _check(lib().gx_serve_footprint(_dims(dims), int(nsrc), int(bool(mixed)), ctypes.byref(v)))
+    _check(lib().gx_serve_footprint(_dims(dims), int(nsrc), int(mixed), ctypes.byref(v)))
     return v.value
 
 
 def serve(reqs, slots=4, budget=64e9, n_gauge=8, gauge_seed=1000, gauge_eps=0.3, tol=1e-10, mixed=True,
-          maxit=20000, realtime=True, device=0, want_solutions=False):
+          maxit=20000, realtime=True, device=0, want_solutions=False, progress=0):
     """Run a trace through gx_serve.  Returns (responses, info): one dict per request (status,
     slot, iterations, residual, t_admit, t_done, footprint, digest, latency) and info with the
     wall time and resident link bytes.  With want_solutions the FP64 solutions come back as host
-    arrays (Lt, Lz, Ly, Lx, nsrc, 4, 3) in responses[i]['solution']."""
+    arrays (Lt, Lz, Ly, Lx, nsrc, 4, 3) in responses[i]['solution'].  progress = N > 0 makes the
+    runtime print a line on stderr after every N completed requests."""
     n = len(reqs)
     arr = (Request * max(n, 1))()
     for i, r in enumerate(reqs):
@@ -56,7 +58,8 @@ def serve(reqs, slots=4, budget=64e9, n_gauge=8, gauge_seed=1000, gauge_eps=0.3,
         arr[i].seed, arr[i].arrival = int(r['seed']), float(r['arrival'])
     out = (Response * max(n, 1))()
     p = ServeParams(device=device, slots=slots, budget=float(budget), n_gauge=n_gauge, gauge_seed=gauge_seed,
-                    gauge_eps=gauge_eps, tol=tol, mixed=int(bool(mixed)), maxit=maxit, realtime=int(bool(realtime)))
+                    gauge_eps=gauge_eps, tol=tol, mixed=int(mixed), maxit=maxit, realtime=int(bool(realtime)),
+                    progress=int(progress))
     sols, ptrs = None, None
     if want_solutions:
         sols = []

@@ -248,9 +248,10 @@ typedef struct {
   uint64_t gauge_seed;
   double gauge_eps;
   double tol;              /* FP64 true-residual target of every right-hand side */
-  int mixed;               /* 1: FP32 inner solves inside FP64 defect correction */
+  int mixed;               /* 1 (2): FP32 (FP16-storage) inner solves inside FP64 defect correction */
   int maxit;
   int realtime;
+  int progress;            /* 0: silent; N > 0: one line on stderr after every N completed requests */
 } gx_serve_params;
 
 /* Device bytes a request of nsrc sources on lattice dims holds while it runs (fields and the

@@ -89,25 +89,29 @@ def test_multi_rhs_schur_solve(ctx):
         assert abs(worst - st['unprec_residual']) <= 1e-3 * worst + 1e-15
 
 
-def test_serve_footprint_matches_allocation(ctx):
-    """The admission footprint equals what a served request really holds on the device."""
+@pytest.mark.parametrize('mixed', [1, 2])
+def test_serve_footprint_matches_allocation(ctx, mixed):
+    """The admission footprint equals what a served request really holds on the device (FP32 and
+    FP16-storage inner solves)."""
     dims, nrhs = (8, 4, 4, 8), 5
     base = gx.FermionOp(ctx, gx.random_gauge(dims, 51, 0.3), dims, 1, 'wilson', mass=0.1)
     cl = base.clone(ctx, nrhs, 0.2)
     fs, fx = cl.field().gaussian(52), cl.field()
-    cl.schur_solve(fs, fx, tol=1e-10, mixed=True)
+    cl.schur_solve(fs, fx, tol=1e-10, mixed=mixed)
     held = cl.footprint() + 2 * (2 * 12 * 16 * np.prod(dims) // 2 * nrhs)
-    assert held == serving.footprint(dims, nrhs, mixed=True)
+    assert held == serving.footprint(dims, nrhs, mixed=mixed)
 
 
 SMALL = ((4, 4, 4, 4), (8, 4, 4, 4), (4, 4, 4, 8), (8, 8, 4, 4))
 
 
-def test_serve_alone_equals_mix(ctx):
+@pytest.mark.parametrize('mixed', [1, 2])
+def test_serve_alone_equals_mix(ctx, mixed):
     """Continuous batching: requests admitted while others run give bitwise the result they give
-    alone, every one reaches its tolerance, and the served solutions agree with the oracle."""
+    alone, every one reaches its tolerance, and the served solutions agree with the oracle (FP32
+    and FP16-storage inner solves)."""
     reqs = serving.make_trace(14, rate=400.0, seed=77, volumes=SMALL, nsrc_range=(1, 4), n_gauge=2)
-    kw = dict(n_gauge=2, gauge_seed=900, gauge_eps=0.4, tol=1e-10, mixed=True)
+    kw = dict(n_gauge=2, gauge_seed=900, gauge_eps=0.4, tol=1e-10, mixed=mixed)
     mix, info = serving.serve(reqs, slots=3, budget=2e9, want_solutions=True, **kw)
     assert all(r['status'] == 0 for r in mix), [r['status'] for r in mix]
     assert len({r['slot'] for r in mix}) > 1
