@@ -1151,6 +1151,7 @@ static int share_links(Op *op, Op *base) {
   std::lock_guard<std::mutex> lk(g_share_mu);  // base is not a clone: op_gauge takes no lock here
   void *ue, *uo;
   GX_TRY(op_gauge(base, GX_F32, &ue, &uo));
+  GX_TRY(op_gauge(base, GX_F16, &ue, &uo));
   GX_CHECK_HIP(hipStreamSynchronize(base->ctx->stream));
   for (int p = 0; p < 3; p++)
     for (int cb = 0; cb < 2; cb++) op->U[p][cb] = base->U[p][cb];

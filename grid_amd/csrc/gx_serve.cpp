@@ -312,7 +312,7 @@ int gx_serve(const gx_serve_params *p, const gx_request *req, int n, gx_response
   Server sv(*p, req, n, resp, solutions);
   gx_ctx loader = nullptr;
   int rc = gx_ctx_create(p->device, &loader);
-  // resident gauge fields of every lattice in the trace, each with its FP32 copy of the links
+  // resident gauge fields of every lattice in the trace, each with FP32 and FP16 copies of the links
   double res_bytes = 0;
   for (int i = 0; i < n && rc == GX_OK; i++) {
     Lattice lat{{req[i].dims[0], req[i].dims[1], req[i].dims[2], req[i].dims[3]}};
@@ -325,7 +325,7 @@ int gx_serve(const gx_serve_params *p, const gx_request *req, int n, gx_response
       if (rc == GX_OK) rc = gx_wilson_create(loader, lat.d, 1, 0.1, U.data(), nullptr, &base);
       if (rc != GX_OK) break;
       sv.bases[{lat, g}] = base;
-      rc = gx_wilson_clone(base, loader, 1, 0.1, &warm);  // builds the FP32 links
+      rc = gx_wilson_clone(base, loader, 1, 0.1, &warm);  // builds the FP32 and FP16 links
       gx_op_destroy(warm);
       size_t b = 0;
       if (rc == GX_OK) rc = gx_op_footprint(base, &b);
