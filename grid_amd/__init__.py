@@ -125,6 +125,7 @@ _SIG = {
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
     'gx_block_cg': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_reliable_update_cg': ([_P, _P, _P, _D, _I, _D, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_reliable_update_cg_fallback': ([_P, _P, _P, _D, _I, _D, _I, _I, _D, ctypes.POINTER(SolverStats)], _I),
     'gx_tune_set': ([_I, _I], _I),
     'gx_tune_get': ([_I, _IP], _I),
     'gx_random_gauge': ([_IP, ctypes.c_uint64, _D, _DP], _I),
@@ -410,6 +411,17 @@ class FermionOp:
         """ConjugateGradientReliableUpdate: sloppy-precision CG with FP64 reliable updates."""
         st = SolverStats()
         rc = lib().gx_reliable_update_cg(self.h, src.h, psi.h, tol, maxit, delta, sloppy_prec, ctypes.byref(st))
+        if rc not in (0, 3):
+            _check(rc)
+        return st.as_dict()
+
+    def reliable_update_cg_fallback(self, src, psi, tol=1e-10, maxit=10000, delta=0.1, sloppy_prec=F16,
+                                    fallback_prec=F32, fallback_transition_tol=0.0):
+        """ConjugateGradientReliableUpdate with setFallbackLinop: FP16 sloppy vectors switch to FP32 once
+        the sloppy |r|^2 drops below fallback_transition_tol (absolute)."""
+        st = SolverStats()
+        rc = lib().gx_reliable_update_cg_fallback(self.h, src.h, psi.h, tol, maxit, delta, sloppy_prec, fallback_prec,
+                                                  fallback_transition_tol, ctypes.byref(st))
         if rc not in (0, 3):
             _check(rc)
         return st.as_dict()

@@ -90,7 +90,8 @@ struct Gate {
 struct CGState : Gate {
   double c, cp, a, b, d, rsq;
   double delta, maxres, cprev;  // cprev: c of the last finished iteration
-  int ru, pad2;
+  double fbtol;                 // > 0: stop when cp < fbtol so the host can switch precision
+  int ru, pad2;                 // ru: 1 reliable update pending, 2 precision switch pending
 };
 
 // Per-RHS solver scalars for the multi-RHS CG (right-hand side = slice s of the fifth dimension).

@@ -1032,6 +1032,9 @@ static __global__ void __launch_bounds__(256) k_cg_fin(const double *partial, in
         } else if (st->delta > 0 && cp < st->delta * st->maxres && st->k_conv == INT_MAX) {
           st->ru = 1;
           st->k_conv = st->k;
+        } else if (st->fbtol > 0 && cp < st->fbtol && st->k_conv == INT_MAX) {
+          st->ru = 2;
+          st->k_conv = st->k;
         }
       }
       st->k = st->k + 1;
@@ -1047,7 +1050,7 @@ __global__ void __launch_bounds__(256) k_cg_update(int64_t nsites, int64_t strid
   typedef typename Sto<S>::C T;
   if (gated_prev(st)) return;
   const T a = (T)st->a, b = (T)st->b;
-  const bool keep_p = st->ru != 0;  // reliable update pending: the host forms the new p
+  const bool keep_p = st->ru == 1;  // reliable update pending: the host forms the new p
   if constexpr (!Sto<S>::H) {
     const int64_t n = 12 * nsites;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {

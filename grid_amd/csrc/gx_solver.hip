@@ -373,13 +373,28 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
 // last update; the host then folds psi_f into psi, recomputes r = src - A psi in FP64, restarts
 // psi_f at zero, forms p = (cp_new / c) p + r, and resumes.  After convergence an FP64 CG from psi
 // cleans up (DoFinalCleanup).
-int ru_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, double delta, int sp, gx_solver_stats *out) {
+//
+// setFallbackLinop (ConjugateGradientReliableUpdate.h:72-75,255-259): with fb_prec >= 0 the sloppy
+// iteration switches to precision fb_prec once its residual norm cp drops below fb_tol (an absolute
+// |r|^2, as in Grid).  Grid switches the operator of its half-precision communications to a
+// single-precision one; here the sloppy vectors themselves change storage (FP16 to FP32): p and r
+// are promoted exactly, psi_f is folded into psi, and the iteration continues with the same
+// scalars.
+static int ru_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, double delta, int sp, int fb_prec,
+                     double fb_tol, gx_solver_stats *out) {
   auto t0 = std::chrono::steady_clock::now();
+  const bool fallback = fb_prec >= 0 && fb_prec != sp && fb_tol > 0;
   GX_TRY(op_prepare(op, GX_F64));
   GX_TRY(op_prepare(op, sp));
+  if (fallback) GX_TRY(op_prepare(op, fb_prec));
   Field *md = op_tmp(op, GX_F64, 7), *rd = op_tmp(op, GX_F64, 8);
   Field *pf = op_tmp(op, sp, 4), *mf = op_tmp(op, sp, 5), *rf = op_tmp(op, sp, 6), *xf = op_tmp(op, sp, 8);
   if (!md || !rd || !pf || !mf || !rf || !xf) return GX_ENOMEM;
+  Field *p2 = nullptr, *m2 = nullptr, *r2 = nullptr, *x2 = nullptr;  // fallback-precision vectors
+  if (fallback) {
+    p2 = op_tmp(op, fb_prec, 4), m2 = op_tmp(op, fb_prec, 5), r2 = op_tmp(op, fb_prec, 6), x2 = op_tmp(op, fb_prec, 8);
+    if (!p2 || !m2 || !r2 || !x2) return GX_ENOMEM;
+  }
   const int cb = src->cb;
   psi->cb = md->cb = rd->cb = pf->cb = mf->cb = rf->cb = xf->cb = cb;
   Ctx *ctx = op->ctx;
@@ -416,34 +431,46 @@ int ru_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, double delt
   init.c = init.cp = init.maxres = cp;
   init.rsq = rsq;
   init.delta = delta;
+  init.fbtol = fallback ? fb_tol : 0.0;
   init.k = 1;
   init.k_conv = INT_MAX;
   init.kmax = maxit;
   w.host[0] = init;
   GX_CHECK_HIP(hipMemcpyAsync(w.dev, &w.host[0], sizeof(CGState), hipMemcpyHostToDevice, ctx->stream));
   GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  int enqueued = 0, updates = 0, sloppy_its = maxit;
+  int enqueued = 0, updates = 0, sloppy_its = maxit, cur = sp;
   for (;;) {
     CGState fin{};
-    GX_TRY(cg_device_loop(op, sp, xf, pf, mf, rf, w, maxit, enqueued, &fin));
+    GX_TRY(cg_device_loop(op, cur, xf, pf, mf, rf, w, maxit, enqueued, &fin));
     GX_TRY(convert_field(md, xf));
     GX_TRY(blas_axpby(psi, 1.0, psi, 1.0, md));  // psi += psi_f
     if (fin.k_conv == INT_MAX || !fin.ru) {        // converged, or out of iterations
       if (fin.k_conv != INT_MAX) sloppy_its = fin.k_conv;
       break;
     }
-    // reliable update (ConjugateGradientReliableUpdate.h:209-231)
-    double cpn;
-    GX_TRY(apply_mpcdagmpc(op, psi, md));
-    GX_TRY(blas_axpy_norm(rd, -1.0, md, src, &cpn));
-    GX_TRY(convert_field(rf, rd));
-    GX_CHECK_HIP(hipMemsetAsync(xf->data, 0, xf->bytes, ctx->stream));
-    const double b = cpn / fin.cprev;
-    GX_TRY(blas_axpby(pf, b, pf, 1.0, rf));  // p = b p + r (ConjugateGradientReliableUpdate.h:234)
-    updates++;
     CGState rs = fin;
-    rs.c = rs.cp = rs.maxres = cpn;
-    rs.b = b;
+    GX_CHECK_HIP(hipMemsetAsync(xf->data, 0, xf->bytes, ctx->stream));
+    if (fin.ru == 1) {
+      // reliable update (ConjugateGradientReliableUpdate.h:209-231)
+      double cpn;
+      GX_TRY(apply_mpcdagmpc(op, psi, md));
+      GX_TRY(blas_axpy_norm(rd, -1.0, md, src, &cpn));
+      GX_TRY(convert_field(rf, rd));
+      const double b = cpn / fin.cprev;
+      GX_TRY(blas_axpby(pf, b, pf, 1.0, rf));  // p = b p + r (ConjugateGradientReliableUpdate.h:234)
+      updates++;
+      rs.c = rs.cp = rs.maxres = cpn;
+      rs.b = b;
+    }
+    if (rs.fbtol > 0 && rs.cp < rs.fbtol) {  // switch to the fallback precision (:255-259)
+      GX_TRY(convert_field(p2, pf));
+      GX_TRY(convert_field(r2, rf));
+      x2->cb = m2->cb = cb;
+      GX_CHECK_HIP(hipMemsetAsync(x2->data, 0, x2->bytes, ctx->stream));
+      pf = p2, mf = m2, rf = r2, xf = x2;
+      cur = fb_prec;
+      rs.fbtol = 0.0;
+    }
     rs.ru = 0;
     rs.k = fin.k_conv + 1;
     rs.k_conv = INT_MAX;
@@ -759,7 +786,27 @@ int gx_reliable_update_cg(gx_op op, gx_field src, gx_field psi, double tol, int 
     set_error("reliable_update_cg: needs 0 < delta < 1 and a GX_F32 or GX_F16 sloppy precision");
     return GX_EINVAL;
   }
-  return ru_cg_run(op, src, psi, tol, maxit, delta, sloppy_prec, st);
+  return ru_cg_run(op, src, psi, tol, maxit, delta, sloppy_prec, -1, 0.0, st);
+}
+
+int gx_reliable_update_cg_fallback(gx_op op, gx_field src, gx_field psi, double tol, int maxit, double delta,
+                                   int sloppy_prec, int fallback_prec, double fallback_transition_tol,
+                                   gx_solver_stats *st) {
+  if (sloppy_prec != GX_F16 || fallback_prec != GX_F32 || !(fallback_transition_tol >= 0.0)) {
+    set_error("reliable_update_cg_fallback: needs GX_F16 sloppy, GX_F32 fallback precision and a transition "
+              "tolerance >= 0");
+    return GX_EINVAL;
+  }
+  if (!op || !src || !psi || src == psi || src->prec != GX_F64 || psi->prec != GX_F64 || src->subset == GX_FULL ||
+      psi->subset == GX_FULL || !same_geom(src->g, op->g) || !same_geom(psi->g, op->g)) {
+    set_error("reliable_update_cg_fallback: needs F64 half fields on the operator's lattice");
+    return GX_EINVAL;
+  }
+  if (!(delta > 0.0 && delta < 1.0)) {
+    set_error("reliable_update_cg_fallback: needs 0 < delta < 1");
+    return GX_EINVAL;
+  }
+  return ru_cg_run(op, src, psi, tol, maxit, delta, sloppy_prec, fallback_prec, fallback_transition_tol, st);
 }
 
 int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, int inner_prec,

@@ -756,7 +756,10 @@ class MixedPrecisionConjugateGradient {
 // Grid: ConjugateGradientReliableUpdate<FieldD,FieldF>(tol, maxit, delta, sp_grid, Linop_f, Linop_d,
 // err_on_no_conv)(src, psi), ConjugateGradientReliableUpdate.h:36-269.  The sloppy Krylov vectors
 // are FieldF's precision (FP32); SloppyHalf = true keeps them in FP16 storage instead.  The final
-// FP64 cleanup CG always runs (Grid's DoFinalCleanup default).
+// FP64 cleanup CG always runs (Grid's DoFinalCleanup default).  setFallbackLinop (:72-75,255-259):
+// with SloppyHalf the FP16 vectors move to FP32 storage once the sloppy |r|^2 drops below the
+// transition tolerance; with FP32 sloppy vectors the fallback operator is the same single-precision
+// operator (there is no compressed-communication form to leave), so the call changes nothing.
 template <class FieldD, class FieldF>
 class ConjugateGradientReliableUpdate {
  public:
@@ -771,6 +774,12 @@ class ConjugateGradientReliableUpdate {
   GridBase *SinglePrecGrid;
   RealD Delta;
   RealD TrueResidual = 0;
+  LinearOperatorBase<FieldF> *Linop_fallback = nullptr;
+  RealD fallback_transition_tol = 0;
+  void setFallbackLinop(LinearOperatorBase<FieldF> &lf, const RealD tol) {
+    Linop_fallback = &lf;
+    fallback_transition_tol = tol;
+  }
   ConjugateGradientReliableUpdate(RealD tol, Integer maxit, RealD delta, GridBase *sp_grid,
                                   LinearOperatorBase<FieldF> &lf, LinearOperatorBase<FieldD> &ld,
                                   bool err_on_no_conv = true)
@@ -783,8 +792,11 @@ class ConjugateGradientReliableUpdate {
     assert(op && Linop_f.schur_handle());
     psi.Checkerboard() = src.Checkerboard();
     gx_solver_stats st{};
-    const int rc = gx_reliable_update_cg(op, src.f, psi.f, Tolerance, MaxIterations, Delta,
-                                         SloppyHalf ? GX_F16 : GX_F32, &st);
+    const int rc = SloppyHalf && Linop_fallback
+                       ? gx_reliable_update_cg_fallback(op, src.f, psi.f, Tolerance, MaxIterations, Delta, GX_F16,
+                                                        GX_F32, fallback_transition_tol, &st)
+                       : gx_reliable_update_cg(op, src.f, psi.f, Tolerance, MaxIterations, Delta,
+                                               SloppyHalf ? GX_F16 : GX_F32, &st);
     IterationsToComplete = st.inner_iterations;
     ReliableUpdatesPerformed = st.outer_iterations;
     IterationsToCleanup = st.final_iterations;

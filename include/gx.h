@@ -187,6 +187,16 @@ int gx_mixed_cg(gx_op op, gx_field src, gx_field psi, double tol, double inner_t
 int gx_reliable_update_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, double delta,
                           int sloppy_prec, gx_solver_stats *st);
 
+/* Grid: ConjugateGradientReliableUpdate::setFallbackLinop(Linop_fallback, fallback_transition_tol)
+ * then operator(), ConjugateGradientReliableUpdate.h:72-75,255-259.  gx_reliable_update_cg with
+ * GX_F16 sloppy vectors that switch to GX_F32 storage once the sloppy residual |r|^2 (absolute,
+ * as in Grid) falls below fallback_transition_tol; p and r are promoted exactly and the iteration
+ * continues.  FP16 Krylov vectors lose conjugacy on ill-conditioned systems and can stall far
+ * above tol; the switch lets the FP16 phase take the cheap early iterations.  0 never switches. */
+int gx_reliable_update_cg_fallback(gx_op op, gx_field src, gx_field psi, double tol, int maxit, double delta,
+                                   int sloppy_prec, int fallback_prec, double fallback_transition_tol,
+                                   gx_solver_stats *st);
+
 /* Grid: SchurRedBlackDiagMooeeSolve<Field>(CG)(Dw, src, sol), SchurRedBlack.h:236-291,373-436.
  * src, sol: full F64 fields.  mixed = 1 (2) runs the red-black solve as gx_mixed_cg with FP32
  * (FP16-storage) inner solves, mixed = 0 as FP64 gx_cg.

@@ -1,7 +1,7 @@
 // Mirrors Grid's tests/solver/Test_dwf_relupcg_prec.cc: domain-wall (Shamir) Schur system solved
 // with ConjugateGradientReliableUpdate (FP32 sloppy vectors, FP64 reliable updates, delta 0.2),
-// with FP16-storage sloppy vectors, and with plain FP64 ConjugateGradient; all reach the FP64
-// tolerance and agree.
+// with FP16-storage sloppy vectors, with FP16 sloppy vectors and a single-precision fallback
+// (setFallbackLinop), and with plain FP64 ConjugateGradient; all reach the FP64 tolerance and agree.
 #include <Grid/Grid.h>
 
 using namespace Grid;
@@ -30,12 +30,14 @@ int main(int argc, char **argv) {
   RealD mass = 0.1, M5 = 1.8;
   DomainWallFermionD Ddwf(Umu, *FGrid, *FrbGrid, *UGrid, *UrbGrid, mass, M5);
   DomainWallFermionF Ddwf_f(Umu_f, *FGrid, *FrbGrid, *UGrid, *UrbGrid, mass, M5);
-  LatticeFermionD src_o(FrbGrid), result_o(FrbGrid), result_h(FrbGrid), result_o_2(FrbGrid);
+  LatticeFermionD src_o(FrbGrid), result_o(FrbGrid), result_h(FrbGrid), result_f(FrbGrid), result_o_2(FrbGrid);
   pickCheckerboard(Odd, src_o, src);
   result_o = Zero();
   result_o.Checkerboard() = Odd;
   result_h = Zero();
   result_h.Checkerboard() = Odd;
+  result_f = Zero();
+  result_f.Checkerboard() = Odd;
   result_o_2 = Zero();
   result_o_2.Checkerboard() = Odd;
 
@@ -49,15 +51,21 @@ int main(int argc, char **argv) {
   mCG.SloppyHalf = true;
   mCG(src_o, result_h);
   const double res16 = mCG.TrueResidual;
+  mCG.setFallbackLinop(HermOpEO_f, 1e-6 * norm2(src_o));
+  mCG(src_o, result_f);
+  const double resfb = mCG.TrueResidual;
   ConjugateGradient<LatticeFermionD> CG(1.0e-10, 10000);
   CG(HermOpEO, src_o, result_o_2);
-  LatticeFermionD diff = result_o - result_o_2, diffh = result_h - result_o_2;
-  const double rel = std::sqrt(norm2(diff) / norm2(result_o_2)), relh = std::sqrt(norm2(diffh) / norm2(result_o_2));
+  LatticeFermionD diff = result_o - result_o_2, diffh = result_h - result_o_2, difff = result_f - result_o_2;
+  const double nref = norm2(result_o_2);
+  const double rel = std::sqrt(norm2(diff) / nref), relh = std::sqrt(norm2(diffh) / nref),
+               relf = std::sqrt(norm2(difff) / nref);
   std::cout << GridLogMessage << "reliable-update true residual " << res32 << " (FP16 sloppy " << res16
-            << "), FP64 true residual " << CG.TrueResidual << ", relative differences " << rel << " " << relh
-            << ", reliable updates " << relups << std::endl;
-  const int fail = !(res32 < 1e-9) || !(res16 < 1e-9) || !(CG.TrueResidual < 1e-9) || !(rel < 1e-7) ||
-                   !(relh < 1e-7) || relups < 1;
+            << ", FP16 with FP32 fallback " << resfb << "), FP64 true residual " << CG.TrueResidual
+            << ", relative differences " << rel << " " << relh << " " << relf << ", reliable updates " << relups
+            << std::endl;
+  const int fail = !(res32 < 1e-9) || !(res16 < 1e-9) || !(resfb < 1e-9) || !(CG.TrueResidual < 1e-9) ||
+                   !(rel < 1e-7) || !(relh < 1e-7) || !(relf < 1e-7) || relups < 1;
   Grid_finalize();
   std::cout << (fail ? "FAILED" : "PASSED") << std::endl;
   return fail;

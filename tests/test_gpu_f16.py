@@ -114,3 +114,37 @@ def test_reliable_update_cg(ctx, kind, Ls, sloppy):
     assert sru['outer_iterations'] >= 2, sru          # reliable updates performed
     assert sru['final_iterations'] < 0.2 * s64['iterations'] + 5, (sru, s64)
     assert rel_err(xru.download(), x64.download()) < 1e-8
+
+
+@pytest.mark.parametrize('kind,Ls', [('wilson', 1), ('mobius', 8)])
+def test_reliable_update_fallback(ctx, kind, Ls):
+    """setFallbackLinop: FP16 sloppy vectors switch to FP32 below the transition |r|^2.  Transition 0
+    never switches (the same digest as plain FP16 reliable updates); a transition above |src|^2
+    switches after the first iteration and then matches the FP32 iteration count within a few; a
+    middle one switches part way.  Every variant reaches the FP64 tolerance and solution."""
+    dims = (8, 8, 8, 4)
+    U = gx.random_gauge(dims, 87, 0.3)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, mass=0.05)
+    src = half(gx.gaussian_fermion(dims, Ls, 88), dims, gx.ODD)
+    fs = D.field(gx.F64, gx.ODD).upload(src)
+    ssq = fs.norm2()
+    x64 = D.field(gx.F64, gx.ODD)
+    s64 = D.cg(fs, x64, tol=1e-10)
+    ref = x64.download()
+    x16, x32 = D.field(gx.F64, gx.ODD), D.field(gx.F64, gx.ODD)
+    s16 = D.reliable_update_cg(fs, x16, tol=1e-10, delta=0.1, sloppy_prec=gx.F16)
+    s32 = D.reliable_update_cg(fs, x32, tol=1e-10, delta=0.1, sloppy_prec=gx.F32)
+    for ftol in (0.0, 1e-5 * ssq, 10.0 * ssq):
+        x = D.field(gx.F64, gx.ODD)
+        st = D.reliable_update_cg_fallback(fs, x, tol=1e-10, delta=0.1, fallback_transition_tol=ftol)
+        assert st['converged'] and st['true_residual'] < 1e-9, (ftol, st)
+        assert rel_err(x.download(), ref) < 1e-8, ftol
+        if ftol == 0.0:
+            assert x.digest() == x16.digest() and st['inner_iterations'] == s16['inner_iterations'], (st, s16)
+        if ftol > ssq:
+            assert abs(st['inner_iterations'] - s32['inner_iterations']) <= 3, (st, s32)
+    bad = D.field(gx.F64, gx.ODD)
+    for args in ((gx.F32, gx.F32, 0.0), (gx.F16, gx.F64, 0.0), (gx.F16, gx.F32, -1.0)):
+        with pytest.raises(gx.GxError):
+            D.reliable_update_cg_fallback(fs, bad, sloppy_prec=args[0], fallback_prec=args[1],
+                                          fallback_transition_tol=args[2])
