@@ -124,6 +124,9 @@ _SIG = {
     'gx_mixed_cg': ([_P, _P, _P, _D, _D, _I, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
     'gx_block_cg': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_block_cgrq': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_slice_inner_product_matrix': ([_P, _P, _DP], _I),
+    'gx_slice_madd_matrix': ([_P, _DP, _P, _P, _D], _I),
     'gx_reliable_update_cg': ([_P, _P, _P, _D, _I, _D, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_reliable_update_cg_fallback': ([_P, _P, _P, _D, _I, _D, _I, _I, _D, ctypes.POINTER(SolverStats)], _I),
     'gx_tune_set': ([_I, _I], _I),
@@ -315,6 +318,21 @@ def precision_change(out, inp):
     _check(lib().gx_precision_change(out.h, inp.h))
 
 
+def slice_inner_product_matrix(lhs, rhs):
+    """sliceInnerProductMatrix (Orthog = the fifth dimension): m[s, t] = <lhs_s, rhs_t>."""
+    n = lhs.Ls
+    m = np.zeros((n, n), dtype=np.complex128)
+    _check(lib().gx_slice_inner_product_matrix(lhs.h, rhs.h, _dp(m)))
+    return m
+
+
+def slice_madd_matrix(r, aa, x, y=None, scale=1.0):
+    """sliceMaddMatrix: r_t = y_t + scale sum_s x_s aa[s, t] (y = None: sliceMulMatrix)."""
+    a = np.ascontiguousarray(aa, dtype=np.complex128)
+    assert a.shape == (x.Ls, x.Ls), a.shape
+    _check(lib().gx_slice_madd_matrix(r.h, _dp(a), x.h, y.h if y is not None else None, scale))
+
+
 def pick_checkerboard(cb, half, full):
     _check(lib().gx_pick_checkerboard(cb, half.h, full.h))
 
@@ -430,6 +448,15 @@ class FermionOp:
         """Multi-RHS CG (right-hand side k = slice s = k); returns one stats dict per RHS."""
         st = (SolverStats * self.Ls)()
         rc = lib().gx_block_cg(self.h, src.h, psi.h, tol, maxit, inner_prec, st)
+        if rc not in (0, 3):
+            _check(rc)
+        return [s.as_dict() for s in st]
+
+    def block_cgrq(self, src, psi, tol=1e-10, maxit=10000):
+        """BlockConjugateGradient(BlockCGrQ): one block Krylov space for the Ls right-hand sides;
+        returns one stats dict per RHS."""
+        st = (SolverStats * self.Ls)()
+        rc = lib().gx_block_cgrq(self.h, src.h, psi.h, tol, maxit, st)
         if rc not in (0, 3):
             _check(rc)
         return [s.as_dict() for s in st]

@@ -237,6 +237,15 @@ int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y
 template <typename T>
 int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state);
 
+// Slice matrices of FP64 fields with nhalf (1 or 2) checkerboard halves (sliceInnerProductMatrix,
+// sliceMaddMatrix).  ipm: out_dev[s * Ls + t] = sum conj(x_s) y_t (double2), through nparts
+// workgroup partials of Ls^2 double2 each.  madd: r_t = y_t + scale sum_s x_s a(s, t), a_dev
+// row-major Ls x Ls double2, y may be null (zero); r may alias x or y.
+int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const void *y, void *partial, int nparts,
+                     void *out_dev);
+int launch_slice_madd(Ctx *ctx, const Geom &g, int nhalf, const void *a_dev, double scale, const void *x,
+                      const void *y, void *r);
+
 // ---------------- api helpers (gx_api.hip) ----------------
 int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out);
 void field_destroy(Field *f);

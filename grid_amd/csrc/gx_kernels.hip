@@ -1395,7 +1395,144 @@ static __global__ void __launch_bounds__(256) k_digest(const uint64_t *__restric
   if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+// ---------------- slice matrices (Lattice_reduction.h:717-776) ----------------
+// FP64 fields viewed as (rows) x Ls with the fifth-dimension slice s innermost: entry e = h Ls + s
+// of half hf stores component k at hf * hstride + ((e / kBlk) * 12 + k) * kBlk + e % kBlk.
+// Global site g in [0, nhalf H4) is 4D half-site g % H4 of half g / H4.
+static __device__ __forceinline__ int64_t slice_idx(int64_t gs, int s, int k, int64_t H4, int Ls, int64_t hstride) {
+  const int64_t hf = gs >= H4 ? 1 : 0, e = (gs - hf * H4) * Ls + s;
+  return hf * hstride + ((e / kBlk) * 12 + k) * kBlk + (e & (kBlk - 1));
+}
+
+// sliceInnerProductMatrix: m(s, t) = sum conj(x_s) y_t.  A workgroup stages kIpmSites sites (every
+// slice and component) of x and y in LDS; thread s * Ls + t accumulates m(s, t) over them and
+// writes its workgroup's partial sum; k_slice_ipm_fin adds the partials in a fixed order.
+constexpr int kIpmSites = 8;
+static __global__ void __launch_bounds__(256) k_slice_ipm(int64_t nsite, int64_t H4, int Ls, int64_t hstride,
+                                                          const cx<double> *__restrict__ x,
+                                                          const cx<double> *__restrict__ y, double2 *partial) {
+  __shared__ double2 sx[kIpmSites * kMaxLs * 12], sy[kIpmSites * kMaxLs * 12];
+  const int t = threadIdx.x, ls2 = Ls * Ls;
+  const int si = t / Ls, ti = t - si * Ls;
+  const int per = kIpmSites * Ls;
+  double ar = 0, ai = 0;
+  for (int64_t g0 = (int64_t)blockIdx.x * kIpmSites; g0 < nsite; g0 += (int64_t)gridDim.x * kIpmSites) {
+    const int ns = (int)min((int64_t)kIpmSites, nsite - g0);
+    __syncthreads();
+    for (int q = t; q < 12 * per; q += 256) {
+      const int k = q / per, j = q - k * per;  // component k, staged entry j = site * Ls + s
+      double2 vx = {0, 0}, vy = {0, 0};
+      if (j < ns * Ls) {
+        const int hs = j / Ls;
+        const int64_t i = slice_idx(g0 + hs, j - hs * Ls, k, H4, Ls, hstride);
+        vx = {x[i].r, x[i].i};
+        vy = {y[i].r, y[i].i};
+      }
+      sx[j * 12 + k] = vx;
+      sy[j * 12 + k] = vy;
+    }
+    __syncthreads();
+    if (t < ls2)
+      for (int hs = 0; hs < ns; hs++) {
+        const double2 *px = sx + (hs * Ls + si) * 12, *py = sy + (hs * Ls + ti) * 12;
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+          ar = fma(px[k].x, py[k].x, fma(px[k].y, py[k].y, ar));
+          ai = fma(px[k].x, py[k].y, fma(-px[k].y, py[k].x, ai));
+        }
+      }
+  }
+  if (t < ls2) partial[(int64_t)blockIdx.x * ls2 + t] = {ar, ai};
+}
+
+static __global__ void __launch_bounds__(256) k_slice_ipm_fin(const double2 *partial, int nparts, int ls2,
+                                                              double2 *out) {
+  const int t = threadIdx.x;
+  if (t >= ls2) return;
+  double ar = 0, ai = 0;
+  for (int b = 0; b < nparts; b++) {
+    ar += partial[(int64_t)b * ls2 + t].x;
+    ai += partial[(int64_t)b * ls2 + t].y;
+  }
+  out[t] = {ar, ai};
+}
+
+// sliceMaddMatrix: r_t = y_t + scale sum_s x_s a(s, t), a row-major (a[s * Ls + t]); y = null reads
+// as zero (sliceMulMatrix).  One thread per (site, component) reads every slice of x and y before
+// writing r, so r may alias x or y.
+static __global__ void __launch_bounds__(256) k_slice_madd(int64_t nsite, int64_t H4, int Ls, int64_t hstride,
+                                                           const double2 *a, double scale, const cx<double> *x,
+                                                           const cx<double> *y, cx<double> *r) {
+  __shared__ double2 sa[kMaxLs * kMaxLs];
+  for (int q = threadIdx.x; q < Ls * Ls; q += blockDim.x) sa[q] = {scale * a[q].x, scale * a[q].y};
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 12 * nsite;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i / nsite);
+    const int64_t gs = i - k * nsite;
+    double2 xs[kMaxLs];
+#pragma unroll
+    for (int s = 0; s < kMaxLs; s++)
+      if (s < Ls) {
+        const cx<double> v = x[slice_idx(gs, s, k, H4, Ls, hstride)];
+        xs[s] = {v.r, v.i};
+      }
+    double2 out[kMaxLs];
+#pragma unroll
+    for (int t = 0; t < kMaxLs; t++)
+      if (t < Ls) {
+        double orr = 0, oi = 0;
+        if (y) {
+          const cx<double> v = y[slice_idx(gs, t, k, H4, Ls, hstride)];
+          orr = v.r, oi = v.i;
+        }
+#pragma unroll
+        for (int s = 0; s < kMaxLs; s++)
+          if (s < Ls) {
+            const double2 c = sa[s * Ls + t];
+            orr = fma(xs[s].x, c.x, fma(-xs[s].y, c.y, orr));
+            oi = fma(xs[s].x, c.y, fma(xs[s].y, c.x, oi));
+          }
+        out[t] = {orr, oi};
+      }
+#pragma unroll
+    for (int t = 0; t < kMaxLs; t++)
+      if (t < Ls) r[slice_idx(gs, t, k, H4, Ls, hstride)] = {out[t].x, out[t].y};
+  }
+}
+
 #if GX_PART == 0
+int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const void *y, void *partial, int nparts,
+                     void *out_dev) {
+  if (g.Ls > kMaxLs || nparts < 1) {
+    set_error("slice inner product matrix: Ls %d > %d or no partials", g.Ls, kMaxLs);
+    return GX_EINVAL;
+  }
+  const int64_t nsite = nhalf * g.H4, hstride = 12 * g.half_pad();
+  int64_t nb = (nsite + kIpmSites - 1) / kIpmSites;
+  if (nb > nparts) nb = nparts;
+  hipLaunchKernelGGL(k_slice_ipm, dim3(nb), dim3(256), 0, ctx->stream, nsite, g.H4, g.Ls, hstride,
+                     (const cx<double> *)x, (const cx<double> *)y, (double2 *)partial);
+  hipLaunchKernelGGL(k_slice_ipm_fin, dim3(1), dim3(256), 0, ctx->stream, (const double2 *)partial, (int)nb,
+                     g.Ls * g.Ls, (double2 *)out_dev);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int launch_slice_madd(Ctx *ctx, const Geom &g, int nhalf, const void *a_dev, double scale, const void *x,
+                      const void *y, void *r) {
+  if (g.Ls > kMaxLs) {
+    set_error("slice madd matrix: Ls %d > %d", g.Ls, kMaxLs);
+    return GX_EINVAL;
+  }
+  const int64_t nsite = nhalf * g.H4, hstride = 12 * g.half_pad();
+  hipLaunchKernelGGL(k_slice_madd, dim3(stream_blocks(12 * nsite)), dim3(256), 0, ctx->stream, nsite, g.H4, g.Ls,
+                     hstride, (const double2 *)a_dev, scale, (const cx<double> *)x, (const cx<double> *)y,
+                     (cx<double> *)r);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
 int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
   GX_CHECK_HIP(hipMemsetAsync(ctx->digest, 0, sizeof(unsigned long long), ctx->stream));
   hipLaunchKernelGGL(k_digest, dim3(stream_blocks(nwords)), dim3(256), 0, ctx->stream, (const uint64_t *)data, nwords,

@@ -14,8 +14,11 @@
 // SchurRedBlackDiagMooeeSolve (SchurRedBlack.h:236-291,385-425).
 #include <chrono>
 #include <climits>
+#include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
+#include <vector>
 
 #include "gx_internal.h"
 
@@ -738,6 +741,230 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
   return rc;
 }
 
+// ---------------- BlockCGrQ (BlockConjugateGradient.h:223-410) ----------------
+// Small dense complex matrices (n = Ls <= 16, row-major) for the block solver's host algebra.
+typedef std::complex<double> cd;
+struct CMat {
+  int n = 0;
+  std::vector<cd> a;
+  explicit CMat(int n_ = 0, bool ident = false) : n(n_), a((size_t)n_ * n_) {
+    if (ident)
+      for (int i = 0; i < n; i++) a[(size_t)i * n + i] = 1.0;
+  }
+  cd &operator()(int i, int j) { return a[(size_t)i * n + j]; }
+  cd operator()(int i, int j) const { return a[(size_t)i * n + j]; }
+};
+
+static CMat cm_mul(const CMat &x, const CMat &y) {
+  CMat z(x.n);
+  for (int i = 0; i < x.n; i++)
+    for (int k = 0; k < x.n; k++) {
+      const cd v = x(i, k);
+      for (int j = 0; j < x.n; j++) z(i, j) += v * y(k, j);
+    }
+  return z;
+}
+
+static CMat cm_adj(const CMat &x) {
+  CMat z(x.n);
+  for (int i = 0; i < x.n; i++)
+    for (int j = 0; j < x.n; j++) z(i, j) = std::conj(x(j, i));
+  return z;
+}
+
+// Gauss-Jordan inverse with partial pivoting; false if a pivot is zero or not finite.
+static bool cm_inv(const CMat &x, CMat &out) {
+  const int n = x.n;
+  CMat w = x;
+  out = CMat(n, true);
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int r = c + 1; r < n; r++)
+      if (std::abs(w(r, c)) > std::abs(w(piv, c))) piv = r;
+    const double pm = std::abs(w(piv, c));
+    if (!(pm > 0.0) || !std::isfinite(pm)) return false;
+    if (piv != c)
+      for (int j = 0; j < n; j++) {
+        std::swap(w(piv, j), w(c, j));
+        std::swap(out(piv, j), out(c, j));
+      }
+    const cd inv = 1.0 / w(c, c);
+    for (int j = 0; j < n; j++) {
+      w(c, j) *= inv;
+      out(c, j) *= inv;
+    }
+    for (int r = 0; r < n; r++)
+      if (r != c && w(r, c) != 0.0) {
+        const cd f = w(r, c);
+        for (int j = 0; j < n; j++) {
+          w(r, j) -= f * w(c, j);
+          out(r, j) -= f * out(c, j);
+        }
+      }
+  }
+  return true;
+}
+
+// Cholesky m = L L^dag of a Hermitian positive-definite m (Eigen's llt().matrixL()); false if m is
+// not positive definite to working precision.
+static bool cm_chol(const CMat &m, CMat &L) {
+  const int n = m.n;
+  L = CMat(n);
+  for (int j = 0; j < n; j++) {
+    double d = m(j, j).real();
+    for (int k = 0; k < j; k++) d -= std::norm(L(j, k));
+    if (!(d > 0.0) || !std::isfinite(d)) return false;
+    const double ljj = std::sqrt(d);
+    L(j, j) = ljj;
+    for (int i = j + 1; i < n; i++) {
+      cd v = m(i, j);
+      for (int k = 0; k < j; k++) v -= L(i, k) * std::conj(L(j, k));
+      L(i, j) = v / ljj;
+    }
+  }
+  return true;
+}
+
+struct BlockWork {
+  Ctx *ctx = nullptr;
+  void *partial = nullptr;   // workgroup partials of the slice inner products
+  double2 *mat_dev = nullptr;   // result matrix + 4 coefficient matrices, kMaxLs^2 each
+  double2 *mat_host = nullptr;  // pinned mirror
+  int nparts = 512;
+  ~BlockWork() {
+    if (partial) (void)hipFree(partial);
+    if (mat_dev) (void)hipFree(mat_dev);
+    if (mat_host) (void)hipHostFree(mat_host);
+  }
+  int init(Ctx *c) {
+    ctx = c;
+    const size_t m = (size_t)kMaxLs * kMaxLs * sizeof(double2);
+    hipError_t e = hipMalloc(&partial, (size_t)nparts * m);
+    if (e == hipSuccess) e = hipMalloc(&mat_dev, 5 * m);
+    if (e == hipSuccess) e = hipHostMalloc(&mat_host, 5 * m, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      set_error("block solver workspace: %s", hipGetErrorString(e));
+      return GX_ENOMEM;
+    }
+    return GX_OK;
+  }
+  // m(s, t) = <x_s, y_t> (sliceInnerProductMatrix); synchronises the stream
+  int ipm(const Geom &g, int nhalf, Field *x, Field *y, CMat &m) {
+    const int n = g.Ls;
+    GX_TRY(launch_slice_ipm(ctx, g, nhalf, x->data, y->data, partial, nparts, mat_dev));
+    GX_CHECK_HIP(hipMemcpyAsync(mat_host, mat_dev, sizeof(double2) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    m = CMat(n);
+    for (int i = 0; i < n * n; i++) m.a[i] = cd(mat_host[i].x, mat_host[i].y);
+    return GX_OK;
+  }
+  // r_t = y_t + scale sum_s x_s a(s, t) (sliceMaddMatrix; y = null: sliceMulMatrix) with coefficient
+  // slot 1..4: a slot's pinned copy is rewritten only after a later ipm() has synchronised the
+  // stream past its use.
+  int madd(const Geom &g, int nhalf, Field *r, const CMat &a, Field *x, Field *y, double scale, int slot) {
+    const int n = g.Ls;
+    double2 *h = mat_host + (size_t)slot * kMaxLs * kMaxLs, *d = mat_dev + (size_t)slot * kMaxLs * kMaxLs;
+    for (int i = 0; i < n * n; i++) h[i] = make_double2(a.a[i].real(), a.a[i].imag());
+    GX_CHECK_HIP(hipMemcpyAsync(d, h, sizeof(double2) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    GX_TRY(launch_slice_madd(ctx, g, nhalf, d, scale, x->data, y ? y->data : nullptr, r->data));
+    r->cb = x->cb;
+    return GX_OK;
+  }
+};
+
+// ThinQRfact (BlockConjugateGradient.h:132-168): m_rr = R^dag R (made manifestly Hermitian),
+// L = chol(m_rr), C = L^dag, Q = R C^-1.
+static int thin_qr(BlockWork &bw, const Geom &g, Field *Q, Field *R, CMat &C, int slot) {
+  CMat m, L, Cinv;
+  GX_TRY(bw.ipm(g, 1, R, R, m));
+  CMat h = m;
+  for (int i = 0; i < m.n; i++)
+    for (int j = 0; j < m.n; j++) h(i, j) = 0.5 * (m(i, j) + std::conj(m(j, i)));
+  if (!cm_chol(h, L)) {
+    set_error("BlockCGrQ: the residual block is rank deficient (Cholesky of R^dag R failed)");
+    return GX_ENOCONV;
+  }
+  C = cm_adj(L);
+  if (!cm_inv(C, Cinv)) {
+    set_error("BlockCGrQ: singular triangular factor");
+    return GX_ENOCONV;
+  }
+  return bw.madd(g, 1, Q, Cinv, R, nullptr, 1.0, slot);  // sliceMulMatrix(Q, Cinv, R)
+}
+
+// BlockCGrQ (Birk's thesis after Dubrulle 2001; BlockConjugateGradient.h:223-410) on HermOp =
+// MpcDagMpc of a multi-RHS Wilson operator; the block is its Ls right-hand sides (Grid's code pins
+// Nblock to 8 at :227; here Nblock = Ls).
+//   QC = R = B - A X (thin QR), D = Q;  for k:  Z = A D,  M = (D^dag Z)^-1,  X += D (M C),
+//   QS = Q - Z M (thin QR),  D = Q + D S^dag,  C = S C;  stop when max_b (C^dag C)_bb / |B_b|^2 < tol^2.
+static int block_cgrq_run(Op *op, Field *B, Field *X, double tol, int maxit, gx_solver_stats *st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const Geom &g = op->g;
+  const int n = g.Ls;
+  GX_TRY(op_prepare(op, GX_F64));
+  Field *R = op_tmp(op, GX_F64, 4), *Q = op_tmp(op, GX_F64, 5), *D = op_tmp(op, GX_F64, 6), *Z = op_tmp(op, GX_F64, 9);
+  if (!R || !Q || !D || !Z) return GX_ENOMEM;
+  X->cb = R->cb = Q->cb = D->cb = Z->cb = B->cb;
+  BlockWork bw;
+  GX_TRY(bw.init(op->ctx));
+  std::vector<double> ssq(n), xn(n), res(n);
+  GX_TRY(slice_reduce(op, 1, B, nullptr, nullptr, ssq.data()));
+  GX_TRY(slice_reduce(op, 1, X, nullptr, nullptr, xn.data()));
+  for (int b = 0; b < n; b++) {
+    if (std::isnan(ssq[b]) || std::isnan(xn[b])) {
+      set_error("BlockCGrQ: NaN in source or guess (right-hand side %d)", b);
+      return GX_ENAN;
+    }
+    if (ssq[b] == 0.0) {
+      set_error("BlockCGrQ: right-hand side %d is zero (the block must have full rank)", b);
+      return GX_EINVAL;
+    }
+  }
+  hipStream_t stream = op->ctx->stream;
+  GX_TRY(apply_mpcdagmpc(op, X, Z));
+  GX_TRY(blas_axpby(R, 1.0, B, -1.0, Z));  // R = B - A X
+  CMat C, S, M, DZ, rr;
+  GX_TRY(thin_qr(bw, g, Q, R, C, 3));
+  GX_CHECK_HIP(hipMemcpyAsync(D->data, Q->data, Q->bytes, hipMemcpyDeviceToDevice, stream));
+  int rc = GX_ENOCONV, k;
+  for (k = 1; k <= maxit; k++) {
+    GX_TRY(apply_mpcdagmpc(op, D, Z));  // Z = A D
+    GX_TRY(bw.ipm(g, 1, D, Z, DZ));
+    if (!cm_inv(DZ, M)) {
+      set_error("BlockCGrQ: D^dag A D is singular at iteration %d", k);
+      break;
+    }
+    GX_TRY(bw.madd(g, 1, X, cm_mul(M, C), D, X, 1.0, 1));  // X = X + D M C
+    GX_TRY(bw.madd(g, 1, R, M, Z, Q, -1.0, 2));            // R = Q - Z M
+    const int qr = thin_qr(bw, g, Q, R, S, 3);             // Q S = R
+    if (qr == GX_ENOCONV) break;
+    GX_TRY(qr);
+    GX_TRY(bw.madd(g, 1, D, cm_adj(S), D, Q, 1.0, 4));     // D = Q + D S^dag
+    C = cm_mul(S, C);
+    rr = cm_mul(cm_adj(C), C);
+    double mx = 0;
+    for (int b = 0; b < n; b++) mx = std::max(mx, rr(b, b).real() / ssq[b]);
+    if (mx < tol * tol) {
+      rc = GX_OK;
+      break;
+    }
+  }
+  // true residuals |B_b - A X_b| / |B_b|
+  GX_TRY(apply_mpcdagmpc(op, X, Z));
+  GX_TRY(slice_reduce(op, 3, Z, B, R, res.data()));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st)
+    for (int b = 0; b < n; b++) {
+      st[b] = gx_solver_stats{};
+      st[b].iterations = std::min(k, maxit);
+      st[b].converged = rc == GX_OK;
+      st[b].true_residual = std::sqrt(res[b] / ssq[b]);
+      st[b].computed_residual = rr.n ? std::sqrt(std::max(0.0, rr(b, b).real()) / ssq[b]) : 0.0;
+      st[b].seconds = secs;
+    }
+  return rc;
+}
+
 }  // namespace gx
 
 using namespace gx;
@@ -823,6 +1050,54 @@ int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, 
   if (inner_prec == GX_F32 || inner_prec == GX_F16) return mrhs_mixed_run(op, src, psi, tol, 1e-5, inner_prec, maxit, 100, st);
   set_error("block_cg: inner precision %d unsupported", inner_prec);
   return GX_EINVAL;
+}
+
+int gx_block_cgrq(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, gx_solver_stats *st) {
+  Op *op = op_;
+  Field *src = src_, *psi = psi_;
+  if (!op || !src || !psi || src == psi || op->kind != GX_WILSON || src->subset == GX_FULL ||
+      psi->subset == GX_FULL || src->prec != GX_F64 || psi->prec != GX_F64 || !same_geom(src->g, op->g) ||
+      !same_geom(psi->g, op->g)) {
+    set_error("block_cgrq: needs a multi-RHS Wilson operator and F64 half fields");
+    return GX_EINVAL;
+  }
+  return block_cgrq_run(op, src, psi, tol, maxit, st);
+}
+
+static bool slice_conformable(const Field *a, const Field *b) {
+  return a && b && a->prec == GX_F64 && b->prec == GX_F64 && same_geom(a->g, b->g) &&
+         (a->subset == GX_FULL) == (b->subset == GX_FULL);
+}
+
+int gx_slice_inner_product_matrix(gx_field lhs, gx_field rhs, double *mat) {
+  if (!slice_conformable(lhs, rhs) || !mat) {
+    set_error("slice_inner_product_matrix: needs two F64 fields of one lattice and subset");
+    return GX_EINVAL;
+  }
+  BlockWork bw;
+  GX_TRY(bw.init(lhs->ctx));
+  CMat m;
+  GX_TRY(bw.ipm(lhs->g, lhs->subset == GX_FULL ? 2 : 1, lhs, rhs, m));
+  for (int i = 0; i < m.n * m.n; i++) {
+    mat[2 * i] = m.a[i].real();
+    mat[2 * i + 1] = m.a[i].imag();
+  }
+  return GX_OK;
+}
+
+int gx_slice_madd_matrix(gx_field R, const double *aa, gx_field X, gx_field Y, double scale) {
+  if (!slice_conformable(R, X) || (Y && !slice_conformable(R, Y)) || !aa) {
+    set_error("slice_madd_matrix: needs F64 fields of one lattice and subset");
+    return GX_EINVAL;
+  }
+  const int n = X->g.Ls;
+  CMat a(n);
+  for (int i = 0; i < n * n; i++) a.a[i] = cd(aa[2 * i], aa[2 * i + 1]);
+  BlockWork bw;
+  GX_TRY(bw.init(X->ctx));
+  GX_TRY(bw.madd(X->g, X->subset == GX_FULL ? 2 : 1, R, a, X, Y, scale, 1));
+  GX_CHECK_HIP(hipStreamSynchronize(X->ctx->stream));  // the workspace is freed on return
+  return GX_OK;
 }
 
 int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxit, int mixed, gx_solver_stats *st,

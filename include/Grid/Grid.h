@@ -23,6 +23,7 @@
 #ifndef GRID_AMD_GRID_H
 #define GRID_AMD_GRID_H
 
+#include <algorithm>
 #include <cassert>
 #include <cmath>
 #include <complex>
@@ -810,6 +811,61 @@ class ConjugateGradientReliableUpdate {
     std::cout << GridLogMessage << "ConjugateGradientReliableUpdate Converged on iteration " << IterationsToComplete
               << " after " << ReliableUpdatesPerformed << " reliable updates; cleanup iterations "
               << IterationsToCleanup << "; true residual " << TrueResidual << std::endl;
+  }
+};
+
+// Grid: BlockConjugateGradient<Field>(BlockCGtype, Orthog, tol, maxit, err_on_no_conv),
+// BlockConjugateGradient.h:86-111,197-207.  The block is the fifth dimension (Orthog = 0) of a
+// multi-RHS Wilson operator's Schur system: CGmultiRHS runs gx_block_cg, BlockCGrQ gx_block_cgrq;
+// like Grid, the other types assert for a single field.  TrueResidual is Grid's block aggregate
+// sqrt(sum_b |A X_b - B_b|^2 / sum_b |B_b|^2).
+enum BlockCGtype { BlockCG, BlockCGrQ, CGmultiRHS, BlockCGVec, BlockCGrQVec };
+
+template <class Field>
+class BlockConjugateGradient : public OperatorFunction<Field> {
+ public:
+  int blockDim;
+  int Nblock = 0;
+  BlockCGtype CGtype;
+  bool ErrorOnNoConverge;
+  RealD Tolerance;
+  Integer MaxIterations;
+  Integer IterationsToComplete = 0;
+  Integer PrintInterval = 100;
+  RealD TrueResidual = 0;
+  BlockConjugateGradient(BlockCGtype cgtype, int _Orthog, RealD tol, Integer maxit, bool err_on_no_conv = true)
+      : blockDim(_Orthog), CGtype(cgtype), ErrorOnNoConverge(err_on_no_conv), Tolerance(tol), MaxIterations(maxit) {}
+
+  void operator()(LinearOperatorBase<Field> &Linop, const Field &Src, Field &Psi) override {
+    gx_op op = Linop.schur_handle();
+    assert(op && "BlockConjugateGradient runs on a Schur operator over a multi-RHS WilsonFermion");
+    assert(blockDim == 0 && "the block direction is the fifth dimension (Orthog = 0)");
+    assert((CGtype == BlockCGrQ || CGtype == CGmultiRHS) && "single-field BlockConjugateGradient: BlockCGrQ or CGmultiRHS");
+    Nblock = Src.Grid()->Ls;
+    Psi.Checkerboard() = Src.Checkerboard();
+    std::vector<gx_solver_stats> st(Nblock);
+    const int rc = CGtype == BlockCGrQ ? gx_block_cgrq(op, Src.f, Psi.f, Tolerance, MaxIterations, st.data())
+                                       : gx_block_cg(op, Src.f, Psi.f, Tolerance, MaxIterations, GX_F64, st.data());
+    if (rc != GX_ENOCONV) detail::check(rc, "BlockConjugateGradient");
+    std::vector<double> m(2 * (size_t)Nblock * Nblock);
+    detail::check(gx_slice_inner_product_matrix(Src.f, Src.f, m.data()), "sliceInnerProductMatrix");
+    double rr = 0, ss = 0;
+    IterationsToComplete = 0;
+    for (int b = 0; b < Nblock; b++) {
+      const double nb = m[2 * ((size_t)b * Nblock + b)];
+      rr += st[b].true_residual * st[b].true_residual * nb;
+      ss += nb;
+      IterationsToComplete = std::max<Integer>(IterationsToComplete, st[b].iterations);
+    }
+    TrueResidual = ss > 0 ? std::sqrt(rr / ss) : 0.0;
+    if (rc == GX_ENOCONV) {
+      std::cout << GridLogMessage << "BlockConjugateGradient did NOT converge " << IterationsToComplete << " / "
+                << MaxIterations << std::endl;
+      if (ErrorOnNoConverge) assert(0);
+      return;
+    }
+    std::cout << GridLogMessage << (CGtype == BlockCGrQ ? "BlockCGrQ" : "MultiRHS solver") << " converged in "
+              << IterationsToComplete << " iterations; true residual " << TrueResidual << std::endl;
   }
 };
 

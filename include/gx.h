@@ -219,6 +219,25 @@ int gx_schur_solve(gx_op op, gx_field src, gx_field sol, double tol, int maxit, 
 int gx_block_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, int inner_prec,
                 gx_solver_stats *st);
 
+/* Grid: BlockConjugateGradient(BlockCGrQ, Orthog = 0, tol, maxit)(Linop, src, psi),
+ * BlockConjugateGradient.h:223-410: block CG on MpcDagMpc of a multi-RHS Wilson operator.  It
+ * uses one block Krylov space for all Ls right-hand sides, with a thin QR of the residual block
+ * (Cholesky of R^dag R) each iteration, and stops when the largest relative residual is below tol.
+ * Nblock = Ls.  Grid's code fixes Nblock = 8 at :227, which is the same thing for Ls = 8.  Every
+ * right-hand side must be nonzero, and the block must keep full rank.  src, psi: F64 half fields;
+ * st: Ls statistics with per-right-hand-side true residuals.  FP64 only. */
+int gx_block_cgrq(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver_stats *st);
+
+/* Grid: sliceInnerProductMatrix(mat, lhs, rhs, Orthog = 0), Lattice_reduction.h:757-776:
+ * mat(s, t) = <lhs_s, rhs_t> over fifth-dimension slices s, t.  mat: 2 Ls^2 doubles (re, im),
+ * row-major.  F64 fields of one lattice and subset. */
+int gx_slice_inner_product_matrix(gx_field lhs, gx_field rhs, double *mat);
+
+/* Grid: sliceMaddMatrix(R, aa, X, Y, Orthog = 0, scale), Lattice_reduction.h:717-742:
+ * R_t = Y_t + scale sum_s X_s aa(s, t).  aa: 2 Ls^2 doubles, row-major.  Y = NULL gives
+ * sliceMulMatrix (R_t = scale sum_s X_s aa(s, t)).  R may be X or Y. */
+int gx_slice_madd_matrix(gx_field R, const double *aa, gx_field X, gx_field Y, double scale);
+
 /* ---- solve serving (no Grid counterpart) ----
  * A request asks for the Wilson solves M x_k = b_k, k < nsrc, on one of the resident gauge fields
  * of its lattice, with Gaussian sources b (gx_field_gaussian, Ls = nsrc, the request's seed).
