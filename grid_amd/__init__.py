@@ -125,6 +125,7 @@ _SIG = {
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
     'gx_block_cg': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_block_cgrq': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_cg_multishift': ([_P, _P, _I, _DP, _DP, ctypes.POINTER(_P), _I, ctypes.POINTER(SolverStats)], _I),
     'gx_slice_inner_product_matrix': ([_P, _P, _DP], _I),
     'gx_slice_madd_matrix': ([_P, _DP, _P, _P, _D], _I),
     'gx_reliable_update_cg': ([_P, _P, _P, _D, _I, _D, _I, ctypes.POINTER(SolverStats)], _I),
@@ -448,6 +449,20 @@ class FermionOp:
         """Multi-RHS CG (right-hand side k = slice s = k); returns one stats dict per RHS."""
         st = (SolverStats * self.Ls)()
         rc = lib().gx_block_cg(self.h, src.h, psi.h, tol, maxit, inner_prec, st)
+        if rc not in (0, 3):
+            _check(rc)
+        return [s.as_dict() for s in st]
+
+    def cg_multishift(self, src, psis, shifts, tolerances, maxit=10000):
+        """ConjugateGradientMultiShift: (MpcDagMpc + shifts[s]) psis[s] = src for every shift at once;
+        shifts[0] must be the smallest.  Returns one stats dict per shift."""
+        n = len(shifts)
+        assert len(psis) == n and len(tolerances) == n
+        sh = np.ascontiguousarray(shifts, dtype=np.float64)
+        tl = np.ascontiguousarray(tolerances, dtype=np.float64)
+        hs = (_P * n)(*[f.h for f in psis])
+        st = (SolverStats * n)()
+        rc = lib().gx_cg_multishift(self.h, src.h, n, _dp(sh), _dp(tl), hs, maxit, st)
         if rc not in (0, 3):
             _check(rc)
         return [s.as_dict() for s in st]

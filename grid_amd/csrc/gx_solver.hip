@@ -965,6 +965,143 @@ static int block_cgrq_run(Op *op, Field *B, Field *X, double tol, int maxit, gx_
   return rc;
 }
 
+// ConjugateGradientMultiShift (ConjugateGradientMultiShift.h:89-343): (MpcDagMpc + m_s) psi_s = src
+// for all shifts m_s >= m_0 from the Krylov space of the lightest (Jegerlehner's shifted CG).  The
+// scalar recurrences (a, b, z_s, bs_s) run on the host in FP64, in Grid's order; psi_s start at
+// zero; a shift stops updating once c z_s^2 < m_residual_s^2 |src|^2.  FP64 fields.
+static int multishift_run(Op *op, Field *src, int nshift, const double *mass, const double *mres, Field **psi,
+                          int maxit, gx_solver_stats *st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  GX_TRY(op_prepare(op, GX_F64));
+  Field *r = op_tmp(op, GX_F64, 4), *p = op_tmp(op, GX_F64, 5), *mmp = op_tmp(op, GX_F64, 6),
+        *tmp = op_tmp(op, GX_F64, 9);
+  if (!r || !p || !mmp || !tmp) return GX_ENOMEM;
+  const int cb = src->cb;
+  r->cb = p->cb = mmp->cb = tmp->cb = cb;
+  std::vector<Field *> ps(nshift, nullptr);
+  struct Owned {
+    std::vector<Field *> &v;
+    ~Owned() {
+      for (Field *f : v)
+        if (f) field_destroy(f);
+    }
+  } owned{ps};
+  for (int s = 0; s < nshift; s++) {
+    GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_EVEN, &ps[s]));
+    ps[s]->cb = cb;
+    psi[s]->cb = cb;
+  }
+  hipStream_t stream = op->ctx->stream;
+  std::vector<double> bs(nshift), rsq(nshift), zc(nshift), zp(nshift);  // z[s][iz], z[s][1-iz]
+  std::vector<int> conv(nshift, 0), its(nshift, 0);
+  double cp;
+  GX_TRY(blas_norm2(src, &cp));
+  if (std::isnan(cp)) {
+    set_error("multishift CG: NaN in source");
+    return GX_ENAN;
+  }
+  if (cp == 0.0) {
+    for (int s = 0; s < nshift; s++) {
+      GX_CHECK_HIP(hipMemsetAsync(psi[s]->data, 0, psi[s]->bytes, stream));
+      if (st) {
+        st[s] = gx_solver_stats{};
+        st[s].iterations = 1;
+        st[s].converged = 1;
+      }
+    }
+    GX_CHECK_HIP(hipStreamSynchronize(stream));
+    return GX_OK;
+  }
+  for (int s = 0; s < nshift; s++) {
+    rsq[s] = cp * mres[s] * mres[s];
+    GX_CHECK_HIP(hipMemcpyAsync(ps[s]->data, src->data, src->bytes, hipMemcpyDeviceToDevice, stream));
+  }
+  GX_CHECK_HIP(hipMemcpyAsync(r->data, src->data, src->bytes, hipMemcpyDeviceToDevice, stream));
+  GX_CHECK_HIP(hipMemcpyAsync(p->data, src->data, src->bytes, hipMemcpyDeviceToDevice, stream));
+  // d = Re<p, (A + m_0) p>, b = -cp / d
+  double d, rn, c;
+  auto shifted_op = [&](double &dd) -> int {
+    GX_TRY(apply_mpcdagmpc(op, p, mmp));
+    GX_TRY(blas_inner(p, mmp, &dd, nullptr));
+    GX_TRY(blas_axpby(mmp, mass[0], p, 1.0, mmp));
+    GX_TRY(blas_norm2(p, &rn));
+    dd += rn * mass[0];
+    return GX_OK;
+  };
+  GX_TRY(shifted_op(d));
+  double b = -cp / d;
+  zp[0] = zc[0] = 1.0;
+  bs[0] = b;
+  for (int s = 1; s < nshift; s++) {
+    zp[s] = 1.0;
+    zc[s] = 1.0 / (1.0 - b * (mass[s] - mass[0]));
+    bs[s] = b * zc[s];
+  }
+  GX_TRY(blas_axpy_norm(r, b, mmp, r, &c));
+  for (int s = 0; s < nshift; s++) GX_TRY(blas_axpby(psi[s], -bs[s], src, 0.0, src));
+  int k, rc = GX_ENOCONV;
+  for (k = 1; k <= maxit; k++) {
+    const double a = c / cp;
+    GX_TRY(blas_axpby(p, a, p, 1.0, r));  // p = a p + r
+    for (int s = 0; s < nshift; s++) {
+      if (conv[s]) continue;
+      if (s == 0) {
+        GX_TRY(blas_axpby(ps[0], a, ps[0], 1.0, r));
+      } else {
+        const double as = a * zc[s] * bs[s] / (zp[s] * b);
+        GX_TRY(blas_axpby(ps[s], zc[s], r, as, ps[s]));
+      }
+    }
+    cp = c;
+    GX_TRY(shifted_op(d));
+    const double bp = b;
+    b = -cp / d;
+    GX_TRY(blas_axpy_norm(r, b, mmp, r, &c));
+    bs[0] = b;
+    for (int s = 1; s < nshift; s++) {  // toggling z[s][iz] <-> z[s][1-iz]: the new z1 is the old z0
+      if (conv[s]) continue;
+      const double z0 = zc[s], z1 = zp[s];
+      const double zn = z0 * z1 * bp / (b * a * (z1 - z0) + z1 * bp * (1.0 - (mass[s] - mass[0]) * b));
+      zp[s] = z0;
+      zc[s] = zn;
+      bs[s] = b * zn / z0;
+    }
+    for (int s = 0; s < nshift; s++)
+      if (!conv[s]) GX_TRY(blas_axpby(psi[s], -bs[s], ps[s], 1.0, psi[s]));
+    bool all = true;
+    for (int s = 0; s < nshift; s++) {
+      if (conv[s]) continue;
+      its[s] = k;
+      const double css = c * zc[s] * zc[s];
+      if (css < rsq[s]) conv[s] = 1;
+      else all = false;
+    }
+    if (all) {
+      rc = GX_OK;
+      break;
+    }
+  }
+  // true residuals |(A + m_s) psi_s - src| / |src|
+  double ssq;
+  GX_TRY(blas_norm2(src, &ssq));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int s = 0; s < nshift; s++) {
+    double rr;
+    GX_TRY(apply_mpcdagmpc(op, psi[s], mmp));
+    GX_TRY(blas_axpby(tmp, mass[s], psi[s], 1.0, mmp));
+    GX_TRY(blas_axpy_norm(r, -1.0, src, tmp, &rr));
+    if (st) {
+      st[s] = gx_solver_stats{};
+      st[s].iterations = its[s];
+      st[s].converged = conv[s];
+      st[s].true_residual = std::sqrt(rr / ssq);
+      st[s].computed_residual = std::sqrt(std::max(0.0, c * zc[s] * zc[s]) / ssq);
+      st[s].seconds = secs;
+    }
+  }
+  return rc;
+}
+
 }  // namespace gx
 
 using namespace gx;
@@ -1050,6 +1187,36 @@ int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, 
   if (inner_prec == GX_F32 || inner_prec == GX_F16) return mrhs_mixed_run(op, src, psi, tol, 1e-5, inner_prec, maxit, 100, st);
   set_error("block_cg: inner precision %d unsupported", inner_prec);
   return GX_EINVAL;
+}
+
+int gx_cg_multishift(gx_op op_, gx_field src_, int nshift, const double *shifts, const double *tolerances,
+                     gx_field *psi_, int maxit, gx_solver_stats *st) {
+  Op *op = op_;
+  Field *src = src_;
+  if (!op || !src || src->prec != GX_F64 || src->subset == GX_FULL || !same_geom(src->g, op->g) || nshift < 1 ||
+      nshift > 64 || !shifts || !tolerances || !psi_) {
+    set_error("cg_multishift: needs an F64 half source on the operator's lattice and 1..64 shifts");
+    return GX_EINVAL;
+  }
+  std::vector<Field *> psi(nshift);
+  for (int s = 0; s < nshift; s++) {
+    psi[s] = psi_[s];
+    if (!psi[s] || psi[s] == src || psi[s]->prec != GX_F64 || psi[s]->subset == GX_FULL ||
+        !same_geom(psi[s]->g, op->g)) {
+      set_error("cg_multishift: solution %d must be an F64 half field on the operator's lattice", s);
+      return GX_EINVAL;
+    }
+    for (int t = 0; t < s; t++)
+      if (psi[t] == psi[s]) {
+        set_error("cg_multishift: solutions %d and %d are the same field", t, s);
+        return GX_EINVAL;
+      }
+    if (!(shifts[s] >= shifts[0]) || !(tolerances[s] > 0.0)) {
+      set_error("cg_multishift: shift %d must be >= shift 0 (the lightest) with a positive tolerance", s);
+      return GX_EINVAL;
+    }
+  }
+  return multishift_run(op, src, nshift, shifts, tolerances, psi.data(), maxit, st);
 }
 
 int gx_block_cgrq(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, gx_solver_stats *st) {

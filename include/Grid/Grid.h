@@ -814,6 +814,84 @@ class ConjugateGradientReliableUpdate {
   }
 };
 
+// Grid: MultiShiftFunction (Grid/algorithms/approx/MultiShiftFunction.h:33-62) without the Remez
+// initialisation (AlgRemez needs GMP): the caller fills poles, residues, tolerances and norm.  The
+// (n, lo, hi) constructor also sets order = n, which Grid leaves to the caller.
+class MultiShiftFunction {
+ public:
+  int order = 0;
+  std::vector<RealD> poles, residues, tolerances;
+  RealD norm = 0, lo = 0, hi = 0;
+  MultiShiftFunction() = default;
+  MultiShiftFunction(int n, RealD _lo, RealD _hi)
+      : order(n), poles(n), residues(n), tolerances(n), lo(_lo), hi(_hi) {}
+  // norm + sum_i residues[i] / (x + poles[i])
+  RealD approx(RealD x) const {
+    RealD a = norm;
+    for (int i = 0; i < order; i++) a += residues[i] / (x + poles[i]);
+    return a;
+  }
+};
+
+// Grid: ConjugateGradientMultiShift<Field>(maxit, shifts), ConjugateGradientMultiShift.h:40-343.  On
+// a Schur operator over a libgx action: gx_cg_multishift for (HermOp + poles[s]) psi_s = src.  Like
+// Grid it prints, but does not assert, when a shift fails to converge.
+template <class Field>
+class ConjugateGradientMultiShift : public OperatorFunction<Field> {
+ public:
+  Integer MaxIterations;
+  Integer IterationsToComplete = 0;
+  std::vector<int> IterationsToCompleteShift;
+  int verbose = 1;
+  MultiShiftFunction shifts;
+  std::vector<RealD> TrueResidualShift;
+  ConjugateGradientMultiShift(Integer maxit, const MultiShiftFunction &_shifts)
+      : MaxIterations(maxit), shifts(_shifts) {
+    IterationsToCompleteShift.resize(shifts.order);
+    TrueResidualShift.resize(shifts.order);
+  }
+  // psi = norm src + sum_s residues[s] (HermOp + poles[s])^-1 src
+  void operator()(LinearOperatorBase<Field> &Linop, const Field &src, Field &psi) override {
+    std::vector<Field> results;
+    results.reserve(shifts.order);
+    for (int s = 0; s < shifts.order; s++) results.emplace_back(src.Grid());
+    (*this)(Linop, src, results, psi);
+  }
+  void operator()(LinearOperatorBase<Field> &Linop, const Field &src, std::vector<Field> &results, Field &psi) {
+    (*this)(Linop, src, results);
+    psi.Checkerboard() = src.Checkerboard();
+    psi = shifts.norm * src;
+    for (int s = 0; s < shifts.order; s++) psi = psi + shifts.residues[s] * results[s];
+  }
+  void operator()(LinearOperatorBase<Field> &Linop, const Field &src, std::vector<Field> &psi) {
+    gx_op op = Linop.schur_handle();
+    assert(op && "ConjugateGradientMultiShift runs on a Schur operator over a libgx action");
+    const int n = shifts.order;
+    assert((int)psi.size() == n && (int)shifts.poles.size() == n && (int)shifts.tolerances.size() == n);
+    std::vector<gx_field> h(n);
+    for (int s = 0; s < n; s++) {
+      psi[s].Checkerboard() = src.Checkerboard();
+      h[s] = psi[s].f;
+    }
+    std::vector<gx_solver_stats> st(n);
+    const int rc = gx_cg_multishift(op, src.f, n, shifts.poles.data(), shifts.tolerances.data(), h.data(),
+                                    MaxIterations, st.data());
+    if (rc != GX_ENOCONV) detail::check(rc, "ConjugateGradientMultiShift");
+    IterationsToCompleteShift.assign(n, 0);
+    TrueResidualShift.assign(n, 0.0);
+    IterationsToComplete = 0;
+    for (int s = 0; s < n; s++) {
+      IterationsToCompleteShift[s] = st[s].iterations;
+      TrueResidualShift[s] = st[s].true_residual;
+      IterationsToComplete = std::max<Integer>(IterationsToComplete, st[s].iterations);
+      if (verbose)
+        std::cout << GridLogMessage << "CGMultiShift: shift[" << s << "] true residual " << TrueResidualShift[s]
+                  << " after " << st[s].iterations << " iterations" << std::endl;
+    }
+    if (rc == GX_ENOCONV) std::cout << GridLogMessage << "CG multi shift did not converge" << std::endl;
+  }
+};
+
 // Grid: BlockConjugateGradient<Field>(BlockCGtype, Orthog, tol, maxit, err_on_no_conv),
 // BlockConjugateGradient.h:86-111,197-207.  The block is the fifth dimension (Orthog = 0) of a
 // multi-RHS Wilson operator's Schur system: CGmultiRHS runs gx_block_cg, BlockCGrQ gx_block_cgrq;

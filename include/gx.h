@@ -228,6 +228,16 @@ int gx_block_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, int
  * st: Ls statistics with per-right-hand-side true residuals.  FP64 only. */
 int gx_block_cgrq(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver_stats *st);
 
+/* Grid: ConjugateGradientMultiShift<Field>(maxit, MultiShiftFunction)(Linop, src, results),
+ * ConjugateGradientMultiShift.h:89-343: solves (MpcDagMpc + shifts[s]) psi[s] = src for nshift
+ * shifts at once (poles = shifts, tolerances = per-shift relative residuals), from the Krylov
+ * space of shifts[0], which must be the smallest.  Guesses are ignored (zero), as in Grid.
+ * psi: nshift distinct F64 half fields; st: nshift statistics (iterations until each shift
+ * converged, true residual |(A + shift) psi - src| / |src|).  GX_ENOCONV when a shift is not
+ * converged after maxit iterations. */
+int gx_cg_multishift(gx_op op, gx_field src, int nshift, const double *shifts, const double *tolerances,
+                     gx_field *psi, int maxit, gx_solver_stats *st);
+
 /* Grid: sliceInnerProductMatrix(mat, lhs, rhs, Orthog = 0), Lattice_reduction.h:757-776:
  * mat(s, t) = <lhs_s, rhs_t> over fifth-dimension slices s, t.  mat: 2 Ls^2 doubles (re, im),
  * row-major.  F64 fields of one lattice and subset. */
