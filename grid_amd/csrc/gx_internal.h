@@ -237,6 +237,19 @@ int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y
 template <typename T>
 int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state);
 
+// Multishift CG vector updates over n stored complex numbers of FP64 fields (one pass over r):
+// bit s of do_psi: psi_s -= bs_s ps_s; then bit s of do_ps: ps_s = z_s r + as_s ps_s; do_p: p = a p + r.
+constexpr int kMaxShift = 32;
+struct MultiShiftArgs {
+  void *ps[kMaxShift];
+  void *psi[kMaxShift];
+  double bs[kMaxShift], z[kMaxShift], as[kMaxShift];
+  double a;
+  uint32_t do_psi, do_ps;
+  int nshift, do_p;
+};
+int launch_multishift(Ctx *ctx, int64_t n, const void *r, void *p, const MultiShiftArgs &m);
+
 // Slice matrices of FP64 fields with nhalf (1 or 2) checkerboard halves (sliceInnerProductMatrix,
 // sliceMaddMatrix).  ipm: out_dev[s * Ls + t] = sum conj(x_s) y_t (double2), through nparts
 // workgroup partials of Ls^2 double2 each.  madd: r_t = y_t + scale sum_s x_s a(s, t), a_dev

@@ -1395,6 +1395,36 @@ static __global__ void __launch_bounds__(256) k_digest(const uint64_t *__restric
   if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+// ---------------- multishift CG vector updates (ConjugateGradientMultiShift.h:236-300) ------------
+// One pass over r for every shift (the fusion the reference's comments propose): shift s with bit s
+// of do_psi set takes psi_s -= bs_s ps_s (closing its iteration k); with bit s of do_ps set, then
+// ps_s = z_s r + as_s ps_s (opening iteration k + 1); with do_p, p = a p + r.
+static __global__ void __launch_bounds__(256) k_multishift(int64_t n, const cx<double> *__restrict__ r,
+                                                           cx<double> *__restrict__ p, MultiShiftArgs m) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const cx<double> rv = r[i];
+    if (m.do_p) {
+      const cx<double> pv = p[i];
+      p[i] = {fma(m.a, pv.r, rv.r), fma(m.a, pv.i, rv.i)};
+    }
+    for (int s = 0; s < m.nshift; s++) {
+      const bool dpsi = (m.do_psi >> s) & 1u, dps = (m.do_ps >> s) & 1u;
+      if (!dpsi && !dps) continue;
+      cx<double> *ps = (cx<double> *)m.ps[s];
+      cx<double> pv = ps[i];
+      if (dpsi) {
+        cx<double> *psi = (cx<double> *)m.psi[s];
+        const cx<double> xv = psi[i];
+        psi[i] = {fma(-m.bs[s], pv.r, xv.r), fma(-m.bs[s], pv.i, xv.i)};
+      }
+      if (dps) {
+        pv = {fma(m.z[s], rv.r, m.as[s] * pv.r), fma(m.z[s], rv.i, m.as[s] * pv.i)};
+        ps[i] = pv;
+      }
+    }
+  }
+}
+
 // ---------------- slice matrices (Lattice_reduction.h:717-776) ----------------
 // FP64 fields viewed as (rows) x Ls with the fifth-dimension slice s innermost: entry e = h Ls + s
 // of half hf stores component k at hf * hstride + ((e / kBlk) * 12 + k) * kBlk + e % kBlk.
@@ -1515,6 +1545,17 @@ int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const vo
                      (const cx<double> *)x, (const cx<double> *)y, (double2 *)partial);
   hipLaunchKernelGGL(k_slice_ipm_fin, dim3(1), dim3(256), 0, ctx->stream, (const double2 *)partial, (int)nb,
                      g.Ls * g.Ls, (double2 *)out_dev);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int launch_multishift(Ctx *ctx, int64_t n, const void *r, void *p, const MultiShiftArgs &m) {
+  if (m.nshift < 1 || m.nshift > kMaxShift) {
+    set_error("multishift update: %d shifts (1..%d)", m.nshift, kMaxShift);
+    return GX_EINVAL;
+  }
+  hipLaunchKernelGGL(k_multishift, dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, (const cx<double> *)r,
+                     (cx<double> *)p, m);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }

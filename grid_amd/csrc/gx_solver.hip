@@ -1039,19 +1039,33 @@ static int multishift_run(Op *op, Field *src, int nshift, const double *mass, co
   }
   GX_TRY(blas_axpy_norm(r, b, mmp, r, &c));
   for (int s = 0; s < nshift; s++) GX_TRY(blas_axpby(psi[s], -bs[s], src, 0.0, src));
+  // The vector updates of every shift run in one pass over r (k_multishift): the one that closes
+  // iteration k (psi_s -= bs_s ps_s) is fused with the one that opens k + 1 (p = a p + r,
+  // ps_s = z_s r + as_s ps_s), with Grid's coefficients.
+  MultiShiftArgs m{};
+  m.nshift = nshift;
+  for (int s = 0; s < nshift; s++) {
+    m.ps[s] = ps[s]->data;
+    m.psi[s] = psi[s]->data;
+  }
+  const int64_t nst = src->stored();
+  auto update = [&](uint32_t close, uint32_t open, double an) -> int {
+    m.do_psi = close;
+    m.do_ps = open;
+    m.do_p = open != 0;
+    m.a = an;
+    for (int s = 0; s < nshift; s++) {
+      m.bs[s] = bs[s];
+      m.z[s] = s == 0 ? 1.0 : zc[s];
+      m.as[s] = s == 0 ? an : an * zc[s] * bs[s] / (zp[s] * b);
+    }
+    return launch_multishift(op->ctx, nst, r->data, p->data, m);
+  };
+  uint32_t active = nshift == 32 ? 0xffffffffu : ((1u << nshift) - 1u);
+  double a = c / cp;  // the a of the iteration being opened
+  GX_TRY(update(0u, active, a));  // iteration 1: p and ps_s
   int k, rc = GX_ENOCONV;
   for (k = 1; k <= maxit; k++) {
-    const double a = c / cp;
-    GX_TRY(blas_axpby(p, a, p, 1.0, r));  // p = a p + r
-    for (int s = 0; s < nshift; s++) {
-      if (conv[s]) continue;
-      if (s == 0) {
-        GX_TRY(blas_axpby(ps[0], a, ps[0], 1.0, r));
-      } else {
-        const double as = a * zc[s] * bs[s] / (zp[s] * b);
-        GX_TRY(blas_axpby(ps[s], zc[s], r, as, ps[s]));
-      }
-    }
     cp = c;
     GX_TRY(shifted_op(d));
     const double bp = b;
@@ -1066,17 +1080,18 @@ static int multishift_run(Op *op, Field *src, int nshift, const double *mass, co
       zc[s] = zn;
       bs[s] = b * zn / z0;
     }
-    for (int s = 0; s < nshift; s++)
-      if (!conv[s]) GX_TRY(blas_axpby(psi[s], -bs[s], ps[s], 1.0, psi[s]));
-    bool all = true;
+    const uint32_t closing = active;
     for (int s = 0; s < nshift; s++) {
       if (conv[s]) continue;
       its[s] = k;
-      const double css = c * zc[s] * zc[s];
-      if (css < rsq[s]) conv[s] = 1;
-      else all = false;
+      if (c * zc[s] * zc[s] < rsq[s]) {
+        conv[s] = 1;
+        active &= ~(1u << s);
+      }
     }
-    if (all) {
+    a = c / cp;
+    GX_TRY(update(closing, active, a));
+    if (!active) {
       rc = GX_OK;
       break;
     }
@@ -1194,8 +1209,8 @@ int gx_cg_multishift(gx_op op_, gx_field src_, int nshift, const double *shifts,
   Op *op = op_;
   Field *src = src_;
   if (!op || !src || src->prec != GX_F64 || src->subset == GX_FULL || !same_geom(src->g, op->g) || nshift < 1 ||
-      nshift > 64 || !shifts || !tolerances || !psi_) {
-    set_error("cg_multishift: needs an F64 half source on the operator's lattice and 1..64 shifts");
+      nshift > kMaxShift || !shifts || !tolerances || !psi_) {
+    set_error("cg_multishift: needs an F64 half source on the operator's lattice and 1..%d shifts", kMaxShift);
     return GX_EINVAL;
   }
   std::vector<Field *> psi(nshift);

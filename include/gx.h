@@ -232,7 +232,7 @@ int gx_block_cgrq(gx_op op, gx_field src, gx_field psi, double tol, int maxit, g
  * ConjugateGradientMultiShift.h:89-343: solves (MpcDagMpc + shifts[s]) psi[s] = src for nshift
  * shifts at once (poles = shifts, tolerances = per-shift relative residuals), from the Krylov
  * space of shifts[0], which must be the smallest.  Guesses are ignored (zero), as in Grid.
- * psi: nshift distinct F64 half fields; st: nshift statistics (iterations until each shift
+ * 1 <= nshift <= 32.  psi: nshift distinct F64 half fields; st: nshift statistics (iterations until each shift
  * converged, true residual |(A + shift) psi - src| / |src|).  GX_ENOCONV when a shift is not
  * converged after maxit iterations. */
 int gx_cg_multishift(gx_op op, gx_field src, int nshift, const double *shifts, const double *tolerances,
