@@ -39,6 +39,22 @@ class SolverStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class NerscInfo(ctypes.Structure):
+    _fields_ = [('dims', ctypes.c_int * 4), ('two_row', ctypes.c_int), ('checksum_header', ctypes.c_uint32),
+                ('checksum_data', ctypes.c_uint32), ('plaquette_header', ctypes.c_double),
+                ('plaquette', ctypes.c_double), ('link_trace_header', ctypes.c_double),
+                ('link_trace', ctypes.c_double), ('sequence_number', ctypes.c_int),
+                ('floating_point', ctypes.c_char * 16), ('ensemble_id', ctypes.c_char * 64),
+                ('ensemble_label', ctypes.c_char * 64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d['dims'] = tuple(self.dims)
+        for k in ('floating_point', 'ensemble_id', 'ensemble_label'):
+            d[k] = d[k].decode()
+        return d
+
+
 class Request(ctypes.Structure):
     _fields_ = [('dims', ctypes.c_int * 4), ('nsrc', ctypes.c_int), ('mass', ctypes.c_double),
                 ('gauge', ctypes.c_int), ('seed', ctypes.c_uint64), ('arrival', ctypes.c_double)]
@@ -125,6 +141,9 @@ _SIG = {
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
     'gx_block_cg': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_block_cgrq': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_gauge_statistics': ([_P, _IP, _DP, _DP, _DP], _I),
+    'gx_nersc_read': ([_P, ctypes.c_char_p, _IP, _DP, ctypes.POINTER(NerscInfo)], _I),
+    'gx_nersc_write': ([_P, ctypes.c_char_p, _IP, _DP, _I, ctypes.c_char_p, ctypes.c_char_p, _I], _I),
     'gx_cg_multishift': ([_P, _P, _I, _DP, _DP, ctypes.POINTER(_P), _I, ctypes.POINTER(SolverStats)], _I),
     'gx_slice_inner_product_matrix': ([_P, _P, _DP], _I),
     'gx_slice_madd_matrix': ([_P, _DP, _P, _P, _D], _I),
@@ -317,6 +336,35 @@ def tune_get(knob):
 
 def precision_change(out, inp):
     _check(lib().gx_precision_change(out.h, inp.h))
+
+
+def gauge_statistics(ctx, U, dims):
+    """(avgPlaquette, linkTrace) of a host gauge field, computed on the device."""
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    p, t = _D(), _D()
+    _check(lib().gx_gauge_statistics(ctx.h, _dims(dims), _dp(U), ctypes.byref(p), ctypes.byref(t)))
+    return p.value, t.value
+
+
+def nersc_read(ctx, path, dims, check=True):
+    """NerscIO::readConfiguration: returns (U, header info).  With check=False a failed checksum,
+    plaquette or link-trace comparison returns (U, info) with info['ok'] = False instead of raising."""
+    lx, ly, lz, lt = dims
+    U = np.zeros((lt, lz, ly, lx, 4, 3, 3), dtype=np.complex128)
+    info = NerscInfo()
+    rc = lib().gx_nersc_read(ctx.h, os.fsencode(path), _dims(dims), _dp(U), ctypes.byref(info))
+    if check:
+        _check(rc)
+    d = info.as_dict()
+    d['ok'] = rc == 0
+    return U, d
+
+
+def nersc_write(ctx, path, U, dims, two_row=False, ensemble_label='DWF', ensemble_id='UKQCD', sequence_number=1):
+    """NerscIO::writeConfiguration (IEEE64BIG)."""
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    _check(lib().gx_nersc_write(ctx.h, os.fsencode(path), _dims(dims), _dp(U), int(two_row),
+                                ensemble_label.encode(), ensemble_id.encode(), sequence_number))
 
 
 def slice_inner_product_matrix(lhs, rhs):

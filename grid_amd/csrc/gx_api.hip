@@ -806,6 +806,38 @@ int gx_ctx_destroy(gx_ctx c) {
   return GX_OK;
 }
 
+// WilsonLoops<Gimpl>::avgPlaquette / linkTrace (WilsonLoops.h:121-126,203-219) of a host gauge field
+int gx_gauge_statistics(gx_ctx c, const int dims[4], const double *U, double *plaquette, double *link_trace) {
+  GX_REQUIRE(c && dims && U, GX_EINVAL, "gauge_statistics: null argument");
+  for (int d = 0; d < 4; d++) GX_REQUIRE(dims[d] >= 1, GX_EINVAL, "gauge_statistics: bad extent %d", dims[d]);
+  const int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+  const size_t ub = (size_t)V * 4 * 9 * 2 * sizeof(double);
+  const int nparts = 1024;
+  void *Ud = nullptr;
+  double *part = nullptr;
+  hipError_t e = hipMalloc(&Ud, ub);
+  if (e == hipSuccess) e = hipMalloc((void **)&part, sizeof(double) * 2 * nparts);
+  if (e != hipSuccess) {
+    if (Ud) (void)hipFree(Ud);
+    set_error("gauge_statistics: %s", hipGetErrorString(e));
+    return GX_ENOMEM;
+  }
+  std::vector<double> host(2 * nparts);
+  int rc = GX_OK;
+  e = hipMemcpyAsync(Ud, U, ub, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) {
+    set_error("gauge_statistics: %s", hipGetErrorString(e));
+    rc = GX_EHIP;
+  }
+  if (rc == GX_OK) rc = launch_gauge_stats(c, dims, Ud, part, nparts, host.data());
+  (void)hipFree(Ud);
+  (void)hipFree(part);
+  if (rc != GX_OK) return rc;
+  if (plaquette) *plaquette = host[0] / (double)V / 6.0 / 3.0;
+  if (link_trace) *link_trace = host[1] / (double)V / 12.0;
+  return GX_OK;
+}
+
 int gx_ctx_sync(gx_ctx c) {
   GX_REQUIRE(c, GX_EINVAL, "null ctx");
   GX_CHECK_HIP(hipStreamSynchronize(c->stream));

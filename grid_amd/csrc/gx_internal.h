@@ -249,6 +249,11 @@ struct MultiShiftArgs {
   int nshift, do_p;
 };
 int launch_multishift(Ctx *ctx, int64_t n, const void *r, void *p, const MultiShiftArgs &m);
+// Sums over the lattice of the six plaquettes' and four links' real traces of a device gauge field
+// ([site][mu][3][3] complex double): out_host[0] (plaquettes), out_host[1] (links); out_host holds
+// 2 nparts doubles.  Synchronises the stream.
+int launch_gauge_stats(Ctx *ctx, const int dims[4], const void *U_dev, double *partial_dev, int nparts,
+                       double *out_host);
 
 // Slice matrices of FP64 fields with nhalf (1 or 2) checkerboard halves (sliceInnerProductMatrix,
 // sliceMaddMatrix).  ipm: out_dev[s * Ls + t] = sum conj(x_s) y_t (double2), through nparts

@@ -1395,6 +1395,88 @@ static __global__ void __launch_bounds__(256) k_digest(const uint64_t *__restric
   if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+// ---------------- gauge statistics (WilsonLoops.h:102-126,203-219) ----------------
+// U: [site][mu][a][b] complex double, sites lexicographic (x fastest).  Per site: the real traces
+// of the six plaquettes U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag (mu > nu) and of the four
+// links; workgroup partials in partial[2 * block + {0, 1}], summed in a fixed order by k_reduce_fin.
+struct C3 {
+  double2 m[3][3];
+};
+static __device__ __forceinline__ C3 ld_link(const double2 *U, int64_t site, int mu) {
+  C3 c;
+  const double2 *p = U + (site * 4 + mu) * 9;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) c.m[a][b] = p[a * 3 + b];
+  return c;
+}
+// x y (dx = 0) or x y^dag (dx = 1)
+static __device__ __forceinline__ C3 mul3(const C3 &x, const C3 &y, bool ydag) {
+  C3 z;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      double re = 0, im = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double2 u = x.m[a][k];
+        double2 v = ydag ? y.m[b][k] : y.m[k][b];
+        if (ydag) v.y = -v.y;
+        re = fma(u.x, v.x, fma(-u.y, v.y, re));
+        im = fma(u.x, v.y, fma(u.y, v.x, im));
+      }
+      z.m[a][b] = {re, im};
+    }
+  return z;
+}
+static __global__ void __launch_bounds__(256) k_gauge_stats(const double2 *__restrict__ U, int L0, int L1, int L2,
+                                                            int L3, double *partial) {
+  const int64_t V = (int64_t)L0 * L1 * L2 * L3;
+  const int L[4] = {L0, L1, L2, L3};
+  double plaq = 0, tr = 0;
+  for (int64_t site = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; site < V; site += (int64_t)gridDim.x * blockDim.x) {
+    int x[4];
+    int64_t rem = site;
+    for (int d = 0; d < 4; d++) {
+      x[d] = (int)(rem % L[d]);
+      rem /= L[d];
+    }
+    int64_t stride[4] = {1, L0, (int64_t)L0 * L1, (int64_t)L0 * L1 * L2};
+    int64_t fwd[4];
+    for (int d = 0; d < 4; d++) fwd[d] = site + (x[d] + 1 == L[d] ? -(int64_t)(L[d] - 1) : 1) * stride[d];
+    for (int mu = 0; mu < 4; mu++) {
+      const C3 umu = ld_link(U, site, mu);
+      tr += umu.m[0][0].x + umu.m[1][1].x + umu.m[2][2].x;
+      for (int nu = 0; nu < mu; nu++) {
+        const C3 a = mul3(umu, ld_link(U, fwd[mu], nu), false);          // U_mu(x) U_nu(x+mu)
+        const C3 b = mul3(ld_link(U, site, nu), ld_link(U, fwd[nu], mu), false);  // U_nu(x) U_mu(x+nu)
+        // Re tr(a b^dag) = sum_ij Re(a_ij conj(b_ij))
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++) plaq += a.m[i][j].x * b.m[i][j].x + a.m[i][j].y * b.m[i][j].y;
+      }
+    }
+  }
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = plaq;
+  sh[1][threadIdx.x] = tr;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sh[0][0];
+    partial[2 * blockIdx.x + 1] = sh[1][0];
+  }
+}
+
 // ---------------- multishift CG vector updates (ConjugateGradientMultiShift.h:236-300) ------------
 // One pass over r for every shift (the fusion the reference's comments propose): shift s with bit s
 // of do_psi set takes psi_s -= bs_s ps_s (closing its iteration k); with bit s of do_ps set, then
@@ -1546,6 +1628,26 @@ int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const vo
   hipLaunchKernelGGL(k_slice_ipm_fin, dim3(1), dim3(256), 0, ctx->stream, (const double2 *)partial, (int)nb,
                      g.Ls * g.Ls, (double2 *)out_dev);
   GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int launch_gauge_stats(Ctx *ctx, const int dims[4], const void *U_dev, double *partial_dev, int nparts,
+                       double *out_host) {
+  const int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+  int64_t nb = (V + 255) / 256;
+  if (nb > nparts) nb = nparts;
+  hipLaunchKernelGGL(k_gauge_stats, dim3(nb), dim3(256), 0, ctx->stream, (const double2 *)U_dev, dims[0], dims[1],
+                     dims[2], dims[3], partial_dev);
+  GX_CHECK_HIP(hipGetLastError());
+  GX_CHECK_HIP(hipMemcpyAsync(out_host, partial_dev, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  double p = 0, t = 0;
+  for (int64_t b = 0; b < nb; b++) {  // fixed order: bitwise reproducible
+    p += out_host[2 * b];
+    t += out_host[2 * b + 1];
+  }
+  out_host[0] = p;
+  out_host[1] = t;
   return GX_OK;
 }
 

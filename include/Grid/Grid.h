@@ -373,6 +373,61 @@ typedef LatticeGaugeFieldD LatticeGaugeFieldF;
 typedef LatticeGaugeFieldD LatticeGaugeField;
 inline void precisionChange(LatticeGaugeFieldD &out, const LatticeGaugeFieldD &in) { out.data = in.data; }
 
+// Grid: WilsonLoops<Gimpl>::avgPlaquette / linkTrace (Grid/qcd/utils/WilsonLoops.h:121-126,203-219)
+template <class Gimpl = void>
+struct WilsonLoops {
+  static RealD avgPlaquette(const LatticeGaugeFieldD &U) {
+    double p = 0;
+    detail::check(gx_gauge_statistics(detail::context(), U.Grid()->dims, U.raw(), &p, nullptr), "avgPlaquette");
+    return p;
+  }
+  static RealD linkTrace(const LatticeGaugeFieldD &U) {
+    double t = 0;
+    detail::check(gx_gauge_statistics(detail::context(), U.Grid()->dims, U.raw(), nullptr, &t), "linkTrace");
+    return t;
+  }
+};
+typedef void PeriodicGimplR;
+
+// Grid: FieldMetaData (Grid/parallelIO/MetaData.h:62-98), the NERSC header fields a reader returns
+struct FieldMetaData {
+  std::vector<int> dimension = std::vector<int>(4);
+  std::string data_type, floating_point, ensemble_id, ensemble_label;
+  RealD link_trace = 0, plaquette = 0;
+  uint32_t checksum = 0;
+  int sequence_number = 0;
+};
+
+// Grid: NerscIO::readConfiguration / writeConfiguration (Grid/parallelIO/NerscIO.h:148-290).  A
+// checksum, plaquette or link-trace mismatch asserts, as in Grid.
+struct NerscIO {
+  static void readConfiguration(LatticeGaugeFieldD &Umu, FieldMetaData &header, const std::string &file) {
+    gx_nersc_info info{};
+    const int rc = gx_nersc_read(detail::context(), file.c_str(), Umu.Grid()->dims, Umu.raw(), &info);
+    header.dimension.assign(info.dims, info.dims + 4);
+    header.data_type = info.two_row ? "4D_SU3_GAUGE" : "4D_SU3_GAUGE_3x3";
+    header.floating_point = info.floating_point;
+    header.ensemble_id = info.ensemble_id;
+    header.ensemble_label = info.ensemble_label;
+    header.link_trace = info.link_trace_header;
+    header.plaquette = info.plaquette_header;
+    header.checksum = info.checksum_header;
+    header.sequence_number = info.sequence_number;
+    detail::check(rc, "NerscIO::readConfiguration");
+    std::cout << GridLogMessage << "NERSC Configuration " << file << " checksum " << std::hex << info.checksum_data
+              << std::dec << " plaquette " << info.plaquette << " link_trace " << info.link_trace
+              << ": plaquette, link trace, and checksum agree" << std::endl;
+  }
+  static void writeConfiguration(LatticeGaugeFieldD &Umu, const std::string &file, int two_row = 0, int bits32 = 1,
+                                 const std::string &ens_label = "DWF", const std::string &ens_id = "UKQCD",
+                                 unsigned int sequence_number = 1) {
+    (void)bits32;  // Grid writes IEEE64BIG whatever bits32 says
+    detail::check(gx_nersc_write(detail::context(), file.c_str(), Umu.Grid()->dims, Umu.raw(), two_row,
+                                 ens_label.c_str(), ens_id.c_str(), (int)sequence_number),
+                  "NerscIO::writeConfiguration");
+  }
+};
+
 template <int N>
 struct SU {
   // Random SU(3) links exp(i eps H), H Hermitian traceless Gaussian; eps = 1 for a hot start.

@@ -337,6 +337,38 @@ int gx_field_point_source(gx_field f, const int x[4], int spin, int colour);
  * probability) bitwise equal fields.  No Grid counterpart; used to compare solutions cheaply. */
 int gx_field_digest(gx_field f, uint64_t *digest);
 
+/* ---- gauge configurations ----
+ * Grid: WilsonLoops<Gimpl>::avgPlaquette and linkTrace (WilsonLoops.h:121-126,203-219) of a host
+ * gauge field (layout above), computed on the device: the average over sites and the 6 planes of
+ * Re tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag) / 3, and over links of Re tr U / 3. */
+int gx_gauge_statistics(gx_ctx ctx, const int dims[4], const double *U, double *plaquette, double *link_trace);
+
+/* Header fields of a NERSC file as read, next to the values recomputed from its data. */
+typedef struct {
+  int dims[4];
+  int two_row;                  /* DATATYPE 4D_SU3_GAUGE (1) or 4D_SU3_GAUGE_3x3 (0) */
+  uint32_t checksum_header, checksum_data;
+  double plaquette_header, plaquette;
+  double link_trace_header, link_trace;
+  int sequence_number;
+  char floating_point[16];
+  char ensemble_id[64];
+  char ensemble_label[64];
+} gx_nersc_info;
+
+/* Grid: NerscIO::readConfiguration(Umu, header, file), NerscIO.h:63-221.  Reads a NERSC gauge file
+ * (4D_SU3_GAUGE two-row or 4D_SU3_GAUGE_3x3; IEEE32BIG, IEEE32, IEEE64BIG, IEEE64/IEEE64LITTLE)
+ * on the lattice dims into U (layout above).  As in Grid, it fails (GX_EINVAL, with the values in
+ * the message and in *info) when the 32-bit word checksum differs from CHECKSUM, or when the
+ * recomputed plaquette or link trace differs from the header by 1e-5 or 1e-6. */
+int gx_nersc_read(gx_ctx ctx, const char *path, const int dims[4], double *U, gx_nersc_info *info);
+
+/* Grid: NerscIO::writeConfiguration(Umu, file, two_row, bits32, ens_label, ens_id, sequence_number),
+ * NerscIO.h:224-290.  Writes U as IEEE64BIG, three rows (two_row = 0) or two (two_row = 1), with
+ * the plaquette, link trace and checksum in the header (bits32 has no effect in Grid either). */
+int gx_nersc_write(gx_ctx ctx, const char *path, const int dims[4], const double *U, int two_row,
+                   const char *ensemble_label, const char *ensemble_id, int sequence_number);
+
 #ifdef __cplusplus
 }
 #endif
