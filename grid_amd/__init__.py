@@ -74,7 +74,7 @@ class ServeParams(ctypes.Structure):
     _fields_ = [('device', ctypes.c_int), ('slots', ctypes.c_int), ('budget', ctypes.c_double),
                 ('n_gauge', ctypes.c_int), ('gauge_seed', ctypes.c_uint64), ('gauge_eps', ctypes.c_double),
                 ('tol', ctypes.c_double), ('mixed', ctypes.c_int), ('maxit', ctypes.c_int),
-                ('realtime', ctypes.c_int), ('progress', ctypes.c_int)]
+                ('realtime', ctypes.c_int), ('progress', ctypes.c_int), ('gauge_dir', ctypes.c_char_p)]
 
 
 _lib = None

@@ -320,7 +320,14 @@ int gx_serve(const gx_serve_params *p, const gx_request *req, int n, gx_response
     const size_t V = (size_t)lat.d[0] * lat.d[1] * lat.d[2] * lat.d[3];
     std::vector<double> U(V * 4 * 18);
     for (int g = 0; g < p->n_gauge && rc == GX_OK; g++) {
-      rc = gx_random_gauge(lat.d, p->gauge_seed + (uint64_t)g, p->gauge_eps, U.data());
+      if (p->gauge_dir) {  // a real ensemble: NERSC file, checked against its header
+        char path[4096];
+        std::snprintf(path, sizeof(path), "%s/ckpoint_lat.%dx%dx%dx%d.%d", p->gauge_dir, lat.d[0], lat.d[1],
+                      lat.d[2], lat.d[3], g);
+        rc = gx_nersc_read(loader, path, lat.d, U.data(), nullptr);
+      } else {
+        rc = gx_random_gauge(lat.d, p->gauge_seed + (uint64_t)g, p->gauge_eps, U.data());
+      }
       gx_op base = nullptr, warm = nullptr;
       if (rc == GX_OK) rc = gx_wilson_create(loader, lat.d, 1, 0.1, U.data(), nullptr, &base);
       if (rc != GX_OK) break;

@@ -6,6 +6,7 @@ libgx): FIFO admission by idle slot and device-memory footprint, one stream and 
 slot, so solves overlap on the device while requests keep arriving.  No Grid counterpart.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -44,7 +45,7 @@ def footprint(dims, nsrc, mixed=True):
 
 
 def serve(reqs, slots=4, budget=64e9, n_gauge=8, gauge_seed=1000, gauge_eps=0.3, tol=1e-10, mixed=True,
-          maxit=20000, realtime=True, device=0, want_solutions=False, progress=0):
+          maxit=20000, realtime=True, device=0, want_solutions=False, progress=0, gauge_dir=None):
     """Run a trace through gx_serve.  Returns (responses, info): one dict per request (status,
     slot, iterations, residual, t_admit, t_done, footprint, digest, latency) and info with the
     wall time and resident link bytes.  With want_solutions the FP64 solutions come back as host
@@ -59,7 +60,7 @@ def serve(reqs, slots=4, budget=64e9, n_gauge=8, gauge_seed=1000, gauge_eps=0.3,
     out = (Response * max(n, 1))()
     p = ServeParams(device=device, slots=slots, budget=float(budget), n_gauge=n_gauge, gauge_seed=gauge_seed,
                     gauge_eps=gauge_eps, tol=tol, mixed=int(mixed), maxit=maxit, realtime=int(bool(realtime)),
-                    progress=int(progress))
+                    progress=int(progress), gauge_dir=None if gauge_dir is None else os.fsencode(gauge_dir))
     sols, ptrs = None, None
     if want_solutions:
         sols = []

@@ -257,7 +257,8 @@ int gx_slice_madd_matrix(gx_field R, const double *aa, gx_field X, gx_field Y, d
  * (Schur) solve of all its sources at once (gx_schur_solve on a multi-RHS clone of the resident
  * operator, gx_wilson_clone) on that slot's own stream and host thread, overlapping on the device
  * with the solves of the other slots.  Resident gauge fields are built before the clock starts:
- * field g of a lattice comes from gx_random_gauge(seed = gauge_seed + g, eps = gauge_eps). */
+ * field g of a lattice comes from gx_random_gauge(seed = gauge_seed + g, eps = gauge_eps), or
+ * from a NERSC file when gauge_dir is set. */
 typedef struct {
   int dims[4];
   int nsrc;          /* right-hand sides, 1..16 */
@@ -291,6 +292,8 @@ typedef struct {
   int maxit;
   int realtime;
   int progress;            /* 0: silent; N > 0: one line on stderr after every N completed requests */
+  const char *gauge_dir;   /* NULL: synthetic fields; else field g of lattice LxxLyxLzxLt is the NERSC
+                            * file <gauge_dir>/ckpoint_lat.<Lx>x<Ly>x<Lz>x<Lt>.<g> (gx_nersc_read) */
 } gx_serve_params;
 
 /* Device bytes a request of nsrc sources on lattice dims holds while it runs (fields and the

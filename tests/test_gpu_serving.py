@@ -169,3 +169,26 @@ def test_device_point_sources_and_slices(ctx):
     gx.insert_slice(f4, g5, 1)
     got = g5.download()
     assert np.array_equal(got[..., 1:2, :, :], a[..., 3:4, :, :]) and not np.any(got[..., [0, 2, 3, 4], :, :])
+
+
+def test_serve_reads_nersc_gauge(ctx, tmp_path):
+    """Resident gauge fields from NERSC files (gauge_dir): files holding the synthetic fields give
+    bitwise the synthetic run's solutions; another configuration in one file changes exactly the
+    requests on that field; a missing file is refused before any solve."""
+    dims = (8, 8, 8, 8)
+    reqs = [dict(dims=dims, nsrc=2, mass=0.1, gauge=i % 2, seed=300 + i, arrival=0.0) for i in range(4)]
+    kw = dict(slots=2, budget=2e9, n_gauge=2, gauge_seed=910, gauge_eps=0.3, tol=1e-10, mixed=2, realtime=False)
+    for g in range(2):
+        gx.nersc_write(ctx, str(tmp_path / f'ckpoint_lat.8x8x8x8.{g}'), gx.random_gauge(dims, 910 + g, 0.3), dims)
+    syn, _ = serving.serve(reqs, **kw)
+    real, _ = serving.serve(reqs, gauge_dir=str(tmp_path), **kw)
+    assert all(r['status'] == 0 for r in syn + real)
+    assert [r['digest'] for r in real] == [r['digest'] for r in syn]
+    gx.nersc_write(ctx, str(tmp_path / 'ckpoint_lat.8x8x8x8.0'), gx.random_gauge(dims, 2000, 0.3), dims, two_row=True)
+    other, _ = serving.serve(reqs, gauge_dir=str(tmp_path), **kw)
+    assert all(r['status'] == 0 and r['residual'] < 1e-8 for r in other)
+    for r, o in zip(reqs, other):
+        same = o['digest'] == syn[reqs.index(r)]['digest']
+        assert same == (r['gauge'] == 1), r
+    with pytest.raises(gx.GxError):
+        serving.serve(reqs, gauge_dir=str(tmp_path / 'missing'), **kw)
