@@ -126,6 +126,10 @@ _SIG = {
     'gx_Dhop': ([_P, _P, _P, _I], _I),
     'gx_DhopEO': ([_P, _P, _P, _I], _I),
     'gx_DhopOE': ([_P, _P, _P, _I], _I),
+    'gx_DhopDir': ([_P, _P, _P, _I, _I], _I),
+    'gx_DhopDeriv': ([_P, _DP, _P, _P, _I], _I),
+    'gx_DhopDerivEO': ([_P, _DP, _P, _P, _I], _I),
+    'gx_DhopDerivOE': ([_P, _DP, _P, _P, _I], _I),
     'gx_M5D': ([_P, _P, _P], _I),
     'gx_M5Ddag': ([_P, _P, _P], _I),
     'gx_Meooe5D': ([_P, _P, _P], _I),
@@ -450,6 +454,15 @@ class FermionOp:
     def Dhop(self, i, o, dag=0): self._call('Dhop', i.h, o.h, int(dag))
     def DhopEO(self, i, o, dag=0): self._call('DhopEO', i.h, o.h, int(dag))
     def DhopOE(self, i, o, dag=0): self._call('DhopOE', i.h, o.h, int(dag))
+    def DhopDir(self, i, o, dir, disp): self._call('DhopDir', i.h, o.h, int(dir), int(disp))
+
+    def DhopDeriv(self, A, B, dag=0, which=''):
+        """DhopDeriv / DhopDerivEO / DhopDerivOE (which = '', 'EO', 'OE'): the force matrices
+        mat[t, z, y, x, mu, a, b] (zeros on the sites the EO/OE forms do not fill)."""
+        lx, ly, lz, lt = self.dims
+        mat = np.zeros((lt, lz, ly, lx, 4, 3, 3), dtype=np.complex128)
+        self._call('DhopDeriv' + which, _dp(mat), A.h, B.h, int(dag))
+        return mat
     def M5D(self, i, o): self._call('M5D', i.h, o.h)
     def M5Ddag(self, i, o): self._call('M5Ddag', i.h, o.h)
     def Meooe5D(self, i, o): self._call('Meooe5D', i.h, o.h)

@@ -1323,6 +1323,93 @@ int gx_DhopOE(gx_op op, gx_field in, gx_field out, int dag) {
   return apply_dhop_half(O(op), F(in), F(out), GX_ODD, dag);
 }
 
+// Grid: WilsonFermion(5D)::DhopDir / Mdir (WilsonFermionImplementation.h:345-366): the leg of
+// direction dir, displacement disp = +1 (forward neighbour, (1 - gamma_dir), Uds[dir]) or -1 (backward
+// neighbour, (1 + gamma_dir), Uds[dir + 4]) of the non-dagger hop, on full F64 fields.
+int gx_DhopDir(gx_op op, gx_field in, gx_field out, int dir, int disp) {
+  GX_TRY(check_io(O(op), F(in), F(out), true));
+  GX_REQUIRE(in->prec == GX_F64, GX_EINVAL, "DhopDir: F64 fields");
+  GX_REQUIRE(dir >= 0 && dir < 4 && (disp == 1 || disp == -1), GX_EINVAL, "DhopDir: dir 0..3, disp +-1");
+  void *Ue, *Uo;
+  GX_TRY(op_gauge(O(op), GX_F64, &Ue, &Uo));
+  const bool fwd = disp == 1;
+  for (int cb = 0; cb < 2; cb++) {
+    Field i = half_view(F(in), 1 - cb), o = half_view(F(out), cb);
+    GX_TRY(launch_hop_leg(op->g, dir, fwd, !fwd, 0, cb, i.data, cb == 0 ? Ue : Uo, nullptr, o.data, nullptr,
+                          op->ctx->stream));
+  }
+  return GX_OK;
+}
+
+// Grid: DhopDeriv / DhopDerivEO / DhopDerivOE (WilsonFermion5DImplementation.h:211-305,
+// WilsonFermionImplementation.h DerivInternal): for every mu, Btilde = Uds[mu](x) P B(x + mu) with
+// P = (1 - gamma_mu) (dag = 0) or (1 + gamma_mu) (dag = 1), and mat_mu(x) = sum_{s, spin} Btilde A^dag
+// (InsertForce5D).  cbs: the output checkerboards (both for DhopDeriv).  mat: host gauge layout,
+// sites of other checkerboards left untouched.
+static int dhop_deriv(Op *op, double *mat, Field *A, Field *B, int dag, int cb_first, int cb_last) {
+  void *Ue, *Uo;
+  GX_TRY(op_gauge(op, GX_F64, &Ue, &Uo));
+  const Geom &g = op->g;
+  const size_t per = (size_t)g.H4 * 9 * 2 * sizeof(double);
+  double *dev = nullptr;
+  GX_CHECK_HIP(hipMalloc((void **)&dev, 4 * per));
+  std::vector<double> host(4 * per / sizeof(double));
+  int rc = GX_OK;
+  for (int cb = cb_first; cb <= cb_last && rc == GX_OK; cb++) {
+    const Field a = A->subset == GX_FULL ? half_view(A, cb) : *A;
+    const Field b = B->subset == GX_FULL ? half_view(B, 1 - cb) : *B;
+    for (int mu = 0; mu < 4 && rc == GX_OK; mu++)
+      rc = launch_hop_leg(g, mu, 1, dag ? 1 : 0, 1, cb, b.data, cb == 0 ? Ue : Uo, a.data, nullptr,
+                          dev + (size_t)mu * g.H4 * 18, op->ctx->stream);
+    hipError_t e = rc == GX_OK ? hipMemcpyAsync(host.data(), dev, 4 * per, hipMemcpyDeviceToHost, op->ctx->stream)
+                               : hipSuccess;
+    if (e == hipSuccess && rc == GX_OK) e = hipStreamSynchronize(op->ctx->stream);
+    if (e != hipSuccess) {
+      set_error("DhopDeriv: %s", hipGetErrorString(e));
+      rc = GX_EHIP;
+      break;
+    }
+    if (rc != GX_OK) break;
+    const int Lx = g.L[0], Ly = g.L[1], Lz = g.L[2], Lxh = Lx / 2;
+    for (int64_t h = 0; h < g.H4; h++) {
+      int64_t r = h / Lxh;
+      const int xh = (int)(h - r * Lxh), y = (int)(r % Ly);
+      r /= Ly;
+      const int z = (int)(r % Lz), t = (int)(r / Lz);
+      const int x = 2 * xh + ((cb + y + z + t) & 1);
+      const int64_t site = (((int64_t)t * Lz + z) * Ly + y) * Lx + x;
+      for (int mu = 0; mu < 4; mu++)
+        std::memcpy(mat + (site * 4 + mu) * 18, host.data() + ((size_t)mu * g.H4 + h) * 18, 18 * sizeof(double));
+    }
+  }
+  (void)hipFree(dev);
+  return rc;
+}
+
+static int deriv_args(Op *op, double *mat, Field *A, Field *B, bool full) {
+  GX_REQUIRE(op && mat && A && B, GX_EINVAL, "DhopDeriv: null argument");
+  GX_REQUIRE(same_geom(A->g, op->g) && same_geom(B->g, op->g), GX_EINVAL, "DhopDeriv: field geometry");
+  GX_REQUIRE(A->prec == GX_F64 && B->prec == GX_F64, GX_EINVAL, "DhopDeriv: F64 fields");
+  GX_REQUIRE((A->subset == GX_FULL) == full && (B->subset == GX_FULL) == full, GX_EINVAL,
+             full ? "DhopDeriv: full fields" : "DhopDerivEO/OE: half fields");
+  return GX_OK;
+}
+
+int gx_DhopDeriv(gx_op op, double *mat, gx_field A, gx_field B, int dag) {
+  GX_TRY(deriv_args(O(op), mat, F(A), F(B), true));
+  return dhop_deriv(O(op), mat, F(A), F(B), dag, 0, 1);
+}
+int gx_DhopDerivEO(gx_op op, double *mat, gx_field A, gx_field B, int dag) {
+  GX_TRY(deriv_args(O(op), mat, F(A), F(B), false));
+  GX_REQUIRE(A->cb == GX_EVEN && B->cb == GX_ODD, GX_ECB, "DhopDerivEO: A even, B odd");
+  return dhop_deriv(O(op), mat, F(A), F(B), dag, 0, 0);
+}
+int gx_DhopDerivOE(gx_op op, double *mat, gx_field A, gx_field B, int dag) {
+  GX_TRY(deriv_args(O(op), mat, F(A), F(B), false));
+  GX_REQUIRE(A->cb == GX_ODD && B->cb == GX_EVEN, GX_ECB, "DhopDerivOE: A odd, B even");
+  return dhop_deriv(O(op), mat, F(A), F(B), dag, 1, 1);
+}
+
 static int m5d_common(gx_op op, gx_field psi, gx_field chi, int which, bool with_phi) {
   GX_TRY(check_io(O(op), F(psi), F(chi), false));
   GX_REQUIRE(op->kind == GX_MOBIUS, GX_EINVAL, "M5D family needs a Mobius operator");

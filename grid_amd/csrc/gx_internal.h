@@ -254,6 +254,11 @@ int launch_multishift(Ctx *ctx, int64_t n, const void *r, void *p, const MultiSh
 // 2 nparts doubles.  Synchronises the stream.
 int launch_gauge_stats(Ctx *ctx, const int dims[4], const void *U_dev, double *partial_dev, int nparts,
                        double *out_host);
+// One leg of the FP64 hop on the half fields of checkerboard cb_out (links U of cb_out): direction
+// mu, forward (fwd) or backward neighbour, projector (1 + gamma) (plus) or (1 - gamma).  mode 0:
+// out = the leg (half field of cb_out);  mode 1: mat[h][3][3] = sum over s and spin of leg (x) A^dag.
+int launch_hop_leg(const Geom &g, int mu, int fwd, int plus, int mode, int cb_out, const void *in, const void *U,
+                   const void *A, void *out, void *mat, hipStream_t st);
 
 // Slice matrices of FP64 fields with nhalf (1 or 2) checkerboard halves (sliceInnerProductMatrix,
 // sliceMaddMatrix).  ipm: out_dev[s * Ls + t] = sum conj(x_s) y_t (double2), through nparts

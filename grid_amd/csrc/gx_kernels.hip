@@ -1395,6 +1395,76 @@ static __global__ void __launch_bounds__(256) k_digest(const uint64_t *__restric
   if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
+// ---------------- single-direction hop and force legs (WilsonKernels DhopDirKernel,
+// WilsonKernelsImplementation.h:375-410; WilsonImpl::InsertForce5D, WilsonImpl.h:193-238) ----------
+// One thread per output half-site h of checkerboard cb_out, all Ls slices: the leg of direction MU
+// at the forward (FWD: point MU, link Uds[MU](x)) or backward (point MU + 4, link Uds[MU + 4](x))
+// neighbour with projector (1 + gamma_MU) (PLUS) or (1 - gamma_MU):
+//   chi(h, s) = Uds[P](h) recon(project(in(nb_P(h), s))).
+// MODE 0 writes out(h, s) = chi (DhopDir); MODE 1 writes mat[h][a][b] = sum_{s, spin} chi_a conj(A_b)
+// (the colour outer product of InsertForce5D).  FP64.
+template <int MU, bool FWD, bool PLUS, int MODE>
+static __global__ void __launch_bounds__(256) k_hop_leg(KGeom g, int Ls, int cb_out, const cx<double> *__restrict__ in,
+                                                        const cx<double> *__restrict__ U,
+                                                        const cx<double> *__restrict__ A, cx<double> *__restrict__ out,
+                                                        cx<double> *__restrict__ mat) {
+  const uint32_t H4 = (uint32_t)g.H4;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H4; h += gridDim.x * blockDim.x) {
+    const uint32_t r1 = fdiv(h, g.fLxh), r2 = fdiv(r1, g.fL1), r3 = fdiv(r2, g.fL2);
+    const int xh = (int)(h - r1 * (uint32_t)g.Lxh);
+    const int y = (int)(r1 - r2 * (uint32_t)g.L[1]);
+    const int z = (int)(r2 - r3 * (uint32_t)g.L[2]);
+    const int t = (int)r3;
+    const int x = 2 * xh + ((cb_out + y + z + t) & 1);
+    uint32_t n;
+    if (MU == 0) {
+      const int xn = FWD ? (x + 1 == g.L[0] ? 0 : x + 1) : (x == 0 ? g.L[0] - 1 : x - 1);
+      n = h - xh + (xn >> 1);
+    } else {
+      const int c = MU == 1 ? y : (MU == 2 ? z : t), L = g.L[MU];
+      const uint32_t st = (uint32_t)(MU == 1 ? g.sY : (MU == 2 ? g.sZ : g.sT));
+      if (FWD) n = (c + 1 == L) ? h - (uint32_t)(L - 1) * st : h + st;
+      else n = (c == 0) ? h + (uint32_t)(L - 1) * st : h - st;
+    }
+    constexpr int P = FWD ? MU : MU + 4;
+    cx<double> u[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[k] = U[(int64_t)(P * 9 + k) * H4 + h];
+    cx<double> m[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) m[k] = {0, 0};
+    for (int s = 0; s < Ls; s++) {
+      cx<double> f[12], hs[6], acc[12];
+      load_site<double>(f, in, (int64_t)n * Ls + s);
+      project<double, MU, PLUS>(f, hs);
+#pragma unroll
+      for (int k = 0; k < 12; k++) acc[k] = {0, 0};
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        recon_colour<double, MU, PLUS>(acc, r, su3_row<double>(u + 3 * r, hs), su3_row<double>(u + 3 * r, hs + 3));
+      if (MODE == 0) {
+        store_site<double>(out, acc, (int64_t)h * Ls + s, false);
+      } else {
+        cx<double> a[12];
+        load_site<double>(a, A, (int64_t)h * Ls + s);
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+              const cx<double> c = acc[sp * 3 + i], d = a[sp * 3 + j];  // c conj(d)
+              m[i * 3 + j].r = fma(c.r, d.r, fma(c.i, d.i, m[i * 3 + j].r));
+              m[i * 3 + j].i = fma(c.i, d.r, fma(-c.r, d.i, m[i * 3 + j].i));
+            }
+      }
+    }
+    if (MODE == 1)
+#pragma unroll
+      for (int k = 0; k < 9; k++) mat[(int64_t)h * 9 + k] = m[k];
+  }
+}
+
 // ---------------- gauge statistics (WilsonLoops.h:102-126,203-219) ----------------
 // U: [site][mu][a][b] complex double, sites lexicographic (x fastest).  Per site: the real traces
 // of the six plaquettes U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag (mu > nu) and of the four
@@ -1627,6 +1697,36 @@ int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const vo
                      (const cx<double> *)x, (const cx<double> *)y, (double2 *)partial);
   hipLaunchKernelGGL(k_slice_ipm_fin, dim3(1), dim3(256), 0, ctx->stream, (const double2 *)partial, (int)nb,
                      g.Ls * g.Ls, (double2 *)out_dev);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int launch_hop_leg(const Geom &g, int mu, int fwd, int plus, int mode, int cb_out, const void *in, const void *U,
+                   const void *A, void *out, void *mat, hipStream_t st) {
+  if (mu < 0 || mu > 3) {
+    set_error("hop leg: direction %d", mu);
+    return GX_EINVAL;
+  }
+  const KGeom k = kgeom(g);
+  const uint32_t nb = (uint32_t)((g.H4 + 255) / 256);
+  const cx<double> *pin = (const cx<double> *)in, *pU = (const cx<double> *)U, *pA = (const cx<double> *)A;
+  cx<double> *pout = (cx<double> *)out, *pmat = (cx<double> *)mat;
+#define GX_LEG(M, F, P, MODE)                                                                                    \
+  hipLaunchKernelGGL((k_hop_leg<M, F, P, MODE>), dim3(nb), dim3(256), 0, st, k, g.Ls, cb_out, pin, pU, pA, pout, pmat)
+#define GX_LEG_MODE(M, F, P) \
+  if (mode == 0) GX_LEG(M, F, P, 0); else GX_LEG(M, F, P, 1)
+#define GX_LEG_FP(M)                                              \
+  if (fwd) { if (plus) GX_LEG_MODE(M, true, true); else GX_LEG_MODE(M, true, false); } \
+  else { if (plus) GX_LEG_MODE(M, false, true); else GX_LEG_MODE(M, false, false); }
+  switch (mu) {
+    case 0: GX_LEG_FP(0); break;
+    case 1: GX_LEG_FP(1); break;
+    case 2: GX_LEG_FP(2); break;
+    default: GX_LEG_FP(3); break;
+  }
+#undef GX_LEG_FP
+#undef GX_LEG_MODE
+#undef GX_LEG
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }

@@ -524,6 +524,25 @@ class FermionOperatorBase : public CheckerBoardedSparseMatrixBase<FermionLattice
   void DhopOE(const FermionField &in, FermionField &out, int dag) {
     detail::check(gx_DhopOE(op, in.f, out.f, dag), "DhopOE");
   }
+  // DhopDir / Mdir (FermionOperator.h:78,92): one leg of the hop, dir 0..3, disp +-1
+  void DhopDir(const FermionField &in, FermionField &out, int dir, int disp) {
+    out.Checkerboard() = in.Checkerboard();
+    detail::check(gx_DhopDir(op, in.f, out.f, dir, disp), "DhopDir");
+  }
+  void Mdir(const FermionField &in, FermionField &out, int dir, int disp) { DhopDir(in, out, dir, disp); }
+  // the force terms (FermionOperator.h:81-89): mat is the host gauge field the derivative lands in
+  void DhopDeriv(GaugeField &mat, const FermionField &U, const FermionField &V, int dag) {
+    detail::check(gx_DhopDeriv(op, mat.raw(), U.f, V.f, dag), "DhopDeriv");
+  }
+  void DhopDerivEO(GaugeField &mat, const FermionField &U, const FermionField &V, int dag) {
+    detail::check(gx_DhopDerivEO(op, mat.raw(), U.f, V.f, dag), "DhopDerivEO");
+  }
+  void DhopDerivOE(GaugeField &mat, const FermionField &U, const FermionField &V, int dag) {
+    detail::check(gx_DhopDerivOE(op, mat.raw(), U.f, V.f, dag), "DhopDerivOE");
+  }
+  void MDeriv(GaugeField &mat, const FermionField &U, const FermionField &V, int dag) { DhopDeriv(mat, U, V, dag); }
+  void MoeDeriv(GaugeField &mat, const FermionField &U, const FermionField &V, int dag) { DhopDerivOE(mat, U, V, dag); }
+  void MeoDeriv(GaugeField &mat, const FermionField &U, const FermionField &V, int dag) { DhopDerivEO(mat, U, V, dag); }
 
  protected:
   static const double *phases_of(const ImplParams &p, std::vector<double> &buf) {

@@ -132,6 +132,21 @@ int gx_MooeeInvDag(gx_op op, gx_field in, gx_field out);
 int gx_Dhop(gx_op op, gx_field in, gx_field out, int dag);     /* full lattice */
 int gx_DhopEO(gx_op op, gx_field in, gx_field out, int dag);   /* odd in -> even out */
 int gx_DhopOE(gx_op op, gx_field in, gx_field out, int dag);   /* even in -> odd out */
+/* Grid: WilsonFermion::DhopDir(in, out, dir, disp) / Mdir (WilsonFermionImplementation.h:345-366;
+ * WilsonFermion5D likewise): the one leg of the non-dagger hop in direction dir (0..3) with
+ * displacement disp = +1 (out(x) = Uds_dir(x) (1 - gamma_dir) in(x + dir)) or -1 (out(x) =
+ * Uds_{dir+4}(x) (1 + gamma_dir) in(x - dir)), Uds the -1/2-scaled double-stored links; the eight
+ * legs sum to Dhop.  Full F64 fields. */
+int gx_DhopDir(gx_op op, gx_field in, gx_field out, int dir, int disp);
+/* Grid: DhopDeriv / DhopDerivEO / DhopDerivOE(mat, A, B, dag) (WilsonFermion5DImplementation.h:211-305;
+ * WilsonImpl::InsertForce5D, WilsonImpl.h:193-238): for every mu, mat_mu(x) = sum over s and spin
+ * of the colour outer product [Uds_mu(x) P B(x + mu)] A(x)^dag, P = (1 - gamma_mu) for dag = 0 and
+ * (1 + gamma_mu) for dag = 1.  mat: host gauge layout (above); DhopDeriv fills every site from full
+ * F64 fields A, B; DhopDerivEO (A even, B odd) the even sites and DhopDerivOE (A odd, B even) the
+ * odd ones, leaving the others as they were. */
+int gx_DhopDeriv(gx_op op, double *mat, gx_field A, gx_field B, int dag);
+int gx_DhopDerivEO(gx_op op, double *mat, gx_field A, gx_field B, int dag);
+int gx_DhopDerivOE(gx_op op, double *mat, gx_field A, gx_field B, int dag);
 /* Cayley 5th-dimension pieces (Grid: CayleyFermion5D.h:43-116); Mobius operators only. */
 int gx_M5D(gx_op op, gx_field psi, gx_field chi);       /* chi = M5D(psi, phi=chi) as Grid's M5D(psi,chi) */
 int gx_M5Ddag(gx_op op, gx_field psi, gx_field chi);

@@ -18,7 +18,7 @@ CASES = [('Test_wilson_even_odd', '8.8.8.8'), ('Test_wilson_even_odd', '4.6.8.6'
          ('Test_mobius_even_odd', '8.8.4.4'), ('Test_wilson_cg_schur', '8.8.8.8'),
          ('Test_dwf_mixedcg_prec', '8.8.8.8'), ('Test_dwf_relupcg_prec', '8.8.8.8'),
          ('Test_wilson_block_cg', '8.8.8.8'), ('Test_wilson_multishift', '8.8.8.8'),
-         ('Test_nersc_io', '8.8.8.8')]
+         ('Test_nersc_io', '8.8.8.8'), ('Test_wilson_dhopdir', '8.8.4.8')]
 
 
 @pytest.mark.parametrize('name,grid', CASES)
