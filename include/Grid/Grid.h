@@ -530,6 +530,13 @@ class FermionOperatorBase : public CheckerBoardedSparseMatrixBase<FermionLattice
     detail::check(gx_DhopDir(op, in.f, out.f, dir, disp), "DhopDir");
   }
   void Mdir(const FermionField &in, FermionField &out, int dir, int disp) { DhopDir(in, out, dir, disp); }
+  // DhopDirAll / MdirAll (WilsonFermionImplementation.h:350-384): out[dir + 4 * (disp == -1)], 8 or 9 fields
+  void DhopDirAll(const FermionField &in, std::vector<FermionField> &out) {
+    assert(out.size() == 8 || out.size() == 9);
+    for (int dir = 0; dir < Nd; dir++)
+      for (int disp = -1; disp <= 1; disp += 2) DhopDir(in, out[dir + (disp == 1 ? 0 : 4)], dir, disp);
+  }
+  void MdirAll(const FermionField &in, std::vector<FermionField> &out) { DhopDirAll(in, out); }
   // the force terms (FermionOperator.h:81-89): mat is the host gauge field the derivative lands in
   void DhopDeriv(GaugeField &mat, const FermionField &U, const FermionField &V, int dag) {
     detail::check(gx_DhopDeriv(op, mat.raw(), U.f, V.f, dag), "DhopDeriv");

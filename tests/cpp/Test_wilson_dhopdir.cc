@@ -1,5 +1,5 @@
 // The single-direction hop and the force through the Grid API: the eight DhopDir legs sum to Dhop,
-// and for every mu the trace of MDeriv's colour matrices summed over the lattice is <A, Mdir(B, mu, +1)>
+// MdirAll returns them in Grid's order, and for every mu the trace of MDeriv's colour matrices summed over the lattice is <A, Mdir(B, mu, +1)>
 // (InsertForce5D's outer product traced over colour).
 #include <Grid/Grid.h>
 
@@ -26,6 +26,17 @@ int main(int argc, char **argv) {
   Dw.Dhop(B, ref, DaggerNo);
   LatticeFermion d = sum - ref;
   const double rsum = std::sqrt(norm2(d) / norm2(ref));
+  // MdirAll gives the same eight legs, indexed dir + 4 (disp = -1)
+  std::vector<LatticeFermion> all;
+  for (int k = 0; k < 8; k++) all.emplace_back(&Grid);
+  Dw.MdirAll(B, all);
+  double rall = 0;
+  for (int dir = 0; dir < Nd; dir++)
+    for (int disp = -1; disp <= 1; disp += 2) {
+      Dw.Mdir(B, out, dir, disp);
+      LatticeFermion e = out - all[dir + (disp == 1 ? 0 : 4)];
+      rall = std::max(rall, norm2(e));
+    }
   LatticeGaugeField mat(&Grid);
   Dw.MDeriv(mat, A, B, DaggerNo);
   double worst = 0;
@@ -40,7 +51,8 @@ int main(int argc, char **argv) {
   }
   std::cout << GridLogMessage << "sum of DhopDir legs vs Dhop: " << rsum << "; trace of MDeriv vs <A, Mdir B>: " << worst
             << std::endl;
-  const int fail = !(rsum < 1e-14) || !(worst < 1e-12);
+  std::cout << GridLogMessage << "MdirAll vs Mdir: " << rall << std::endl;
+  const int fail = !(rsum < 1e-14) || !(worst < 1e-12) || rall != 0.0;
   Grid_finalize();
   std::cout << (fail ? "FAILED" : "PASSED") << std::endl;
   return fail;
