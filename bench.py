@@ -105,9 +105,10 @@ def cpu_baseline(U, psi_o, seconds=12.0):
 
 def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tune=None):
     """Mobius 24^4 x Ls12 (b=1.5, c=0.5, M5=1.8, mf=0.005, eps 0.3), Gaussian 5D source: red-black
-    solve in FP64 CG and in defect correction with FP32 and with FP16-storage inner solves (inner
-    tolerance 1e-5, FP64 outer residual and patch-up), each to |b - M x| / |b| <= 1e-10.  Wall
-    clock around each solve."""
+    solve in FP64 CG, in defect correction with FP32 and with FP16-storage inner solves (inner
+    tolerance 1e-5, FP64 outer residual and patch-up), and in reliable-update CG (FP32 Krylov
+    vectors, FP64 reliable updates with delta 0.1, FP64 clean-up), each to |b - M x| / |b| <= 1e-10.
+    Wall clock around each solve."""
     kw = dict(mass=0.005, M5=1.8, b=1.5, c=0.5)
     U = gx.random_gauge(dims, 1003, 0.3)
     D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
@@ -116,7 +117,7 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tu
     src = D.field().gaussian(2003)
     out, ref = {'workload': 'Mobius 24^4 x Ls12 mf=0.005, Gaussian source', 'cg_tol': tol,
                 'target_unprec_residual': TARGET_RESIDUAL}, None
-    for name, mixed in (('fp64', 0), ('mixed_f32', 1), ('mixed_f16', 2)):
+    for name, mixed in (('fp64', 0), ('mixed_f32', 1), ('mixed_f16', 2), ('reliable_update_f32', 3)):
         x = D.field()
         ctx.sync()
         t0 = time.perf_counter()

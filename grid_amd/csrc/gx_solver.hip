@@ -1297,8 +1297,9 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
   if (!se || !so || !t || !Mt || !solo) return GX_ENOMEM;
   const bool multi = op->kind == GX_WILSON && op->g.Ls > 1;
   const int Ls = op->g.Ls;
-  if (mixed < 0 || mixed > 2) {
-    set_error("schur_solve: mixed must be 0 (FP64), 1 (FP32 inner) or 2 (FP16 inner)");
+  if (mixed < 0 || mixed > 3 || (multi && mixed == 3)) {
+    set_error("schur_solve: mixed must be 0 (FP64), 1 (FP32 inner), 2 (FP16 inner) or, on single-RHS operators, "
+              "3 (reliable updates)");
     return GX_EINVAL;
   }
   hipStream_t s = op->ctx->stream;
@@ -1333,6 +1334,8 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
           st->computed_residual = std::max(st->computed_residual, bst[k].computed_residual);
         }
       }
+    } else if (mixed == 3) {  // ConjugateGradientReliableUpdate, FP32 sloppy vectors, delta 0.1
+      rc = ru_cg_run(op, so, solo, tol, maxit, 0.1, GX_F32, -1, 0.0, st);
     } else if (mixed) {
       rc = mixed_cg_run(op, so, solo, tol, 1e-5, mixed == 2 ? GX_F16 : GX_F32, maxit, 100, st);
     } else {

@@ -250,16 +250,24 @@ def test_cg_matches_oracle(ctx, kind, dims, Ls, mass):
     assert rel_err(fm.download(), ref) < 1e-7
 
 
-def test_schur_solve_matches_oracle(ctx):
+@pytest.mark.parametrize('mixed', [0, 1, 2, 3])
+def test_schur_solve_matches_oracle(ctx, mixed):
+    """SchurRedBlackDiagMooeeSolve with FP64 CG, defect correction (FP32, FP16 inner) and
+    reliable-update CG (mixed = 3) all reach the oracle's red-black solution."""
     dims = (8, 8, 8, 8)
     U, src = inputs(dims, 1, 51)
     O = Oracle(U, dims, 1, 'wilson', mass=0.1)
     ref, rst = O.schur_solve(src, tol=1e-10)
     D = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=0.1)
     fs, fsol = D.field().upload(src), D.field()
-    st = D.schur_solve(fs, fsol, tol=1e-10)
+    st = D.schur_solve(fs, fsol, tol=1e-10, mixed=mixed)
     assert st['converged'] and st['unprec_residual'] < 1e-8
     assert rel_err(fsol.download(), ref) < 1e-8
+    if mixed == 3:
+        assert st['outer_iterations'] >= 1 and st['inner_iterations'] > 0, st   # reliable updates ran
+        M = gx.FermionOp(ctx, U, dims, 4, 'wilson', mass=0.1)
+        with pytest.raises(gx.GxError):        # multi-RHS operators take CGmultiRHS forms only
+            M.schur_solve(M.field(), M.field(), mixed=3)
 
 
 @pytest.mark.parametrize('kind,Ls', [('wilson', 1), ('mobius', 8), ('wilson', 4)])
