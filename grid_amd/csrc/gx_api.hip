@@ -204,6 +204,18 @@ int blas_inner(Field *x, Field *y, double *re, double *im) {
   return GX_OK;
 }
 
+// Re<x|y> and |x|^2 in one pass and one read-back
+int blas_inner_norm(Field *x, Field *y, double *re, double *nrm) {
+  GX_TRY(by_prec(x->prec, [&](auto t) {
+    return launch_reduce<decltype(t)>(x->ctx, 3, nsites_of(x), x->g.half_pad(), x->data, y->data, nullptr, 0, 0);
+  }));
+  double v[2];
+  GX_TRY(reduce_read(x->ctx, 0, 2, v));
+  *re = v[0];
+  *nrm = v[1];
+  return GX_OK;
+}
+
 int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm) {
   GX_TRY(by_prec(x->prec, [&](auto t) {
     return launch_reduce<decltype(t)>(x->ctx, 2, nsites_of(x), x->g.half_pad(), x->data, y->data, z->data, a, 0);

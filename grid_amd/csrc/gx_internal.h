@@ -207,7 +207,7 @@ template <typename S>
 int launch_axpby(int64_t nsites, int64_t stride, double a, const void *x, double b, const void *y, void *z,
                  const Gate *gate, hipStream_t st);
 // reductions into ctx slot(s); mode 0: <x|y> -> slot, slot+1 (re, im); 1: |x|^2 -> slot;
-// 2: z = a x + y and |z|^2 -> slot (fused axpy_norm)
+// 2: z = a x + y and |z|^2 -> slot (fused axpy_norm); 3: Re<x|y> -> slot, |x|^2 -> slot+1
 template <typename T>
 int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void *x, const void *y, void *z, double a,
                   int slot);
@@ -290,6 +290,7 @@ int apply_full_m(Op *op, Field *in, Field *out, int dag);
 int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const Gate *gate = nullptr);
 int blas_norm2(Field *x, double *nrm);
 int blas_inner(Field *x, Field *y, double *re, double *im);
+int blas_inner_norm(Field *x, Field *y, double *re, double *nrm);  // Re<x|y>, |x|^2 in one pass
 int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm);
 int convert_field(Field *out, Field *in);
 

@@ -894,6 +894,11 @@ __global__ void __launch_bounds__(256) k_reduce(int64_t nsites, int64_t stride, 
       } else if (MODE == 1) {
         sr = fma(xv[k].r, xv[k].r, sr);
         sr = fma(xv[k].i, xv[k].i, sr);
+      } else if (MODE == 3) {  // Re<x|y> and |x|^2 in one pass
+        sr = fma(xv[k].r, yv[k].r, sr);
+        sr = fma(xv[k].i, yv[k].i, sr);
+        si = fma(xv[k].r, xv[k].r, si);
+        si = fma(xv[k].i, xv[k].i, si);
       } else {
         yv[k] = {fma(a, xv[k].r, yv[k].r), fma(a, xv[k].i, yv[k].i)};
         sr = fma(yv[k].r, yv[k].r, sr);
@@ -938,11 +943,14 @@ int launch_reduce(Ctx *ctx, int mode, int64_t nsites, int64_t stride, const void
   else if (mode == 1)
     hipLaunchKernelGGL((k_reduce<S, 1>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
                        ctx->red_partial);
+  else if (mode == 3)
+    hipLaunchKernelGGL((k_reduce<S, 3>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
+                       ctx->red_partial);
   else
     hipLaunchKernelGGL((k_reduce<S, 2>), dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, px, py, pz, (T)a,
                        ctx->red_partial);
   hipLaunchKernelGGL(k_reduce_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb,
-                     ctx->red_result + slot, mode == 0 ? 2 : 1);
+                     ctx->red_result + slot, (mode == 0 || mode == 3) ? 2 : 1);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }

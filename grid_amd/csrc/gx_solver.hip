@@ -1022,9 +1022,8 @@ static int multishift_run(Op *op, Field *src, int nshift, const double *mass, co
   double d, rn, c;
   auto shifted_op = [&](double &dd) -> int {
     GX_TRY(apply_mpcdagmpc(op, p, mmp));
-    GX_TRY(blas_inner(p, mmp, &dd, nullptr));
+    GX_TRY(blas_inner_norm(p, mmp, &dd, &rn));  // Re<p|A p> and |p|^2
     GX_TRY(blas_axpby(mmp, mass[0], p, 1.0, mmp));
-    GX_TRY(blas_norm2(p, &rn));
     dd += rn * mass[0];
     return GX_OK;
   };
