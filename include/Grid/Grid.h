@@ -813,6 +813,7 @@ class MixedPrecisionConjugateGradient {
   LinearOperatorBase<FieldD> &Linop_d;
   Integer TotalInnerIterations = 0, TotalOuterIterations = 0, TotalFinalStepIterations = 0;
   RealD TrueResidual = 0;
+  bool InnerHalf = false;  // no Grid counterpart: FP16-storage inner solves instead of FP32
   MixedPrecisionConjugateGradient(RealD tol, Integer maxinnerit, Integer maxouterit, GridBase *sp_grid,
                                   LinearOperatorBase<FieldF> &lf, LinearOperatorBase<FieldD> &ld)
       : Tolerance(tol), InnerTolerance(tol), MaxInnerIterations(maxinnerit), MaxOuterIterations(maxouterit),
@@ -822,8 +823,8 @@ class MixedPrecisionConjugateGradient {
     assert(op && Linop_f.schur_handle());
     sol_d.Checkerboard() = src_d_in.Checkerboard();
     gx_solver_stats st{};
-    const int rc = gx_mixed_cg(op, src_d_in.f, sol_d.f, Tolerance, InnerTolerance, GX_F32, MaxInnerIterations,
-                               MaxOuterIterations, &st);
+    const int rc = gx_mixed_cg(op, src_d_in.f, sol_d.f, Tolerance, InnerTolerance, InnerHalf ? GX_F16 : GX_F32,
+                               MaxInnerIterations, MaxOuterIterations, &st);
     TotalInnerIterations = st.inner_iterations;
     TotalOuterIterations = st.outer_iterations;
     TotalFinalStepIterations = st.final_iterations;

@@ -1,6 +1,6 @@
 // Mirrors Grid's tests/Test_dwf_mixedcg_prec.cc: Mobius DWF Schur system solved with
-// MixedPrecisionConjugateGradient (FP32 inner, FP64 outer) and with plain FP64 ConjugateGradient;
-// both reach the FP64 tolerance and agree.
+// MixedPrecisionConjugateGradient (FP32 inner, then FP16-storage inner, FP64 outer) and with plain
+// FP64 ConjugateGradient; all reach the FP64 tolerance and agree.
 #include <Grid/Grid.h>
 
 using namespace Grid;
@@ -43,9 +43,17 @@ int main(int argc, char **argv) {
   CG(HermOpEO, src_o, result_o_2);
   LatticeFermionD diff = result_o - result_o_2;
   const double rel = std::sqrt(norm2(diff) / norm2(result_o_2));
-  std::cout << GridLogMessage << "mixed true residual " << mCG.TrueResidual << ", FP64 true residual "
-            << CG.TrueResidual << ", relative difference " << rel << std::endl;
-  const int fail = !(mCG.TrueResidual < 1e-9) || !(CG.TrueResidual < 1e-9) || !(rel < 1e-7);
+  // the same solve with FP16-storage inner solves
+  LatticeFermionD result_h(FrbGrid);
+  result_h = Zero();
+  result_h.Checkerboard() = Odd;
+  mCG.InnerHalf = true;
+  mCG(src_o, result_h);
+  LatticeFermionD diffh = result_h - result_o_2;
+  const double relh = std::sqrt(norm2(diffh) / norm2(result_o_2));
+  std::cout << GridLogMessage << "mixed true residual " << mCG.TrueResidual << " (FP16 inner), FP64 true residual "
+            << CG.TrueResidual << ", relative differences " << rel << " " << relh << std::endl;
+  const int fail = !(mCG.TrueResidual < 1e-9) || !(CG.TrueResidual < 1e-9) || !(rel < 1e-7) || !(relh < 1e-7);
   Grid_finalize();
   std::cout << (fail ? "FAILED" : "PASSED") << std::endl;
   return fail;
