@@ -35,10 +35,14 @@ def op(ctx, kind, Ls, dims, seed):
     return U, gx.FermionOp(ctx, U, dims, Ls, kind, mass=0.1)
 
 
+@pytest.mark.parametrize('phases', [None, (1, 1, 1, -1), (1j, 1, np.exp(0.3j), -1)])
 @pytest.mark.parametrize('kind,Ls', [('wilson', 1), ('mobius', 4)])
-def test_dhopdir_legs_sum_to_dhop(ctx, kind, Ls):
+def test_dhopdir_legs_sum_to_dhop(ctx, kind, Ls, phases):
+    """The eight legs sum to Dhop, also with antiperiodic and twisted boundary phases (the legs read
+    the same phase-carrying double-stored links)."""
     dims = (4, 6, 4, 8)
-    _, D = op(ctx, kind, Ls, dims, 601)
+    U = gx.random_gauge(dims, 601, 0.5)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, mass=0.1, phases=phases)
     psi = D.field().upload(gx.gaussian_fermion(dims, Ls, 602))
     total = np.zeros(psi.shape, dtype=np.complex128)
     out = D.field()

@@ -78,3 +78,22 @@ def test_multishift_tolerances_and_errors(ctx):
         D.cg_multishift(fs, psis, [0.1, 0.01, 1.0], tols)
     with pytest.raises(gx.GxError):
         D.cg_multishift(fs, [psis[0], psis[0], psis[2]], shifts, tols)
+
+
+def test_multishift_maximum_shifts(ctx):
+    """32 shifts (the kernel's maximum) all converge; 33 are refused."""
+    dims, mass = (4, 4, 4, 8), 0.1
+    U = gx.random_gauge(dims, 421, 0.3)
+    D = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=mass)
+    src = R.pick(gx.gaussian_fermion(dims, 1, 422), dims, ODD)
+    fs = D.field(gx.F64, ODD).upload(src)
+    shifts = list(np.geomspace(1e-3, 10.0, 32))
+    psis = [D.field(gx.F64, ODD) for _ in range(33)]
+    sts = D.cg_multishift(fs, psis[:32], shifts, [1e-10] * 32)
+    O = Oracle(U, dims, 1, 'wilson', mass=mass)
+    for s in (0, 15, 31):
+        x = psis[s].download()
+        res = norm(O.schur('MpcDagMpc', x) + shifts[s] * x - src) / norm(src)
+        assert sts[s]['converged'] and res < 1e-9, (s, res)
+    with pytest.raises(gx.GxError):
+        D.cg_multishift(fs, psis, shifts + [20.0], [1e-10] * 33)
