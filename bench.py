@@ -139,6 +139,84 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tu
     return out
 
 
+def bench_multishift(gx, ctx, dims=(16, 16, 16, 16), Ls=16, nshift=12, tol=1e-10):
+    """Opt-in leg 'ms': ConjugateGradientMultiShift on the Mobius 16^4 x Ls16 Schur system (m 0.01,
+    eps 0.3, Gaussian odd source), nshift shifts geometrically spaced in [1e-4, 3], every shift to
+    tol, against one plain CG solve of the lightest system.  Wall clock, best of two."""
+    U = gx.random_gauge(dims, 1002, 0.3)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', mass=0.01, M5=1.8, b=1.5, c=0.5)
+    full = D.field().gaussian(2002)
+    src = D.field(gx.F64, gx.ODD)
+    gx.pick_checkerboard(1, src, full)
+    out = {'workload': f'Mobius {dims[0]}^4 x Ls{Ls} m=0.01, {nshift} shifts in [1e-4, 3]', 'tol': tol}
+    best = None
+    for _ in range(2):
+        x = D.field(gx.F64, gx.ODD)
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = D.cg(src, x, tol=tol, maxit=20000)
+        ctx.sync()
+        sec = time.perf_counter() - t0
+        best = sec if best is None else min(best, sec)
+    out['cg'] = {'seconds': round(best, 4), 'iterations': st['iterations'],
+                 'ms_per_iteration': round(1e3 * best / st['iterations'], 4)}
+    shifts = list(np.geomspace(1e-4, 3.0, nshift))
+    psis = [D.field(gx.F64, gx.ODD) for _ in range(nshift)]
+    best = None
+    for _ in range(2):
+        ctx.sync()
+        t0 = time.perf_counter()
+        sts = D.cg_multishift(src, psis, shifts, [tol] * nshift, maxit=20000)
+        ctx.sync()
+        sec = time.perf_counter() - t0
+        best = sec if best is None else min(best, sec)
+    k = max(s['iterations'] for s in sts)
+    worst = max(s['true_residual'] for s in sts)
+    out['multishift'] = {'seconds': round(best, 4), 'iterations': k, 'ms_per_iteration': round(1e3 * best / k, 4),
+                         'max_true_residual': worst, 'converged': all(s['converged'] for s in sts),
+                         'vs_cg_per_iteration': round((best / k) / (out['cg']['seconds'] / out['cg']['iterations']), 3)}
+    return out
+
+
+def bench_block_cgrq(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-11):
+    """Opt-in leg 'block': the C4 Schur system (Wilson 32^3x64 m 0.05, the red-black sources of the
+    12 point sources of one site) solved by CGmultiRHS and by BlockCGrQ in FP64, to tol; wall clock,
+    iterations and the agreement of the two solutions."""
+    nrhs = 12
+    U = gx.random_gauge(dims, 1004, eps)
+    rng = np.random.default_rng(4004)
+    x0 = tuple(int(rng.integers(d)) for d in dims)
+    B = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+    full = B.field().point_source(x0)
+    se, so = B.field(gx.F64, gx.EVEN), B.field(gx.F64, gx.ODD)
+    gx.pick_checkerboard(0, se, full)
+    gx.pick_checkerboard(1, so, full)
+    te, to, so2, b = B.field(gx.F64, gx.EVEN), B.field(gx.F64, gx.ODD), B.field(gx.F64, gx.ODD), B.field(gx.F64, gx.ODD)
+    B.MooeeInv(se, te)
+    B.Meooe(te, to)
+    gx.axpby(so2, 1.0, -1.0, so, to)
+    B.MpcDag(so2, b)          # RedBlackSource
+    out = {'workload': 'Wilson 32^3x64 m=0.05, 12 point sources (red-black system)', 'tol': tol}
+    sols = {}
+    for name in ('cgmultirhs', 'blockcgrq'):
+        best = None
+        for _ in range(2):
+            x = B.field(gx.F64, gx.ODD)
+            ctx.sync()
+            t0 = time.perf_counter()
+            st = B.block_cg(b, x, tol=tol, maxit=20000) if name == 'cgmultirhs' else B.block_cgrq(b, x, tol=tol, maxit=20000)
+            ctx.sync()
+            sec = time.perf_counter() - t0
+            best = sec if best is None else min(best, sec)
+        sols[name] = x
+        out[name] = {'seconds': round(best, 4), 'iterations': max(s['iterations'] for s in st),
+                     'max_true_residual': max(s['true_residual'] for s in st)}
+    t = B.field(gx.F64, gx.ODD)
+    gx.axpby(t, 1.0, -1.0, sols['blockcgrq'], sols['cgmultirhs'])
+    out['rel_diff'] = (t.norm2() / sols['cgmultirhs'].norm2()) ** 0.5
+    return out
+
+
 def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=CG_TOL, tune=None):
     """Wilson 32^3x64 (eps 0.3, m 0.05), the 12 spin-colour point sources of one seeded site:
     mixed-precision multi-RHS red-black solves (CGmultiRHS inner solves in FP32, and in FP16
@@ -235,7 +313,8 @@ def main():
     ap.add_argument('--serve-mixed', type=int, default=2, choices=(1, 2),
                     help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
     ap.add_argument('--legs', default='dhop,cg,c3,c4,serve,cpu',
-                    help='comma list of legs to run: dhop, cg, c3, c4, serve, cpu (profiling runs pick one)')
+                    help='comma list of legs to run: dhop, cg, c3, c4, serve, cpu, and the opt-in ms (multishift '
+                         'CG) and block (BlockCGrQ); profiling runs pick one')
     ap.add_argument('--no-graphs', action='store_true',
                     help='solver iterations as plain launches (for profiler runs; same kernels and timings)')
     ap.add_argument('--no-tune', action='store_true',
@@ -323,6 +402,8 @@ def main():
     c3 = None if args.no_c3 else bench_mixed_dwf(gx, ctx, tune=tune)
     c4 = None if args.no_c4 else bench_multi_rhs(gx, ctx, tune=tune)
     serve = bench_serving(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_mixed) if args.serve_requests > 0 else None
+    ms = bench_multishift(gx, ctx) if 'ms' in legs else None
+    blk = bench_block_cgrq(gx, ctx) if 'block' in legs else None
 
     cpu = None
     if not args.no_cpu:
@@ -359,6 +440,8 @@ def main():
         'cg': cg,
         'c3_mixed_dwf': c3,
         'c4_multi_rhs': c4,
+        'multishift': ms,
+        'block_cgrq': blk,
         'serving': serve,
     }
     print(json.dumps(out))

@@ -1635,59 +1635,75 @@ static __global__ void __launch_bounds__(256) k_slice_ipm(int64_t nsite, int64_t
   if (t < ls2) partial[(int64_t)blockIdx.x * ls2 + t] = {ar, ai};
 }
 
+// workgroup p adds the partials of matrix entry p: thread t the partials t, t + 256, ... in order,
+// then a fixed tree over the threads (bitwise reproducible)
 static __global__ void __launch_bounds__(256) k_slice_ipm_fin(const double2 *partial, int nparts, int ls2,
                                                               double2 *out) {
-  const int t = threadIdx.x;
-  if (t >= ls2) return;
+  const int t = threadIdx.x, p = blockIdx.x;
   double ar = 0, ai = 0;
-  for (int b = 0; b < nparts; b++) {
-    ar += partial[(int64_t)b * ls2 + t].x;
-    ai += partial[(int64_t)b * ls2 + t].y;
+  for (int b = t; b < nparts; b += 256) {
+    ar += partial[(int64_t)b * ls2 + p].x;
+    ai += partial[(int64_t)b * ls2 + p].y;
   }
-  out[t] = {ar, ai};
+  __shared__ double sh[2][256];
+  sh[0][t] = ar;
+  sh[1][t] = ai;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) {
+      sh[0][t] += sh[0][t + off];
+      sh[1][t] += sh[1][t + off];
+    }
+    __syncthreads();
+  }
+  if (t == 0) out[p] = {sh[0][0], sh[1][0]};
 }
 
 // sliceMaddMatrix: r_t = y_t + scale sum_s x_s a(s, t), a row-major (a[s * Ls + t]); y = null reads
-// as zero (sliceMulMatrix).  One thread per (site, component) reads every slice of x and y before
-// writing r, so r may alias x or y.
+// as zero (sliceMulMatrix).  A workgroup stages kMaddSites sites of x (every slice and component)
+// in LDS with coalesced loads, then each thread forms outputs (site, t, component) from the staged
+// slices and writes them, again coalesced.  Every x of a stage is read before any r of it is
+// written (and y at an index only by the thread that writes r there), so r may alias x or y.
+constexpr int kMaddSites = 4;
 static __global__ void __launch_bounds__(256) k_slice_madd(int64_t nsite, int64_t H4, int Ls, int64_t hstride,
                                                            const double2 *a, double scale, const cx<double> *x,
                                                            const cx<double> *y, cx<double> *r) {
   __shared__ double2 sa[kMaxLs * kMaxLs];
-  for (int q = threadIdx.x; q < Ls * Ls; q += blockDim.x) sa[q] = {scale * a[q].x, scale * a[q].y};
-  __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 12 * nsite;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int k = (int)(i / nsite);
-    const int64_t gs = i - k * nsite;
-    double2 xs[kMaxLs];
-#pragma unroll
-    for (int s = 0; s < kMaxLs; s++)
-      if (s < Ls) {
-        const cx<double> v = x[slice_idx(gs, s, k, H4, Ls, hstride)];
-        xs[s] = {v.r, v.i};
+  __shared__ double2 sx[kMaddSites * kMaxLs * 12];
+  const int t = threadIdx.x;
+  for (int q = t; q < Ls * Ls; q += blockDim.x) sa[q] = {scale * a[q].x, scale * a[q].y};
+  const int per = kMaddSites * Ls;
+  for (int64_t g0 = (int64_t)blockIdx.x * kMaddSites; g0 < nsite; g0 += (int64_t)gridDim.x * kMaddSites) {
+    const int ns = (int)min((int64_t)kMaddSites, nsite - g0);
+    __syncthreads();  // the previous stage's LDS reads are done (and sa is written)
+    for (int q = t; q < 12 * per; q += 256) {
+      const int k = q / per, j = q - k * per;  // component k, staged entry j = site * Ls + s
+      if (j < ns * Ls) {
+        const int hs = j / Ls;
+        const cx<double> v = x[slice_idx(g0 + hs, j - hs * Ls, k, H4, Ls, hstride)];
+        sx[j * 12 + k] = {v.r, v.i};
       }
-    double2 out[kMaxLs];
-#pragma unroll
-    for (int t = 0; t < kMaxLs; t++)
-      if (t < Ls) {
+    }
+    __syncthreads();
+    for (int q = t; q < 12 * per; q += 256) {
+      const int k = q / per, j = q - k * per;  // output component k, entry j = site * Ls + tt
+      if (j < ns * Ls) {
+        const int hs = j / Ls, tt = j - hs * Ls;
+        const int64_t idx = slice_idx(g0 + hs, tt, k, H4, Ls, hstride);
         double orr = 0, oi = 0;
         if (y) {
-          const cx<double> v = y[slice_idx(gs, t, k, H4, Ls, hstride)];
+          const cx<double> v = y[idx];
           orr = v.r, oi = v.i;
         }
-#pragma unroll
-        for (int s = 0; s < kMaxLs; s++)
-          if (s < Ls) {
-            const double2 c = sa[s * Ls + t];
-            orr = fma(xs[s].x, c.x, fma(-xs[s].y, c.y, orr));
-            oi = fma(xs[s].x, c.y, fma(xs[s].y, c.x, oi));
-          }
-        out[t] = {orr, oi};
+        const double2 *xs = sx + hs * Ls * 12 + k;
+        for (int s2 = 0; s2 < Ls; s2++) {
+          const double2 xv = xs[s2 * 12], c = sa[s2 * Ls + tt];
+          orr = fma(xv.x, c.x, fma(-xv.y, c.y, orr));
+          oi = fma(xv.x, c.y, fma(xv.y, c.x, oi));
+        }
+        r[idx] = {orr, oi};
       }
-#pragma unroll
-    for (int t = 0; t < kMaxLs; t++)
-      if (t < Ls) r[slice_idx(gs, t, k, H4, Ls, hstride)] = {out[t].x, out[t].y};
+    }
   }
 }
 
@@ -1703,8 +1719,8 @@ int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const vo
   if (nb > nparts) nb = nparts;
   hipLaunchKernelGGL(k_slice_ipm, dim3(nb), dim3(256), 0, ctx->stream, nsite, g.H4, g.Ls, hstride,
                      (const cx<double> *)x, (const cx<double> *)y, (double2 *)partial);
-  hipLaunchKernelGGL(k_slice_ipm_fin, dim3(1), dim3(256), 0, ctx->stream, (const double2 *)partial, (int)nb,
-                     g.Ls * g.Ls, (double2 *)out_dev);
+  hipLaunchKernelGGL(k_slice_ipm_fin, dim3(g.Ls * g.Ls), dim3(256), 0, ctx->stream, (const double2 *)partial,
+                     (int)nb, g.Ls * g.Ls, (double2 *)out_dev);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1777,7 +1793,9 @@ int launch_slice_madd(Ctx *ctx, const Geom &g, int nhalf, const void *a_dev, dou
     return GX_EINVAL;
   }
   const int64_t nsite = nhalf * g.H4, hstride = 12 * g.half_pad();
-  hipLaunchKernelGGL(k_slice_madd, dim3(stream_blocks(12 * nsite)), dim3(256), 0, ctx->stream, nsite, g.H4, g.Ls,
+  int64_t nb = (nsite + kMaddSites - 1) / kMaddSites;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_slice_madd, dim3(nb), dim3(256), 0, ctx->stream, nsite, g.H4, g.Ls,
                      hstride, (const double2 *)a_dev, scale, (const cx<double> *)x, (const cx<double> *)y,
                      (cx<double> *)r);
   GX_CHECK_HIP(hipGetLastError());

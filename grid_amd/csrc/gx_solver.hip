@@ -830,7 +830,7 @@ struct BlockWork {
   void *partial = nullptr;   // workgroup partials of the slice inner products
   double2 *mat_dev = nullptr;   // result matrix + 4 coefficient matrices, kMaxLs^2 each
   double2 *mat_host = nullptr;  // pinned mirror
-  int nparts = 512;
+  int nparts = 2048;  // workgroups of the slice inner products (8 per CU)
   ~BlockWork() {
     if (partial) (void)hipFree(partial);
     if (mat_dev) (void)hipFree(mat_dev);
