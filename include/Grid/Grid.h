@@ -947,9 +947,12 @@ class ConjugateGradientMultiShift : public OperatorFunction<Field> {
   }
   void operator()(LinearOperatorBase<Field> &Linop, const Field &src, std::vector<Field> &psi) {
     gx_op op = Linop.schur_handle();
-    assert(op && "ConjugateGradientMultiShift runs on a Schur operator over a libgx action");
     const int n = shifts.order;
     assert((int)psi.size() == n && (int)shifts.poles.size() == n && (int)shifts.tolerances.size() == n);
+    if (!op) {
+      host_loop(Linop, src, psi);
+      return;
+    }
     std::vector<gx_field> h(n);
     for (int s = 0; s < n; s++) {
       psi[s].Checkerboard() = src.Checkerboard();
@@ -971,6 +974,102 @@ class ConjugateGradientMultiShift : public OperatorFunction<Field> {
                   << " after " << st[s].iterations << " iterations" << std::endl;
     }
     if (rc == GX_ENOCONV) std::cout << GridLogMessage << "CG multi shift did not converge" << std::endl;
+  }
+
+ private:
+  // Any other LinearOperatorBase: Grid's loop (ConjugateGradientMultiShift.h:130-338) over the
+  // facade's device BLAS1, one operator call per iteration through the user's HermOp.
+  void host_loop(LinearOperatorBase<Field> &Linop, const Field &src, std::vector<Field> &psi) {
+    const int n = shifts.order;
+    const std::vector<RealD> &mass = shifts.poles, &mres = shifts.tolerances;
+    for (int s = 0; s < n; s++) assert(mass[s] >= mass[0]);
+    GridBase *grid = src.Grid();
+    std::vector<Field> ps;
+    for (int s = 0; s < n; s++) {
+      ps.emplace_back(grid);
+      ps[s].Checkerboard() = src.Checkerboard();
+      psi[s].Checkerboard() = src.Checkerboard();
+    }
+    Field r(grid), p(grid), mmp(grid), tmp(grid);
+    r.Checkerboard() = p.Checkerboard() = mmp.Checkerboard() = tmp.Checkerboard() = src.Checkerboard();
+    std::vector<RealD> bs(n), rsq(n), z0(n, 1.0), z1(n, 1.0);  // z1 = z[s][iz], z0 = z[s][1-iz]
+    std::vector<int> conv(n, 0);
+    IterationsToCompleteShift.assign(n, 0);
+    TrueResidualShift.assign(n, 0.0);
+    RealD cp = norm2(src);
+    if (cp == 0.0) {
+      for (int s = 0; s < n; s++) {
+        psi[s] = Zero();
+        IterationsToCompleteShift[s] = 1;
+      }
+      return;
+    }
+    for (int s = 0; s < n; s++) {
+      rsq[s] = cp * mres[s] * mres[s];
+      ps[s] = src;
+    }
+    r = src;
+    p = src;
+    RealD d, qq;
+    Linop.HermOpAndNorm(p, mmp, d, qq);
+    axpy(mmp, mass[0], p, mmp);
+    d += norm2(p) * mass[0];
+    RealD b = -cp / d, c;
+    bs[0] = b;
+    for (int s = 1; s < n; s++) {
+      z1[s] = 1.0 / (1.0 - b * (mass[s] - mass[0]));
+      bs[s] = b * z1[s];
+    }
+    c = axpy_norm(r, b, mmp, r);
+    for (int s = 0; s < n; s++) axpby(psi[s], 0.0, -bs[s], src, src);
+    int k;
+    for (k = 1; k <= MaxIterations; k++) {
+      const RealD a = c / cp;
+      axpy(p, a, p, r);
+      for (int s = 0; s < n; s++) {
+        if (conv[s]) continue;
+        if (s == 0) axpy(ps[0], a, ps[0], r);
+        else axpby(ps[s], z1[s], a * z1[s] * bs[s] / (z0[s] * b), r, ps[s]);
+      }
+      cp = c;
+      Linop.HermOp(p, mmp);
+      d = innerProduct(p, mmp).real();
+      axpy(mmp, mass[0], p, mmp);
+      d += norm2(p) * mass[0];
+      const RealD bp = b;
+      b = -cp / d;
+      c = axpy_norm(r, b, mmp, r);
+      bs[0] = b;
+      for (int s = 1; s < n; s++) {
+        if (conv[s]) continue;
+        const RealD zo = z1[s], zp = z0[s];  // current and previous before the toggle
+        const RealD zn = zo * zp * bp / (b * a * (zp - zo) + zp * bp * (1.0 - (mass[s] - mass[0]) * b));
+        z0[s] = zo;
+        z1[s] = zn;
+        bs[s] = b * zn / zo;
+      }
+      for (int s = 0; s < n; s++)
+        if (!conv[s]) axpy(psi[s], -bs[s], ps[s], psi[s]);
+      bool all = true;
+      for (int s = 0; s < n; s++) {
+        if (conv[s]) continue;
+        IterationsToCompleteShift[s] = k;
+        if (c * z1[s] * z1[s] < rsq[s]) conv[s] = 1;
+        else all = false;
+      }
+      if (all) break;
+    }
+    IterationsToComplete = std::min<Integer>(k, MaxIterations);
+    const RealD cn = norm2(src);
+    for (int s = 0; s < n; s++) {
+      Linop.HermOp(psi[s], mmp);
+      axpy(tmp, mass[s], psi[s], mmp);
+      axpy(r, -1.0, src, tmp);
+      TrueResidualShift[s] = std::sqrt(norm2(r) / cn);
+      if (verbose)
+        std::cout << GridLogMessage << "CGMultiShift: shift[" << s << "] true residual " << TrueResidualShift[s]
+                  << " after " << IterationsToCompleteShift[s] << " iterations" << std::endl;
+    }
   }
 };
 
