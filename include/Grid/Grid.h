@@ -814,13 +814,17 @@ class MixedPrecisionConjugateGradient {
   Integer TotalInnerIterations = 0, TotalOuterIterations = 0, TotalFinalStepIterations = 0;
   RealD TrueResidual = 0;
   bool InnerHalf = false;  // no Grid counterpart: FP16-storage inner solves instead of FP32
+  RealD OuterLoopNormMult = 100.;
   MixedPrecisionConjugateGradient(RealD tol, Integer maxinnerit, Integer maxouterit, GridBase *sp_grid,
                                   LinearOperatorBase<FieldF> &lf, LinearOperatorBase<FieldD> &ld)
       : Tolerance(tol), InnerTolerance(tol), MaxInnerIterations(maxinnerit), MaxOuterIterations(maxouterit),
         SinglePrecGrid(sp_grid), Linop_f(lf), Linop_d(ld) {}
   void operator()(const FieldD &src_d_in, FieldD &sol_d) {
     gx_op op = Linop_d.schur_handle();
-    assert(op && Linop_f.schur_handle());
+    if (!op || !Linop_f.schur_handle()) {
+      host_loop(src_d_in, sol_d);
+      return;
+    }
     sol_d.Checkerboard() = src_d_in.Checkerboard();
     gx_solver_stats st{};
     const int rc = gx_mixed_cg(op, src_d_in.f, sol_d.f, Tolerance, InnerTolerance, InnerHalf ? GX_F16 : GX_F32,
@@ -833,6 +837,43 @@ class MixedPrecisionConjugateGradient {
     std::cout << GridLogMessage << "MixedPrecisionConjugateGradient: Inner CG iterations " << TotalInnerIterations
               << " Restarts " << TotalOuterIterations << " Final CG iterations " << TotalFinalStepIterations
               << " True residual " << TrueResidual << std::endl;
+  }
+
+ private:
+  // Operators without a device handle: Grid's loop (ConjugateGradientMixedPrec.h:71-165) with the
+  // facade's ConjugateGradient in each precision and precisionChange between them.
+  void host_loop(const FieldD &src_d_in, FieldD &sol_d) {
+    TotalInnerIterations = 0;
+    const int cb = src_d_in.Checkerboard();
+    sol_d.Checkerboard() = cb;
+    const RealD stop = norm2(src_d_in) * Tolerance * Tolerance;
+    GridBase *gd = src_d_in.Grid();
+    FieldD tmp_d(gd), src_d(gd);
+    tmp_d.Checkerboard() = cb;
+    src_d = src_d_in;
+    RealD inner_tol = InnerTolerance;
+    FieldF src_f(SinglePrecGrid), sol_f(SinglePrecGrid);
+    src_f.Checkerboard() = cb;
+    sol_f.Checkerboard() = cb;
+    ConjugateGradient<FieldF> CG_f(inner_tol, MaxInnerIterations, false);
+    for (TotalOuterIterations = 0; TotalOuterIterations < MaxOuterIterations; TotalOuterIterations++) {
+      Linop_d.HermOp(sol_d, tmp_d);
+      const RealD norm = axpy_norm(src_d, -1., tmp_d, src_d_in);  // residual
+      if (norm < OuterLoopNormMult * stop) break;
+      while (norm * inner_tol * inner_tol < stop) inner_tol *= 2;
+      precisionChange(src_f, src_d);
+      sol_f = Zero();
+      sol_f.Checkerboard() = cb;
+      CG_f.Tolerance = inner_tol;
+      CG_f(Linop_f, src_f, sol_f);
+      TotalInnerIterations += CG_f.IterationsToComplete;
+      precisionChange(tmp_d, sol_f);
+      axpy(sol_d, 1.0, tmp_d, sol_d);
+    }
+    ConjugateGradient<FieldD> CG_d(Tolerance, MaxInnerIterations);
+    CG_d(Linop_d, src_d_in, sol_d);
+    TotalFinalStepIterations = CG_d.IterationsToComplete;
+    TrueResidual = CG_d.TrueResidual;
   }
 };
 
