@@ -913,7 +913,10 @@ class ConjugateGradientReliableUpdate {
   }
   void operator()(const FieldD &src, FieldD &psi) {
     gx_op op = Linop_d.schur_handle();
-    assert(op && Linop_f.schur_handle());
+    if (!op || !Linop_f.schur_handle()) {
+      host_loop(src, psi);
+      return;
+    }
     psi.Checkerboard() = src.Checkerboard();
     gx_solver_stats st{};
     const int rc = SloppyHalf && Linop_fallback
@@ -934,6 +937,88 @@ class ConjugateGradientReliableUpdate {
     std::cout << GridLogMessage << "ConjugateGradientReliableUpdate Converged on iteration " << IterationsToComplete
               << " after " << ReliableUpdatesPerformed << " reliable updates; cleanup iterations "
               << IterationsToCleanup << "; true residual " << TrueResidual << std::endl;
+  }
+
+ private:
+  // Operators without a device handle: Grid's loop (ConjugateGradientReliableUpdate.h:77-269) over the
+  // facade's BLAS1 with FieldF sloppy vectors (SloppyHalf does not apply), fallback operator included.
+  void host_loop(const FieldD &src, FieldD &psi) {
+    const int cb = src.Checkerboard();
+    psi.Checkerboard() = cb;
+    GridBase *gd = src.Grid();
+    LinearOperatorBase<FieldF> *Linop_f_use = &Linop_f;
+    bool using_fallback = false;
+    FieldD p(gd), mmp(gd), r(gd);
+    p.Checkerboard() = mmp.Checkerboard() = r.Checkerboard() = cb;
+    RealD cp, c, a, d, b, ssq, qq;
+    assert(!std::isnan(norm2(psi)));
+    Linop_d.HermOpAndNorm(psi, mmp, d, b);
+    r = src - mmp;
+    p = r;
+    a = norm2(p);
+    cp = a;
+    ssq = norm2(src);
+    const RealD rsq = Tolerance * Tolerance * ssq;
+    IterationsToComplete = ReliableUpdatesPerformed = IterationsToCleanup = 0;
+    if (cp <= rsq) {
+      TrueResidual = std::sqrt(cp / ssq);
+      return;
+    }
+    FieldF r_f(SinglePrecGrid), psi_f(SinglePrecGrid), p_f(SinglePrecGrid), mmp_f(SinglePrecGrid);
+    r_f.Checkerboard() = psi_f.Checkerboard() = p_f.Checkerboard() = mmp_f.Checkerboard() = cb;
+    precisionChange(r_f, r);
+    psi_f = Zero();
+    psi_f.Checkerboard() = cb;
+    p_f = r_f;
+    RealD MaxResidSinceLastRelUp = cp;
+    int k, l = 0;
+    for (k = 1; k <= MaxIterations; k++) {
+      c = cp;
+      Linop_f_use->HermOpAndNorm(p_f, mmp_f, d, qq);
+      a = c / d;
+      cp = axpy_norm(r_f, -a, mmp_f, r_f);
+      b = cp / c;
+      axpy(psi_f, a, p_f, psi_f);
+      if (cp > MaxResidSinceLastRelUp) MaxResidSinceLastRelUp = cp;
+      if (cp <= rsq) {
+        precisionChange(mmp, psi_f);
+        psi = psi + mmp;
+        IterationsToComplete = k;
+        ReliableUpdatesPerformed = l;
+        if (DoFinalCleanup) {
+          ConjugateGradient<FieldD> CG(Tolerance, MaxIterations, ErrorOnNoConverge);
+          CG(Linop_d, src, psi);
+          IterationsToCleanup = CG.IterationsToComplete;
+          TrueResidual = CG.TrueResidual;
+        } else {
+          Linop_d.HermOpAndNorm(psi, mmp, d, qq);
+          p = mmp - src;
+          TrueResidual = std::sqrt(norm2(p) / ssq);
+        }
+        return;
+      } else if (cp < Delta * MaxResidSinceLastRelUp) {  // reliable update
+        precisionChange(mmp, psi_f);
+        psi = psi + mmp;
+        Linop_d.HermOpAndNorm(psi, mmp, d, qq);
+        r = src - mmp;
+        psi_f = Zero();
+        psi_f.Checkerboard() = cb;
+        precisionChange(r_f, r);
+        cp = norm2(r);
+        MaxResidSinceLastRelUp = cp;
+        b = cp / c;
+        l++;
+      }
+      axpy(p_f, b, p_f, r_f);  // p = b p + r
+      if (!using_fallback && Linop_fallback != nullptr && cp < fallback_transition_tol) {
+        Linop_f_use = Linop_fallback;
+        using_fallback = true;
+      }
+    }
+    IterationsToComplete = MaxIterations;
+    ReliableUpdatesPerformed = l;
+    std::cout << GridLogMessage << "ConjugateGradientReliableUpdate did NOT converge" << std::endl;
+    if (ErrorOnNoConverge) assert(0);
   }
 };
 
