@@ -14,13 +14,15 @@ same workload (cpu_baseline).  Every solve runs its red-black CG to 1e-11 (Wilso
 unprec_residual, with meets_target) is <= 1e-10.
 
 Further legs, reported beside the headline value (each can be switched off):
-  c3     Mobius 24^4 x Ls12 (mf 0.005): red-black solve in FP64 CG and in defect
-         correction with FP32 and FP16-storage inner solves;
+  c3     Mobius 24^4 x Ls12 (mf 0.005): red-black solve in FP64 CG, in defect correction with
+         FP32 and FP16-storage inner solves, and in reliable-update CG;
   c4     Wilson 32^3x64 (m 0.05), the 12 spin-colour point sources of one site: one mixed-precision
          multi-RHS solve against 12 sequential FP64 solves;
   serve  a Poisson trace of Wilson solve requests (volumes 8^4 .. 24^4, 1-12 sources, masses
          0.05-0.3, 8 resident gauge fields per volume) through the serving runtime: throughput and
-         p50/p99 latency (--serve-requests 4096 for the full trace).
+         p50/p99 latency (--serve-requests 4096 for the full trace);
+  ms     multishift CG, Mobius 16^4 x Ls16, 12 shifts, against one plain CG solve;
+  block  BlockCGrQ against CGmultiRHS on the c4 red-black system.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg] [--no-cpu] [--no-c3] [--no-c4]
                        [--serve-requests N] [--serve-rate R]
@@ -140,7 +142,7 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tu
 
 
 def bench_multishift(gx, ctx, dims=(16, 16, 16, 16), Ls=16, nshift=12, tol=1e-10):
-    """Opt-in leg 'ms': ConjugateGradientMultiShift on the Mobius 16^4 x Ls16 Schur system (m 0.01,
+    """Leg 'ms': ConjugateGradientMultiShift on the Mobius 16^4 x Ls16 Schur system (m 0.01,
     eps 0.3, Gaussian odd source), nshift shifts geometrically spaced in [1e-4, 3], every shift to
     tol, against one plain CG solve of the lightest system.  Wall clock, best of two."""
     U = gx.random_gauge(dims, 1002, 0.3)
@@ -179,7 +181,7 @@ def bench_multishift(gx, ctx, dims=(16, 16, 16, 16), Ls=16, nshift=12, tol=1e-10
 
 
 def bench_block_cgrq(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=1e-11):
-    """Opt-in leg 'block': the C4 Schur system (Wilson 32^3x64 m 0.05, the red-black sources of the
+    """Leg 'block': the C4 Schur system (Wilson 32^3x64 m 0.05, the red-black sources of the
     12 point sources of one site) solved by CGmultiRHS and by BlockCGrQ in FP64, to tol; wall clock,
     iterations and the agreement of the two solutions."""
     nrhs = 12
@@ -312,9 +314,9 @@ def main():
     ap.add_argument('--serve-slots', type=int, default=4)
     ap.add_argument('--serve-mixed', type=int, default=2, choices=(1, 2),
                     help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
-    ap.add_argument('--legs', default='dhop,cg,c3,c4,serve,cpu',
-                    help='comma list of legs to run: dhop, cg, c3, c4, serve, cpu, and the opt-in ms (multishift '
-                         'CG) and block (BlockCGrQ); profiling runs pick one')
+    ap.add_argument('--legs', default='dhop,cg,c3,c4,serve,cpu,ms,block',
+                    help='comma list of legs to run: dhop, cg, c3, c4, serve, cpu, ms (multishift CG), block '
+                         '(BlockCGrQ); profiling runs pick one')
     ap.add_argument('--no-graphs', action='store_true',
                     help='solver iterations as plain launches (for profiler runs; same kernels and timings)')
     ap.add_argument('--no-tune', action='store_true',
