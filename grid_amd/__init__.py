@@ -161,6 +161,7 @@ _SIG = {
     'gx_wilson_clone': ([_P, _P, _I, _D, ctypes.POINTER(_P)], _I),
     'gx_op_share_links': ([_P, _P], _I),
     'gx_op_footprint': ([_P, ctypes.POINTER(ctypes.c_size_t)], _I),
+    'gx_op_same_action': ([_P, _P, _IP], _I),
     'gx_field_gaussian': ([_P, ctypes.c_uint64], _I),
     'gx_field_digest': ([_P, ctypes.POINTER(ctypes.c_uint64)], _I),
     'gx_field_point_source': ([_P, _IP, _I, _I], _I),
@@ -436,6 +437,11 @@ class FermionOp:
         self._call('op_footprint', ctypes.byref(v))
         return v.value
 
+    def same_action(self, other):
+        v = ctypes.c_int()
+        self._call('op_same_action', other.h, ctypes.byref(v))
+        return bool(v.value)
+
     def import_gauge(self, U):
         U = np.ascontiguousarray(U, dtype=np.complex128)
         self._call('op_import_gauge', _dp(U))
@@ -548,8 +554,10 @@ class FermionOp:
         return d
 
     def close(self):
+        """Releases the operator.  A base whose links live clones still read raises (GX_EINVAL) and
+        stays alive: close its clones first."""
         if self.h:
-            lib().gx_op_destroy(self.h)
+            _check(lib().gx_op_destroy(self.h))
             self.h = None
 
 

@@ -308,7 +308,7 @@ static void free_gauge(Op *op) {
   for (int p = 0; p < 3; p++)
     for (int cb = 0; cb < 2; cb++)
       if (op->U[p][cb]) {
-        if (!op->link_owner) (void)hipFree(op->U[p][cb]);
+        if (op->owns_links) (void)hipFree(op->U[p][cb]);
         op->U[p][cb] = nullptr;
       }
 }
@@ -1178,6 +1178,13 @@ int gx_mobius_create(gx_ctx ctx, const int dims[4], int Ls, double mass, double 
 
 int gx_op_destroy(gx_op op) {
   if (!op) return GX_OK;
+  {
+    // a base whose links clones still read stays alive (and its links with it)
+    std::lock_guard<std::mutex> lk(g_share_mu);
+    GX_REQUIRE(op->clones == 0, GX_EINVAL, "op_destroy: %d clone(s) still read this operator's links", op->clones);
+    if (op->link_owner) op->link_owner->clones--;
+    op->link_owner = nullptr;
+  }
   (void)hipStreamSynchronize(op->ctx->stream);
   solver_work_free(op->work);
   free_gauge(op);
@@ -1199,7 +1206,10 @@ static int share_links(Op *op, Op *base) {
   GX_CHECK_HIP(hipStreamSynchronize(base->ctx->stream));
   for (int p = 0; p < 3; p++)
     for (int cb = 0; cb < 2; cb++) op->U[p][cb] = base->U[p][cb];
+  if (op->link_owner) op->link_owner->clones--;
+  base->clones++;
   op->link_owner = base;
+  op->owns_links = false;
   for (int i = 0; i < 8; i++) op->phases[i] = base->phases[i];
   op->epoch++;
   return GX_OK;
@@ -1251,9 +1261,26 @@ int gx_op_footprint(gx_op op, size_t *bytes) {
   return GX_OK;
 }
 
+int gx_op_same_action(gx_op a, gx_op b, int *same) {
+  GX_REQUIRE(a && b && same, GX_EINVAL, "null argument");
+  // a clone's gauge field is its link owner's
+  const Op *ga = a->link_owner ? a->link_owner : O(a), *gb = b->link_owner ? b->link_owner : O(b);
+  bool eq = a->kind == b->kind && same_geom(a->g, b->g) && a->mass == b->mass && a->M5 == b->M5 && a->b == b->b &&
+            a->c == b->c && std::memcmp(a->phases, b->phases, sizeof(a->phases)) == 0;
+  eq = eq && (ga == gb || (ga->host_U.size() == gb->host_U.size() &&
+                           std::memcmp(ga->host_U.data(), gb->host_U.data(), ga->host_U.size() * sizeof(double)) == 0 &&
+                           std::memcmp(ga->phases, gb->phases, sizeof(ga->phases)) == 0));
+  *same = eq ? 1 : 0;
+  return GX_OK;
+}
+
 int gx_op_import_gauge(gx_op op, const double *U) {
   GX_REQUIRE(op && U, GX_EINVAL, "null argument");
   GX_REQUIRE(!op->link_owner, GX_EINVAL, "import_gauge: the operator shares another operator's links");
+  {
+    std::lock_guard<std::mutex> lk(g_share_mu);
+    GX_REQUIRE(op->clones == 0, GX_EINVAL, "import_gauge: %d clone(s) still read this operator's links", op->clones);
+  }
   GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
   op->host_U.assign(U, U + (size_t)op->g.V4 * 4 * 18);
   op->epoch++;

@@ -176,6 +176,8 @@ struct Op {
   SolverWork *work = nullptr;   // solver state, created on the first solve
   uint64_t epoch = 0;           // bumped whenever links, mass or matrices change (invalidates graphs)
   Op *link_owner = nullptr;     // clones read this operator's links and own none
+  bool owns_links = true;       // false for clones: the links belong to link_owner
+  int clones = 0;               // live clones reading this operator's links (guarded by the share lock)
 };
 
 // ---------------- launchers (gx_kernels.hip) ----------------

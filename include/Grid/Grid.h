@@ -89,6 +89,14 @@ inline void check(int rc, const char *what) {
     std::abort();
   }
 }
+// The device mixed-precision solvers run the sloppy solves with the FP64 operator's own links in
+// lower precision, so they apply only when the sloppy operator is the same action.
+inline bool same_action(gx_op d, gx_op f) {
+  if (!d || !f) return false;
+  int same = 0;
+  check(gx_op_same_action(d, f, &same), "same_action");
+  return same != 0;
+}
 inline uint64_t mix(uint64_t x) {
   x += 0x9E3779B97F4A7C15ULL;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -206,6 +214,7 @@ typedef GridParallelRNG GridSerialRNG;
 template <int PREC>
 class FermionLattice {
  public:
+  static constexpr int Prec = PREC;
   GridBase *_grid = nullptr;
   gx_field f = nullptr;
 
@@ -821,7 +830,7 @@ class MixedPrecisionConjugateGradient {
         SinglePrecGrid(sp_grid), Linop_f(lf), Linop_d(ld) {}
   void operator()(const FieldD &src_d_in, FieldD &sol_d) {
     gx_op op = Linop_d.schur_handle();
-    if (!op || !Linop_f.schur_handle()) {
+    if (!detail::same_action(op, Linop_f.schur_handle())) {
       host_loop(src_d_in, sol_d);
       return;
     }
@@ -913,7 +922,8 @@ class ConjugateGradientReliableUpdate {
   }
   void operator()(const FieldD &src, FieldD &psi) {
     gx_op op = Linop_d.schur_handle();
-    if (!op || !Linop_f.schur_handle()) {
+    if (!detail::same_action(op, Linop_f.schur_handle()) ||
+        (Linop_fallback && !detail::same_action(op, Linop_fallback->schur_handle()))) {
       host_loop(src, psi);
       return;
     }
@@ -1075,7 +1085,8 @@ class ConjugateGradientMultiShift : public OperatorFunction<Field> {
     gx_op op = Linop.schur_handle();
     const int n = shifts.order;
     assert((int)psi.size() == n && (int)shifts.poles.size() == n && (int)shifts.tolerances.size() == n);
-    if (!op) {
+    // the device solver takes FP64 fields and at most 32 poles; Grid takes any, so the rest run here
+    if (!op || Field::Prec != GX_F64 || n > 32) {
       host_loop(Linop, src, psi);
       return;
     }

@@ -117,6 +117,11 @@ int gx_wilson_clone(gx_op base, gx_ctx ctx, int Ls, double mass, gx_op *out);
 int gx_op_share_links(gx_op clone, gx_op base);
 /* Device bytes the operator holds now (links it owns, fifth-dimension matrices, temporaries). */
 int gx_op_footprint(gx_op op, size_t *bytes);
+/* No Grid counterpart (Grid's mixed-precision solvers trust the caller to pass the same action in
+ * two precisions, ConjugateGradientMixedPrec.h:57-69): *same = 1 when a and b define the same
+ * operator (kind, lattice, Ls, mass, M5, b, c, boundary phases and gauge field, bit for bit), so a
+ * solver may run b's sloppy solves with a's links. */
+int gx_op_same_action(gx_op a, gx_op b, int *same);
 
 /* Operator applications; fields may be F64 or F32, in and out of the same precision.
  * Grid: FermionOperator.h:63-77; WilsonFermionImplementation.h:113-176,310-342;

@@ -20,7 +20,6 @@
 //   Schur operator                        Grid/algorithms/LinearOperator.h:286-349
 //   Red-black source / solution           Grid/algorithms/iterative/SchurRedBlack.h:236-291,385-425
 //   CG loop                               Grid/algorithms/iterative/ConjugateGradient.h:68-257
-//   Mixed-precision defect correction     Grid/algorithms/iterative/ConjugateGradientMixedPrec.h:71-165
 //   Reductions (per-thread partial sums, serial combine in double)
 //                                         Grid/lattice/Lattice_reduction.h:71-100,255-287
 //   Checkerboard indexing                 Grid/cartesian/Cartesian_red_black.h:39-95, Grid/qcd/utils/SpaceTimeGrid.cc:66-78

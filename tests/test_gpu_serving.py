@@ -66,6 +66,28 @@ def test_clone_matches_fresh_operator(ctx):
     ctx2.close()
 
 
+def test_base_with_clones_refuses_import_and_destroy(ctx):
+    """While a clone reads a base operator's links, the base refuses ImportGauge and destroy with
+    GX_EINVAL (its links stay valid); once the clone is gone both work.  Only error codes are
+    checked: the refused calls do no device work."""
+    dims = (4, 4, 4, 4)
+    U1, U2 = gx.random_gauge(dims, 51, 0.4), gx.random_gauge(dims, 52, 0.4)
+    base = gx.FermionOp(ctx, U1, dims, 1, 'wilson', mass=0.1)
+    cl = base.clone(ctx, 2, 0.2)
+    for call in (lambda: base.import_gauge(U2), base.close):
+        with pytest.raises(gx.GxError) as e:
+            call()
+        assert e.value.code == 1  # GX_EINVAL
+    assert base.h
+    cl2 = base.clone(ctx, 3, 0.3)
+    cl.close()
+    with pytest.raises(gx.GxError):
+        base.close()
+    cl2.close()
+    base.import_gauge(U2)
+    base.close()
+
+
 def test_multi_rhs_schur_solve(ctx):
     """Red-black solve of a multi-RHS Wilson operator (FP64, FP32 and FP16-storage inner solves):
     every right-hand side agrees with the oracle's single-RHS solve and the reported residual is the
