@@ -30,6 +30,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cg] [--no-cpu] 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -89,20 +90,108 @@ def time_dhop(ctx, D, fi, fo, steps, warmup):
     return ctx.elapsed_ms(0, 1) / steps
 
 
-def cpu_baseline(U, psi_o, seconds=12.0):
+def physical_cores():
+    """Physical cores of the host (lscpu: cores per socket x sockets), or os.cpu_count()."""
+    try:
+        out = subprocess.run(['lscpu'], capture_output=True, text=True, timeout=10).stdout
+        f = {k.strip(): v.strip() for k, v in (ln.split(':', 1) for ln in out.splitlines() if ':' in ln)}
+        return int(f['Core(s) per socket']) * int(f['Socket(s)'])
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_threads():
+    """Threads for the all-cores CPU run: the host's physical cores, capped at the CPU share this
+    job is given (OMP_NUM_THREADS when set), since more threads than the share only time-slice."""
+    cores = physical_cores()
+    share = int(os.environ.get('OMP_NUM_THREADS', '0') or 0)
+    return (min(cores, share) if share > 0 else cores), cores
+
+
+def median_time(fn, calls, warm=2):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(gx, U, psi_o):
+    """The CPU restatement of Grid's algorithm (oracle/cpu_ref.cpp, OpenMP, -O3 -march) timed on this
+    host: C2 DhopEO median over 20 calls with all physical cores (capped at the job's CPU share) and
+    with one thread; red-black solve time-to-solution of C1, C2 and C4 (one point source) with all
+    cores, at the tolerances of the GPU legs.  About 20-30 s of CPU work."""
     from oracle import pyoracle
-    ncores = min(16, os.cpu_count() or 1)
-    pyoracle.set_threads(ncores)
+    nthr, cores = cpu_threads()
+    H = np.prod(DIMS) // 2
+    byts = H * (24 * LS + 72) * 16
     O = pyoracle.Oracle(U, DIMS, LS, 'mobius', **PARAMS)
-    O.dhop(psi_o, 0, 0)  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        O.dhop(psi_o, 0, 0)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > seconds or n >= 200:
-            break
-    return el / n, n, ncores
+    out = {'threads_all': nthr, 'host_physical_cores': cores}
+    for name, n in (('all_cores', nthr), ('one_thread', 1)):
+        pyoracle.set_threads(n)
+        sec = median_time(lambda: O.dhop(psi_o, 0, 0), 20, warm=1)
+        out[name] = {'threads': n, 'dhop_eo_median_ms': round(sec * 1e3, 3), 'gbs': round(byts / sec / 1e9, 3),
+                     'gflops_1320': round(1320.0 * H * LS / sec / 1e9, 3),
+                     'gflops_1344': round(1344.0 * H * LS / sec / 1e9, 3)}
+    pyoracle.set_threads(nthr)
+    solves = {}
+    # C2: the GPU cg leg's source and tolerance
+    src = gx.gaussian_fermion(DIMS, LS, 3003)
+    t0 = time.perf_counter()
+    _, st = O.schur_solve(src, tol=CG_TOL_MOBIUS, maxit=20000)
+    solves['c2_mobius_16x16x16x16_ls16'] = {'seconds': round(time.perf_counter() - t0, 3), 'iterations': st['iterations'],
+                                            'unprec_residual': st['unprec_residual'], 'cg_tol': CG_TOL_MOBIUS}
+    del O
+    # C1: Wilson 8^4, m 0.1, eps 0.5
+    d1 = (8, 8, 8, 8)
+    O1 = pyoracle.Oracle(gx.random_gauge(d1, 1001, 0.5), d1, 1, 'wilson', mass=0.1)
+    src1 = gx.gaussian_fermion(d1, 1, 2001)
+    t0 = time.perf_counter()
+    _, st = O1.schur_solve(src1, tol=CG_TOL, maxit=20000)
+    solves['c1_wilson_8x8x8x8'] = {'seconds': round(time.perf_counter() - t0, 4), 'iterations': st['iterations'],
+                                   'unprec_residual': st['unprec_residual'], 'cg_tol': CG_TOL}
+    # C4: Wilson 32^3x64, m 0.05, one of the 12 point sources (spin 0, colour 0)
+    d4 = (32, 32, 32, 64)
+    rng = np.random.default_rng(4004)
+    x0 = tuple(int(rng.integers(d)) for d in d4)
+    O4 = pyoracle.Oracle(gx.random_gauge(d4, 1004, 0.3), d4, 1, 'wilson', mass=0.05)
+    src4 = gx.point_source(d4, 1, x0, 0, 0)
+    t0 = time.perf_counter()
+    _, st = O4.schur_solve(src4, tol=CG_TOL, maxit=20000)
+    solves['c4_wilson_32x32x32x64_1rhs'] = {'seconds': round(time.perf_counter() - t0, 3), 'iterations': st['iterations'],
+                                            'unprec_residual': st['unprec_residual'], 'cg_tol': CG_TOL}
+    out['solves'] = solves
+    return out, byts
+
+
+def bench_dhop_large(gx, ctx, tune):
+    """DhopEO at working sets well above the 256 MB Infinity Cache, so HBM sets the rate: Mobius
+    24^4 x Ls16 FP64 (1.21 GB per call) and 32^4 x Ls16 FP32 (1.91 GB), C2's parameters."""
+    out = {}
+    for dims, prec, name, c in (((24, 24, 24, 24), gx.F64, 'mobius_24x24x24x24_ls16_f64', 16),
+                                ((32, 32, 32, 32), gx.F32, 'mobius_32x32x32x32_ls16_f32', 8)):
+        U = gx.random_gauge(dims, 1012, EPS)
+        D = gx.FermionOp(ctx, U, dims, LS, 'mobius', **PARAMS)
+        tune('dhop_' + name, D, prec)
+        fi = D.field(prec, 1)
+        full = D.field(prec)
+        full.gaussian(2012)
+        gx.pick_checkerboard(1, fi, full)
+        full.close()
+        fo = D.field(prec, 0)
+        ms = time_dhop(ctx, D, fi, fo, 50, 5)
+        H = np.prod(dims) // 2
+        byts = H * (24 * LS + 72) * c
+        out[name] = {'ms': round(ms, 5), 'bytes': int(byts), 'gbs': round(byts / (ms * 1e-3) / 1e9, 2),
+                     'frac_of_8tbs': round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     'gflops_1320': round(1320.0 * H * LS / (ms * 1e-3) / 1e9, 1)}
+        fi.close()
+        fo.close()
+        D.close()
+    return out
 
 
 def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tune=None):
@@ -314,9 +403,9 @@ def main():
     ap.add_argument('--serve-slots', type=int, default=4)
     ap.add_argument('--serve-mixed', type=int, default=2, choices=(1, 2),
                     help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
-    ap.add_argument('--legs', default='dhop,cg,c3,c4,serve,cpu,ms,block',
-                    help='comma list of legs to run: dhop, cg, c3, c4, serve, cpu, ms (multishift CG), block '
-                         '(BlockCGrQ); profiling runs pick one')
+    ap.add_argument('--legs', default='dhop,large,cg,c3,c4,serve,cpu,ms,block',
+                    help='comma list of legs to run: dhop, large (DhopEO at >= 1 GB working sets), cg, c3, c4, '
+                         'serve, cpu, ms (multishift CG), block (BlockCGrQ); profiling runs pick one')
     ap.add_argument('--no-graphs', action='store_true',
                     help='solver iterations as plain launches (for profiler runs; same kernels and timings)')
     ap.add_argument('--no-tune', action='store_true',
@@ -407,12 +496,19 @@ def main():
     ms = bench_multishift(gx, ctx) if 'ms' in legs else None
     blk = bench_block_cgrq(gx, ctx) if 'block' in legs else None
 
+    large = bench_dhop_large(gx, ctx, tune) if 'large' in legs else None
+
     cpu = None
     if not args.no_cpu:
-        sec, n, cores = cpu_baseline(U, psi_o)
-        cpu = {'value': res['f64']['bytes'] / sec / 1e9, 'unit': 'GB/s', 'cores': cores, 'kind': 'port',
-               'sample': f'{n} DhopEO calls, Mobius 16^4xLs16 FP64, oracle/cpu_ref.cpp with {cores} OpenMP threads '
-                         f'({sec * 1e3:.1f} ms per call, {1320.0 * H * LS / sec / 1e9:.2f} GF/s)'}
+        cb, byts = cpu_baseline(gx, U, psi_o)
+        a = cb['all_cores']
+        cpu = {'value': a['gbs'], 'unit': 'GB/s', 'cores': a['threads'], 'kind': 'port',
+               'sample': f'C2 DhopEO (Mobius 16^4xLs16 FP64, {byts} B per call), median of 20 calls after warm-up, '
+                         f'oracle/cpu_ref.cpp (the CPU restatement of Grid\'s algorithm; Grid itself cannot be built '
+                         f'here) with {a["threads"]} OpenMP threads ({cb["host_physical_cores"]} physical cores on '
+                         f'the host, capped at the job\'s CPU share), and with 1 thread; red-black solves of C1, C2 '
+                         f'and C4 (one point source) at the GPU legs\' tolerances with all threads',
+               **cb}
 
     r = res['f64']
     out = {
@@ -432,8 +528,11 @@ def main():
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': round(r['gbs'], 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(r['gbs'] / HBM_PEAK_GBS, 4), 'traffic': traffic_from_profile(),
+                     'traffic_source': 'profiles/dhop_mobius_16x16x16x16_ls16_f64_traffic.json (rocprofv3 FETCH_SIZE '
+                                       'and WRITE_SIZE runs of bench.py --legs dhop, scaled by a streaming pass)',
                      'kernel': DHOP_KERNEL, 'algorithmic_bytes_per_launch': r['bytes']},
         'cpu_baseline': cpu,
+        'dhop_hbm_sized': large,
         'pct_hbm_peak': round(100 * r['gbs'] / HBM_PEAK_GBS, 2),
         'gflops_1320': round(r['gflops'], 1),
         'f32': {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res['f32'].items()},

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, 'libgx.so')
 F64, F32, F16 = 0, 1, 2
 FULL, EVEN, ODD = -1, 0, 1
 WILSON, MOBIUS = 0, 1
+HERM_SCHUR, HERM_MDAGM = 0, 1
 STATUS = {0: 'GX_OK', 1: 'GX_EINVAL', 2: 'GX_ENOMEM', 3: 'GX_ENOCONV', 4: 'GX_EHIP', 5: 'GX_ECB', 6: 'GX_ENAN'}
 
 
@@ -145,6 +146,12 @@ _SIG = {
     'gx_schur_solve': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats), _DP], _I),
     'gx_block_cg': ([_P, _P, _P, _D, _I, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_block_cgrq': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
+    'gx_mixed_cg_batched': ([_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), _D, _D, _I, _I, _I, _I,
+                             ctypes.POINTER(SolverStats)], _I),
+    'gx_inner_product_matrix': ([_I, ctypes.POINTER(_P), ctypes.POINTER(_P), _DP], _I),
+    'gx_madd_matrix': ([_I, ctypes.POINTER(_P), _DP, ctypes.POINTER(_P), ctypes.POINTER(_P), _D], _I),
+    'gx_block_cgrq_vec': ([_P, _I, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), _D, _I, ctypes.POINTER(SolverStats)],
+                          _I),
     'gx_gauge_statistics': ([_P, _IP, _DP, _DP, _DP], _I),
     'gx_nersc_read': ([_P, ctypes.c_char_p, _IP, _DP, ctypes.POINTER(NerscInfo)], _I),
     'gx_nersc_write': ([_P, ctypes.c_char_p, _IP, _DP, _I, ctypes.c_char_p, ctypes.c_char_p, _I], _I),
@@ -387,6 +394,22 @@ def slice_madd_matrix(r, aa, x, y=None, scale=1.0):
     _check(lib().gx_slice_madd_matrix(r.h, _dp(a), x.h, y.h if y is not None else None, scale))
 
 
+def inner_product_matrix(xs, ys):
+    """InnerProductMatrix over lists of fields: m[b, bp] = <xs[b], ys[bp]>."""
+    n = len(xs)
+    m = np.zeros((n, n), dtype=np.complex128)
+    _check(lib().gx_inner_product_matrix(n, (_P * n)(*[f.h for f in xs]), (_P * n)(*[f.h for f in ys]), _dp(m)))
+    return m
+
+
+def madd_matrix(rs, aa, xs, ys=None, scale=1.0):
+    """MaddMatrix over lists of fields: rs[b] = ys[b] + scale sum_bp xs[bp] aa[bp, b] (ys None: MulMatrix)."""
+    n = len(xs)
+    a = np.ascontiguousarray(aa, dtype=np.complex128)
+    yp = (_P * n)(*[f.h for f in ys]) if ys is not None else None
+    _check(lib().gx_madd_matrix(n, (_P * n)(*[f.h for f in rs]), _dp(a), (_P * n)(*[f.h for f in xs]), yp, scale))
+
+
 def pick_checkerboard(cb, half, full):
     _check(lib().gx_pick_checkerboard(cb, half.h, full.h))
 
@@ -539,6 +562,36 @@ class FermionOp:
         returns one stats dict per RHS."""
         st = (SolverStats * self.Ls)()
         rc = lib().gx_block_cgrq(self.h, src.h, psi.h, tol, maxit, st)
+        if rc not in (0, 3):
+            _check(rc)
+        return [s.as_dict() for s in st]
+
+    def mixed_cg_batched(self, srcs, sols, tol=1e-10, inner_tol=1e-5, inner_prec=F32, maxinner=10000, maxouter=100,
+                         maxpatch=10000):
+        """MixedPrecisionConjugateGradientBatched over a list of half fields; one stats dict per source."""
+        n = len(srcs)
+        assert len(sols) == n
+        hs = (_P * n)(*[f.h for f in srcs])
+        hx = (_P * n)(*[f.h for f in sols])
+        st = (SolverStats * n)()
+        rc = lib().gx_mixed_cg_batched(self.h, n, hs, hx, tol, inner_tol, int(inner_prec), maxinner, maxouter, maxpatch,
+                                       st)
+        if rc not in (0, 3):
+            _check(rc)
+        return [s.as_dict() for s in st]
+
+    def block_cgrq_vec(self, srcs, psis, tol=1e-10, maxit=10000, herm=None):
+        """BlockConjugateGradient(BlockCGrQVec) over a list of fields: MpcDagMpc on half fields
+        (herm HERM_SCHUR, the default for half fields) or Mdag M on full fields (HERM_MDAGM); returns
+        one stats dict per field."""
+        n = len(srcs)
+        assert len(psis) == n
+        if herm is None:
+            herm = HERM_MDAGM if srcs[0].subset == FULL else HERM_SCHUR
+        hs = (_P * n)(*[f.h for f in srcs])
+        hp = (_P * n)(*[f.h for f in psis])
+        st = (SolverStats * n)()
+        rc = lib().gx_block_cgrq_vec(self.h, int(herm), n, hs, hp, tol, maxit, st)
         if rc not in (0, 3):
             _check(rc)
         return [s.as_dict() for s in st]

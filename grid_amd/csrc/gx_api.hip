@@ -751,13 +751,33 @@ int apply_full_m(Op *op, Field *in, Field *out, int dag) {
   return GX_OK;
 }
 
+int slice_transfer(Field *lo, Field *hi, int s, bool extract) {
+  bool ok = lo && hi && (lo->subset == GX_FULL) == (hi->subset == GX_FULL) && lo->g.Ls == 1 && lo->prec == hi->prec &&
+            (lo->prec == GX_F64 || lo->prec == GX_F32) && s >= 0 && s < hi->g.Ls;
+  for (int d = 0; d < 4 && ok; d++) ok = lo->g.L[d] == hi->g.L[d];
+  if (!ok) {
+    set_error("slice: needs two full or two half fields of one precision on one 4D lattice, the 4D one with Ls = 1, "
+              "and 0 <= s < Ls");
+    return GX_EINVAL;
+  }
+  const int nhalf = lo->subset == GX_FULL ? 2 : 1;
+  if (nhalf == 1) (extract ? lo : hi)->cb = (extract ? hi : lo)->cb;  // a half field takes its source's tag
+  hipStream_t st = hi->ctx->stream;
+  const int Ls = hi->g.Ls;
+  if (lo->prec == GX_F64) {
+    if (extract) GX_TRY(launch_slice_proj<double>(hi->g, hi->data, Ls, s, lo->data, 1, 0, 0xF, 0, st, nhalf));
+    else GX_TRY(launch_slice_proj<double>(hi->g, lo->data, 1, 0, hi->data, Ls, s, 0xF, 0, st, nhalf));
+  } else {
+    if (extract) GX_TRY(launch_slice_proj<float>(hi->g, hi->data, Ls, s, lo->data, 1, 0, 0xF, 0, st, nhalf));
+    else GX_TRY(launch_slice_proj<float>(hi->g, lo->data, 1, 0, hi->data, Ls, s, 0xF, 0, st, nhalf));
+  }
+  return GX_OK;
+}
+
 }  // namespace gx
 
 using namespace gx;
 
-struct gx_ctx_s : public Ctx {};
-struct gx_field_s : public Field {};
-struct gx_op_s : public Op {};
 
 static Field *F(gx_field f) { return static_cast<Field *>(f); }
 static Op *O(gx_op o) { return static_cast<Op *>(o); }
@@ -996,23 +1016,6 @@ int gx_field_point_source(gx_field f, const int x[4], int spin, int colour) {
   }
   GX_CHECK_HIP(hipStreamSynchronize(st));
   f->cb = 0;
-  return GX_OK;
-}
-
-static int slice_transfer(Field *lo, Field *hi, int s, bool extract) {
-  GX_REQUIRE(lo && hi && lo->subset == GX_FULL && hi->subset == GX_FULL && lo->g.Ls == 1 && lo->prec == hi->prec &&
-                 (lo->prec == GX_F64 || lo->prec == GX_F32) && s >= 0 && s < hi->g.Ls,
-             GX_EINVAL, "slice: needs full fields of one precision, the 4D one with Ls = 1, and 0 <= s < Ls");
-  for (int d = 0; d < 4; d++) GX_REQUIRE(lo->g.L[d] == hi->g.L[d], GX_EINVAL, "slice: 4D dimensions differ");
-  hipStream_t st = hi->ctx->stream;
-  const int Ls = hi->g.Ls;
-  if (lo->prec == GX_F64) {
-    if (extract) GX_TRY(launch_slice_proj<double>(hi->g, hi->data, Ls, s, lo->data, 1, 0, 0xF, 0, st));
-    else GX_TRY(launch_slice_proj<double>(hi->g, lo->data, 1, 0, hi->data, Ls, s, 0xF, 0, st));
-  } else {
-    if (extract) GX_TRY(launch_slice_proj<float>(hi->g, hi->data, Ls, s, lo->data, 1, 0, 0xF, 0, st));
-    else GX_TRY(launch_slice_proj<float>(hi->g, lo->data, 1, 0, hi->data, Ls, s, 0xF, 0, st));
-  }
   return GX_OK;
 }
 

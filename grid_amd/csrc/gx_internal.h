@@ -217,10 +217,11 @@ template <typename Tin, typename Tout>
 int launch_convert(int64_t n, const void *in, void *out, hipStream_t st);       // n plain entries
 template <typename Sin, typename Sout>
 int launch_convert_site(int64_t nsites, int64_t stride, const void *in, void *out, hipStream_t st);
-// z[sz] (=|+=) (spins in mask) x[sx] over both checkerboards of full fields (rows Lsx / Lsz)
+// z[sz] (=|+=) (spins in mask) x[sx] over nhalf checkerboard halves (2: full fields, 1: half fields;
+// rows Lsx / Lsz)
 template <typename T>
 int launch_slice_proj(const Geom &g, const void *x, int Lsx, int sx, void *z, int Lsz, int sz, int mask,
-                      int accumulate, hipStream_t st);
+                      int accumulate, hipStream_t st, int nhalf = 2);
 
 // CG iteration kernels on device scalars (gx_kernels.hip)
 template <typename T>
@@ -271,6 +272,19 @@ int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const vo
 int launch_slice_madd(Ctx *ctx, const Geom &g, int nhalf, const void *a_dev, double scale, const void *x,
                       const void *y, void *r);
 
+// Matrices over lists of n <= kMaxLs FP64 fields of nelem stored complex numbers each
+// (InnerProductMatrix, MaddMatrix of BlockConjugateGradient.h:35-74).  ipm: out_dev[b * n + bp] =
+// <x_b, y_bp> (double2) through nparts workgroup partials of n^2 double2 each.  madd: r_b = y_b +
+// scale sum_bp x_bp a(bp, b), a_dev row-major n x n double2; y.p[b] may be null (zero); r may
+// alias x or y.
+struct VecPtrs {
+  const void *p[kMaxLs];
+};
+int launch_vec_ipm(Ctx *ctx, int64_t nelem, int n, const VecPtrs &x, const VecPtrs &y, void *partial, int nparts,
+                   void *out_dev);
+int launch_vec_madd(Ctx *ctx, int64_t nelem, int n, const void *a_dev, double scale, const VecPtrs &x, const VecPtrs &y,
+                    const VecPtrs &r);
+
 // ---------------- api helpers (gx_api.hip) ----------------
 int field_create(Ctx *ctx, const Geom &g, int prec, int subset, Field **out);
 void field_destroy(Field *f);
@@ -295,5 +309,13 @@ int blas_inner(Field *x, Field *y, double *re, double *im);
 int blas_inner_norm(Field *x, Field *y, double *re, double *nrm);  // Re<x|y>, |x|^2 in one pass
 int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm);
 int convert_field(Field *out, Field *in);
+// ExtractSlice (extract) / InsertSlice between a 4D field lo (Ls = 1) and slice s of hi: two full or
+// two half fields of one precision
+int slice_transfer(Field *lo, Field *hi, int s, bool extract);
 
 }  // namespace gx
+
+// The ABI's opaque handles are the library's objects.
+struct gx_ctx_s : public gx::Ctx {};
+struct gx_field_s : public gx::Field {};
+struct gx_op_s : public gx::Op {};

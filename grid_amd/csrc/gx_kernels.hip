@@ -1300,14 +1300,16 @@ int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *
 }
 
 // ---------------- fifth-dimension slice transfer (LinalgUtils.h:140-183, axpby_ssp_pplus/pminus) ------
-// z[s = sz] (=|+=) (spins in mask) x[s = sx] over both checkerboards of full fields; x has Lsx
-// slices (Px padded entries per half), z has Lsz (Pz).  Used for the physical 4D <-> 5D source /
-// solution maps and for ExtractSlice / InsertSlice.
+// z[s = sz] (=|+=) (spins in mask) x[s = sx] over nhalf checkerboard halves (2: full fields, 1: half
+// fields); x has Lsx slices (Px padded entries per half), z has Lsz (Pz).  Used for the physical
+// 4D <-> 5D source / solution maps, ExtractSlice / InsertSlice and the packing of sources into a
+// multi-RHS field.
 template <typename T>
 __global__ void __launch_bounds__(256) k_slice_proj(const cx<T> *__restrict__ x, int Lsx, int sx, int64_t Px,
                                                     cx<T> *__restrict__ z, int Lsz, int sz, int64_t Pz, int mask,
-                                                    int accumulate, int64_t H4) {
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < 2 * H4; j += (int64_t)gridDim.x * blockDim.x) {
+                                                    int accumulate, int64_t H4, int nhalf) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nhalf * H4;
+       j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t cb = j / H4, h = j - cb * H4;
     const cx<T> *xb = x + cb * 12 * Px;
     cx<T> *zb = z + cb * 12 * Pz;
@@ -1327,10 +1329,10 @@ __global__ void __launch_bounds__(256) k_slice_proj(const cx<T> *__restrict__ x,
 
 template <typename T>
 int launch_slice_proj(const Geom &gx_, const void *x, int Lsx, int sx, void *z, int Lsz, int sz, int mask,
-                      int accumulate, hipStream_t st) {
+                      int accumulate, hipStream_t st, int nhalf) {
   const int64_t Px = (gx_.H4 * Lsx + kBlk - 1) / kBlk * kBlk, Pz = (gx_.H4 * Lsz + kBlk - 1) / kBlk * kBlk;
-  hipLaunchKernelGGL((k_slice_proj<T>), dim3(stream_blocks(2 * gx_.H4)), dim3(256), 0, st, (const cx<T> *)x, Lsx, sx,
-                     Px, (cx<T> *)z, Lsz, sz, Pz, mask, accumulate, gx_.H4);
+  hipLaunchKernelGGL((k_slice_proj<T>), dim3(stream_blocks(nhalf * gx_.H4)), dim3(256), 0, st, (const cx<T> *)x, Lsx,
+                     sx, Px, (cx<T> *)z, Lsz, sz, Pz, mask, accumulate, gx_.H4, nhalf);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1707,7 +1709,97 @@ static __global__ void __launch_bounds__(256) k_slice_madd(int64_t nsite, int64_
   }
 }
 
+// ---------------- matrices over lists of fields (BlockConjugateGradient.h:35-74) ----------------
+// InnerProductMatrix: m(b, bp) = <X_b, Y_bp> over n FP64 fields of nelem stored complex numbers
+// each (padding entries are zero).  A workgroup stages kVecChunk elements of every X_b and Y_b in
+// LDS; thread b * n + bp accumulates m(b, bp) over them and writes its workgroup's partial sum,
+// which k_slice_ipm_fin adds in a fixed order.
+constexpr int kVecChunk = 64;
+static __global__ void __launch_bounds__(256) k_vec_ipm(int64_t nelem, int n, VecPtrs x, VecPtrs y, double2 *partial) {
+  __shared__ double2 sx[kMaxLs * kVecChunk], sy[kMaxLs * kVecChunk];
+  const int t = threadIdx.x, n2 = n * n;
+  const int bi = t / n, bj = t - bi * n;
+  double ar = 0, ai = 0;
+  for (int64_t i0 = (int64_t)blockIdx.x * kVecChunk; i0 < nelem; i0 += (int64_t)gridDim.x * kVecChunk) {
+    const int ne = (int)min((int64_t)kVecChunk, nelem - i0);
+    __syncthreads();
+    for (int q = t; q < n * kVecChunk; q += 256) {
+      const int f = q / kVecChunk, j = q - f * kVecChunk;  // field f, element i0 + j (coalesced in j)
+      double2 vx = {0, 0}, vy = {0, 0};
+      if (j < ne) {
+        vx = reinterpret_cast<const double2 *>(x.p[f])[i0 + j];
+        vy = reinterpret_cast<const double2 *>(y.p[f])[i0 + j];
+      }
+      sx[q] = vx;
+      sy[q] = vy;
+    }
+    __syncthreads();
+    if (t < n2) {
+      const double2 *px = sx + bi * kVecChunk, *py = sy + bj * kVecChunk;
+      for (int j = 0; j < ne; j++) {
+        ar = fma(px[j].x, py[j].x, fma(px[j].y, py[j].y, ar));
+        ai = fma(px[j].x, py[j].y, fma(-px[j].y, py[j].x, ai));
+      }
+    }
+  }
+  if (t < n2) partial[(int64_t)blockIdx.x * n2 + t] = {ar, ai};
+}
+
+// MaddMatrix: R_b = Y_b + scale sum_bp X_bp a(bp, b) (Y null reads as zero: MulMatrix), one thread per
+// element with every X_bp of that element in registers before any R_b is written, so R may alias X
+// or Y as in Grid.
+static __global__ void __launch_bounds__(256) k_vec_madd(int64_t nelem, int n, const double2 *a, double scale, VecPtrs x,
+                                                         VecPtrs y, VecPtrs r) {
+  __shared__ double2 sa[kMaxLs * kMaxLs];
+  for (int q = threadIdx.x; q < n * n; q += blockDim.x) sa[q] = {scale * a[q].x, scale * a[q].y};
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nelem; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 xv[kMaxLs];
+#pragma unroll
+    for (int b = 0; b < kMaxLs; b++)
+      if (b < n) xv[b] = reinterpret_cast<const double2 *>(x.p[b])[i];
+    for (int b = 0; b < n; b++) {
+      double2 o = y.p[b] ? reinterpret_cast<const double2 *>(y.p[b])[i] : double2{0, 0};
+#pragma unroll
+      for (int bp = 0; bp < kMaxLs; bp++)
+        if (bp < n) {
+          const double2 c = sa[bp * n + b];
+          o.x = fma(xv[bp].x, c.x, fma(-xv[bp].y, c.y, o.x));
+          o.y = fma(xv[bp].x, c.y, fma(xv[bp].y, c.x, o.y));
+        }
+      reinterpret_cast<double2 *>(const_cast<void *>(r.p[b]))[i] = o;
+    }
+  }
+}
+
 #if GX_PART == 0
+int launch_vec_ipm(Ctx *ctx, int64_t nelem, int n, const VecPtrs &x, const VecPtrs &y, void *partial, int nparts,
+                   void *out_dev) {
+  if (n < 1 || n > kMaxLs || nparts < 1) {
+    set_error("inner product matrix: %d fields (1..%d)", n, kMaxLs);
+    return GX_EINVAL;
+  }
+  int64_t nb = (nelem + kVecChunk - 1) / kVecChunk;
+  if (nb > nparts) nb = nparts;
+  hipLaunchKernelGGL(k_vec_ipm, dim3(nb), dim3(256), 0, ctx->stream, nelem, n, x, y, (double2 *)partial);
+  hipLaunchKernelGGL(k_slice_ipm_fin, dim3(n * n), dim3(256), 0, ctx->stream, (const double2 *)partial, (int)nb, n * n,
+                     (double2 *)out_dev);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int launch_vec_madd(Ctx *ctx, int64_t nelem, int n, const void *a_dev, double scale, const VecPtrs &x, const VecPtrs &y,
+                    const VecPtrs &r) {
+  if (n < 1 || n > kMaxLs) {
+    set_error("madd matrix: %d fields (1..%d)", n, kMaxLs);
+    return GX_EINVAL;
+  }
+  hipLaunchKernelGGL(k_vec_madd, dim3(stream_blocks(nelem)), dim3(256), 0, ctx->stream, nelem, n, (const double2 *)a_dev,
+                     scale, x, y, r);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
 int launch_slice_ipm(Ctx *ctx, const Geom &g, int nhalf, const void *x, const void *y, void *partial, int nparts,
                      void *out_dev) {
   if (g.Ls > kMaxLs || nparts < 1) {
@@ -1831,7 +1923,8 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
   template int launch_cg_axpy_norm<S>(Ctx *, int64_t, int64_t, void *, const void *, CGState *);                   \
   template int launch_cg_update<S>(Ctx *, int64_t, int64_t, void *, void *, const void *, CGState *);
 #define GX_INST_REAL(T)                                                                                            \
-  template int launch_slice_proj<T>(const Geom &, const void *, int, int, void *, int, int, int, int, hipStream_t); \
+  template int launch_slice_proj<T>(const Geom &, const void *, int, int, void *, int, int, int, int, hipStream_t, \
+                                    int);                                                                          \
   template int launch_mrhs_reduce<T>(Ctx *, int, const Geom &, void *, const void *, void *, MrhsState *, double *); \
   template int launch_mrhs_update<T>(Ctx *, const Geom &, void *, void *, const void *, MrhsState *);             \
   template int launch_gaussian<T>(const Geom &, void *, uint64_t, hipStream_t);

@@ -304,7 +304,7 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
 }
 
 int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int inner_prec, int maxinner,
-                 int maxouter, gx_solver_stats *out) {
+                 int maxouter, gx_solver_stats *out, int maxpatch = -1) {
   auto t0 = std::chrono::steady_clock::now();
   gx_solver_stats st{};
   const int cb = src_in->cb;
@@ -359,7 +359,7 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
     fo.r0 = res;
     fo.r0_norm = &norm;
   }
-  const int rc = cg_run(op, src_in, sol, tol, maxinner, &fst, fo);
+  const int rc = cg_run(op, src_in, sol, tol, maxpatch > 0 ? maxpatch : maxinner, &fst, fo);
   st.final_iterations = fst.iterations;
   st.iterations = st.inner_iterations + fst.iterations;
   st.true_residual = fst.true_residual;
@@ -668,7 +668,7 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
 // all right-hand sides done together by one multi-RHS CG in FP32 or FP16 storage (inner_prec), so
 // they share every link load.
 int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int inner_prec, int maxinner,
-                   int maxouter, gx_solver_stats *st) {
+                   int maxouter, gx_solver_stats *st, int maxpatch = -1) {
   auto t0 = std::chrono::steady_clock::now();
   const int Ls = op->g.Ls, cb = src_in->cb;
   sol->cb = cb;
@@ -727,7 +727,7 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
     fo.r0 = res;
     fo.r0_norm = norm.data();
   }
-  const int rc = mrhs_cg_run(op, src_in, sol, tol, maxinner, fst.data(), fo);
+  const int rc = mrhs_cg_run(op, src_in, sol, tol, maxpatch > 0 ? maxpatch : maxinner, fst.data(), fo);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st)
     for (int s = 0; s < Ls; s++) {
@@ -738,6 +738,56 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
       st[s].iterations = inner_total + fst[s].iterations;
       st[s].seconds = secs;
     }
+  return rc;
+}
+
+// MixedPrecisionConjugateGradientBatched (ConjugateGradientMixedPrecBatched.h:77-202) over n
+// sources of one checkerboard.  On a single-RHS Wilson operator the sources are packed into the
+// slices of one multi-RHS field of a clone that reads the operator's links, and the batch runs as
+// mrhs_mixed_run: one FP64 residual per source per outer iteration, one inner tolerance for the
+// batch from the largest residual (as Grid's updateResidual), inner solves of all sources in one
+// multi-RHS CG that loads every link once for the whole batch, and the patch-up CG.  Other
+// operators (Mobius, or more sources than a clone holds) run Grid's per-source defect correction.
+static int mixed_batched_run(Op *op, int n, Field **src, Field **sol, double tol, double inner_tol, int inner_prec,
+                             int maxinner, int maxouter, int maxpatch, gx_solver_stats *st) {
+  const bool pack = op->kind == GX_WILSON && op->g.Ls == 1 && n >= 2 && n <= kMaxLs;
+  if (!pack) {
+    int rc = GX_OK;
+    for (int k = 0; k < n; k++) {
+      const int r = mixed_cg_run(op, src[k], sol[k], tol, inner_tol, inner_prec, maxinner, maxouter, st ? &st[k] : nullptr,
+                                 maxpatch);
+      if (r != GX_OK && r != GX_ENOCONV) return r;
+      if (r == GX_ENOCONV) rc = r;
+    }
+    return rc;
+  }
+  Op *base = op->link_owner ? op->link_owner : op;
+  gx_op cl = nullptr;
+  GX_TRY(gx_wilson_clone(static_cast<gx_op>(base), static_cast<gx_ctx>(op->ctx), n, op->mass, &cl));
+  Op *B = static_cast<Op *>(cl);
+  Field *ps = nullptr, *px = nullptr;
+  int rc = field_create(op->ctx, B->g, GX_F64, src[0]->cb, &ps);
+  if (rc == GX_OK) rc = field_create(op->ctx, B->g, GX_F64, src[0]->cb, &px);
+  for (int k = 0; k < n && rc == GX_OK; k++) {
+    rc = slice_transfer(src[k], ps, k, false);
+    if (rc == GX_OK) rc = slice_transfer(sol[k], px, k, false);
+  }
+  std::vector<gx_solver_stats> bst(n);
+  if (rc == GX_OK) {
+    ps->cb = px->cb = src[0]->cb;
+    rc = mrhs_mixed_run(B, ps, px, tol, inner_tol, inner_prec, maxinner, maxouter, bst.data(), maxpatch);
+    if (rc == GX_OK || rc == GX_ENOCONV) {
+      const int solve_rc = rc;
+      for (int k = 0; k < n && rc == solve_rc; k++)
+        if (slice_transfer(sol[k], px, k, true) != GX_OK) rc = GX_EHIP;
+    }
+  }
+  if (hipStreamSynchronize(op->ctx->stream) != hipSuccess && (rc == GX_OK || rc == GX_ENOCONV)) rc = GX_EHIP;
+  field_destroy(ps);
+  field_destroy(px);
+  gx_op_destroy(cl);
+  if (st)
+    for (int k = 0; k < n; k++) st[k] = bst[k];
   return rc;
 }
 
@@ -858,6 +908,23 @@ struct BlockWork {
     for (int i = 0; i < n * n; i++) m.a[i] = cd(mat_host[i].x, mat_host[i].y);
     return GX_OK;
   }
+  // InnerProductMatrix over n fields of nelem stored complex numbers; synchronises the stream
+  int ipm_vec(int n, int64_t nelem, const VecPtrs &x, const VecPtrs &y, CMat &m) {
+    GX_TRY(launch_vec_ipm(ctx, nelem, n, x, y, partial, nparts, mat_dev));
+    GX_CHECK_HIP(hipMemcpyAsync(mat_host, mat_dev, sizeof(double2) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    m = CMat(n);
+    for (int i = 0; i < n * n; i++) m.a[i] = cd(mat_host[i].x, mat_host[i].y);
+    return GX_OK;
+  }
+  // MaddMatrix over n fields (coefficient slots as in madd)
+  int madd_vec(int n, int64_t nelem, const VecPtrs &r, const CMat &a, const VecPtrs &x, const VecPtrs &y, double scale,
+               int slot) {
+    double2 *h = mat_host + (size_t)slot * kMaxLs * kMaxLs, *d = mat_dev + (size_t)slot * kMaxLs * kMaxLs;
+    for (int i = 0; i < n * n; i++) h[i] = make_double2(a.a[i].real(), a.a[i].imag());
+    GX_CHECK_HIP(hipMemcpyAsync(d, h, sizeof(double2) * n * n, hipMemcpyHostToDevice, ctx->stream));
+    return launch_vec_madd(ctx, nelem, n, d, scale, x, y, r);
+  }
   // r_t = y_t + scale sum_s x_s a(s, t) (sliceMaddMatrix; y = null: sliceMulMatrix) with coefficient
   // slot 1..4: a slot's pinned copy is rewritten only after a later ipm() has synchronised the
   // stream past its use.
@@ -872,11 +939,98 @@ struct BlockWork {
   }
 };
 
-// ThinQRfact (BlockConjugateGradient.h:132-168): m_rr = R^dag R (made manifestly Hermitian),
+// The block of Nblock right-hand sides BlockCGrQ works on, in one of two forms:
+//   SliceBlock  the fifth-dimension slices of one field (Orthog = 0, BlockCGrQsolve,
+//               BlockConjugateGradient.h:223-410); A = MpcDagMpc of a multi-RHS Wilson operator;
+//   VecBlock    a list of fields (BlockCGrQsolveVec, :554-741); A = MpcDagMpc on half fields or
+//               Mdag M on full fields of any operator, applied field by field.
+// Block vectors are named by id: B (sources), X (solutions), R, Q, D, Z (work).
+enum { BV_B = 0, BV_X, BV_R, BV_Q, BV_D, BV_Z, BV_COUNT };
+
+struct SliceBlock {
+  Op *op;
+  BlockWork *bw;
+  Field *v[BV_COUNT] = {};
+  int n() const { return op->g.Ls; }
+  int norms(int id, double *out) { return slice_reduce(op, 1, v[id], nullptr, nullptr, out); }
+  int apply(int in, int out) { return apply_mpcdagmpc(op, v[in], v[out]); }
+  int sub(int out, int x, int y) { return blas_axpby(v[out], 1.0, v[x], -1.0, v[y]); }  // out = x - y
+  int copy(int out, int in) {
+    GX_CHECK_HIP(hipMemcpyAsync(v[out]->data, v[in]->data, v[in]->bytes, hipMemcpyDeviceToDevice, op->ctx->stream));
+    v[out]->cb = v[in]->cb;
+    return GX_OK;
+  }
+  int ipm(int x, int y, CMat &m) { return bw->ipm(op->g, 1, v[x], v[y], m); }
+  int madd(int r, const CMat &a, int x, int y, double scale, int slot) {
+    return bw->madd(op->g, 1, v[r], a, v[x], y < 0 ? nullptr : v[y], scale, slot);
+  }
+  // res[b] = |A X_b - B_b|^2 (R is overwritten)
+  int residuals(double *res) {
+    GX_TRY(apply_mpcdagmpc(op, v[BV_X], v[BV_Z]));
+    return slice_reduce(op, 3, v[BV_Z], v[BV_B], v[BV_R], res);
+  }
+};
+
+struct VecBlock {
+  Op *op;
+  BlockWork *bw;
+  int herm;                  // GX_HERM_SCHUR or GX_HERM_MDAGM
+  int nb = 0;
+  Field *full_tmp = nullptr;  // Mdag M: M x
+  std::vector<Field *> v[BV_COUNT];
+  int n() const { return nb; }
+  int apply_one(Field *in, Field *out) {
+    if (herm == GX_HERM_SCHUR) return apply_mpcdagmpc(op, in, out);
+    GX_TRY(apply_full_m(op, in, full_tmp, 0));
+    return apply_full_m(op, full_tmp, out, 1);
+  }
+  int norms(int id, double *out) {
+    for (int b = 0; b < nb; b++) GX_TRY(blas_norm2(v[id][b], &out[b]));
+    return GX_OK;
+  }
+  int apply(int in, int out) {
+    for (int b = 0; b < nb; b++) GX_TRY(apply_one(v[in][b], v[out][b]));
+    return GX_OK;
+  }
+  int sub(int out, int x, int y) {
+    for (int b = 0; b < nb; b++) GX_TRY(blas_axpby(v[out][b], 1.0, v[x][b], -1.0, v[y][b]));
+    return GX_OK;
+  }
+  int copy(int out, int in) {
+    for (int b = 0; b < nb; b++) {
+      GX_CHECK_HIP(hipMemcpyAsync(v[out][b]->data, v[in][b]->data, v[in][b]->bytes, hipMemcpyDeviceToDevice,
+                                  op->ctx->stream));
+      v[out][b]->cb = v[in][b]->cb;
+    }
+    return GX_OK;
+  }
+  VecPtrs ptrs(int id) const {
+    VecPtrs p{};
+    if (id >= 0)
+      for (int b = 0; b < nb; b++) p.p[b] = v[id][b]->data;
+    return p;
+  }
+  int ipm(int x, int y, CMat &m) { return bw->ipm_vec(nb, v[x][0]->stored(), ptrs(x), ptrs(y), m); }
+  int madd(int r, const CMat &a, int x, int y, double scale, int slot) {
+    GX_TRY(bw->madd_vec(nb, v[x][0]->stored(), ptrs(r), a, ptrs(x), ptrs(y), scale, slot));
+    for (int b = 0; b < nb; b++) v[r][b]->cb = v[x][b]->cb;
+    return GX_OK;
+  }
+  int residuals(double *res) {
+    for (int b = 0; b < nb; b++) {
+      GX_TRY(apply_one(v[BV_X][b], v[BV_Z][b]));
+      GX_TRY(blas_axpy_norm(v[BV_R][b], -1.0, v[BV_B][b], v[BV_Z][b], &res[b]));
+    }
+    return GX_OK;
+  }
+};
+
+// ThinQRfact (BlockConjugateGradient.h:132-190): m_rr = R^dag R (made manifestly Hermitian),
 // L = chol(m_rr), C = L^dag, Q = R C^-1.
-static int thin_qr(BlockWork &bw, const Geom &g, Field *Q, Field *R, CMat &C, int slot) {
+template <class Blk>
+static int thin_qr(Blk &blk, int Q, int R, CMat &C, int slot) {
   CMat m, L, Cinv;
-  GX_TRY(bw.ipm(g, 1, R, R, m));
+  GX_TRY(blk.ipm(R, R, m));
   CMat h = m;
   for (int i = 0; i < m.n; i++)
     for (int j = 0; j < m.n; j++) h(i, j) = 0.5 * (m(i, j) + std::conj(m(j, i)));
@@ -889,27 +1043,21 @@ static int thin_qr(BlockWork &bw, const Geom &g, Field *Q, Field *R, CMat &C, in
     set_error("BlockCGrQ: singular triangular factor");
     return GX_ENOCONV;
   }
-  return bw.madd(g, 1, Q, Cinv, R, nullptr, 1.0, slot);  // sliceMulMatrix(Q, Cinv, R)
+  return blk.madd(Q, Cinv, R, -1, 1.0, slot);  // MulMatrix(Q, Cinv, R)
 }
 
-// BlockCGrQ (Birk's thesis after Dubrulle 2001; BlockConjugateGradient.h:223-410) on HermOp =
-// MpcDagMpc of a multi-RHS Wilson operator; the block is its Ls right-hand sides (Grid's code pins
-// Nblock to 8 at :227; here Nblock = Ls).
+// BlockCGrQ (Birk's thesis after Dubrulle 2001; BlockConjugateGradient.h:223-410 and the Vec form
+// :554-741):
 //   QC = R = B - A X (thin QR), D = Q;  for k:  Z = A D,  M = (D^dag Z)^-1,  X += D (M C),
 //   QS = Q - Z M (thin QR),  D = Q + D S^dag,  C = S C;  stop when max_b (C^dag C)_bb / |B_b|^2 < tol^2.
-static int block_cgrq_run(Op *op, Field *B, Field *X, double tol, int maxit, gx_solver_stats *st) {
+// Every right-hand side must be nonzero (the block must have full rank).
+template <class Blk>
+static int block_cgrq_core(Blk &blk, double tol, int maxit, gx_solver_stats *st) {
   const auto t0 = std::chrono::steady_clock::now();
-  const Geom &g = op->g;
-  const int n = g.Ls;
-  GX_TRY(op_prepare(op, GX_F64));
-  Field *R = op_tmp(op, GX_F64, 4), *Q = op_tmp(op, GX_F64, 5), *D = op_tmp(op, GX_F64, 6), *Z = op_tmp(op, GX_F64, 9);
-  if (!R || !Q || !D || !Z) return GX_ENOMEM;
-  X->cb = R->cb = Q->cb = D->cb = Z->cb = B->cb;
-  BlockWork bw;
-  GX_TRY(bw.init(op->ctx));
+  const int n = blk.n();
   std::vector<double> ssq(n), xn(n), res(n);
-  GX_TRY(slice_reduce(op, 1, B, nullptr, nullptr, ssq.data()));
-  GX_TRY(slice_reduce(op, 1, X, nullptr, nullptr, xn.data()));
+  GX_TRY(blk.norms(BV_B, ssq.data()));
+  GX_TRY(blk.norms(BV_X, xn.data()));
   for (int b = 0; b < n; b++) {
     if (std::isnan(ssq[b]) || std::isnan(xn[b])) {
       set_error("BlockCGrQ: NaN in source or guess (right-hand side %d)", b);
@@ -920,26 +1068,25 @@ static int block_cgrq_run(Op *op, Field *B, Field *X, double tol, int maxit, gx_
       return GX_EINVAL;
     }
   }
-  hipStream_t stream = op->ctx->stream;
-  GX_TRY(apply_mpcdagmpc(op, X, Z));
-  GX_TRY(blas_axpby(R, 1.0, B, -1.0, Z));  // R = B - A X
+  GX_TRY(blk.apply(BV_X, BV_Z));
+  GX_TRY(blk.sub(BV_R, BV_B, BV_Z));  // R = B - A X
   CMat C, S, M, DZ, rr;
-  GX_TRY(thin_qr(bw, g, Q, R, C, 3));
-  GX_CHECK_HIP(hipMemcpyAsync(D->data, Q->data, Q->bytes, hipMemcpyDeviceToDevice, stream));
+  GX_TRY(thin_qr(blk, BV_Q, BV_R, C, 3));
+  GX_TRY(blk.copy(BV_D, BV_Q));
   int rc = GX_ENOCONV, k;
   for (k = 1; k <= maxit; k++) {
-    GX_TRY(apply_mpcdagmpc(op, D, Z));  // Z = A D
-    GX_TRY(bw.ipm(g, 1, D, Z, DZ));
+    GX_TRY(blk.apply(BV_D, BV_Z));  // Z = A D
+    GX_TRY(blk.ipm(BV_D, BV_Z, DZ));
     if (!cm_inv(DZ, M)) {
       set_error("BlockCGrQ: D^dag A D is singular at iteration %d", k);
       break;
     }
-    GX_TRY(bw.madd(g, 1, X, cm_mul(M, C), D, X, 1.0, 1));  // X = X + D M C
-    GX_TRY(bw.madd(g, 1, R, M, Z, Q, -1.0, 2));            // R = Q - Z M
-    const int qr = thin_qr(bw, g, Q, R, S, 3);             // Q S = R
+    GX_TRY(blk.madd(BV_X, cm_mul(M, C), BV_D, BV_X, 1.0, 1));  // X = X + D M C
+    GX_TRY(blk.madd(BV_R, M, BV_Z, BV_Q, -1.0, 2));            // R = Q - Z M
+    const int qr = thin_qr(blk, BV_Q, BV_R, S, 3);             // Q S = R
     if (qr == GX_ENOCONV) break;
     GX_TRY(qr);
-    GX_TRY(bw.madd(g, 1, D, cm_adj(S), D, Q, 1.0, 4));     // D = Q + D S^dag
+    GX_TRY(blk.madd(BV_D, cm_adj(S), BV_D, BV_Q, 1.0, 4));     // D = Q + D S^dag
     C = cm_mul(S, C);
     rr = cm_mul(cm_adj(C), C);
     double mx = 0;
@@ -949,9 +1096,7 @@ static int block_cgrq_run(Op *op, Field *B, Field *X, double tol, int maxit, gx_
       break;
     }
   }
-  // true residuals |B_b - A X_b| / |B_b|
-  GX_TRY(apply_mpcdagmpc(op, X, Z));
-  GX_TRY(slice_reduce(op, 3, Z, B, R, res.data()));
+  GX_TRY(blk.residuals(res.data()));  // true residuals |B_b - A X_b| / |B_b|
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st)
     for (int b = 0; b < n; b++) {
@@ -962,6 +1107,56 @@ static int block_cgrq_run(Op *op, Field *B, Field *X, double tol, int maxit, gx_
       st[b].computed_residual = rr.n ? std::sqrt(std::max(0.0, rr(b, b).real()) / ssq[b]) : 0.0;
       st[b].seconds = secs;
     }
+  return rc;
+}
+
+// BlockCGrQ over the fifth-dimension slices of a multi-RHS Wilson operator (Nblock = Ls; Grid's
+// code pins Nblock to 8 at :227).
+static int block_cgrq_run(Op *op, Field *B, Field *X, double tol, int maxit, gx_solver_stats *st) {
+  GX_TRY(op_prepare(op, GX_F64));
+  Field *R = op_tmp(op, GX_F64, 4), *Q = op_tmp(op, GX_F64, 5), *D = op_tmp(op, GX_F64, 6), *Z = op_tmp(op, GX_F64, 9);
+  if (!R || !Q || !D || !Z) return GX_ENOMEM;
+  X->cb = R->cb = Q->cb = D->cb = Z->cb = B->cb;
+  BlockWork bw;
+  GX_TRY(bw.init(op->ctx));
+  SliceBlock blk{op, &bw};
+  blk.v[BV_B] = B, blk.v[BV_X] = X, blk.v[BV_R] = R, blk.v[BV_Q] = Q, blk.v[BV_D] = D, blk.v[BV_Z] = Z;
+  return block_cgrq_core(blk, tol, maxit, st);
+}
+
+// BlockCGrQsolveVec over n FP64 fields of one subset: half fields with A = MpcDagMpc, or full
+// fields with A = Mdag M.  The work fields (R, Q, D, Z: 4 n fields) live for the solve.
+static int block_cgrq_vec_run(Op *op, int herm, int n, Field **B, Field **X, double tol, int maxit,
+                              gx_solver_stats *st) {
+  GX_TRY(op_prepare(op, GX_F64));
+  BlockWork bw;
+  GX_TRY(bw.init(op->ctx));
+  VecBlock blk{op, &bw, herm, n};
+  struct Owned {
+    std::vector<Field *> list;
+    ~Owned() {
+      for (Field *f : list) field_destroy(f);
+    }
+  } owned;
+  const int subset = herm == GX_HERM_SCHUR ? GX_EVEN : GX_FULL;
+  for (int b = 0; b < n; b++) {
+    X[b]->cb = B[b]->cb;
+    blk.v[BV_B].push_back(B[b]);
+    blk.v[BV_X].push_back(X[b]);
+    for (int id = BV_R; id < BV_COUNT; id++) {
+      Field *f = nullptr;
+      GX_TRY(field_create(op->ctx, op->g, GX_F64, subset, &f));
+      owned.list.push_back(f);
+      f->cb = B[b]->cb;
+      blk.v[id].push_back(f);
+    }
+  }
+  if (herm == GX_HERM_MDAGM) {
+    GX_TRY(field_create(op->ctx, op->g, GX_F64, GX_FULL, &blk.full_tmp));
+    owned.list.push_back(blk.full_tmp);
+  }
+  const int rc = block_cgrq_core(blk, tol, maxit, st);
+  GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));  // the work fields are freed on return
   return rc;
 }
 
@@ -1120,8 +1315,6 @@ static int multishift_run(Op *op, Field *src, int nshift, const double *mass, co
 
 using namespace gx;
 
-struct gx_field_s : public Field {};
-struct gx_op_s : public Op {};
 
 extern "C" {
 
@@ -1245,6 +1438,68 @@ int gx_block_cgrq(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit
   return block_cgrq_run(op, src, psi, tol, maxit, st);
 }
 
+int gx_mixed_cg_batched(gx_op op_, int n, gx_field *src_, gx_field *sol_, double tol, double inner_tol, int inner_prec,
+                        int maxinner, int maxouter, int maxpatch, gx_solver_stats *st) {
+  Op *op = op_;
+  if (!op || !src_ || !sol_ || n < 1 || (inner_prec != GX_F32 && inner_prec != GX_F16)) {
+    set_error("mixed_cg_batched: needs an operator, n >= 1 fields and a GX_F32 or GX_F16 inner precision");
+    return GX_EINVAL;
+  }
+  std::vector<Field *> S(n), X(n);
+  for (int k = 0; k < n; k++) {
+    S[k] = src_[k];
+    X[k] = sol_[k];
+    for (Field *f : {S[k], X[k]})
+      if (!f || f->prec != GX_F64 || f->subset == GX_FULL || !same_geom(f->g, op->g)) {
+        set_error("mixed_cg_batched: field %d must be an F64 half field on the operator's lattice", k);
+        return GX_EINVAL;
+      }
+    if (S[k]->cb != S[0]->cb) {
+      set_error("mixed_cg_batched: sources on different checkerboards");
+      return GX_ECB;
+    }
+  }
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      if (X[k] == S[j] || (j != k && X[k] == X[j])) {
+        set_error("mixed_cg_batched: solutions must be distinct fields, apart from the sources");
+        return GX_EINVAL;
+      }
+  return mixed_batched_run(op, n, S.data(), X.data(), tol, inner_tol, inner_prec, maxinner, maxouter, maxpatch, st);
+}
+
+int gx_block_cgrq_vec(gx_op op_, int herm, int n, gx_field *src_, gx_field *psi_, double tol, int maxit,
+                      gx_solver_stats *st) {
+  Op *op = op_;
+  if (!op || !src_ || !psi_ || n < 1 || n > kMaxLs || (herm != GX_HERM_SCHUR && herm != GX_HERM_MDAGM)) {
+    set_error("block_cgrq_vec: needs an operator, 1..%d fields and herm GX_HERM_SCHUR or GX_HERM_MDAGM", kMaxLs);
+    return GX_EINVAL;
+  }
+  std::vector<Field *> B(n), X(n);
+  const bool half = herm == GX_HERM_SCHUR;
+  for (int b = 0; b < n; b++) {
+    B[b] = src_[b];
+    X[b] = psi_[b];
+    for (Field *f : {B[b], X[b]})
+      if (!f || f->prec != GX_F64 || (f->subset == GX_FULL) == half || !same_geom(f->g, op->g)) {
+        set_error("block_cgrq_vec: field %d must be an F64 %s field on the operator's lattice", b,
+                  half ? "half" : "full");
+        return GX_EINVAL;
+      }
+    if (half && B[b]->cb != B[0]->cb) {
+      set_error("block_cgrq_vec: sources on different checkerboards");
+      return GX_ECB;
+    }
+  }
+  for (int b = 0; b < n; b++)
+    for (int c = 0; c < n; c++)
+      if (X[b] == B[c] || (c != b && (X[b] == X[c]))) {
+        set_error("block_cgrq_vec: solutions must be distinct fields, apart from the sources");
+        return GX_EINVAL;
+      }
+  return block_cgrq_vec_run(op, herm, n, B.data(), X.data(), tol, maxit, st);
+}
+
 static bool slice_conformable(const Field *a, const Field *b) {
   return a && b && a->prec == GX_F64 && b->prec == GX_F64 && same_geom(a->g, b->g) &&
          (a->subset == GX_FULL) == (b->subset == GX_FULL);
@@ -1278,6 +1533,59 @@ int gx_slice_madd_matrix(gx_field R, const double *aa, gx_field X, gx_field Y, d
   GX_TRY(bw.init(X->ctx));
   GX_TRY(bw.madd(X->g, X->subset == GX_FULL ? 2 : 1, R, a, X, Y, scale, 1));
   GX_CHECK_HIP(hipStreamSynchronize(X->ctx->stream));  // the workspace is freed on return
+  return GX_OK;
+}
+
+// n fields of one lattice, subset and precision (F64) as a pointer list; false if they are not
+static bool vec_fields(int n, gx_field *in, VecPtrs &p, const Field **first) {
+  if (n < 1 || n > kMaxLs || !in) return false;
+  p = VecPtrs{};
+  for (int b = 0; b < n; b++) {
+    const Field *f = in[b];
+    if (!f || f->prec != GX_F64) return false;
+    if (b > 0 && (!same_geom(f->g, in[0]->g) || (f->subset == GX_FULL) != (in[0]->subset == GX_FULL))) return false;
+    p.p[b] = f->data;
+  }
+  *first = in[0];
+  return true;
+}
+
+int gx_inner_product_matrix(int n, gx_field *X, gx_field *Y, double *mat) {
+  VecPtrs px, py;
+  const Field *fx = nullptr, *fy = nullptr;
+  if (!mat || !vec_fields(n, X, px, &fx) || !vec_fields(n, Y, py, &fy) || !same_geom(fx->g, fy->g) ||
+      (fx->subset == GX_FULL) != (fy->subset == GX_FULL)) {
+    set_error("inner_product_matrix: needs 1..%d F64 fields of one lattice and subset on each side", kMaxLs);
+    return GX_EINVAL;
+  }
+  BlockWork bw;
+  GX_TRY(bw.init(fx->ctx));
+  CMat m;
+  GX_TRY(bw.ipm_vec(n, fx->stored(), px, py, m));
+  for (int i = 0; i < n * n; i++) {
+    mat[2 * i] = m.a[i].real();
+    mat[2 * i + 1] = m.a[i].imag();
+  }
+  return GX_OK;
+}
+
+int gx_madd_matrix(int n, gx_field *R, const double *aa, gx_field *X, gx_field *Y, double scale) {
+  VecPtrs pr, px, py{};
+  const Field *fr = nullptr, *fx = nullptr, *fy = nullptr;
+  bool ok = aa && vec_fields(n, R, pr, &fr) && vec_fields(n, X, px, &fx) && same_geom(fr->g, fx->g) &&
+            (fr->subset == GX_FULL) == (fx->subset == GX_FULL);
+  if (ok && Y) ok = vec_fields(n, Y, py, &fy) && same_geom(fr->g, fy->g) && (fr->subset == GX_FULL) == (fy->subset == GX_FULL);
+  if (!ok) {
+    set_error("madd_matrix: needs 1..%d F64 fields of one lattice and subset in R, X (and Y)", kMaxLs);
+    return GX_EINVAL;
+  }
+  CMat a(n);
+  for (int i = 0; i < n * n; i++) a.a[i] = cd(aa[2 * i], aa[2 * i + 1]);
+  BlockWork bw;
+  GX_TRY(bw.init(fx->ctx));
+  GX_TRY(bw.madd_vec(n, fx->stored(), pr, a, px, py, scale, 1));
+  for (int b = 0; b < n; b++) R[b]->cb = X[b]->cb;
+  GX_CHECK_HIP(hipStreamSynchronize(fx->ctx->stream));  // the workspace is freed on return
   return GX_OK;
 }
 

@@ -97,6 +97,82 @@ inline bool same_action(gx_op d, gx_op f) {
   check(gx_op_same_action(d, f, &same), "same_action");
   return same != 0;
 }
+// the libgx operator behind a matrix type, when it has one
+template <class M>
+auto handle_of(const M &m, int) -> decltype(m.handle()) {
+  return m.handle();
+}
+template <class M>
+gx_op handle_of(const M &, long) {
+  return nullptr;
+}
+// small dense complex matrices of the block solvers' host algebra (Eigen's role in Grid), row-major
+struct DenseC {
+  int n = 0;
+  std::vector<ComplexD> a;
+  explicit DenseC(int n_ = 0, bool ident = false) : n(n_), a((size_t)n_ * n_) {
+    for (int i = 0; ident && i < n; i++) a[(size_t)i * n + i] = 1.0;
+  }
+  ComplexD &operator()(int i, int j) { return a[(size_t)i * n + j]; }
+  ComplexD operator()(int i, int j) const { return a[(size_t)i * n + j]; }
+  DenseC adj() const {
+    DenseC z(n);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) z(i, j) = std::conj((*this)(j, i));
+    return z;
+  }
+  DenseC operator*(const DenseC &y) const {
+    DenseC z(n);
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) z(i, j) += (*this)(i, k) * y(k, j);
+    return z;
+  }
+  // Gauss-Jordan inverse with partial pivoting (asserts on a singular matrix, as Eigen's would fail)
+  DenseC inverse() const {
+    DenseC w = *this, out(n, true);
+    for (int c = 0; c < n; c++) {
+      int piv = c;
+      for (int r = c + 1; r < n; r++)
+        if (std::abs(w(r, c)) > std::abs(w(piv, c))) piv = r;
+      assert(std::abs(w(piv, c)) > 0.0 && "singular block matrix");
+      for (int j = 0; j < n && piv != c; j++) {
+        std::swap(w(piv, j), w(c, j));
+        std::swap(out(piv, j), out(c, j));
+      }
+      const ComplexD inv = 1.0 / w(c, c);
+      for (int j = 0; j < n; j++) {
+        w(c, j) *= inv;
+        out(c, j) *= inv;
+      }
+      for (int r = 0; r < n; r++)
+        if (r != c) {
+          const ComplexD f = w(r, c);
+          for (int j = 0; j < n; j++) {
+            w(r, j) -= f * w(c, j);
+            out(r, j) -= f * out(c, j);
+          }
+        }
+    }
+    return out;
+  }
+  // lower Cholesky factor L of a Hermitian positive-definite matrix (Eigen's llt().matrixL())
+  DenseC cholesky() const {
+    DenseC L(n);
+    for (int j = 0; j < n; j++) {
+      double d = (*this)(j, j).real();
+      for (int k = 0; k < j; k++) d -= std::norm(L(j, k));
+      assert(d > 0.0 && "block matrix not positive definite");
+      L(j, j) = std::sqrt(d);
+      for (int i = j + 1; i < n; i++) {
+        ComplexD v = (*this)(i, j);
+        for (int k = 0; k < j; k++) v -= L(i, k) * std::conj(L(j, k));
+        L(i, j) = v / L(j, j);
+      }
+    }
+    return L;
+  }
+};
 inline uint64_t mix(uint64_t x) {
   x += 0x9E3779B97F4A7C15ULL;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -645,6 +721,8 @@ class LinearOperatorBase {
   virtual void HermOp(const Field &in, Field &out) = 0;
   // set when the operator is a Schur operator over a libgx action: solvers then run on device
   virtual gx_op schur_handle() const { return nullptr; }
+  // set when the operator is Mdag M of a libgx action (full fields): block solvers run on device
+  virtual gx_op mdagm_handle() const { return nullptr; }
   virtual ~LinearOperatorBase() = default;
 };
 
@@ -705,6 +783,7 @@ class MdagMLinearOperator : public LinearOperatorBase<Field> {
     n1 = innerProduct(in, out).real();
     n2 = norm2(out);
   }
+  gx_op mdagm_handle() const override { return detail::handle_of(_Mat, 0); }
 };
 
 template <class Field>
@@ -883,6 +962,130 @@ class MixedPrecisionConjugateGradient {
     CG_d(Linop_d, src_d_in, sol_d);
     TotalFinalStepIterations = CG_d.IterationsToComplete;
     TrueResidual = CG_d.TrueResidual;
+  }
+};
+
+// Grid: MixedPrecisionConjugateGradientBatched<FieldD,FieldF>(tol, maxinner, maxouter, maxpatchup, sp_grid,
+// Linop_f, Linop_d, updateResidual) (ConjugateGradientMixedPrecBatched.h:37-213): defect correction
+// for a batch of sources with one inner tolerance for the batch.  When Linop_f is the same action
+// as Linop_d (a Schur operator over libgx) the batch runs on the device (gx_mixed_cg_batched): on a
+// Wilson operator as one multi-RHS solve sharing every link load, otherwise source by source;
+// InnerHalf = true keeps the inner vectors in FP16 storage.  Other operators, and updateResidual =
+// false, run Grid's loop here with the facade's ConjugateGradient.
+template <class FieldD, class FieldF>
+class MixedPrecisionConjugateGradientBatched {
+ public:
+  RealD Tolerance, InnerTolerance;
+  Integer MaxInnerIterations, MaxOuterIterations, MaxPatchupIterations;
+  GridBase *SinglePrecGrid;
+  RealD OuterLoopNormMult = 100.;
+  LinearOperatorBase<FieldF> &Linop_f;
+  LinearOperatorBase<FieldD> &Linop_d;
+  bool updateResidual;
+  bool InnerHalf = false;  // no Grid counterpart: FP16-storage inner solves instead of FP32
+  Integer TotalOuterIterations = 0;
+  std::vector<Integer> TotalInnerIterations, TotalFinalStepIterations;
+  MixedPrecisionConjugateGradientBatched(RealD tol, Integer maxinnerit, Integer maxouterit, Integer maxpatchit,
+                                         GridBase *sp_grid, LinearOperatorBase<FieldF> &lf,
+                                         LinearOperatorBase<FieldD> &ld, bool update_residual = true)
+      : Tolerance(tol), InnerTolerance(tol), MaxInnerIterations(maxinnerit), MaxOuterIterations(maxouterit),
+        MaxPatchupIterations(maxpatchit), SinglePrecGrid(sp_grid), Linop_f(lf), Linop_d(ld),
+        updateResidual(update_residual) {}
+  void operator()(const FieldD &src_d_in, FieldD &sol_d) {
+    std::vector<FieldD> srcs, sols;
+    srcs.emplace_back(src_d_in.Grid());
+    sols.emplace_back(sol_d.Grid());
+    srcs[0] = src_d_in;
+    sols[0] = sol_d;
+    (*this)(srcs, sols);
+    sol_d = sols[0];
+  }
+  void operator()(const std::vector<FieldD> &src_d_in, std::vector<FieldD> &sol_d) {
+    const int n = (int)src_d_in.size();
+    assert(n == (int)sol_d.size() && n >= 1);
+    gx_op op = Linop_d.schur_handle();
+    if (!updateResidual || !detail::same_action(op, Linop_f.schur_handle())) {
+      host_loop(src_d_in, sol_d);
+      return;
+    }
+    std::vector<gx_field> hs(n), hx(n);
+    for (int i = 0; i < n; i++) {
+      sol_d[i].Checkerboard() = src_d_in[i].Checkerboard();
+      hs[i] = src_d_in[i].f;
+      hx[i] = sol_d[i].f;
+    }
+    std::vector<gx_solver_stats> st(n);
+    const int rc = gx_mixed_cg_batched(op, n, hs.data(), hx.data(), Tolerance, InnerTolerance,
+                                       InnerHalf ? GX_F16 : GX_F32, MaxInnerIterations, MaxOuterIterations,
+                                       MaxPatchupIterations, st.data());
+    if (rc != GX_ENOCONV) detail::check(rc, "MixedPrecisionConjugateGradientBatched");
+    TotalInnerIterations.assign(n, 0);
+    TotalFinalStepIterations.assign(n, 0);
+    TotalOuterIterations = st[0].outer_iterations;
+    for (int i = 0; i < n; i++) {
+      TotalInnerIterations[i] = st[i].inner_iterations;
+      TotalFinalStepIterations[i] = st[i].final_iterations;
+      std::cout << GridLogMessage << "MixedPrecisionConjugateGradientBatched: solve " << i << " Inner CG iterations "
+                << TotalInnerIterations[i] << " Restarts " << st[i].outer_iterations << " Final CG iterations "
+                << TotalFinalStepIterations[i] << " true residual " << st[i].true_residual << std::endl;
+    }
+  }
+
+ private:
+  // Grid's loop (ConjugateGradientMixedPrecBatched.h:77-202) over the facade's BLAS1 and CG
+  void host_loop(const std::vector<FieldD> &src_d_in, std::vector<FieldD> &sol_d) {
+    const int n = (int)src_d_in.size();
+    const int cb = src_d_in[0].Checkerboard();
+    GridBase *gd = src_d_in[0].Grid();
+    std::vector<RealD> src_norm(n), norm(n, 0.0), stop(n);
+    std::vector<FieldD> src_d;
+    std::vector<FieldF> src_f, sol_f;
+    FieldD tmp_d(gd);
+    tmp_d.Checkerboard() = cb;
+    for (int i = 0; i < n; i++) {
+      sol_d[i].Checkerboard() = cb;
+      src_norm[i] = norm2(src_d_in[i]);
+      stop[i] = src_norm[i] * Tolerance * Tolerance;
+      src_d.emplace_back(gd);
+      src_d[i] = src_d_in[i];
+      src_f.emplace_back(SinglePrecGrid);
+      sol_f.emplace_back(SinglePrecGrid);
+      src_f[i].Checkerboard() = cb;
+      sol_f[i].Checkerboard() = cb;
+    }
+    TotalInnerIterations.assign(n, 0);
+    TotalFinalStepIterations.assign(n, 0);
+    RealD inner_tol = InnerTolerance;
+    ConjugateGradient<FieldF> CG_f(inner_tol, MaxInnerIterations, false);
+    for (TotalOuterIterations = 0; TotalOuterIterations < MaxOuterIterations; TotalOuterIterations++) {
+      bool all = true;
+      for (int i = 0; i < n; i++) {
+        Linop_d.HermOp(sol_d[i], tmp_d);
+        norm[i] = axpy_norm(src_d[i], -1., tmp_d, src_d_in[i]);
+        precisionChange(src_f[i], src_d[i]);
+        sol_f[i] = Zero();
+        sol_f[i].Checkerboard() = cb;
+        if (norm[i] > OuterLoopNormMult * stop[i]) all = false;
+      }
+      if (all) break;
+      if (updateResidual) {
+        const RealD nmax = *std::max_element(norm.begin(), norm.end());
+        const RealD smax = *std::max_element(stop.begin(), stop.end());
+        while (nmax * inner_tol * inner_tol < smax) inner_tol *= 2;
+        CG_f.Tolerance = inner_tol;
+      }
+      for (int i = 0; i < n; i++) {
+        CG_f(Linop_f, src_f[i], sol_f[i]);
+        TotalInnerIterations[i] += CG_f.IterationsToComplete;
+        precisionChange(tmp_d, sol_f[i]);
+        axpy(sol_d[i], 1.0, tmp_d, sol_d[i]);
+      }
+    }
+    for (int i = 0; i < n; i++) {
+      ConjugateGradient<FieldD> CG_d(Tolerance, MaxPatchupIterations);
+      CG_d(Linop_d, src_d_in[i], sol_d[i]);
+      TotalFinalStepIterations[i] += CG_d.IterationsToComplete;
+    }
   }
 };
 
@@ -1262,6 +1465,134 @@ class BlockConjugateGradient : public OperatorFunction<Field> {
     }
     std::cout << GridLogMessage << (CGtype == BlockCGrQ ? "BlockCGrQ" : "MultiRHS solver") << " converged in "
               << IterationsToComplete << " iterations; true residual " << TrueResidual << std::endl;
+  }
+
+  // Grid: operator()(Linop, std::vector<Field> Src, std::vector<Field> Psi) (BlockConjugateGradient.h:
+  // 206-213): BlockCGrQVec only, as in Grid.  On a Schur operator (MpcDagMpc, half fields) or an
+  // MdagMLinearOperator (full fields) over a libgx action with FP64 fields the whole solve runs on the
+  // device (gx_block_cgrq_vec); otherwise Grid's BlockCGrQsolveVec loop (:554-741) runs here with the
+  // user's HermOp field by field and the device InnerProductMatrix / MaddMatrix.
+  virtual void operator()(LinearOperatorBase<Field> &Linop, const std::vector<Field> &Src, std::vector<Field> &Psi) {
+    assert(CGtype == BlockCGrQVec && "BlockConjugateGradient over std::vector<Field>: BlockCGrQVec");
+    Nblock = (int)Src.size();
+    assert(Nblock == (int)Psi.size() && Nblock >= 1);
+    for (int b = 0; b < Nblock; b++) Psi[b].Checkerboard() = Src[b].Checkerboard();
+    const bool half = Src[0].Grid()->checkerboarded;
+    gx_op op = half ? Linop.schur_handle() : Linop.mdagm_handle();
+    if (!op || Field::Prec != GX_F64 || Nblock > 16) {
+      host_vec(Linop, Src, Psi);
+      return;
+    }
+    std::vector<gx_field> hs(Nblock), hx(Nblock);
+    for (int b = 0; b < Nblock; b++) {
+      hs[b] = Src[b].f;
+      hx[b] = Psi[b].f;
+    }
+    std::vector<gx_solver_stats> st(Nblock);
+    const int rc = gx_block_cgrq_vec(op, half ? GX_HERM_SCHUR : GX_HERM_MDAGM, Nblock, hs.data(), hx.data(), Tolerance,
+                                     MaxIterations, st.data());
+    if (rc != GX_ENOCONV) detail::check(rc, "BlockConjugateGradient(BlockCGrQVec)");
+    double rr = 0, ss = 0;
+    IterationsToComplete = 0;
+    for (int b = 0; b < Nblock; b++) {
+      const double nb = norm2(Src[b]);
+      rr += st[b].true_residual * st[b].true_residual * nb;
+      ss += nb;
+      IterationsToComplete = std::max<Integer>(IterationsToComplete, st[b].iterations);
+    }
+    TrueResidual = ss > 0 ? std::sqrt(rr / ss) : 0.0;
+    report(rc == GX_OK);
+  }
+
+ private:
+  void report(bool converged) {
+    if (!converged) {
+      std::cout << GridLogMessage << "BlockConjugateGradient(rQ) did NOT converge" << std::endl;
+      if (ErrorOnNoConverge) assert(0);
+      return;
+    }
+    std::cout << GridLogMessage << "BlockCGrQ converged in " << IterationsToComplete << " iterations; true residual "
+              << TrueResidual << std::endl;
+  }
+  static std::vector<gx_field> handles(const std::vector<Field> &v) {
+    std::vector<gx_field> h(v.size());
+    for (size_t b = 0; b < v.size(); b++) h[b] = v[b].f;
+    return h;
+  }
+  static detail::DenseC ipm(const std::vector<Field> &X, const std::vector<Field> &Y) {
+    const int n = (int)X.size();
+    std::vector<double> m(2 * (size_t)n * n);
+    detail::check(gx_inner_product_matrix(n, handles(X).data(), handles(Y).data(), m.data()), "InnerProductMatrix");
+    detail::DenseC c(n);
+    for (int i = 0; i < n * n; i++) c.a[i] = ComplexD(m[2 * i], m[2 * i + 1]);
+    return c;
+  }
+  // R = Y + scale X a (MaddMatrix); Y = nullptr: MulMatrix
+  static void madd(std::vector<Field> &R, const detail::DenseC &a, const std::vector<Field> &X,
+                   const std::vector<Field> *Y, double scale = 1.0) {
+    const int n = (int)X.size();
+    std::vector<double> m(2 * (size_t)n * n);
+    for (int i = 0; i < n * n; i++) {
+      m[2 * i] = a.a[i].real();
+      m[2 * i + 1] = a.a[i].imag();
+    }
+    std::vector<gx_field> hy = Y ? handles(*Y) : std::vector<gx_field>();
+    detail::check(gx_madd_matrix(n, handles(R).data(), m.data(), handles(X).data(), Y ? hy.data() : nullptr, scale),
+                  "MaddMatrix");
+  }
+  // ThinQRfact (BlockConjugateGradient.h:168-190): C = chol(R^dag R)^dag, Q = R C^-1
+  static void thin_qr(std::vector<Field> &Q, detail::DenseC &C, const std::vector<Field> &R) {
+    const detail::DenseC m = ipm(R, R);
+    detail::DenseC h(m.n);
+    for (int i = 0; i < m.n; i++)
+      for (int j = 0; j < m.n; j++) h(i, j) = 0.5 * (m(i, j) + std::conj(m(j, i)));
+    C = h.cholesky().adj();
+    madd(Q, C.inverse(), R, nullptr);
+  }
+  void host_vec(LinearOperatorBase<Field> &Linop, const std::vector<Field> &B, std::vector<Field> &X) {
+    const int n = Nblock;
+    GridBase *g = B[0].Grid();
+    std::vector<Field> tmp, Q, D, Z;
+    for (int b = 0; b < n; b++) {
+      tmp.emplace_back(g), Q.emplace_back(g), D.emplace_back(g), Z.emplace_back(g);
+      tmp[b].Checkerboard() = Q[b].Checkerboard() = D[b].Checkerboard() = Z[b].Checkerboard() = B[b].Checkerboard();
+    }
+    std::vector<RealD> ssq(n);
+    for (int b = 0; b < n; b++) {
+      ssq[b] = norm2(B[b]);
+      assert(!std::isnan(ssq[b]) && !std::isnan(norm2(X[b])));
+    }
+    for (int b = 0; b < n; b++) {
+      Linop.HermOp(X[b], Z[b]);
+      tmp[b] = B[b] - Z[b];
+    }
+    detail::DenseC C, S, M;
+    thin_qr(Q, C, tmp);
+    for (int b = 0; b < n; b++) D[b] = Q[b];
+    int k;
+    bool converged = false;
+    for (k = 1; k <= MaxIterations && !converged; k++) {
+      for (int b = 0; b < n; b++) Linop.HermOp(D[b], Z[b]);
+      M = ipm(D, Z).inverse();
+      madd(X, M * C, D, &X);
+      madd(tmp, M, Z, &Q, -1.0);
+      thin_qr(Q, S, tmp);
+      madd(D, S.adj(), D, &Q);
+      C = S * C;
+      const detail::DenseC rr = C.adj() * C;
+      double mx = 0;
+      for (int b = 0; b < n; b++) mx = std::max(mx, rr(b, b).real() / ssq[b]);
+      converged = mx < Tolerance * Tolerance;
+    }
+    IterationsToComplete = k - 1;
+    double num = 0, den = 0;
+    for (int b = 0; b < n; b++) {
+      Linop.HermOp(X[b], Z[b]);
+      num += norm2(Z[b] - B[b]);
+      den += ssq[b];
+    }
+    TrueResidual = std::sqrt(num / den);
+    report(converged);
   }
 };
 

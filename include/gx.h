@@ -78,7 +78,8 @@ int gx_set_checkerboard(gx_field full, gx_field half);
 int gx_precision_change(gx_field out, gx_field in);
 /* Grid: ExtractSlice(lowDim, higherDim, slice, orthog) / InsertSlice(lowDim, higherDim, slice,
  * orthog), Lattice_transfer.h:949-1047, with orthog the fifth dimension: a 4D field (Ls = 1) from
- * or into slice s of a field on the same 4D lattice.  Full fields of one precision. */
+ * or into slice s of a field on the same 4D lattice.  Two full or two half fields of one precision
+ * (F64 or F32); the receiving half field takes the other's checkerboard. */
 int gx_extract_slice(gx_field out4d, gx_field in5d, int s);
 int gx_insert_slice(gx_field in4d, gx_field out5d, int s);
 
@@ -249,6 +250,29 @@ int gx_block_cg(gx_op op, gx_field src, gx_field psi, double tol, int maxit, int
  * st: Ls statistics with per-right-hand-side true residuals.  FP64 only. */
 int gx_block_cgrq(gx_op op, gx_field src, gx_field psi, double tol, int maxit, gx_solver_stats *st);
 
+/* Grid: MixedPrecisionConjugateGradientBatched<FieldD,FieldF>(tol, maxinner, maxouter, maxpatchup,
+ * sp_grid, Linop_f, Linop_d)(std::vector<FieldD> src, std::vector<FieldD> sol),
+ * ConjugateGradientMixedPrecBatched.h:37-213: defect correction for n sources with FP64 outer
+ * residuals, one inner tolerance for the batch (doubled while the largest residual times its
+ * square is below the largest stopping norm, starting from inner_tol), inner solves in inner_prec
+ * (GX_F32, or GX_F16 storage) and an FP64 patch-up CG of at most maxpatch iterations per source.
+ * On a single-RHS Wilson operator and 2..16 sources the batch runs as one multi-RHS solve on a
+ * clone of the operator (every link load shared by the batch); otherwise source by source.  src,
+ * sol: n F64 half fields of one checkerboard; sol holds the guesses; st: n statistics. */
+int gx_mixed_cg_batched(gx_op op, int n, gx_field *src, gx_field *sol, double tol, double inner_tol, int inner_prec,
+                        int maxinner, int maxouter, int maxpatch, gx_solver_stats *st);
+
+/* Grid: BlockConjugateGradient(BlockCGrQVec, _, tol, maxit)(Linop, std::vector<Field> src, psi),
+ * BlockConjugateGradient.h:554-741 (BlockCGrQsolveVec): block CG over a list of n fields, with the
+ * operator applied field by field.  herm = GX_HERM_SCHUR: A = MpcDagMpc (SchurDiagMooeeOperator) on
+ * F64 half fields of one checkerboard; GX_HERM_MDAGM: A = Mdag M (MdagMLinearOperator) on full F64
+ * fields.  Any operator kind (Wilson, Mobius).  psi holds the guesses.  Stops when the largest
+ * relative residual of the block recursion is below tol; st: n statistics with true residuals
+ * |A psi_b - src_b| / |src_b|.  1 <= n <= 16; every source must be nonzero. */
+enum gx_herm { GX_HERM_SCHUR = 0, GX_HERM_MDAGM = 1 };
+int gx_block_cgrq_vec(gx_op op, int herm, int n, gx_field *src, gx_field *psi, double tol, int maxit,
+                      gx_solver_stats *st);
+
 /* Grid: ConjugateGradientMultiShift<Field>(maxit, MultiShiftFunction)(Linop, src, results),
  * ConjugateGradientMultiShift.h:89-343: solves (MpcDagMpc + shifts[s]) psi[s] = src for nshift
  * shifts at once (poles = shifts, tolerances = per-shift relative residuals), from the Krylov
@@ -268,6 +292,16 @@ int gx_slice_inner_product_matrix(gx_field lhs, gx_field rhs, double *mat);
  * R_t = Y_t + scale sum_s X_s aa(s, t).  aa: 2 Ls^2 doubles, row-major.  Y = NULL gives
  * sliceMulMatrix (R_t = scale sum_s X_s aa(s, t)).  R may be X or Y. */
 int gx_slice_madd_matrix(gx_field R, const double *aa, gx_field X, gx_field Y, double scale);
+
+/* Grid: InnerProductMatrix(m, X, Y) over std::vector<Field> (BlockConjugateGradient.h:35-42):
+ * mat(b, bp) = <X_b, Y_bp> for n F64 fields of one lattice and subset.  mat: 2 n^2 doubles (re, im),
+ * row-major.  1 <= n <= 16. */
+int gx_inner_product_matrix(int n, gx_field *X, gx_field *Y, double *mat);
+
+/* Grid: MaddMatrix(R, aa, X, Y, scale) / MulMatrix(R, aa, X) over std::vector<Field>
+ * (BlockConjugateGradient.h:44-74): R_b = Y_b + scale sum_bp X_bp aa(bp, b); Y = NULL gives
+ * MulMatrix.  R may be X or Y.  aa: 2 n^2 doubles, row-major. */
+int gx_madd_matrix(int n, gx_field *R, const double *aa, gx_field *X, gx_field *Y, double scale);
 
 /* ---- solve serving (no Grid counterpart) ----
  * A request asks for the Wilson solves M x_k = b_k, k < nsrc, on one of the resident gauge fields

@@ -121,3 +121,140 @@ def test_block_cgrq_guess_and_errors(ctx):
     M = gx.FermionOp(ctx, U, dims, 4, 'mobius', mass=0.1)
     with pytest.raises(gx.GxError):
         M.block_cgrq(M.field(gx.F64, ODD), M.field(gx.F64, ODD))
+
+
+@pytest.mark.parametrize('nrhs', [1, 3, 6])
+def test_block_cgrq_vec_mobius_schur(ctx, nrhs):
+    """BlockCGrQVec over a list of Mobius half fields (A = MpcDagMpc, the operator applied field by
+    field): every solution equals the oracle's single-RHS FP64 CG solution, and the block's
+    iteration count is at most the slowest single solve's."""
+    dims, Ls = (8, 4, 4, 8), 8
+    kw = dict(mass=0.05, M5=1.8, b=1.5, c=0.5)
+    U = gx.random_gauge(dims, 341, 0.3)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    O = Oracle(U, dims, Ls, 'mobius', **kw)
+    srcs = [R.pick(gx.gaussian_fermion(dims, Ls, 342 + k), dims, ODD) for k in range(nrhs)]
+    fs = [D.field(gx.F64, ODD).upload(s) for s in srcs]
+    fx = [D.field(gx.F64, ODD) for _ in range(nrhs)]
+    sts = D.block_cgrq_vec(fs, fx, tol=1e-10, maxit=5000)
+    worst = 0
+    for k in range(nrhs):
+        ref, rst = O.cg(srcs[k], tol=1e-10, maxit=5000)
+        worst = max(worst, rst['iterations'])
+        assert sts[k]['converged'] and sts[k]['true_residual'] < 1e-9, sts[k]
+        assert rel_err(fx[k].download(), ref) < 1e-8, k
+    assert sts[0]['iterations'] <= worst + 1, (sts[0], worst)
+
+
+def test_block_cgrq_vec_mobius_mdagm_full(ctx):
+    """BlockCGrQVec with Mdag M on full Mobius fields (MdagMLinearOperator, as in Grid's
+    tests/solver/Test_mobius_bcg_nosplit.cc): each solution's residual |Mdag M x - b| / |b|,
+    recomputed with the oracle's M and Mdag, is at the tolerance; guesses are honoured."""
+    dims, Ls, nrhs = (4, 4, 4, 4), 6, 2
+    kw = dict(mass=0.1, M5=1.8, b=1.5, c=0.5)
+    U = gx.random_gauge(dims, 351, 0.3)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    O = Oracle(U, dims, Ls, 'mobius', **kw)
+    srcs = [gx.gaussian_fermion(dims, Ls, 352 + k) for k in range(nrhs)]
+    fs = [D.field().upload(s) for s in srcs]
+    fx = [D.field().upload(0.2 * gx.gaussian_fermion(dims, Ls, 360 + k)) for k in range(nrhs)]
+    sts = D.block_cgrq_vec(fs, fx, tol=1e-10, maxit=20000)
+    for k in range(nrhs):
+        x = fx[k].download()
+        r = O.apply('Mdag', O.apply('M', x)) - srcs[k]
+        res = np.linalg.norm(r) / np.linalg.norm(srcs[k])
+        assert sts[k]['converged'] and res < 2e-10 and abs(res - sts[k]['true_residual']) < 1e-3 * res, (k, res, sts[k])
+
+
+def test_block_cgrq_vec_matches_slice_form(ctx):
+    """On a multi-RHS Wilson operator the list form (single-RHS operator, one field per source)
+    and the slice form (one field, right-hand side = fifth-dimension slice) run the same algorithm:
+    same iteration count and solutions within rounding."""
+    dims, nrhs, mass = (8, 8, 4, 8), 4, 0.1
+    U = gx.random_gauge(dims, 371, 0.5)
+    src = R.pick(gx.gaussian_fermion(dims, nrhs, 372), dims, ODD)
+    B = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+    fs, fp = B.field(gx.F64, ODD).upload(src), B.field(gx.F64, ODD)
+    sl = B.block_cgrq(fs, fp, tol=1e-10)
+    D = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=mass)
+    vs = [D.field(gx.F64, ODD).upload(np.ascontiguousarray(src[..., k:k + 1, :, :])) for k in range(nrhs)]
+    vx = [D.field(gx.F64, ODD) for _ in range(nrhs)]
+    ve = D.block_cgrq_vec(vs, vx, tol=1e-10)
+    x = fp.download()
+    assert abs(sl[0]['iterations'] - ve[0]['iterations']) <= 1
+    for k in range(nrhs):
+        assert rel_err(vx[k].download(), x[..., k:k + 1, :, :]) < 1e-9
+
+
+def test_block_cgrq_vec_errors(ctx):
+    dims = (4, 4, 4, 4)
+    D = gx.FermionOp(ctx, gx.random_gauge(dims, 381, 0.5), dims, 1, 'wilson', mass=0.1)
+    half = [D.field(gx.F64, ODD) for _ in range(2)]
+    full = [D.field() for _ in range(2)]
+    with pytest.raises(gx.GxError):   # half fields with Mdag M
+        D.block_cgrq_vec(half, [D.field(gx.F64, ODD) for _ in range(2)], herm=gx.HERM_MDAGM)
+    with pytest.raises(gx.GxError):   # full fields with MpcDagMpc
+        D.block_cgrq_vec(full, [D.field() for _ in range(2)], herm=gx.HERM_SCHUR)
+    with pytest.raises(gx.GxError):   # a solution that is also a source
+        D.block_cgrq_vec(half, [half[1], D.field(gx.F64, ODD)])
+    with pytest.raises(gx.GxError):   # zero sources: rank-deficient block
+        D.block_cgrq_vec(half, [D.field(gx.F64, ODD) for _ in range(2)])
+
+
+@pytest.mark.parametrize('inner', [gx.F32, gx.F16])
+def test_mixed_cg_batched_wilson(ctx, inner):
+    """MixedPrecisionConjugateGradientBatched on a single-RHS Wilson operator (the batch packed into
+    one multi-RHS solve): every source reaches the oracle's FP64 CG solution; a nonzero guess is
+    honoured; the sources stay unchanged."""
+    dims, n, mass = (8, 8, 4, 8), 5, 0.08
+    U = gx.random_gauge(dims, 391, 0.5)
+    D = gx.FermionOp(ctx, U, dims, 1, 'wilson', mass=mass)
+    O = Oracle(U, dims, 1, 'wilson', mass=mass)
+    srcs = [R.pick(gx.gaussian_fermion(dims, 1, 392 + k), dims, ODD) for k in range(n)]
+    fs = [D.field(gx.F64, ODD).upload(s) for s in srcs]
+    fx = [D.field(gx.F64, ODD) for _ in range(n)]
+    fx[2].upload(0.1 * srcs[2])
+    sts = D.mixed_cg_batched(fs, fx, tol=1e-10, inner_tol=1e-5, inner_prec=inner)
+    for k in range(n):
+        ref, _ = O.cg(srcs[k], tol=1e-10)
+        assert sts[k]['converged'] and sts[k]['true_residual'] < 2e-10, sts[k]
+        assert rel_err(fx[k].download(), ref) < 1e-8, k
+        assert np.array_equal(fs[k].download(), srcs[k])
+
+
+def test_mixed_cg_batched_mobius(ctx):
+    """On a Mobius operator the batch runs source by source and gives what gx_mixed_cg gives each."""
+    dims, Ls, n = (8, 4, 4, 8), 6, 2
+    kw = dict(mass=0.05, M5=1.8, b=1.5, c=0.5)
+    U = gx.random_gauge(dims, 401, 0.3)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    srcs = [R.pick(gx.gaussian_fermion(dims, Ls, 402 + k), dims, ODD) for k in range(n)]
+    fs = [D.field(gx.F64, ODD).upload(s) for s in srcs]
+    fx = [D.field(gx.F64, ODD) for _ in range(n)]
+    sts = D.mixed_cg_batched(fs, fx, tol=1e-10)
+    for k in range(n):
+        one = D.field(gx.F64, ODD)
+        D.mixed_cg(fs[k], one, tol=1e-10, inner_tol=1e-5)
+        assert sts[k]['converged'] and np.array_equal(fx[k].download(), one.download()), k
+
+
+@pytest.mark.parametrize('subset', [ODD, gx.FULL])
+def test_inner_product_and_madd_matrix_lists(ctx, subset):
+    """InnerProductMatrix and MaddMatrix over lists of fields against numpy, with R aliasing X or Y."""
+    dims, Ls, n = (4, 4, 2, 6), 3, 4
+    xs = fields(ctx, dims, Ls, subset, range(410, 410 + n))
+    ys = fields(ctx, dims, Ls, subset, range(420, 420 + n))
+    X = np.stack([a for _, a in xs]).reshape(n, -1)
+    Y = np.stack([a for _, a in ys]).reshape(n, -1)
+    m = gx.inner_product_matrix([f for f, _ in xs], [f for f, _ in ys])
+    assert np.max(np.abs(m - X.conj() @ Y.T)) < 1e-12 * np.linalg.norm(X) * np.linalg.norm(Y)
+    rng = np.random.default_rng(9)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    ref = Y + 0.5 * (a.T @ X)
+    rs = [gx.Field(ctx, dims, Ls, gx.F64, subset) for _ in range(n)]
+    gx.madd_matrix(rs, a, [f for f, _ in xs], [f for f, _ in ys], 0.5)
+    got = np.stack([r.download() for r in rs]).reshape(n, -1)
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
+    gx.madd_matrix([f for f, _ in xs], a, [f for f, _ in xs], [f for f, _ in ys], 0.5)   # R aliases X
+    got = np.stack([f.download() for f, _ in xs]).reshape(n, -1)
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
