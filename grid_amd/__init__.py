@@ -333,7 +333,7 @@ def field_copy(dst, src):
     _check(lib().gx_field_copy(dst.h, src.h))
 
 
-TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH, TUNE_FUSED = 0, 1, 2, 3, 4, 5
+TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH, TUNE_FUSED, TUNE_DHOP_SHAPE = 0, 1, 2, 3, 4, 5, 6
 
 
 def tune_set(knob, value):

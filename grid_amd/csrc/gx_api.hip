@@ -28,7 +28,7 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
-static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0};
+static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0};  // map, pf, nt, order, graph, fused, shape
 static int g_tune_version = 0;
 int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
 int tune_version() { return g_tune_version; }
@@ -64,7 +64,7 @@ static bool geom_ok(const int dims[4], int Ls) {
   for (int d = 0; d < 4; d++)
     if (dims[d] < 2) return false;
   if (dims[0] % 2) return false;
-  return Ls >= 1 && Ls <= kMaxLs;
+  return Ls >= 1 && Ls <= kMaxFifth;
 }
 
 static Geom make_geom(const int dims[4], int Ls) {
@@ -799,6 +799,7 @@ int gx_tune_set(int knob, int value) {
   GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT, GX_EINVAL, "unknown tuning knob %d", knob);
   if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 4, GX_EINVAL, "tile map takes 0..4");
   else if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
+  else if (knob == GX_TUNE_DHOP_SHAPE) GX_REQUIRE(value >= 0 && value <= 8, GX_EINVAL, "tile shape takes 0..8 waves");
   g_tune[knob] = value;
   g_tune_version++;
   return GX_OK;
@@ -817,7 +818,7 @@ int gx_ctx_create(int device, gx_ctx *out) {
   c->device = device;
   GX_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (int i = 0; i < 16; i++) GX_CHECK_HIP(hipEventCreate(&c->ev[i]));
-  GX_CHECK_HIP(hipMalloc(&c->red_partial, sizeof(double) * kRedBlocks * kMaxLs));
+  GX_CHECK_HIP(hipMalloc(&c->red_partial, sizeof(double) * kRedBlocks * kMaxRhs));
   GX_CHECK_HIP(hipMalloc(&c->digest, sizeof(unsigned long long)));
   GX_CHECK_HIP(hipMalloc(&c->red_result, sizeof(double) * kRedSlots));
   GX_CHECK_HIP(hipHostMalloc(&c->pinned, sizeof(double) * kRedSlots, hipHostMallocDefault));
@@ -1221,7 +1222,7 @@ static int share_links(Op *op, Op *base) {
 int gx_wilson_clone(gx_op base, gx_ctx ctx, int Ls, double mass, gx_op *out) {
   GX_REQUIRE(base && ctx && out && base->kind == GX_WILSON, GX_EINVAL, "wilson_clone: needs a Wilson base operator");
   GX_REQUIRE(!base->link_owner, GX_EINVAL, "wilson_clone: base is itself a clone");
-  GX_REQUIRE(Ls >= 1 && Ls <= kMaxLs, GX_EINVAL, "wilson_clone: Ls=%d out of range", Ls);
+  GX_REQUIRE(Ls >= 1 && Ls <= kMaxRhs, GX_EINVAL, "wilson_clone: Ls=%d out of range (1..%d)", Ls, kMaxRhs);
   gx_op op = new gx_op_s();
   op->kind = GX_WILSON;
   op->ctx = ctx;

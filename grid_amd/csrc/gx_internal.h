@@ -55,7 +55,9 @@ int tune_version();
     if (r_ != GX_OK) return r_;                                                                        \
   } while (0)
 
-constexpr int kMaxLs = 16;         // fifth-dimension extent / right-hand sides per launch
+constexpr int kMaxLs = 16;         // largest extent of the templated kernels and of the block solvers' blocks
+constexpr int kMaxFifth = 128;     // largest fifth-dimension extent (Ls, or right-hand sides x Ls)
+constexpr int kMaxRhs = 64;        // right-hand sides of one multi-RHS solve
 constexpr int kRedBlocks = 1024;   // fixed grid of the deterministic reductions
 constexpr int kRedSlots = 64;      // device result slots per context
 
@@ -98,7 +100,7 @@ struct CGState : Gate {
 // As in Grid's CGmultiRHS every right-hand side iterates until the largest relative residual
 // max_s cp_s / ssq_s drops below tol^2; k and k_conv gate the kernels as in CGState.
 struct MrhsState : Gate {
-  double c[kMaxLs], cp[kMaxLs], a[kMaxLs], b[kMaxLs], d[kMaxLs], ssq[kMaxLs];
+  double c[kMaxRhs], cp[kMaxRhs], a[kMaxRhs], b[kMaxRhs], d[kMaxRhs], ssq[kMaxRhs];
   double tol2;
 };
 
@@ -106,7 +108,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[16] = {};
-  double *red_partial = nullptr;     // [kRedBlocks * kMaxLs]
+  double *red_partial = nullptr;     // [kRedBlocks * kMaxRhs]
   unsigned long long *digest = nullptr;  // field digest accumulator
   double *red_result = nullptr;      // [kRedSlots]
   double *pinned = nullptr;          // host mirror of red_result
@@ -192,18 +194,19 @@ template <typename S>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st);
 // hop with an epilogue: epi 0 out = hop; 1 out = alpha hop + beta phi; 2 out = alpha A(hop) + beta phi
-// with A a per-chirality Ls x Ls matrix (2*Ls*Ls in T) applied across each site's s-row
+// with A a per-chirality matrix applied across each site's s-row, block diagonal in groups of lb
+// consecutive slices (2*lb*lb in T; lb = 0: one group of Ls)
 template <typename T>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
-                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta);
-// out = A_chi(in) per chirality (+ phi if phi), mats = 2*Ls*Ls device matrix in T
+                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb = 0);
+// out = A_chi(in) per chirality (+ phi if phi), mats = 2*lb*lb device matrix in T (groups as above)
 template <typename T>
 int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
-                hipStream_t st);
+                hipStream_t st, int lb = 0);
 // out = A(in) + beta phi and, if out2, out2 = C(in) from one read of in
 template <typename T>
 int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, void *out, const void *mats, void *out2,
-                 const void *mats2, const Gate *gate, hipStream_t st);
+                 const void *mats2, const Gate *gate, hipStream_t st, int lb = 0);
 // z = a x + b y over nsites sites (stride sites per half; a, b real; y optional)
 template <typename S>
 int launch_axpby(int64_t nsites, int64_t stride, double a, const void *x, double b, const void *y, void *z,

@@ -284,6 +284,29 @@ __device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t nblocks, 
   return b;
 }
 
+// Half-lattice indices of the 8 neighbours (x+, y+, z+, t+, x-, y-, z-, t-) of half-site h of
+// checkerboard cb_out, with periodic wrap computed in place (no stencil table).
+__device__ __forceinline__ void site_neighbours(const KGeom &g, uint32_t h, int cb_out, uint32_t *nb) {
+  const uint32_t r1 = fdiv(h, g.fLxh), r2 = fdiv(r1, g.fL1), r3 = fdiv(r2, g.fL2);
+  const int xh = (int)(h - r1 * (uint32_t)g.Lxh);
+  const int y = (int)(r1 - r2 * (uint32_t)g.L[1]);
+  const int z = (int)(r2 - r3 * (uint32_t)g.L[2]);
+  const int t = (int)r3;
+  const int x = 2 * xh + ((cb_out + y + z + t) & 1);
+  const uint32_t base = h - xh;
+  const int xp = (x + 1 == g.L[0]) ? 0 : x + 1;
+  const int xm = (x == 0) ? g.L[0] - 1 : x - 1;
+  const uint32_t sY = (uint32_t)g.sY, sZ = (uint32_t)g.sZ, sT = (uint32_t)g.sT;
+  nb[0] = base + (xp >> 1);
+  nb[4] = base + (xm >> 1);
+  nb[1] = (y + 1 == g.L[1]) ? h - (uint32_t)(g.L[1] - 1) * sY : h + sY;
+  nb[5] = (y == 0) ? h + (uint32_t)(g.L[1] - 1) * sY : h - sY;
+  nb[2] = (z + 1 == g.L[2]) ? h - (uint32_t)(g.L[2] - 1) * sZ : h + sZ;
+  nb[6] = (z == 0) ? h + (uint32_t)(g.L[2] - 1) * sZ : h - sZ;
+  nb[3] = (t + 1 == g.L[3]) ? h - (uint32_t)(g.L[3] - 1) * sT : h + sT;
+  nb[7] = (t == 0) ? h + (uint32_t)(g.L[3] - 1) * sT : h - sT;
+}
+
 // One leg on its neighbour spinor f: project to the half-spinor, multiply by the link and
 // reconstruct, one colour row at a time (three link entries and two results live at once).
 template <typename T, typename S, int MU, bool PLUS, bool LDSU>
@@ -318,17 +341,17 @@ __device__ __forceinline__ void do_leg(cx<T> *acc, const cx<T> *f, const cx<T> *
 }
 
 template <bool DAG, int ORD, int LEG>
-__device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, int s) {
+__device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, uint32_t s) {
   return nb[LegSel<DAG, ORD, LEG>::P] * Ls + s;
 }
 
 // The 8 legs of one output site.  The caller has issued the neighbour loads of leg 0 (into fa)
 // and, with PF, of leg 1 (into fb); with PF the loads of leg L+1 are in flight while leg L is
 // computed (sched_barrier keeps the compiler from sinking them).
-template <typename T, typename S, int LS, bool DAG, bool PF, int ORD, bool LDSU>
+template <typename T, typename S, bool DAG, bool PF, int ORD, bool LDSU>
 __device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const cx<S> *__restrict__ in,
-                                         const cx<S> *__restrict__ U, const cx<T> *ul, const uint32_t *nb, int s,
-                                         uint32_t h, uint32_t H4) {
+                                         const cx<S> *__restrict__ U, const cx<T> *ul, const uint32_t *nb,
+                                         uint32_t LS, uint32_t s, uint32_t h, uint32_t H4) {
 #define IDX(L) leg_idx<DAG, ORD, L>(nb, LS, s)
 #define LEG(L, F) do_leg<T, S, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4)
   if (PF) {
@@ -416,25 +439,8 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   const bool valid = (ls < SPW) && (h0 + lsb < H4);
   // idle lanes compute in-bounds addresses and skip the work
   const uint32_t h = sweep ? sweep_site(g, valid ? h0 + lsb : h0) : (valid ? h0 + lsb : h0);
-  const uint32_t r1 = fdiv(h, g.fLxh), r2 = fdiv(r1, g.fL1), r3 = fdiv(r2, g.fL2);
-  const int xh = (int)(h - r1 * (uint32_t)g.Lxh);
-  const int y = (int)(r1 - r2 * (uint32_t)g.L[1]);
-  const int z = (int)(r2 - r3 * (uint32_t)g.L[2]);
-  const int t = (int)r3;
-  const int x = 2 * xh + ((cb_out + y + z + t) & 1);
-  const uint32_t base = h - xh;
-  const int xp = (x + 1 == g.L[0]) ? 0 : x + 1;
-  const int xm = (x == 0) ? g.L[0] - 1 : x - 1;
-  const uint32_t sY = (uint32_t)g.sY, sZ = (uint32_t)g.sZ, sT = (uint32_t)g.sT;
   uint32_t nb[8];
-  nb[0] = base + (xp >> 1);
-  nb[4] = base + (xm >> 1);
-  nb[1] = (y + 1 == g.L[1]) ? h - (uint32_t)(g.L[1] - 1) * sY : h + sY;
-  nb[5] = (y == 0) ? h + (uint32_t)(g.L[1] - 1) * sY : h - sY;
-  nb[2] = (z + 1 == g.L[2]) ? h - (uint32_t)(g.L[2] - 1) * sZ : h + sZ;
-  nb[6] = (z == 0) ? h + (uint32_t)(g.L[2] - 1) * sZ : h - sZ;
-  nb[3] = (t + 1 == g.L[3]) ? h - (uint32_t)(g.L[3] - 1) * sT : h + sT;
-  nb[7] = (t == 0) ? h + (uint32_t)(g.L[3] - 1) * sT : h - sT;
+  site_neighbours(g, h, cb_out, nb);
   // Link tile of the workgroup: global -> registers first, then the first neighbour loads, then
   // registers -> LDS and a barrier that waits for LDS traffic only, so the neighbour loads stay in
   // flight across it.
@@ -471,7 +477,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  if (valid) hop_site<T, S, LS, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, s, h, H4);
+  if (valid) hop_site<T, S, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, LS, s, h, H4);
   const uint32_t o = h * LS + s;
   if (MAT) {
     // every wave has finished reading links: reuse the LDS for the hop results [w][k][lane]
@@ -513,6 +519,201 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   store_site<S>(out, acc, o, (flags & 1) != 0);
 }
 
+// ---------------- general fifth-dimension extent ----------------
+// Tile shape of the general kernels: a workgroup of W waves covers EPB = 64 W consecutive positions
+// of the site order times LS slices, i.e. SPB = EPB / LS whole sites, and lane t of the workgroup
+// works on slice t % LS of tile site t / LS.  With the linear tile maps every wave reads whole
+// aligned 64-entry storage blocks and no lane idles, for any LS with W a multiple of
+// LS / gcd(LS, 64).  LS, W and the group LB of the fifth-dimension matrices (block diagonal in
+// groups of LB consecutive slices; LB = LS for one Mobius operator) are run-time values.  Links
+// are staged per workgroup in LDS (LS >= 8: at most 9 link entries per thread).
+struct GShape {
+  uint32_t LS, LB, SPB;
+  FDiv fLS, fLB;
+};
+constexpr int kGenMaxWaves = 8;
+constexpr int kGenMinLs = 8;
+constexpr int kGenLinkPerThread = (72 + kGenMinLs - 1) / kGenMinLs;
+
+// the matrix epilogue across the s-rows of the workgroup's sites, one chirality at a time: the 6
+// components of chirality chi of acc are staged in LDS (6 columns of EPB), then acc[6 chi + k] =
+// sum over the slices tt of this lane's group of A_chi[s_g][tt] * staged[k][site, group + tt]
+template <typename T>
+__device__ __forceinline__ void group_matrix(cx<T> *acc, cx<T> *stg, const T *mat, const GShape &sh, uint32_t t,
+                                             uint32_t hl, uint32_t s, uint32_t epb) {
+  const uint32_t sg = s - fdiv(s, sh.fLB) * sh.LB;   // slice within the group
+  const uint32_t row = hl * sh.LS + (s - sg);        // first slice of the group in the tile
+#pragma unroll
+  for (int chi = 0; chi < 2; chi++) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; k++) stg[k * epb + t] = acc[6 * chi + k];
+    __syncthreads();
+    cx<T> r[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) r[k] = {0, 0};
+    const T *a = mat + (size_t)chi * sh.LB * sh.LB + (size_t)sg * sh.LB;
+    for (uint32_t tt = 0; tt < sh.LB; tt++) {
+      const T c = a[tt];
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        const cx<T> v = stg[k * epb + row + tt];
+        r[k].r = fma(c, v.r, r[k].r);
+        r[k].i = fma(c, v.i, r[k].i);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) acc[6 * chi + k] = r[k];
+  }
+}
+
+// The hop of k_dhop (same legs, projectors, link staging and epilogues) in the general tile shape.
+template <typename S, bool DAG, int EPI>
+__global__ void __launch_bounds__(64 * kGenMaxWaves)
+    k_dhop_g(const cx<S> *__restrict__ in, cx<S> *__restrict__ out, const cx<S> *__restrict__ U, KGeom g, int cb_out,
+             const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags, HopEpi<S> ep, GShape sh) {
+  typedef typename Sto<S>::C T;
+  if (gated(gate)) return;
+  constexpr bool MAT = EPI >= 2, PHI = (EPI & 1) != 0;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(lds_raw);
+  const bool sweep = map == 4;
+  const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, sweep ? 1 : map);
+  if (tile >= ntiles) return;  // uniform across the workgroup
+  const uint32_t H4 = (uint32_t)g.H4, LS = sh.LS, epb = blockDim.x;
+  const uint32_t t = threadIdx.x;
+  const uint32_t hl = fdiv(t, sh.fLS), s = t - hl * LS;
+  const uint32_t p0 = tile * sh.SPB;
+  const bool valid = p0 + hl < H4;
+  // lanes past the lattice take the tile's first site: in-bounds addresses, work skipped
+  const uint32_t pos = valid ? p0 + hl : p0;
+  const uint32_t h = sweep ? sweep_site(g, pos) : pos;
+  uint32_t nb[8];
+  site_neighbours(g, h, cb_out, nb);
+  // links of tile site hl: thread (hl, s) loads entries s, s + LS, ... (< 72)
+  cx<T> ureg[kGenLinkPerThread];
+#pragma unroll
+  for (int j = 0; j < kGenLinkPerThread; j++) {
+    const uint32_t pk = s + LS * j;
+    ureg[j] = pk < 72 ? ldc<S>(U, pk * H4 + h) : cx<T>{0, 0};
+  }
+  cx<T> fa[12], fb[12];
+  load_site<S>(fa, in, leg_idx<DAG, 0, 0>(nb, LS, s));
+  load_site<S>(fb, in, leg_idx<DAG, 0, 1>(nb, LS, s));
+#pragma unroll
+  for (int j = 0; j < kGenLinkPerThread; j++) {
+    const uint32_t pk = s + LS * j;
+    if (pk < 72) lds[hl * kLinkPad + pk] = ureg[j];
+  }
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  cx<T> acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc[k] = {0, 0};
+  if (valid) hop_site<T, S, DAG, true, 0, true>(acc, fa, fb, in, U, lds + hl * kLinkPad, nb, LS, s, h, H4);
+  if (MAT) group_matrix<T>(acc, lds, ep.mat, sh, t, hl, s, epb);
+  if (!valid) return;
+  const uint32_t o = h * LS + s;
+  if (EPI != 0) {
+    const cx<S> *__restrict__ qp = PHI ? ep.phi + bidx<S>(o, 0) : ep.phi;
+    const T bsc = PHI ? ep.beta * site_scale<S>(qp, 0u) : T(0);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      cx<T> v = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
+      if (PHI) {
+        const cx<T> p = ldc<S>(qp, k * kBlk);
+        v.r = fma(bsc, p.r, v.r);
+        v.i = fma(bsc, p.i, v.i);
+      }
+      acc[k] = v;
+    }
+  }
+  store_site<S>(out, acc, o, (flags & 1) != 0);
+}
+
+// k_smat in the general tile shape: out = A(in) (+ beta phi) and optionally out2 = C(in), with A and
+// C block diagonal in groups of LB slices.
+template <typename S, bool OUT2, bool PHI>
+__global__ void __launch_bounds__(64 * kGenMaxWaves)
+    k_smat_g(const cx<S> *__restrict__ in, const cx<S> *__restrict__ phi, typename Sto<S>::C beta,
+             cx<S> *__restrict__ out, const typename Sto<S>::C *__restrict__ mats, cx<S> *__restrict__ out2,
+             const typename Sto<S>::C *__restrict__ mats2, uint32_t H4, const Gate *__restrict__ gate, GShape sh) {
+  typedef typename Sto<S>::C T;
+  if (gated(gate)) return;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(lds_raw);
+  const uint32_t LS = sh.LS, epb = blockDim.x, t = threadIdx.x;
+  const uint32_t hl = fdiv(t, sh.fLS), s = t - hl * LS;
+  const uint32_t h = blockIdx.x * sh.SPB + hl;
+  const bool valid = h < H4;
+  const uint32_t o = h * LS + s;
+  cx<T> f[12], f2[12];
+  if (valid) {
+    load_site<S>(f, in, o);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) f[k] = {0, 0};
+  }
+  if (OUT2)
+#pragma unroll
+    for (int k = 0; k < 12; k++) f2[k] = f[k];
+  group_matrix<T>(f, lds, mats, sh, t, hl, s, epb);
+  if (OUT2) group_matrix<T>(f2, lds, mats2, sh, t, hl, s, epb);
+  if (!valid) return;
+  if (PHI) {
+    const cx<S> *__restrict__ qp = phi + bidx<S>(o, 0);
+    const T bsc = beta * site_scale<S>(qp, 0u);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      const cx<T> p = ldc<S>(qp, k * kBlk);
+      f[k].r = fma(bsc, p.r, f[k].r);
+      f[k].i = fma(bsc, p.i, f[k].i);
+    }
+  }
+  store_site<S>(out, f, o, false);
+  if (OUT2) store_site<S>(out2, f2, o, false);
+}
+
+// Waves per workgroup of the general kernels for extent LS (0: the templated kernels, -1: no
+// shape fits): a multiple of LS / gcd(LS, 64), at most kGenMaxWaves, with the link tile (and the
+// matrix staging) within 64 KB of LDS.  knob > 0 asks for that many waves when it is a valid
+// shape; knob 0 keeps the templated kernels for LS <= 16 and otherwise takes the smallest valid
+// shape of at least 4 waves (or the largest below).
+static int general_waves(int LS, int knob, size_t csz, bool mat, int lb) {
+  int gcd = 64, a = LS;
+  while (a) {
+    const int r = gcd % a;
+    gcd = a;
+    a = r;
+  }
+  const int w0 = LS / gcd;
+  auto fits = [&](int w) {
+    const size_t spb = (size_t)64 * w / LS;
+    const size_t links = spb * kLinkPad * csz, stage = mat ? (size_t)6 * 64 * w * csz : 0;
+    return w % w0 == 0 && w <= kGenMaxWaves && std::max(links, stage) <= 65536;
+  };
+  const bool templated_ok = LS <= kMaxLs && lb == LS;
+  if (LS < kGenMinLs) return templated_ok ? 0 : -1;
+  if (knob > 0 && fits(knob)) return knob;
+  if (knob == 0 && templated_ok) return 0;
+  int best = -1;
+  for (int w = w0; w <= kGenMaxWaves; w += w0)
+    if (fits(w)) {
+      best = w;
+      if (w >= 4) break;
+    }
+  return best < 0 && templated_ok ? 0 : best;
+}
+
+static GShape make_shape(int LS, int LB, int W) {
+  GShape sh;
+  sh.LS = (uint32_t)LS;
+  sh.LB = (uint32_t)LB;
+  sh.SPB = (uint32_t)(64 * W / LS);
+  sh.fLS = make_fdiv((uint32_t)LS);
+  sh.fLB = make_fdiv((uint32_t)LB);
+  return sh;
+}
+
 #ifndef GX_LS_CASES
 #define GX_LS_CASES(M) \
   M(1) M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12) M(13) M(14) M(15) M(16)
@@ -522,8 +723,9 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
 // given] (A: mat)
 template <typename S>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
-                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta) {
+                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb) {
   typedef typename Sto<S>::C T;
+  if (lb <= 0) lb = g.Ls;
   if (13 * 2 * g.half_pad() >= (int64_t)1 << 31) {
     set_error("dhop: lattice too large for 32-bit site indexing");
     return GX_EINVAL;
@@ -559,6 +761,33 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   cx<S> *pout = (cx<S> *)out;
   const cx<S> *pU = (const cx<S> *)U;
   HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta};
+  const int W = general_waves(g.Ls, tune_get(GX_TUNE_DHOP_SHAPE), sizeof(cx<T>), kepi >= 2, lb);
+  if (W < 0 || g.Ls % lb != 0) {
+    set_error("dhop: no tile shape for Ls=%d (matrix group %d): Ls > 16 needs an odd part of at most 7", g.Ls, lb);
+    return GX_EINVAL;
+  }
+  if (W > 0) {
+    const GShape sh = make_shape(g.Ls, lb, W);
+    const uint32_t ntiles = (uint32_t)((g.H4 + sh.SPB - 1) / sh.SPB);
+    const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
+    const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
+    const size_t lds = std::max((size_t)sh.SPB * kLinkPad, kepi >= 2 ? (size_t)6 * 64 * W : (size_t)0) * sizeof(cx<T>);
+#define GX_DHOPG_GO(DAG, EPI)                                                                                        \
+  hipLaunchKernelGGL((k_dhop_g<S, DAG, EPI>), dim3(blocks), dim3(64 * W), lds, st, pin, pout, pU, k, cb_out, gate,   \
+                     ntiles, map, flags, ep, sh)
+#define GX_DHOPG_EPI(EPI) \
+  if (dag) GX_DHOPG_GO(true, EPI); else GX_DHOPG_GO(false, EPI)
+    switch (kepi) {
+      case 0: GX_DHOPG_EPI(0); break;
+      case 1: GX_DHOPG_EPI(1); break;
+      case 2: GX_DHOPG_EPI(2); break;
+      default: GX_DHOPG_EPI(3); break;
+    }
+#undef GX_DHOPG_EPI
+#undef GX_DHOPG_GO
+    GX_CHECK_HIP(hipGetLastError());
+    return GX_OK;
+  }
   const int spb = 4 * (64 / g.Ls);
   const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
   const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
@@ -610,7 +839,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
 template <typename S>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st) {
-  return launch_dhop_epi<S>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0);
+  return launch_dhop_epi<S>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0, 0);
 }
 
 // out_s = sum_t A_chi[s][t] in_t (+ beta phi_s with PHI), and with OUT2 also
@@ -690,8 +919,35 @@ __global__ void __launch_bounds__(256) k_smat(const cx<S> *__restrict__ in, cons
 
 template <typename S>
 int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, void *out, const void *mats, void *out2,
-                 const void *mats2, const Gate *gate, hipStream_t st) {
+                 const void *mats2, const Gate *gate, hipStream_t st, int lb) {
   typedef typename Sto<S>::C T;
+  if (lb <= 0) lb = g.Ls;
+  if (g.Ls > kMaxLs || lb != g.Ls) {
+    const int W = general_waves(g.Ls, 0, sizeof(cx<T>), true, lb);
+    if (W <= 0 || g.Ls % lb != 0) {
+      set_error("smat: no tile shape for Ls=%d (matrix group %d)", g.Ls, lb);
+      return GX_EINVAL;
+    }
+    const GShape sh = make_shape(g.Ls, lb, W);
+    const uint32_t blocks = (uint32_t)((g.H4 + sh.SPB - 1) / sh.SPB);
+    const size_t lds = (size_t)6 * 64 * W * sizeof(cx<T>);
+    const cx<S> *pin = (const cx<S> *)in, *pphi = (const cx<S> *)phi;
+    cx<S> *pout = (cx<S> *)out, *pout2 = (cx<S> *)out2;
+    const T *pm = (const T *)mats, *pm2 = (const T *)mats2;
+#define GX_SMATG_GO(OUT2, PHI)                                                                                       \
+  hipLaunchKernelGGL((k_smat_g<S, OUT2, PHI>), dim3(blocks), dim3(64 * W), lds, st, pin, pphi, (T)beta, pout, pm,   \
+                     pout2, pm2, (uint32_t)g.H4, gate, sh)
+    if (out2) {
+      if (phi) GX_SMATG_GO(true, true);
+      else GX_SMATG_GO(true, false);
+    } else {
+      if (phi) GX_SMATG_GO(false, true);
+      else GX_SMATG_GO(false, false);
+    }
+#undef GX_SMATG_GO
+    GX_CHECK_HIP(hipGetLastError());
+    return GX_OK;
+  }
   const int spw = 64 / g.Ls;
   const int64_t waves = (g.H4 + spw - 1) / spw;
   const int64_t blocks = (waves * 64 + 255) / 256;
@@ -727,8 +983,8 @@ int launch_smat2(const Geom &g, const void *in, const void *phi, double beta, vo
 
 template <typename S>
 int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
-                hipStream_t st) {
-  return launch_smat2<S>(g, in, phi, 1.0, out, mats, nullptr, nullptr, gate, st);
+                hipStream_t st, int lb) {
+  return launch_smat2<S>(g, in, phi, 1.0, out, mats, nullptr, nullptr, gate, st, lb);
 }
 
 // ---------------- BLAS1 ----------------
@@ -1194,7 +1450,7 @@ static __global__ void __launch_bounds__(256) k_mrhs_fin(const double *partial, 
     if (s < Ls) out[s] = v;
     return;
   }
-  __shared__ double rel[64];
+  __shared__ double rel[kMaxRhs];
   if (which == 0) {
     if (s < Ls) {
       st->d[s] = v;
@@ -1909,13 +2165,13 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
 
 #define GX_INST(S)                                                                                                 \
   template int launch_dhop_epi<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,       \
-                                  hipStream_t, int, const void *, const void *, double, double);                  \
+                                  hipStream_t, int, const void *, const void *, double, double, int);             \
   template int launch_smat2<S>(const Geom &, const void *, const void *, double, void *, const void *, void *,     \
-                               const void *, const Gate *, hipStream_t);                                         \
+                               const void *, const Gate *, hipStream_t, int);                                    \
   template int launch_dhop<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,           \
                               hipStream_t);                                                                        \
   template int launch_smat<S>(const Geom &, const void *, const void *, void *, const void *, const Gate *,        \
-                              hipStream_t);                                                                        \
+                              hipStream_t, int);                                                                   \
   template int launch_axpby<S>(int64_t, int64_t, double, const void *, double, const void *, void *, const Gate *, \
                                hipStream_t);                                                                       \
   template int launch_reduce<S>(Ctx *, int, int64_t, int64_t, const void *, const void *, void *, double, int);   \

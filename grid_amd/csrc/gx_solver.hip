@@ -119,8 +119,8 @@ static int solver_work(Op *op, SolverWork **out) {
     if (e == hipSuccess) e = hipHostMalloc(&c.host, 2 * sizeof(CGState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m.dev, sizeof(MrhsState));
     if (e == hipSuccess) e = hipHostMalloc(&m.host, 2 * sizeof(MrhsState), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&m.out_dev, sizeof(double) * kMaxLs);
-    if (e == hipSuccess) e = hipHostMalloc(&m.out_host, sizeof(double) * kMaxLs, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&m.out_dev, sizeof(double) * kMaxRhs);
+    if (e == hipSuccess) e = hipHostMalloc(&m.out_host, sizeof(double) * kMaxRhs, hipHostMallocDefault);
     for (int i = 0; i < 2 && e == hipSuccess; i++) {
       e = hipEventCreateWithFlags(&c.ev[i], hipEventDisableTiming);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&m.ev[i], hipEventDisableTiming);
@@ -530,6 +530,10 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
                 const CGOpts &opt = CGOpts()) {
   auto t0 = std::chrono::steady_clock::now();
   const int Ls = op->g.Ls, prec = src->prec;
+  if (Ls > kMaxRhs) {
+    set_error("multi-RHS CG: %d right-hand sides (at most %d)", Ls, kMaxRhs);
+    return GX_EINVAL;
+  }
   GX_TRY(op_prepare(op, prec));
   Field *p = op_tmp(op, prec, 4), *mmp = op_tmp(op, prec, 5), *r = op_tmp(op, prec, 6);
   if (!p || !mmp || !r) return GX_ENOMEM;

@@ -372,6 +372,9 @@ enum gx_tune_knob {
   GX_TUNE_DHOP_ORDER = 3,   /* 0: Grid's leg order, 1: the two legs of each direction adjacent */
   GX_TUNE_CG_GRAPH = 4,     /* 1: solver iteration batches are captured once and replayed as hipGraphs */
   GX_TUNE_FUSED = 5,        /* 1: Schur operator with fused hop epilogues (2-3 kernels per Mpc) */
+  GX_TUNE_DHOP_SHAPE = 6,   /* hop tile shape: 0 = whole sites per wave, 4 waves (Ls <= 16); W > 0 = W waves of
+                               64 consecutive (site, s) entries each, whole sites per workgroup (any Ls whose odd
+                               part is at most 7, W a multiple of Ls / gcd(Ls, 64), W <= 8) */
   GX_TUNE_COUNT = 8
 };
 int gx_tune_set(int knob, int value);

@@ -56,3 +56,22 @@ def test_knobs_context_restores():
     with at.Knobs(((gx.TUNE_DHOP_NT, 1 - before),)):
         assert gx.tune_get(gx.TUNE_DHOP_NT) == 1 - before
     assert gx.tune_get(gx.TUNE_DHOP_NT) == before
+
+
+def test_shape_candidates_follow_the_library_rule():
+    """Tile shapes: W a multiple of Ls / gcd(Ls, 64), W <= 8, link tile within 64 KB of LDS, and the
+    sites-per-wave shape (0) only for Ls <= 16."""
+    assert at.shape_candidates(4) == [0]                      # Ls < 8: templated kernel only
+    assert at.shape_candidates(16, gx.F64) == [0, 1, 2, 4, 8]
+    assert at.shape_candidates(8, gx.F64) == [0, 1, 2, 4]         # 8 waves: a 74 KB link tile
+    assert at.shape_candidates(8, gx.F32) == [0, 1, 2, 4, 8]
+    assert at.shape_candidates(12, gx.F32) == [0, 3, 6]
+    assert at.shape_candidates(128) == [2, 4, 8]
+    assert at.shape_candidates(24, gx.F64) == [3, 6]
+    assert at.shape_candidates(20) == [5]
+    assert at.shape_candidates(96) == [3, 6]
+    assert at.shape_candidates(17) == []                      # odd part 17: no shape
+    for Ls in (8, 12, 16, 24, 32, 48, 64):
+        for w in at.shape_candidates(Ls):
+            if w:
+                assert (64 * w) % Ls == 0 and w <= 8

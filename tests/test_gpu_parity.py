@@ -413,3 +413,130 @@ def test_tile_maps_bitwise(ctx, kind, dims, Ls, prec):
         assert np.array_equal(outs[m], outs[0]), m
     O = Oracle(U, dims, Ls, kind, mass=0.1)
     assert max_site_rel_err(outs[0], O.dhop(half(psi, dims, ODD), EVEN, 0)) < {gx.F64: 1e-13, gx.F32: 1e-6, gx.F16: 5e-3}[prec]
+
+
+class _Knobs:
+    def __init__(self, **kw):
+        self.kw = {getattr(gx, 'TUNE_' + k.upper()): v for k, v in kw.items()}
+
+    def __enter__(self):
+        self.old = {k: gx.tune_get(k) for k in self.kw}
+        for k, v in self.kw.items():
+            gx.tune_set(k, v)
+
+    def __exit__(self, *a):
+        for k, v in self.old.items():
+            gx.tune_set(k, v)
+
+
+@pytest.mark.parametrize('Ls,dims', [(24, (8, 4, 4, 8)), (20, (4, 4, 4, 8)), (32, (4, 4, 2, 8)), (48, (4, 2, 2, 4))])
+@pytest.mark.parametrize('prec', [gx.F64, gx.F32])
+def test_mobius_ops_large_ls(ctx, Ls, dims, prec):
+    """Mobius operators with Ls > 16 (Grid runs physical-point DWF at Ls = 24,
+    tests/Test_dwf_mixedcg_prec_halfcomms.cc:43) through the general-extent kernels, against the oracle."""
+    U, psi = inputs(dims, Ls, 27, eps=0.3)
+    kw = dict(mass=0.01, M5=1.8, b=1.5, c=0.5)
+    O = Oracle(U, dims, Ls, 'mobius', **kw)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    tol = TOL[prec]
+    fi, fo = D.field(prec).upload(psi), D.field(prec)
+    D.M(fi, fo)
+    assert max_site_rel_err(fo.download(), O.apply('M', psi)) < tol
+    D.Mdag(fi, fo)
+    assert max_site_rel_err(fo.download(), O.apply('Mdag', psi)) < tol
+    po, pe = half(psi, dims, ODD), half(psi, dims, EVEN)
+    for dag in (0, 1):
+        out = run_half(D, prec, lambda i, o: D.DhopEO(i, o, dag), po, ODD)
+        assert max_site_rel_err(out, O.dhop(po, EVEN, dag)) < tol
+    assert max_site_rel_err(run_half(D, prec, D.Meooe, po, ODD), O.apply('Meooe', po, ODD)) < tol
+    assert max_site_rel_err(run_half(D, prec, D.MeooeDag, po, ODD), O.apply('MeooeDag', po, ODD)) < tol
+    for name in ['Mooee', 'MooeeDag', 'MooeeInv', 'MooeeInvDag', 'Meooe5D', 'MeooeDag5D']:
+        i = D.field(prec, EVEN).upload(pe)
+        o = D.field(prec, EVEN)
+        getattr(D, name)(i, o)
+        assert max_site_rel_err(o.download(), O.apply(name, pe, EVEN)) < tol, name
+    for fused in (1, 0):
+        with _Knobs(fused=fused):
+            for name in ['Mpc', 'MpcDag', 'MpcDagMpc']:
+                i = D.field(prec, ODD).upload(po)
+                o = D.field(prec, ODD)
+                getattr(D, name)(i, o)
+                assert max_site_rel_err(o.download(), O.schur(name, po)) < tol * 10, (name, fused)
+    D.close()
+
+
+def test_mobius_ls24_red_black_solves(ctx):
+    """Red-black solves at Ls = 24 (FP64 CG, FP32 and FP16-storage defect correction) reach the
+    oracle's solution."""
+    dims, Ls = (4, 4, 4, 8), 24
+    kw = dict(mass=0.01, M5=1.8, b=1.5, c=0.5)
+    U = gx.random_gauge(dims, 61, 0.3)
+    src = gx.gaussian_fermion(dims, Ls, 62)
+    O = Oracle(U, dims, Ls, 'mobius', **kw)
+    ref, rst = O.schur_solve(src, tol=1e-12, maxit=20000)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    fs = D.field().upload(src)
+    for mixed in (0, 1, 2):
+        fx = D.field()
+        st = D.schur_solve(fs, fx, tol=1e-12, maxit=20000, mixed=mixed)
+        assert st['converged'] and st['unprec_residual'] < 1e-10, (mixed, st)
+        assert rel_err(fx.download(), ref) < 1e-8, mixed
+        if mixed == 0:
+            assert abs(st['iterations'] - rst['iterations']) <= 2, (st, rst)
+    D.close()
+
+
+@pytest.mark.parametrize('kind,Ls,dims', [('mobius', 8, (8, 4, 4, 8)), ('mobius', 12, (4, 6, 4, 8)),
+                                          ('mobius', 16, (8, 4, 4, 8)), ('wilson', 12, (8, 4, 6, 8)),
+                                          ('wilson', 16, (8, 4, 4, 8)), ('wilson', 8, (8, 8, 4, 4))])
+@pytest.mark.parametrize('prec', [gx.F64, gx.F32, gx.F16])
+def test_tile_shapes_bitwise(ctx, kind, Ls, dims, prec):
+    """Every tile shape (whole sites per wave, or W waves of 64 consecutive entries) and tile map
+    computes the same legs in the same order: DhopEO, Mpc and MpcDag are bitwise equal across
+    shapes."""
+    from grid_amd.autotune import shape_candidates
+    U, psi = inputs(dims, Ls, 29, eps=0.3)
+    kw = dict(mass=0.05, M5=1.8, b=1.5, c=0.5)
+    D = gx.FermionOp(ctx, U, dims, Ls, kind, **kw)
+    po = half(psi, dims, ODD)
+    fi = D.field(prec, ODD).upload(po)
+    ref = None
+    for w in shape_candidates(Ls, prec):
+        for tile_map in (0, 4):
+            with _Knobs(dhop_shape=w, dhop_map=tile_map, dhop_order=0, fused=1):
+                outs = []
+                for fn in (lambda o: D.DhopEO(fi, o, 0), lambda o: D.DhopEO(fi, o, 1), lambda o: D.Mpc(fi, o),
+                           lambda o: D.MpcDag(fi, o)):
+                    fo = D.field(prec, EVEN if len(outs) < 2 else ODD)
+                    fn(fo)
+                    outs.append(fo.download())
+            if ref is None:
+                ref = outs
+            else:
+                for a, b in zip(outs, ref):
+                    assert np.array_equal(a, b), (w, tile_map)
+    D.close()
+
+
+@pytest.mark.parametrize('nrhs', [24, 48])
+def test_multi_rhs_wilson_large(ctx, nrhs):
+    """A multi-RHS Wilson operator with more than 16 right-hand sides: the hop of every right-hand
+    side equals the oracle's single-RHS hop, and the mixed-precision multi-RHS red-black solve
+    brings every right-hand side to the tolerance."""
+    dims, mass = (8, 4, 4, 8), 0.1
+    U, psi = inputs(dims, nrhs, 33)
+    O1 = Oracle(U, dims, 1, 'wilson', mass=mass)
+    D = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+    po = half(psi, dims, ODD)
+    out = run_half(D, gx.F64, lambda i, o: D.DhopEO(i, o, 0), po, ODD)
+    for k in range(nrhs):
+        ref = O1.dhop(np.ascontiguousarray(po[..., k:k + 1, :, :]), EVEN, 0)
+        assert max_site_rel_err(out[..., k:k + 1, :, :], ref) < 1e-13
+    fs, fx = D.field().upload(psi), D.field()
+    st = D.schur_solve(fs, fx, tol=1e-11, maxit=5000, mixed=1)
+    assert st['converged'] and st['unprec_residual'] < 1e-10, st
+    x = fx.download()
+    for k in (0, nrhs // 2, nrhs - 1):
+        Mx = O1.apply('M', np.ascontiguousarray(x[..., k:k + 1, :, :]))
+        assert rel_err(Mx, psi[..., k:k + 1, :, :]) < 1e-10, k
+    D.close()
