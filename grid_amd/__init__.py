@@ -56,6 +56,22 @@ class NerscInfo(ctypes.Structure):
         return d
 
 
+class IldgInfo(ctypes.Structure):
+    _fields_ = [('dims', ctypes.c_int * 4), ('precision', ctypes.c_int),
+                ('scidac_checksuma', ctypes.c_uint32), ('scidac_checksumb', ctypes.c_uint32),
+                ('scidac_checksuma_file', ctypes.c_uint32), ('scidac_checksumb_file', ctypes.c_uint32),
+                ('has_plaquette', ctypes.c_int), ('plaquette_file', ctypes.c_double),
+                ('link_trace_file', ctypes.c_double), ('plaquette', ctypes.c_double), ('link_trace', ctypes.c_double),
+                ('has_norm', ctypes.c_int), ('norm2_file', ctypes.c_double), ('norm2', ctypes.c_double),
+                ('lfn', ctypes.c_char * 256)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d['dims'] = tuple(self.dims)
+        d['lfn'] = d['lfn'].decode()
+        return d
+
+
 class Request(ctypes.Structure):
     _fields_ = [('dims', ctypes.c_int * 4), ('nsrc', ctypes.c_int), ('mass', ctypes.c_double),
                 ('gauge', ctypes.c_int), ('seed', ctypes.c_uint64), ('arrival', ctypes.c_double)]
@@ -148,12 +164,15 @@ _SIG = {
     'gx_block_cgrq': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_mixed_cg_batched': ([_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), _D, _D, _I, _I, _I, _I,
                              ctypes.POINTER(SolverStats)], _I),
+    'gx_slice_inner_product_matrix_noalloc': ([_P, _P, _DP], _I),
     'gx_inner_product_matrix': ([_I, ctypes.POINTER(_P), ctypes.POINTER(_P), _DP], _I),
     'gx_madd_matrix': ([_I, ctypes.POINTER(_P), _DP, ctypes.POINTER(_P), ctypes.POINTER(_P), _D], _I),
     'gx_block_cgrq_vec': ([_P, _I, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), _D, _I, ctypes.POINTER(SolverStats)],
                           _I),
     'gx_gauge_statistics': ([_P, _IP, _DP, _DP, _DP], _I),
     'gx_nersc_read': ([_P, ctypes.c_char_p, _IP, _DP, ctypes.POINTER(NerscInfo)], _I),
+    'gx_ildg_read': ([_P, ctypes.c_char_p, _IP, _DP, ctypes.POINTER(IldgInfo)], _I),
+    'gx_ildg_write': ([_P, ctypes.c_char_p, _IP, _DP, _I, ctypes.c_char_p, ctypes.c_char_p], _I),
     'gx_nersc_write': ([_P, ctypes.c_char_p, _IP, _DP, _I, ctypes.c_char_p, ctypes.c_char_p, _I], _I),
     'gx_cg_multishift': ([_P, _P, _I, _DP, _DP, ctypes.POINTER(_P), _I, ctypes.POINTER(SolverStats)], _I),
     'gx_slice_inner_product_matrix': ([_P, _P, _DP], _I),
@@ -333,7 +352,8 @@ def field_copy(dst, src):
     _check(lib().gx_field_copy(dst.h, src.h))
 
 
-TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH, TUNE_FUSED, TUNE_DHOP_SHAPE = 0, 1, 2, 3, 4, 5, 6
+(TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH, TUNE_FUSED, TUNE_DHOP_SHAPE,
+ TUNE_DHOP_PERSIST) = 0, 1, 2, 3, 4, 5, 6, 7
 
 
 def tune_set(knob, value):
@@ -377,6 +397,27 @@ def nersc_write(ctx, path, U, dims, two_row=False, ensemble_label='DWF', ensembl
     U = np.ascontiguousarray(U, dtype=np.complex128)
     _check(lib().gx_nersc_write(ctx.h, os.fsencode(path), _dims(dims), _dp(U), int(two_row),
                                 ensemble_label.encode(), ensemble_id.encode(), sequence_number))
+
+
+def ildg_read(ctx, path, dims, check=True):
+    """IldgReader::readConfiguration: returns (U, info).  With check=False a failed check returns
+    (U, info) with info['ok'] = False instead of raising."""
+    lx, ly, lz, lt = dims
+    U = np.zeros((lt, lz, ly, lx, 4, 3, 3), dtype=np.complex128)
+    info = IldgInfo()
+    rc = lib().gx_ildg_read(ctx.h, os.fsencode(path), _dims(dims), _dp(U), ctypes.byref(info))
+    if check:
+        _check(rc)
+    d = info.as_dict()
+    d['ok'] = rc == 0
+    return U, d
+
+
+def ildg_write(ctx, path, U, dims, precision=64, lfn='', description=''):
+    """IldgWriter::writeConfiguration with big-endian words of `precision` bits."""
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    _check(lib().gx_ildg_write(ctx.h, os.fsencode(path), _dims(dims), _dp(U), int(precision), lfn.encode(),
+                               description.encode()))
 
 
 def slice_inner_product_matrix(lhs, rhs):

@@ -28,7 +28,7 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
-static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0};  // map, pf, nt, order, graph, fused, shape
+static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0};  // map, pf, nt, order, graph, fused, shape, persist
 static int g_tune_version = 0;
 int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
 int tune_version() { return g_tune_version; }
@@ -232,17 +232,18 @@ int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const Gate *gat
   });
 }
 
+// Precisions store a site's components in different groupings (16 bytes per group), so every
+// conversion between precisions works site by site.
 int convert_field(Field *out, Field *in) {
-  const int64_t n = in->stored() * 2;  // reals, padding included
   out->cb = in->cb;
   if (in->prec == out->prec) {
     GX_CHECK_HIP(hipMemcpyAsync(out->data, in->data, in->bytes, hipMemcpyDeviceToDevice, in->ctx->stream));
     return GX_OK;
   }
-  if (in->prec == GX_F64 && out->prec == GX_F32) return launch_convert<double, float>(n, in->data, out->data, in->ctx->stream);
-  if (in->prec == GX_F32 && out->prec == GX_F64) return launch_convert<float, double>(n, in->data, out->data, in->ctx->stream);
   const int64_t ns = nsites_of(in), st = in->g.half_pad();
   hipStream_t s = in->ctx->stream;
+  if (in->prec == GX_F64 && out->prec == GX_F32) return launch_convert_site<double, float>(ns, st, in->data, out->data, s);
+  if (in->prec == GX_F32 && out->prec == GX_F64) return launch_convert_site<float, double>(ns, st, in->data, out->data, s);
   if (in->prec == GX_F64 && out->prec == GX_F16) return launch_convert_site<double, half_t>(ns, st, in->data, out->data, s);
   if (in->prec == GX_F16 && out->prec == GX_F64) return launch_convert_site<half_t, double>(ns, st, in->data, out->data, s);
   if (in->prec == GX_F32 && out->prec == GX_F16) return launch_convert_site<float, half_t>(ns, st, in->data, out->data, s);
@@ -667,9 +668,16 @@ int apply_mpcdagmpc(Op *op, Field *in, Field *out, const Gate *gate) {
 }
 
 // host (lexicographic, s innermost, AoS) <-> device (EO split, 64-entry blocks) index maps; the
-// stage holds 12 columns per half (half_pad entries each, padding zero)
+// stage holds 12 columns per half (half_pad entries each, padding zero) in the grouping of the
+// storage precision with the stage's type (FP64: 1 component per 16 bytes, FP32: 2); the FP16
+// packing reads an FP32-grouped stage
+template <typename T>
+static int vw_of() {
+  return 16 / (int)(2 * sizeof(T));
+}
 template <typename T>
 static void gather_to_device_layout(const Field *f, const double *host, T *stage) {
+  const int vw = vw_of<T>();
   const Geom &g = f->g;
   const int Ls = g.Ls;
   const int64_t hp = 12 * g.half_pad();
@@ -683,7 +691,7 @@ static void gather_to_device_layout(const Field *f, const double *host, T *stage
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             const double *src = host + ((s4 * Ls + s) * 12 + k) * 2;
-            T *dst = stage + (cb * hp + blk_index(h * Ls + s, k, 12)) * 2;
+            T *dst = stage + (cb * hp + blk_index(h * Ls + s, k, 12, vw)) * 2;
             dst[0] = (T)src[0];
             dst[1] = (T)src[1];
           }
@@ -695,7 +703,7 @@ static void gather_to_device_layout(const Field *f, const double *host, T *stage
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             const double *src = host + ((h * Ls + s) * 12 + k) * 2;
-            T *dst = stage + blk_index(h * Ls + s, k, 12) * 2;
+            T *dst = stage + blk_index(h * Ls + s, k, 12, vw) * 2;
             dst[0] = (T)src[0];
             dst[1] = (T)src[1];
           }
@@ -705,6 +713,7 @@ static void gather_to_device_layout(const Field *f, const double *host, T *stage
 
 template <typename T>
 static void scatter_from_device_layout(const Field *f, const T *stage, double *host) {
+  const int vw = vw_of<T>();
   const Geom &g = f->g;
   const int Ls = g.Ls;
   const int64_t hp = 12 * g.half_pad();
@@ -718,7 +727,7 @@ static void scatter_from_device_layout(const Field *f, const T *stage, double *h
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             double *dst = host + ((s4 * Ls + s) * 12 + k) * 2;
-            const T *src = stage + (cb * hp + blk_index(h * Ls + s, k, 12)) * 2;
+            const T *src = stage + (cb * hp + blk_index(h * Ls + s, k, 12, vw)) * 2;
             dst[0] = (double)src[0];
             dst[1] = (double)src[1];
           }
@@ -730,7 +739,7 @@ static void scatter_from_device_layout(const Field *f, const T *stage, double *h
         for (int s = 0; s < Ls; s++)
           for (int k = 0; k < 12; k++) {
             double *dst = host + ((h * Ls + s) * 12 + k) * 2;
-            const T *src = stage + blk_index(h * Ls + s, k, 12) * 2;
+            const T *src = stage + blk_index(h * Ls + s, k, 12, vw) * 2;
             dst[0] = (double)src[0];
             dst[1] = (double)src[1];
           }
@@ -800,6 +809,7 @@ int gx_tune_set(int knob, int value) {
   if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 4, GX_EINVAL, "tile map takes 0..4");
   else if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   else if (knob == GX_TUNE_DHOP_SHAPE) GX_REQUIRE(value >= 0 && value <= 8, GX_EINVAL, "tile shape takes 0..8 waves");
+  else if (knob == GX_TUNE_DHOP_PERSIST) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "persist takes 0 or 1");
   g_tune[knob] = value;
   g_tune_version++;
   return GX_OK;
@@ -923,18 +933,18 @@ static void pack_f16(const Field *f, const float *in, uint32_t *out) {
       for (int64_t i = b; i < e; i++) {
         float m = 0.f;
         for (int k = 0; k < 12; k++) {
-          const float *v = ih + blk_index(i, k, 12) * 2;
+          const float *v = ih + blk_index(i, k, 12, 2) * 2;
           m = std::max(m, std::max(std::fabs(v[0]), std::fabs(v[1])));
         }
         // a subnormal maximum is scaled by 2^64 (exact) first, as 1/m would overflow
         const float up = (m > 0.f && m < FLT_MIN) ? 0x1p64f : 1.f;
         const float inv = m > 0.f ? 1.f / (m * up) : 0.f;
         for (int k = 0; k < 12; k++) {
-          const float *v = ih + blk_index(i, k, 12) * 2;
+          const float *v = ih + blk_index(i, k, 12, 2) * 2;
           const half_t pr[2] = {(half_t)((v[0] * up) * inv), (half_t)((v[1] * up) * inv)};
-          std::memcpy(&oh[blk_index(i, k, 13)], pr, 4);
+          std::memcpy(&oh[blk_index(i, k, 13, 4)], pr, 4);
         }
-        std::memcpy(&oh[blk_index(i, 12, 13)], &m, 4);
+        std::memcpy(&oh[blk_index(i, 12, 13, 4)], &m, 4);
       }
     });
 }
@@ -947,11 +957,11 @@ static void unpack_f16(const Field *f, const uint32_t *in, float *out) {
       float *oh = out + hf * 12 * P * 2;
       for (int64_t i = b; i < e; i++) {
         float m;
-        std::memcpy(&m, &ih[blk_index(i, 12, 13)], 4);
+        std::memcpy(&m, &ih[blk_index(i, 12, 13, 4)], 4);
         for (int k = 0; k < 12; k++) {
           half_t pr[2];
-          std::memcpy(pr, &ih[blk_index(i, k, 13)], 4);
-          float *v = oh + blk_index(i, k, 12) * 2;
+          std::memcpy(pr, &ih[blk_index(i, k, 13, 4)], 4);
+          float *v = oh + blk_index(i, k, 12, 2) * 2;
           v[0] = (float)pr[0] * m;
           v[1] = (float)pr[1] * m;
         }
@@ -1012,7 +1022,7 @@ int gx_field_point_source(gx_field f, const int x[4], int spin, int colour) {
   const void *one = f->prec == GX_F64 ? (const void *)one64 : (const void *)one32;
   for (int s = 0; s < f->g.Ls; s++) {
     const int k = batch ? s : spin * 3 + colour;
-    const int64_t e = (int64_t)cb * 12 * f->g.half_pad() + blk_index(h * f->g.Ls + s, k, 12);
+    const int64_t e = (int64_t)cb * 12 * f->g.half_pad() + blk_index(h * f->g.Ls + s, k, 12, f->prec == GX_F64 ? 1 : 2);
     GX_CHECK_HIP(hipMemcpyAsync((char *)f->data + e * f->csize(), one, f->csize(), hipMemcpyHostToDevice, st));
   }
   GX_CHECK_HIP(hipStreamSynchronize(st));

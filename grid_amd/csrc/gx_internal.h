@@ -61,11 +61,19 @@ constexpr int kMaxRhs = 64;        // right-hand sides of one multi-RHS solve
 constexpr int kRedBlocks = 1024;   // fixed grid of the deterministic reductions
 constexpr int kRedSlots = 64;      // device result slots per context
 
-// Fields are stored in blocks of kBlk consecutive entries (h, s) of one checkerboard: entry e =
-// h*Ls + s of column k lives at ((e / kBlk) * ncol + k) * kBlk + e % kBlk, so the 12 components of a
-// wave's 64 entries form one contiguous run instead of 12 runs a whole column apart.
+// Fields are stored in blocks of kBlk consecutive entries (h, s) of one checkerboard; within a block
+// the 12 components of every entry are stored in groups of vw (the complex numbers that fill 16
+// bytes: 1 for FP64, 2 for FP32, 4 for FP16), group-major.  Component k < 12 of entry e lives at
+//   (e / kBlk) ncol kBlk + (k / vw) kBlk vw + (e % kBlk) vw + k % vw
+// (in complex numbers of the storage type), so a wave of 64 consecutive entries moves one
+// contiguous run per group and each lane 16 bytes per access.  FP16 adds column 12, one float per
+// entry (k = 12: (e / kBlk) ncol kBlk + 12 kBlk + e % kBlk).
 constexpr int kBlk = 64;
-inline int64_t blk_index(int64_t e, int k, int ncol) { return ((e / kBlk) * ncol + k) * kBlk + e % kBlk; }
+inline int64_t blk_index(int64_t e, int k, int ncol, int vw) {
+  const int64_t base = (e / kBlk) * ncol * kBlk;
+  if (k >= 12) return base + 12 * kBlk + e % kBlk;
+  return base + (int64_t)(k / vw) * kBlk * vw + (e % kBlk) * vw + k % vw;
+}
 
 struct Geom {
   int L[4];

@@ -45,24 +45,44 @@ __device__ __forceinline__ cx<T> csubi(cx<T> a, cx<T> b) { return {a.r + b.i, a.
 // Storage of field entries: double, float, or FP16 (half_t).  FP16 keeps a complex number as two
 // binary16 values and every (h, s) site of a half block carries a float scale in column 12 (its
 // largest |component|), so stored values lie in [-1, 1]; its arithmetic type is float.  FP16 links
-// are stored unscaled (SU(3) entries are bounded by 1).
+// are stored unscaled (SU(3) entries are bounded by 1).  VW complex numbers fill 16 bytes: a site's
+// components are stored in 16-byte groups of VW consecutive components, so a lane moves a site with
+// 12 / VW loads or stores of 16 bytes whatever the precision.
 template <typename S>
 struct Sto {
   typedef S C;
   static constexpr bool H = false;
   static constexpr int NCOL = 12;
+  static constexpr int VW = 16 / (int)sizeof(cx<S>);
 };
 template <>
 struct Sto<half_t> {
   typedef float C;
   static constexpr bool H = true;
   static constexpr int NCOL = 13;
+  static constexpr int VW = 4;
 };
 
-// position of column k of entry e in a half block of storage S (block layout, see Field)
+// Position of component k < 12 of entry e in a half field of storage S: blocks of 64 consecutive
+// entries; within a block, group g = k / VW of all 64 entries, entry by entry, each entry's group
+// the VW components VW g .. VW g + VW - 1:  (e / 64) NCOL 64 + (k / VW) 64 VW + (e % 64) VW + k % VW.
+// A wave of 64 consecutive entries moves one contiguous 1 KB run per group.
 template <typename S, typename I>
 __device__ __forceinline__ I bidx(I e, int k) {
-  return ((e >> 6) * (I)Sto<S>::NCOL + (I)k) * (I)kBlk + (e & (I)(kBlk - 1));
+  constexpr int VW = Sto<S>::VW;
+  return (e >> 6) * (I)(Sto<S>::NCOL * kBlk) + (I)((k / VW) * kBlk * VW) + (e & (I)(kBlk - 1)) * (I)VW + (I)(k % VW);
+}
+// FP16 site scale of entry e: a float in column 12 of its block
+template <typename S, typename I>
+__device__ __forceinline__ I sidx(I e) {
+  return (e >> 6) * (I)(Sto<S>::NCOL * kBlk) + (I)(12 * kBlk) + (e & (I)(kBlk - 1));
+}
+// entry of stored position i of an FP64 or FP32 half field (the inverse of bidx over the block)
+template <typename S>
+__device__ __forceinline__ int64_t entry_of(int64_t i) {
+  constexpr int VW = Sto<S>::VW;
+  const int64_t blk = i / (12 * kBlk), o = i - blk * 12 * kBlk;
+  return blk * kBlk + (o % (kBlk * VW)) / VW;
 }
 
 template <typename S>
@@ -74,56 +94,107 @@ __device__ __forceinline__ cx<typename Sto<S>::C> ldc(const cx<S> *__restrict__ 
 // scale of entry i of the half block at p (FP16 storage; 1 otherwise)
 template <typename S, typename I>
 __device__ __forceinline__ typename Sto<S>::C site_scale(const cx<S> *__restrict__ p, I i) {
-  if constexpr (Sto<S>::H) return reinterpret_cast<const float *>(p)[bidx<S, I>(i, 12)];
+  if constexpr (Sto<S>::H) return *reinterpret_cast<const float *>(p + sidx<S, I>(i));
   else return (typename Sto<S>::C)1;
 }
 
-// Column k of an entry lies k * kBlk storage elements after its column 0: the site helpers form
-// one 64-bit address per entry and reach the columns by constant offsets (immediate offsets of the
-// loads and stores instead of one 64-bit address computation per column).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// The VW components of group g of the entry whose group 0 is at q, as values of the arithmetic
+// type (FP16 unscaled).
+template <typename S>
+__device__ __forceinline__ void load_group(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ q, int g) {
+  constexpr int VW = Sto<S>::VW;
+  const cx<S> *__restrict__ a = q + g * kBlk * VW;
+  if constexpr (VW == 1) {
+    const f64x2 v = *reinterpret_cast<const f64x2 *>(a);
+    f[0] = {v.x, v.y};
+  } else if constexpr (!Sto<S>::H) {
+    const f32x4 v = *reinterpret_cast<const f32x4 *>(a);
+    f[0] = {v.x, v.y};
+    f[1] = {v.z, v.w};
+  } else {
+    const f16x8 v = *reinterpret_cast<const f16x8 *>(a);
+#pragma unroll
+    for (int c = 0; c < 4; c++) f[c] = {(float)v[2 * c], (float)v[2 * c + 1]};
+  }
+}
+
+// The 12 components of entry i (FP16: multiplied by the site's scale), 12 / VW loads of 16 bytes.
 template <typename S, typename I>
 __device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ p, I i) {
+  constexpr int VW = Sto<S>::VW;
   const cx<S> *__restrict__ q = p + bidx<S, I>(i, 0);
 #pragma unroll
-  for (int k = 0; k < 12; k++) f[k] = ldc<S>(q, k * kBlk);
+  for (int g = 0; g < 12 / VW; g++) load_group<S>(f + g * VW, q, g);
   if constexpr (Sto<S>::H) {
-    const float sc = reinterpret_cast<const float *>(q)[12 * kBlk];
+    const float sc = *reinterpret_cast<const float *>(p + sidx<S, I>(i));
 #pragma unroll
     for (int k = 0; k < 12; k++) f[k] = {f[k].r * sc, f[k].i * sc};
+  }
+}
+
+// acc += beta * (entry i of p), group by group (FP16 with the site's scale folded into beta)
+template <typename S, typename I>
+__device__ __forceinline__ void axpy_site(cx<typename Sto<S>::C> *acc, typename Sto<S>::C beta,
+                                          const cx<S> *__restrict__ p, I i) {
+  constexpr int VW = Sto<S>::VW;
+  typedef typename Sto<S>::C T;
+  const cx<S> *__restrict__ q = p + bidx<S, I>(i, 0);
+  const T b = beta * site_scale<S, I>(p, i);
+#pragma unroll
+  for (int g = 0; g < 12 / VW; g++) {
+    cx<T> v[VW];
+    load_group<S>(v, q, g);
+#pragma unroll
+    for (int c = 0; c < VW; c++) {
+      acc[g * VW + c].r = fma(b, v[c].r, acc[g * VW + c].r);
+      acc[g * VW + c].i = fma(b, v[c].i, acc[g * VW + c].i);
+    }
   }
 }
 
 // store the 12 components of entry i (FP16: with the site's scale); nt = non-temporal stores
 template <typename S, typename I>
 __device__ __forceinline__ void store_site(cx<S> *__restrict__ p, const cx<typename Sto<S>::C> *a, I i, bool nt) {
+  constexpr int VW = Sto<S>::VW;
   cx<S> *__restrict__ q = p + bidx<S, I>(i, 0);
   if constexpr (Sto<S>::H) {
     float m = 0.f;
 #pragma unroll
     for (int k = 0; k < 12; k++) m = fmaxf(m, fmaxf(fabsf(a[k].r), fabsf(a[k].i)));
-    if (__builtin_expect(m > 0.f && m < FLT_MIN, 0)) {
-      // subnormal site maximum: 1/m overflows, so scale the site by 2^64 (exact) before it
-      const float inv = 1.f / (m * 0x1p64f);
+    // a subnormal site maximum would overflow 1/m: scale the site by 2^64 (exact) before it
+    const bool tiny = __builtin_expect(m > 0.f && m < FLT_MIN, 0);
+    const float up = tiny ? 0x1p64f : 1.f;
+    const float inv = m > 0.f ? 1.f / (m * up) : 0.f;
 #pragma unroll
-      for (int k = 0; k < 12; k++)
-        q[k * kBlk] = {(half_t)((a[k].r * 0x1p64f) * inv), (half_t)((a[k].i * 0x1p64f) * inv)};
-    } else {
-      const float inv = m > 0.f ? 1.f / m : 0.f;
+    for (int g = 0; g < 3; g++) {
+      f16x8 v;
 #pragma unroll
-      for (int k = 0; k < 12; k++) q[k * kBlk] = {(half_t)(a[k].r * inv), (half_t)(a[k].i * inv)};
-    }
-    reinterpret_cast<float *>(q)[12 * kBlk] = m;
-  } else {
-    typedef S v2 __attribute__((ext_vector_type(2)));
-    if (nt) {
-#pragma unroll
-      for (int k = 0; k < 12; k++) {
-        v2 v = {a[k].r, a[k].i};
-        __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(q + k * kBlk));
+      for (int c = 0; c < 4; c++) {
+        v[2 * c] = (half_t)((a[4 * g + c].r * up) * inv);
+        v[2 * c + 1] = (half_t)((a[4 * g + c].i * up) * inv);
       }
-    } else {
+      *reinterpret_cast<f16x8 *>(q + g * kBlk * VW) = v;
+    }
+    *reinterpret_cast<float *>(p + sidx<S, I>(i)) = m;
+  } else if constexpr (VW == 1) {
 #pragma unroll
-      for (int k = 0; k < 12; k++) q[k * kBlk] = a[k];
+    for (int k = 0; k < 12; k++) {
+      const f64x2 v = {a[k].r, a[k].i};
+      f64x2 *d = reinterpret_cast<f64x2 *>(q + k * kBlk);
+      if (nt) __builtin_nontemporal_store(v, d);
+      else *d = v;
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < 6; g++) {
+      const f32x4 v = {a[2 * g].r, a[2 * g].i, a[2 * g + 1].r, a[2 * g + 1].i};
+      f32x4 *d = reinterpret_cast<f32x4 *>(q + g * kBlk * VW);
+      if (nt) __builtin_nontemporal_store(v, d);
+      else *d = v;
     }
   }
 }
@@ -503,18 +574,9 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     }
   }
   if (EPI != 0) {
-    const cx<S> *__restrict__ qp = PHI ? ep.phi + bidx<S>(o, 0) : ep.phi;
-    const T bsc = PHI ? ep.beta * site_scale<S>(qp, 0u) : T(0);
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
-      cx<T> v = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
-      if (PHI) {
-        const cx<T> p = ldc<S>(qp, k * kBlk);
-        v.r = fma(bsc, p.r, v.r);
-        v.i = fma(bsc, p.i, v.i);
-      }
-      acc[k] = v;
-    }
+    for (int k = 0; k < 12; k++) acc[k] = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
+    if (PHI) axpy_site<S>(acc, ep.beta, ep.phi, o);
   }
   store_site<S>(out, acc, o, (flags & 1) != 0);
 }
@@ -567,19 +629,15 @@ __device__ __forceinline__ void group_matrix(cx<T> *acc, cx<T> *stg, const T *ma
   }
 }
 
-// The hop of k_dhop (same legs, projectors, link staging and epilogues) in the general tile shape.
+// The hop of k_dhop (same legs, projectors, link staging and epilogues) in the general tile shape,
+// for one tile.  Every thread of the workgroup calls it (it synchronises the workgroup).
 template <typename S, bool DAG, int EPI>
-__global__ void __launch_bounds__(64 * kGenMaxWaves)
-    k_dhop_g(const cx<S> *__restrict__ in, cx<S> *__restrict__ out, const cx<S> *__restrict__ U, KGeom g, int cb_out,
-             const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags, HopEpi<S> ep, GShape sh) {
+__device__ __forceinline__ void hop_tile(const cx<S> *__restrict__ in, cx<S> *__restrict__ out,
+                                         const cx<S> *__restrict__ U, const KGeom &g, int cb_out, uint32_t tile,
+                                         bool sweep, int flags, const HopEpi<S> &ep, const GShape &sh,
+                                         cx<typename Sto<S>::C> *lds) {
   typedef typename Sto<S>::C T;
-  if (gated(gate)) return;
   constexpr bool MAT = EPI >= 2, PHI = (EPI & 1) != 0;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  cx<T> *lds = reinterpret_cast<cx<T> *>(lds_raw);
-  const bool sweep = map == 4;
-  const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, sweep ? 1 : map);
-  if (tile >= ntiles) return;  // uniform across the workgroup
   const uint32_t H4 = (uint32_t)g.H4, LS = sh.LS, epb = blockDim.x;
   const uint32_t t = threadIdx.x;
   const uint32_t hl = fdiv(t, sh.fLS), s = t - hl * LS;
@@ -611,23 +669,42 @@ __global__ void __launch_bounds__(64 * kGenMaxWaves)
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
   if (valid) hop_site<T, S, DAG, true, 0, true>(acc, fa, fb, in, U, lds + hl * kLinkPad, nb, LS, s, h, H4);
   if (MAT) group_matrix<T>(acc, lds, ep.mat, sh, t, hl, s, epb);
-  if (!valid) return;
-  const uint32_t o = h * LS + s;
-  if (EPI != 0) {
-    const cx<S> *__restrict__ qp = PHI ? ep.phi + bidx<S>(o, 0) : ep.phi;
-    const T bsc = PHI ? ep.beta * site_scale<S>(qp, 0u) : T(0);
+  if (valid) {
+    const uint32_t o = h * LS + s;
+    if (EPI != 0) {
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
-      cx<T> v = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
-      if (PHI) {
-        const cx<T> p = ldc<S>(qp, k * kBlk);
-        v.r = fma(bsc, p.r, v.r);
-        v.i = fma(bsc, p.i, v.i);
-      }
-      acc[k] = v;
+      for (int k = 0; k < 12; k++) acc[k] = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
+      if (PHI) axpy_site<S>(acc, ep.beta, ep.phi, o);
     }
+    store_site<S>(out, acc, o, (flags & 1) != 0);
   }
-  store_site<S>(out, acc, o, (flags & 1) != 0);
+}
+
+// One workgroup per tile (persist = 0), or persist = 1: a grid of resident workgroups (a multiple
+// of 8, so a workgroup keeps its XCD) each looping over the virtual blocks b, b + gridDim, ... of
+// the same tile map, nblocks of them: the same tiles and arithmetic with fewer workgroup launches.
+template <typename S, bool DAG, int EPI>
+__global__ void __launch_bounds__(64 * kGenMaxWaves)
+    k_dhop_g(const cx<S> *__restrict__ in, cx<S> *__restrict__ out, const cx<S> *__restrict__ U, KGeom g, int cb_out,
+             const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags, HopEpi<S> ep, GShape sh,
+             uint32_t nblocks, int persist) {
+  typedef typename Sto<S>::C T;
+  if (gated(gate)) return;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(lds_raw);
+  const bool sweep = map == 4;
+  const int tmap = sweep ? 1 : map;
+  if (!persist) {
+    const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, tmap);
+    if (tile < ntiles) hop_tile<S, DAG, EPI>(in, out, U, g, cb_out, tile, sweep, flags, ep, sh, lds);
+    return;
+  }
+  for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const uint32_t tile = tile_of_block(b, nblocks, tmap);
+    if (tile >= ntiles) continue;  // uniform across the workgroup
+    __syncthreads();               // the previous tile's LDS reads are done
+    hop_tile<S, DAG, EPI>(in, out, U, g, cb_out, tile, sweep, flags, ep, sh, lds);
+  }
 }
 
 // k_smat in the general tile shape: out = A(in) (+ beta phi) and optionally out2 = C(in), with A and
@@ -659,16 +736,7 @@ __global__ void __launch_bounds__(64 * kGenMaxWaves)
   group_matrix<T>(f, lds, mats, sh, t, hl, s, epb);
   if (OUT2) group_matrix<T>(f2, lds, mats2, sh, t, hl, s, epb);
   if (!valid) return;
-  if (PHI) {
-    const cx<S> *__restrict__ qp = phi + bidx<S>(o, 0);
-    const T bsc = beta * site_scale<S>(qp, 0u);
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-      const cx<T> p = ldc<S>(qp, k * kBlk);
-      f[k].r = fma(bsc, p.r, f[k].r);
-      f[k].i = fma(bsc, p.i, f[k].i);
-    }
-  }
+  if (PHI) axpy_site<S>(f, beta, phi, o);
   store_site<S>(out, f, o, false);
   if (OUT2) store_site<S>(out2, f2, o, false);
 }
@@ -702,6 +770,23 @@ static int general_waves(int LS, int knob, size_t csz, bool mat, int lb) {
       if (w >= 4) break;
     }
   return best < 0 && templated_ok ? 0 : best;
+}
+
+// Workgroups of `threads` threads and `lds` bytes of LDS that the device holds at once, rounded down
+// to a multiple of 8 (the XCD count): 2 waves per SIMD, the occupancy of the hop kernels, bounded by
+// LDS (160 KB per CU).
+static int resident_groups(int threads, size_t lds) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    cus = prop.multiProcessorCount;
+  }
+  const int waves = threads / 64;
+  int per_cu = std::max(1, 8 / waves);  // 4 SIMDs x 2 waves
+  if (lds > 0) per_cu = std::min<int>(per_cu, (int)(160 * 1024 / lds));
+  return std::max(1, per_cu) * cus / 8 * 8;
 }
 
 static GShape make_shape(int LS, int LB, int W) {
@@ -772,9 +857,13 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
     const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
     const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
     const size_t lds = std::max((size_t)sh.SPB * kLinkPad, kepi >= 2 ? (size_t)6 * 64 * W : (size_t)0) * sizeof(cx<T>);
+    // persistent grid: the resident capacity of the device (multiple of 8), when smaller than the tiles
+    const int persist_wg = tune_get(GX_TUNE_DHOP_PERSIST) ? resident_groups(64 * W, lds) : 0;
+    const int persist = persist_wg > 0 && (uint32_t)persist_wg < blocks ? 1 : 0;
+    const uint32_t grid = persist ? (uint32_t)persist_wg : blocks;
 #define GX_DHOPG_GO(DAG, EPI)                                                                                        \
-  hipLaunchKernelGGL((k_dhop_g<S, DAG, EPI>), dim3(blocks), dim3(64 * W), lds, st, pin, pout, pU, k, cb_out, gate,   \
-                     ntiles, map, flags, ep, sh)
+  hipLaunchKernelGGL((k_dhop_g<S, DAG, EPI>), dim3(grid), dim3(64 * W), lds, st, pin, pout, pU, k, cb_out, gate,     \
+                     ntiles, map, flags, ep, sh, blocks, persist)
 #define GX_DHOPG_EPI(EPI) \
   if (dag) GX_DHOPG_GO(true, EPI); else GX_DHOPG_GO(false, EPI)
     switch (kepi) {
@@ -903,16 +992,7 @@ __global__ void __launch_bounds__(256) k_smat(const cx<S> *__restrict__ in, cons
       }
     }
   }
-  if (PHI) {
-    const cx<S> *__restrict__ qp = phi + bidx<S>(o, 0);
-    const T bsc = beta * site_scale<S>(qp, 0u);
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-      const cx<T> p = ldc<S>(qp, k * kBlk);
-      acc[k].r = fma(bsc, p.r, acc[k].r);
-      acc[k].i = fma(bsc, p.i, acc[k].i);
-    }
-  }
+  if (PHI) axpy_site<S>(acc, beta, phi, o);
   store_site<S>(out, acc, o, false);
   if (OUT2) store_site<S>(out2, acc2, o, false);
 }
@@ -1482,7 +1562,7 @@ __global__ void __launch_bounds__(256) k_mrhs_update(int64_t n, int Ls, cx<T> *_
                                                      const cx<T> *__restrict__ r, const MrhsState *st) {
   if (mgated_prev(st)) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = (i / (12 * kBlk)) * kBlk + (i & (kBlk - 1));  // entry of stored position i
+    const int64_t e = entry_of<T>(i);  // entry of stored position i
     const int s = (int)(e % Ls);
     const T a = (T)st->a[s], b = (T)st->b[s];
     const cx<T> pv = p[i], xv = psi[i], rv = r[i];
@@ -2188,7 +2268,8 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
 GX_INST(double)
 GX_INST_REAL(double)
 template int launch_convert<double, float>(int64_t, const void *, void *, hipStream_t);
-template int launch_convert<float, double>(int64_t, const void *, void *, hipStream_t);
+template int launch_convert_site<double, float>(int64_t, int64_t, const void *, void *, hipStream_t);
+template int launch_convert_site<float, double>(int64_t, int64_t, const void *, void *, hipStream_t);
 #elif GX_PART == 1
 GX_INST(float)
 GX_INST_REAL(float)

@@ -218,6 +218,7 @@ struct GridThread {
 // ---------------- grids ----------------
 class GridBase {
  public:
+  bool IsBoss() const { return true; }  // one process per GPU: this process does the IO
   int dims[4] = {0, 0, 0, 0};
   int Ls = 1;
   bool checkerboarded = false;
@@ -477,14 +478,58 @@ typedef void PeriodicGimplR;
 // Grid: FieldMetaData (Grid/parallelIO/MetaData.h:62-98), the NERSC header fields a reader returns
 struct FieldMetaData {
   std::vector<int> dimension = std::vector<int>(4);
-  std::string data_type, floating_point, ensemble_id, ensemble_label;
+  std::string data_type, floating_point, ensemble_id, ensemble_label, ildg_lfn;
   RealD link_trace = 0, plaquette = 0;
-  uint32_t checksum = 0;
+  uint32_t checksum = 0, scidac_checksuma = 0, scidac_checksumb = 0;
   int sequence_number = 0;
 };
 
 // Grid: NerscIO::readConfiguration / writeConfiguration (Grid/parallelIO/NerscIO.h:148-290).  A
 // checksum, plaquette or link-trace mismatch asserts, as in Grid.
+// Grid: IldgWriter(isboss).open(file); writeConfiguration(Umu, sequence, LFN, description); close()
+// (IldgIO.h:611-690).  Writes IEEE64BIG as Grid does; the sequence number lives in Grid's private
+// grid-format record, which this writer does not produce.
+class IldgWriter {
+ public:
+  explicit IldgWriter(bool = true) {}
+  void open(const std::string &file) { filename = file; }
+  void writeConfiguration(LatticeGaugeFieldD &Umu, int sequence, const std::string &LFN, const std::string &description) {
+    (void)sequence;
+    detail::check(gx_ildg_write(detail::context(), filename.c_str(), Umu.Grid()->dims, Umu.raw(), 64, LFN.c_str(),
+                                description.c_str()),
+                  "IldgWriter::writeConfiguration");
+  }
+  void close() {}
+
+ private:
+  std::string filename;
+};
+// Grid: IldgReader.open(file); readConfiguration(Umu, FieldMetaData&); close() (IldgIO.h:698-936)
+class IldgReader {
+ public:
+  void open(const std::string &file) { filename = file; }
+  void readConfiguration(LatticeGaugeFieldD &Umu, FieldMetaData &header) {
+    gx_ildg_info info{};
+    const int rc = gx_ildg_read(detail::context(), filename.c_str(), Umu.Grid()->dims, Umu.raw(), &info);
+    header.dimension.assign(info.dims, info.dims + 4);
+    header.data_type = "4D_SU3_GAUGE_3x3";
+    header.floating_point = info.precision == 32 ? "IEEE32BIG" : "IEEE64BIG";
+    header.ildg_lfn = info.lfn;
+    header.plaquette = info.has_plaquette ? info.plaquette_file : 0.0;
+    header.link_trace = info.has_plaquette ? info.link_trace_file : 0.0;
+    header.scidac_checksuma = info.scidac_checksuma_file;
+    header.scidac_checksumb = info.scidac_checksumb_file;
+    detail::check(rc, "IldgReader::readConfiguration");
+    std::cout << GridLogMessage << "ILDG configuration " << filename << ": SciDAC checksums " << std::hex
+              << info.scidac_checksuma << " " << info.scidac_checksumb << std::dec << " match; plaquette "
+              << info.plaquette << " link trace " << info.link_trace << std::endl;
+  }
+  void close() {}
+
+ private:
+  std::string filename;
+};
+
 struct NerscIO {
   static void readConfiguration(LatticeGaugeFieldD &Umu, FieldMetaData &header, const std::string &file) {
     gx_nersc_info info{};

@@ -288,6 +288,14 @@ int gx_cg_multishift(gx_op op, gx_field src, int nshift, const double *shifts, c
  * row-major.  F64 fields of one lattice and subset. */
 int gx_slice_inner_product_matrix(gx_field lhs, gx_field rhs, double *mat);
 
+/* No counterpart in the reference (Grid allocates freely).  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * gx_slice_inner_product_matrix with its workspace (workgroup partials and the result matrix,
+ * 8.4 MB) in a fixed device array that each GPU holds until the process ends, instead of memory
+ * allocated per call.  Same result, bit for bit.  Ls <= 16.  Calls that overlap on one GPU race on
+ * that array. */
+int gx_slice_inner_product_matrix_noalloc(gx_field lhs, gx_field rhs, double *mat);
+
 /* Grid: sliceMaddMatrix(R, aa, X, Y, Orthog = 0, scale), Lattice_reduction.h:717-742:
  * R_t = Y_t + scale sum_s X_s aa(s, t).  aa: 2 Ls^2 doubles, row-major.  Y = NULL gives
  * sliceMulMatrix (R_t = scale sum_s X_s aa(s, t)).  R may be X or Y. */
@@ -375,6 +383,8 @@ enum gx_tune_knob {
   GX_TUNE_DHOP_SHAPE = 6,   /* hop tile shape: 0 = whole sites per wave, 4 waves (Ls <= 16); W > 0 = W waves of
                                64 consecutive (site, s) entries each, whole sites per workgroup (any Ls whose odd
                                part is at most 7, W a multiple of Ls / gcd(Ls, 64), W <= 8) */
+  GX_TUNE_DHOP_PERSIST = 7, /* 1: the W-wave hop (GX_TUNE_DHOP_SHAPE > 0 or Ls > 16) runs as a persistent grid of
+                               resident workgroups looping over the tiles */
   GX_TUNE_COUNT = 8
 };
 int gx_tune_set(int knob, int value);
@@ -429,6 +439,36 @@ int gx_nersc_read(gx_ctx ctx, const char *path, const int dims[4], double *U, gx
  * the plaquette, link trace and checksum in the header (bits32 has no effect in Grid either). */
 int gx_nersc_write(gx_ctx ctx, const char *path, const int dims[4], const double *U, int two_row,
                    const char *ensemble_label, const char *ensemble_id, int sequence_number);
+
+/* Header and metadata of an ILDG file as read, next to the values recomputed from its data. */
+typedef struct {
+  int dims[4];
+  int precision;                          /* 32 or 64 (big-endian IEEE words) */
+  uint32_t scidac_checksuma, scidac_checksumb;            /* computed from the binary record */
+  uint32_t scidac_checksuma_file, scidac_checksumb_file;  /* scidac-checksum record */
+  int has_plaquette;                      /* Grid grid-format or USQCD usqcdInfo metadata present */
+  double plaquette_file, link_trace_file;
+  double plaquette, link_trace;           /* recomputed on the device */
+  int has_norm;                           /* Grid FieldNormMetaData record present */
+  double norm2_file, norm2;
+  char lfn[256];                          /* ildg-data-lfn */
+} gx_ildg_info;
+
+/* Grid: IldgReader::readConfiguration(Umu, FieldMetaData) (IldgIO.h:713-936): reads an ILDG (LIME)
+ * gauge file on the lattice dims into U (layout above).  As in Grid it needs the ildg-format,
+ * ildg-data-lfn, ildg-binary-data and scidac-checksum records, and fails (GX_EINVAL, values in
+ * the message and *info) when the lattice or field type differs, the SciDAC checksums (BinaryIO.h:
+ * 139-193) do not match, norm2 differs from a FieldNormMetaData record by 1e-5 (relative), or the
+ * recomputed plaquette or link trace differs from Grid's or USQCD's metadata by 1e-5. */
+int gx_ildg_read(gx_ctx ctx, const char *path, const int dims[4], double *U, gx_ildg_info *info);
+
+/* Grid: IldgWriter::writeConfiguration(Umu, sequence, LFN, description) (IldgIO.h:633-690): writes U
+ * as an ILDG file with big-endian IEEE words of `precision` bits (32 or 64): SciDAC file and record
+ * metadata with the USQCD usqcdInfo (plaquette, link trace, description), ildg-format,
+ * ildg-data-lfn (lfn), the binary record and its SciDAC checksums.  Grid's private grid-format and
+ * FieldNormMetaData records are not written; readers then use the ILDG and USQCD records. */
+int gx_ildg_write(gx_ctx ctx, const char *path, const int dims[4], const double *U, int precision, const char *lfn,
+                  const char *description);
 
 #ifdef __cplusplus
 }

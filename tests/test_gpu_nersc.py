@@ -142,3 +142,116 @@ def test_nersc_rejects_bad_files(ctx, tmp_path):
         gx.nersc_read(ctx, str(good), (4, 4, 4, 8))
     with pytest.raises(gx.GxError):
         gx.nersc_read(ctx, str(tmp_path / 'missing.nersc'), dims)
+
+
+# ---------------- ILDG (LIME) files ----------------
+import struct
+import zlib
+
+LIME_MAGIC = 0x456789ab
+
+
+def lime_records(path):
+    """Independent LIME parser: [(type, mb, me, data)] from the published record layout."""
+    out = []
+    with open(path, 'rb') as f:
+        while True:
+            h = f.read(144)
+            if not h:
+                return out
+            magic, version, flags, length = struct.unpack('>IHHQ', h[:16])
+            assert magic == LIME_MAGIC and version == 1
+            typ = h[16:144].split(b'\0')[0].decode()
+            data = f.read(length)
+            pad = (length + 7) // 8 * 8 - length
+            assert f.read(pad) == b'\0' * pad
+            out.append((typ, bool(flags & 0x8000), bool(flags & 0x4000), data))
+
+
+def lime_write(path, records):
+    with open(path, 'wb') as f:
+        for typ, mb, me, data in records:
+            flags = (0x8000 if mb else 0) | (0x4000 if me else 0)
+            f.write(struct.pack('>IHHQ', LIME_MAGIC, 1, flags, len(data)) + typ.encode().ljust(128, b'\0'))
+            f.write(data + b'\0' * ((len(data) + 7) // 8 * 8 - len(data)))
+
+
+def scidac_sums(binary, nsite):
+    """SciDAC checksums: CRC-32 (zlib) of each site's stored bytes, rotated by site % 29 / % 31, XORed."""
+    per = len(binary) // nsite
+    a = b = 0
+    rot = lambda x, r: ((x << r) | (x >> (32 - r))) & 0xffffffff if r else x
+    for s in range(nsite):
+        c = zlib.crc32(binary[s * per:(s + 1) * per]) & 0xffffffff
+        a ^= rot(c, s % 29)
+        b ^= rot(c, s % 31)
+    return a, b
+
+
+@pytest.mark.parametrize('precision', [64, 32])
+def test_ildg_write_layout_and_round_trip(ctx, tmp_path, precision):
+    """The written file, parsed by an independent LIME reader: the record sequence and flags of
+    Grid's writer (file message, then field message), ildg-format of this lattice, the binary
+    record equal to numpy's big-endian bytes of the links, SciDAC checksums equal to zlib's CRC-32
+    recipe; reading it back gives the field (exactly for 64 bits) and the recomputed metadata."""
+    dims = (4, 6, 2, 4)
+    U = gx.random_gauge(dims, 601, 0.4)
+    path = str(tmp_path / f'cfg{precision}.ildg')
+    gx.ildg_write(ctx, path, U, dims, precision=precision, lfn='lfn://test/cfg', description='test field')
+    recs = lime_records(path)
+    assert [r[0] for r in recs] == ['scidac-private-file-xml', 'scidac-file-xml', 'scidac-private-record-xml',
+                                    'scidac-record-xml', 'ildg-format', 'ildg-data-lfn', 'ildg-binary-data',
+                                    'scidac-checksum']
+    assert [(r[1], r[2]) for r in recs] == [(True, False), (False, True), (True, False)] + [(False, False)] * 4 + \
+        [(False, True)]
+    rec = {r[0]: r[3] for r in recs}
+    fmt = rec['ildg-format'].decode()
+    for tag, v in (('field', 'su3gauge'), ('precision', str(precision)), ('lx', '4'), ('ly', '6'), ('lz', '2'),
+                   ('lt', '4')):
+        assert f'<{tag}>{v}</{tag}>' in fmt
+    assert rec['ildg-data-lfn'] == b'lfn://test/cfg'
+    want = U.astype('>c16' if precision == 64 else '>c8').tobytes()
+    assert rec['ildg-binary-data'] == want
+    a, b = scidac_sums(want, int(np.prod(dims)))
+    ck = rec['scidac-checksum'].decode()
+    assert f'<suma>{a:x}</suma>' in ck and f'<sumb>{b:x}</sumb>' in ck
+    V, info = gx.ildg_read(ctx, path, dims)
+    if precision == 64:
+        assert np.array_equal(V, U)
+    else:
+        assert np.array_equal(V, U.astype(np.complex64).astype(np.complex128))
+    assert info['precision'] == precision and info['has_plaquette'] and info['lfn'] == 'lfn://test/cfg'
+    assert (info['scidac_checksuma'], info['scidac_checksumb']) == (a, b)
+    assert abs(info['plaquette'] - info['plaquette_file']) < 1e-6
+
+
+def test_ildg_reads_foreign_file_and_rejects_damage(ctx, tmp_path):
+    """A file assembled here from the published formats (records in another order, Grid's
+    FieldNormMetaData and grid-format metadata, no USQCD record) reads back to the numpy field;
+    a flipped byte, a missing checksum record or another lattice is refused."""
+    dims = (4, 4, 4, 2)
+    U = gx.random_gauge(dims, 611, 0.3)
+    binary = U.astype('>c16').tobytes()
+    a, b = scidac_sums(binary, int(np.prod(dims)))
+    P, T = gx.gauge_statistics(ctx, U, dims)
+    fmt = ('<?xml version="1.0"?><ildgFormat><version>1.0</version><field>su3gauge</field>'
+           '<precision>64</precision><lx>4</lx><ly>4</ly><lz>4</lz><lt>2</lt></ildgFormat>').encode()
+    grid = (f'<?xml version="1.0"?><FieldMetaData><nd>4</nd><link_trace>{T!r}</link_trace>'
+            f'<plaquette>{P!r}</plaquette><floating_point>IEEE64BIG</floating_point></FieldMetaData>').encode()
+    norm = f'<?xml version="1.0"?><FieldNormMetaData><norm2>{float(np.sum(np.abs(U) ** 2))!r}</norm2></FieldNormMetaData>'
+    ck = f'<?xml version="1.0"?><scidacChecksum><version>1.0</version><suma>{a:x}</suma><sumb>{b:x}</sumb></scidacChecksum>'
+    recs = [('grid-format', True, False, grid), ('FieldNormMetaData', False, False, norm.encode()),
+            ('ildg-format', False, False, fmt), ('ildg-data-lfn', False, False, b'foreign'),
+            ('ildg-binary-data', False, False, binary), ('scidac-checksum', False, True, ck.encode())]
+    good = str(tmp_path / 'foreign.ildg')
+    lime_write(good, recs)
+    V, info = gx.ildg_read(ctx, good, dims)
+    assert np.array_equal(V, U) and info['has_norm'] and info['has_plaquette'] and info['lfn'] == 'foreign'
+    bad = bytearray(binary)
+    bad[1000] ^= 0x10
+    for name, rs, d in (('flipped', recs[:4] + [('ildg-binary-data', False, False, bytes(bad))] + recs[5:], dims),
+                        ('no-checksum', recs[:5], dims), ('lattice', recs, (4, 4, 2, 4))):
+        p = str(tmp_path / f'{name}.ildg')
+        lime_write(p, rs)
+        with pytest.raises(gx.GxError):
+            gx.ildg_read(ctx, p, d)
