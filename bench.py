@@ -371,7 +371,71 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=CG_T
             best = name
     out['speedup'] = out[best]['speedup']
     out['speedup_variant'] = best
+    # the FP32 12-RHS DhopEO (3.02 GB per call) with the link products on the VALU and on the
+    # matrix cores (v_mfma_f32_16x16x4f32), tuned knobs otherwise
+    H = np.prod(dims) // 2
+    byts = H * (24 * nrhs + 72) * 8
+    fi, fo = B.field(gx.F32, gx.ODD), B.field(gx.F32, gx.EVEN)
+    full = B.field(gx.F32)
+    full.gaussian(2030)
+    gx.pick_checkerboard(1, fi, full)
+    full.close()
+    hop = {}
+    for name, mf in (('valu', 0), ('mfma', 1)):
+        gx.tune_set(gx.TUNE_DHOP_MFMA, mf)
+        ms = time_dhop(ctx, B, fi, fo, 20, 3)
+        hop[name] = {'ms': round(ms, 4), 'gbs': round(byts / (ms * 1e-3) / 1e9, 1),
+                     'frac_of_8tbs': round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    gx.tune_set(gx.TUNE_DHOP_MFMA, 0)
+    out['dhop_eo_12rhs_f32'] = dict(hop, bytes=int(byts))
+    fi.close()
+    fo.close()
     return out
+
+
+def bench_graphs(gx, ctx):
+    """Leg 'graphs': CG iterations replayed as captured hipGraphs against plain launches, on three
+    workloads: C1 (Wilson 8^4 FP64, launch-bound), C2 (Mobius 16^4 x Ls16 FP32) and the C4 lattice
+    (Wilson 32^3x64 with 12 right-hand sides as one FP32 CG).  A fixed number of iterations
+    (tolerance 0), best of two runs, ms per iteration; the other knobs at the library defaults."""
+    out = {}
+    cases = (('c1_wilson_8x8x8x8_f64', (8, 8, 8, 8), 1, 'wilson', dict(mass=0.1), gx.F64, 0.5, 400),
+             ('c2_mobius_16x16x16x16_ls16_f32', (16, 16, 16, 16), 16, 'mobius', dict(mass=0.01, **{
+                 k: v for k, v in PARAMS.items() if k != 'mass'}), gx.F32, EPS, 100),
+             ('c4_wilson_32x32x32x64_12rhs_f32', (32, 32, 32, 64), 12, 'wilson', dict(mass=0.05), gx.F32, EPS, 24))
+    for name, dims, Ls, kind, kw, prec, eps, iters in cases:
+        D = gx.FermionOp(ctx, gx.random_gauge(dims, 1020, eps), dims, Ls, kind, **kw)
+        full = D.field(prec)
+        full.gaussian(2020)
+        src, x = D.field(prec, 1), D.field(prec, 1)
+        gx.pick_checkerboard(1, src, full)
+        full.close()
+        res = {}
+        for graph in (0, 1):
+            gx.tune_set(gx.TUNE_CG_GRAPH, graph)
+            best = None
+            for _ in range(2):
+                x.zero()
+                ctx.sync()
+                t0 = time.perf_counter()
+                D.cg(src, x, tol=0.0, maxit=iters)
+                ctx.sync()
+                sec = time.perf_counter() - t0
+                best = sec if best is None else min(best, sec)
+            res['graph' if graph else 'eager'] = round(1e3 * best / iters, 4)
+        res['graph_speedup'] = round(res['eager'] / res['graph'], 3)
+        out[name] = res
+        for f in (src, x):
+            f.close()
+        D.close()
+    gx.tune_set(gx.TUNE_CG_GRAPH, 1)
+    return out
+
+
+def bench_serve_sweep(n, rates, slots=4, mixed=2):
+    """Leg 'sweep': the same Poisson trace shape at several arrival rates, so capacity and latency
+    are reported together (p50/p99 grow once the rate passes the GPU's service rate)."""
+    return {f'{r:g}_per_s': bench_serving(n, r, slots, mixed) for r in rates}
 
 
 def bench_serving(n, rate, slots=4, mixed=1):
@@ -403,9 +467,12 @@ def main():
     ap.add_argument('--serve-slots', type=int, default=4)
     ap.add_argument('--serve-mixed', type=int, default=2, choices=(1, 2),
                     help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
-    ap.add_argument('--legs', default='dhop,large,cg,c3,c4,serve,cpu,ms,block',
+    ap.add_argument('--sweep-requests', type=int, default=384)
+    ap.add_argument('--sweep-rates', default='20,30,36')
+    ap.add_argument('--legs', default='dhop,large,cg,c3,c4,serve,sweep,graphs,cpu,ms,block',
                     help='comma list of legs to run: dhop, large (DhopEO at >= 1 GB working sets), cg, c3, c4, '
-                         'serve, cpu, ms (multishift CG), block (BlockCGrQ); profiling runs pick one')
+                         'serve, sweep (serving at several rates), graphs (hipGraph against eager CG), cpu, '
+                         'ms (multishift CG), block (BlockCGrQ); profiling runs pick one')
     ap.add_argument('--no-graphs', action='store_true',
                     help='solver iterations as plain launches (for profiler runs; same kernels and timings)')
     ap.add_argument('--no-tune', action='store_true',
@@ -493,6 +560,9 @@ def main():
     c3 = None if args.no_c3 else bench_mixed_dwf(gx, ctx, tune=tune)
     c4 = None if args.no_c4 else bench_multi_rhs(gx, ctx, tune=tune)
     serve = bench_serving(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_mixed) if args.serve_requests > 0 else None
+    sweep = (bench_serve_sweep(args.sweep_requests, [float(r) for r in args.sweep_rates.split(',')], args.serve_slots,
+                               args.serve_mixed) if 'sweep' in legs and args.sweep_requests > 0 else None)
+    graphs = bench_graphs(gx, ctx) if 'graphs' in legs else None
     ms = bench_multishift(gx, ctx) if 'ms' in legs else None
     blk = bench_block_cgrq(gx, ctx) if 'block' in legs else None
 
@@ -544,6 +614,8 @@ def main():
         'multishift': ms,
         'block_cgrq': blk,
         'serving': serve,
+        'serving_rate_sweep': sweep,
+        'cg_graph_vs_eager': graphs,
     }
     print(json.dumps(out))
 

@@ -352,7 +352,7 @@ def field_copy(dst, src):
 
 
 (TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH, TUNE_FUSED, TUNE_DHOP_SHAPE,
- TUNE_DHOP_PERSIST) = 0, 1, 2, 3, 4, 5, 6, 7
+ TUNE_DHOP_PERSIST, TUNE_DHOP_MFMA) = 0, 1, 2, 3, 4, 5, 6, 7, 8
 
 
 def tune_set(knob, value):

@@ -28,7 +28,7 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
-static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0};  // map, pf, nt, order, graph, fused, shape, persist
+static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0, 0};  // map pf nt order graph fused shape persist mfma
 static int g_tune_version = 0;
 int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
 int tune_version() { return g_tune_version; }
@@ -809,7 +809,8 @@ int gx_tune_set(int knob, int value) {
   if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 4, GX_EINVAL, "tile map takes 0..4");
   else if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   else if (knob == GX_TUNE_DHOP_SHAPE) GX_REQUIRE(value >= 0 && value <= 8, GX_EINVAL, "tile shape takes 0..8 waves");
-  else if (knob == GX_TUNE_DHOP_PERSIST) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "persist takes 0 or 1");
+  else if (knob == GX_TUNE_DHOP_PERSIST || knob == GX_TUNE_DHOP_MFMA)
+    GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   g_tune[knob] = value;
   g_tune_version++;
   return GX_OK;

@@ -15,6 +15,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstring>
+#include <type_traits>
 
 #include "gx_internal.h"
 
@@ -707,6 +708,123 @@ __global__ void __launch_bounds__(64 * kGenMaxWaves)
   }
 }
 
+// ---------------- multi-RHS Wilson hop with the link products on the matrix cores ----------------
+// For a multi-RHS operator (Ls = Nrhs <= 16 right-hand sides sharing the links) the product of each
+// leg is a small dense contraction: chi[spin][a][n] = sum_b U[a][b] h[spin][b][n] over colours
+// a, b and right-hand sides n.  In real form it is D (6 x 16) = A (6 x 6) B (6 x 16), rows
+// m = 2 a + re/im, columns k = 2 b + re/im, A the realified link [[Ur, -Ui], [Ui, Ur]] and B the
+// projected half-spinors of the 16 right-hand sides, computed per (site, spin) with two
+// v_mfma_f32_16x16x4f32 (K = 8, rows and columns past 6 zero).  Operand layout of that instruction
+// (lane l of the wave): A[l % 16][l / 16], B[l / 16][l % 16], D[4 (l / 16) + v][l % 16] in result
+// register v.  Each leg stages the lanes' half-spinors in LDS in B's layout, the waves run the
+// (site, spin) products, and the lanes read their results back for the reconstruction.  FP32
+// storage, general tile shape (whole sites per workgroup), Grid's leg order.
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+constexpr int kMfmaRows = 8;   // K and M padded to 8 (6 used) in the LDS staging
+
+template <bool DAG, int LEG>
+__device__ __forceinline__ void mfma_leg(cx<float> *acc, const cx<float> *f, float *hsl, float *dl,
+                                         const cx<float> *links, uint32_t hl, uint32_t s, uint32_t LS, uint32_t spb) {
+  typedef LegSel<DAG, 0, LEG> L;
+  cx<float> hs[6];
+  project<float, L::MU, L::PLUS>(f, hs);
+#pragma unroll
+  for (int sp = 0; sp < 2; sp++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float *row = hsl + ((hl * 2 + sp) * kMfmaRows + 2 * c) * 16 + s;
+      row[0] = hs[sp * 3 + c].r;
+      row[16] = hs[sp * 3 + c].i;
+    }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int i = lane & 15, q = lane >> 4;
+  for (uint32_t job = wave; job < 2 * spb; job += nwaves) {
+    const uint32_t site = job >> 1, sp = job & 1;
+    const cx<float> *u = links + site * kLinkPad + L::P * 9;
+    const float *B = hsl + (site * 2 + sp) * kMfmaRows * 16;
+    f32x4v d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+      const int k = 4 * kk + q;
+      float av = 0.f, bv = 0.f;
+      if (i < 6 && k < 6) {
+        const cx<float> uab = u[(i >> 1) * 3 + (k >> 1)];
+        const int ra = i & 1, rb = k & 1;
+        av = ra == rb ? uab.r : (ra == 0 ? -uab.i : uab.i);
+      }
+      if (k < 6 && (uint32_t)i < LS) bv = B[k * 16 + i];
+      d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
+    }
+    float *D = dl + (site * 2 + sp) * kMfmaRows * 16;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int m = 4 * q + v;
+      if (m < 6 && (uint32_t)i < LS) D[m * 16 + i] = d[v];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float *d0 = dl + ((hl * 2 + 0) * kMfmaRows + 2 * a) * 16 + s;
+    const float *d1 = dl + ((hl * 2 + 1) * kMfmaRows + 2 * a) * 16 + s;
+    recon_colour<float, L::MU, L::PLUS>(acc, a, cx<float>{d0[0], d0[16]}, cx<float>{d1[0], d1[16]});
+  }
+}
+
+template <bool DAG, int EPI>
+__global__ void __launch_bounds__(64 * kGenMaxWaves)
+    k_dhop_mfma(const cx<float> *__restrict__ in, cx<float> *__restrict__ out, const cx<float> *__restrict__ U,
+                KGeom g, int cb_out, const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags,
+                HopEpi<float> ep, GShape sh) {
+  if (gated(gate)) return;
+  constexpr bool PHI = (EPI & 1) != 0;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const uint32_t spb = sh.SPB;
+  cx<float> *links = reinterpret_cast<cx<float> *>(lds_raw);
+  float *hsl = reinterpret_cast<float *>(links + spb * kLinkPad);
+  float *dl = hsl + spb * 2 * kMfmaRows * 16;
+  const bool sweep = map == 4;
+  const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, sweep ? 1 : map);
+  if (tile >= ntiles) return;  // uniform across the workgroup
+  const uint32_t H4 = (uint32_t)g.H4, LS = sh.LS;
+  const uint32_t t = threadIdx.x;
+  const uint32_t hl = fdiv(t, sh.fLS), s = t - hl * LS;
+  const uint32_t p0 = tile * spb;
+  const bool valid = p0 + hl < H4;
+  const uint32_t pos = valid ? p0 + hl : p0;
+  const uint32_t h = sweep ? sweep_site(g, pos) : pos;
+  uint32_t nb[8];
+  site_neighbours(g, h, cb_out, nb);
+  for (uint32_t pk = s; pk < 72; pk += LS) links[hl * kLinkPad + pk] = ldc<float>(U, pk * H4 + h);
+  cx<float> f[12], acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc[k] = {0.f, 0.f};
+  // every lane takes part in the staging and the products (lanes past the lattice read their
+  // tile's first site and store nothing); the next leg's neighbour load is issued before the
+  // products of this one
+  load_site<float>(f, in, leg_idx<DAG, 0, 0>(nb, LS, s));
+  __syncthreads();
+#define GX_MFMA_LEG(L)                                                                           \
+  {                                                                                              \
+    cx<float> fc[12];                                                                            \
+    _Pragma("unroll") for (int k = 0; k < 12; k++) fc[k] = f[k];                                 \
+    if (L < 7) load_site<float>(f, in, leg_idx<DAG, 0, (L < 7 ? L + 1 : 7)>(nb, LS, s));         \
+    mfma_leg<DAG, L>(acc, fc, hsl, dl, links, hl, s, LS, spb);                                   \
+  }
+  GX_MFMA_LEG(0) GX_MFMA_LEG(1) GX_MFMA_LEG(2) GX_MFMA_LEG(3) GX_MFMA_LEG(4) GX_MFMA_LEG(5) GX_MFMA_LEG(6)
+  GX_MFMA_LEG(7)
+#undef GX_MFMA_LEG
+  if (!valid) return;
+  const uint32_t o = h * LS + s;
+  if (EPI != 0) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
+    if (PHI) axpy_site<float>(acc, ep.beta, ep.phi, o);
+  }
+  store_site<float>(out, acc, o, (flags & 1) != 0);
+}
+
 // k_smat in the general tile shape: out = A(in) (+ beta phi) and optionally out2 = C(in), with A and
 // C block diagonal in groups of LB slices.
 template <typename S, bool OUT2, bool PHI>
@@ -760,8 +878,15 @@ static int general_waves(int LS, int knob, size_t csz, bool mat, int lb) {
     return w % w0 == 0 && w <= kGenMaxWaves && std::max(links, stage) <= 65536;
   };
   const bool templated_ok = LS <= kMaxLs && lb == LS;
-  if (LS < kGenMinLs) return templated_ok ? 0 : -1;
   if (knob > 0 && fits(knob)) return knob;
+  if (knob < 0) {  // the smallest whole-site shape of at least 4 waves, for any LS
+    for (int w = w0; w <= kGenMaxWaves; w += w0)
+      if (w >= 4 && fits(w)) return w;
+    for (int w = kGenMaxWaves / w0 * w0; w >= w0; w -= w0)
+      if (fits(w)) return w;
+    return -1;
+  }
+  if (LS < kGenMinLs) return templated_ok ? 0 : -1;
   if (knob == 0 && templated_ok) return 0;
   int best = -1;
   for (int w = w0; w <= kGenMaxWaves; w += w0)
@@ -846,6 +971,32 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   cx<S> *pout = (cx<S> *)out;
   const cx<S> *pU = (const cx<S> *)U;
   HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta};
+  if constexpr (std::is_same<S, float>::value) {
+    // the link products on the matrix cores: FP32 multi-RHS (Wilson, no matrix epilogue), Ls <= 16
+    if (tune_get(GX_TUNE_DHOP_MFMA) && kepi <= 1 && g.Ls >= 2 && g.Ls <= 16 && lb == g.Ls) {
+      const int Wm = general_waves(g.Ls, -1, sizeof(cx<T>), false, lb);  // smallest shape of >= 4 waves
+      if (Wm > 0) {
+        const GShape sh = make_shape(g.Ls, lb, Wm);
+        const uint32_t ntiles = (uint32_t)((g.H4 + sh.SPB - 1) / sh.SPB);
+        const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
+        const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
+        const size_t lds = (size_t)sh.SPB * kLinkPad * sizeof(cx<float>) + 2 * (size_t)sh.SPB * 2 * kMfmaRows * 16 * 4;
+        if (lds <= 65536) {
+#define GX_MFMA_GO(DAG, EPI)                                                                                     \
+  hipLaunchKernelGGL((k_dhop_mfma<DAG, EPI>), dim3(blocks), dim3(64 * Wm), lds, st, pin, pout, pU, k, cb_out, gate, \
+                     ntiles, map, flags, ep, sh)
+          if (dag) {
+            if (kepi == 1) GX_MFMA_GO(true, 1); else GX_MFMA_GO(true, 0);
+          } else {
+            if (kepi == 1) GX_MFMA_GO(false, 1); else GX_MFMA_GO(false, 0);
+          }
+#undef GX_MFMA_GO
+          GX_CHECK_HIP(hipGetLastError());
+          return GX_OK;
+        }
+      }
+    }
+  }
   const int W = general_waves(g.Ls, tune_get(GX_TUNE_DHOP_SHAPE), sizeof(cx<T>), kepi >= 2, lb);
   if (W < 0 || g.Ls % lb != 0) {
     set_error("dhop: no tile shape for Ls=%d (matrix group %d): Ls > 16 needs an odd part of at most 7", g.Ls, lb);

@@ -377,7 +377,9 @@ enum gx_tune_knob {
                                part is at most 7, W a multiple of Ls / gcd(Ls, 64), W <= 8) */
   GX_TUNE_DHOP_PERSIST = 7, /* 1: the W-wave hop (GX_TUNE_DHOP_SHAPE > 0 or Ls > 16) runs as a persistent grid of
                                resident workgroups looping over the tiles */
-  GX_TUNE_COUNT = 8
+  GX_TUNE_DHOP_MFMA = 8,    /* 1: FP32 multi-RHS hops (Ls = right-hand sides, 2..16, no matrix epilogue) do the link
+                               products on the matrix cores (v_mfma_f32_16x16x4f32) */
+  GX_TUNE_COUNT = 12
 };
 int gx_tune_set(int knob, int value);
 int gx_tune_get(int knob, int *value);

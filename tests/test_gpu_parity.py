@@ -540,3 +540,36 @@ def test_multi_rhs_wilson_large(ctx, nrhs):
         Mx = O1.apply('M', np.ascontiguousarray(x[..., k:k + 1, :, :]))
         assert rel_err(Mx, psi[..., k:k + 1, :, :]) < 1e-10, k
     D.close()
+
+
+@pytest.mark.parametrize('nrhs,dims', [(4, (8, 4, 4, 8)), (12, (8, 4, 6, 8)), (16, (4, 4, 4, 8))])
+def test_mfma_multi_rhs_hop(ctx, nrhs, dims):
+    """The FP32 multi-RHS Wilson hop with the link products on the matrix cores: DhopEO (both
+    daggers) and the fused Mpc agree with the oracle's per-right-hand-side results to the FP32
+    tolerance, and a mixed-precision multi-RHS solve with it brings every right-hand side to the
+    FP64 tolerance with the solution of the VALU kernel."""
+    mass = 0.1
+    U, psi = inputs(dims, nrhs, 37)
+    O1 = Oracle(U, dims, 1, 'wilson', mass=mass)
+    D = gx.FermionOp(ctx, U, dims, nrhs, 'wilson', mass=mass)
+    po = half(psi, dims, ODD)
+    with _Knobs(dhop_mfma=1):
+        for dag in (0, 1):
+            out = run_half(D, gx.F32, lambda i, o: D.DhopEO(i, o, dag), po, ODD)
+            for k in range(nrhs):
+                ref = O1.dhop(np.ascontiguousarray(po[..., k:k + 1, :, :]), EVEN, dag)
+                assert max_site_rel_err(out[..., k:k + 1, :, :], ref) < 1e-6, (dag, k)
+        i, o = D.field(gx.F32, ODD).upload(po), D.field(gx.F32, ODD)
+        D.Mpc(i, o)
+        mpc = o.download()
+        for k in range(nrhs):
+            ref = O1.schur('Mpc', np.ascontiguousarray(po[..., k:k + 1, :, :]))
+            assert max_site_rel_err(mpc[..., k:k + 1, :, :], ref) < 1e-5, k
+        fs, fx = D.field().upload(psi), D.field()
+        st = D.schur_solve(fs, fx, tol=1e-11, maxit=5000, mixed=1)
+        x_mfma = fx.download()
+    fx2 = D.field()
+    D.schur_solve(fs, fx2, tol=1e-11, maxit=5000, mixed=1)
+    assert st['converged'] and st['unprec_residual'] < 1e-10, st
+    assert rel_err(x_mfma, fx2.download()) < 1e-9
+    D.close()
