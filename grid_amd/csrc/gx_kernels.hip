@@ -782,8 +782,10 @@ __global__ void __launch_bounds__(64 * kGenMaxWaves)
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const uint32_t spb = sh.SPB;
   cx<float> *links = reinterpret_cast<cx<float> *>(lds_raw);
+  // the products of a (site, spin) overwrite its staged half-spinors: the wave that reads them writes
+  // them, and a lane reads back exactly the entries it staged
   float *hsl = reinterpret_cast<float *>(links + spb * kLinkPad);
-  float *dl = hsl + spb * 2 * kMfmaRows * 16;
+  float *dl = hsl;
   const bool sweep = map == 4;
   const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, sweep ? 1 : map);
   if (tile >= ntiles) return;  // uniform across the workgroup
@@ -878,7 +880,8 @@ static int general_waves(int LS, int knob, size_t csz, bool mat, int lb) {
     return w % w0 == 0 && w <= kGenMaxWaves && std::max(links, stage) <= 65536;
   };
   const bool templated_ok = LS <= kMaxLs && lb == LS;
-  if (knob > 0 && fits(knob)) return knob;
+  // the general hop stages at most kGenLinkPerThread link entries per thread: LS >= kGenMinLs
+  if (knob > 0 && LS >= kGenMinLs && fits(knob)) return knob;
   if (knob < 0) {  // the smallest whole-site shape of at least 4 waves, for any LS
     for (int w = w0; w <= kGenMaxWaves; w += w0)
       if (w >= 4 && fits(w)) return w;
@@ -980,7 +983,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
         const uint32_t ntiles = (uint32_t)((g.H4 + sh.SPB - 1) / sh.SPB);
         const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
         const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
-        const size_t lds = (size_t)sh.SPB * kLinkPad * sizeof(cx<float>) + 2 * (size_t)sh.SPB * 2 * kMfmaRows * 16 * 4;
+        const size_t lds = (size_t)sh.SPB * kLinkPad * sizeof(cx<float>) + (size_t)sh.SPB * 2 * kMfmaRows * 16 * 4;
         if (lds <= 65536) {
 #define GX_MFMA_GO(DAG, EPI)                                                                                     \
   hipLaunchKernelGGL((k_dhop_mfma<DAG, EPI>), dim3(blocks), dim3(64 * Wm), lds, st, pin, pout, pU, k, cb_out, gate, \
