@@ -119,11 +119,15 @@ def median_time(fn, calls, warm=2):
     return float(np.median(ts))
 
 
-def cpu_baseline(gx, U, psi_o):
+C2_CPU_ITERS = 40
+
+
+def cpu_baseline(gx, U, psi_o, c2_iterations=463):
     """The CPU restatement of Grid's algorithm (oracle/cpu_ref.cpp, OpenMP, -O3 -march) timed on this
     host: C2 DhopEO median over 20 calls with all physical cores (capped at the job's CPU share) and
-    with one thread; red-black solve time-to-solution of C1, C2 and C4 (one point source) with all
-    cores, at the tolerances of the GPU legs.  About 20-30 s of CPU work."""
+    with one thread; red-black solve time-to-solution of C1 and C4 (one point source) with all
+    cores, at the tolerances of the GPU legs, and of C2 projected from its first iterations.  About
+    30-40 s of CPU work."""
     from oracle import pyoracle
     nthr, cores = cpu_threads()
     H = np.prod(DIMS) // 2
@@ -138,12 +142,20 @@ def cpu_baseline(gx, U, psi_o):
                      'gflops_1344': round(1344.0 * H * LS / sec / 1e9, 3)}
     pyoracle.set_threads(nthr)
     solves = {}
-    # C2: the GPU cg leg's source and tolerance
+    # C2: the GPU cg leg's source and tolerance.  A full CPU solve takes about 80 s on 16 cores, so
+    # the sample is the first C2_CPU_ITERS iterations of the same solve, projected to the iteration
+    # count of the converged solve (463 on this system: the GPU FP64 solve, whose count the oracle's
+    # CG matches within 2 iterations).
     src = gx.gaussian_fermion(DIMS, LS, 3003)
     t0 = time.perf_counter()
-    _, st = O.schur_solve(src, tol=CG_TOL_MOBIUS, maxit=20000)
-    solves['c2_mobius_16x16x16x16_ls16'] = {'seconds': round(time.perf_counter() - t0, 3), 'iterations': st['iterations'],
-                                            'unprec_residual': st['unprec_residual'], 'cg_tol': CG_TOL_MOBIUS}
+    _, st = O.schur_solve(src, tol=CG_TOL_MOBIUS, maxit=C2_CPU_ITERS)
+    sec = time.perf_counter() - t0
+    per_it = sec / max(st['iterations'], 1)
+    solves['c2_mobius_16x16x16x16_ls16'] = {'sample_seconds': round(sec, 3), 'sample_iterations': st['iterations'],
+                                            'seconds_per_iteration': round(per_it, 5),
+                                            'iterations_to_solution': c2_iterations,
+                                            'projected_seconds': round(per_it * c2_iterations, 2),
+                                            'cg_tol': CG_TOL_MOBIUS}
     del O
     # C1: Wilson 8^4, m 0.1, eps 0.5
     d1 = (8, 8, 8, 8)
@@ -453,6 +465,18 @@ def bench_serving(n, rate, slots=4, mixed=1):
     return {k: (float(f'{v:.6g}') if isinstance(v, float) else v) for k, v in s.items()}
 
 
+T_START = time.perf_counter()
+
+
+def leg(name, fn):
+    """Run one leg, with a progress line on stderr before and after it."""
+    print(f'bench: {name} ... ({time.perf_counter() - T_START:.0f} s)', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    r = fn()
+    print(f'bench: {name} done in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -468,7 +492,7 @@ def main():
     ap.add_argument('--serve-mixed', type=int, default=2, choices=(1, 2),
                     help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
     ap.add_argument('--sweep-requests', type=int, default=384)
-    ap.add_argument('--sweep-rates', default='20,30,36')
+    ap.add_argument('--sweep-rates', default='20,30,40,50')
     ap.add_argument('--legs', default='dhop,large,cg,c3,c4,serve,sweep,graphs,cpu,ms,block',
                     help='comma list of legs to run: dhop, large (DhopEO at >= 1 GB working sets), cg, c3, c4, '
                          'serve, sweep (serving at several rates), graphs (hipGraph against eager CG), cpu, '
@@ -478,7 +502,7 @@ def main():
     ap.add_argument('--no-tune', action='store_true',
                     help='keep the library default launch knobs instead of autotuning each workload')
     ap.add_argument('--knobs', default='',
-                    help='explicit hop knobs map,pf,nt,order (implies --no-tune), e.g. 0,1,1,0')
+                    help='explicit hop knobs map,pf,nt,order[,shape,persist] (implies --no-tune), e.g. 0,1,1,0')
     args = ap.parse_args()
     legs = set(args.legs.split(','))
     args.no_cg = args.no_cg or 'cg' not in legs
@@ -499,7 +523,8 @@ def main():
     from grid_amd import autotune
     if args.knobs:
         args.no_tune = True
-        for k, v in zip((gx.TUNE_DHOP_MAP, gx.TUNE_DHOP_PF, gx.TUNE_DHOP_NT, gx.TUNE_DHOP_ORDER),
+        for k, v in zip((gx.TUNE_DHOP_MAP, gx.TUNE_DHOP_PF, gx.TUNE_DHOP_NT, gx.TUNE_DHOP_ORDER,
+                         gx.TUNE_DHOP_SHAPE, gx.TUNE_DHOP_PERSIST),
                         (int(x) for x in args.knobs.split(','))):
             gx.tune_set(k, v)
     tuner = None if args.no_tune else autotune.Autotuner(ctx, cache_path=None)
@@ -518,6 +543,7 @@ def main():
     H = np.prod(DIMS) // 2
     res = {}
     for prec, name, c in ((gx.F64, 'f64', 16), (gx.F32, 'f32', 8), (gx.F16, 'f16', 4)):
+        print(f'bench: dhop {name} ... ({time.perf_counter() - T_START:.0f} s)', file=sys.stderr, flush=True)
         tune('dhop_' + name, D, prec)
         fi = D.field(prec, 1).upload(psi_o)
         fo = D.field(prec, 0)
@@ -538,6 +564,7 @@ def main():
 
     cg = None
     if not args.no_cg:
+        print(f'bench: cg ... ({time.perf_counter() - T_START:.0f} s)', file=sys.stderr, flush=True)
         tune('cg', D, gx.F32)
         src_full = gx.gaussian_fermion(DIMS, LS, 3003)
         fs, fsol = D.field().upload(src_full), D.field()
@@ -557,27 +584,31 @@ def main():
                                  rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))),
                   speedup_mixed_vs_fp64=st64['seconds'] / stmx['seconds'])
 
-    c3 = None if args.no_c3 else bench_mixed_dwf(gx, ctx, tune=tune)
-    c4 = None if args.no_c4 else bench_multi_rhs(gx, ctx, tune=tune)
-    serve = bench_serving(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_mixed) if args.serve_requests > 0 else None
-    sweep = (bench_serve_sweep(args.sweep_requests, [float(r) for r in args.sweep_rates.split(',')], args.serve_slots,
-                               args.serve_mixed) if 'sweep' in legs and args.sweep_requests > 0 else None)
-    graphs = bench_graphs(gx, ctx) if 'graphs' in legs else None
-    ms = bench_multishift(gx, ctx) if 'ms' in legs else None
-    blk = bench_block_cgrq(gx, ctx) if 'block' in legs else None
+    c3 = None if args.no_c3 else leg('c3', lambda: bench_mixed_dwf(gx, ctx, tune=tune))
+    c4 = None if args.no_c4 else leg('c4', lambda: bench_multi_rhs(gx, ctx, tune=tune))
+    serve = (leg('serve', lambda: bench_serving(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_mixed))
+             if args.serve_requests > 0 else None)
+    sweep = (leg('sweep', lambda: bench_serve_sweep(args.sweep_requests, [float(r) for r in args.sweep_rates.split(',')],
+                                                    args.serve_slots, args.serve_mixed))
+             if 'sweep' in legs and args.sweep_requests > 0 else None)
+    graphs = leg('graphs', lambda: bench_graphs(gx, ctx)) if 'graphs' in legs else None
+    ms = leg('ms', lambda: bench_multishift(gx, ctx)) if 'ms' in legs else None
+    blk = leg('block', lambda: bench_block_cgrq(gx, ctx)) if 'block' in legs else None
 
-    large = bench_dhop_large(gx, ctx, tune) if 'large' in legs else None
+    large = leg('large', lambda: bench_dhop_large(gx, ctx, tune)) if 'large' in legs else None
 
     cpu = None
     if not args.no_cpu:
-        cb, byts = cpu_baseline(gx, U, psi_o)
+        c2_its = cg['fp64']['iterations'] if cg else 463
+        cb, byts = leg('cpu', lambda: cpu_baseline(gx, U, psi_o, c2_its))
         a = cb['all_cores']
         cpu = {'value': a['gbs'], 'unit': 'GB/s', 'cores': a['threads'], 'kind': 'port',
                'sample': f'C2 DhopEO (Mobius 16^4xLs16 FP64, {byts} B per call), median of 20 calls after warm-up, '
                          f'oracle/cpu_ref.cpp (the CPU restatement of Grid\'s algorithm; Grid itself cannot be built '
                          f'here) with {a["threads"]} OpenMP threads ({cb["host_physical_cores"]} physical cores on '
-                         f'the host, capped at the job\'s CPU share), and with 1 thread; red-black solves of C1, C2 '
-                         f'and C4 (one point source) at the GPU legs\' tolerances with all threads',
+                         f'the host, capped at the job\'s CPU share), and with 1 thread; red-black solves of C1 and C4 '
+                         f'(one point source) at the GPU legs\' tolerances with all threads, and the first '
+                         f'{C2_CPU_ITERS} iterations of the C2 solve projected to its iteration count',
                **cb}
 
     r = res['f64']
