@@ -164,6 +164,7 @@ _SIG = {
     'gx_block_cgrq': ([_P, _P, _P, _D, _I, ctypes.POINTER(SolverStats)], _I),
     'gx_mixed_cg_batched': ([_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), _D, _D, _I, _I, _I, _I,
                              ctypes.POINTER(SolverStats)], _I),
+    'gx_slice_inner_product_matrix_noalloc': ([_P, _P, _DP], _I),
     'gx_inner_product_matrix': ([_I, ctypes.POINTER(_P), ctypes.POINTER(_P), _DP], _I),
     'gx_madd_matrix': ([_I, ctypes.POINTER(_P), _DP, ctypes.POINTER(_P), ctypes.POINTER(_P), _D], _I),
     'gx_block_cgrq_vec': ([_P, _I, _I, ctypes.POINTER(_P), ctypes.POINTER(_P), _D, _I, ctypes.POINTER(SolverStats)],

@@ -1507,6 +1507,43 @@ int gx_block_cgrq_vec(gx_op op_, int herm, int n, gx_field *src_, gx_field *psi_
   return block_cgrq_vec_run(op, herm, n, B.data(), X.data(), tol, maxit, st);
 }
 
+// ---------------- allocation-free slice inner-product matrix ----------------
+// Workspace of gx_slice_inner_product_matrix_noalloc: the workgroup partials (the same partition as
+// BlockWork's, so the sums match gx_slice_inner_product_matrix bit for bit) and the result matrix,
+// in a fixed device array.  Each GPU the library runs on holds its own copy until the process
+// ends; its address is looked up once per device with that device current.
+constexpr int kScratchParts = kIpmParts;
+constexpr size_t kScratchMat = (size_t)kMaxLs * kMaxLs * sizeof(double2);
+constexpr size_t kScratchBytes = (size_t)(kScratchParts + 1) * kScratchMat;  // 8.4 MB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_addr;  // per device, null until looked up
+
+static int scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0) {
+    set_error("scratch: bad device %d", device);
+    return GX_EINVAL;
+  }
+  if ((int)g_scratch_addr.size() <= device) g_scratch_addr.resize(device + 1, nullptr);
+  if (!g_scratch_addr[device]) {
+    int cur = 0;
+    GX_CHECK_HIP(hipGetDevice(&cur));
+    if (cur != device) GX_CHECK_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      set_error("scratch: hipGetSymbolAddress failed on device %d: %s", device, hipGetErrorString(e));
+      return GX_EHIP;
+    }
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return GX_OK;
+}
+
 static bool slice_conformable(const Field *a, const Field *b) {
   return a && b && a->prec == GX_F64 && b->prec == GX_F64 && same_geom(a->g, b->g) &&
          (a->subset == GX_FULL) == (b->subset == GX_FULL);
@@ -1525,6 +1562,23 @@ int gx_slice_inner_product_matrix(gx_field lhs, gx_field rhs, double *mat) {
     mat[2 * i] = m.a[i].real();
     mat[2 * i + 1] = m.a[i].imag();
   }
+  return GX_OK;
+}
+
+int gx_slice_inner_product_matrix_noalloc(gx_field lhs, gx_field rhs, double *mat) {
+  if (!slice_conformable(lhs, rhs) || !mat || lhs->g.Ls > kMaxLs) {
+    set_error("slice_inner_product_matrix_noalloc: needs two F64 fields of one lattice and subset, Ls <= %d", kMaxLs);
+    return GX_EINVAL;
+  }
+  Ctx *ctx = lhs->ctx;
+  void *base = nullptr;
+  GX_TRY(scratch_address(ctx->device, &base));
+  void *partial = base, *out_dev = (char *)base + (size_t)kScratchParts * kScratchMat;
+  const int n = lhs->g.Ls;
+  GX_TRY(launch_slice_ipm(ctx, lhs->g, lhs->subset == GX_FULL ? 2 : 1, lhs->data, rhs->data, partial, kScratchParts,
+                          out_dev));
+  GX_CHECK_HIP(hipMemcpyAsync(mat, out_dev, sizeof(double2) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return GX_OK;
 }
 

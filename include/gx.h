@@ -288,6 +288,14 @@ int gx_cg_multishift(gx_op op, gx_field src, int nshift, const double *shifts, c
  * row-major.  F64 fields of one lattice and subset. */
 int gx_slice_inner_product_matrix(gx_field lhs, gx_field rhs, double *mat);
 
+/* No counterpart in the reference (Grid allocates freely).  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * gx_slice_inner_product_matrix with its workspace (workgroup partials and the result matrix,
+ * 8.4 MB) in a fixed device array that each GPU holds until the process ends, instead of memory
+ * allocated per call.  Same result, bit for bit.  Ls <= 16.  Calls that overlap on one GPU race on
+ * that array. */
+int gx_slice_inner_product_matrix_noalloc(gx_field lhs, gx_field rhs, double *mat);
+
 /* Grid: sliceMaddMatrix(R, aa, X, Y, Orthog = 0, scale), Lattice_reduction.h:717-742:
  * R_t = Y_t + scale sum_s X_s aa(s, t).  aa: 2 Ls^2 doubles, row-major.  Y = NULL gives
  * sliceMulMatrix (R_t = scale sum_s X_s aa(s, t)).  R may be X or Y. */
