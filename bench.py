@@ -451,22 +451,27 @@ def bench_graphs(gx, ctx):
     return out
 
 
-def bench_serve_sweep(n, rates, slots=4, mixed=2):
+def bench_serve_sweep(n, rates, slots=4, mixed=2, fused_rates=(), fuse_rhs=16):
     """Leg 'sweep': the same Poisson trace shape at several arrival rates, so capacity and latency
-    are reported together (p50/p99 grow once the rate passes the GPU's service rate)."""
-    return {f'{r:g}_per_s': bench_serving(n, r, slots, mixed) for r in rates}
+    are reported together (p50/p99 grow once the rate passes the GPU's service rate); at
+    `fused_rates` also with fused solves (waiting requests of one lattice and gauge field join an
+    admitted request's solve, up to fuse_rhs right-hand sides, a mass per slice)."""
+    out = {f'{r:g}_per_s': bench_serving(n, r, slots, mixed) for r in rates}
+    for r in fused_rates:
+        out[f'{r:g}_per_s_fused{fuse_rhs}'] = bench_serving(n, r, slots, mixed, fuse_rhs=fuse_rhs)
+    return out
 
 
-def bench_serving(n, rate, slots=4, mixed=1):
+def bench_serving(n, rate, slots=4, mixed=1, fuse_rhs=0):
     """Poisson trace of n Wilson solve requests at `rate` per second through the serving runtime
-    (mixed = 1: FP32 inner solves, 2: FP16-storage inner solves)."""
+    (mixed = 1: FP32 inner solves, 2: FP16-storage inner solves; fuse_rhs > 0: fused solves)."""
     from grid_amd import serving
     trace = serving.make_trace(n, rate, seed=5000)
     res, info = serving.serve(trace, slots=slots, budget=128e9, n_gauge=8, gauge_eps=0.3, tol=CG_TOL, mixed=mixed,
-                              progress=max(64, n // 16))
+                              progress=max(64, n // 16), fuse_rhs=fuse_rhs)
     s = serving.summarize(res, info)
     s['meets_target'] = bool(s['max_residual'] <= TARGET_RESIDUAL)
-    s.update({'arrival_rate_per_s': rate, 'slots': slots, 'admission_budget_bytes': 128e9,
+    s.update({'arrival_rate_per_s': rate, 'slots': slots, 'admission_budget_bytes': 128e9, 'fuse_rhs': fuse_rhs,
               'inner_storage': {1: 'fp32', 2: 'fp16'}[mixed],
               'mix': 'volumes 8^4,12^4,16^4,16^3x32,24^4; 1-12 Gaussian sources; m in [0.05,0.3]; mixed CG to 1e-11 (|b - M x|/|b| <= 1e-10)'})
     return {k: (float(f'{v:.6g}') if isinstance(v, float) else v) for k, v in s.items()}
@@ -500,6 +505,8 @@ def main():
                     help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
     ap.add_argument('--sweep-requests', type=int, default=384)
     ap.add_argument('--sweep-rates', default='20,30,40,50')
+    ap.add_argument('--sweep-fused-rates', default='40,50',
+                    help='rates at which the sweep also serves with fused solves (16 right-hand sides)')
     ap.add_argument('--legs', default='dhop,large,cg,c3,c4,serve,sweep,graphs,cpu,ms,block',
                     help='comma list of legs to run: dhop, large (DhopEO at >= 1 GB working sets), cg, c3, c4, '
                          'serve, sweep (serving at several rates), graphs (hipGraph against eager CG), cpu, '
@@ -596,7 +603,8 @@ def main():
     serve = (leg('serve', lambda: bench_serving(args.serve_requests, args.serve_rate, args.serve_slots, args.serve_mixed))
              if args.serve_requests > 0 else None)
     sweep = (leg('sweep', lambda: bench_serve_sweep(args.sweep_requests, [float(r) for r in args.sweep_rates.split(',')],
-                                                    args.serve_slots, args.serve_mixed))
+                                                    args.serve_slots, args.serve_mixed,
+                                                    [float(r) for r in args.sweep_fused_rates.split(',') if r]))
              if 'sweep' in legs and args.sweep_requests > 0 else None)
     graphs = leg('graphs', lambda: bench_graphs(gx, ctx)) if 'graphs' in legs else None
     ms = leg('ms', lambda: bench_multishift(gx, ctx)) if 'ms' in legs else None

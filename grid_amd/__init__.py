@@ -81,7 +81,8 @@ class Response(ctypes.Structure):
     _fields_ = [('status', ctypes.c_int), ('slot', ctypes.c_int), ('iterations', ctypes.c_int),
                 ('outer_iterations', ctypes.c_int), ('residual', ctypes.c_double),
                 ('solver_residual', ctypes.c_double), ('t_admit', ctypes.c_double),
-                ('t_done', ctypes.c_double), ('footprint', ctypes.c_double), ('digest', ctypes.c_uint64)]
+                ('t_done', ctypes.c_double), ('footprint', ctypes.c_double), ('digest', ctypes.c_uint64),
+                ('batch', ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -91,7 +92,8 @@ class ServeParams(ctypes.Structure):
     _fields_ = [('device', ctypes.c_int), ('slots', ctypes.c_int), ('budget', ctypes.c_double),
                 ('n_gauge', ctypes.c_int), ('gauge_seed', ctypes.c_uint64), ('gauge_eps', ctypes.c_double),
                 ('tol', ctypes.c_double), ('mixed', ctypes.c_int), ('maxit', ctypes.c_int),
-                ('realtime', ctypes.c_int), ('progress', ctypes.c_int), ('gauge_dir', ctypes.c_char_p)]
+                ('realtime', ctypes.c_int), ('progress', ctypes.c_int), ('gauge_dir', ctypes.c_char_p),
+                ('fuse_rhs', ctypes.c_int)]
 
 
 _lib = None
@@ -132,6 +134,8 @@ _SIG = {
     'gx_op_destroy': ([_P], _I),
     'gx_op_import_gauge': ([_P, _DP], _I),
     'gx_op_set_mass': ([_P, _D], _I),
+    'gx_op_set_slice_masses': ([_P, _I, _DP], _I),
+    'gx_op_slice_masses': ([_P, _I, _DP], _I),
     'gx_M': ([_P, _P, _P], _I),
     'gx_Mdag': ([_P, _P, _P], _I),
     'gx_Meooe': ([_P, _P, _P], _I),
@@ -190,6 +194,8 @@ _SIG = {
     'gx_op_same_action': ([_P, _P, _IP], _I),
     'gx_field_gaussian': ([_P, ctypes.c_uint64], _I),
     'gx_field_digest': ([_P, ctypes.POINTER(ctypes.c_uint64)], _I),
+    'gx_field_gaussian_slices': ([_P, ctypes.c_uint64, _I, _I], _I),
+    'gx_field_digest_slices': ([_P, _I, _I, ctypes.POINTER(ctypes.c_uint64)], _I),
     'gx_field_point_source': ([_P, _IP, _I, _I], _I),
     'gx_extract_slice': ([_P, _P, _I], _I),
     'gx_insert_slice': ([_P, _P, _I], _I),
@@ -312,6 +318,17 @@ class Field:
         _check(lib().gx_field_digest(self.h, ctypes.byref(v)))
         return v.value
 
+    def gaussian_slices(self, seed, s0, ns):
+        """Slices s0 .. s0+ns-1 get the ns-slice field gaussian(seed) would give."""
+        _check(lib().gx_field_gaussian_slices(self.h, ctypes.c_uint64(seed), int(s0), int(ns)))
+        return self
+
+    def digest_slices(self, s0, ns):
+        """Digest of slices s0 .. s0+ns-1 by logical position (full F64 fields)."""
+        v = ctypes.c_uint64()
+        _check(lib().gx_field_digest_slices(self.h, int(s0), int(ns), ctypes.byref(v)))
+        return v.value
+
     def close(self):
         if self.h:
             lib().gx_field_destroy(self.h)
@@ -353,7 +370,7 @@ def field_copy(dst, src):
 
 
 (TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER, TUNE_CG_GRAPH, TUNE_FUSED, TUNE_DHOP_SHAPE,
- TUNE_DHOP_PERSIST, TUNE_DHOP_MFMA) = 0, 1, 2, 3, 4, 5, 6, 7, 8
+ TUNE_DHOP_PERSIST, TUNE_DHOP_MFMA, TUNE_DHOP_TILE) = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
 
 
 def tune_set(knob, value):
@@ -481,6 +498,19 @@ class FermionOp:
 
     def set_mass(self, mass):
         self._call('op_set_mass', ctypes.c_double(mass))
+
+    def set_slice_masses(self, masses):
+        """Wilson multi-RHS operator: mass masses[s] on slice s (None: back to the single mass)."""
+        if masses is None:
+            self._call('op_set_slice_masses', 0, None)
+            return
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        self._call('op_set_slice_masses', len(m), _dp(m))
+
+    def slice_masses(self):
+        m = np.zeros(self.Ls)
+        self._call('op_slice_masses', self.Ls, _dp(m))
+        return m
 
     def clone(self, ctx, Ls, mass):
         """Wilson operator reading this operator's resident links, with its own context, Ls and mass."""

@@ -8,12 +8,15 @@ The knobs are libgx's process-wide launch choices (gx_tune_set):
   GX_TUNE_DHOP_SHAPE  tile shape: whole sites per wave in 4-wave workgroups, or W waves of 64
                       consecutive (site, s) entries each (sites per workgroup = 64 W / Ls)
   GX_TUNE_DHOP_PERSIST  the W-wave hop as a persistent grid of resident workgroups over the tiles
+  GX_TUNE_DHOP_TILE   tiles of 2 x 2 rows of x with the x, y and z legs' neighbours staged in LDS
   GX_TUNE_FUSED       fused Schur operator (hop epilogues) or the unfused composition
   GX_TUNE_CG_GRAPH    solver iteration batches replayed as hipGraphs
 The hop knobs are chosen by timing DhopEO on the caller's own lattice, Ls and precision (HIP
 events over repeated launches, best of several rounds), by coordinate search: map, prefetch,
-stores and leg order in the default tile shape, then the tile shape, then the map again in the
-chosen shape (with and without the persistent grid); the solver knobs by timing a fixed number of CG iterations.  Results are cached in a JSON file keyed by device and workload, so a later
+stores and leg order in the default tile shape, then the tile shape (W-wave shapes with and
+without the persistent grid, and the LDS row tiles), then the map again in the chosen shape; the
+solver knobs by timing a fixed number of CG iterations.  Results are cached in a JSON file keyed
+by device and workload, so a later
 process applies them without measuring.  No Grid counterpart: Grid fixes its launch shape when it
 is compiled (Grid/threads/Accelerator.h:416-461).
 """
@@ -26,13 +29,14 @@ import time
 import numpy as np
 
 from . import (EVEN, F16, F32, F64, ODD, TUNE_CG_GRAPH, TUNE_DHOP_MAP, TUNE_DHOP_NT, TUNE_DHOP_ORDER,
-               TUNE_DHOP_PERSIST, TUNE_DHOP_PF, TUNE_DHOP_SHAPE, TUNE_FUSED, tune_get, tune_set)
+               TUNE_DHOP_PERSIST, TUNE_DHOP_PF, TUNE_DHOP_SHAPE, TUNE_DHOP_TILE, TUNE_FUSED, tune_get, tune_set)
 
 HOP_KNOBS = (TUNE_DHOP_MAP, TUNE_DHOP_PF, TUNE_DHOP_NT, TUNE_DHOP_ORDER)
 SOLVER_KNOBS = (TUNE_FUSED, TUNE_CG_GRAPH)
 KNOB_NAMES = {TUNE_DHOP_MAP: 'dhop_map', TUNE_DHOP_PF: 'dhop_pf', TUNE_DHOP_NT: 'dhop_nt',
               TUNE_DHOP_ORDER: 'dhop_order', TUNE_FUSED: 'fused', TUNE_CG_GRAPH: 'cg_graph',
-              TUNE_DHOP_SHAPE: 'dhop_shape', TUNE_DHOP_PERSIST: 'dhop_persist'}
+              TUNE_DHOP_SHAPE: 'dhop_shape', TUNE_DHOP_PERSIST: 'dhop_persist', TUNE_DHOP_TILE: 'dhop_tile'}
+LAUNCH_KNOBS = (TUNE_DHOP_SHAPE, TUNE_DHOP_PERSIST, TUNE_DHOP_TILE)
 COMPLEX_BYTES = {F64: 16, F32: 8, F16: 8}   # arithmetic complex of a precision (FP16 storage computes in FP32)
 PREC_NAMES = {F64: 'f64', F32: 'f32', F16: 'f16'}
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'grid_amd', 'tune.json')
@@ -70,6 +74,15 @@ def shape_candidates(Ls, prec=F64):
             if (64 * w // Ls) * 73 * COMPLEX_BYTES[prec] <= 65536:
                 out.append(w)
             w *= 2
+    return out
+
+
+def launch_candidates(shapes):
+    """Launch forms: each tile shape (W > 0 with and without the persistent grid), and the LDS row
+    tiles (the library falls back to the W-wave or templated form where a row tile does not fit)."""
+    out = [((TUNE_DHOP_SHAPE, w), (TUNE_DHOP_PERSIST, p), (TUNE_DHOP_TILE, 0))
+           for w in shapes for p in ((0, 1) if w else (0,))]
+    out.append(((TUNE_DHOP_SHAPE, shapes[0]), (TUNE_DHOP_PERSIST, 0), (TUNE_DHOP_TILE, 1)))
     return out
 
 
@@ -157,14 +170,14 @@ class Autotuner:
                 return self._time_dhop(D, fi, fo, dhop_iters)
 
         shapes = shape_candidates(D.Ls, prec)
-        default_shape = ((TUNE_DHOP_SHAPE, shapes[0]), (TUNE_DHOP_PERSIST, 0))
+        default_shape = ((TUNE_DHOP_SHAPE, shapes[0]), (TUNE_DHOP_PERSIST, 0), (TUNE_DHOP_TILE, 0))
         first, hop_times = search([c + default_shape for c in hop_candidates(D.Ls)], hop, rounds)
-        others = tuple(kv for kv in first if kv[0] not in (TUNE_DHOP_SHAPE, TUNE_DHOP_PERSIST))
-        launch = [((TUNE_DHOP_SHAPE, w), (TUNE_DHOP_PERSIST, p)) for w in shapes for p in ((0, 1) if w else (0,))]
+        others = tuple(kv for kv in first if kv[0] not in LAUNCH_KNOBS)
+        launch = launch_candidates(shapes)
         if len(launch) > 1:
             best_shape, t2 = search([others + c for c in launch], hop, rounds)
             hop_times.update(t2)
-            chosen = tuple(kv for kv in best_shape if kv[0] in (TUNE_DHOP_SHAPE, TUNE_DHOP_PERSIST))
+            chosen = tuple(kv for kv in best_shape if kv[0] in LAUNCH_KNOBS)
             rest = tuple(kv for kv in others if kv[0] != TUNE_DHOP_MAP)
             hop_best, t3 = search([((TUNE_DHOP_MAP, m),) + rest + chosen for m in range(5)], hop, rounds)
             hop_times.update(t3)

@@ -28,7 +28,7 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
-static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0, 0};  // map pf nt order graph fused shape persist mfma
+static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0, 0, 0};  // map pf nt order graph fused shape persist mfma tile
 static int g_tune_version = 0;
 int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
 int tune_version() { return g_tune_version; }
@@ -552,6 +552,50 @@ static int upload_smats(Op *op) {
   return GX_OK;
 }
 
+// Per-slice Schur coefficients (see Op::coef_dev): built on the host in FP64, copied to the device in
+// FP64 and FP32, in place when the arrays exist (captured graphs keep their pointers valid).
+int op_upload_coefs(Op *op) {
+  const int Ls = op->g.Ls;
+  std::vector<double> c(4 * (size_t)Ls);
+  for (int s = 0; s < Ls; s++) {
+    if (op->kind == GX_WILSON) {
+      const double m = op->slice_mass.empty() ? op->mass : op->slice_mass[s];
+      const double d = 4.0 + m;  // WilsonFermionImplementation.h:71, diag_mass = 4 + m
+      c[s] = -1.0 / d;
+      c[Ls + s] = d;
+      c[2 * Ls + s] = 1.0 / d;
+      c[3 * Ls + s] = d;
+    } else {
+      c[s] = -1.0;
+      c[Ls + s] = 1.0;
+      c[2 * Ls + s] = 1.0;
+      c[3 * Ls + s] = 1.0;
+    }
+  }
+  op->coef_host = c;
+  op->coef_host_f.assign(c.begin(), c.end());
+  if (!op->coef_dev[0]) GX_CHECK_HIP(hipMalloc(&op->coef_dev[0], c.size() * sizeof(double)));
+  if (!op->coef_dev[1]) GX_CHECK_HIP(hipMalloc(&op->coef_dev[1], c.size() * sizeof(float)));
+  // copies on the operator's own stream: a synchronous copy would touch the null stream, which other
+  // host threads' graph captures (serving slots) forbid
+  hipStream_t st = op->ctx->stream;
+  GX_CHECK_HIP(hipMemcpyAsync(op->coef_dev[0], op->coef_host.data(), c.size() * sizeof(double),
+                              hipMemcpyHostToDevice, st));
+  GX_CHECK_HIP(hipMemcpyAsync(op->coef_dev[1], op->coef_host_f.data(), c.size() * sizeof(float),
+                              hipMemcpyHostToDevice, st));
+  GX_CHECK_HIP(hipStreamSynchronize(st));
+  return GX_OK;
+}
+
+int op_coef(Op *op, int prec, const void **coef) {
+  *coef = op->coef_dev[prec == GX_F64 ? 0 : 1];
+  if (!*coef) {
+    set_error("operator coefficients missing");
+    return GX_EINVAL;
+  }
+  return GX_OK;
+}
+
 // matrices in the arithmetic precision of storage prec (FP16 storage computes in float)
 int op_smat(Op *op, int prec, int which, const void **mats) {
   const size_t per = 2 * (size_t)op->g.Ls * op->g.Ls;
@@ -596,6 +640,16 @@ int apply_meooe(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
 // which: 0 Mooee, 1 MooeeDag, 2 MooeeInv, 3 MooeeInvDag
 int apply_mooee(Op *op, Field *in, Field *out, int which, const Gate *gate) {
   if (op->kind == GX_WILSON) {
+    if (!op->slice_mass.empty()) {  // d_s (Mooee) or 1/d_s (MooeeInv) on each slice
+      const void *c;
+      GX_TRY(op_coef(op, in->prec, &c));
+      const size_t off = (size_t)(which < 2 ? 3 : 2) * op->g.Ls;
+      const void *cs = in->prec == GX_F64 ? (const void *)((const double *)c + off) : (const void *)((const float *)c + off);
+      out->cb = in->cb;
+      return by_prec(in->prec, [&](auto t) {
+        return launch_scale_slices<decltype(t)>(op->g, in->data, out->data, cs, gate, op->ctx->stream);
+      });
+    }
     const double d = 4.0 + op->mass;
     return blas_axpby(out, which < 2 ? d : 1.0 / d, in, 0.0, nullptr, gate);
   }
@@ -620,10 +674,12 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
   const Geom &g = op->g;
   Field *t0 = op_tmp(op, prec, 0), *t1 = op_tmp(op, prec, 1), *t3 = op_tmp(op, prec, 3);
   if (!t0 || !t1 || !t3) return GX_ENOMEM;
+  const void *coef;  // per-slice (alpha_s, beta_s) of the EPI 1 hops: Wilson (-1/d_s, d_s), Cayley (-1, 1)
+  GX_TRY(op_coef(op, prec, &coef));
   if (op->kind == GX_WILSON) {
-    const double d = 4.0 + op->mass;
     GX_TRY(launch_dhop_epi<T>(g, in->data, t0->data, U_mid, cb_mid, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0));
-    GX_TRY(launch_dhop_epi<T>(g, t0->data, out->data, U_in, cb_in, dag, gate, st, 1, in->data, nullptr, -1.0 / d, d));
+    GX_TRY(launch_dhop_epi<T>(g, t0->data, out->data, U_in, cb_in, dag, gate, st, 1, in->data, nullptr, 0.0, 0.0, 0,
+                              coef));
   } else if (!dag) {
     const void *m_meo, *m_mooee, *m_p1;
     GX_TRY(op_smat(op, prec, Op::S_MEOOE5D, &m_meo));
@@ -632,7 +688,8 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
     // t0 = Meooe5D in, t1 = Mooee in (one read of in)
     GX_TRY(launch_smat2<T>(g, in->data, nullptr, 0.0, t0->data, m_meo, t1->data, m_mooee, gate, st));
     GX_TRY(launch_dhop_epi<T>(g, t0->data, t3->data, U_mid, cb_mid, 0, gate, st, 2, nullptr, m_p1, 1.0, 0.0));
-    GX_TRY(launch_dhop_epi<T>(g, t3->data, out->data, U_in, cb_in, 0, gate, st, 1, t1->data, nullptr, -1.0, 1.0));
+    GX_TRY(launch_dhop_epi<T>(g, t3->data, out->data, U_in, cb_in, 0, gate, st, 1, t1->data, nullptr, 0.0, 0.0, 0,
+                              coef));
   } else {
     const void *m_mooeedag, *m_p2, *m_meodag;
     GX_TRY(op_smat(op, prec, Op::S_MOOEEDAG, &m_mooeedag));
@@ -809,7 +866,7 @@ int gx_tune_set(int knob, int value) {
   if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 4, GX_EINVAL, "tile map takes 0..4");
   else if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   else if (knob == GX_TUNE_DHOP_SHAPE) GX_REQUIRE(value >= 0 && value <= 8, GX_EINVAL, "tile shape takes 0..8 waves");
-  else if (knob == GX_TUNE_DHOP_PERSIST || knob == GX_TUNE_DHOP_MFMA)
+  else if (knob == GX_TUNE_DHOP_PERSIST || knob == GX_TUNE_DHOP_MFMA || knob == GX_TUNE_DHOP_TILE)
     GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   g_tune[knob] = value;
   g_tune_version++;
@@ -993,18 +1050,25 @@ int gx_field_import(gx_field f, const double *host) {
   return GX_OK;
 }
 
-int gx_field_gaussian(gx_field f, uint64_t seed) {
+int gx_field_gaussian_slices(gx_field f, uint64_t seed, int s0, int ns) {
   GX_REQUIRE(f && f->subset == GX_FULL && (f->prec == GX_F64 || f->prec == GX_F32), GX_EINVAL,
              "field_gaussian: needs a full F64 or F32 field");
+  GX_REQUIRE(s0 >= 0 && ns >= 1 && s0 + ns <= f->g.Ls, GX_EINVAL, "field_gaussian: slices %d..%d outside Ls = %d",
+             s0, s0 + ns - 1, f->g.Ls);
   // the host generator's key schedule (SplitMix64 of an affine map of the seed)
   uint64_t x = seed * 0x9E6C63D0676A9A99ULL + 0xBB67AE8584CAA73BULL + 0x9E3779B97F4A7C15ULL;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
   x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
   const uint64_t key = x ^ (x >> 31);
-  if (f->prec == GX_F64) GX_TRY(launch_gaussian<double>(f->g, f->data, key, f->ctx->stream));
-  else GX_TRY(launch_gaussian<float>(f->g, f->data, key, f->ctx->stream));
+  if (f->prec == GX_F64) GX_TRY(launch_gaussian<double>(f->g, f->data, key, f->ctx->stream, s0, ns));
+  else GX_TRY(launch_gaussian<float>(f->g, f->data, key, f->ctx->stream, s0, ns));
   f->cb = 0;
   return GX_OK;
+}
+
+int gx_field_gaussian(gx_field f, uint64_t seed) {
+  GX_REQUIRE(f, GX_EINVAL, "null field");
+  return gx_field_gaussian_slices(f, seed, 0, f->g.Ls);
 }
 
 int gx_field_point_source(gx_field f, const int x[4], int spin, int colour) {
@@ -1038,6 +1102,12 @@ int gx_field_digest(gx_field f, uint64_t *digest) {
   GX_REQUIRE(f && digest, GX_EINVAL, "null argument");
   GX_REQUIRE(f->bytes % 8 == 0, GX_EINVAL, "field_digest: size not a multiple of 8 bytes");
   return launch_digest(f->ctx, f->data, (int64_t)(f->bytes / 8), digest);
+}
+
+int gx_field_digest_slices(gx_field f, int s0, int ns, uint64_t *digest) {
+  GX_REQUIRE(f && digest && f->subset == GX_FULL && f->prec == GX_F64, GX_EINVAL,
+             "field_digest_slices: needs a full F64 field");
+  return launch_digest_slices(f->ctx, f->g, f->data, s0, ns, digest);
 }
 
 int gx_field_export(gx_field f, double *host) {
@@ -1152,6 +1222,7 @@ static int op_common(gx_ctx ctx, const int dims[4], int Ls, double mass, const d
     for (int i = 0; i < 8; i++) op->phases[i] = phases[i];
   op->host_U.assign(U, U + (size_t)op->g.V4 * 4 * 18);
   op->epoch++;
+  GX_TRY(op_upload_coefs(op));
   return upload_gauge(op);
 }
 
@@ -1207,6 +1278,8 @@ int gx_op_destroy(gx_op op) {
     if (op->smat_dev[p]) (void)hipFree(op->smat_dev[p]);
     for (int i = 0; i < Op::kTmp; i++) field_destroy(op->tmp[p][i]);
   }
+  for (int p = 0; p < 2; p++)
+    if (op->coef_dev[p]) (void)hipFree(op->coef_dev[p]);
   field_destroy(op->full_tmp);
   delete op;
   return GX_OK;
@@ -1239,8 +1312,11 @@ int gx_wilson_clone(gx_op base, gx_ctx ctx, int Ls, double mass, gx_op *out) {
   op->ctx = ctx;
   op->g = make_geom(base->g.L, Ls);
   op->mass = mass;
-  const int rc = share_links(op, base);
+  int rc = op_upload_coefs(op);
+  if (rc == GX_OK) rc = share_links(op, base);
   if (rc != GX_OK) {
+    for (int p = 0; p < 2; p++)
+      if (op->coef_dev[p]) (void)hipFree(op->coef_dev[p]);
     delete op;
     return rc;
   }
@@ -1266,6 +1342,8 @@ int gx_op_footprint(gx_op op, size_t *bytes) {
     if (op->U[GX_F32][0]) b += nlink * 8;
     if (op->U[GX_F16][0]) b += nlink * 4;
   }
+  for (int p = 0; p < 2; p++)
+    if (op->coef_dev[p]) b += (size_t)4 * op->g.Ls * (p == 0 ? 8 : 4);
   for (int p = 0; p < 3; p++) {
     if (op->smat_dev[p]) b += (size_t)Op::S_COUNT * 2 * op->g.Ls * op->g.Ls * (p == GX_F64 ? 8 : 4);
     for (int i = 0; i < Op::kTmp; i++)
@@ -1281,7 +1359,7 @@ int gx_op_same_action(gx_op a, gx_op b, int *same) {
   // a clone's gauge field is its link owner's
   const Op *ga = a->link_owner ? a->link_owner : O(a), *gb = b->link_owner ? b->link_owner : O(b);
   bool eq = a->kind == b->kind && same_geom(a->g, b->g) && a->mass == b->mass && a->M5 == b->M5 && a->b == b->b &&
-            a->c == b->c && std::memcmp(a->phases, b->phases, sizeof(a->phases)) == 0;
+            a->c == b->c && a->slice_mass == b->slice_mass && std::memcmp(a->phases, b->phases, sizeof(a->phases)) == 0;
   eq = eq && (ga == gb || (ga->host_U.size() == gb->host_U.size() &&
                            std::memcmp(ga->host_U.data(), gb->host_U.data(), ga->host_U.size() * sizeof(double)) == 0 &&
                            std::memcmp(ga->phases, gb->phases, sizeof(ga->phases)) == 0));
@@ -1306,12 +1384,33 @@ int gx_op_set_mass(gx_op op, double mass) {
   GX_REQUIRE(op, GX_EINVAL, "null op");
   GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
   op->mass = mass;
+  op->slice_mass.clear();
   op->epoch++;
+  GX_TRY(op_upload_coefs(op));
   if (op->kind == GX_MOBIUS) {
     cayley_coeffs(op);
     build_smats(op);
     return upload_smats(op);
   }
+  return GX_OK;
+}
+
+int gx_op_set_slice_masses(gx_op op, int n, const double *masses) {
+  GX_REQUIRE(op && op->kind == GX_WILSON, GX_EINVAL, "set_slice_masses: needs a Wilson operator");
+  GX_REQUIRE(!masses || n == op->g.Ls, GX_EINVAL, "set_slice_masses: %d masses for Ls = %d", n, op->g.Ls);
+  if (masses)
+    for (int s = 0; s < n; s++) GX_REQUIRE(std::isfinite(masses[s]) && 4.0 + masses[s] != 0.0, GX_EINVAL,
+                                           "set_slice_masses: mass %d is %g", s, masses[s]);
+  GX_CHECK_HIP(hipStreamSynchronize(op->ctx->stream));
+  if (masses) op->slice_mass.assign(masses, masses + n);
+  else op->slice_mass.clear();
+  op->epoch++;
+  return op_upload_coefs(op);
+}
+
+int gx_op_slice_masses(gx_op op, int n, double *masses) {
+  GX_REQUIRE(op && masses && n == op->g.Ls, GX_EINVAL, "slice_masses: needs Ls = %d values", op ? op->g.Ls : 0);
+  for (int s = 0; s < n; s++) masses[s] = op->slice_mass.empty() ? op->mass : op->slice_mass[s];
   return GX_OK;
 }
 

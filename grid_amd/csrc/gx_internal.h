@@ -185,28 +185,44 @@ struct Op {
   Field *full_tmp = nullptr;    // F64 full field: the red-black solve's residual check
   SolverWork *work = nullptr;   // solver state, created on the first solve
   uint64_t epoch = 0;           // bumped whenever links, mass or matrices change (invalidates graphs)
+  // Per-slice coefficients of the Schur operator, in FP64 (coef_dev[0]) and FP32 (coef_dev[1], also
+  // FP16 storage), 4 Ls values: [0, Ls) alpha_s and [Ls, 2 Ls) beta_s of the fused Mpc epilogue
+  // (Wilson -1/d_s, d_s with d_s = 4 + m_s; Cayley -1, 1), [2 Ls, 3 Ls) 1/d_s, [3 Ls, 4 Ls) d_s.
+  // slice_mass: Wilson per-slice masses m_s (empty: every slice has `mass`).
+  std::vector<double> slice_mass;
+  void *coef_dev[2] = {};
+  std::vector<double> coef_host;   // host sources of the coef_dev copies
+  std::vector<float> coef_host_f;
   Op *link_owner = nullptr;     // clones read this operator's links and own none
   bool owns_links = true;       // false for clones: the links belong to link_owner
   int clones = 0;               // live clones reading this operator's links (guarded by the share lock)
 };
 
 // ---------------- launchers (gx_kernels.hip) ----------------
-// full field of Gaussian complex numbers with the host generator's counter numbering
+// full field of Gaussian complex numbers with the host generator's counter numbering; with ns > 0
+// only slices s0 .. s0 + ns - 1, holding the ns-slice field of the key
 template <typename T>
-int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st);
+int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st, int s0 = 0, int ns = 0);
 // order-independent 64-bit digest of nwords 64-bit words: sum over i of mix(word_i, i) mod 2^64
 int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out);
+// the same over the logical elements of slices s0 .. s0 + ns - 1 of a full FP64 field
+int launch_digest_slices(Ctx *ctx, const Geom &g, const void *full, int s0, int ns, uint64_t *out);
 // All pointers are device pointers to fields in the layout above.  S is the storage type: double,
 // float or half_t (FP16 storage with float arithmetic; its matrices and scalars are float).
 template <typename S>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st);
-// hop with an epilogue: epi 0 out = hop; 1 out = alpha hop + beta phi; 2 out = alpha A(hop) + beta phi
-// with A a per-chirality matrix applied across each site's s-row, block diagonal in groups of lb
-// consecutive slices (2*lb*lb in T; lb = 0: one group of Ls)
+// hop with an epilogue: epi 0 out = hop; 1 out_s = alpha_s hop_s + beta_s phi_s with per-slice
+// coefficients coef (device, 2 Ls in T: alpha_s then beta_s); 2 out = alpha A(hop) [+ beta phi] with A
+// a per-chirality matrix applied across each site's s-row, block diagonal in groups of lb consecutive
+// slices (2*lb*lb in T; lb = 0: one group of Ls)
 template <typename T>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
-                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb = 0);
+                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb = 0,
+                    const void *coef = nullptr);
+// out_e = c[s] in_e on the (h, s) entries of half fields (c: Ls values in the arithmetic type)
+template <typename S>
+int launch_scale_slices(const Geom &g, const void *in, void *out, const void *c, const Gate *gate, hipStream_t st);
 // out = A_chi(in) per chirality (+ phi if phi), mats = 2*lb*lb device matrix in T (groups as above)
 template <typename T>
 int launch_smat(const Geom &g, const void *in, const void *phi, void *out, const void *mats, const Gate *gate,
@@ -243,8 +259,9 @@ template <typename S>
 int launch_cg_update(Ctx *ctx, int64_t nsites, int64_t stride, void *psi, void *p, const void *r, CGState *state);
 
 // Multi-RHS (slice) kernels; a half field is viewed as rows x Ls with s innermost.
-// mode 0: d_s = Re<x_s|y_s> -> a_s = c_s / d_s (state);   mode 2: x_s -= a_s y_s, cp_s = |x_s|^2 ->
-// b_s, convergence, k += 1 (state);   mode 1: out_s = |x_s|^2;   mode 3: z_s = y_s - x_s, out_s = |z_s|^2
+// mode 0: d_s = Re<x_s|y_s> -> a_s = c_s / d_s (state);   mode 4: d_s = |x_s|^2 -> a_s (state);
+// mode 2: x_s -= a_s y_s, cp_s = |x_s|^2 -> b_s, convergence, k += 1 (state);   mode 1: out_s = |x_s|^2;
+// mode 3: z_s = y_s - x_s, out_s = |z_s|^2
 template <typename T>
 int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y, void *z, MrhsState *state,
                        double *out_dev);
@@ -305,6 +322,8 @@ bool same_geom(const Geom &a, const Geom &b);
 Field half_view(const Field *full, int cb);  // non-owning view of one checkerboard of a full field
 int op_prepare(Op *op, int prec);  // allocate lazily created resources of a precision ahead of capture
 int op_smat(Op *op, int prec, int which, const void **mats);
+int op_coef(Op *op, int prec, const void **coef);  // coef_dev of the arithmetic precision of prec
+int op_upload_coefs(Op *op);                       // (re)build coef_dev from mass / slice_mass
 int reduce_read(Ctx *ctx, int slot, int n, double *out);
 
 // Operator building blocks on half fields (in/out precision must match)

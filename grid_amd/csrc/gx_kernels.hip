@@ -131,9 +131,11 @@ __device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S>
 #pragma unroll
   for (int g = 0; g < 12 / VW; g++) load_group<S>(f + g * VW, q, g);
   if constexpr (Sto<S>::H) {
+    // rounded products (no contraction into the consumer's FMAs): a site's values are the same
+    // whether a kernel uses them at once or stages them in LDS first
     const float sc = *reinterpret_cast<const float *>(p + sidx<S, I>(i));
 #pragma unroll
-    for (int k = 0; k < 12; k++) f[k] = {f[k].r * sc, f[k].i * sc};
+    for (int k = 0; k < 12; k++) f[k] = {__fmul_rn(f[k].r, sc), __fmul_rn(f[k].i, sc)};
   }
 }
 
@@ -469,8 +471,9 @@ __device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const
 #undef LEG
 }
 
-// Epilogue of the hop.  EPI 0: out = hop; 1: out = alpha hop + beta phi; 2: out = alpha A(hop);
-// 3: out = alpha A(hop) + beta phi, with A a per-chirality Ls x Ls matrix applied across the s-row
+// Epilogue of the hop.  EPI 0: out = hop; 1: out_s = alpha_s hop_s + beta_s phi_s (per-slice
+// coefficients, so one multi-RHS launch can carry a different mass per right-hand side);
+// 2: out = alpha A(hop); 3: out = alpha A(hop) + beta phi, with A a per-chirality Ls x Ls matrix applied across the s-row
 // of each site from LDS and phi in the layout of out.  Whether phi is read is a template choice: a
 // run-time null test inside the unrolled epilogue makes the compiler spill the whole hop.
 template <typename S>
@@ -478,7 +481,8 @@ struct HopEpi {
   typedef typename Sto<S>::C T;
   const cx<S> *phi;
   const T *mat;   // EPI 2, 3: [2][LS][LS], chirality 0 = spins 0,1
-  T alpha, beta;
+  T alpha, beta;  // EPI 2, 3
+  const T *coef;  // EPI 1: per-slice coefficients, alpha_s = coef[s], beta_s = coef[LS + s]
 };
 
 // One lane per (4D half-site h, s): s fastest, 64/LS sites per wave, 4 waves per workgroup, so a
@@ -575,9 +579,10 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     }
   }
   if (EPI != 0) {
+    const T al = EPI == 1 ? ep.coef[s] : ep.alpha, be = EPI == 1 ? ep.coef[LS + s] : ep.beta;
 #pragma unroll
-    for (int k = 0; k < 12; k++) acc[k] = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
-    if (PHI) axpy_site<S>(acc, ep.beta, ep.phi, o);
+    for (int k = 0; k < 12; k++) acc[k] = {al * acc[k].r, al * acc[k].i};
+    if (PHI) axpy_site<S>(acc, be, ep.phi, o);
   }
   store_site<S>(out, acc, o, (flags & 1) != 0);
 }
@@ -673,9 +678,10 @@ __device__ __forceinline__ void hop_tile(const cx<S> *__restrict__ in, cx<S> *__
   if (valid) {
     const uint32_t o = h * LS + s;
     if (EPI != 0) {
+      const T al = EPI == 1 ? ep.coef[s] : ep.alpha, be = EPI == 1 ? ep.coef[LS + s] : ep.beta;
 #pragma unroll
-      for (int k = 0; k < 12; k++) acc[k] = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
-      if (PHI) axpy_site<S>(acc, ep.beta, ep.phi, o);
+      for (int k = 0; k < 12; k++) acc[k] = {al * acc[k].r, al * acc[k].i};
+      if (PHI) axpy_site<S>(acc, be, ep.phi, o);
     }
     store_site<S>(out, acc, o, (flags & 1) != 0);
   }
@@ -706,6 +712,164 @@ __global__ void __launch_bounds__(64 * kGenMaxWaves)
     __syncthreads();               // the previous tile's LDS reads are done
     hop_tile<S, DAG, EPI>(in, out, U, g, cb_out, tile, sweep, flags, ep, sh, lds);
   }
+}
+
+// ---------------- hop with x, y and z legs from a tile of input rows in LDS ----------------
+// Tile = the 2 x 2 rows (y0, y0 + 1) x (z0, z0 + 1) of one t-slice, every x and every slice s:
+// RW = Lxh LS entries per row and 4 RW threads, thread t on row r = t / RW (y = y0 + (r & 1),
+// z = z0 + (r >> 1)) and entry w = t % RW of it (site xh = w / LS, slice s = w % LS).  Each thread
+// first loads the input entry at its own position: the four input rows of the tile (opposite
+// checkerboard) land in LDS, one 16-byte group after another.  The x legs of every lane, and the
+// y and z legs whose neighbour row is in the tile, read their neighbour there; the t legs and the
+// y / z legs at the tile's faces load from global memory.  That is 5 global neighbour loads per
+// output entry instead of 8 (1 own + 1 y face + 1 z face + 2 t), for kernels bound by the vector
+// memory pipeline (the address and data units of each CU) rather than by HBM.  Tiles are ordered
+// t fastest, so a tile's t neighbours were loaded by the tile before it on the same XCD (map 1
+// slabs).  Same legs, leg order, link products and epilogues as k_dhop: the results are bitwise
+// those of the other tile shapes.
+struct RowTile {
+  uint32_t LS, RW, nyb;   // slices, entries per row, y blocks (Ly / 2)
+  FDiv fLS, fRW, fLt, fnyb, fQ;  // division by LS, RW, Lt, nyb, 4 Lxh (tile sites)
+};
+
+template <typename T>
+__device__ __forceinline__ void lds_load_site(cx<T> *f, const cx<T> *lin, uint32_t e, uint32_t nent) {
+  constexpr int VWL = 16 / (int)sizeof(cx<T>);
+  const cx<T> *q = lin + (size_t)e * VWL;
+#pragma unroll
+  for (int g = 0; g < 12 / VWL; g++) {
+#pragma unroll
+    for (int c = 0; c < VWL; c++) f[g * VWL + c] = q[(size_t)g * nent * VWL + c];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_store_site(cx<T> *lin, const cx<T> *f, uint32_t e, uint32_t nent) {
+  constexpr int VWL = 16 / (int)sizeof(cx<T>);
+  cx<T> *q = lin + (size_t)e * VWL;
+#pragma unroll
+  for (int g = 0; g < 12 / VWL; g++) {
+#pragma unroll
+    for (int c = 0; c < VWL; c++) q[(size_t)g * nent * VWL + c] = f[g * VWL + c];
+  }
+}
+
+// up to 1024 threads (16 waves, 128 VGPRs) for FP32 arithmetic, 512 for FP64
+template <typename S>
+constexpr int rows_max_threads() { return sizeof(typename Sto<S>::C) == 8 ? 512 : 1024; }
+
+template <typename S, bool DAG, int EPI>
+__global__ void __launch_bounds__(rows_max_threads<S>())
+    k_dhop_rows(const cx<S> *__restrict__ in, cx<S> *__restrict__ out, const cx<S> *__restrict__ U, KGeom g,
+                int cb_out, const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags, HopEpi<S> ep,
+                GShape sh, RowTile rt) {
+  typedef typename Sto<S>::C T;
+  constexpr bool MAT = EPI >= 2, PHI = (EPI & 1) != 0;
+  if (gated(gate)) return;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const uint32_t nent = 4 * rt.RW;                 // entries of the tile (= threads)
+  cx<T> *lin = reinterpret_cast<cx<T> *>(lds_raw);  // [12 / VWL groups][nent][VWL]
+  cx<T> *links = lin + (size_t)12 * nent;           // [4 Lxh sites][kLinkPad]
+  const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, map);
+  if (tile >= ntiles) return;  // uniform across the workgroup
+  const uint32_t H4 = (uint32_t)g.H4, LS = rt.LS, Lxh = (uint32_t)g.Lxh;
+  const uint32_t tt = fdiv(tile, rt.fLt), tslice = tile - tt * (uint32_t)g.L[3];
+  const uint32_t zb = fdiv(tt, rt.fnyb), yb = tt - zb * rt.nyb;
+  const uint32_t y0 = 2 * yb, z0 = 2 * zb;
+  const uint32_t t = threadIdx.x;
+  const uint32_t r = fdiv(t, rt.fRW), w = t - r * rt.RW;
+  const uint32_t xh = fdiv(w, rt.fLS), s = w - xh * LS;
+  const uint32_t y = y0 + (r & 1), z = z0 + (r >> 1);
+  const uint32_t rowh = Lxh * (y + (uint32_t)g.L[1] * (z + (uint32_t)g.L[2] * tslice));  // h of x = 0
+  const uint32_t h = rowh + xh;
+  uint32_t nb[8];
+  site_neighbours(g, h, cb_out, nb);
+  // which legs find their neighbour in the tile: x always; y and z when the neighbour row is one
+  // of the tile's (uniform over a row)
+  const uint32_t yp = y + 1 == (uint32_t)g.L[1] ? 0 : y + 1, ym = y == 0 ? g.L[1] - 1 : y - 1;
+  const uint32_t zp = z + 1 == (uint32_t)g.L[2] ? 0 : z + 1, zm = z == 0 ? g.L[2] - 1 : z - 1;
+  bool inl[8];
+  uint32_t le[8];  // tile entry of the neighbour when in the tile
+  inl[0] = inl[4] = true;
+  le[0] = r * rt.RW + (nb[0] - rowh) * LS + s;
+  le[4] = r * rt.RW + (nb[4] - rowh) * LS + s;
+  inl[1] = (yp >> 1) == yb;
+  le[1] = (r & 2u) * rt.RW + (yp & 1u) * rt.RW + w;
+  inl[5] = (ym >> 1) == yb;
+  le[5] = (r & 2u) * rt.RW + (ym & 1u) * rt.RW + w;
+  inl[2] = (zp >> 1) == zb;
+  le[2] = (r & 1u) * rt.RW + 2 * (zp & 1u) * rt.RW + w;
+  inl[6] = (zm >> 1) == zb;
+  le[6] = (r & 1u) * rt.RW + 2 * (zm & 1u) * rt.RW + w;
+  inl[3] = inl[7] = false;
+  le[3] = le[7] = 0;
+  // the tile's links: 72 entries of each of its 4 Lxh sites
+  const uint32_t nsite = 4 * Lxh;
+  for (uint32_t i = t; i < 72 * nsite; i += blockDim.x) {
+    const uint32_t pk = fdiv(i, rt.fQ), q = i - pk * nsite;
+    const uint32_t qr = fdiv(q, g.fLxh), qx = q - qr * Lxh;
+    const uint32_t hq = qx + Lxh * (y0 + (qr & 1) + (uint32_t)g.L[1] * (z0 + (qr >> 1) + (uint32_t)g.L[2] * tslice));
+    links[q * kLinkPad + pk] = ldc<S>(U, pk * H4 + hq);
+  }
+  // own input entry -> LDS; the first global leg's load in flight across the barrier
+  {
+    cx<T> f[12];
+    load_site<S>(f, in, h * LS + s);
+    lds_store_site<T>(lin, f, t, nent);
+  }
+  // neighbour index P of leg L in Grid's order (LegSel<DAG, 0, L>::P)
+  auto pidx = [](int L) -> int { return L < 4 ? ((DAG ? 0 : 4) + (L & 3)) : ((DAG ? 4 : 0) + (L & 3)); };
+  cx<T> fg[12];
+  uint32_t gmask = 0;  // legs (in leg order) that load from global memory
+#pragma unroll
+  for (int L = 0; L < 8; L++)
+    if (!inl[pidx(L)]) gmask |= 1u << L;
+  // neighbour index of the first global leg after leg L (selects, no dynamically indexed arrays)
+  auto next_nb = [&](int L, bool &any) -> uint32_t {
+    uint32_t v = 0;
+    any = false;
+#pragma unroll
+    for (int j = 7; j > L; j--)
+      if (gmask & (1u << j)) {
+        v = nb[pidx(j)];
+        any = true;
+      }
+    return v;
+  };
+  {
+    bool any;
+    const uint32_t n0 = next_nb(-1, any);
+    if (any) load_site<S>(fg, in, n0 * LS + s);
+  }
+  __syncthreads();
+  const cx<T> *ul = links + (t / LS) * kLinkPad;
+  cx<T> acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) acc[k] = {0, 0};
+#define GX_ROW_LEG(L)                                                                             \
+  {                                                                                               \
+    cx<T> f[12];                                                                                  \
+    if (gmask & (1u << L)) {                                                                      \
+      _Pragma("unroll") for (int k = 0; k < 12; k++) f[k] = fg[k];                                \
+      bool any;                                                                                   \
+      const uint32_t nn = next_nb(L, any);                                                        \
+      if (any) load_site<S>(fg, in, nn * LS + s);                                                 \
+    } else {                                                                                      \
+      lds_load_site<T>(f, lin, le[pidx(L)], nent);                                                \
+    }                                                                                             \
+    do_leg<T, S, DAG, 0, L, true>(acc, f, ul, U, h, H4);                                          \
+  }
+  GX_ROW_LEG(0) GX_ROW_LEG(1) GX_ROW_LEG(2) GX_ROW_LEG(3) GX_ROW_LEG(4) GX_ROW_LEG(5) GX_ROW_LEG(6) GX_ROW_LEG(7)
+#undef GX_ROW_LEG
+  if (MAT) group_matrix<T>(acc, lin, ep.mat, sh, t, t / LS, s, blockDim.x);
+  const uint32_t o = h * LS + s;
+  if (EPI != 0) {
+    const T al = EPI == 1 ? ep.coef[s] : ep.alpha, be = EPI == 1 ? ep.coef[LS + s] : ep.beta;
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = {al * acc[k].r, al * acc[k].i};
+    if (PHI) axpy_site<S>(acc, be, ep.phi, o);
+  }
+  store_site<S>(out, acc, o, (flags & 1) != 0);
 }
 
 // ---------------- multi-RHS Wilson hop with the link products on the matrix cores ----------------
@@ -820,9 +984,10 @@ __global__ void __launch_bounds__(64 * kGenMaxWaves)
   if (!valid) return;
   const uint32_t o = h * LS + s;
   if (EPI != 0) {
+    const float al = ep.coef[s], be = ep.coef[LS + s];  // EPI 1
 #pragma unroll
-    for (int k = 0; k < 12; k++) acc[k] = {ep.alpha * acc[k].r, ep.alpha * acc[k].i};
-    if (PHI) axpy_site<float>(acc, ep.beta, ep.phi, o);
+    for (int k = 0; k < 12; k++) acc[k] = {al * acc[k].r, al * acc[k].i};
+    if (PHI) axpy_site<float>(acc, be, ep.phi, o);
   }
   store_site<float>(out, acc, o, (flags & 1) != 0);
 }
@@ -936,7 +1101,8 @@ static GShape make_shape(int LS, int LB, int W) {
 // given] (A: mat)
 template <typename S>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
-                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb) {
+                    hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb,
+                    const void *coef) {
   typedef typename Sto<S>::C T;
   if (lb <= 0) lb = g.Ls;
   if (13 * 2 * g.half_pad() >= (int64_t)1 << 31) {
@@ -947,7 +1113,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
     set_error("dhop: matrix epilogue needs Ls >= 2");
     return GX_EINVAL;
   }
-  if ((epi == 1 && !phi) || (epi == 2 && !mat)) {
+  if ((epi == 1 && (!phi || !coef)) || (epi == 2 && !mat)) {
     set_error("dhop: epilogue operand missing");
     return GX_EINVAL;
   }
@@ -973,7 +1139,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   const cx<S> *pin = (const cx<S> *)in;
   cx<S> *pout = (cx<S> *)out;
   const cx<S> *pU = (const cx<S> *)U;
-  HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta};
+  HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta, (const T *)coef};
   if constexpr (std::is_same<S, float>::value) {
     // the link products on the matrix cores: FP32 multi-RHS (Wilson, no matrix epilogue), Ls <= 16
     if (tune_get(GX_TUNE_DHOP_MFMA) && kepi <= 1 && g.Ls >= 2 && g.Ls <= 16 && lb == g.Ls) {
@@ -998,6 +1164,46 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
           return GX_OK;
         }
       }
+    }
+  }
+  // row tiles: 2 x 2 rows of x in LDS (whole tiles only: Ly and Lz are even)
+  if (tune_get(GX_TUNE_DHOP_TILE) && g.Ls % lb == 0) {
+    const uint32_t RW = (uint32_t)(k.Lxh * g.Ls), nthr = 4 * RW;
+    const size_t lds = ((size_t)12 * nthr + (size_t)4 * k.Lxh * kLinkPad) * sizeof(cx<T>);
+    if (nthr <= (uint32_t)rows_max_threads<S>() && lds <= 160 * 1024 && g.L[1] % 2 == 0 && g.L[2] % 2 == 0) {
+      RowTile rt;
+      rt.LS = (uint32_t)g.Ls;
+      rt.RW = RW;
+      rt.nyb = (uint32_t)g.L[1] / 2;
+      rt.fLS = make_fdiv(rt.LS);
+      rt.fRW = make_fdiv(RW);
+      rt.fLt = make_fdiv((uint32_t)g.L[3]);
+      rt.fnyb = make_fdiv(rt.nyb);
+      rt.fQ = make_fdiv((uint32_t)(4 * k.Lxh));
+      const GShape sh = make_shape(g.Ls, lb, 1);  // LS and the matrix groups for the epilogue
+      const uint32_t ntiles = (uint32_t)g.L[3] * rt.nyb * (uint32_t)(g.L[2] / 2);
+      const int tmap = map == 4 ? 1 : map;  // tiles are already t-sweeps of 2 x 2 rows
+      const uint32_t pad = tmap == 1 ? 8 : (tmap == 2 ? 64 : (tmap == 3 ? 256 : 1));
+      const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
+#define GX_ROWS_GO(DAG, EPI)                                                                                    \
+  do {                                                                                                           \
+    GX_CHECK_HIP(hipFuncSetAttribute((const void *)k_dhop_rows<S, DAG, EPI>,                                   \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                    \
+    hipLaunchKernelGGL((k_dhop_rows<S, DAG, EPI>), dim3(blocks), dim3(nthr), lds, st, pin, pout, pU, k, cb_out, \
+                       gate, ntiles, tmap, flags, ep, sh, rt);                                                   \
+  } while (0)
+#define GX_ROWS_EPI(EPI) \
+  if (dag) GX_ROWS_GO(true, EPI); else GX_ROWS_GO(false, EPI)
+      switch (kepi) {
+        case 0: GX_ROWS_EPI(0); break;
+        case 1: GX_ROWS_EPI(1); break;
+        case 2: GX_ROWS_EPI(2); break;
+        default: GX_ROWS_EPI(3); break;
+      }
+#undef GX_ROWS_EPI
+#undef GX_ROWS_GO
+      GX_CHECK_HIP(hipGetLastError());
+      return GX_OK;
     }
   }
   const int W = general_waves(g.Ls, tune_get(GX_TUNE_DHOP_SHAPE), sizeof(cx<T>), kepi >= 2, lb);
@@ -1082,7 +1288,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
 template <typename S>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st) {
-  return launch_dhop_epi<S>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0, 0);
+  return launch_dhop_epi<S>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0, 0, nullptr);
 }
 
 // out_s = sum_t A_chi[s][t] in_t (+ beta phi_s with PHI), and with OUT2 also
@@ -1273,6 +1479,34 @@ int launch_axpby(int64_t nsites, int64_t stride, double a, const void *x, double
   const int64_t work = Sto<S>::H ? nsites : 12 * nsites;
   hipLaunchKernelGGL((k_axpby<S>), dim3(stream_blocks(work)), dim3(256), 0, st, nsites, stride, (T)a,
                      (const cx<S> *)x, (T)b, (const cx<S> *)y, (cx<S> *)z, gate);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+// out_e = c[e % Ls] in_e over the (h, s) entries of a half field: a diagonal operator with one
+// coefficient per fifth-dimension slice (padding entries are left as they are: zero)
+template <typename S>
+__global__ void __launch_bounds__(256) k_scale_slices(int64_t nent, int Ls, const cx<S> *__restrict__ in,
+                                                      cx<S> *__restrict__ out, const typename Sto<S>::C *__restrict__ c,
+                                                      const Gate *__restrict__ gate) {
+  typedef typename Sto<S>::C T;
+  if (gated(gate)) return;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nent; e += (int64_t)gridDim.x * blockDim.x) {
+    const T a = c[e % Ls];
+    cx<T> v[12];
+    load_site<S>(v, in, e);
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = {a * v[k].r, a * v[k].i};
+    store_site<S>(out, v, e, false);
+  }
+}
+
+template <typename S>
+int launch_scale_slices(const Geom &g, const void *in, void *out, const void *c, const Gate *gate, hipStream_t st) {
+  typedef typename Sto<S>::C T;
+  const int64_t nent = g.H4 * g.Ls;
+  hipLaunchKernelGGL((k_scale_slices<S>), dim3(stream_blocks(nent)), dim3(256), 0, st, nent, g.Ls, (const cx<S> *)in,
+                     (cx<S> *)out, (const T *)c, gate);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
@@ -1620,7 +1854,7 @@ __global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<S> *
                                                      const cx<S> *__restrict__ y, cx<S> *__restrict__ z,
                                                      const MrhsState *st, double *partial) {
   typedef typename Sto<S>::C T;
-  if ((MODE == 0 || MODE == 2) && mgated(st)) return;
+  if ((MODE == 0 || MODE == 2 || MODE == 4) && mgated(st)) return;
   __shared__ double sh[256];
   const int rpb = 256 / Ls;
   const int s = threadIdx.x % Ls;
@@ -1632,12 +1866,12 @@ __global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<S> *
       const int64_t e = h * Ls + s;
       cx<T> xv[12], yv[12];
       load_site<S>(xv, x, e);
-      if (MODE != 1) load_site<S>(yv, y, e);
+      if (MODE != 1 && MODE != 4) load_site<S>(yv, y, e);
 #pragma unroll
       for (int k = 0; k < 12; k++) {
         if (MODE == 0) {
           acc += (double)fma(xv[k].r, yv[k].r, xv[k].i * yv[k].i);
-        } else if (MODE == 1) {
+        } else if (MODE == 1 || MODE == 4) {
           acc += (double)fma(xv[k].r, xv[k].r, xv[k].i * xv[k].i);
         } else if (MODE == 2) {
           xv[k] = {fma(a, yv[k].r, xv[k].r), fma(a, yv[k].i, xv[k].i)};
@@ -1660,11 +1894,12 @@ __global__ void __launch_bounds__(256) k_mrhs_reduce(int64_t H4, int Ls, cx<S> *
   }
 }
 
-// which: 0 -> d_s, a_s;  2 -> cp_s, b_s, convergence, k += 1;  1, 3 -> out[s] = sum
+// which: 0, 4 -> d_s, a_s;  2 -> cp_s, b_s, convergence, k += 1;  1, 3 -> out[s] = sum
 // 256 threads: thread t adds the partials b = t/Ls, t/Ls + 256/Ls, ... of slice t % Ls, and the
 // per-thread sums of each slice are then added in thread order (fixed order, so deterministic).
 static __global__ void __launch_bounds__(256) k_mrhs_fin(const double *partial, int nparts, int Ls, MrhsState *st,
                                                   double *out, int which) {
+  if (which == 4) which = 0;
   const bool skip = (which == 0 || which == 2) && mgated(st);
   if (skip && which == 0) return;
   __shared__ double sh[256];
@@ -1764,6 +1999,7 @@ int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y
     case 1: GX_MR(1); break;
     case 2: GX_MR(2); break;
     case 3: GX_MR(3); break;
+    case 4: GX_MR(4); break;
     default: set_error("mrhs reduce mode %d", mode); return GX_EINVAL;
   }
 #undef GX_MR
@@ -1840,17 +2076,20 @@ __device__ __forceinline__ double d_u01(uint64_t key, uint64_t ctr) {
   return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-// Element (cb, h, s, k) of a full field gets the Box-Muller pair of counter (lex * Ls + s) * 12 + k,
-// lex the lexicographic 4D site: the numbering of the host generator gx_gaussian_fermion, so the
-// two agree up to the rounding of log, sqrt, sin and cos.
+// Element (cb, h, s0 + s, k) of a full field, s < ns, gets the Box-Muller pair of counter
+// (lex * ns + s) * 12 + k, lex the lexicographic 4D site: the numbering of the host generator
+// gx_gaussian_fermion for a field of ns slices, so the two agree up to the rounding of log, sqrt,
+// sin and cos, and slices s0 .. s0 + ns - 1 of a wider field hold the ns-slice field of the seed.
 template <typename T>
-__global__ void __launch_bounds__(256) k_gaussian(cx<T> *__restrict__ out, KGeom g, int Ls, uint64_t key) {
-  const int64_t per_cb = g.H4 * Ls;  // entries per checkerboard; g.stride = the padded count
+__global__ void __launch_bounds__(256) k_gaussian(cx<T> *__restrict__ out, KGeom g, int Ls, int s0, int ns,
+                                                  uint64_t key) {
+  const int64_t per_cb = g.H4 * ns;  // generated entries per checkerboard; g.stride = the padded count
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < 2 * per_cb;
        j += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t cb = j / per_cb, rem = j - cb * per_cb;
-    const int64_t h = rem / Ls;
-    const int s = (int)(rem - h * Ls);
+    const int64_t cb = j / per_cb, rel = j - cb * per_cb;
+    const int64_t h = rel / ns;
+    const int s = (int)(rel - h * ns);
+    const int64_t rem = h * Ls + s0 + s;  // entry of the stored field
     int64_t r = h;
     const int xh = (int)(r % g.Lxh);
     r /= g.Lxh;
@@ -1860,7 +2099,7 @@ __global__ void __launch_bounds__(256) k_gaussian(cx<T> *__restrict__ out, KGeom
     const int t = (int)(r / g.L[2]);
     const int x = 2 * xh + (int)((cb + y + z + t) & 1);
     const int64_t lex = x + (int64_t)g.L[0] * (y + (int64_t)g.L[1] * (z + (int64_t)g.L[2] * t));
-    const uint64_t c0 = (uint64_t)(lex * Ls + s) * 12;
+    const uint64_t c0 = (uint64_t)(lex * ns + s) * 12;
     cx<T> *o = out + cb * 12 * g.stride;
 #pragma unroll 2
     for (int k = 0; k < 12; k++) {
@@ -1874,12 +2113,44 @@ __global__ void __launch_bounds__(256) k_gaussian(cx<T> *__restrict__ out, KGeom
 }
 
 template <typename T>
-int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st) {
+int launch_gaussian(const Geom &g, void *full, uint64_t key, hipStream_t st, int s0, int ns) {
+  if (ns <= 0) ns = g.Ls;
+  if (s0 < 0 || s0 + ns > g.Ls) {
+    set_error("gaussian: slices %d..%d outside Ls = %d", s0, s0 + ns - 1, g.Ls);
+    return GX_EINVAL;
+  }
   const KGeom k = kgeom(g);
-  hipLaunchKernelGGL((k_gaussian<T>), dim3(stream_blocks(2 * g.H4 * g.Ls)), dim3(256), 0, st, (cx<T> *)full, k, g.Ls,
-                     key);
+  hipLaunchKernelGGL((k_gaussian<T>), dim3(stream_blocks(2 * g.H4 * ns)), dim3(256), 0, st, (cx<T> *)full, k, g.Ls,
+                     s0, ns, key);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
+}
+
+// digest of slices s0 .. s0 + ns - 1 of a full FP64 field over its logical elements: word i =
+// (((cb * H4 + h) * ns + s - s0) * 12 + k) * 2 + part, so the slices of a wider field and the
+// ns-slice field holding the same values have the same digest
+static __global__ void __launch_bounds__(256) k_digest_slices(const cx<double> *__restrict__ f, int64_t H4,
+                                                              int64_t stride, int Ls, int s0, int ns,
+                                                              unsigned long long *out) {
+  uint64_t acc = 0;
+  const int64_t n = 2 * H4 * ns;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cb = j / (H4 * ns), rel = j - cb * H4 * ns, h = rel / ns;
+    const int s = (int)(rel - h * ns);
+    const cx<double> *half = f + cb * 12 * stride;
+    const int64_t e = h * Ls + s0 + s;
+    for (int k = 0; k < 12; k++) {
+      const cx<double> v = half[bidx<double>(e, k)];
+      const uint64_t i = (uint64_t)(j * 12 + k) * 2;
+      acc += d_splitmix64(__double_as_longlong(v.r) ^ d_splitmix64(i));
+      acc += d_splitmix64(__double_as_longlong(v.i) ^ d_splitmix64(i + 1));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  __shared__ uint64_t part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
 // digest = sum_i splitmix64(word_i ^ splitmix64(i)) mod 2^64: integer addition is exact and
@@ -2395,11 +2666,27 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
   std::memcpy(out, ctx->pinned, sizeof(uint64_t));
   return GX_OK;
 }
+
+int launch_digest_slices(Ctx *ctx, const Geom &g, const void *full, int s0, int ns, uint64_t *out) {
+  if (s0 < 0 || ns < 1 || s0 + ns > g.Ls) {
+    set_error("digest: slices %d..%d outside Ls = %d", s0, s0 + ns - 1, g.Ls);
+    return GX_EINVAL;
+  }
+  GX_CHECK_HIP(hipMemsetAsync(ctx->digest, 0, sizeof(unsigned long long), ctx->stream));
+  hipLaunchKernelGGL(k_digest_slices, dim3(stream_blocks(2 * g.H4 * ns)), dim3(256), 0, ctx->stream,
+                     (const cx<double> *)full, g.H4, g.half_pad(), g.Ls, s0, ns, ctx->digest);
+  GX_CHECK_HIP(hipGetLastError());
+  GX_CHECK_HIP(hipMemcpyAsync(ctx->pinned, ctx->digest, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  GX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(out, ctx->pinned, sizeof(uint64_t));
+  return GX_OK;
+}
 #endif
 
 #define GX_INST(S)                                                                                                 \
   template int launch_dhop_epi<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,       \
-                                  hipStream_t, int, const void *, const void *, double, double, int);             \
+                                  hipStream_t, int, const void *, const void *, double, double, int, const void *); \
   template int launch_smat2<S>(const Geom &, const void *, const void *, double, void *, const void *, void *,     \
                                const void *, const Gate *, hipStream_t, int);                                    \
   template int launch_dhop<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,           \
@@ -2408,6 +2695,7 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
                               hipStream_t, int);                                                                   \
   template int launch_axpby<S>(int64_t, int64_t, double, const void *, double, const void *, void *, const Gate *, \
                                hipStream_t);                                                                       \
+  template int launch_scale_slices<S>(const Geom &, const void *, void *, const void *, const Gate *, hipStream_t); \
   template int launch_reduce<S>(Ctx *, int, int64_t, int64_t, const void *, const void *, void *, double, int);   \
   template int launch_cg_dot<S>(Ctx *, int64_t, int64_t, const void *, const void *, CGState *);                   \
   template int launch_cg_axpy_norm<S>(Ctx *, int64_t, int64_t, void *, const void *, CGState *);                   \
@@ -2417,7 +2705,7 @@ int launch_digest(Ctx *ctx, const void *data, int64_t nwords, uint64_t *out) {
                                     int);                                                                          \
   template int launch_mrhs_reduce<T>(Ctx *, int, const Geom &, void *, const void *, void *, MrhsState *, double *); \
   template int launch_mrhs_update<T>(Ctx *, const Geom &, void *, void *, const void *, MrhsState *);             \
-  template int launch_gaussian<T>(const Geom &, void *, uint64_t, hipStream_t);
+  template int launch_gaussian<T>(const Geom &, void *, uint64_t, hipStream_t, int, int);
 #if GX_PART == 0
 GX_INST(double)
 GX_INST_REAL(double)

@@ -43,20 +43,22 @@ struct Lattice {
 // operator temporaries 0-3, CG vectors 4-6, the red-black halves 10-14, the residual check's
 // full field (2), source and solution (4), and with mixed precision the FP64 outer vectors 7-8
 // plus nine inner-precision halves: FP32 (mixed = 1, 4.5 FP64 halves) or FP16 storage (mixed =
-// 2, 52 of every 192 bytes: 12 binary16 pairs and the scale per entry).
+// 2, 52 of every 192 bytes: 12 binary16 pairs and the scale per entry), and the operator's
+// per-slice Schur coefficients.
 double footprint(const int d[4], int nsrc, int mixed) {
   const double hb = (double)d[0] * d[1] * d[2] * d[3] / 2 * nsrc * 12 * 16;
-  if (mixed == 2) return hb * (20.0 + 9.0 * 52.0 / 192.0);
-  return hb * (mixed ? 24.5 : 18.0);
+  const double coef = 48.0 * nsrc;  // the operator's per-slice coefficients (4 doubles and 4 floats)
+  if (mixed == 2) return hb * (20.0 + 9.0 * 52.0 / 192.0) + coef;
+  return hb * (mixed ? 24.5 : 18.0) + coef;
 }
 
 struct Entry {
   Lattice lat{};
-  int nsrc = 0;
+  int nsrc = 0;             // right-hand sides of the cached operator and fields
   gx_op op = nullptr;
   gx_field src = nullptr, sol = nullptr;
   gx_op base = nullptr;     // links the clone currently reads
-  double mass = -1;
+  std::vector<double> masses;  // mass of every slice the operator was last set to
   double bytes = 0;
   uint64_t last_use = 0;
 };
@@ -65,8 +67,10 @@ struct Slot {
   gx_ctx ctx = nullptr;
   std::thread th;
   std::vector<Entry> cache;
-  int req = -1;             // assigned request, -1 when idle
-  double reserve = 0;       // bytes reserved for a new cache entry of the assigned request
+  int req = -1;             // first request of the assigned solve, -1 when idle
+  std::vector<int> batch;   // the requests of the assigned solve (req first), slices in this order
+  int nsrc = 0;             // right-hand sides of the assigned solve
+  double reserve = 0;       // bytes reserved for a new cache entry of the assigned solve
 };
 
 void destroy_entry(Entry &e) {
@@ -97,31 +101,37 @@ struct Server {
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-  // One request on slot k: find or build the cached operator and fields, solve, digest.
-  int run(int k, int r) {
+  // The assigned solve of slot k: its requests' sources side by side in the slices of one
+  // multi-RHS field (request b at slices off_b .. off_b + nsrc_b - 1, mass m_b on those slices),
+  // one red-black solve, then each request's statistics, digest and solution.
+  int run(int k) {
     Slot &s = slots[k];
-    const gx_request &q = req[r];
+    const gx_request &q = req[s.batch[0]];
     Lattice lat;
     std::memcpy(lat.d, q.dims, sizeof(lat.d));
     gx_op base = bases.at({lat, q.gauge});
+    const int N = s.nsrc;
+    std::vector<double> masses;
+    for (int r : s.batch)
+      for (int j = 0; j < req[r].nsrc; j++) masses.push_back(req[r].mass);
     Entry *e = nullptr;
     for (auto &c : s.cache)
-      if (c.lat == lat && c.nsrc == q.nsrc) e = &c;
+      if (c.lat == lat && c.nsrc == N) e = &c;
     int rc = GX_OK;
     if (!e) {
       Entry ne;
       ne.lat = lat;
-      ne.nsrc = q.nsrc;
+      ne.nsrc = N;
       ne.bytes = s.reserve;
-      rc = gx_wilson_clone(base, s.ctx, q.nsrc, q.mass, &ne.op);
-      if (rc == GX_OK) rc = gx_field_create(s.ctx, q.dims, q.nsrc, GX_F64, GX_FULL, &ne.src);
-      if (rc == GX_OK) rc = gx_field_create(s.ctx, q.dims, q.nsrc, GX_F64, GX_FULL, &ne.sol);
+      rc = gx_wilson_clone(base, s.ctx, N, q.mass, &ne.op);
+      if (rc == GX_OK) rc = gx_field_create(s.ctx, q.dims, N, GX_F64, GX_FULL, &ne.src);
+      if (rc == GX_OK) rc = gx_field_create(s.ctx, q.dims, N, GX_F64, GX_FULL, &ne.sol);
       if (rc != GX_OK) {
         destroy_entry(ne);
         return rc;
       }
       ne.base = base;
-      ne.mass = q.mass;
+      ne.masses.assign(N, q.mass);
       s.cache.push_back(ne);
       e = &s.cache.back();
     }
@@ -129,22 +139,48 @@ struct Server {
       if ((rc = gx_op_share_links(e->op, base))) return rc;
       e->base = base;
     }
-    if (e->mass != q.mass) {
-      if ((rc = gx_op_set_mass(e->op, q.mass))) return rc;
-      e->mass = q.mass;
+    if (e->masses != masses) {
+      bool uniform = true;
+      for (double m : masses) uniform = uniform && m == masses[0];
+      rc = uniform ? gx_op_set_mass(e->op, masses[0]) : gx_op_set_slice_masses(e->op, N, masses.data());
+      if (rc) return rc;
+      e->masses = masses;
     }
-    if ((rc = gx_field_gaussian(e->src, q.seed))) return rc;
+    int off = 0;
+    for (int r : s.batch) {
+      if ((rc = gx_field_gaussian_slices(e->src, req[r].seed, off, req[r].nsrc))) return rc;
+      off += req[r].nsrc;
+    }
     gx_solver_stats st{};
     double res = 0;
     const int src = gx_schur_solve(e->op, e->src, e->sol, p.tol, p.maxit, p.mixed, &st, &res);
     if (src != GX_OK && src != GX_ENOCONV) return src;
-    gx_response &o = resp[r];
-    o.iterations = st.iterations;
-    o.outer_iterations = st.outer_iterations;
-    o.residual = res;
-    o.solver_residual = st.true_residual;
-    if ((rc = gx_field_digest(e->sol, &o.digest))) return rc;
-    if (solutions && solutions[r] && (rc = gx_field_export(e->sol, solutions[r]))) return rc;
+    std::vector<double> all;  // the whole solution on the host, when a fused request wants its own
+    const size_t V = (size_t)q.dims[0] * q.dims[1] * q.dims[2] * q.dims[3];
+    off = 0;
+    for (int r : s.batch) {
+      const int n = req[r].nsrc;
+      gx_response &o = resp[r];
+      o.iterations = st.iterations;
+      o.outer_iterations = st.outer_iterations;
+      o.residual = res;
+      o.solver_residual = st.true_residual;
+      o.batch = (int)s.batch.size();
+      if ((rc = gx_field_digest_slices(e->sol, off, n, &o.digest))) return rc;
+      if (solutions && solutions[r]) {
+        if (s.batch.size() == 1) {
+          if ((rc = gx_field_export(e->sol, solutions[r]))) return rc;
+        } else {
+          if (all.empty()) {
+            all.resize(V * N * 24);
+            if ((rc = gx_field_export(e->sol, all.data()))) return rc;
+          }
+          for (size_t x = 0; x < V; x++)
+            std::memcpy(solutions[r] + x * n * 24, all.data() + (x * N + off) * 24, sizeof(double) * 24 * n);
+        }
+      }
+      off += n;
+    }
     return src;
   }
 
@@ -158,19 +194,23 @@ struct Server {
       const int r = s.req;
       const size_t before = s.cache.size();
       lk.unlock();
-      const int rc = run(k, r);
+      const int rc = run(k);
       lk.lock();
-      resp[r].status = rc;
-      resp[r].slot = k;
-      resp[r].t_done = now();
+      const double t = now();
+      for (int b : s.batch) {
+        resp[b].status = rc;
+        resp[b].slot = k;
+        resp[b].t_done = t;
+        resp[b].batch = (int)s.batch.size();
+      }
       if (s.cache.size() == before) in_use -= s.reserve;  // nothing new was cached
       for (auto &c : s.cache)
-        if (c.lat == Lattice{{req[r].dims[0], req[r].dims[1], req[r].dims[2], req[r].dims[3]}} &&
-            c.nsrc == req[r].nsrc)
+        if (c.lat == Lattice{{req[r].dims[0], req[r].dims[1], req[r].dims[2], req[r].dims[3]}} && c.nsrc == s.nsrc)
           c.last_use = ++tick;
       s.reserve = 0;
       s.req = -1;
-      note_done();
+      for (size_t i = 0; i < s.batch.size(); i++) note_done();
+      s.batch.clear();
       cv_disp.notify_all();
     }
   }
@@ -225,6 +265,19 @@ struct Server {
         const int r = pending.front();
         const gx_request &q = req[r];
         const Lattice lat{{q.dims[0], q.dims[1], q.dims[2], q.dims[3]}};
+        // with fusion, waiting requests of the same lattice and gauge field join the head's solve
+        // (FIFO order among them) while the right-hand sides stay within fuse_rhs
+        std::vector<int> batch{r};
+        int N = q.nsrc;
+        if (p.fuse_rhs > 0)
+          for (size_t i = 1; i < pending.size(); i++) {
+            const gx_request &o = req[pending[i]];
+            if (std::memcmp(o.dims, q.dims, sizeof(q.dims)) == 0 && o.gauge == q.gauge && N + o.nsrc <= p.fuse_rhs &&
+                footprint(q.dims, N + o.nsrc, p.mixed) <= p.budget) {
+              batch.push_back(pending[i]);
+              N += o.nsrc;
+            }
+          }
         // an idle slot that has this shape cached needs no new memory; otherwise the idle slot
         // with the least cached memory takes it
         int k = -1;
@@ -235,7 +288,7 @@ struct Server {
           double mine = 0;
           for (auto &c : slots[j].cache) {
             mine += c.bytes;
-            if (c.lat == lat && c.nsrc == q.nsrc) hit = &c;
+            if (c.lat == lat && c.nsrc == N) hit = &c;
           }
           if (hit || k < 0 || mine < least) {
             k = j;
@@ -243,8 +296,8 @@ struct Server {
           }
         }
         if (k < 0) break;  // no idle slot
-        const double need = hit ? 0 : footprint(q.dims, q.nsrc, p.mixed);
-        if (need > p.budget) {  // can never fit: refuse it
+        const double need = hit ? 0 : footprint(q.dims, N, p.mixed);
+        if (need > p.budget) {  // the head alone can never fit (members join only within the budget): refuse it
           resp[r].status = GX_ENOMEM;
           resp[r].slot = -1;
           resp[r].footprint = need;
@@ -256,10 +309,17 @@ struct Server {
         if (in_use + need > p.budget) evict_for(need, k, hit);
         if (in_use + need > p.budget) break;  // wait for running solves to finish
         in_use += need;
-        slots[k].reserve = need;
-        slots[k].req = r;
-        resp[r].t_admit = t;
-        resp[r].footprint = hit ? hit->bytes : need;
+        Slot &sl = slots[k];
+        sl.reserve = need;
+        sl.batch = batch;
+        sl.nsrc = N;
+        sl.req = r;
+        for (int b : batch) {
+          resp[b].t_admit = t;
+          resp[b].footprint = (hit ? hit->bytes : need) * req[b].nsrc / N;
+        }
+        for (size_t i = batch.size(); i-- > 1;)
+          pending.erase(std::find(pending.begin(), pending.end(), batch[i]));
         pending.pop_front();
         cv_slot.notify_all();
       }
@@ -294,8 +354,9 @@ int gx_serve_footprint(const int dims[4], int nsrc, int mixed, double *bytes) {
 
 int gx_serve(const gx_serve_params *p, const gx_request *req, int n, gx_response *resp, double **solutions,
              double *wall, double *resident) {
-  if (!p || !resp || !wall || n < 0 || (n > 0 && !req) || p->slots < 1 || p->n_gauge < 1 || p->budget <= 0) {
-    gx::set_error("serve: bad parameters (slots, n_gauge and budget must be positive)");
+  if (!p || !resp || !wall || n < 0 || (n > 0 && !req) || p->slots < 1 || p->n_gauge < 1 || p->budget <= 0 ||
+      p->fuse_rhs < 0 || p->fuse_rhs > 64) {
+    gx::set_error("serve: bad parameters (slots, n_gauge and budget must be positive, fuse_rhs 0..64)");
     return GX_EINVAL;
   }
   for (int i = 0; i < n; i++) {

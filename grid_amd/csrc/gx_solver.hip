@@ -517,10 +517,20 @@ static int slice_reduce(Op *op, int mode, Field *x, Field *y, Field *z, double *
   return GX_OK;
 }
 
+// One CGmultiRHS iteration.  d_s = Re<p_s, MpcDag Mpc p_s> as in Grid (BlockConjugateGradient.h:474-478)
+// for FP64; the FP32 / FP16-storage inner solves of defect correction take the same quantity as
+// |Mpc p_s|^2 from the intermediate t = Mpc p that MpcDagMpc leaves in temporary 2, one field read
+// instead of two (equal in exact arithmetic since MpcDag is the adjoint of Mpc).
 template <typename T>
 static int mrhs_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, MrhsState *st) {
   GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
-  GX_TRY(launch_mrhs_reduce<T>(op->ctx, 0, op->g, p->data, mmp->data, nullptr, st, nullptr));
+  if (std::is_same<T, double>::value) {
+    GX_TRY(launch_mrhs_reduce<T>(op->ctx, 0, op->g, p->data, mmp->data, nullptr, st, nullptr));
+  } else {
+    Field *t = op_tmp(op, p->prec, 2);
+    if (!t) return GX_ENOMEM;
+    GX_TRY(launch_mrhs_reduce<T>(op->ctx, 4, op->g, t->data, nullptr, nullptr, st, nullptr));
+  }
   GX_TRY(launch_mrhs_reduce<T>(op->ctx, 2, op->g, r->data, mmp->data, nullptr, st, nullptr));
   GX_TRY(launch_mrhs_update<T>(op->ctx, op->g, psi->data, p->data, r->data, st));
   return GX_OK;
@@ -768,7 +778,8 @@ static int mixed_batched_run(Op *op, int n, Field **src, Field **sol, double tol
   }
   Op *base = op->link_owner ? op->link_owner : op;
   gx_op cl = nullptr;
-  GX_TRY(gx_wilson_clone(static_cast<gx_op>(base), static_cast<gx_ctx>(op->ctx), n, op->mass, &cl));
+  const double mass = op->slice_mass.empty() ? op->mass : op->slice_mass[0];  // Ls = 1
+  GX_TRY(gx_wilson_clone(static_cast<gx_op>(base), static_cast<gx_ctx>(op->ctx), n, mass, &cl));
   Op *B = static_cast<Op *>(cl);
   Field *ps = nullptr, *px = nullptr;
   int rc = field_create(op->ctx, B->g, GX_F64, src[0]->cb, &ps);

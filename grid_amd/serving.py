@@ -45,12 +45,14 @@ def footprint(dims, nsrc, mixed=True):
 
 
 def serve(reqs, slots=4, budget=64e9, n_gauge=8, gauge_seed=1000, gauge_eps=0.3, tol=1e-10, mixed=True,
-          maxit=20000, realtime=True, device=0, want_solutions=False, progress=0, gauge_dir=None):
+          maxit=20000, realtime=True, device=0, want_solutions=False, progress=0, gauge_dir=None, fuse_rhs=0):
     """Run a trace through gx_serve.  Returns (responses, info): one dict per request (status,
     slot, iterations, residual, t_admit, t_done, footprint, digest, latency) and info with the
     wall time and resident link bytes.  With want_solutions the FP64 solutions come back as host
     arrays (Lt, Lz, Ly, Lx, nsrc, 4, 3) in responses[i]['solution'].  progress = N > 0 makes the
-    runtime print a line on stderr after every N completed requests."""
+    runtime print a line on stderr after every N completed requests.  fuse_rhs = N > 0 lets waiting
+    requests of the same lattice and gauge field join an admitted request's solve (at most N
+    right-hand sides, a mass per slice); responses[i]['batch'] counts the requests of its solve."""
     n = len(reqs)
     arr = (Request * max(n, 1))()
     for i, r in enumerate(reqs):
@@ -60,7 +62,8 @@ def serve(reqs, slots=4, budget=64e9, n_gauge=8, gauge_seed=1000, gauge_eps=0.3,
     out = (Response * max(n, 1))()
     p = ServeParams(device=device, slots=slots, budget=float(budget), n_gauge=n_gauge, gauge_seed=gauge_seed,
                     gauge_eps=gauge_eps, tol=tol, mixed=int(mixed), maxit=maxit, realtime=int(bool(realtime)),
-                    progress=int(progress), gauge_dir=None if gauge_dir is None else os.fsencode(gauge_dir))
+                    progress=int(progress), gauge_dir=None if gauge_dir is None else os.fsencode(gauge_dir),
+                    fuse_rhs=int(fuse_rhs))
     sols, ptrs = None, None
     if want_solutions:
         sols = []
@@ -96,4 +99,5 @@ def summarize(res, info):
             'rhs_per_s': nsrc / wall if wall > 0 else 0.0,
             'p50_latency_s': float(np.percentile(lat, 50)), 'p99_latency_s': float(np.percentile(lat, 99)),
             'max_residual': max((r['residual'] for r in ok), default=0.0),
+            'fused_requests': sum(1 for r in ok if r.get('batch', 1) > 1),
             'resident_link_bytes': info['resident_bytes']}

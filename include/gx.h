@@ -108,6 +108,13 @@ int gx_mobius_create(gx_ctx ctx, const int dims[4], int Ls, double mass, double 
 int gx_op_destroy(gx_op op);
 int gx_op_import_gauge(gx_op op, const double *U);           /* Grid: ImportGauge */
 int gx_op_set_mass(gx_op op, double mass);                   /* Grid: CayleyFermion5D::SetMass / new op */
+/* No Grid counterpart (a Grid operator has one mass): a multi-right-hand-side Wilson operator
+ * (Ls = Nrhs) with mass masses[s] on slice s, so solves of one gauge field at different masses
+ * share one launch per hop (WilsonFermion::Mooee = 4 + m per slice, WilsonFermionImplementation.h:71).
+ * n must equal Ls; masses = NULL returns to the operator's single mass.  gx_op_set_mass clears the
+ * per-slice masses. */
+int gx_op_set_slice_masses(gx_op op, int n, const double *masses);
+int gx_op_slice_masses(gx_op op, int n, double *masses);     /* the mass of every slice (n = Ls) */
 /* No Grid counterpart (Grid copies the gauge field into every operator, WilsonFermion.h:161-169):
  * a Wilson operator on base's 4D lattice that reads base's resident device links instead of
  * holding its own, with its own context (stream), Ls (right-hand sides) and mass, so concurrent
@@ -340,7 +347,8 @@ typedef struct {
   double solver_residual;  /* largest true residual of the red-black CG (normal equations) */
   double t_admit, t_done;  /* seconds after the start */
   double footprint;        /* bytes charged to the budget when first placed on its slot */
-  uint64_t digest;         /* gx_field_digest of the FP64 solution */
+  uint64_t digest;         /* gx_field_digest_slices of the request's FP64 solution slices */
+  int batch;               /* requests in the solve that served it (1: a solve of its own) */
 } gx_response;
 
 typedef struct {
@@ -357,6 +365,11 @@ typedef struct {
   int progress;            /* 0: silent; N > 0: one line on stderr after every N completed requests */
   const char *gauge_dir;   /* NULL: synthetic fields; else field g of lattice LxxLyxLzxLt is the NERSC
                             * file <gauge_dir>/ckpoint_lat.<Lx>x<Ly>x<Lz>x<Lt>.<g> (gx_nersc_read) */
+  int fuse_rhs;            /* 0: every request is a solve of its own.  N > 0: when a request is admitted,
+                            * waiting requests of the same lattice and gauge field join its solve (in
+                            * arrival order) while the right-hand sides stay <= N and the footprint fits:
+                            * one multi-RHS solve, each request's mass on its slices
+                            * (gx_op_set_slice_masses).  Every request still reaches tol. */
 } gx_serve_params;
 
 /* Device bytes a request of nsrc sources on lattice dims holds while it runs (fields and the
@@ -387,6 +400,8 @@ enum gx_tune_knob {
                                resident workgroups looping over the tiles */
   GX_TUNE_DHOP_MFMA = 8,    /* 1: FP32 multi-RHS hops (Ls = right-hand sides, 2..16, no matrix epilogue) do the link
                                products on the matrix cores (v_mfma_f32_16x16x4f32) */
+  GX_TUNE_DHOP_TILE = 9,    /* 1: hops in tiles of 2 x 2 rows of x staged in LDS (x, y and z legs from the tile;
+                               when 4 Lx/2 Ls threads and the tile fit a workgroup) */
   GX_TUNE_COUNT = 12
 };
 int gx_tune_set(int knob, int value);
@@ -402,6 +417,9 @@ int gx_point_source(const int dims[4], int Ls, const int x[4], int spin, int col
 /* Device-side gx_gaussian_fermion straight into a full F64 or F32 field (same numbering; equal to
  * the host field up to the rounding of log, sqrt, sin and cos). */
 int gx_field_gaussian(gx_field f, uint64_t seed);
+/* No Grid counterpart: slices s0 .. s0 + ns - 1 of a full F64 or F32 field get the ns-slice field
+ * gx_field_gaussian would give (several requests' sources side by side in one multi-RHS field). */
+int gx_field_gaussian_slices(gx_field f, uint64_t seed, int s0, int ns);
 /* Device-side gx_point_source into a full field: 1 at (x, spin, colour) in every slice s; with
  * spin = colour = -1, slice s (s < 12) gets spin-colour component s instead, the 12 point sources
  * of one site as one multi-RHS field. */
@@ -409,6 +427,10 @@ int gx_field_point_source(gx_field f, const int x[4], int spin, int colour);
 /* Order-independent 64-bit digest of a field's stored bits: equal digests <=> (with overwhelming
  * probability) bitwise equal fields.  No Grid counterpart; used to compare solutions cheaply. */
 int gx_field_digest(gx_field f, uint64_t *digest);
+/* The same over the values of slices s0 .. s0 + ns - 1 of a full F64 field, indexed by their
+ * logical position (site, slice - s0, component): a request's slices inside a wider multi-RHS
+ * solution digest like the ns-slice field holding the same values. */
+int gx_field_digest_slices(gx_field f, int s0, int ns, uint64_t *digest);
 
 /* ---- gauge configurations ----
  * Grid: WilsonLoops<Gimpl>::avgPlaquette and linkTrace (WilsonLoops.h:121-126,203-219) of a host

@@ -75,3 +75,14 @@ def test_shape_candidates_follow_the_library_rule():
         for w in at.shape_candidates(Ls):
             if w:
                 assert (64 * w) % Ls == 0 and w <= 8
+
+
+def test_launch_candidates_include_row_tiles():
+    """The shape stage tries every W-wave shape with and without the persistent grid, and the LDS
+    row tiles once (in the default tile shape)."""
+    c = at.launch_candidates([0, 3, 6])
+    d = [dict(x) for x in c]
+    assert len(c) == 6 and len(set(c)) == 6
+    assert sum(x[gx.TUNE_DHOP_TILE] for x in d) == 1
+    assert all(x[gx.TUNE_DHOP_PERSIST] == 0 for x in d if x[gx.TUNE_DHOP_SHAPE] == 0)
+    assert all(set(x) == set(at.LAUNCH_KNOBS) for x in d)

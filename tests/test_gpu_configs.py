@@ -108,14 +108,14 @@ def c2():
 
 
 @pytest.mark.parametrize('prec', [gx.F64, gx.F32])
-@pytest.mark.parametrize('tile_map', [0, 1, 2, 3, 4])
-def test_c2_operators_full_volume(ctx, c2, prec, tile_map):
+@pytest.mark.parametrize('tile_map,rows', [(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (1, 1), (4, 1)])
+def test_c2_operators_full_volume(ctx, c2, prec, tile_map, rows):
     U, _, po, _, ref = c2
-    print(f'C2 16^4xLs16 prec {prec} tile map {tile_map}', flush=True)
+    print(f'C2 16^4xLs16 prec {prec} tile map {tile_map} row tiles {rows}', flush=True)
     D = gx.FermionOp(ctx, U, C2_DIMS, C2_LS, 'mobius', mass=0.01, **MOBIUS)
     tol = TOL[prec]
     fi = D.field(prec, ODD).upload(po)
-    with Knobs(dhop_map=tile_map, dhop_pf=1, dhop_nt=1):
+    with Knobs(dhop_map=tile_map, dhop_pf=1, dhop_nt=1, dhop_tile=rows):
         for order in (0, 1):
             with Knobs(dhop_order=order):
                 for dag in (0, 1):

@@ -492,8 +492,8 @@ def test_mobius_ls24_red_black_solves(ctx):
 @pytest.mark.parametrize('prec', [gx.F64, gx.F32, gx.F16])
 def test_tile_shapes_bitwise(ctx, kind, Ls, dims, prec):
     """Every tile shape (whole sites per wave, or W waves of 64 consecutive entries, one workgroup
-    per tile or a persistent grid) and tile map computes the same legs in the same order: DhopEO,
-    Mpc and MpcDag are bitwise equal across shapes."""
+    per tile or a persistent grid, or 2 x 2 row tiles staged in LDS) and tile map computes the same
+    legs in the same order: DhopEO, Mpc and MpcDag are bitwise equal across shapes."""
     from grid_amd.autotune import shape_candidates
     U, psi = inputs(dims, Ls, 29, eps=0.3)
     kw = dict(mass=0.05, M5=1.8, b=1.5, c=0.5)
@@ -501,9 +501,10 @@ def test_tile_shapes_bitwise(ctx, kind, Ls, dims, prec):
     po = half(psi, dims, ODD)
     fi = D.field(prec, ODD).upload(po)
     ref = None
-    for w, persist in [(w, p) for w in shape_candidates(Ls, prec) for p in ((0, 1) if w else (0,))]:
+    shapes = [(w, p, 0) for w in shape_candidates(Ls, prec) for p in ((0, 1) if w else (0,))] + [(0, 0, 1)]
+    for w, persist, rows in shapes:
         for tile_map in (0, 4):
-            with _Knobs(dhop_shape=w, dhop_persist=persist, dhop_map=tile_map, dhop_order=0, fused=1):
+            with _Knobs(dhop_shape=w, dhop_persist=persist, dhop_map=tile_map, dhop_order=0, fused=1, dhop_tile=rows):
                 outs = []
                 for fn in (lambda o: D.DhopEO(fi, o, 0), lambda o: D.DhopEO(fi, o, 1), lambda o: D.Mpc(fi, o),
                            lambda o: D.MpcDag(fi, o)):
@@ -514,7 +515,7 @@ def test_tile_shapes_bitwise(ctx, kind, Ls, dims, prec):
                 ref = outs
             else:
                 for a, b in zip(outs, ref):
-                    assert np.array_equal(a, b), (w, persist, tile_map)
+                    assert np.array_equal(a, b), (w, persist, tile_map, rows)
     D.close()
 
 

@@ -214,3 +214,56 @@ def test_serve_reads_nersc_gauge(ctx, tmp_path):
         assert same == (r['gauge'] == 1), r
     with pytest.raises(gx.GxError):
         serving.serve(reqs, gauge_dir=str(tmp_path / 'missing'), **kw)
+
+
+@pytest.mark.parametrize('mixed', [1, 2])
+def test_serve_fuses_masses(ctx, mixed):
+    """fuse_rhs: requests waiting for the same lattice and gauge field join one multi-RHS solve
+    with a mass per slice.  Every request reaches the tolerance; a fused request's solution agrees
+    with the one it gets alone within the solve tolerance's reach (the fused solve stops when its
+    hardest right-hand side converges, so the two differ in the last iterations); a request served
+    in a solve of its own gives bitwise its alone result; a fused solution satisfies the oracle's
+    M of its own mass."""
+    dims = (8, 4, 4, 8)
+    rng = np.random.default_rng(7)
+    reqs = [dict(dims=dims, nsrc=int(rng.integers(1, 5)), mass=float(rng.uniform(0.05, 0.3)),
+                 gauge=int(i % 3 == 2), seed=700 + i, arrival=0.0) for i in range(9)]
+    kw = dict(n_gauge=2, gauge_seed=930, gauge_eps=0.4, tol=1e-10, mixed=mixed, realtime=False, want_solutions=True)
+    mix, _ = serving.serve(reqs, slots=2, budget=4e9, fuse_rhs=12, **kw)
+    assert all(r['status'] == 0 and r['residual'] < 1e-9 for r in mix), [(r['status'], r['residual']) for r in mix]
+    assert any(r['batch'] > 1 for r in mix)
+    for i, r in enumerate(reqs):
+        alone, _ = serving.serve([r], slots=1, budget=4e9, **kw)
+        assert alone[0]['status'] == 0 and alone[0]['batch'] == 1
+        if mix[i]['batch'] == 1:
+            assert alone[0]['digest'] == mix[i]['digest'], i
+        else:
+            assert rel_err(mix[i]['solution'], alone[0]['solution']) < 1e-7, i
+    i = next(i for i, m in enumerate(mix) if m['batch'] > 1)
+    r = reqs[i]
+    O = Oracle(gx.random_gauge(dims, 930 + r['gauge'], 0.4), dims, 1, 'wilson', mass=r['mass'])
+    src = gx.gaussian_fermion(dims, r['nsrc'], r['seed'])
+    for k in range(r['nsrc']):
+        b = np.ascontiguousarray(src[..., k:k + 1, :, :])
+        x = np.ascontiguousarray(mix[i]['solution'][..., k:k + 1, :, :])
+        assert np.linalg.norm(O.apply('M', x) - b) / np.linalg.norm(b) < 1e-9
+
+
+def test_slice_sources_and_digests(ctx):
+    """gaussian_slices puts the n-slice field of a seed into slices s0.. of a wider field (bitwise
+    the narrow field's values); digest_slices of those slices equals the narrow field's slice
+    digest and changes when one value in them changes."""
+    dims = (4, 6, 4, 8)
+    narrow = gx.Field(ctx, dims, 3).gaussian(91)
+    wide = gx.Field(ctx, dims, 7)
+    wide.zero()
+    wide.gaussian_slices(91, 2, 3)
+    w = wide.download()
+    assert np.array_equal(w[..., 2:5, :, :], narrow.download())
+    assert not np.any(w[..., [0, 1, 5, 6], :, :])
+    assert wide.digest_slices(2, 3) == narrow.digest_slices(0, 3)
+    assert wide.digest_slices(1, 3) != narrow.digest_slices(0, 3)
+    w[0, 0, 0, 0, 3, 1, 2] += 1e-9
+    assert gx.Field(ctx, dims, 7).upload(w).digest_slices(2, 3) != narrow.digest_slices(0, 3)
+    with pytest.raises(gx.GxError):
+        wide.gaussian_slices(91, 5, 3)

@@ -171,6 +171,24 @@ def test_mobius_identities(lat8):
     assert np.vdot(po, O.schur('MpcDagMpc', po)).real > 0
 
 
+def test_fifth_dimension_ops_closed_form(lat8):
+    """The standalone fifth-dimension operators against their closed forms, independently of the
+    compositions (CayleyFermion5DImplementation.h:155-271 with Mobius bs = b, cs = c): M5D(psi, chi)
+    adds the off-diagonal part -S psi to chi (upper -1 with mass_minus at s = Ls-1, lower -1 with
+    mass_plus at s = 0); the oracle's entry point starts chi from psi, giving (1 - S) psi.
+    Meooe5D = b + c S, and the dagger forms with S^dag."""
+    dims, U, rng = lat8
+    Ls, mf, b, c = 6, 0.07, 1.5, 0.5
+    O = Oracle(U, dims, Ls, 'mobius', mass=mf, M5=1.8, b=b, c=c)
+    po = R.pick(R.gaussian_fermion(rng, dims, Ls), dims, ODD)
+    for dag, m5, meo in ((False, 'M5D', 'Meooe5D'), (True, 'M5Ddag', 'MeooeDag5D')):
+        S = R.s_shift(po, mf, dag=dag)
+        assert rel_err(O.apply(m5, po, ODD), po - S) < TOL64, m5
+        assert rel_err(O.apply(meo, po, ODD), b * po + c * S) < TOL64, meo
+    # a mistaken corner (mass on the wrong end) or a swapped chirality fails this: the two differ
+    assert rel_err(R.s_shift(po, mf, dag=True), R.s_shift(po, mf)) > 0.1
+
+
 def test_mobius_reduces_to_shamir(lat8):
     dims, U, rng = lat8
     Ls, mf, M5 = 6, 0.05, 1.8
