@@ -45,7 +45,16 @@ LS = 16
 PARAMS = dict(mass=0.01, M5=1.8, b=1.5, c=0.5)
 EPS = 0.3
 METRIC = 'Dslash GB/s (% HBM peak); CG time-to-1e-10 mixed vs FP64; solve p50/p99'
-DHOP_KERNEL = 'gx::k_dhop<double, 16, false, true, 0, 0>'
+
+
+def dhop_kernel_name(gx):
+    """Name of the kernel the FP64 DhopEO launches under the current knobs (as rocprofv3 lists it)."""
+    if gx.tune_get(gx.TUNE_DHOP_TILE):
+        return 'gx::k_dhop_rows<double, false, 0>'
+    if gx.tune_get(gx.TUNE_DHOP_SHAPE):
+        return 'gx::k_dhop_g<double, false, 0>'
+    pf = 'true' if gx.tune_get(gx.TUNE_DHOP_PF) else 'false'
+    return f'gx::k_dhop<double, {LS}, false, {pf}, {gx.tune_get(gx.TUNE_DHOP_ORDER)}, 0>'
 # Every solve must reach |b - M x| / |b| <= 1e-10 on the full (unpreconditioned) system.  The
 # red-black CG's tolerance applies to the normal equations of the Schur system, whose residual
 # runs below the full-system one by a factor the operator sets: 1.5-2x for these Wilson systems,
@@ -215,8 +224,10 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tu
     kw = dict(mass=0.005, M5=1.8, b=1.5, c=0.5)
     U = gx.random_gauge(dims, 1003, 0.3)
     D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
-    if tune:
+    if tune:  # each precision's knobs on this operator: FP64 outer, FP32 and FP16 inner
+        tune('c3_f64', D, gx.F64, rounds=2, cg_iters=32)
         tune('c3', D, gx.F32, rounds=2, cg_iters=32)
+        tune('c3_f16', D, gx.F16, rounds=2, cg_iters=32)
     src = D.field().gaussian(2003)
     out, ref = {'workload': 'Mobius 24^4 x Ls12 mf=0.005, Gaussian source', 'cg_tol': tol,
                 'target_unprec_residual': TARGET_RESIDUAL}, None
@@ -355,6 +366,8 @@ def bench_multi_rhs(gx, ctx, dims=(32, 32, 32, 64), mass=0.05, eps=0.3, tol=CG_T
         seq_res = max(seq_res, st['unprec_residual'])
         seq.append(x)
     B = base.clone(ctx, nrhs, mass)
+    if tune:  # the 12-RHS FP64 outer operator's own knobs (the inner precisions are tuned below)
+        tune('c4_f64', B, gx.F64, dhop_iters=5, rounds=2, cg_iters=8)
     sb = B.field().point_source(x0)
     t4 = base.field()
     out = {'workload': 'Wilson 32^3x64 m=0.05, 12 point sources', 'origin': list(x0),
@@ -556,9 +569,12 @@ def main():
     D = gx.FermionOp(ctx, U, DIMS, LS, 'mobius', **PARAMS)
     H = np.prod(DIMS) // 2
     res = {}
+    kernel64 = None
     for prec, name, c in ((gx.F64, 'f64', 16), (gx.F32, 'f32', 8), (gx.F16, 'f16', 4)):
         print(f'bench: dhop {name} ... ({time.perf_counter() - T_START:.0f} s)', file=sys.stderr, flush=True)
         tune('dhop_' + name, D, prec)
+        if prec == gx.F64:
+            kernel64 = dhop_kernel_name(gx)
         fi = D.field(prec, 1).upload(psi_o)
         fo = D.field(prec, 0)
         ms = time_dhop(ctx, D, fi, fo, args.steps, args.warmup)
@@ -646,7 +662,7 @@ def main():
                      'frac': round(r['gbs'] / HBM_PEAK_GBS, 4), 'traffic': traffic_from_profile(),
                      'traffic_source': 'profiles/dhop_mobius_16x16x16x16_ls16_f64_traffic.json (rocprofv3 FETCH_SIZE '
                                        'and WRITE_SIZE runs of bench.py --legs dhop, scaled by a streaming pass)',
-                     'kernel': DHOP_KERNEL, 'algorithmic_bytes_per_launch': r['bytes']},
+                     'kernel': kernel64, 'algorithmic_bytes_per_launch': r['bytes']},
         'cpu_baseline': cpu,
         'dhop_hbm_sized': large,
         'pct_hbm_peak': round(100 * r['gbs'] / HBM_PEAK_GBS, 2),

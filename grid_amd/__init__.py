@@ -185,6 +185,8 @@ _SIG = {
     'gx_reliable_update_cg_fallback': ([_P, _P, _P, _D, _I, _D, _I, _I, _D, ctypes.POINTER(SolverStats)], _I),
     'gx_tune_set': ([_I, _I], _I),
     'gx_tune_get': ([_I, _IP], _I),
+    'gx_tune_set_prec': ([_I, _I, _I], _I),
+    'gx_tune_get_prec': ([_I, _I, _IP], _I),
     'gx_random_gauge': ([_IP, ctypes.c_uint64, _D, _DP], _I),
     'gx_gaussian_fermion': ([_IP, _I, ctypes.c_uint64, _DP], _I),
     'gx_point_source': ([_IP, _I, _IP, _I, _I, _DP], _I),
@@ -373,13 +375,21 @@ def field_copy(dst, src):
  TUNE_DHOP_PERSIST, TUNE_DHOP_MFMA, TUNE_DHOP_TILE) = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
 
 
-def tune_set(knob, value):
-    _check(lib().gx_tune_set(int(knob), int(value)))
+def tune_set(knob, value, prec=None):
+    """Set a launch knob for every precision, or for one storage precision (F64, F32, F16)."""
+    if prec is None:
+        _check(lib().gx_tune_set(int(knob), int(value)))
+    else:
+        _check(lib().gx_tune_set_prec(int(knob), int(prec), int(value)))
 
 
-def tune_get(knob):
+def tune_get(knob, prec=None):
+    """A launch knob's value (of the FP64 set, or of precision prec)."""
     v = _I()
-    _check(lib().gx_tune_get(int(knob), ctypes.byref(v)))
+    if prec is None:
+        _check(lib().gx_tune_get(int(knob), ctypes.byref(v)))
+    else:
+        _check(lib().gx_tune_get_prec(int(knob), int(prec), ctypes.byref(v)))
     return v.value
 
 

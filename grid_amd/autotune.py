@@ -11,6 +11,8 @@ The knobs are libgx's process-wide launch choices (gx_tune_set):
   GX_TUNE_DHOP_TILE   tiles of 2 x 2 rows of x with the x, y and z legs' neighbours staged in LDS
   GX_TUNE_FUSED       fused Schur operator (hop epilogues) or the unfused composition
   GX_TUNE_CG_GRAPH    solver iteration batches replayed as hipGraphs
+Every knob holds one value per storage precision; tune(D, prec) measures and sets the values of
+precision prec only, so the FP64 and inner-precision operators of a mixed solve keep their own.
 The hop knobs are chosen by timing DhopEO on the caller's own lattice, Ls and precision (HIP
 events over repeated launches, best of several rounds), by coordinate search: map, prefetch,
 stores and leg order in the default tile shape, then the tile shape (W-wave shapes with and
@@ -90,21 +92,26 @@ def solver_candidates():
     return [tuple(zip(SOLVER_KNOBS, v)) for v in itertools.product((1, 0), (1, 0))]
 
 
-class Knobs:
-    """Set knobs for the duration of a with-block and restore the previous values."""
+PRECS = (F64, F32, F16)
 
-    def __init__(self, values):
+
+class Knobs:
+    """Set knobs (for every precision, or for `prec` only) for the duration of a with-block and
+    restore every precision's previous values."""
+
+    def __init__(self, values, prec=None):
         self.values = dict(values)
+        self.prec = prec
 
     def __enter__(self):
-        self.saved = {k: tune_get(k) for k in self.values}
+        self.saved = {(k, p): tune_get(k, p) for k in self.values for p in PRECS}
         for k, v in self.values.items():
-            tune_set(k, v)
+            tune_set(k, v, self.prec)
         return self
 
     def __exit__(self, *exc):
-        for k, v in self.saved.items():
-            tune_set(k, v)
+        for (k, p), v in self.saved.items():
+            tune_set(k, v, p)
 
 
 def _device_name():
@@ -166,7 +173,7 @@ class Autotuner:
         fi, fo = D.field(prec, ODD).upload(psi), D.field(prec, EVEN)
 
         def hop(c):
-            with Knobs(c):
+            with Knobs(c, prec):
                 return self._time_dhop(D, fi, fo, dhop_iters)
 
         shapes = shape_candidates(D.Ls, prec)
@@ -186,7 +193,7 @@ class Autotuner:
         fs, fx = D.field(prec, ODD).upload(psi), D.field(prec, ODD)
 
         def cg(c):
-            with Knobs(hop_best + c):
+            with Knobs(hop_best + c, prec):
                 return self._time_cg(D, fs, fx, cg_iters)
 
         solver_best, solver_times = search(solver_candidates(), cg, rounds)
@@ -198,17 +205,19 @@ class Autotuner:
         self.save()
         for f in (fi, fo, fs, fx):
             f.close()
-        return self.apply_entry(entry)
+        return self.apply_entry(entry, prec)
 
     @staticmethod
-    def apply_entry(entry):
+    def apply_entry(entry, prec=None):
+        """Set a cached entry's knobs for storage precision prec (every precision when None)."""
         byname = {v: k for k, v in KNOB_NAMES.items()}
         vals = {byname[n]: int(v) for n, v in entry['knobs'].items()}
         for k, v in vals.items():
-            tune_set(k, v)
+            tune_set(k, v, prec)
         return vals
 
     def apply(self, kind, dims, Ls, prec=F64):
-        """Apply cached knobs for a workload; returns them, or None when it was never tuned."""
+        """Apply cached knobs for a workload (to the knob set of prec); returns them, or None when it
+        was never tuned."""
         e = self.cache.get(workload_key(self.device, kind, tuple(dims), Ls, prec))
-        return self.apply_entry(e) if e else None
+        return self.apply_entry(e, prec) if e else None

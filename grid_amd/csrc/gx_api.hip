@@ -28,9 +28,16 @@ namespace gx {
 
 static thread_local char g_err[1024] = "";
 
-static int g_tune[GX_TUNE_COUNT] = {1, 1, 1, 0, 1, 1, 0, 0, 0, 0};  // map pf nt order graph fused shape persist mfma tile
+// knob values per storage precision (GX_F64, GX_F32, GX_F16): a mixed-precision solve runs its
+// FP64 and its inner operators each with the launch shape tuned for that precision
+static int g_tune[3][GX_TUNE_COUNT] = {{1, 1, 1, 0, 1, 1, 0, 0, 0, 0},   // map pf nt order graph fused shape
+                                       {1, 1, 1, 0, 1, 1, 0, 0, 0, 0},   // persist mfma tile
+                                       {1, 1, 1, 0, 1, 1, 0, 0, 0, 0}};
 static int g_tune_version = 0;
-int tune_get(int knob) { return (knob >= 0 && knob < GX_TUNE_COUNT) ? g_tune[knob] : 0; }
+int tune_get(int knob, int prec) {
+  if (knob < 0 || knob >= GX_TUNE_COUNT) return 0;
+  return g_tune[(prec >= GX_F64 && prec <= GX_F16) ? prec : GX_F64][knob];
+}
 int tune_version() { return g_tune_version; }
 
 void set_error(const char *fmt, ...) {
@@ -704,7 +711,7 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
 }
 
 int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
-  if (tune_get(GX_TUNE_FUSED))
+  if (tune_get(GX_TUNE_FUSED, in->prec))
     return by_prec(in->prec, [&](auto t) { return mpc_fused<decltype(t)>(op, in, out, dag, gate); });
   Field *t1 = op_tmp(op, in->prec, 0), *t2 = op_tmp(op, in->prec, 1);
   if (!t1 || !t2) return GX_ENOMEM;
@@ -862,20 +869,29 @@ const char *gx_last_error(void) { return g_err; }
 const char *gx_version(void) { return "gx 0.1 (gfx950)"; }
 
 int gx_tune_set(int knob, int value) {
+  for (int p = GX_F64; p <= GX_F16; p++) GX_TRY(gx_tune_set_prec(knob, p, value));
+  return GX_OK;
+}
+
+int gx_tune_set_prec(int knob, int prec, int value) {
   GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT, GX_EINVAL, "unknown tuning knob %d", knob);
+  GX_REQUIRE(prec >= GX_F64 && prec <= GX_F16, GX_EINVAL, "unknown precision %d", prec);
   if (knob == GX_TUNE_DHOP_MAP) GX_REQUIRE(value >= 0 && value <= 4, GX_EINVAL, "tile map takes 0..4");
   else if (knob <= GX_TUNE_FUSED) GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
   else if (knob == GX_TUNE_DHOP_SHAPE) GX_REQUIRE(value >= 0 && value <= 8, GX_EINVAL, "tile shape takes 0..8 waves");
   else if (knob == GX_TUNE_DHOP_PERSIST || knob == GX_TUNE_DHOP_MFMA || knob == GX_TUNE_DHOP_TILE)
     GX_REQUIRE(value == 0 || value == 1, GX_EINVAL, "knob %d takes 0 or 1", knob);
-  g_tune[knob] = value;
+  g_tune[prec][knob] = value;
   g_tune_version++;
   return GX_OK;
 }
 
-int gx_tune_get(int knob, int *value) {
+int gx_tune_get(int knob, int *value) { return gx_tune_get_prec(knob, GX_F64, value); }
+
+int gx_tune_get_prec(int knob, int prec, int *value) {
   GX_REQUIRE(knob >= 0 && knob < GX_TUNE_COUNT && value, GX_EINVAL, "unknown tuning knob %d", knob);
-  *value = g_tune[knob];
+  GX_REQUIRE(prec >= GX_F64 && prec <= GX_F16, GX_EINVAL, "unknown precision %d", prec);
+  *value = g_tune[prec][knob];
   return GX_OK;
 }
 

@@ -36,8 +36,15 @@ int by_prec(int prec, F &&f) {
   return GX_EINVAL;
 }
 
-// process-wide launch-shape knobs (gx_tune_set / gx_tune_get); the version changes on every set
-int tune_get(int knob);
+// storage precision of a storage type
+template <typename S>
+constexpr int prec_of() {
+  return sizeof(S) == 8 ? GX_F64 : (sizeof(S) == 4 ? GX_F32 : GX_F16);
+}
+
+// process-wide launch-shape knobs (gx_tune_set / gx_tune_get), one set per storage precision
+// (prec < 0: the FP64 set); the version changes on every set
+int tune_get(int knob, int prec = -1);
 int tune_version();
 
 #define GX_CHECK_HIP(call)                                                                             \

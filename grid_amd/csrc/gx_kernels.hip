@@ -123,6 +123,12 @@ __device__ __forceinline__ void load_group(cx<typename Sto<S>::C> *f, const cx<S
   }
 }
 
+// a * b rounded on its own: never contracted into an FMA with the consumer of the product
+__device__ __forceinline__ float mul_rounded(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 // The 12 components of entry i (FP16: multiplied by the site's scale), 12 / VW loads of 16 bytes.
 template <typename S, typename I>
 __device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ p, I i) {
@@ -135,7 +141,7 @@ __device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S>
     // whether a kernel uses them at once or stages them in LDS first
     const float sc = *reinterpret_cast<const float *>(p + sidx<S, I>(i));
 #pragma unroll
-    for (int k = 0; k < 12; k++) f[k] = {__fmul_rn(f[k].r, sc), __fmul_rn(f[k].i, sc)};
+    for (int k = 0; k < 12; k++) f[k] = {mul_rounded(f[k].r, sc), mul_rounded(f[k].i, sc)};
   }
 }
 
@@ -723,13 +729,21 @@ __global__ void __launch_bounds__(64 * kGenMaxWaves)
 // y and z legs whose neighbour row is in the tile, read their neighbour there; the t legs and the
 // y / z legs at the tile's faces load from global memory.  That is 5 global neighbour loads per
 // output entry instead of 8 (1 own + 1 y face + 1 z face + 2 t), for kernels bound by the vector
-// memory pipeline (the address and data units of each CU) rather than by HBM.  Tiles are ordered
-// t fastest, so a tile's t neighbours were loaded by the tile before it on the same XCD (map 1
-// slabs).  Same legs, leg order, link products and epilogues as k_dhop: the results are bitwise
-// those of the other tile shapes.
+// memory pipeline (the address and data units of each CU) rather than by HBM.  Every lane of a row
+// has the same sources, so each of the four rows runs its own compiled leg sequence.  Tiles are
+// ordered t fastest (maps 0-3), or in XCD boxes swept t-slice by t-slice (map 4).  Same legs, leg
+// order, link products and epilogues as k_dhop: the results are bitwise those of the other tile
+// shapes.  Needs Ly, Lz >= 4.
 struct RowTile {
   uint32_t LS, RW, nyb;   // slices, entries per row, y blocks (Ly / 2)
   FDiv fLS, fRW, fLt, fnyb, fQ;  // division by LS, RW, Lt, nyb, 4 Lxh (tile sites)
+  // XCD blocks (xblk = 1): the tiles split into 8 boxes of bY x bZ tile columns x tper t-slices
+  // (pY x pZ x pT boxes), one per XCD (per = ntiles / 8 tiles each, the map-1 slab of that XCD),
+  // swept t-slice by t-slice with the box's columns innermost, so a tile's y, z and t neighbour
+  // rows inside the box were loaded shortly before by the same XCD
+  int xblk;
+  uint32_t per, nc, bY, bZ, tper, pY, pZ;
+  FDiv fper, fnc, fbY, fpY, fpZ;
 };
 
 template <typename T>
@@ -758,6 +772,53 @@ __device__ __forceinline__ void lds_store_site(cx<T> *lin, const cx<T> *f, uint3
 template <typename S>
 constexpr int rows_max_threads() { return sizeof(typename Sto<S>::C) == 8 ? 512 : 1024; }
 
+// The 8 legs of one lane of a row tile whose row has y - y0 = RY and z - z0 = RZ (compile time, so
+// every leg's source is fixed: x legs and the y / z legs towards the tile's other row from LDS, the
+// t legs and the y / z legs across the tile's faces from global memory, one global leg ahead).
+template <typename T, typename S, bool DAG, int RY, int RZ>
+__device__ __forceinline__ void rows_legs(cx<T> *acc, const cx<S> *__restrict__ in, const cx<S> *__restrict__ U,
+                                          const cx<T> *lin, const cx<T> *ul, const uint32_t *nb, uint32_t rowh,
+                                          uint32_t r, uint32_t w, uint32_t RW, uint32_t nent, uint32_t LS, uint32_t s,
+                                          uint32_t h, uint32_t H4) {
+  // neighbour P of leg L (Grid's order) and whether it is in the tile
+  constexpr auto P = [](int L) constexpr { return L < 4 ? ((DAG ? 0 : 4) + (L & 3)) : ((DAG ? 4 : 0) + (L & 3)); };
+  constexpr auto in_tile = [](int p) constexpr {
+    return p == 0 || p == 4 || (p == 1 && RY == 0) || (p == 5 && RY == 1) || (p == 2 && RZ == 0) ||
+           (p == 6 && RZ == 1);
+  };
+  // tile entry of an in-tile neighbour
+  auto entry = [&](int p) -> uint32_t {
+    if (p == 0 || p == 4) return r * RW + (nb[p] - rowh) * LS + s;
+    if (p == 1 || p == 5) return (r ^ 1u) * RW + w;
+    return (r ^ 2u) * RW + w;
+  };
+  cx<T> fg[12];
+  // the first global leg's neighbour, loaded before the LDS legs that precede it
+  constexpr int G0 = !in_tile(P(0)) ? 0 : !in_tile(P(1)) ? 1 : !in_tile(P(2)) ? 2 : !in_tile(P(3)) ? 3
+                     : !in_tile(P(4)) ? 4 : !in_tile(P(5)) ? 5 : !in_tile(P(6)) ? 6 : 7;
+  load_site<S>(fg, in, nb[P(G0)] * LS + s);
+#define GX_ROW_LEG(L)                                                                                  \
+  {                                                                                                    \
+    constexpr int PL = P(L);                                                                           \
+    cx<T> f[12];                                                                                       \
+    if constexpr (in_tile(PL)) {                                                                       \
+      lds_load_site<T>(f, lin, entry(PL), nent);                                                       \
+    } else {                                                                                           \
+      _Pragma("unroll") for (int k = 0; k < 12; k++) f[k] = fg[k];                                     \
+      constexpr int NX = L >= 7 ? -1 : !in_tile(P(L + 1 > 7 ? 7 : L + 1)) ? L + 1                      \
+                       : L + 2 <= 7 && !in_tile(P(L + 2 > 7 ? 7 : L + 2)) ? L + 2                      \
+                       : L + 3 <= 7 && !in_tile(P(L + 3 > 7 ? 7 : L + 3)) ? L + 3                      \
+                       : L + 4 <= 7 && !in_tile(P(L + 4 > 7 ? 7 : L + 4)) ? L + 4                      \
+                       : L + 5 <= 7 && !in_tile(P(L + 5 > 7 ? 7 : L + 5)) ? L + 5                      \
+                       : L + 6 <= 7 && !in_tile(P(L + 6 > 7 ? 7 : L + 6)) ? L + 6 : -1;                \
+      if constexpr (NX >= 0) load_site<S>(fg, in, nb[P(NX)] * LS + s);                                 \
+    }                                                                                                  \
+    do_leg<T, S, DAG, 0, L, true>(acc, f, ul, U, h, H4);                                               \
+  }
+  GX_ROW_LEG(0) GX_ROW_LEG(1) GX_ROW_LEG(2) GX_ROW_LEG(3) GX_ROW_LEG(4) GX_ROW_LEG(5) GX_ROW_LEG(6) GX_ROW_LEG(7)
+#undef GX_ROW_LEG
+}
+
 template <typename S, bool DAG, int EPI>
 __global__ void __launch_bounds__(rows_max_threads<S>())
     k_dhop_rows(const cx<S> *__restrict__ in, cx<S> *__restrict__ out, const cx<S> *__restrict__ U, KGeom g,
@@ -773,8 +834,22 @@ __global__ void __launch_bounds__(rows_max_threads<S>())
   const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, map);
   if (tile >= ntiles) return;  // uniform across the workgroup
   const uint32_t H4 = (uint32_t)g.H4, LS = rt.LS, Lxh = (uint32_t)g.Lxh;
-  const uint32_t tt = fdiv(tile, rt.fLt), tslice = tile - tt * (uint32_t)g.L[3];
-  const uint32_t zb = fdiv(tt, rt.fnyb), yb = tt - zb * rt.nyb;
+  uint32_t yb, zb, tslice;
+  if (rt.xblk) {
+    const uint32_t xcd = fdiv(tile, rt.fper), l = tile - xcd * rt.per;
+    const uint32_t tl = fdiv(l, rt.fnc), cl = l - tl * rt.nc;
+    const uint32_t cz = fdiv(cl, rt.fbY), cy = cl - cz * rt.bY;
+    const uint32_t q = fdiv(xcd, rt.fpY), iy = xcd - q * rt.pY;
+    const uint32_t it = fdiv(q, rt.fpZ), iz = q - it * rt.pZ;
+    yb = iy * rt.bY + cy;
+    zb = iz * rt.bZ + cz;
+    tslice = it * rt.tper + tl;
+  } else {
+    const uint32_t tt = fdiv(tile, rt.fLt);
+    tslice = tile - tt * (uint32_t)g.L[3];
+    zb = fdiv(tt, rt.fnyb);
+    yb = tt - zb * rt.nyb;
+  }
   const uint32_t y0 = 2 * yb, z0 = 2 * zb;
   const uint32_t t = threadIdx.x;
   const uint32_t r = fdiv(t, rt.fRW), w = t - r * rt.RW;
@@ -782,27 +857,6 @@ __global__ void __launch_bounds__(rows_max_threads<S>())
   const uint32_t y = y0 + (r & 1), z = z0 + (r >> 1);
   const uint32_t rowh = Lxh * (y + (uint32_t)g.L[1] * (z + (uint32_t)g.L[2] * tslice));  // h of x = 0
   const uint32_t h = rowh + xh;
-  uint32_t nb[8];
-  site_neighbours(g, h, cb_out, nb);
-  // which legs find their neighbour in the tile: x always; y and z when the neighbour row is one
-  // of the tile's (uniform over a row)
-  const uint32_t yp = y + 1 == (uint32_t)g.L[1] ? 0 : y + 1, ym = y == 0 ? g.L[1] - 1 : y - 1;
-  const uint32_t zp = z + 1 == (uint32_t)g.L[2] ? 0 : z + 1, zm = z == 0 ? g.L[2] - 1 : z - 1;
-  bool inl[8];
-  uint32_t le[8];  // tile entry of the neighbour when in the tile
-  inl[0] = inl[4] = true;
-  le[0] = r * rt.RW + (nb[0] - rowh) * LS + s;
-  le[4] = r * rt.RW + (nb[4] - rowh) * LS + s;
-  inl[1] = (yp >> 1) == yb;
-  le[1] = (r & 2u) * rt.RW + (yp & 1u) * rt.RW + w;
-  inl[5] = (ym >> 1) == yb;
-  le[5] = (r & 2u) * rt.RW + (ym & 1u) * rt.RW + w;
-  inl[2] = (zp >> 1) == zb;
-  le[2] = (r & 1u) * rt.RW + 2 * (zp & 1u) * rt.RW + w;
-  inl[6] = (zm >> 1) == zb;
-  le[6] = (r & 1u) * rt.RW + 2 * (zm & 1u) * rt.RW + w;
-  inl[3] = inl[7] = false;
-  le[3] = le[7] = 0;
   // the tile's links: 72 entries of each of its 4 Lxh sites
   const uint32_t nsite = 4 * Lxh;
   for (uint32_t i = t; i < 72 * nsite; i += blockDim.x) {
@@ -811,56 +865,25 @@ __global__ void __launch_bounds__(rows_max_threads<S>())
     const uint32_t hq = qx + Lxh * (y0 + (qr & 1) + (uint32_t)g.L[1] * (z0 + (qr >> 1) + (uint32_t)g.L[2] * tslice));
     links[q * kLinkPad + pk] = ldc<S>(U, pk * H4 + hq);
   }
-  // own input entry -> LDS; the first global leg's load in flight across the barrier
+  // own input entry -> LDS
   {
     cx<T> f[12];
     load_site<S>(f, in, h * LS + s);
     lds_store_site<T>(lin, f, t, nent);
   }
-  // neighbour index P of leg L in Grid's order (LegSel<DAG, 0, L>::P)
-  auto pidx = [](int L) -> int { return L < 4 ? ((DAG ? 0 : 4) + (L & 3)) : ((DAG ? 4 : 0) + (L & 3)); };
-  cx<T> fg[12];
-  uint32_t gmask = 0;  // legs (in leg order) that load from global memory
-#pragma unroll
-  for (int L = 0; L < 8; L++)
-    if (!inl[pidx(L)]) gmask |= 1u << L;
-  // neighbour index of the first global leg after leg L (selects, no dynamically indexed arrays)
-  auto next_nb = [&](int L, bool &any) -> uint32_t {
-    uint32_t v = 0;
-    any = false;
-#pragma unroll
-    for (int j = 7; j > L; j--)
-      if (gmask & (1u << j)) {
-        v = nb[pidx(j)];
-        any = true;
-      }
-    return v;
-  };
-  {
-    bool any;
-    const uint32_t n0 = next_nb(-1, any);
-    if (any) load_site<S>(fg, in, n0 * LS + s);
-  }
+  uint32_t nb[8];
+  site_neighbours(g, h, cb_out, nb);
   __syncthreads();
   const cx<T> *ul = links + (t / LS) * kLinkPad;
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-#define GX_ROW_LEG(L)                                                                             \
-  {                                                                                               \
-    cx<T> f[12];                                                                                  \
-    if (gmask & (1u << L)) {                                                                      \
-      _Pragma("unroll") for (int k = 0; k < 12; k++) f[k] = fg[k];                                \
-      bool any;                                                                                   \
-      const uint32_t nn = next_nb(L, any);                                                        \
-      if (any) load_site<S>(fg, in, nn * LS + s);                                                 \
-    } else {                                                                                      \
-      lds_load_site<T>(f, lin, le[pidx(L)], nent);                                                \
-    }                                                                                             \
-    do_leg<T, S, DAG, 0, L, true>(acc, f, ul, U, h, H4);                                          \
+  switch (r) {  // uniform over a wave when a row is a multiple of 64 entries
+    case 0: rows_legs<T, S, DAG, 0, 0>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
+    case 1: rows_legs<T, S, DAG, 1, 0>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
+    case 2: rows_legs<T, S, DAG, 0, 1>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
+    default: rows_legs<T, S, DAG, 1, 1>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
   }
-  GX_ROW_LEG(0) GX_ROW_LEG(1) GX_ROW_LEG(2) GX_ROW_LEG(3) GX_ROW_LEG(4) GX_ROW_LEG(5) GX_ROW_LEG(6) GX_ROW_LEG(7)
-#undef GX_ROW_LEG
   if (MAT) group_matrix<T>(acc, lin, ep.mat, sh, t, t / LS, s, blockDim.x);
   const uint32_t o = h * LS + s;
   if (EPI != 0) {
@@ -1118,7 +1141,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
     return GX_EINVAL;
   }
   const int kepi = (epi == 2 && phi) ? 3 : epi;
-  int map = tune_get(GX_TUNE_DHOP_MAP);
+  int map = tune_get(GX_TUNE_DHOP_MAP, prec_of<S>());
   KGeom k = kgeom(g);
   if (map == 4) {
     // blocks of Lxh x by x bz sites per t-slice with three t-slices of the input (and the links)
@@ -1133,16 +1156,16 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
     }
     set_sweep_blocks(k, by, bz);
   }
-  const bool pf = tune_get(GX_TUNE_DHOP_PF) != 0;
-  const int flags = tune_get(GX_TUNE_DHOP_NT) ? 1 : 0;
-  const bool ord = epi == 0 && tune_get(GX_TUNE_DHOP_ORDER) != 0 && (g.Ls == 1 || g.Ls == 12 || g.Ls == 16);
+  const bool pf = tune_get(GX_TUNE_DHOP_PF, prec_of<S>()) != 0;
+  const int flags = tune_get(GX_TUNE_DHOP_NT, prec_of<S>()) ? 1 : 0;
+  const bool ord = epi == 0 && tune_get(GX_TUNE_DHOP_ORDER, prec_of<S>()) != 0 && (g.Ls == 1 || g.Ls == 12 || g.Ls == 16);
   const cx<S> *pin = (const cx<S> *)in;
   cx<S> *pout = (cx<S> *)out;
   const cx<S> *pU = (const cx<S> *)U;
   HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta, (const T *)coef};
   if constexpr (std::is_same<S, float>::value) {
     // the link products on the matrix cores: FP32 multi-RHS (Wilson, no matrix epilogue), Ls <= 16
-    if (tune_get(GX_TUNE_DHOP_MFMA) && kepi <= 1 && g.Ls >= 2 && g.Ls <= 16 && lb == g.Ls) {
+    if (tune_get(GX_TUNE_DHOP_MFMA, prec_of<S>()) && kepi <= 1 && g.Ls >= 2 && g.Ls <= 16 && lb == g.Ls) {
       const int Wm = general_waves(g.Ls, -1, sizeof(cx<T>), false, lb);  // smallest shape of >= 4 waves
       if (Wm > 0) {
         const GShape sh = make_shape(g.Ls, lb, Wm);
@@ -1167,10 +1190,10 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
     }
   }
   // row tiles: 2 x 2 rows of x in LDS (whole tiles only: Ly and Lz are even)
-  if (tune_get(GX_TUNE_DHOP_TILE) && g.Ls % lb == 0) {
+  if (tune_get(GX_TUNE_DHOP_TILE, prec_of<S>()) && g.Ls % lb == 0) {
     const uint32_t RW = (uint32_t)(k.Lxh * g.Ls), nthr = 4 * RW;
     const size_t lds = ((size_t)12 * nthr + (size_t)4 * k.Lxh * kLinkPad) * sizeof(cx<T>);
-    if (nthr <= (uint32_t)rows_max_threads<S>() && lds <= 160 * 1024 && g.L[1] % 2 == 0 && g.L[2] % 2 == 0) {
+    if (nthr <= (uint32_t)rows_max_threads<S>() && lds <= 160 * 1024 && g.L[1] >= 4 && g.L[2] >= 4) {
       RowTile rt;
       rt.LS = (uint32_t)g.Ls;
       rt.RW = RW;
@@ -1181,8 +1204,43 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
       rt.fnyb = make_fdiv(rt.nyb);
       rt.fQ = make_fdiv((uint32_t)(4 * k.Lxh));
       const GShape sh = make_shape(g.Ls, lb, 1);  // LS and the matrix groups for the epilogue
-      const uint32_t ntiles = (uint32_t)g.L[3] * rt.nyb * (uint32_t)(g.L[2] / 2);
-      const int tmap = map == 4 ? 1 : map;  // tiles are already t-sweeps of 2 x 2 rows
+      const uint32_t nzb = (uint32_t)g.L[2] / 2, ntiles = (uint32_t)g.L[3] * rt.nyb * nzb;
+      rt.xblk = 0;
+      if (map == 4) {
+        // XCD boxes: the split pY x pZ x pT = 8 (each dividing its extent) with the fewest face rows
+        // per box, counting a t face as a whole box face when two t-slices of the box's input and
+        // links exceed 3.5 MB of the XCD's L2 (then the t neighbours are not reused)
+        const double row_bytes = (double)k.Lxh * (g.Ls * 12.0 + 72.0) * sizeof(cx<S>);
+        double best = 1e30;
+        for (uint32_t pY = 1; pY <= 8; pY *= 2)
+          for (uint32_t pZ = 1; pY * pZ <= 8; pZ *= 2) {
+            const uint32_t pT = 8 / (pY * pZ);
+            if (rt.nyb % pY || nzb % pZ || (uint32_t)g.L[3] % pT) continue;
+            const double ry = 2.0 * rt.nyb / pY, rz = 2.0 * nzb / pZ, tl = (double)g.L[3] / pT;
+            const bool treuse = 2.0 * ry * rz * row_bytes <= 3.5e6;
+            const double face = (pY > 1 ? 2.0 / ry : 0.0) + (pZ > 1 ? 2.0 / rz : 0.0) +
+                                (treuse ? (pT > 1 ? 2.0 / tl : 0.0) : 2.0);
+            if (face < best) {
+              best = face;
+              rt.pY = pY;
+              rt.pZ = pZ;
+              rt.bY = rt.nyb / pY;
+              rt.bZ = nzb / pZ;
+              rt.tper = (uint32_t)g.L[3] / pT;
+              rt.xblk = 1;
+            }
+          }
+        if (rt.xblk) {
+          rt.per = ntiles / 8;
+          rt.nc = rt.bY * rt.bZ;
+          rt.fper = make_fdiv(rt.per);
+          rt.fnc = make_fdiv(rt.nc);
+          rt.fbY = make_fdiv(rt.bY);
+          rt.fpY = make_fdiv(rt.pY);
+          rt.fpZ = make_fdiv(rt.pZ);
+        }
+      }
+      const int tmap = map == 4 ? 1 : map;  // map 4: the XCD boxes above, one map-1 slab each
       const uint32_t pad = tmap == 1 ? 8 : (tmap == 2 ? 64 : (tmap == 3 ? 256 : 1));
       const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
 #define GX_ROWS_GO(DAG, EPI)                                                                                    \
@@ -1206,7 +1264,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
       return GX_OK;
     }
   }
-  const int W = general_waves(g.Ls, tune_get(GX_TUNE_DHOP_SHAPE), sizeof(cx<T>), kepi >= 2, lb);
+  const int W = general_waves(g.Ls, tune_get(GX_TUNE_DHOP_SHAPE, prec_of<S>()), sizeof(cx<T>), kepi >= 2, lb);
   if (W < 0 || g.Ls % lb != 0) {
     set_error("dhop: no tile shape for Ls=%d (matrix group %d): Ls > 16 needs an odd part of at most 7", g.Ls, lb);
     return GX_EINVAL;
@@ -1218,7 +1276,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
     const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
     const size_t lds = std::max((size_t)sh.SPB * kLinkPad, kepi >= 2 ? (size_t)6 * 64 * W : (size_t)0) * sizeof(cx<T>);
     // persistent grid: the resident capacity of the device (multiple of 8), when smaller than the tiles
-    const int persist_wg = tune_get(GX_TUNE_DHOP_PERSIST) ? resident_groups(64 * W, lds) : 0;
+    const int persist_wg = tune_get(GX_TUNE_DHOP_PERSIST, prec_of<S>()) ? resident_groups(64 * W, lds) : 0;
     const int persist = persist_wg > 0 && (uint32_t)persist_wg < blocks ? 1 : 0;
     const uint32_t grid = persist ? (uint32_t)persist_wg : blocks;
 #define GX_DHOPG_GO(DAG, EPI)                                                                                        \

@@ -157,7 +157,7 @@ static int cg_device_loop(Op *op, int prec, Field *psi, Field *p, Field *mmp, Fi
   Ctx *ctx = op->ctx;
   const int64_t vol = op->g.half_elems();
   const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
-  const bool use_graph = tune_get(GX_TUNE_CG_GRAPH) != 0;
+  const bool use_graph = tune_get(GX_TUNE_CG_GRAPH, prec) != 0;
   for (int i = 0; i < 4; i++)
     if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
   auto one_batch = [&]() -> int {
@@ -610,7 +610,7 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
     const int64_t vol = op->g.half_elems();
     const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
     int enqueued = 0, slot = 0;
-    const bool use_graph = tune_get(GX_TUNE_CG_GRAPH) != 0;
+    const bool use_graph = tune_get(GX_TUNE_CG_GRAPH, prec) != 0;
     for (int i = 0; i < 4; i++)
       if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
     auto one_batch = [&]() -> int {

@@ -404,8 +404,13 @@ enum gx_tune_knob {
                                when 4 Lx/2 Ls threads and the tile fit a workgroup) */
   GX_TUNE_COUNT = 12
 };
-int gx_tune_set(int knob, int value);
-int gx_tune_get(int knob, int *value);
+int gx_tune_set(int knob, int value);   /* every precision's value */
+int gx_tune_get(int knob, int *value);  /* the FP64 value */
+/* The knobs hold one value per storage precision (GX_F64, GX_F32, GX_F16): kernels on fields of a
+ * precision take that precision's value, so the FP64 outer and the FP32 / FP16 inner operators of a
+ * mixed-precision solve each run their own tuned launch shape. */
+int gx_tune_set_prec(int knob, int prec, int value);
+int gx_tune_get_prec(int knob, int prec, int *value);
 
 /* ---- synthetic inputs (host utilities, no device work) ----
  * Gauge: per link G with Re,Im ~ N(0,1); H = (G+G^dag)/2 - tr/3; U = exp(i eps H) by

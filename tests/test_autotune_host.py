@@ -86,3 +86,21 @@ def test_launch_candidates_include_row_tiles():
     assert sum(x[gx.TUNE_DHOP_TILE] for x in d) == 1
     assert all(x[gx.TUNE_DHOP_PERSIST] == 0 for x in d if x[gx.TUNE_DHOP_SHAPE] == 0)
     assert all(set(x) == set(at.LAUNCH_KNOBS) for x in d)
+
+
+def test_knobs_per_precision():
+    """A knob holds one value per storage precision: tune_set(..., prec) changes that precision's
+    value only, tune_set without a precision changes all three, and Knobs restores every one."""
+    k = gx.TUNE_DHOP_MAP
+    saved = {p: gx.tune_get(k, p) for p in at.PRECS}
+    try:
+        gx.tune_set(k, 2)
+        gx.tune_set(k, 4, gx.F32)
+        assert (gx.tune_get(k, gx.F64), gx.tune_get(k, gx.F32), gx.tune_get(k, gx.F16)) == (2, 4, 2)
+        assert gx.tune_get(k) == 2
+        with at.Knobs(((k, 3),), gx.F16):
+            assert (gx.tune_get(k, gx.F64), gx.tune_get(k, gx.F32), gx.tune_get(k, gx.F16)) == (2, 4, 3)
+        assert (gx.tune_get(k, gx.F64), gx.tune_get(k, gx.F32), gx.tune_get(k, gx.F16)) == (2, 4, 2)
+    finally:
+        for p, v in saved.items():
+            gx.tune_set(k, v, p)
