@@ -672,7 +672,7 @@ int apply_mooee(Op *op, Field *in, Field *out, int which, const Gate *gate) {
 //                    P2 = MooeeInvDag MeooeDag5D                                   3 kernels
 // The fifth-dimension matrices act across each site's s-row in the hop kernel's epilogue.
 template <typename T>
-static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
+static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate, const HopFuse *fz) {
   const int prec = in->prec, cb_in = in->cb, cb_mid = 1 - cb_in;
   void *Ue, *Uo;
   GX_TRY(op_gauge(op, prec, &Ue, &Uo));
@@ -686,7 +686,7 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
   if (op->kind == GX_WILSON) {
     GX_TRY(launch_dhop_epi<T>(g, in->data, t0->data, U_mid, cb_mid, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0));
     GX_TRY(launch_dhop_epi<T>(g, t0->data, out->data, U_in, cb_in, dag, gate, st, 1, in->data, nullptr, 0.0, 0.0, 0,
-                              coef));
+                              coef, fz));
   } else if (!dag) {
     const void *m_meo, *m_mooee, *m_p1;
     GX_TRY(op_smat(op, prec, Op::S_MEOOE5D, &m_meo));
@@ -710,9 +710,10 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
   return GX_OK;
 }
 
-int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate) {
+int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate, const HopFuse *fz) {
+  if (fz && fz->nparts) *fz->nparts = 0;
   if (tune_get(GX_TUNE_FUSED, in->prec))
-    return by_prec(in->prec, [&](auto t) { return mpc_fused<decltype(t)>(op, in, out, dag, gate); });
+    return by_prec(in->prec, [&](auto t) { return mpc_fused<decltype(t)>(op, in, out, dag, gate, fz); });
   Field *t1 = op_tmp(op, in->prec, 0), *t2 = op_tmp(op, in->prec, 1);
   if (!t1 || !t2) return GX_ENOMEM;
   GX_TRY(apply_meooe(op, in, t1, dag, gate));
@@ -1366,6 +1367,7 @@ int gx_op_footprint(gx_op op, size_t *bytes) {
       if (op->tmp[p][i]) b += op->tmp[p][i]->bytes;
   }
   if (op->full_tmp) b += op->full_tmp->bytes;
+  b += solver_work_bytes(op->work);
   *bytes = b;
   return GX_OK;
 }

@@ -165,6 +165,8 @@ struct SMat {
 // (gx_solver.hip); released with the operator.
 struct SolverWork;
 void solver_work_free(SolverWork *w);
+// device bytes of the solver state that scale with the lattice (the multi-RHS tile sums)
+size_t solver_work_bytes(const SolverWork *w);
 
 struct Op {
   Ctx *ctx = nullptr;
@@ -222,11 +224,36 @@ int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_
 // hop with an epilogue: epi 0 out = hop; 1 out_s = alpha_s hop_s + beta_s phi_s with per-slice
 // coefficients coef (device, 2 Ls in T: alpha_s then beta_s); 2 out = alpha A(hop) [+ beta phi] with A
 // a per-chirality matrix applied across each site's s-row, block diagonal in groups of lb consecutive
-// slices (2*lb*lb in T; lb = 0: one group of Ls)
+// slices (2*lb*lb in T; lb = 0: one group of Ls).  fz: reductions fused into the epilogue (HopFuse).
+//
+// HopFuse, for epi 1 on Ls >= 2 where the tuned kernel is the 4-wave whole-site one:
+//   r == nullptr, dag 0: out as epi 1, and the per-tile slice sums of |out|^2 in npart;
+//   r != nullptr, dag 1: r_s -= a_s (the epi 1 result) and out is NOT written; per-tile slice sums
+//   of |r_s|^2 in npart (a: Ls doubles on the device).
+// npart holds [tile][Ls] doubles, at most hop_norm_tiles(g.H4, g.Ls) tiles; *nparts = the tiles
+// written, or 0 when the kernel has no such epilogue: out is then written as plain epi 1 and the
+// caller reduces itself.
+struct HopFuse {
+  double *npart = nullptr;
+  int64_t *nparts = nullptr;
+  void *r = nullptr;
+  const double *a = nullptr;
+};
 template <typename T>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                     hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb = 0,
-                    const void *coef = nullptr);
+                    const void *coef = nullptr, const HopFuse *fz = nullptr);
+// upper bound of the tiles of a norm-producing hop over H4 half sites of Ls slices, and the bytes
+// of their slice sums
+inline int64_t hop_norm_tiles(int64_t H4, int Ls) {
+  const int spb = 4 * (Ls <= 64 ? 64 / Ls : 1);
+  return (H4 + spb - 1) / spb;
+}
+size_t hop_norm_bytes(int64_t H4, int Ls);
+// per-tile slice sums left by a fused hop (ntiles tiles of Ls) folded, in fixed order, into the
+// multi-RHS CG state as the reduction of that mode: which 4 -> d_s, a_s; 2 -> cp_s, b_s,
+// convergence, k += 1
+int launch_fold_norms(Ctx *ctx, const double *npart, int64_t ntiles, int Ls, MrhsState *state, int which);
 // out_e = c[s] in_e on the (h, s) entries of half fields (c: Ls values in the arithmetic type)
 template <typename S>
 int launch_scale_slices(const Geom &g, const void *in, void *out, const void *c, const Gate *gate, hipStream_t st);
@@ -337,7 +364,9 @@ int reduce_read(Ctx *ctx, int slot, int n, double *out);
 int apply_dhop_half(Op *op, Field *in, Field *out, int cb_out, int dag, const Gate *gate = nullptr);
 int apply_meooe(Op *op, Field *in, Field *out, int dag, const Gate *gate = nullptr);
 int apply_mooee(Op *op, Field *in, Field *out, int which, const Gate *gate = nullptr);
-int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate = nullptr);
+// fz: HopFuse for the second hop (Wilson, fused Mpc path; *fz->nparts = 0 otherwise)
+int apply_mpc(Op *op, Field *in, Field *out, int dag, const Gate *gate = nullptr, const HopFuse *fz = nullptr);
+// out = MpcDag Mpc in through temporary 2 (left holding Mpc in)
 int apply_mpcdagmpc(Op *op, Field *in, Field *out, const Gate *gate = nullptr);
 int apply_full_m(Op *op, Field *in, Field *out, int dag);
 int blas_axpby(Field *z, double a, Field *x, double b, Field *y, const Gate *gate = nullptr);

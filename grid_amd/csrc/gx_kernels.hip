@@ -488,7 +488,12 @@ struct HopEpi {
   const cx<S> *phi;
   const T *mat;   // EPI 2, 3: [2][LS][LS], chirality 0 = spins 0,1
   T alpha, beta;  // EPI 2, 3
-  const T *coef;  // EPI 1: per-slice coefficients, alpha_s = coef[s], beta_s = coef[LS + s]
+  const T *coef;  // EPI 1, 5, 6: per-slice coefficients, alpha_s = coef[s], beta_s = coef[LS + s]
+  // k_dhop only.  EPI 5: out as EPI 1, and the per-tile slice sums of |out|^2 in npart [tile][LS];
+  // EPI 6: rv_s -= acoef_s (EPI 1 result) in place of storing out, per-tile slice sums of |rv_s|^2
+  double *npart;
+  cx<S> *rv;
+  const double *acoef;
 };
 
 // One lane per (4D half-site h, s): s fastest, 64/LS sites per wave, 4 waves per workgroup, so a
@@ -505,7 +510,8 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   constexpr int SPB = 4 * SPW;
   constexpr bool LDSU = (LS > 1) && (SPB * kLinkPad * sizeof(cx<T>) <= 40960);
   constexpr int NLINK = LDSU ? SPB * kLinkPad : 1;
-  constexpr bool MAT = EPI >= 2, PHI = (EPI & 1) != 0;
+  constexpr bool MAT = EPI == 2 || EPI == 3, PHI = EPI == 1 || EPI == 3 || EPI >= 5;
+  constexpr bool NORM = EPI >= 5, RUPD = EPI == 6;
   constexpr int NSTAGE = MAT ? 4 * 12 * 64 : 1;
   __shared__ cx<T> lds[NLINK > NSTAGE ? NLINK : NSTAGE];  // links, then (MAT) the hop results
   const bool sweep = map == 4;  // blocked sweep: tiles are runs of the sweep order, XCD slabs
@@ -554,7 +560,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     }
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
   }
-  if (!MAT && !valid) return;
+  if (!MAT && !NORM && !valid) return;
   const cx<T> *ul = lds + (LDSU ? lsb * kLinkPad : 0);
   cx<T> acc[12];
 #pragma unroll
@@ -584,11 +590,44 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
       }
     }
   }
-  if (EPI != 0) {
-    const T al = EPI == 1 ? ep.coef[s] : ep.alpha, be = EPI == 1 ? ep.coef[LS + s] : ep.beta;
+  if (EPI != 0 && valid) {
+    const bool pc = EPI == 1 || EPI >= 5;
+    const T al = pc ? ep.coef[s] : ep.alpha, be = pc ? ep.coef[LS + s] : ep.beta;
 #pragma unroll
     for (int k = 0; k < 12; k++) acc[k] = {al * acc[k].r, al * acc[k].i};
     if (PHI) axpy_site<S>(acc, be, ep.phi, o);
+  }
+  if constexpr (NORM) {
+    // per-slice |out|^2 (EPI 5) or |rv|^2 (EPI 6) of the tile's sites: lane sums in FP64, then the
+    // tile's sites in fixed order
+    __shared__ double nsh[256];
+    double nv = 0;
+    if (valid) {
+      if (RUPD) {
+        cx<T> rv[12];
+        load_site<S>(rv, ep.rv, o);
+        const T a = (T)(-ep.acoef[s]);
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+          rv[k] = {fma(a, acc[k].r, rv[k].r), fma(a, acc[k].i, rv[k].i)};
+          nv += (double)fma(rv[k].r, rv[k].r, rv[k].i * rv[k].i);
+        }
+        store_site<S>(ep.rv, rv, o, false);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) nv += (double)fma(acc[k].r, acc[k].r, acc[k].i * acc[k].i);
+        store_site<S>(out, acc, o, (flags & 1) != 0);
+      }
+    }
+    nsh[threadIdx.x] = nv;
+    __syncthreads();
+    if (threadIdx.x < LS) {
+      double v = 0;
+      for (int ww = 0; ww < 4; ww++)
+        for (int j = 0; j < SPW; j++) v += nsh[ww * 64 + j * LS + threadIdx.x];
+      ep.npart[(size_t)tile * LS + threadIdx.x] = v;
+    }
+    return;
   }
   store_site<S>(out, acc, o, (flags & 1) != 0);
 }
@@ -1121,12 +1160,13 @@ static GShape make_shape(int LS, int LB, int W) {
 #endif
 
 // epi: 0 out = hop; 1 out = alpha hop + beta phi; 2 out = alpha A(hop) [+ beta phi when phi is
-// given] (A: mat)
+// given] (A: mat).  fz: reductions in the epilogue of the 4-wave whole-site kernel (HopFuse).
 template <typename S>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                     hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb,
-                    const void *coef) {
+                    const void *coef, const HopFuse *fz) {
   typedef typename Sto<S>::C T;
+  if (fz && fz->nparts) *fz->nparts = 0;
   if (lb <= 0) lb = g.Ls;
   if (13 * 2 * g.half_pad() >= (int64_t)1 << 31) {
     set_error("dhop: lattice too large for 32-bit site indexing");
@@ -1162,7 +1202,8 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   const cx<S> *pin = (const cx<S> *)in;
   cx<S> *pout = (cx<S> *)out;
   const cx<S> *pU = (const cx<S> *)U;
-  HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta, (const T *)coef};
+  HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta, (const T *)coef,
+               fz ? fz->npart : nullptr,  (cx<S> *)(fz ? fz->r : nullptr),  fz ? fz->a : nullptr};
   if constexpr (std::is_same<S, float>::value) {
     // the link products on the matrix cores: FP32 multi-RHS (Wilson, no matrix epilogue), Ls <= 16
     if (tune_get(GX_TUNE_DHOP_MFMA, prec_of<S>()) && kepi <= 1 && g.Ls >= 2 && g.Ls <= 16 && lb == g.Ls) {
@@ -1299,6 +1340,9 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
   const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
   const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
+  const bool norm = fz && fz->npart && fz->nparts && kepi == 1 && g.Ls >= 2 &&
+                    (fz->r ? (dag && fz->a) : !dag);
+  if (norm) *fz->nparts = ntiles;
 #define GX_DHOP_GO(LS, DAG, PF, ORD, EPI)                                                                         \
   hipLaunchKernelGGL((k_dhop<S, LS, DAG, PF, ORD, EPI>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
                      gate, ntiles, map, flags, ep)
@@ -1312,7 +1356,17 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   }
 #define GX_DHOP_CASE(LS)                                         \
   case LS:                                                       \
-    if (kepi == 1) {                                             \
+    if (norm) {                                                  \
+      if constexpr (LS >= 2) {                                   \
+        if (dag) {                                               \
+          if (pf) GX_DHOP_GO(LS, true, true, 0, 6);              \
+          else GX_DHOP_GO(LS, true, false, 0, 6);                \
+        } else {                                                 \
+          if (pf) GX_DHOP_GO(LS, false, true, 0, 5);             \
+          else GX_DHOP_GO(LS, false, false, 0, 5);               \
+        }                                                        \
+      }                                                          \
+    } else if (kepi == 1) {                                      \
       GX_DHOP_CASE_O(LS, 0, 1)                                   \
     } else if (kepi == 2) {                                      \
       if constexpr (LS >= 2) {                                   \
@@ -1346,7 +1400,7 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
 template <typename S>
 int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                 hipStream_t st) {
-  return launch_dhop_epi<S>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0, 0, nullptr);
+  return launch_dhop_epi<S>(g, in, out, U, cb_out, dag, gate, st, 0, nullptr, nullptr, 1.0, 0.0, 0, nullptr, nullptr);
 }
 
 // out_s = sum_t A_chi[s][t] in_t (+ beta phi_s with PHI), and with OUT2 also
@@ -2003,18 +2057,32 @@ static __global__ void __launch_bounds__(256) k_mrhs_fin(const double *partial, 
   }
 }
 
-// psi_s += a_s p_s; p_s = b_s p_s + r_s over every stored entry (padding entries are zero)
+// psi_s += a_s p_s; p_s = b_s p_s + r_s over every stored entry (padding entries are zero), one
+// 16-byte group (the VW components of one entry that it holds) per thread and step
 template <typename T>
 __global__ void __launch_bounds__(256) k_mrhs_update(int64_t n, int Ls, cx<T> *__restrict__ psi, cx<T> *__restrict__ p,
                                                      const cx<T> *__restrict__ r, const MrhsState *st) {
   if (mgated_prev(st)) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = entry_of<T>(i);  // entry of stored position i
+  constexpr int VW = Sto<T>::VW;
+  struct alignas(16) G {
+    cx<T> v[VW];
+  };
+  G *gx_ = reinterpret_cast<G *>(psi), *gp = reinterpret_cast<G *>(p);
+  const G *gr = reinterpret_cast<const G *>(r);
+  const int64_t ng = n / VW;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ng; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = entry_of<T>(q * VW);  // entry of the group's stored positions
     const int s = (int)(e % Ls);
     const T a = (T)st->a[s], b = (T)st->b[s];
-    const cx<T> pv = p[i], xv = psi[i], rv = r[i];
-    psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
-    p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
+    const G pv = gp[q], rv = gr[q];
+    G xv = gx_[q], np;
+#pragma unroll
+    for (int j = 0; j < VW; j++) {
+      xv.v[j] = {fma(a, pv.v[j].r, xv.v[j].r), fma(a, pv.v[j].i, xv.v[j].i)};
+      np.v[j] = {fma(b, pv.v[j].r, rv.v[j].r), fma(b, pv.v[j].i, rv.v[j].i)};
+    }
+    gx_[q] = xv;
+    gp[q] = np;
   }
 }
 
@@ -2076,7 +2144,8 @@ int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *
                        (cx<half_t> *)psi, (cx<half_t> *)p, (const cx<half_t> *)r, (const MrhsState *)state);
   } else {
     const int64_t n = 12 * g.half_pad();
-    hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, g.Ls, (cx<T> *)psi,
+    hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(n / Sto<T>::VW)), dim3(256), 0, ctx->stream, n, g.Ls,
+                       (cx<T> *)psi,
                        (cx<T> *)p, (const cx<T> *)r, (const MrhsState *)state);
   }
   GX_CHECK_HIP(hipGetLastError());
@@ -2592,6 +2661,43 @@ static __global__ void __launch_bounds__(256) k_vec_madd(int64_t nelem, int n, c
 }
 
 #if GX_PART == 0
+// the per-tile slice sums of a norm-producing hop folded into kRedBlocks block partials: thread t
+// of a block adds the tiles blockIdx * rpb + t / Ls, + gridDim * rpb, ... of slice t % Ls, the
+// threads of a slice are then added in thread order (fixed order, so deterministic)
+static __global__ void __launch_bounds__(256) k_fold_tiles(const double *__restrict__ tp, int64_t ntiles, int Ls,
+                                                           const MrhsState *st, double *partial) {
+  if (mgated(st)) return;
+  __shared__ double sh[256];
+  const int rpb = 256 / Ls;
+  const int s = threadIdx.x % Ls, rt = threadIdx.x / Ls;
+  double acc = 0;
+  if (rt < rpb)
+    for (int64_t t = (int64_t)blockIdx.x * rpb + rt; t < ntiles; t += (int64_t)gridDim.x * rpb) acc += tp[t * Ls + s];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < Ls) {
+    double v = 0;
+    for (int r = 0; r < rpb; r++) v += sh[r * Ls + threadIdx.x];
+    partial[blockIdx.x * Ls + threadIdx.x] = v;
+  }
+}
+
+int launch_fold_norms(Ctx *ctx, const double *npart, int64_t ntiles, int Ls, MrhsState *state, int which) {
+  if (Ls < 1 || Ls > kMaxRhs || ntiles < 1 || (which != 2 && which != 4)) {
+    set_error("fold norms: %lld tiles of %d slices (which %d)", (long long)ntiles, Ls, which);
+    return GX_EINVAL;
+  }
+  const int rpb = 256 / Ls;
+  int64_t nb = (ntiles + rpb - 1) / rpb;
+  if (nb > kRedBlocks) nb = kRedBlocks;
+  hipLaunchKernelGGL(k_fold_tiles, dim3(nb), dim3(256), 0, ctx->stream, npart, ntiles, Ls, (const MrhsState *)state,
+                     ctx->red_partial);
+  hipLaunchKernelGGL(k_mrhs_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, (int)nb, Ls,
+                     state, nullptr, which);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
 int launch_vec_ipm(Ctx *ctx, int64_t nelem, int n, const VecPtrs &x, const VecPtrs &y, void *partial, int nparts,
                    void *out_dev) {
   if (n < 1 || n > kMaxLs || nparts < 1) {
@@ -2744,7 +2850,8 @@ int launch_digest_slices(Ctx *ctx, const Geom &g, const void *full, int s0, int 
 
 #define GX_INST(S)                                                                                                 \
   template int launch_dhop_epi<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,       \
-                                  hipStream_t, int, const void *, const void *, double, double, int, const void *); \
+                                  hipStream_t, int, const void *, const void *, double, double, int, const void *, \
+                                  const HopFuse *);                                                                \
   template int launch_smat2<S>(const Geom &, const void *, const void *, double, void *, const void *, void *,     \
                                const void *, const Gate *, hipStream_t, int);                                    \
   template int launch_dhop<S>(const Geom &, const void *, void *, const void *, int, int, const Gate *,           \

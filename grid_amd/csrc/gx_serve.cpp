@@ -27,6 +27,8 @@
 
 namespace gx {
 void set_error(const char *fmt, ...);  // gx_last_error() text (gx_api.hip)
+// bytes of the multi-RHS CG's per-tile slice sums over H4 half sites of Ls slices (gx_internal.h)
+size_t hop_norm_bytes(int64_t H4, int Ls);
 }
 
 namespace {
@@ -43,13 +45,15 @@ struct Lattice {
 // operator temporaries 0-3, CG vectors 4-6, the red-black halves 10-14, the residual check's
 // full field (2), source and solution (4), and with mixed precision the FP64 outer vectors 7-8
 // plus nine inner-precision halves: FP32 (mixed = 1, 4.5 FP64 halves) or FP16 storage (mixed =
-// 2, 52 of every 192 bytes: 12 binary16 pairs and the scale per entry), and the operator's
-// per-slice Schur coefficients.
+// 2, 52 of every 192 bytes: 12 binary16 pairs and the scale per entry), the operator's per-slice
+// Schur coefficients, and (nsrc >= 2) the multi-RHS CG's per-tile slice sums.
 double footprint(const int d[4], int nsrc, int mixed) {
-  const double hb = (double)d[0] * d[1] * d[2] * d[3] / 2 * nsrc * 12 * 16;
+  const int64_t H4 = (int64_t)d[0] * d[1] * d[2] * d[3] / 2;
+  const double hb = (double)H4 * nsrc * 12 * 16;
   const double coef = 48.0 * nsrc;  // the operator's per-slice coefficients (4 doubles and 4 floats)
-  if (mixed == 2) return hb * (20.0 + 9.0 * 52.0 / 192.0) + coef;
-  return hb * (mixed ? 24.5 : 18.0) + coef;
+  const double tiles = nsrc >= 2 ? (double)gx::hop_norm_bytes(H4, nsrc) : 0.0;
+  if (mixed == 2) return hb * (20.0 + 9.0 * 52.0 / 192.0) + coef + tiles;
+  return hb * (mixed ? 24.5 : 18.0) + coef + tiles;
 }
 
 struct Entry {

@@ -77,6 +77,8 @@ struct MrhsWork {
   MrhsState *host = nullptr;  // pinned, 2 slots
   double *out_dev = nullptr;  // per-slice results
   double *out_host = nullptr; // pinned mirror
+  double *npart = nullptr;    // per-tile slice sums of |Mpc p|^2 (multi-RHS Wilson operators)
+  size_t npart_bytes = 0;
   hipEvent_t ev[2] = {};
   BatchGraph graph[3];  // per precision
 };
@@ -103,6 +105,7 @@ void solver_work_free(SolverWork *w) {
   if (m.out_dev) (void)hipFree(m.out_dev);
   if (m.host) (void)hipHostFree(m.host);
   if (m.out_host) (void)hipHostFree(m.out_host);
+  if (m.npart) (void)hipFree(m.npart);
   for (int i = 0; i < 2; i++) {
     if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
     if (m.ev[i]) (void)hipEventDestroy(m.ev[i]);
@@ -122,6 +125,10 @@ static int solver_work(Op *op, SolverWork **out) {
     if (e == hipSuccess) e = hipHostMalloc(&m.host, 2 * sizeof(MrhsState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m.out_dev, sizeof(double) * kMaxRhs);
     if (e == hipSuccess) e = hipHostMalloc(&m.out_host, sizeof(double) * kMaxRhs, hipHostMallocDefault);
+    if (e == hipSuccess && op->kind == GX_WILSON && op->g.Ls >= 2 && op->g.Ls <= kMaxRhs) {
+      m.npart_bytes = hop_norm_bytes(op->g.H4, op->g.Ls);
+      e = hipMalloc(&m.npart, m.npart_bytes);
+    }
     for (int i = 0; i < 2 && e == hipSuccess; i++) {
       e = hipEventCreateWithFlags(&c.ev[i], hipEventDisableTiming);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&m.ev[i], hipEventDisableTiming);
@@ -136,6 +143,10 @@ static int solver_work(Op *op, SolverWork **out) {
   *out = op->work;
   return GX_OK;
 }
+
+size_t solver_work_bytes(const SolverWork *w) { return w ? w->mrhs.npart_bytes : 0; }
+
+size_t hop_norm_bytes(int64_t H4, int Ls) { return sizeof(double) * (size_t)hop_norm_tiles(H4, Ls) * Ls; }
 
 template <typename T>
 static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGState *st) {
@@ -519,20 +530,44 @@ static int slice_reduce(Op *op, int mode, Field *x, Field *y, Field *z, double *
 
 // One CGmultiRHS iteration.  d_s = Re<p_s, MpcDag Mpc p_s> as in Grid (BlockConjugateGradient.h:474-478)
 // for FP64; the FP32 / FP16-storage inner solves of defect correction take the same quantity as
-// |Mpc p_s|^2 from the intermediate t = Mpc p that MpcDagMpc leaves in temporary 2, one field read
-// instead of two (equal in exact arithmetic since MpcDag is the adjoint of Mpc).
+// |Mpc p_s|^2 from the intermediate t = Mpc p (equal in exact arithmetic since MpcDag is the adjoint
+// of Mpc).  There, where the tuned hop kernel has the fused epilogues (HopFuse), the Mpc hop sums
+// |t_s|^2 per tile (folded to d_s, a_s), and the last MpcDag hop updates r_s -= a_s (MpcDag t)_s
+// and sums |r_s|^2 per tile without storing MpcDag t: the iteration reads and writes no field
+// beyond the hops and the x, p update.  Otherwise t, then (r, mmp) are reduced by their own passes.
 template <typename T>
-static int mrhs_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, MrhsState *st) {
-  GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
-  if (std::is_same<T, double>::value) {
-    GX_TRY(launch_mrhs_reduce<T>(op->ctx, 0, op->g, p->data, mmp->data, nullptr, st, nullptr));
+static int mrhs_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, MrhsState *st, double *npart) {
+  Ctx *ctx = op->ctx;
+  if (std::is_same<T, double>::value || !npart) {
+    GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
+    if (std::is_same<T, double>::value) {
+      GX_TRY(launch_mrhs_reduce<T>(ctx, 0, op->g, p->data, mmp->data, nullptr, st, nullptr));
+    } else {
+      Field *t = op_tmp(op, p->prec, 2);
+      if (!t) return GX_ENOMEM;
+      GX_TRY(launch_mrhs_reduce<T>(ctx, 4, op->g, t->data, nullptr, nullptr, st, nullptr));
+    }
+    GX_TRY(launch_mrhs_reduce<T>(ctx, 2, op->g, r->data, mmp->data, nullptr, st, nullptr));
   } else {
     Field *t = op_tmp(op, p->prec, 2);
     if (!t) return GX_ENOMEM;
-    GX_TRY(launch_mrhs_reduce<T>(op->ctx, 4, op->g, t->data, nullptr, nullptr, st, nullptr));
+    int64_t n1 = 0, n2 = 0;
+    HopFuse f1;
+    f1.npart = npart;
+    f1.nparts = &n1;
+    GX_TRY(apply_mpc(op, p, t, 0, st, &f1));
+    if (n1 > 0) GX_TRY(launch_fold_norms(ctx, npart, n1, op->g.Ls, st, 4));
+    else GX_TRY(launch_mrhs_reduce<T>(ctx, 4, op->g, t->data, nullptr, nullptr, st, nullptr));
+    HopFuse f2;
+    f2.npart = npart;
+    f2.nparts = &n2;
+    f2.r = r->data;
+    f2.a = st->a;  // device address of the state's a_s (not dereferenced here)
+    GX_TRY(apply_mpc(op, t, mmp, 1, st, &f2));
+    if (n2 > 0) GX_TRY(launch_fold_norms(ctx, npart, n2, op->g.Ls, st, 2));
+    else GX_TRY(launch_mrhs_reduce<T>(ctx, 2, op->g, r->data, mmp->data, nullptr, st, nullptr));
   }
-  GX_TRY(launch_mrhs_reduce<T>(op->ctx, 2, op->g, r->data, mmp->data, nullptr, st, nullptr));
-  GX_TRY(launch_mrhs_update<T>(op->ctx, op->g, psi->data, p->data, r->data, st));
+  GX_TRY(launch_mrhs_update<T>(ctx, op->g, psi->data, p->data, r->data, st));
   return GX_OK;
 }
 
@@ -563,10 +598,17 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
     zero = true;
     for (int s = 0; s < Ls; s++) zero = zero && gn[s] == 0.0;
   }
+  // a caller's residual (true, of this guess) that already meets tol on every slice: no
+  // iteration, and its norms are the true residual
+  bool r0_done = opt.r0 != nullptr;
+  if (opt.r0)
+    for (int s = 0; s < Ls; s++) {
+      rr[s] = opt.r0_norm[s];
+      r0_done = r0_done && rr[s] < tol * tol * (ssq[s] > 0 ? ssq[s] : 1.0);
+    }
   if (opt.r0) {
-    // the caller's residual of this guess
-    GX_CHECK_HIP(hipMemcpyAsync(r->data, opt.r0->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    for (int s = 0; s < Ls; s++) rr[s] = opt.r0_norm[s];
+    if (!r0_done)
+      GX_CHECK_HIP(hipMemcpyAsync(r->data, opt.r0->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
   } else if (zero) {
     // A 0 = 0: R = src
     GX_CHECK_HIP(hipMemcpyAsync(r->data, src->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -581,7 +623,7 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
       set_error("block CG: NaN in guess slice %d", s);
       return GX_ENAN;
     }
-  GX_CHECK_HIP(hipMemcpyAsync(p->data, r->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  if (!r0_done) GX_CHECK_HIP(hipMemcpyAsync(p->data, r->data, r->bytes, hipMemcpyDeviceToDevice, ctx->stream));
   SolverWork *sw = nullptr;
   GX_TRY(solver_work(op, &sw));
   MrhsWork &w = sw->mrhs;
@@ -615,7 +657,7 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
       if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
     auto one_batch = [&]() -> int {
       for (int i = 0; i < batch; i++) {
-        int rc = by_prec(prec, [&](auto tag) { return mrhs_iteration<decltype(tag)>(op, psi, p, mmp, r, w.dev); });
+        int rc = by_prec(prec, [&](auto tag) { return mrhs_iteration<decltype(tag)>(op, psi, p, mmp, r, w.dev, w.npart); });
         GX_TRY(rc);
       }
       return GX_OK;
@@ -628,7 +670,7 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
         enqueued += batch;
       } else {
         for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
-          int rc = by_prec(prec, [&](auto tag) { return mrhs_iteration<decltype(tag)>(op, psi, p, mmp, r, w.dev); });
+          int rc = by_prec(prec, [&](auto tag) { return mrhs_iteration<decltype(tag)>(op, psi, p, mmp, r, w.dev, w.npart); });
           GX_TRY(rc);
         }
       }
@@ -656,7 +698,9 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
   }
   // true residual per right-hand side (inner solves report the recursive one)
   std::vector<double> res(Ls);
-  if (opt.true_residual) {
+  if (r0_done) {
+    res = rr;
+  } else if (opt.true_residual) {
     GX_TRY(apply_mpcdagmpc(op, psi, mmp));
     GX_TRY(slice_reduce(op, 3, src, mmp, p, res.data()));
   } else {
@@ -682,6 +726,18 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
 // Batched defect correction (ConjugateGradientMixedPrecBatched.h:100-213) with the inner solves of
 // all right-hand sides done together by one multi-RHS CG in FP32 or FP16 storage (inner_prec), so
 // they share every link load.
+//
+// Outer exit and leg targets.  Grid's loop ends once every |r_s|^2 <= OuterLoopNormMult (100) x
+// stop_s and leaves the last factor to the FP64 patch-up CG; its inner tolerance only ever doubles
+// from InnerTolerance.  FP16-storage inner solves keep that rule (one such leg gains three to four
+// orders, so the legs, not the tolerance, set the count).  With FP32 inner solves one multi-RHS
+// FP64 iteration costs about two inner ones, so the loop runs until every slice is converged
+// (multiplier 1), and a leg whose needed reduction t = kLand sqrt(stop_max / |r|^2_max) is
+// within one FP32 leg's reach (t >= kReach32) aims there, the others at the caller's inner
+// tolerance: the second leg then lands below stop, and the patch-up CG, still run from the fresh
+// residual as in Grid, finds every slice converged and returns at once.
+constexpr double kLand = 0.5;
+constexpr double kReach32 = 1e-7;
 int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int inner_prec, int maxinner,
                    int maxouter, gx_solver_stats *st, int maxpatch = -1) {
   auto t0 = std::chrono::steady_clock::now();
@@ -694,7 +750,6 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
   Field *src_f = op_tmp(op, inner_prec, 7), *sol_f = op_tmp(op, inner_prec, 8);
   if (!tmp_d || !src_d || !src_f || !sol_f) return GX_ENOMEM;
   tmp_d->cb = src_d->cb = src_f->cb = sol_f->cb = cb;
-  double inner_tol = inner_tol0;
   int outer = 0, inner_total = 0;
   std::vector<double> sn(Ls);
   GX_TRY(slice_reduce(op, 1, sol, nullptr, nullptr, sn.data()));
@@ -702,6 +757,9 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
   for (int s = 0; s < Ls; s++) sol_zero = sol_zero && sn[s] == 0.0;
   Field *res = src_d;   // src - A sol of the latest check
   bool fresh = false;   // res belongs to the current sol
+  const bool grid_rule = inner_prec == GX_F16;
+  const double outer_mult = grid_rule ? 100.0 : 1.0;
+  double inner_tol = inner_tol0;
   for (outer = 0; outer < maxouter; outer++) {
     if (sol_zero) {
       res = src_in;     // A 0 = 0
@@ -715,12 +773,17 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
     bool all = true;
     double nmax = 0, smax = 0;
     for (int s = 0; s < Ls; s++) {
-      if (norm[s] > 100.0 * stop[s]) all = false;
+      if (norm[s] > outer_mult * stop[s]) all = false;
       nmax = std::max(nmax, norm[s]);
       smax = std::max(smax, stop[s]);
     }
     if (all) break;
-    while (nmax * inner_tol * inner_tol < smax) inner_tol *= 2;
+    if (grid_rule) {
+      while (nmax * inner_tol * inner_tol < smax) inner_tol *= 2;
+    } else {
+      const double t = kLand * std::sqrt(smax / nmax);
+      inner_tol = t >= kReach32 ? t : std::max(inner_tol0, t);
+    }
     GX_TRY(convert_field(src_f, res));
     GX_CHECK_HIP(hipMemsetAsync(sol_f->data, 0, sol_f->bytes, op->ctx->stream));
     std::vector<gx_solver_stats> ist(Ls);
