@@ -529,7 +529,9 @@ def main():
     ap.add_argument('--no-tune', action='store_true',
                     help='keep the library default launch knobs instead of autotuning each workload')
     ap.add_argument('--knobs', default='',
-                    help='explicit hop knobs map,pf,nt,order[,shape,persist] (implies --no-tune), e.g. 0,1,1,0')
+                    help='explicit hop knobs map,pf,nt,order[,shape,persist] for every precision (implies '
+                         '--no-tune), e.g. 0,1,1,0; or per storage precision, e.g. '
+                         'f64=4,1,0,0,6,1/f32=4,0,1,0,0,0/f16=4,0,1,1,0,0')
     args = ap.parse_args()
     legs = set(args.legs.split(','))
     args.no_cg = args.no_cg or 'cg' not in legs
@@ -550,10 +552,15 @@ def main():
     from grid_amd import autotune
     if args.knobs:
         args.no_tune = True
-        for k, v in zip((gx.TUNE_DHOP_MAP, gx.TUNE_DHOP_PF, gx.TUNE_DHOP_NT, gx.TUNE_DHOP_ORDER,
-                         gx.TUNE_DHOP_SHAPE, gx.TUNE_DHOP_PERSIST),
-                        (int(x) for x in args.knobs.split(','))):
-            gx.tune_set(k, v)
+        order = (gx.TUNE_DHOP_MAP, gx.TUNE_DHOP_PF, gx.TUNE_DHOP_NT, gx.TUNE_DHOP_ORDER, gx.TUNE_DHOP_SHAPE,
+                 gx.TUNE_DHOP_PERSIST)
+        precs = {'f64': gx.F64, 'f32': gx.F32, 'f16': gx.F16}
+        for part in args.knobs.split('/'):
+            name, _, vals = part.rpartition('=')
+            if name and name not in precs:
+                ap.error(f'--knobs: unknown precision {name!r} (f64, f32, f16)')
+            for k, v in zip(order, (int(x) for x in vals.split(','))):
+                gx.tune_set(k, v, precs[name] if name else None)
     tuner = None if args.no_tune else autotune.Autotuner(ctx, cache_path=None)
     tuned = {}
 

@@ -259,6 +259,17 @@ int convert_field(Field *out, Field *in) {
   return GX_EINVAL;
 }
 
+int accumulate_field(Field *out, Field *in) {
+  if (out->prec != GX_F64 || in->prec == GX_F64 || out->halves() != in->halves() || !same_geom(out->g, in->g)) {
+    set_error("accumulate: needs an F64 field and an F32 / F16 field of one lattice, both full or both half");
+    return GX_EINVAL;
+  }
+  const int64_t ns = nsites_of(in), st = in->g.half_pad();
+  hipStream_t s = in->ctx->stream;
+  if (in->prec == GX_F32) return launch_accum_site<float>(ns, st, in->data, out->data, s);
+  return launch_accum_site<half_t>(ns, st, in->data, out->data, s);
+}
+
 // ---------------- gauge field: DoubleStore into the device layout ----------------
 // Uds[mu] = -1/2 U_mu(x) * phase (x_mu = L-1); Uds[mu+4] = -1/2 U_mu(x-mu)^dag * conj(phase) (x_mu = 0)
 // WilsonImpl.h:127-171, WilsonFermionImplementation.h:82-107.

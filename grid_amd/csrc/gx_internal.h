@@ -278,6 +278,9 @@ template <typename Tin, typename Tout>
 int launch_convert(int64_t n, const void *in, void *out, hipStream_t st);       // n plain entries
 template <typename Sin, typename Sout>
 int launch_convert_site(int64_t nsites, int64_t stride, const void *in, void *out, hipStream_t st);
+// out (FP64) += in (FP32 or FP16 storage), site by site
+template <typename Sin>
+int launch_accum_site(int64_t nsites, int64_t stride, const void *in, void *out, hipStream_t st);
 // z[sz] (=|+=) (spins in mask) x[sx] over nhalf checkerboard halves (2: full fields, 1: half fields;
 // rows Lsx / Lsz)
 template <typename T>
@@ -375,6 +378,8 @@ int blas_inner(Field *x, Field *y, double *re, double *im);
 int blas_inner_norm(Field *x, Field *y, double *re, double *nrm);  // Re<x|y>, |x|^2 in one pass
 int blas_axpy_norm(Field *z, double a, Field *x, Field *y, double *nrm);
 int convert_field(Field *out, Field *in);
+// out += in for an FP64 out and an FP32 / FP16-storage in of one geometry, both full or both half
+int accumulate_field(Field *out, Field *in);
 // ExtractSlice (extract) / InsertSlice between a 4D field lo (Ls = 1) and slice s of hi: two full or
 // two half fields of one precision
 int slice_transfer(Field *lo, Field *hi, int s, bool extract);

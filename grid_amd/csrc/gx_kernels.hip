@@ -1653,6 +1653,31 @@ __global__ void __launch_bounds__(256) k_convert_site(int64_t nsites, int64_t st
   }
 }
 
+// out += in over the sites of an FP64 field (defect correction folding an inner solution into the
+// outer one: the sum of a conversion then an axpby with unit coefficients, in one pass)
+template <typename Sin>
+__global__ void __launch_bounds__(256) k_accum_site(int64_t nsites, int64_t stride, const cx<Sin> *__restrict__ in,
+                                                    cx<double> *__restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t hf = j / stride, i = j - hf * stride;
+    cx<typename Sto<Sin>::C> a[12];
+    cx<double> b[12];
+    load_site<Sin>(a, in + hf * Sto<Sin>::NCOL * stride, i);
+    load_site<double>(b, out + hf * 12 * stride, i);
+#pragma unroll
+    for (int k = 0; k < 12; k++) b[k] = {b[k].r + (double)a[k].r, b[k].i + (double)a[k].i};
+    store_site<double>(out + hf * 12 * stride, b, i, false);
+  }
+}
+
+template <typename Sin>
+int launch_accum_site(int64_t nsites, int64_t stride, const void *in, void *out, hipStream_t st) {
+  hipLaunchKernelGGL((k_accum_site<Sin>), dim3(stream_blocks(nsites)), dim3(256), 0, st, nsites, stride,
+                     (const cx<Sin> *)in, (cx<double> *)out);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
 template <typename Sin, typename Sout>
 int launch_convert_site(int64_t nsites, int64_t stride, const void *in, void *out, hipStream_t st) {
   hipLaunchKernelGGL((k_convert_site<Sin, Sout>), dim3(stream_blocks(nsites)), dim3(256), 0, st, nsites, stride,
@@ -2877,6 +2902,7 @@ GX_INST_REAL(double)
 template int launch_convert<double, float>(int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<double, float>(int64_t, int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<float, double>(int64_t, int64_t, const void *, void *, hipStream_t);
+template int launch_accum_site<float>(int64_t, int64_t, const void *, void *, hipStream_t);
 #elif GX_PART == 1
 GX_INST(float)
 GX_INST_REAL(float)
@@ -2889,6 +2915,7 @@ template int launch_convert_site<double, half_t>(int64_t, int64_t, const void *,
 template int launch_convert_site<half_t, double>(int64_t, int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<float, half_t>(int64_t, int64_t, const void *, void *, hipStream_t);
 template int launch_convert_site<half_t, float>(int64_t, int64_t, const void *, void *, hipStream_t);
+template int launch_accum_site<half_t>(int64_t, int64_t, const void *, void *, hipStream_t);
 #endif
 
 }  // namespace gx
