@@ -707,7 +707,7 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate, c
     GX_TRY(launch_smat2<T>(g, in->data, nullptr, 0.0, t0->data, m_meo, t1->data, m_mooee, gate, st));
     GX_TRY(launch_dhop_epi<T>(g, t0->data, t3->data, U_mid, cb_mid, 0, gate, st, 2, nullptr, m_p1, 1.0, 0.0));
     GX_TRY(launch_dhop_epi<T>(g, t3->data, out->data, U_in, cb_in, 0, gate, st, 1, t1->data, nullptr, 0.0, 0.0, 0,
-                              coef));
+                              coef, fz));
   } else {
     const void *m_mooeedag, *m_p2, *m_meodag;
     GX_TRY(op_smat(op, prec, Op::S_MOOEEDAG, &m_mooeedag));
@@ -715,7 +715,8 @@ static int mpc_fused(Op *op, Field *in, Field *out, int dag, const Gate *gate, c
     GX_TRY(op_smat(op, prec, Op::S_MEOOEDAG5D, &m_meodag));
     GX_TRY(launch_smat2<T>(g, in->data, nullptr, 0.0, t1->data, m_mooeedag, nullptr, nullptr, gate, st));
     GX_TRY(launch_dhop_epi<T>(g, in->data, t3->data, U_mid, cb_mid, 1, gate, st, 2, nullptr, m_p2, 1.0, 0.0));
-    GX_TRY(launch_dhop_epi<T>(g, t3->data, out->data, U_in, cb_in, 1, gate, st, 2, t1->data, m_meodag, -1.0, 1.0));
+    GX_TRY(launch_dhop_epi<T>(g, t3->data, out->data, U_in, cb_in, 1, gate, st, 2, t1->data, m_meodag, -1.0, 1.0, 0,
+                              nullptr, fz));
   }
   out->cb = cb_in;
   return GX_OK;

@@ -226,10 +226,11 @@ int launch_dhop(const Geom &g, const void *in, void *out, const void *U, int cb_
 // a per-chirality matrix applied across each site's s-row, block diagonal in groups of lb consecutive
 // slices (2*lb*lb in T; lb = 0: one group of Ls).  fz: reductions fused into the epilogue (HopFuse).
 //
-// HopFuse, for epi 1 on Ls >= 2 where the tuned kernel is the 4-wave whole-site one:
-//   r == nullptr, dag 0: out as epi 1, and the per-tile slice sums of |out|^2 in npart;
-//   r != nullptr, dag 1: r_s -= a_s (the epi 1 result) and out is NOT written; per-tile slice sums
-//   of |r_s|^2 in npart (a: Ls doubles on the device).
+// HopFuse, on Ls >= 2 where the tuned kernel is the 4-wave whole-site one or a general shape:
+//   r == nullptr, epi 1, dag 0: out as epi 1, and the per-tile slice sums of |out|^2 in npart;
+//   r != nullptr, epi 1 or 2 with phi, dag 1: r_s -= a[astride s] (the epilogue's result)_s and out
+//   is NOT written; per-tile slice sums of |r_s|^2 in npart (a on the device: astride 1, one value
+//   per slice (multi-RHS); astride 0, one value (single-RHS CG)).
 // npart holds [tile][Ls] doubles, at most hop_norm_tiles(g.H4, g.Ls) tiles; *nparts = the tiles
 // written, or 0 when the kernel has no such epilogue: out is then written as plain epi 1 and the
 // caller reduces itself.
@@ -238,15 +239,17 @@ struct HopFuse {
   int64_t *nparts = nullptr;
   void *r = nullptr;
   const double *a = nullptr;
+  int astride = 1;
 };
 template <typename T>
 int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int cb_out, int dag, const Gate *gate,
                     hipStream_t st, int epi, const void *phi, const void *mat, double alpha, double beta, int lb = 0,
                     const void *coef = nullptr, const HopFuse *fz = nullptr);
-// upper bound of the tiles of a norm-producing hop over H4 half sites of Ls slices, and the bytes
-// of their slice sums
+// upper bound of the tiles of a norm-producing hop over H4 half sites of Ls slices (the 4-wave
+// kernel has 4 (64 / Ls) sites per tile, a general shape of W waves 64 W / Ls >= 64 / Ls), and the
+// bytes of their slice sums
 inline int64_t hop_norm_tiles(int64_t H4, int Ls) {
-  const int spb = 4 * (Ls <= 64 ? 64 / Ls : 1);
+  const int spb = Ls <= 64 ? 64 / Ls : 1;
   return (H4 + spb - 1) / spb;
 }
 size_t hop_norm_bytes(int64_t H4, int Ls);
@@ -254,6 +257,9 @@ size_t hop_norm_bytes(int64_t H4, int Ls);
 // multi-RHS CG state as the reduction of that mode: which 4 -> d_s, a_s; 2 -> cp_s, b_s,
 // convergence, k += 1
 int launch_fold_norms(Ctx *ctx, const double *npart, int64_t ntiles, int Ls, MrhsState *state, int which);
+// the same tile sums added over all slices into the single-RHS CG state: which 0 -> d, a; 1 -> cp, b,
+// stopping test, k += 1 (as the dot and axpy-norm finalisers)
+int launch_fold_cg(Ctx *ctx, const double *npart, int64_t ntiles, int Ls, CGState *state, int which);
 // out_e = c[s] in_e on the (h, s) entries of half fields (c: Ls values in the arithmetic type)
 template <typename S>
 int launch_scale_slices(const Geom &g, const void *in, void *out, const void *c, const Gate *gate, hipStream_t st);

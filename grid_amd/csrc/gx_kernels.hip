@@ -490,10 +490,12 @@ struct HopEpi {
   T alpha, beta;  // EPI 2, 3
   const T *coef;  // EPI 1, 5, 6: per-slice coefficients, alpha_s = coef[s], beta_s = coef[LS + s]
   // k_dhop only.  EPI 5: out as EPI 1, and the per-tile slice sums of |out|^2 in npart [tile][LS];
-  // EPI 6: rv_s -= acoef_s (EPI 1 result) in place of storing out, per-tile slice sums of |rv_s|^2
+  // EPI 6 (EPI 1 result) and 7 (EPI 3 result): rv_s -= acoef[astride s] (result)_s in place of
+  // storing out, per-tile slice sums of |rv_s|^2
   double *npart;
   cx<S> *rv;
   const double *acoef;
+  int astride;
 };
 
 // One lane per (4D half-site h, s): s fastest, 64/LS sites per wave, 4 waves per workgroup, so a
@@ -510,8 +512,8 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   constexpr int SPB = 4 * SPW;
   constexpr bool LDSU = (LS > 1) && (SPB * kLinkPad * sizeof(cx<T>) <= 40960);
   constexpr int NLINK = LDSU ? SPB * kLinkPad : 1;
-  constexpr bool MAT = EPI == 2 || EPI == 3, PHI = EPI == 1 || EPI == 3 || EPI >= 5;
-  constexpr bool NORM = EPI >= 5, RUPD = EPI == 6;
+  constexpr bool MAT = EPI == 2 || EPI == 3 || EPI == 7, PHI = EPI == 1 || EPI == 3 || EPI >= 5;
+  constexpr bool NORM = EPI >= 5, RUPD = EPI == 6 || EPI == 7;
   constexpr int NSTAGE = MAT ? 4 * 12 * 64 : 1;
   __shared__ cx<T> lds[NLINK > NSTAGE ? NLINK : NSTAGE];  // links, then (MAT) the hop results
   const bool sweep = map == 4;  // blocked sweep: tiles are runs of the sweep order, XCD slabs
@@ -574,24 +576,26 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
 #pragma unroll
     for (int k = 0; k < 12; k++) stg[k * 64 + lane] = acc[k];
     __syncthreads();
-    if (!valid) return;
+    if (!valid && !NORM) return;
 #pragma unroll
     for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-    const T *ap = ep.mat + s * LS, *am = ep.mat + LS * LS + s * LS;
+    if (valid) {
+      const T *ap = ep.mat + s * LS, *am = ep.mat + LS * LS + s * LS;
 #pragma unroll
-    for (int tt = 0; tt < LS; tt++) {
-      const T cp = ap[tt], cm = am[tt];
+      for (int tt = 0; tt < LS; tt++) {
+        const T cp = ap[tt], cm = am[tt];
 #pragma unroll
-      for (int k = 0; k < 12; k++) {
-        const cx<T> v = stg[k * 64 + ls * LS + tt];
-        const T c = k < 6 ? cp : cm;
-        acc[k].r = fma(c, v.r, acc[k].r);
-        acc[k].i = fma(c, v.i, acc[k].i);
+        for (int k = 0; k < 12; k++) {
+          const cx<T> v = stg[k * 64 + ls * LS + tt];
+          const T c = k < 6 ? cp : cm;
+          acc[k].r = fma(c, v.r, acc[k].r);
+          acc[k].i = fma(c, v.i, acc[k].i);
+        }
       }
     }
   }
   if (EPI != 0 && valid) {
-    const bool pc = EPI == 1 || EPI >= 5;
+    const bool pc = EPI == 1 || EPI == 5 || EPI == 6;
     const T al = pc ? ep.coef[s] : ep.alpha, be = pc ? ep.coef[LS + s] : ep.beta;
 #pragma unroll
     for (int k = 0; k < 12; k++) acc[k] = {al * acc[k].r, al * acc[k].i};
@@ -606,7 +610,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
       if (RUPD) {
         cx<T> rv[12];
         load_site<S>(rv, ep.rv, o);
-        const T a = (T)(-ep.acoef[s]);
+        const T a = (T)(-ep.acoef[ep.astride * s]);
 #pragma unroll
         for (int k = 0; k < 12; k++) {
           rv[k] = {fma(a, acc[k].r, rv[k].r), fma(a, acc[k].i, rv[k].i)};
@@ -688,7 +692,8 @@ __device__ __forceinline__ void hop_tile(const cx<S> *__restrict__ in, cx<S> *__
                                          bool sweep, int flags, const HopEpi<S> &ep, const GShape &sh,
                                          cx<typename Sto<S>::C> *lds) {
   typedef typename Sto<S>::C T;
-  constexpr bool MAT = EPI >= 2, PHI = (EPI & 1) != 0;
+  constexpr bool MAT = EPI == 2 || EPI == 3 || EPI == 7, PHI = EPI == 1 || EPI == 3 || EPI >= 5;
+  constexpr bool NORM = EPI >= 5, RUPD = EPI == 6 || EPI == 7;
   const uint32_t H4 = (uint32_t)g.H4, LS = sh.LS, epb = blockDim.x;
   const uint32_t t = threadIdx.x;
   const uint32_t hl = fdiv(t, sh.fLS), s = t - hl * LS;
@@ -720,15 +725,45 @@ __device__ __forceinline__ void hop_tile(const cx<S> *__restrict__ in, cx<S> *__
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
   if (valid) hop_site<T, S, DAG, true, 0, true>(acc, fa, fb, in, U, lds + hl * kLinkPad, nb, LS, s, h, H4);
   if (MAT) group_matrix<T>(acc, lds, ep.mat, sh, t, hl, s, epb);
+  double nv = 0;
   if (valid) {
     const uint32_t o = h * LS + s;
     if (EPI != 0) {
-      const T al = EPI == 1 ? ep.coef[s] : ep.alpha, be = EPI == 1 ? ep.coef[LS + s] : ep.beta;
+      const bool pc = EPI == 1 || EPI == 5 || EPI == 6;
+      const T al = pc ? ep.coef[s] : ep.alpha, be = pc ? ep.coef[LS + s] : ep.beta;
 #pragma unroll
       for (int k = 0; k < 12; k++) acc[k] = {al * acc[k].r, al * acc[k].i};
       if (PHI) axpy_site<S>(acc, be, ep.phi, o);
     }
-    store_site<S>(out, acc, o, (flags & 1) != 0);
+    if (RUPD) {
+      // the residual update in place of storing the result (k_dhop's EPI 6 / 7)
+      cx<T> rv[12];
+      load_site<S>(rv, ep.rv, o);
+      const T a = (T)(-ep.acoef[ep.astride * s]);
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        rv[k] = {fma(a, acc[k].r, rv[k].r), fma(a, acc[k].i, rv[k].i)};
+        nv += (double)fma(rv[k].r, rv[k].r, rv[k].i * rv[k].i);
+      }
+      store_site<S>(ep.rv, rv, o, false);
+    } else {
+      if (NORM)
+#pragma unroll
+        for (int k = 0; k < 12; k++) nv += (double)fma(acc[k].r, acc[k].r, acc[k].i * acc[k].i);
+      store_site<S>(out, acc, o, (flags & 1) != 0);
+    }
+  }
+  if constexpr (NORM) {
+    // per-slice sums over the tile's sites in fixed order (the persistent loop's barrier at the
+    // next tile orders these reads before the next writes)
+    __shared__ double nsh[64 * kGenMaxWaves];
+    nsh[t] = nv;
+    __syncthreads();
+    if (t < LS) {
+      double v = 0;
+      for (uint32_t j = 0; j < sh.SPB; j++) v += nsh[j * LS + t];
+      ep.npart[(size_t)tile * LS + t] = v;
+    }
   }
 }
 
@@ -1203,7 +1238,8 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   cx<S> *pout = (cx<S> *)out;
   const cx<S> *pU = (const cx<S> *)U;
   HopEpi<S> ep{(const cx<S> *)phi, (const T *)mat, (T)alpha, (T)beta, (const T *)coef,
-               fz ? fz->npart : nullptr,  (cx<S> *)(fz ? fz->r : nullptr),  fz ? fz->a : nullptr};
+               fz ? fz->npart : nullptr,  (cx<S> *)(fz ? fz->r : nullptr),  fz ? fz->a : nullptr,
+               fz ? fz->astride : 1};
   if constexpr (std::is_same<S, float>::value) {
     // the link products on the matrix cores: FP32 multi-RHS (Wilson, no matrix epilogue), Ls <= 16
     if (tune_get(GX_TUNE_DHOP_MFMA, prec_of<S>()) && kepi <= 1 && g.Ls >= 2 && g.Ls <= 16 && lb == g.Ls) {
@@ -1325,11 +1361,21 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
                      ntiles, map, flags, ep, sh, blocks, persist)
 #define GX_DHOPG_EPI(EPI) \
   if (dag) GX_DHOPG_GO(true, EPI); else GX_DHOPG_GO(false, EPI)
-    switch (kepi) {
-      case 0: GX_DHOPG_EPI(0); break;
-      case 1: GX_DHOPG_EPI(1); break;
-      case 2: GX_DHOPG_EPI(2); break;
-      default: GX_DHOPG_EPI(3); break;
+    // fused reductions: EPI 5 (kepi 1, dag 0), 6 (kepi 1, dag 1, r), 7 (kepi 3, dag 1, r)
+    const bool gnorm = fz && fz->npart && fz->nparts &&
+                       ((kepi == 1 && (fz->r ? (dag && fz->a) : !dag)) || (kepi == 3 && dag && fz->r && fz->a));
+    if (gnorm) {
+      *fz->nparts = ntiles;
+      if (!dag) GX_DHOPG_GO(false, 5);
+      else if (kepi == 3) GX_DHOPG_GO(true, 7);
+      else GX_DHOPG_GO(true, 6);
+    } else {
+      switch (kepi) {
+        case 0: GX_DHOPG_EPI(0); break;
+        case 1: GX_DHOPG_EPI(1); break;
+        case 2: GX_DHOPG_EPI(2); break;
+        default: GX_DHOPG_EPI(3); break;
+      }
     }
 #undef GX_DHOPG_EPI
 #undef GX_DHOPG_GO
@@ -1340,8 +1386,9 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   const uint32_t ntiles = (uint32_t)((g.H4 + spb - 1) / spb);
   const uint32_t pad = (map == 1 || map == 4) ? 8 : (map == 2 ? 64 : (map == 3 ? 256 : 1));
   const uint32_t blocks = (ntiles + pad - 1) / pad * pad;
-  const bool norm = fz && fz->npart && fz->nparts && kepi == 1 && g.Ls >= 2 &&
-                    (fz->r ? (dag && fz->a) : !dag);
+  // EPI 5: kepi 1, dag 0; EPI 6: kepi 1, dag 1, r; EPI 7: kepi 3 (matrix and phi), dag 1, r
+  const bool norm = fz && fz->npart && fz->nparts && g.Ls >= 2 &&
+                    ((kepi == 1 && (fz->r ? (dag && fz->a) : !dag)) || (kepi == 3 && dag && fz->r && fz->a));
   if (norm) *fz->nparts = ntiles;
 #define GX_DHOP_GO(LS, DAG, PF, ORD, EPI)                                                                         \
   hipLaunchKernelGGL((k_dhop<S, LS, DAG, PF, ORD, EPI>), dim3(blocks), dim3(256), 0, st, pin, pout, pU, k, cb_out, \
@@ -1358,7 +1405,10 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   case LS:                                                       \
     if (norm) {                                                  \
       if constexpr (LS >= 2) {                                   \
-        if (dag) {                                               \
+        if (dag && kepi == 3) {                                  \
+          if (pf) GX_DHOP_GO(LS, true, true, 0, 7);              \
+          else GX_DHOP_GO(LS, true, false, 0, 7);                \
+        } else if (dag) {                                        \
           if (pf) GX_DHOP_GO(LS, true, true, 0, 6);              \
           else GX_DHOP_GO(LS, true, false, 0, 6);                \
         } else {                                                 \
@@ -2719,6 +2769,33 @@ int launch_fold_norms(Ctx *ctx, const double *npart, int64_t ntiles, int Ls, Mrh
                      ctx->red_partial);
   hipLaunchKernelGGL(k_mrhs_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, (int)nb, Ls,
                      state, nullptr, which);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+// the same tile sums added over every tile and slice into block partials of the single-RHS CG
+// (fixed order: thread, wave, workgroup)
+static __global__ void __launch_bounds__(256) k_fold_all(const double *__restrict__ tp, int64_t n, const CGState *st,
+                                                         double *partial) {
+  if (gated(st)) return;
+  __shared__ double sh[1][4];
+  double v[1] = {0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[0] += tp[i];
+  block_sum<1>(v, sh);
+  write_partial<1>(v, partial);
+}
+
+int launch_fold_cg(Ctx *ctx, const double *npart, int64_t ntiles, int Ls, CGState *state, int which) {
+  if (Ls < 1 || ntiles < 1 || (which != 0 && which != 1)) {
+    set_error("fold cg: %lld tiles of %d slices (which %d)", (long long)ntiles, Ls, which);
+    return GX_EINVAL;
+  }
+  const int64_t n = ntiles * Ls;
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_fold_all, dim3(nb), dim3(256), 0, ctx->stream, npart, n, (const CGState *)state,
+                     ctx->red_partial);
+  hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb, state, which);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }

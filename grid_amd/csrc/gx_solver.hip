@@ -70,6 +70,7 @@ struct CGWork {
   CGState *host = nullptr;  // pinned, 2 slots
   hipEvent_t ev[2] = {};
   BatchGraph graph[3];  // per precision
+  double *npart = nullptr;  // the hop tile sums (MrhsWork::npart, owned there)
 };
 
 struct MrhsWork {
@@ -77,7 +78,7 @@ struct MrhsWork {
   MrhsState *host = nullptr;  // pinned, 2 slots
   double *out_dev = nullptr;  // per-slice results
   double *out_host = nullptr; // pinned mirror
-  double *npart = nullptr;    // per-tile slice sums of |Mpc p|^2 (multi-RHS Wilson operators)
+  double *npart = nullptr;    // per-tile slice sums of the fused hop epilogues (Ls >= 2)
   size_t npart_bytes = 0;
   hipEvent_t ev[2] = {};
   BatchGraph graph[3];  // per precision
@@ -125,9 +126,10 @@ static int solver_work(Op *op, SolverWork **out) {
     if (e == hipSuccess) e = hipHostMalloc(&m.host, 2 * sizeof(MrhsState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m.out_dev, sizeof(double) * kMaxRhs);
     if (e == hipSuccess) e = hipHostMalloc(&m.out_host, sizeof(double) * kMaxRhs, hipHostMallocDefault);
-    if (e == hipSuccess && op->kind == GX_WILSON && op->g.Ls >= 2 && op->g.Ls <= kMaxRhs) {
+    if (e == hipSuccess && op->g.Ls >= 2 && op->g.Ls <= kMaxRhs) {
       m.npart_bytes = hop_norm_bytes(op->g.H4, op->g.Ls);
       e = hipMalloc(&m.npart, m.npart_bytes);
+      c.npart = m.npart;
     }
     for (int i = 0; i < 2 && e == hipSuccess; i++) {
       e = hipEventCreateWithFlags(&c.ev[i], hipEventDisableTiming);
@@ -148,13 +150,40 @@ size_t solver_work_bytes(const SolverWork *w) { return w ? w->mrhs.npart_bytes :
 
 size_t hop_norm_bytes(int64_t H4, int Ls) { return sizeof(double) * (size_t)hop_norm_tiles(H4, Ls) * Ls; }
 
+// FP64 follows Grid (d = <p, MpcDag Mpc p>, then r -= a MpcDag Mpc p with its norm).  In FP32 and
+// FP16 storage, where the tuned hop kernel has the fused epilogues (HopFuse, Ls >= 2), the Mpc
+// hop sums |t|^2 (t = Mpc p) per tile, folded to d = |t|^2 and a, and the last MpcDag hop updates
+// r -= a MpcDag t and sums |r|^2 per tile without storing MpcDag t, as in the multi-RHS CG.
 template <typename T>
-static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGState *st) {
-  GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
+static int cg_iteration(Op *op, Field *psi, Field *p, Field *mmp, Field *r, CGState *st, double *npart) {
   Ctx *ctx = op->ctx;
   const int64_t nsites = p->g.half_pad();
-  GX_TRY(launch_cg_dot<T>(ctx, nsites, nsites, p->data, mmp->data, st));
-  GX_TRY(launch_cg_axpy_norm<T>(ctx, nsites, nsites, r->data, mmp->data, st));
+  int64_t n1 = 0, n2 = 0;
+  Field *t = nullptr;
+  if (!std::is_same<T, double>::value && npart) {
+    t = op_tmp(op, p->prec, 2);
+    if (!t) return GX_ENOMEM;
+    HopFuse f1;
+    f1.npart = npart;
+    f1.nparts = &n1;
+    GX_TRY(apply_mpc(op, p, t, 0, st, &f1));
+  }
+  if (n1 == 0) {
+    if (t) GX_TRY(apply_mpc(op, t, mmp, 1, st));
+    else GX_TRY(apply_mpcdagmpc(op, p, mmp, st));
+    GX_TRY(launch_cg_dot<T>(ctx, nsites, nsites, p->data, mmp->data, st));
+  } else {
+    GX_TRY(launch_fold_cg(ctx, npart, n1, op->g.Ls, st, 0));
+    HopFuse f2;
+    f2.npart = npart;
+    f2.nparts = &n2;
+    f2.r = r->data;
+    f2.a = &st->a;  // device address of the state's a (not dereferenced here)
+    f2.astride = 0;
+    GX_TRY(apply_mpc(op, t, mmp, 1, st, &f2));
+  }
+  if (n2 > 0) GX_TRY(launch_fold_cg(ctx, npart, n2, op->g.Ls, st, 1));
+  else GX_TRY(launch_cg_axpy_norm<T>(ctx, nsites, nsites, r->data, mmp->data, st));
   GX_TRY(launch_cg_update<T>(ctx, nsites, nsites, psi->data, p->data, r->data, st));
   return GX_OK;
 }
@@ -173,7 +202,7 @@ static int cg_device_loop(Op *op, int prec, Field *psi, Field *p, Field *mmp, Fi
     if (!op_tmp(op, prec, i)) return GX_ENOMEM;  // no allocation inside a capture
   auto one_batch = [&]() -> int {
     for (int i = 0; i < batch; i++) {
-      GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
+      GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev, w.npart); }));
     }
     return GX_OK;
   };
@@ -185,7 +214,7 @@ static int cg_device_loop(Op *op, int prec, Field *psi, Field *p, Field *mmp, Fi
       enqueued += batch;
     } else {
       for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
-        GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev); }));
+        GX_TRY(by_prec(prec, [&](auto t) { return cg_iteration<decltype(t)>(op, psi, p, mmp, r, w.dev, w.npart); }));
       }
     }
     GX_CHECK_HIP(hipMemcpyAsync(&w.host[s], w.dev, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));

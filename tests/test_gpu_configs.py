@@ -153,8 +153,11 @@ def test_c2_schur_solve_matches_oracle(ctx):
         x = fx.download()
         assert st['converged'] and st['unprec_residual'] < TARGET, (mixed, st)
         assert full_residual(O, x, src) < TARGET, mixed
+        # FP64 CG follows the oracle's iteration closely; a defect-correction solution differs from
+        # the FP64 one by up to the condition number (~1e4 at mf 0.01) times the 1e-12 stopping
+        # residuals, whatever the inner arithmetic's summation order
         err = float(np.linalg.norm((x - ref).ravel()) / np.linalg.norm(ref.ravel()))
-        assert err < 1e-8, (mixed, err)
+        assert err < (1e-8 if mixed == 0 else 3e-8), (mixed, err)
         if mixed == 0:
             assert abs(st['iterations'] - rst['iterations']) <= 2, (st, rst)
     D.close()

@@ -1,4 +1,4 @@
-"""Multi-RHS CG with its reductions in the hop epilogues, through the C ABI.
+"""CG with its reductions in the hop epilogues, through the C ABI.
 
 With FP32 / FP16-storage inner solves, CGmultiRHS takes d_s = |Mpc p_s|^2 from per-tile sums the
 Mpc hop leaves, and the last MpcDag hop updates r_s -= a_s (MpcDag Mpc p)_s and sums |r_s|^2 without
@@ -11,12 +11,18 @@ solves the same 12-RHS Wilson system (one mass per right-hand side, 8x4x4x8) bot
     slice's mass (WilsonFermionImplementation.h:71,113-176);
   - the fused and the separate-pass solutions agree to 1e-9 relative per slice (both are FP64
     defect-correction results at the 1e-11 red-black tolerance).
+
+The single-RHS CG of FP32 / FP16-storage inner solves does the same on Mobius operators (the Mpc
+hop's |t|^2 over all slices; r -= a MpcDag t in the matrix epilogue of the last MpcDag hop):
+Mobius 8^4 x Ls8 defect correction, fused and separate-pass, each against the oracle's red-black CG
+solution (ConjugateGradient.h:68-257, CayleyFermion5DImplementation.h:155-329) to 1e-7.
 """
 import numpy as np
 import pytest
 
 import grid_amd as gx
-from oracle.pyoracle import Oracle
+from oracle.pyoracle import Oracle, ODD
+import npref as R
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +91,29 @@ def test_fused_reductions_match_separate_passes(ctx, system, mixed):
         for k in range(NRHS):
             d = np.linalg.norm(sl(sols[name], k) - sl(ref, k)) / np.linalg.norm(sl(ref, k))
             assert d <= 1e-9, (name, k, mixed, d)
+
+
+@pytest.mark.parametrize('inner', [gx.F32, gx.F16])
+def test_single_rhs_fused_reductions(ctx, inner):
+    dims, Ls, mass = (8, 8, 8, 8), 8, 0.1
+    U = gx.random_gauge(dims, 73, 0.4)
+    src = R.pick(gx.gaussian_fermion(dims, Ls, 74), dims, ODD)
+    kw = dict(mass=mass, M5=1.8, b=1.5, c=0.5)
+    ref, _ = Oracle(U, dims, Ls, 'mobius', **kw).cg(src, tol=1e-10)
+    D = gx.FermionOp(ctx, U, dims, Ls, 'mobius', **kw)
+    fs = D.field(gx.F64, ODD).upload(src)
+    for name in ('fused', 'general', 'rows'):
+        knobs = KNOBS[name]
+        saved = {k: gx.tune_get(k, inner) for k in knobs}
+        for k, v in knobs.items():
+            gx.tune_set(k, v, inner)
+        try:
+            fx = D.field(gx.F64, ODD)
+            st = D.mixed_cg(fs, fx, tol=1e-10, inner_tol=1e-5, inner_prec=inner)
+            x = fx.download()
+        finally:
+            for k, v in saved.items():
+                gx.tune_set(k, v, inner)
+        assert st['converged'] and st['true_residual'] < 1e-9, (name, st)
+        assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-7, name
+    D.close()

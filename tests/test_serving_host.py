@@ -30,8 +30,8 @@ def test_footprint_closed_form():
     dims, nsrc = (16, 16, 16, 32), 7
     hb = np.prod(dims) // 2 * nsrc * 12 * 16
     coef = 48 * nsrc  # the operator's per-slice Schur coefficients: 4 doubles and 4 floats per slice
-    # the multi-RHS CG's slice sums of |Mpc p|^2: one double per slice per hop tile of 4 (64 // nsrc) sites
-    spb = 4 * (64 // nsrc)
+    # the fused hop epilogues' slice sums: one double per slice per tile, at most one tile per 64 // nsrc sites
+    spb = 64 // nsrc
     tiles = 8 * nsrc * (-(-np.prod(dims) // 2 // spb))
     assert serving.footprint(dims, nsrc, mixed=True) == 24.5 * hb + coef + tiles
     assert serving.footprint(dims, nsrc, mixed=False) == 18.0 * hb + coef + tiles
