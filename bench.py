@@ -464,14 +464,17 @@ def bench_graphs(gx, ctx):
     return out
 
 
-def bench_serve_sweep(n, rates, slots=4, mixed=2, fused_rates=(), fuse_rhs=16):
+def bench_serve_sweep(n, rates, slots=4, mixed=2, fused_rates=(), fuse_rhs=16, wide_rates=(), wide_slots=8):
     """Leg 'sweep': the same Poisson trace shape at several arrival rates, so capacity and latency
     are reported together (p50/p99 grow once the rate passes the GPU's service rate); at
     `fused_rates` also with fused solves (waiting requests of one lattice and gauge field join an
-    admitted request's solve, up to fuse_rhs right-hand sides, a mass per slice)."""
+    admitted request's solve, up to fuse_rhs right-hand sides, a mass per slice); at `wide_rates`
+    also with `wide_slots` slots (more solves in flight on the one GPU)."""
     out = {f'{r:g}_per_s': bench_serving(n, r, slots, mixed) for r in rates}
     for r in fused_rates:
         out[f'{r:g}_per_s_fused{fuse_rhs}'] = bench_serving(n, r, slots, mixed, fuse_rhs=fuse_rhs)
+    for r in wide_rates:
+        out[f'{r:g}_per_s_{wide_slots}slots'] = bench_serving(n, r, wide_slots, mixed)
     return out
 
 
@@ -520,6 +523,9 @@ def main():
     ap.add_argument('--sweep-rates', default='20,30,40,50')
     ap.add_argument('--sweep-fused-rates', default='40,50',
                     help='rates at which the sweep also serves with fused solves (16 right-hand sides)')
+    ap.add_argument('--sweep-wide-rates', default='50,60',
+                    help='rates at which the sweep also serves with --sweep-wide-slots slots')
+    ap.add_argument('--sweep-wide-slots', type=int, default=8)
     ap.add_argument('--legs', default='dhop,large,cg,c3,c4,serve,sweep,graphs,cpu,ms,block',
                     help='comma list of legs to run: dhop, large (DhopEO at >= 1 GB working sets), cg, c3, c4, '
                          'serve, sweep (serving at several rates), graphs (hipGraph against eager CG), cpu, '
@@ -627,7 +633,9 @@ def main():
              if args.serve_requests > 0 else None)
     sweep = (leg('sweep', lambda: bench_serve_sweep(args.sweep_requests, [float(r) for r in args.sweep_rates.split(',')],
                                                     args.serve_slots, args.serve_mixed,
-                                                    [float(r) for r in args.sweep_fused_rates.split(',') if r]))
+                                                    [float(r) for r in args.sweep_fused_rates.split(',') if r],
+                                                    wide_rates=[float(r) for r in args.sweep_wide_rates.split(',') if r],
+                                                    wide_slots=args.sweep_wide_slots))
              if 'sweep' in legs and args.sweep_requests > 0 else None)
     graphs = leg('graphs', lambda: bench_graphs(gx, ctx)) if 'graphs' in legs else None
     ms = leg('ms', lambda: bench_multishift(gx, ctx)) if 'ms' in legs else None

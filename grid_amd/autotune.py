@@ -196,6 +196,20 @@ class Autotuner:
             with Knobs(hop_best + c, prec):
                 return self._time_cg(D, fs, fx, cg_iters)
 
+        # FP32 / FP16 CG folds its reductions into hop epilogues that only the one-workgroup-per-tile
+        # launches have: when the fastest hop is a persistent grid, keep it only if a CG iteration
+        # under it beats one under the fastest non-persistent launch
+        if prec != F64 and dict(hop_best).get(TUNE_DHOP_PERSIST, 0):
+            plain = [c for c in hop_times if not dict(c).get(TUNE_DHOP_PERSIST, 0)]
+            if plain:
+                alt = min(plain, key=lambda c: hop_times[c])
+
+                def cg_hop(h):
+                    with Knobs(h, prec):
+                        return self._time_cg(D, fs, fx, cg_iters)
+
+                hop_best, t4 = search([hop_best, alt], cg_hop, rounds)
+
         solver_best, solver_times = search(solver_candidates(), cg, rounds)
         entry = {'knobs': {KNOB_NAMES[k]: v for k, v in hop_best + solver_best},
                  'dhop_seconds': hop_times[hop_best],

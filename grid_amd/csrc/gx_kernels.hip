@@ -1361,8 +1361,9 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
                      ntiles, map, flags, ep, sh, blocks, persist)
 #define GX_DHOPG_EPI(EPI) \
   if (dag) GX_DHOPG_GO(true, EPI); else GX_DHOPG_GO(false, EPI)
-    // fused reductions: EPI 5 (kepi 1, dag 0), 6 (kepi 1, dag 1, r), 7 (kepi 3, dag 1, r)
-    const bool gnorm = fz && fz->npart && fz->nparts &&
+    // fused reductions: EPI 5 (kepi 1, dag 0), 6 (kepi 1, dag 1, r), 7 (kepi 3, dag 1, r); one
+    // workgroup per tile only (the persistent grid runs the separate passes)
+    const bool gnorm = !persist && fz && fz->npart && fz->nparts &&
                        ((kepi == 1 && (fz->r ? (dag && fz->a) : !dag)) || (kepi == 3 && dag && fz->r && fz->a));
     if (gnorm) {
       *fz->nparts = ntiles;

@@ -344,8 +344,38 @@ int cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver_stat
   return GX_OK;
 }
 
+// Outer exit and leg targets of defect correction.  kGridRule is Grid's (ConjugateGradientMixedPrec.h
+// :117-151, ConjugateGradientMixedPrecBatched.h:100-213): the loop ends once every |r|^2 <=
+// OuterLoopNormMult (100) x stop and leaves the last factor to the FP64 patch-up CG; the inner
+// tolerance only ever doubles from InnerTolerance.  kConvergeRule (the multi-RHS red-black solves
+// and block CG of this library) applies to FP32 inner solves: one FP64 iteration costs about
+// two FP32 ones, so the loop runs until every right-hand side is converged (multiplier 1), and a leg
+// whose needed reduction t = kLand sqrt(stop_max / |r|^2_max) is within one FP32 leg's reach
+// (t >= kReach32) aims there, the others at the caller's inner tolerance: with a first leg at 1e-6
+// the second lands below stop, and the patch-up CG, still run from the fresh residual as in Grid,
+// finds it converged and returns at once.  FP16-storage inner solves keep Grid's rule (one such leg
+// gains three to four orders, so the legs, not the tolerance, set the count).
+enum { kGridRule = 0, kConvergeRule = 1 };
+constexpr double kLand = 0.3;
+constexpr double kReach32 = 1e-7;
+
+struct OuterPolicy {
+  bool grid;
+  double mult;
+  OuterPolicy(int rule, int inner_prec) : grid(rule == kGridRule || inner_prec != GX_F32), mult(grid ? 100.0 : 1.0) {}
+  // the next leg's inner tolerance from the largest residual and stopping norms
+  void next_tol(double &inner_tol, double inner_tol0, double nmax, double smax) const {
+    if (grid) {
+      while (nmax * inner_tol * inner_tol < smax) inner_tol *= 2;
+    } else {
+      const double t = kLand * std::sqrt(smax / nmax);
+      inner_tol = t >= kReach32 ? t : std::max(inner_tol0, t);
+    }
+  }
+};
+
 int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int inner_prec, int maxinner,
-                 int maxouter, gx_solver_stats *out, int maxpatch = -1) {
+                 int maxouter, gx_solver_stats *out, int maxpatch = -1, int rule = kGridRule) {
   auto t0 = std::chrono::steady_clock::now();
   gx_solver_stats st{};
   const int cb = src_in->cb;
@@ -359,7 +389,7 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
   tmp_d->cb = src_d->cb = src_f->cb = sol_f->cb = cb;
   double inner_tol = inner_tol0;
   int outer = 0;
-  const double OuterLoopNormMult = 100.0;
+  const OuterPolicy pol(rule, inner_prec);
   double sol_norm;
   GX_TRY(blas_norm2(sol, &sol_norm));
   bool sol_zero = sol_norm == 0.0;
@@ -376,8 +406,8 @@ int mixed_cg_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol
       res = src_d;
     }
     fresh = true;
-    if (norm < OuterLoopNormMult * stop) break;
-    while (norm * inner_tol * inner_tol < stop) inner_tol *= 2;
+    if (norm < pol.mult * stop) break;
+    pol.next_tol(inner_tol, inner_tol0, norm, stop);
     GX_TRY(convert_field(src_f, res));
     GX_CHECK_HIP(hipMemsetAsync(sol_f->data, 0, sol_f->bytes, op->ctx->stream));
     gx_solver_stats ist{};
@@ -754,21 +784,10 @@ int mrhs_cg_run(Op *op, Field *src, Field *psi, double tol, int maxit, gx_solver
 
 // Batched defect correction (ConjugateGradientMixedPrecBatched.h:100-213) with the inner solves of
 // all right-hand sides done together by one multi-RHS CG in FP32 or FP16 storage (inner_prec), so
-// they share every link load.
-//
-// Outer exit and leg targets.  Grid's loop ends once every |r_s|^2 <= OuterLoopNormMult (100) x
-// stop_s and leaves the last factor to the FP64 patch-up CG; its inner tolerance only ever doubles
-// from InnerTolerance.  FP16-storage inner solves keep that rule (one such leg gains three to four
-// orders, so the legs, not the tolerance, set the count).  With FP32 inner solves one multi-RHS
-// FP64 iteration costs about two inner ones, so the loop runs until every slice is converged
-// (multiplier 1), and a leg whose needed reduction t = kLand sqrt(stop_max / |r|^2_max) is
-// within one FP32 leg's reach (t >= kReach32) aims there, the others at the caller's inner
-// tolerance: the second leg then lands below stop, and the patch-up CG, still run from the fresh
-// residual as in Grid, finds every slice converged and returns at once.
-constexpr double kLand = 0.3;
-constexpr double kReach32 = 1e-7;
+// they share every link load.  rule: the outer policy (OuterPolicy), on the largest residual and
+// stopping norm over the right-hand sides.
 int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_tol0, int inner_prec, int maxinner,
-                   int maxouter, gx_solver_stats *st, int maxpatch = -1) {
+                   int maxouter, gx_solver_stats *st, int maxpatch = -1, int rule = kGridRule) {
   auto t0 = std::chrono::steady_clock::now();
   const int Ls = op->g.Ls, cb = src_in->cb;
   sol->cb = cb;
@@ -786,8 +805,7 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
   for (int s = 0; s < Ls; s++) sol_zero = sol_zero && sn[s] == 0.0;
   Field *res = src_d;   // src - A sol of the latest check
   bool fresh = false;   // res belongs to the current sol
-  const bool grid_rule = inner_prec == GX_F16;
-  const double outer_mult = grid_rule ? 100.0 : 1.0;
+  const OuterPolicy pol(rule, inner_prec);
   double inner_tol = inner_tol0;
   for (outer = 0; outer < maxouter; outer++) {
     if (sol_zero) {
@@ -802,17 +820,12 @@ int mrhs_mixed_run(Op *op, Field *src_in, Field *sol, double tol, double inner_t
     bool all = true;
     double nmax = 0, smax = 0;
     for (int s = 0; s < Ls; s++) {
-      if (norm[s] > outer_mult * stop[s]) all = false;
+      if (norm[s] > pol.mult * stop[s]) all = false;
       nmax = std::max(nmax, norm[s]);
       smax = std::max(smax, stop[s]);
     }
     if (all) break;
-    if (grid_rule) {
-      while (nmax * inner_tol * inner_tol < smax) inner_tol *= 2;
-    } else {
-      const double t = kLand * std::sqrt(smax / nmax);
-      inner_tol = t >= kReach32 ? t : std::max(inner_tol0, t);
-    }
+    pol.next_tol(inner_tol, inner_tol0, nmax, smax);
     GX_TRY(convert_field(src_f, res));
     GX_CHECK_HIP(hipMemsetAsync(sol_f->data, 0, sol_f->bytes, op->ctx->stream));
     std::vector<gx_solver_stats> ist(Ls);
@@ -1500,7 +1513,9 @@ int gx_block_cg(gx_op op_, gx_field src_, gx_field psi_, double tol, int maxit, 
     return GX_EINVAL;
   }
   if (inner_prec == GX_F64) return mrhs_cg_run(op, src, psi, tol, maxit, st);
-  if (inner_prec == GX_F32 || inner_prec == GX_F16) return mrhs_mixed_run(op, src, psi, tol, 1e-5, inner_prec, maxit, 100, st);
+  if (inner_prec == GX_F32 || inner_prec == GX_F16)
+    return mrhs_mixed_run(op, src, psi, tol, inner_prec == GX_F32 ? 1e-6 : 1e-5, inner_prec, maxit, 100, st, -1,
+                          kConvergeRule);
   set_error("block_cg: inner precision %d unsupported", inner_prec);
   return GX_EINVAL;
 }
@@ -1796,7 +1811,7 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
       // faster); 1e-5 with FP16 storage, whose legs gain three to four orders whatever they aim at.
       std::vector<gx_solver_stats> bst(Ls);
       rc = mixed ? mrhs_mixed_run(op, so, solo, tol, mixed == 2 ? 1e-5 : 1e-6, mixed == 2 ? GX_F16 : GX_F32, maxit,
-                                  100, bst.data())
+                                  100, bst.data(), -1, kConvergeRule)
                  : mrhs_cg_run(op, so, solo, tol, maxit, bst.data());
       if (st) {
         *st = bst[0];
@@ -1811,6 +1826,8 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
     } else if (mixed == 3) {  // ConjugateGradientReliableUpdate, FP32 sloppy vectors, delta 0.1
       rc = ru_cg_run(op, so, solo, tol, maxit, 0.1, GX_F32, -1, 0.0, st);
     } else if (mixed) {
+      // Grid's outer rule: on the ill-conditioned single-RHS Mobius systems the FP32 policy of the
+      // multi-RHS solves took more inner iterations (C2 558 against 482, C3 694 against 594)
       rc = mixed_cg_run(op, so, solo, tol, 1e-5, mixed == 2 ? GX_F16 : GX_F32, maxit, 100, st);
     } else {
       rc = cg_run(op, so, solo, tol, maxit, st);

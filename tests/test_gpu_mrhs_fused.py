@@ -3,8 +3,9 @@
 With FP32 / FP16-storage inner solves, CGmultiRHS takes d_s = |Mpc p_s|^2 from per-tile sums the
 Mpc hop leaves, and the last MpcDag hop updates r_s -= a_s (MpcDag Mpc p)_s and sums |r_s|^2 without
 storing MpcDag Mpc p (BlockConjugateGradient.h:412-545 computes the same scalars from separate
-passes).  Only the default 4-wave hop kernel has these epilogues; under a general tile shape, the
-row-tile kernel or (FP32) the matrix-core kernel the iteration runs the separate passes.  This test
+passes).  The default 4-wave hop kernel and the general tile shape have these epilogues; under the
+persistent general grid, the row-tile kernel or (FP32) the matrix-core kernel the iteration runs
+the separate passes.  This test
 solves the same 12-RHS Wilson system (one mass per right-hand side, 8x4x4x8) both ways:
 
   - every slice of every solve reaches |b - M x| / |b| <= 1e-10 with the oracle's Wilson M of that
