@@ -523,7 +523,7 @@ def main():
     ap.add_argument('--sweep-rates', default='20,30,40,50')
     ap.add_argument('--sweep-fused-rates', default='40,50',
                     help='rates at which the sweep also serves with fused solves (16 right-hand sides)')
-    ap.add_argument('--sweep-wide-rates', default='50,60',
+    ap.add_argument('--sweep-wide-rates', default='60',
                     help='rates at which the sweep also serves with --sweep-wide-slots slots')
     ap.add_argument('--sweep-wide-slots', type=int, default=8)
     ap.add_argument('--legs', default='dhop,large,cg,c3,c4,serve,sweep,graphs,cpu,ms,block',
