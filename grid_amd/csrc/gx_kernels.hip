@@ -1972,13 +1972,26 @@ __global__ void __launch_bounds__(256) k_cg_update(int64_t nsites, int64_t strid
   const T a = (T)st->a, b = (T)st->b;
   const bool keep_p = st->ru == 1;  // reliable update pending: the host forms the new p
   if constexpr (!Sto<S>::H) {
-    const int64_t n = 12 * nsites;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-      const cx<T> pv = p[i], xv = psi[i];
-      psi[i] = {fma(a, pv.r, xv.r), fma(a, pv.i, xv.i)};
+    // one 16-byte group (VW complex numbers) per thread and step
+    constexpr int VW = Sto<S>::VW;
+    struct alignas(16) G {
+      cx<T> v[VW];
+    };
+    G *gx_ = reinterpret_cast<G *>(psi), *gp = reinterpret_cast<G *>(p);
+    const G *gr = reinterpret_cast<const G *>(r);
+    const int64_t ng = 12 * nsites / VW;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ng; q += (int64_t)gridDim.x * blockDim.x) {
+      const G pv = gp[q];
+      G xv = gx_[q];
+#pragma unroll
+      for (int j = 0; j < VW; j++) xv.v[j] = {fma(a, pv.v[j].r, xv.v[j].r), fma(a, pv.v[j].i, xv.v[j].i)};
+      gx_[q] = xv;
       if (!keep_p) {
-        const cx<T> rv = r[i];
-        p[i] = {fma(b, pv.r, rv.r), fma(b, pv.i, rv.i)};
+        const G rv = gr[q];
+        G np;
+#pragma unroll
+        for (int j = 0; j < VW; j++) np.v[j] = {fma(b, pv.v[j].r, rv.v[j].r), fma(b, pv.v[j].i, rv.v[j].i)};
+        gp[q] = np;
       }
     }
   } else {
@@ -2020,7 +2033,7 @@ int launch_cg_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const
 }
 template <typename S>
 int launch_cg_update(Ctx *ctx, int64_t nsites, int64_t stride, void *psi, void *p, const void *r, CGState *state) {
-  const int64_t work = Sto<S>::H ? nsites : 12 * nsites;
+  const int64_t work = Sto<S>::H ? nsites : 12 * nsites / Sto<S>::VW;
   hipLaunchKernelGGL((k_cg_update<S>), dim3(stream_blocks(work)), dim3(256), 0, ctx->stream, nsites, stride,
                      (cx<S> *)psi, (cx<S> *)p, (const cx<S> *)r, (const CGState *)state);
   GX_CHECK_HIP(hipGetLastError());
