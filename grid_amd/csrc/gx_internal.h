@@ -323,6 +323,29 @@ struct MultiShiftArgs {
   int nshift, do_p;
 };
 int launch_multishift(Ctx *ctx, int64_t n, const void *r, void *p, const MultiShiftArgs &m);
+// Device-resident scalars of the multishift CG from its second iteration on (Grid's recurrences,
+// ConjugateGradientMultiShift.h:225-300): the lightest system's c, cp, a, b, bp, d and each shift's
+// z (current zc, previous zp), bs and stopping norm rsq; active: shifts still iterating (bit s),
+// closing: the shifts whose psi the next update closes.  k, k_conv gate the kernels as in CGState.
+struct MsState : Gate {
+  double c, cp, a, b, bp, d, m0;
+  double zc[kMaxShift], zp[kMaxShift], bs[kMaxShift], dm[kMaxShift], rsq[kMaxShift];
+  uint32_t active, closing;
+  int nshift, pad;
+  int its[kMaxShift];
+};
+struct MsPtrs {
+  void *ps[kMaxShift];
+  void *psi[kMaxShift];
+};
+// one multishift iteration's reductions and vector updates on FP64 half fields of nsites sites
+// (stride per half) with the state on the device: d from (p, mmp) (mmp = A p, before the m0
+// shift) and the z / bs recurrences; r += b mmp with |r|^2, the stopping tests and a; then the
+// psi / p / ps updates of k_multishift with the state's coefficients.  dot: 0 -> after the hop,
+// 1 -> after mmp += m0 p
+int launch_ms_dot(Ctx *ctx, int64_t nsites, int64_t stride, const void *p, const void *mmp, MsState *st);
+int launch_ms_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const void *mmp, MsState *st);
+int launch_ms_update(Ctx *ctx, int64_t n, const void *r, void *p, const MsPtrs &m, MsState *st);
 // Sums over the lattice of the six plaquettes' and four links' real traces of a device gauge field
 // ([site][mu][3][3] complex double): out_host[0] (plaquettes), out_host[1] (links); out_host holds
 // 2 nparts doubles.  Synchronises the stream.

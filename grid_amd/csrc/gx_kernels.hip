@@ -2564,6 +2564,158 @@ static __global__ void __launch_bounds__(256) k_multishift(int64_t n, const cx<d
   }
 }
 
+// ---------------- device-resident multishift recurrences ----------------
+// Re<p|A p> and |p|^2 (A p before the m0 shift), block partials
+static __global__ void __launch_bounds__(256) k_ms_dot(int64_t nsites, int64_t stride, const cx<double> *__restrict__ p,
+                                                       const cx<double> *__restrict__ mmp, const MsState *st,
+                                                       double *partial) {
+  if (gated(st)) return;
+  __shared__ double sh[2][4];
+  double v[2] = {0, 0};
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t hf = j / stride, i = j - hf * stride, base = hf * 12 * stride;
+    cx<double> xv[12], yv[12];
+    load_site<double>(xv, p + base, i);
+    load_site<double>(yv, mmp + base, i);
+    double sr = 0, sp = 0;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      sr = fma(xv[k].r, yv[k].r, sr);
+      sr = fma(xv[k].i, yv[k].i, sr);
+      sp = fma(xv[k].r, xv[k].r, sp);
+      sp = fma(xv[k].i, xv[k].i, sp);
+    }
+    v[0] += sr;
+    v[1] += sp;
+  }
+  block_sum<2>(v, sh);
+  write_partial<2>(v, partial);
+}
+
+// d = Re<p|A p> + m0 |p|^2, b = -cp / d, and the shifts' z and bs (ConjugateGradientMultiShift.h
+// :238-262), with the a of the previous iteration
+static __global__ void __launch_bounds__(256) k_ms_fin_d(const double *partial, int nparts, MsState *st) {
+  if (gated(st)) return;
+  __shared__ double sh[2][4];
+  __shared__ double nb[2];  // the new b and the previous one
+  double v[2];
+  sum_partials<2>(v, partial, nparts, sh);
+  if (threadIdx.x == 0) {
+    const double d = v[0] + st->m0 * v[1], cp = st->c, bp = st->b, b = -cp / d;
+    st->d = d;
+    st->cp = cp;
+    st->bp = bp;
+    st->b = b;
+    st->bs[0] = b;
+    nb[0] = b;
+    nb[1] = bp;
+  }
+  __syncthreads();
+  const int s = threadIdx.x;
+  if (s >= 1 && s < st->nshift && ((st->active >> s) & 1u)) {
+    const double a = st->a, b = nb[0], bp = nb[1];
+    const double z0 = st->zc[s], z1 = st->zp[s];
+    const double zn = z0 * z1 * bp / (b * a * (z1 - z0) + z1 * bp * (1.0 - st->dm[s] * b));
+    st->zp[s] = z0;
+    st->zc[s] = zn;
+    st->bs[s] = b * zn / z0;
+  }
+}
+
+// r += b mmp (mmp shifted by m0) and |r|^2, block partials
+static __global__ void __launch_bounds__(256) k_ms_axpy_norm(int64_t nsites, int64_t stride, cx<double> *__restrict__ r,
+                                                             const cx<double> *__restrict__ mmp, const MsState *st,
+                                                             double *partial) {
+  if (gated(st)) return;
+  __shared__ double sh[1][4];
+  const double b = st->b;
+  double v[1] = {0};
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nsites; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t hf = j / stride, i = j - hf * stride, base = hf * 12 * stride;
+    cx<double> xv[12], yv[12];
+    load_site<double>(xv, mmp + base, i);
+    load_site<double>(yv, r + base, i);
+    double sr = 0;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      yv[k] = {fma(b, xv[k].r, yv[k].r), fma(b, xv[k].i, yv[k].i)};
+      sr = fma(yv[k].r, yv[k].r, sr);
+      sr = fma(yv[k].i, yv[k].i, sr);
+    }
+    store_site<double>(r + base, yv, i, false);
+    v[0] += sr;
+  }
+  block_sum<1>(v, sh);
+  write_partial<1>(v, partial);
+}
+
+// c = |r|^2; each active shift's stopping test c z_s^2 < rsq_s; a = c / cp; k += 1 (also once
+// stopped, so the gate of the update that closes the last iteration opens exactly once)
+static __global__ void __launch_bounds__(256) k_ms_fin_c(const double *partial, int nparts, MsState *st) {
+  const bool skip = gated(st);
+  __shared__ double sh[1][4];
+  double v[1] = {0};
+  if (!skip) sum_partials<1>(v, partial, nparts, sh);
+  if (threadIdx.x == 0) {
+    if (!skip) {
+      const double c = v[0];
+      st->c = c;
+      st->closing = st->active;
+      uint32_t active = st->active;
+      for (int s = 0; s < st->nshift; s++) {
+        if (!((active >> s) & 1u)) continue;
+        st->its[s] = st->k;
+        if (c * st->zc[s] * st->zc[s] < st->rsq[s]) active &= ~(1u << s);
+      }
+      st->active = active;
+      st->a = c / st->cp;
+      if (active == 0 && st->k_conv == INT_MAX) st->k_conv = st->k;
+    }
+    st->k = st->k + 1;
+  }
+}
+
+// k_multishift with the coefficients of the state (psi_s -= bs_s ps_s for the closing shifts;
+// p = a p + r and ps_s = z_s r + as_s ps_s for the active ones), gated on the iteration it closes
+static __global__ void __launch_bounds__(256) k_ms_update(int64_t n, const cx<double> *__restrict__ r,
+                                                          cx<double> *__restrict__ p, MsPtrs m, const MsState *st) {
+  if (st->k - 1 > st->k_conv || st->k - 1 > st->kmax) return;
+  __shared__ double cbs[kMaxShift], cz[kMaxShift], cas[kMaxShift];
+  const int ns = st->nshift;
+  const uint32_t closing = st->closing, active = st->active;
+  const double a = st->a;
+  if (threadIdx.x < ns) {
+    const int s = threadIdx.x;
+    cbs[s] = st->bs[s];
+    cz[s] = s == 0 ? 1.0 : st->zc[s];
+    cas[s] = s == 0 ? a : a * st->zc[s] * st->bs[s] / (st->zp[s] * st->b);
+  }
+  __syncthreads();
+  const bool do_p = active != 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const cx<double> rv = r[i];
+    if (do_p) {
+      const cx<double> pv = p[i];
+      p[i] = {fma(a, pv.r, rv.r), fma(a, pv.i, rv.i)};
+    }
+    for (int s = 0; s < ns; s++) {
+      const bool dpsi = (closing >> s) & 1u, dps = (active >> s) & 1u;
+      if (!dpsi && !dps) continue;
+      cx<double> *ps = (cx<double> *)m.ps[s];
+      cx<double> pv = ps[i];
+      if (dpsi) {
+        cx<double> *psi = (cx<double> *)m.psi[s];
+        const cx<double> xv = psi[i];
+        psi[i] = {fma(-cbs[s], pv.r, xv.r), fma(-cbs[s], pv.i, xv.i)};
+      }
+      if (dps) {
+        pv = {fma(cz[s], rv.r, cas[s] * pv.r), fma(cz[s], rv.i, cas[s] * pv.i)};
+        ps[i] = pv;
+      }
+    }
+  }
+}
+
 // ---------------- slice matrices (Lattice_reduction.h:717-776) ----------------
 // FP64 fields viewed as (rows) x Ls with the fifth-dimension slice s innermost: entry e = h Ls + s
 // of half hf stores component k at hf * hstride + ((e / kBlk) * 12 + k) * kBlk + e % kBlk.
@@ -2905,6 +3057,31 @@ int launch_gauge_stats(Ctx *ctx, const int dims[4], const void *U_dev, double *p
   }
   out_host[0] = p;
   out_host[1] = t;
+  return GX_OK;
+}
+
+int launch_ms_dot(Ctx *ctx, int64_t nsites, int64_t stride, const void *p, const void *mmp, MsState *st) {
+  const int nb = red_blocks(nsites);
+  hipLaunchKernelGGL(k_ms_dot, dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (const cx<double> *)p,
+                     (const cx<double> *)mmp, (const MsState *)st, ctx->red_partial);
+  hipLaunchKernelGGL(k_ms_fin_d, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb, st);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int launch_ms_axpy_norm(Ctx *ctx, int64_t nsites, int64_t stride, void *r, const void *mmp, MsState *st) {
+  const int nb = red_blocks(nsites);
+  hipLaunchKernelGGL(k_ms_axpy_norm, dim3(nb), dim3(256), 0, ctx->stream, nsites, stride, (cx<double> *)r,
+                     (const cx<double> *)mmp, (const MsState *)st, ctx->red_partial);
+  hipLaunchKernelGGL(k_ms_fin_c, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->red_partial, nb, st);
+  GX_CHECK_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+int launch_ms_update(Ctx *ctx, int64_t n, const void *r, void *p, const MsPtrs &m, MsState *st) {
+  hipLaunchKernelGGL(k_ms_update, dim3(stream_blocks(n)), dim3(256), 0, ctx->stream, n, (const cx<double> *)r,
+                     (cx<double> *)p, m, (const MsState *)st);
+  GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
 

@@ -84,9 +84,16 @@ struct MrhsWork {
   BatchGraph graph[3];  // per precision
 };
 
+struct MsWork {
+  MsState *dev = nullptr;
+  MsState *host = nullptr;  // pinned, 2 slots
+  hipEvent_t ev[2] = {};
+};
+
 struct SolverWork {
   CGWork cg;
   MrhsWork mrhs;
+  MsWork ms;  // allocated by the first multishift solve
 };
 
 static void free_graphs(BatchGraph *g, int n) {
@@ -107,9 +114,13 @@ void solver_work_free(SolverWork *w) {
   if (m.host) (void)hipHostFree(m.host);
   if (m.out_host) (void)hipHostFree(m.out_host);
   if (m.npart) (void)hipFree(m.npart);
+  MsWork &ms = w->ms;
+  if (ms.dev) (void)hipFree(ms.dev);
+  if (ms.host) (void)hipHostFree(ms.host);
   for (int i = 0; i < 2; i++) {
     if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
     if (m.ev[i]) (void)hipEventDestroy(m.ev[i]);
+    if (ms.ev[i]) (void)hipEventDestroy(ms.ev[i]);
   }
   delete w;
 }
@@ -1284,8 +1295,9 @@ static int block_cgrq_vec_run(Op *op, int herm, int n, Field **B, Field **X, dou
 
 // ConjugateGradientMultiShift (ConjugateGradientMultiShift.h:89-343): (MpcDagMpc + m_s) psi_s = src
 // for all shifts m_s >= m_0 from the Krylov space of the lightest (Jegerlehner's shifted CG).  The
-// scalar recurrences (a, b, z_s, bs_s) run on the host in FP64, in Grid's order; psi_s start at
-// zero; a shift stops updating once c z_s^2 < m_residual_s^2 |src|^2.  FP64 fields.
+// scalar recurrences (a, b, z_s, bs_s) run in FP64 in Grid's order: the start on the host, every
+// iteration after it in device finalisers (MsState), so the host never waits inside an iteration;
+// psi_s start at zero; a shift stops updating once c z_s^2 < m_residual_s^2 |src|^2.  FP64 fields.
 static int multishift_run(Op *op, Field *src, int nshift, const double *mass, const double *mres, Field **psi,
                           int maxit, gx_solver_stats *st) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -1380,38 +1392,90 @@ static int multishift_run(Op *op, Field *src, int nshift, const double *mass, co
   uint32_t active = nshift == 32 ? 0xffffffffu : ((1u << nshift) - 1u);
   double a = c / cp;  // the a of the iteration being opened
   GX_TRY(update(0u, active, a));  // iteration 1: p and ps_s
-  int k, rc = GX_ENOCONV;
-  for (k = 1; k <= maxit; k++) {
-    cp = c;
-    GX_TRY(shifted_op(d));
-    const double bp = b;
-    b = -cp / d;
-    GX_TRY(blas_axpy_norm(r, b, mmp, r, &c));
-    bs[0] = b;
-    for (int s = 1; s < nshift; s++) {  // toggling z[s][iz] <-> z[s][1-iz]: the new z1 is the old z0
-      if (conv[s]) continue;
-      const double z0 = zc[s], z1 = zp[s];
-      const double zn = z0 * z1 * bp / (b * a * (z1 - z0) + z1 * bp * (1.0 - (mass[s] - mass[0]) * b));
-      zp[s] = z0;
-      zc[s] = zn;
-      bs[s] = b * zn / z0;
-    }
-    const uint32_t closing = active;
-    for (int s = 0; s < nshift; s++) {
-      if (conv[s]) continue;
-      its[s] = k;
-      if (c * zc[s] * zc[s] < rsq[s]) {
-        conv[s] = 1;
-        active &= ~(1u << s);
-      }
-    }
-    a = c / cp;
-    GX_TRY(update(closing, active, a));
-    if (!active) {
-      rc = GX_OK;
-      break;
+  // Iterations 1, 2, ... with the recurrences on the device (k_ms_fin_d, k_ms_fin_c) and batches
+  // enqueued two ahead, as in CG: the host only reads the state between batches.
+  SolverWork *sw = nullptr;
+  GX_TRY(solver_work(op, &sw));
+  MsWork &w = sw->ms;
+  if (!w.dev) {
+    hipError_t e = hipMalloc(&w.dev, sizeof(MsState));
+    if (e == hipSuccess) e = hipHostMalloc(&w.host, 2 * sizeof(MsState), hipHostMallocDefault);
+    for (int i = 0; i < 2 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&w.ev[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      set_error("multishift state: %s", hipGetErrorString(e));
+      return GX_ENOMEM;
     }
   }
+  MsState init{};
+  init.k = 1;
+  init.k_conv = INT_MAX;
+  init.kmax = maxit;
+  init.c = c;
+  init.cp = cp;
+  init.a = a;
+  init.b = b;
+  init.m0 = mass[0];
+  for (int s = 0; s < nshift; s++) {
+    init.zc[s] = zc[s];
+    init.zp[s] = zp[s];
+    init.bs[s] = bs[s];
+    init.dm[s] = mass[s] - mass[0];
+    init.rsq[s] = rsq[s];
+  }
+  init.active = active;
+  init.closing = 0;
+  init.nshift = nshift;
+  w.host[0] = init;
+  GX_CHECK_HIP(hipMemcpyAsync(w.dev, &w.host[0], sizeof(MsState), hipMemcpyHostToDevice, stream));
+  GX_CHECK_HIP(hipStreamSynchronize(stream));
+  MsPtrs ptrs{};
+  for (int s = 0; s < nshift; s++) {
+    ptrs.ps[s] = ps[s]->data;
+    ptrs.psi[s] = psi[s]->data;
+  }
+  Ctx *ctx = op->ctx;
+  const int64_t nsites = r->g.half_pad();
+  const int64_t vol = op->g.half_elems();
+  const int batch = vol >= (1 << 20) ? 8 : (vol >= (1 << 16) ? 16 : 32);
+  int enqueued = 0, slot = 0;
+  auto enqueue = [&](int sl) -> int {
+    for (int i = 0; i < batch && enqueued < maxit; i++, enqueued++) {
+      GX_TRY(apply_mpcdagmpc(op, p, mmp, w.dev));
+      GX_TRY(launch_ms_dot(ctx, nsites, nsites, p->data, mmp->data, w.dev));
+      GX_TRY(blas_axpby(mmp, mass[0], p, 1.0, mmp, w.dev));
+      GX_TRY(launch_ms_axpy_norm(ctx, nsites, nsites, r->data, mmp->data, w.dev));
+      GX_TRY(launch_ms_update(ctx, nst, r->data, p->data, ptrs, w.dev));
+    }
+    GX_CHECK_HIP(hipMemcpyAsync(&w.host[sl], w.dev, sizeof(MsState), hipMemcpyDeviceToHost, stream));
+    GX_CHECK_HIP(hipEventRecord(w.ev[sl], stream));
+    return GX_OK;
+  };
+  GX_TRY(enqueue(0));
+  GX_TRY(enqueue(1));
+  MsState fin{};
+  for (;;) {
+    GX_CHECK_HIP(hipEventSynchronize(w.ev[slot]));
+    fin = w.host[slot];
+    if (fin.k_conv != INT_MAX || fin.k > maxit) break;
+    if (enqueued >= maxit) {
+      GX_CHECK_HIP(hipEventSynchronize(w.ev[1 - slot]));
+      fin = w.host[1 - slot];
+      break;
+    }
+    GX_TRY(enqueue(slot));
+    slot = 1 - slot;
+  }
+  // the final state (launches past the stop leave it as the stop found it, but for k)
+  GX_CHECK_HIP(hipMemcpyAsync(&w.host[0], w.dev, sizeof(MsState), hipMemcpyDeviceToHost, stream));
+  GX_CHECK_HIP(hipStreamSynchronize(stream));
+  fin = w.host[0];
+  int rc = fin.active == 0 ? GX_OK : GX_ENOCONV;
+  for (int s = 0; s < nshift; s++) {
+    its[s] = fin.its[s];
+    conv[s] = ((fin.active >> s) & 1u) ? 0 : 1;
+    zc[s] = fin.zc[s];
+  }
+  c = fin.c;
   // true residuals |(A + m_s) psi_s - src| / |src|
   double ssq;
   GX_TRY(blas_norm2(src, &ssq));
