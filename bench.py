@@ -80,16 +80,20 @@ def pick(a, dims, cb):
     return np.ascontiguousarray(np.take_along_axis(a, idx, axis=3))
 
 
-def traffic_from_profile():
-    """Measured HBM bytes per DhopEO launch from the committed rocprofv3 counter summary."""
-    p = os.path.join(ROOT, 'profiles', 'dhop_mobius_16x16x16x16_ls16_f64_traffic.json')
-    if not os.path.exists(p):
-        return None
-    try:
-        with open(p) as f:
-            return json.load(f).get('traffic_bytes_per_launch')
-    except Exception:
-        return None
+def traffic_from_profile(kernel, shape=0):
+    """Measured HBM bytes per launch of the FP64 DhopEO kernel `kernel` at tile shape `shape` from the
+    committed rocprofv3 counter summaries (profiles/dhop_mobius_16x16x16x16_ls16_f64*_traffic.json,
+    one per kernel and shape); (bytes, file) or (None, None) when no summary covers them."""
+    import glob
+    for p in sorted(glob.glob(os.path.join(ROOT, 'profiles', 'dhop_mobius_16x16x16x16_ls16_f64*_traffic.json'))):
+        try:
+            with open(p) as f:
+                d = json.load(f)
+        except Exception:
+            continue
+        if d.get('kernel') == kernel and int(d.get('shape', 0)) == int(shape):
+            return d.get('traffic_bytes_per_launch'), os.path.relpath(p, ROOT)
+    return None, None
 
 
 def time_dhop(ctx, D, fi, fo, steps, warmup):
@@ -201,7 +205,7 @@ def bench_dhop_large(gx, ctx, tune):
                                 ((32, 32, 32, 32), gx.F32, 'mobius_32x32x32x32_ls16_f32', 8)):
         U = gx.random_gauge(dims, 1012, EPS)
         D = gx.FermionOp(ctx, U, dims, LS, 'mobius', **PARAMS)
-        tune('dhop_' + name, D, prec)
+        tune('dhop_' + name, D, prec, dhop_iters=50, rounds=5)
         fi = D.field(prec, 1)
         full = D.field(prec)
         full.gaussian(2012)
@@ -224,7 +228,7 @@ def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tu
     """Mobius 24^4 x Ls12 (b=1.5, c=0.5, M5=1.8, mf=0.005, eps 0.3), Gaussian 5D source: red-black
     solve in FP64 CG, in defect correction with FP32 and with FP16-storage inner solves (inner
     tolerance 1e-5, FP64 outer residual and patch-up), and in reliable-update CG (FP32 Krylov
-    vectors, FP64 reliable updates with delta 0.1, FP64 clean-up), each to |b - M x| / |b| <= 1e-10.
+    vectors, FP64 reliable updates with delta 0.01, FP64 clean-up), each to |b - M x| / |b| <= 1e-10.
     Wall clock around each solve."""
     kw = dict(mass=0.005, M5=1.8, b=1.5, c=0.5)
     U = gx.random_gauge(dims, 1003, 0.3)
@@ -587,12 +591,12 @@ def main():
     D = gx.FermionOp(ctx, U, DIMS, LS, 'mobius', **PARAMS)
     H = np.prod(DIMS) // 2
     res = {}
-    kernel64 = None
+    kernel64, shape64 = None, 0
     for prec, name, c in ((gx.F64, 'f64', 16), (gx.F32, 'f32', 8), (gx.F16, 'f16', 4)):
         print(f'bench: dhop {name} ... ({time.perf_counter() - T_START:.0f} s)', file=sys.stderr, flush=True)
-        tune('dhop_' + name, D, prec)
+        tune('dhop_' + name, D, prec, dhop_iters=50, rounds=5)
         if prec == gx.F64:
-            kernel64 = dhop_kernel_name(gx)
+            kernel64, shape64 = dhop_kernel_name(gx), gx.tune_get(gx.TUNE_DHOP_SHAPE)
         fi = D.field(prec, 1).upload(psi_o)
         fo = D.field(prec, 0)
         ms = time_dhop(ctx, D, fi, fo, args.steps, args.warmup)
@@ -663,6 +667,7 @@ def main():
                **cb}
 
     r = res['f64']
+    traffic = traffic_from_profile(kernel64, shape64)
     out = {
         'metric': METRIC,
         'value': round(r['gbs'], 2),
@@ -679,9 +684,10 @@ def main():
         'config': {'workload': 'Mobius DWF DhopEO odd->even', 'dims': list(DIMS), 'Ls': LS, **PARAMS,
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': round(r['gbs'], 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(r['gbs'] / HBM_PEAK_GBS, 4), 'traffic': traffic_from_profile(),
-                     'traffic_source': 'profiles/dhop_mobius_16x16x16x16_ls16_f64_traffic.json (rocprofv3 FETCH_SIZE '
-                                       'and WRITE_SIZE runs of bench.py --legs dhop, scaled by a streaming pass)',
+                     'frac': round(r['gbs'] / HBM_PEAK_GBS, 4), 'traffic': traffic[0],
+                     'traffic_source': (f'{traffic[1]} (rocprofv3 FETCH_SIZE and WRITE_SIZE runs of bench.py --legs '
+                                        'dhop with this kernel, scaled by a streaming pass)') if traffic[1] else
+                                       f'no committed counter summary for {kernel64}',
                      'kernel': kernel64, 'algorithmic_bytes_per_launch': r['bytes']},
         'cpu_baseline': cpu,
         'dhop_hbm_sized': large,

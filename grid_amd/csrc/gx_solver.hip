@@ -1888,8 +1888,10 @@ int gx_schur_solve(gx_op op_, gx_field src_, gx_field sol_, double tol, int maxi
           st->computed_residual = std::max(st->computed_residual, bst[k].computed_residual);
         }
       }
-    } else if (mixed == 3) {  // ConjugateGradientReliableUpdate, FP32 sloppy vectors, delta 0.1
-      rc = ru_cg_run(op, so, solo, tol, maxit, 0.1, GX_F32, -1, 0.0, st);
+    } else if (mixed == 3) {
+      // ConjugateGradientReliableUpdate, FP32 sloppy vectors, delta 0.01: on the C3 system the
+      // same 636 iterations with 15 FP64 updates instead of 31 at delta 0.1, 6% faster
+      rc = ru_cg_run(op, so, solo, tol, maxit, 0.01, GX_F32, -1, 0.0, st);
     } else if (mixed) {
       // Grid's outer rule: on the ill-conditioned single-RHS Mobius systems the FP32 policy of the
       // multi-RHS solves took more inner iterations (C2 558 against 482, C3 694 against 594).

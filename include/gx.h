@@ -227,8 +227,9 @@ int gx_reliable_update_cg_fallback(gx_op op, gx_field src, gx_field psi, double 
 
 /* Grid: SchurRedBlackDiagMooeeSolve<Field>(CG)(Dw, src, sol), SchurRedBlack.h:236-291,373-436.
  * src, sol: full F64 fields.  mixed = 1 (2) runs the red-black solve as gx_mixed_cg with FP32
- * (FP16-storage) inner solves, mixed = 0 as FP64 gx_cg, mixed = 3 (single-RHS operators) as
- * gx_reliable_update_cg with FP32 sloppy vectors and delta 0.1.
+ * (FP16-storage) inner solves from inner_tol 1e-5 (1e-3), mixed = 0 as FP64 gx_cg, mixed = 3
+ * (single-RHS operators) as
+ * gx_reliable_update_cg with FP32 sloppy vectors and delta 0.01.
  * unprec_residual = |M sol - src| / |src|.  On a multi-RHS Wilson operator (Ls > 1) the
  * red-black solve is gx_block_cg with inner_prec F64, F32 or F16 for mixed = 0, 1, 2 (every
  * right-hand side to tol), st holds the largest iteration
