@@ -529,7 +529,7 @@ def main():
     ap.add_argument('--serve-mixed', type=int, default=2, choices=(1, 2),
                     help='inner solves of the served requests: 1 FP32, 2 FP16 storage')
     ap.add_argument('--sweep-requests', type=int, default=384)
-    ap.add_argument('--sweep-rates', default='20,30,40,50')
+    ap.add_argument('--sweep-rates', default='20,30,40,50,60')
     ap.add_argument('--sweep-fused-rates', default='40,50',
                     help='rates at which the sweep also serves with fused solves (16 right-hand sides)')
     ap.add_argument('--sweep-wide-rates', default='60',
