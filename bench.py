@@ -21,7 +21,7 @@ Further legs, reported beside the headline value (each can be switched off):
   serve  a Poisson trace of Wilson solve requests (volumes 8^4 .. 24^4, 1-12 sources, masses
          0.05-0.3, 8 resident gauge fields per volume) through the serving runtime: throughput and
          p50/p99 latency (--serve-requests 4096 for the full trace);
-  sweep  the serving trace at several arrival rates (20-50/s, 4 slots), with fused solves
+  sweep  the serving trace at several arrival rates (20-60/s, 4 slots), with fused solves
          (--sweep-fused-rates) and with 8 slots (--sweep-wide-rates): capacity against p50/p99;
   graphs hipGraph-captured CG batches against eager ones (C1, C2 FP32, C4 12-RHS FP32);
   large  DhopEO at >= 1 GB working sets (Mobius 24^4 x Ls16 FP64, 32^4 x Ls16 FP32);
