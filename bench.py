@@ -205,7 +205,7 @@ def bench_dhop_large(gx, ctx, tune):
                                 ((32, 32, 32, 32), gx.F32, 'mobius_32x32x32x32_ls16_f32', 8)):
         U = gx.random_gauge(dims, 1012, EPS)
         D = gx.FermionOp(ctx, U, dims, LS, 'mobius', **PARAMS)
-        tune('dhop_' + name, D, prec, dhop_iters=50, rounds=5)
+        tune('dhop_' + name, D, prec)
         fi = D.field(prec, 1)
         full = D.field(prec)
         full.gaussian(2012)
