@@ -145,6 +145,20 @@ __device__ __forceinline__ void load_site(cx<typename Sto<S>::C> *f, const cx<S>
   }
 }
 
+// The 12 components of entry i as stored, and the factor they stand for: FP16 storage returns
+// the site's scale unapplied (the hop folds it into its reconstruction, 24 multiplies fewer per
+// leg), other storages 1.
+template <typename S, typename I>
+__device__ __forceinline__ typename Sto<S>::C load_site_raw(cx<typename Sto<S>::C> *f, const cx<S> *__restrict__ p,
+                                                            I i) {
+  constexpr int VW = Sto<S>::VW;
+  const cx<S> *__restrict__ q = p + bidx<S, I>(i, 0);
+#pragma unroll
+  for (int g = 0; g < 12 / VW; g++) load_group<S>(f + g * VW, q, g);
+  if constexpr (Sto<S>::H) return *reinterpret_cast<const float *>(p + sidx<S, I>(i));
+  else return (typename Sto<S>::C)1;
+}
+
 // acc += beta * (entry i of p), group by group (FP16 with the site's scale folded into beta)
 template <typename S, typename I>
 __device__ __forceinline__ void axpy_site(cx<typename Sto<S>::C> *acc, typename Sto<S>::C beta,
@@ -325,6 +339,45 @@ __device__ __forceinline__ void recon_colour(cx<T> *a, int c, cx<T> h0, cx<T> h1
   }
 }
 
+// recon_colour of (sc h0, sc h1): the factor rides in the accumulation (one FMA per component
+// instead of an add); FP16 storage, whose site scale is applied here
+// x + sc h, x - sc h, x + i sc h, x - i sc h
+template <typename T>
+__device__ __forceinline__ cx<T> cadd_s(cx<T> x, cx<T> h, T sc) { return {fma(sc, h.r, x.r), fma(sc, h.i, x.i)}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub_s(cx<T> x, cx<T> h, T sc) { return {fma(-sc, h.r, x.r), fma(-sc, h.i, x.i)}; }
+template <typename T>
+__device__ __forceinline__ cx<T> caddi_s(cx<T> x, cx<T> h, T sc) { return {fma(-sc, h.i, x.r), fma(sc, h.r, x.i)}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csubi_s(cx<T> x, cx<T> h, T sc) { return {fma(sc, h.i, x.r), fma(-sc, h.r, x.i)}; }
+
+template <typename T, int MU, bool PLUS>
+__device__ __forceinline__ void recon_colour_s(cx<T> *a, int c, cx<T> h0, cx<T> h1, T sc) {
+#define add(x, h) cadd_s<T>(x, h, sc)
+#define sub(x, h) csub_s<T>(x, h, sc)
+#define addi(x, h) caddi_s<T>(x, h, sc)
+#define subi(x, h) csubi_s<T>(x, h, sc)
+  a[c] = add(a[c], h0);
+  a[3 + c] = add(a[3 + c], h1);
+  if (MU == 0) {
+    if (PLUS) { a[6 + c] = subi(a[6 + c], h1); a[9 + c] = subi(a[9 + c], h0); }
+    else      { a[6 + c] = addi(a[6 + c], h1); a[9 + c] = addi(a[9 + c], h0); }
+  } else if (MU == 1) {
+    if (PLUS) { a[6 + c] = add(a[6 + c], h1); a[9 + c] = sub(a[9 + c], h0); }
+    else      { a[6 + c] = sub(a[6 + c], h1); a[9 + c] = add(a[9 + c], h0); }
+  } else if (MU == 2) {
+    if (PLUS) { a[6 + c] = subi(a[6 + c], h0); a[9 + c] = addi(a[9 + c], h1); }
+    else      { a[6 + c] = addi(a[6 + c], h0); a[9 + c] = subi(a[9 + c], h1); }
+  } else {
+    if (PLUS) { a[6 + c] = add(a[6 + c], h0); a[9 + c] = add(a[9 + c], h1); }
+    else      { a[6 + c] = sub(a[6 + c], h0); a[9 + c] = sub(a[9 + c], h1); }
+  }
+#undef add
+#undef sub
+#undef addi
+#undef subi
+}
+
 // row r of U times the colour vector h (one spin row of the projected half-spinor)
 template <typename T>
 __device__ __forceinline__ cx<T> su3_row(const cx<T> *u, const cx<T> *h) {
@@ -391,7 +444,7 @@ __device__ __forceinline__ void site_neighbours(const KGeom &g, uint32_t h, int 
 // reconstruct, one colour row at a time (three link entries and two results live at once).
 template <typename T, typename S, int MU, bool PLUS, bool LDSU>
 __device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<S> *__restrict__ U,
-                                         int P, uint32_t h, uint32_t H4) {
+                                         int P, uint32_t h, uint32_t H4, T sc) {
   cx<T> hs[6];
   project<T, MU, PLUS>(f, hs);
 #pragma unroll
@@ -399,7 +452,10 @@ __device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T>
     cx<T> u[3];
 #pragma unroll
     for (int b = 0; b < 3; b++) u[b] = LDSU ? ul[P * 9 + r * 3 + b] : ldc<S>(U, (P * 9 + r * 3 + b) * H4 + h);
-    recon_colour<T, MU, PLUS>(acc, r, su3_row<T>(u, hs), su3_row<T>(u, hs + 3));
+    if constexpr (Sto<S>::H)  // f as stored; its site scale sc enters the accumulation
+      recon_colour_s<T, MU, PLUS>(acc, r, su3_row<T>(u, hs), su3_row<T>(u, hs + 3), sc);
+    else
+      recon_colour<T, MU, PLUS>(acc, r, su3_row<T>(u, hs), su3_row<T>(u, hs + 3));
   }
 }
 
@@ -415,9 +471,9 @@ struct LegSel {
 
 template <typename T, typename S, bool DAG, int ORD, int LEG, bool LDSU>
 __device__ __forceinline__ void do_leg(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<S> *__restrict__ U,
-                                       uint32_t h, uint32_t H4) {
+                                       uint32_t h, uint32_t H4, T sc) {
   typedef LegSel<DAG, ORD, LEG> L;
-  leg_math<T, S, L::MU, L::PLUS, LDSU>(acc, f, ul, U, L::P, h, H4);
+  leg_math<T, S, L::MU, L::PLUS, LDSU>(acc, f, ul, U, L::P, h, H4, sc);
 }
 
 template <bool DAG, int ORD, int LEG>
@@ -425,53 +481,54 @@ __device__ __forceinline__ uint32_t leg_idx(const uint32_t *nb, uint32_t Ls, uin
   return nb[LegSel<DAG, ORD, LEG>::P] * Ls + s;
 }
 
-// The 8 legs of one output site.  The caller has issued the neighbour loads of leg 0 (into fa)
-// and, with PF, of leg 1 (into fb); with PF the loads of leg L+1 are in flight while leg L is
-// computed (sched_barrier keeps the compiler from sinking them).
+// The 8 legs of one output site.  The caller has issued the neighbour loads of leg 0 (into fa,
+// its factor in sa) and, with PF, of leg 1 (into fb, sb); with PF the loads of leg L+1 are in
+// flight while leg L is computed (sched_barrier keeps the compiler from sinking them).  Inputs
+// stay as stored (load_site_raw): FP16 site scales enter the reconstruction.
 template <typename T, typename S, bool DAG, bool PF, int ORD, bool LDSU>
-__device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, const cx<S> *__restrict__ in,
+__device__ __forceinline__ void hop_site(cx<T> *acc, cx<T> *fa, cx<T> *fb, T &sa, T &sb, const cx<S> *__restrict__ in,
                                          const cx<S> *__restrict__ U, const cx<T> *ul, const uint32_t *nb,
                                          uint32_t LS, uint32_t s, uint32_t h, uint32_t H4) {
 #define IDX(L) leg_idx<DAG, ORD, L>(nb, LS, s)
-#define LEG(L, F) do_leg<T, S, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4)
+#define LEG(L, F, SC) do_leg<T, S, DAG, ORD, L, LDSU>(acc, F, ul, U, h, H4, SC)
   if (PF) {
     __builtin_amdgcn_sched_barrier(0);
-    LEG(0, fa);
-    load_site<S>(fa, in, IDX(2));
+    LEG(0, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(2));
     __builtin_amdgcn_sched_barrier(0);
-    LEG(1, fb);
-    load_site<S>(fb, in, IDX(3));
+    LEG(1, fb, sb);
+    sb = load_site_raw<S>(fb, in, IDX(3));
     __builtin_amdgcn_sched_barrier(0);
-    LEG(2, fa);
-    load_site<S>(fa, in, IDX(4));
+    LEG(2, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(4));
     __builtin_amdgcn_sched_barrier(0);
-    LEG(3, fb);
-    load_site<S>(fb, in, IDX(5));
+    LEG(3, fb, sb);
+    sb = load_site_raw<S>(fb, in, IDX(5));
     __builtin_amdgcn_sched_barrier(0);
-    LEG(4, fa);
-    load_site<S>(fa, in, IDX(6));
+    LEG(4, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(6));
     __builtin_amdgcn_sched_barrier(0);
-    LEG(5, fb);
-    load_site<S>(fb, in, IDX(7));
+    LEG(5, fb, sb);
+    sb = load_site_raw<S>(fb, in, IDX(7));
     __builtin_amdgcn_sched_barrier(0);
-    LEG(6, fa);
-    LEG(7, fb);
+    LEG(6, fa, sa);
+    LEG(7, fb, sb);
   } else {
-    LEG(0, fa);
-    load_site<S>(fa, in, IDX(1));
-    LEG(1, fa);
-    load_site<S>(fa, in, IDX(2));
-    LEG(2, fa);
-    load_site<S>(fa, in, IDX(3));
-    LEG(3, fa);
-    load_site<S>(fa, in, IDX(4));
-    LEG(4, fa);
-    load_site<S>(fa, in, IDX(5));
-    LEG(5, fa);
-    load_site<S>(fa, in, IDX(6));
-    LEG(6, fa);
-    load_site<S>(fa, in, IDX(7));
-    LEG(7, fa);
+    LEG(0, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(1));
+    LEG(1, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(2));
+    LEG(2, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(3));
+    LEG(3, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(4));
+    LEG(4, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(5));
+    LEG(5, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(6));
+    LEG(6, fa, sa);
+    sa = load_site_raw<S>(fa, in, IDX(7));
+    LEG(7, fa, sa);
   }
 #undef IDX
 #undef LEG
@@ -550,8 +607,8 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
     }
   }
   cx<T> fa[12], fb[12];
-  load_site<S>(fa, in, leg_idx<DAG, ORD, 0>(nb, LS, s));
-  if (PF) load_site<S>(fb, in, leg_idx<DAG, ORD, 1>(nb, LS, s));
+  T sa = load_site_raw<S>(fa, in, leg_idx<DAG, ORD, 0>(nb, LS, s)), sb = (T)1;
+  if (PF) sb = load_site_raw<S>(fb, in, leg_idx<DAG, ORD, 1>(nb, LS, s));
   if (LDSU) {
     if (ugrp < (uint32_t)G) {
 #pragma unroll
@@ -567,7 +624,7 @@ __global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  if (valid) hop_site<T, S, DAG, PF, ORD, LDSU>(acc, fa, fb, in, U, ul, nb, LS, s, h, H4);
+  if (valid) hop_site<T, S, DAG, PF, ORD, LDSU>(acc, fa, fb, sa, sb, in, U, ul, nb, LS, s, h, H4);
   const uint32_t o = h * LS + s;
   if (MAT) {
     // every wave has finished reading links: reuse the LDS for the hop results [w][k][lane]
@@ -712,8 +769,8 @@ __device__ __forceinline__ void hop_tile(const cx<S> *__restrict__ in, cx<S> *__
     ureg[j] = pk < 72 ? ldc<S>(U, pk * H4 + h) : cx<T>{0, 0};
   }
   cx<T> fa[12], fb[12];
-  load_site<S>(fa, in, leg_idx<DAG, 0, 0>(nb, LS, s));
-  load_site<S>(fb, in, leg_idx<DAG, 0, 1>(nb, LS, s));
+  T sa = load_site_raw<S>(fa, in, leg_idx<DAG, 0, 0>(nb, LS, s));
+  T sb = load_site_raw<S>(fb, in, leg_idx<DAG, 0, 1>(nb, LS, s));
 #pragma unroll
   for (int j = 0; j < kGenLinkPerThread; j++) {
     const uint32_t pk = s + LS * j;
@@ -723,7 +780,7 @@ __device__ __forceinline__ void hop_tile(const cx<S> *__restrict__ in, cx<S> *__
   cx<T> acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
-  if (valid) hop_site<T, S, DAG, true, 0, true>(acc, fa, fb, in, U, lds + hl * kLinkPad, nb, LS, s, h, H4);
+  if (valid) hop_site<T, S, DAG, true, 0, true>(acc, fa, fb, sa, sb, in, U, lds + hl * kLinkPad, nb, LS, s, h, H4);
   if (MAT) group_matrix<T>(acc, lds, ep.mat, sh, t, hl, s, epb);
   double nv = 0;
   if (valid) {
@@ -851,9 +908,9 @@ constexpr int rows_max_threads() { return sizeof(typename Sto<S>::C) == 8 ? 512 
 // t legs and the y / z legs across the tile's faces from global memory, one global leg ahead).
 template <typename T, typename S, bool DAG, int RY, int RZ>
 __device__ __forceinline__ void rows_legs(cx<T> *acc, const cx<S> *__restrict__ in, const cx<S> *__restrict__ U,
-                                          const cx<T> *lin, const cx<T> *ul, const uint32_t *nb, uint32_t rowh,
-                                          uint32_t r, uint32_t w, uint32_t RW, uint32_t nent, uint32_t LS, uint32_t s,
-                                          uint32_t h, uint32_t H4) {
+                                          const cx<T> *lin, const T *lsc, const cx<T> *ul, const uint32_t *nb,
+                                          uint32_t rowh, uint32_t r, uint32_t w, uint32_t RW, uint32_t nent, uint32_t LS,
+                                          uint32_t s, uint32_t h, uint32_t H4) {
   // neighbour P of leg L (Grid's order) and whether it is in the tile
   constexpr auto P = [](int L) constexpr { return L < 4 ? ((DAG ? 0 : 4) + (L & 3)) : ((DAG ? 4 : 0) + (L & 3)); };
   constexpr auto in_tile = [](int p) constexpr {
@@ -870,24 +927,29 @@ __device__ __forceinline__ void rows_legs(cx<T> *acc, const cx<S> *__restrict__ 
   // the first global leg's neighbour, loaded before the LDS legs that precede it
   constexpr int G0 = !in_tile(P(0)) ? 0 : !in_tile(P(1)) ? 1 : !in_tile(P(2)) ? 2 : !in_tile(P(3)) ? 3
                      : !in_tile(P(4)) ? 4 : !in_tile(P(5)) ? 5 : !in_tile(P(6)) ? 6 : 7;
-  load_site<S>(fg, in, nb[P(G0)] * LS + s);
+  T sg = load_site_raw<S>(fg, in, nb[P(G0)] * LS + s);
 #define GX_ROW_LEG(L)                                                                                  \
   {                                                                                                    \
     constexpr int PL = P(L);                                                                           \
     cx<T> f[12];                                                                                       \
+    T sc;                                                                                              \
     if constexpr (in_tile(PL)) {                                                                       \
-      lds_load_site<T>(f, lin, entry(PL), nent);                                                       \
+      const uint32_t en = entry(PL);                                                                   \
+      lds_load_site<T>(f, lin, en, nent);                                                              \
+      if constexpr (Sto<S>::H) sc = lsc[en];                                                           \
+      else sc = (T)1;                                                                                  \
     } else {                                                                                           \
       _Pragma("unroll") for (int k = 0; k < 12; k++) f[k] = fg[k];                                     \
+      sc = sg;                                                                                         \
       constexpr int NX = L >= 7 ? -1 : !in_tile(P(L + 1 > 7 ? 7 : L + 1)) ? L + 1                      \
                        : L + 2 <= 7 && !in_tile(P(L + 2 > 7 ? 7 : L + 2)) ? L + 2                      \
                        : L + 3 <= 7 && !in_tile(P(L + 3 > 7 ? 7 : L + 3)) ? L + 3                      \
                        : L + 4 <= 7 && !in_tile(P(L + 4 > 7 ? 7 : L + 4)) ? L + 4                      \
                        : L + 5 <= 7 && !in_tile(P(L + 5 > 7 ? 7 : L + 5)) ? L + 5                      \
                        : L + 6 <= 7 && !in_tile(P(L + 6 > 7 ? 7 : L + 6)) ? L + 6 : -1;                \
-      if constexpr (NX >= 0) load_site<S>(fg, in, nb[P(NX)] * LS + s);                                 \
+      if constexpr (NX >= 0) sg = load_site_raw<S>(fg, in, nb[P(NX)] * LS + s);                        \
     }                                                                                                  \
-    do_leg<T, S, DAG, 0, L, true>(acc, f, ul, U, h, H4);                                               \
+    do_leg<T, S, DAG, 0, L, true>(acc, f, ul, U, h, H4, sc);                                           \
   }
   GX_ROW_LEG(0) GX_ROW_LEG(1) GX_ROW_LEG(2) GX_ROW_LEG(3) GX_ROW_LEG(4) GX_ROW_LEG(5) GX_ROW_LEG(6) GX_ROW_LEG(7)
 #undef GX_ROW_LEG
@@ -905,6 +967,7 @@ __global__ void __launch_bounds__(rows_max_threads<S>())
   const uint32_t nent = 4 * rt.RW;                 // entries of the tile (= threads)
   cx<T> *lin = reinterpret_cast<cx<T> *>(lds_raw);  // [12 / VWL groups][nent][VWL]
   cx<T> *links = lin + (size_t)12 * nent;           // [4 Lxh sites][kLinkPad]
+  T *lsc = reinterpret_cast<T *>(links + (size_t)4 * g.Lxh * kLinkPad);  // FP16: the entries' site scales
   const uint32_t tile = tile_of_block(blockIdx.x, gridDim.x, map);
   if (tile >= ntiles) return;  // uniform across the workgroup
   const uint32_t H4 = (uint32_t)g.H4, LS = rt.LS, Lxh = (uint32_t)g.Lxh;
@@ -939,11 +1002,12 @@ __global__ void __launch_bounds__(rows_max_threads<S>())
     const uint32_t hq = qx + Lxh * (y0 + (qr & 1) + (uint32_t)g.L[1] * (z0 + (qr >> 1) + (uint32_t)g.L[2] * tslice));
     links[q * kLinkPad + pk] = ldc<S>(U, pk * H4 + hq);
   }
-  // own input entry -> LDS
+  // own input entry -> LDS (as stored, with its FP16 site scale beside it)
   {
     cx<T> f[12];
-    load_site<S>(f, in, h * LS + s);
+    const T sc = load_site_raw<S>(f, in, h * LS + s);
     lds_store_site<T>(lin, f, t, nent);
+    if constexpr (Sto<S>::H) lsc[t] = sc;
   }
   uint32_t nb[8];
   site_neighbours(g, h, cb_out, nb);
@@ -953,10 +1017,10 @@ __global__ void __launch_bounds__(rows_max_threads<S>())
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = {0, 0};
   switch (r) {  // uniform over a wave when a row is a multiple of 64 entries
-    case 0: rows_legs<T, S, DAG, 0, 0>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
-    case 1: rows_legs<T, S, DAG, 1, 0>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
-    case 2: rows_legs<T, S, DAG, 0, 1>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
-    default: rows_legs<T, S, DAG, 1, 1>(acc, in, U, lin, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
+    case 0: rows_legs<T, S, DAG, 0, 0>(acc, in, U, lin, lsc, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
+    case 1: rows_legs<T, S, DAG, 1, 0>(acc, in, U, lin, lsc, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
+    case 2: rows_legs<T, S, DAG, 0, 1>(acc, in, U, lin, lsc, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
+    default: rows_legs<T, S, DAG, 1, 1>(acc, in, U, lin, lsc, ul, nb, rowh, r, w, rt.RW, nent, LS, s, h, H4); break;
   }
   if (MAT) group_matrix<T>(acc, lin, ep.mat, sh, t, t / LS, s, blockDim.x);
   const uint32_t o = h * LS + s;
@@ -1269,7 +1333,8 @@ int launch_dhop_epi(const Geom &g, const void *in, void *out, const void *U, int
   // row tiles: 2 x 2 rows of x in LDS (whole tiles only: Ly and Lz are even)
   if (tune_get(GX_TUNE_DHOP_TILE, prec_of<S>()) && g.Ls % lb == 0) {
     const uint32_t RW = (uint32_t)(k.Lxh * g.Ls), nthr = 4 * RW;
-    const size_t lds = ((size_t)12 * nthr + (size_t)4 * k.Lxh * kLinkPad) * sizeof(cx<T>);
+    const size_t lds = ((size_t)12 * nthr + (size_t)4 * k.Lxh * kLinkPad) * sizeof(cx<T>) +
+                       (Sto<S>::H ? (size_t)nthr * sizeof(T) : 0);  // FP16: the entries' site scales
     if (nthr <= (uint32_t)rows_max_threads<S>() && lds <= 160 * 1024 && g.L[1] >= 4 && g.L[2] >= 4) {
       RowTile rt;
       rt.LS = (uint32_t)g.Ls;
