@@ -2211,54 +2211,28 @@ static __global__ void __launch_bounds__(256) k_mrhs_fin(const double *partial, 
   }
 }
 
-// psi_s += a_s p_s; p_s = b_s p_s + r_s over every stored entry (padding entries are zero), one
-// 16-byte group (the VW components of one entry that it holds) per thread and step
+// psi_s += a_s p_s; p_s = b_s p_s + r_s, one (h, s) entry (its 12 components, 12 / VW 16-byte
+// groups per field, consecutive lanes on consecutive entries) per thread and step; padding entries
+// stay zero.  FP16 stores rescale each entry.
 template <typename T>
-__global__ void __launch_bounds__(256) k_mrhs_update(int64_t n, int Ls, cx<T> *__restrict__ psi, cx<T> *__restrict__ p,
+__global__ void __launch_bounds__(256) k_mrhs_update(int64_t nent, int Ls, cx<T> *__restrict__ psi, cx<T> *__restrict__ p,
                                                      const cx<T> *__restrict__ r, const MrhsState *st) {
   if (mgated_prev(st)) return;
-  constexpr int VW = Sto<T>::VW;
-  struct alignas(16) G {
-    cx<T> v[VW];
-  };
-  G *gx_ = reinterpret_cast<G *>(psi), *gp = reinterpret_cast<G *>(p);
-  const G *gr = reinterpret_cast<const G *>(r);
-  const int64_t ng = n / VW;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ng; q += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = entry_of<T>(q * VW);  // entry of the group's stored positions
-    const int s = (int)(e % Ls);
-    const T a = (T)st->a[s], b = (T)st->b[s];
-    const G pv = gp[q], rv = gr[q];
-    G xv = gx_[q], np;
-#pragma unroll
-    for (int j = 0; j < VW; j++) {
-      xv.v[j] = {fma(a, pv.v[j].r, xv.v[j].r), fma(a, pv.v[j].i, xv.v[j].i)};
-      np.v[j] = {fma(b, pv.v[j].r, rv.v[j].r), fma(b, pv.v[j].i, rv.v[j].i)};
-    }
-    gx_[q] = xv;
-    gp[q] = np;
-  }
-}
-
-// FP16 storage: the same update per (h, s) entry (each store rescales its entry)
-static __global__ void __launch_bounds__(256) k_mrhs_update_h(int64_t nent, int Ls, cx<half_t> *__restrict__ psi,
-                                                       cx<half_t> *__restrict__ p, const cx<half_t> *__restrict__ r,
-                                                       const MrhsState *st) {
-  if (mgated_prev(st)) return;
+  typedef typename Sto<T>::C C;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nent; e += (int64_t)gridDim.x * blockDim.x) {
     const int s = (int)(e % Ls);
-    const float a = (float)st->a[s], b = (float)st->b[s];
-    cx<float> pv[12], xv[12], rv[12];
-    load_site<half_t>(pv, p, e);
-    load_site<half_t>(xv, psi, e);
-    load_site<half_t>(rv, r, e);
+    const C a = (C)st->a[s], b = (C)st->b[s];
+    cx<C> pv[12], xv[12], rv[12];
+    load_site<T>(pv, p, e);
+    load_site<T>(xv, psi, e);
+    load_site<T>(rv, r, e);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
       xv[k] = {fma(a, pv[k].r, xv[k].r), fma(a, pv[k].i, xv[k].i)};
       pv[k] = {fma(b, pv[k].r, rv[k].r), fma(b, pv[k].i, rv[k].i)};
     }
-    store_site<half_t>(psi, xv, e, false);
-    store_site<half_t>(p, pv, e, false);
+    store_site<T>(psi, xv, e, false);
+    store_site<T>(p, pv, e, false);
   }
 }
 
@@ -2291,17 +2265,9 @@ int launch_mrhs_reduce(Ctx *ctx, int mode, const Geom &g, void *x, const void *y
 
 template <typename T>
 int launch_mrhs_update(Ctx *ctx, const Geom &g, void *psi, void *p, const void *r, MrhsState *state) {
-  if constexpr (Sto<T>::H) {
-    // every (h, s) entry of the half field, padding excluded (its entries stay zero)
-    const int64_t nent = g.H4 * g.Ls;
-    hipLaunchKernelGGL(k_mrhs_update_h, dim3(stream_blocks(nent)), dim3(256), 0, ctx->stream, nent, g.Ls,
-                       (cx<half_t> *)psi, (cx<half_t> *)p, (const cx<half_t> *)r, (const MrhsState *)state);
-  } else {
-    const int64_t n = 12 * g.half_pad();
-    hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(n / Sto<T>::VW)), dim3(256), 0, ctx->stream, n, g.Ls,
-                       (cx<T> *)psi,
-                       (cx<T> *)p, (const cx<T> *)r, (const MrhsState *)state);
-  }
+  const int64_t nent = g.H4 * g.Ls;  // every (h, s) entry of the half field, padding excluded
+  hipLaunchKernelGGL((k_mrhs_update<T>), dim3(stream_blocks(nent)), dim3(256), 0, ctx->stream, nent, g.Ls,
+                     (cx<T> *)psi, (cx<T> *)p, (const cx<T> *)r, (const MrhsState *)state);
   GX_CHECK_HIP(hipGetLastError());
   return GX_OK;
 }
