@@ -295,27 +295,49 @@ __device__ __forceinline__ bool gated(const Gate *gate) {
   return gate && (gate->k > gate->k_conv || gate->k > gate->kmax);
 }
 
+// x + sc h, x - sc h, x + i sc h, x - i sc h
+template <typename T>
+__device__ __forceinline__ cx<T> cadd_s(cx<T> x, cx<T> h, T sc) { return {fma(sc, h.r, x.r), fma(sc, h.i, x.i)}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub_s(cx<T> x, cx<T> h, T sc) { return {fma(-sc, h.r, x.r), fma(-sc, h.i, x.i)}; }
+template <typename T>
+__device__ __forceinline__ cx<T> caddi_s(cx<T> x, cx<T> h, T sc) { return {fma(-sc, h.i, x.r), fma(sc, h.r, x.i)}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csubi_s(cx<T> x, cx<T> h, T sc) { return {fma(sc, h.i, x.r), fma(-sc, h.r, x.i)}; }
+
+
 // spin projection (1 + gamma_mu) (PLUS) or (1 - gamma_mu) on a 12-component spinor f[spin*3+c],
 // giving the 2-spinor h[spin*3+c], spin in {0,1}.  TwoSpinor.h:75-133
-template <typename T, int MU, bool PLUS>
-__device__ __forceinline__ void project(const cx<T> *f, cx<T> *h) {
+// MIX: every sum a +- b is formed as fma(+-one, b, a) with a unit `one` the compiler cannot see
+// (the same rounded sum), so that when a and b are FP16 values widened to FP32 the widening folds
+// into a mixed-precision FMA instead of two conversions and an add.
+template <typename T, int MU, bool PLUS, bool MIX = false>
+__device__ __forceinline__ void project(const cx<T> *f, cx<T> *h, T one = (T)1) {
+#define GX_PADD(x, y) (MIX ? cadd_s<T>(x, y, one) : cadd<T>(x, y))
+#define GX_PSUB(x, y) (MIX ? csub_s<T>(x, y, one) : csub<T>(x, y))
+#define GX_PADDI(x, y) (MIX ? caddi_s<T>(x, y, one) : caddi<T>(x, y))
+#define GX_PSUBI(x, y) (MIX ? csubi_s<T>(x, y, one) : csubi<T>(x, y))
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const cx<T> f0 = f[c], f1 = f[3 + c], f2 = f[6 + c], f3 = f[9 + c];
     if (MU == 0) {
-      if (PLUS) { h[c] = caddi(f0, f3); h[3 + c] = caddi(f1, f2); }
-      else      { h[c] = csubi(f0, f3); h[3 + c] = csubi(f1, f2); }
+      if (PLUS) { h[c] = GX_PADDI(f0, f3); h[3 + c] = GX_PADDI(f1, f2); }
+      else      { h[c] = GX_PSUBI(f0, f3); h[3 + c] = GX_PSUBI(f1, f2); }
     } else if (MU == 1) {
-      if (PLUS) { h[c] = csub(f0, f3); h[3 + c] = cadd(f1, f2); }
-      else      { h[c] = cadd(f0, f3); h[3 + c] = csub(f1, f2); }
+      if (PLUS) { h[c] = GX_PSUB(f0, f3); h[3 + c] = GX_PADD(f1, f2); }
+      else      { h[c] = GX_PADD(f0, f3); h[3 + c] = GX_PSUB(f1, f2); }
     } else if (MU == 2) {
-      if (PLUS) { h[c] = caddi(f0, f2); h[3 + c] = csubi(f1, f3); }
-      else      { h[c] = csubi(f0, f2); h[3 + c] = caddi(f1, f3); }
+      if (PLUS) { h[c] = GX_PADDI(f0, f2); h[3 + c] = GX_PSUBI(f1, f3); }
+      else      { h[c] = GX_PSUBI(f0, f2); h[3 + c] = GX_PADDI(f1, f3); }
     } else {
-      if (PLUS) { h[c] = cadd(f0, f2); h[3 + c] = cadd(f1, f3); }
-      else      { h[c] = csub(f0, f2); h[3 + c] = csub(f1, f3); }
+      if (PLUS) { h[c] = GX_PADD(f0, f2); h[3 + c] = GX_PADD(f1, f3); }
+      else      { h[c] = GX_PSUB(f0, f2); h[3 + c] = GX_PSUB(f1, f3); }
     }
   }
+#undef GX_PADD
+#undef GX_PSUB
+#undef GX_PADDI
+#undef GX_PSUBI
 }
 
 // accumulate the reconstruction of colour c of a projected 2-spinor whose spin rows are h0, h1,
@@ -341,16 +363,6 @@ __device__ __forceinline__ void recon_colour(cx<T> *a, int c, cx<T> h0, cx<T> h1
 
 // recon_colour of (sc h0, sc h1): the factor rides in the accumulation (one FMA per component
 // instead of an add); FP16 storage, whose site scale is applied here
-// x + sc h, x - sc h, x + i sc h, x - i sc h
-template <typename T>
-__device__ __forceinline__ cx<T> cadd_s(cx<T> x, cx<T> h, T sc) { return {fma(sc, h.r, x.r), fma(sc, h.i, x.i)}; }
-template <typename T>
-__device__ __forceinline__ cx<T> csub_s(cx<T> x, cx<T> h, T sc) { return {fma(-sc, h.r, x.r), fma(-sc, h.i, x.i)}; }
-template <typename T>
-__device__ __forceinline__ cx<T> caddi_s(cx<T> x, cx<T> h, T sc) { return {fma(-sc, h.i, x.r), fma(sc, h.r, x.i)}; }
-template <typename T>
-__device__ __forceinline__ cx<T> csubi_s(cx<T> x, cx<T> h, T sc) { return {fma(sc, h.i, x.r), fma(-sc, h.r, x.i)}; }
-
 template <typename T, int MU, bool PLUS>
 __device__ __forceinline__ void recon_colour_s(cx<T> *a, int c, cx<T> h0, cx<T> h1, T sc) {
 #define add(x, h) cadd_s<T>(x, h, sc)
@@ -446,7 +458,13 @@ template <typename T, typename S, int MU, bool PLUS, bool LDSU>
 __device__ __forceinline__ void leg_math(cx<T> *acc, const cx<T> *f, const cx<T> *ul, const cx<S> *__restrict__ U,
                                          int P, uint32_t h, uint32_t H4, T sc) {
   cx<T> hs[6];
-  project<T, MU, PLUS>(f, hs);
+  if constexpr (Sto<S>::H) {
+    T one = (T)1;
+    asm("" : "+s"(one));  // opaque to the optimiser: the projection's sums stay FMAs (see project)
+    project<T, MU, PLUS, true>(f, hs, one);
+  } else {
+    project<T, MU, PLUS>(f, hs);
+  }
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     cx<T> u[3];
