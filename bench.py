@@ -8,7 +8,8 @@ value = algorithmic (compulsory) bytes per call / average kernel time, with
 bytes = H (24 Ls + 72) c   (H = V4/2 half sites, c = bytes per complex): each input spinor and
 each link read once, each output spinor written once.  Kernel time comes from HIP events on the
 stream the kernels are launched on.  Also reported: GF/s at 1320 flop per 5D site, the FP32
-figure, Schur CG time-to-solution in FP64 and mixed FP32/FP64, and the CPU oracle's DhopEO on the
+figure, Schur CG time-to-solution in FP64 and in defect correction with FP32 and with FP16-storage
+inner solves, and the CPU oracle's DhopEO on the
 same workload (cpu_baseline).  Every solve runs its red-black CG to 1e-11 (Wilson) or 1e-12
 (Mobius) on the normal equations so that the full-system residual |b - M x| / |b| (reported as
 unprec_residual, with meets_target) is <= 1e-10.
@@ -226,8 +227,8 @@ def bench_dhop_large(gx, ctx, tune):
 
 def bench_mixed_dwf(gx, ctx, dims=(24, 24, 24, 24), Ls=12, tol=CG_TOL_MOBIUS, tune=None):
     """Mobius 24^4 x Ls12 (b=1.5, c=0.5, M5=1.8, mf=0.005, eps 0.3), Gaussian 5D source: red-black
-    solve in FP64 CG, in defect correction with FP32 and with FP16-storage inner solves (inner
-    tolerance 1e-5, FP64 outer residual and patch-up), and in reliable-update CG (FP32 Krylov
+    solve in FP64 CG, in defect correction with FP32 and with FP16-storage inner solves (first
+    inner tolerance 1e-5 and 1e-3, FP64 outer residual and patch-up), and in reliable-update CG (FP32 Krylov
     vectors, FP64 reliable updates with delta 0.01, FP64 clean-up), each to |b - M x| / |b| <= 1e-10.
     Wall clock around each solve."""
     kw = dict(mass=0.005, M5=1.8, b=1.5, c=0.5)
@@ -625,6 +626,10 @@ def main():
         fsol.zero()
         stmx = D.schur_solve(fs, fsol, tol=CG_TOL_MOBIUS, maxit=20000, mixed=True)
         xmx = fsol.download()
+        tune('cg_f16', D, gx.F16)
+        fsol.zero()
+        st16 = D.schur_solve(fs, fsol, tol=CG_TOL_MOBIUS, maxit=20000, mixed=2)
+        x16 = fsol.download()
         cg = dict(cg_tol=CG_TOL_MOBIUS,
                   fp64=dict(seconds=st64['seconds'], iterations=st64['iterations'], true_residual=st64['true_residual'],
                             unprec_residual=st64['unprec_residual'],
@@ -634,7 +639,13 @@ def main():
                                  true_residual=stmx['true_residual'], unprec_residual=stmx['unprec_residual'],
                                  meets_target=bool(stmx['unprec_residual'] <= TARGET_RESIDUAL),
                                  rel_err_vs_fp64=float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))),
-                  speedup_mixed_vs_fp64=st64['seconds'] / stmx['seconds'])
+                  mixed_f16=dict(seconds=st16['seconds'], inner_iterations=st16['inner_iterations'],
+                                 outer=st16['outer_iterations'], final_iterations=st16['final_iterations'],
+                                 true_residual=st16['true_residual'], unprec_residual=st16['unprec_residual'],
+                                 meets_target=bool(st16['unprec_residual'] <= TARGET_RESIDUAL),
+                                 rel_err_vs_fp64=float(np.linalg.norm(x16 - x64) / np.linalg.norm(x64))),
+                  speedup_mixed_vs_fp64=st64['seconds'] / stmx['seconds'],
+                  speedup_mixed_f16_vs_fp64=st64['seconds'] / st16['seconds'])
 
     c3 = None if args.no_c3 else leg('c3', lambda: bench_mixed_dwf(gx, ctx, tune=tune))
     c4 = None if args.no_c4 else leg('c4', lambda: bench_multi_rhs(gx, ctx, tune=tune))
