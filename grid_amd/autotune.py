@@ -16,8 +16,10 @@ precision prec only, so the FP64 and inner-precision operators of a mixed solve 
 The hop knobs are chosen by timing DhopEO on the caller's own lattice, Ls and precision (HIP
 events over repeated launches, best of several rounds), by coordinate search: map, prefetch,
 stores and leg order in the default tile shape, then the tile shape (W-wave shapes with and
-without the persistent grid, and the LDS row tiles), then the map again in the chosen shape; the
-solver knobs by timing a fixed number of CG iterations.  Results are cached in a JSON file keyed
+without the persistent grid, and the LDS row tiles), then the map again in the chosen shape.  For
+FP32 and FP16 storage, whose CG folds its reductions into hop epilogues, a persistent winner and
+the fastest hop of the other prefetch setting are then compared by CG iteration time.  The
+solver knobs are chosen by timing a fixed number of CG iterations.  Results are cached in a JSON file keyed
 by device and workload, so a later
 process applies them without measuring.  No Grid counterpart: Grid fixes its launch shape when it
 is compiled (Grid/threads/Accelerator.h:416-461).
@@ -198,17 +200,25 @@ class Autotuner:
 
         # FP32 / FP16 CG folds its reductions into hop epilogues that only the one-workgroup-per-tile
         # launches have: when the fastest hop is a persistent grid, keep it only if a CG iteration
-        # under it beats one under the fastest non-persistent launch
+        # under it beats one under the fastest non-persistent launch.  The epilogue kernels also
+        # use registers differently from the plain hop (the FP16 ones fit fewer waves without
+        # prefetch), so the fastest hop of the other prefetch setting is compared by CG time too.
+        def cg_hop(h):
+            with Knobs(h, prec):
+                return self._time_cg(D, fs, fx, cg_iters)
+
         if prec != F64 and dict(hop_best).get(TUNE_DHOP_PERSIST, 0):
             plain = [c for c in hop_times if not dict(c).get(TUNE_DHOP_PERSIST, 0)]
             if plain:
                 alt = min(plain, key=lambda c: hop_times[c])
-
-                def cg_hop(h):
-                    with Knobs(h, prec):
-                        return self._time_cg(D, fs, fx, cg_iters)
-
                 hop_best, t4 = search([hop_best, alt], cg_hop, rounds)
+        if prec != F64:
+            pf = dict(hop_best).get(TUNE_DHOP_PF, 0)
+            other = [c for c in hop_times
+                     if dict(c).get(TUNE_DHOP_PF, 0) != pf and not dict(c).get(TUNE_DHOP_PERSIST, 0)]
+            if other:
+                alt = min(other, key=lambda c: hop_times[c])
+                hop_best, t5 = search([hop_best, alt], cg_hop, rounds)
 
         solver_best, solver_times = search(solver_candidates(), cg, rounds)
         entry = {'knobs': {KNOB_NAMES[k]: v for k, v in hop_best + solver_best},

@@ -576,8 +576,18 @@ struct HopEpi {
 // One lane per (4D half-site h, s): s fastest, 64/LS sites per wave, 4 waves per workgroup, so a
 // wave reads each neighbour's s-row contiguously and the lanes of one site share its links.
 // flags: bit 0 = non-temporal output stores.  ORD: leg order (0 Grid's, 1 paired directions).
+// Waves per SIMD the compiler must leave VGPRs for (512 / waves each).  The FP16-storage hop with
+// prefetch in Grid's leg order fits three waves (168 VGPRs) with at most two spilled against 174
+// unbounded (two waves), which made its 12-RHS form 16% and the 16^4 x 16 one 8% faster; the other
+// FP16 forms (and Ls < 4) spill up to 35 VGPRs under the same bound and stay unbounded.
+template <typename S, int LS, bool PF, int ORD>
+constexpr int dhop_waves() {
+  return (Sto<S>::H && PF && ORD == 0 && LS >= 4) ? 3 : 1;
+}
+
 template <typename S, int LS, bool DAG, bool PF, int ORD, int EPI>
-__global__ void __launch_bounds__(256) k_dhop(const cx<S> *__restrict__ in, cx<S> *__restrict__ out,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(dhop_waves<S, LS, PF, ORD>())))
+k_dhop(const cx<S> *__restrict__ in, cx<S> *__restrict__ out,
                                               const cx<S> *__restrict__ U, KGeom g, int cb_out,
                                               const Gate *__restrict__ gate, uint32_t ntiles, int map, int flags,
                                               HopEpi<S> ep) {
