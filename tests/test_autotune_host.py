@@ -23,6 +23,13 @@ def test_search_breaks_ties_by_order():
     assert best == 'x'
 
 
+def test_search_margin_keeps_default():
+    t = {'x': 1.0, 'y': 0.99, 'z': 0.97}
+    assert at.search(['x', 'y'], t.get, rounds=1, margin=0.02)[0] == 'x'
+    assert at.search(['x', 'z'], t.get, rounds=1, margin=0.02)[0] == 'z'
+    assert at.search(['x', 'y'], t.get, rounds=1)[0] == 'y'
+
+
 def test_candidates_cover_knob_space():
     c16 = at.hop_candidates(16)
     c8 = at.hop_candidates(8)
